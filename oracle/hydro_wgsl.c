@@ -1,0 +1,694 @@
+/*
+ * TEST INFRASTRUCTURE (the oracle): a plain-C restatement of Shaderous's WGSL uniform-grid hydro
+ * step, used only to check the HIP kernels. Compiled twice by oracle/Makefile: REAL=float with
+ * PFX=orc_f32_ and REAL=double with PFX=orc_f64_. Built with -ffp-contract=off so every
+ * operation rounds as written.
+ *
+ * Scope: the hot path with every optional module off (gravity, solids, species, viscosity,
+ * reference state, cosmology, shearing box, geometry: the *_off stubs, each of which is an exact
+ * no-op here, e.g. wall_step returns (across, 1, 1) with no well-balanced reconstruction,
+ * src/gravity/gfield.wgsl:83-84, and add_diffusion returns its flux, src/hydro/diffusion.wgsl:22).
+ *
+ * The face flux is computed once per face with flux_between (src/hydro/flux.wgsl:254-347), as
+ * the tiled update does (src/hydro/tile.wgsl:166-198); the gather form computes the same value
+ * from the same inputs (flux.wgsl:252-253).
+ *
+ * Parity pins (tests/test_oracle.py): Toro known answers through the exact solver; limiter
+ * invariants (tests/unit/reference.test.ts:38-59); HLLC/HLL closed forms (supersonic upwinding,
+ * a stationary contact carries no mass, HLL = HLLC at w = 1, HLLC = HLL on a vacuum face);
+ * conservation to round-off in periodic boxes; x/y/z permutation equality; reflecting symmetry;
+ * the Sod tube against the exact solution at the reference's measured L1; and agreement with the
+ * independent RefHydro restatement (refhydro.c) where the documented WGSL/RefHydro deltas do not
+ * act. No function here is left unpinned.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+typedef REAL real;
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define FN(name) CAT(PFX, name)
+#define RL(x) ((real)(x))
+
+/* Q: primitive record (src/hydro/q.wgsl:56): a = (rho, vx, vy, vz), b = (p, s, c_s, solid). */
+typedef struct { real a[4]; real b[4]; } Q;
+
+/* The Params values as the shader sees them (f32 in the FP32 build). */
+typedef struct {
+  int n[3], bc_lo[3], bc_hi[3], ndim, limiter, riemann, isen;
+  real gamma, gm1, dx, cfl, rho_floor, p_floor, eta_dual, shock_thresh, t_end, dt_max;
+  real fluff[3];
+} P_t;
+
+static P_t load_params(const orc_params* p) {
+  P_t P;
+  for (int a = 0; a < 3; a++) { P.n[a] = p->n[a]; P.bc_lo[a] = p->bc_lo[a]; P.bc_hi[a] = p->bc_hi[a]; }
+  P.ndim = p->ndim; P.limiter = p->limiter; P.riemann = p->riemann; P.isen = p->isentropic;
+  P.gamma = RL(p->gamma);
+  P.gm1 = P.gamma - RL(1.0);          /* eos_gm1(): P.eos.x - 1.0 (src/eos/ideal.wgsl:230-233) */
+  P.dx = RL(p->dx); P.cfl = RL(p->cfl); P.rho_floor = RL(p->rho_floor); P.p_floor = RL(p->p_floor);
+  P.eta_dual = RL(p->eta_dual); P.shock_thresh = RL(p->shock_thresh);
+  P.t_end = RL(p->t_end); P.dt_max = RL(p->dt_max);
+  for (int a = 0; a < 3; a++) P.fluff[a] = RL(p->fluff[a]);
+  return P;
+}
+
+static real rmax(real a, real b) { return a > b ? a : b; }
+static real rmin(real a, real b) { return a < b ? a : b; }
+static real rabs(real a) { return a < 0 ? -a : a; }
+static real rsign(real a) { return a > 0 ? RL(1.0) : (a < 0 ? RL(-1.0) : RL(0.0)); }
+static real rclamp(real x, real lo, real hi) { return rmin(rmax(x, lo), hi); }
+#if REAL_IS_DOUBLE
+static real rsqrt_(real x) { return sqrt(x); }
+static real rpow(real a, real b) { return pow(a, b); }
+#else
+static real rsqrt_(real x) { return sqrtf(x); }
+static real rpow(real a, real b) { return powf(a, b); }
+#endif
+/* WGSL mix(x, y, a) = x * (1 - a) + y * a */
+static real rmix(real x, real y, real a) { return x * (RL(1.0) - a) + y * a; }
+
+/* ---------------- equation of state: src/eos/ideal.wgsl:235-267 ---------------- */
+static real eos_sound_speed(const P_t* P, real rho, real p) { return rsqrt_(P->gamma * p / rho); }
+static real eos_eint_from_p(const P_t* P, real p) { return p / P->gm1; }
+
+/* prim_from_cons (src/hydro/prim.wgsl:131-156) with the ideal gas and pressure_floor without a
+ * Jeans floor (src/hydro/pfloor.wgsl:169-171). Returns Q as cell_q does (recon.wgsl:23-33):
+ * b = (p, scalar, c_s, mark). */
+static Q prim_from_cons(const P_t* P, const real* u1, const real* u2) {
+  Q q;
+  real rho = rmax(u1[0], P->rho_floor);
+  real eint = u2[1];
+  real p = P->gm1 * eint;
+  real cs = eos_sound_speed(P, rho, p);
+  real pmin = P->p_floor;
+  if (!(p >= pmin)) {
+    p = pmin;
+    eint = eos_eint_from_p(P, p);
+    cs = eos_sound_speed(P, rho, p);
+  }
+  (void)eint;
+  q.a[0] = rho;
+  q.a[1] = u1[1] / rho; q.a[2] = u1[2] / rho; q.a[3] = u1[3] / rho;
+  q.b[0] = p; q.b[1] = u2[2] / rho; q.b[2] = cs; q.b[3] = u2[3];
+  return q;
+}
+
+/* ---------------- boundary mapping: src/hydro/common.wgsl:433-539, boundary.wgsl:47-84 ------- */
+typedef struct { int c[3]; real flip[3]; real side[3]; } Mapped;
+
+static int bc_wall(int k) { return k == 2 || k == 4 || k == 5; }
+static int bc_ghost_cell(int i, int n, int k) {
+  int up = i >= n;
+  if (k == 0) { int v = up ? i - n : i + n; return v < 0 ? 0 : (v > n - 1 ? n - 1 : v); }
+  if (bc_wall(k)) return up ? 2 * n - 1 - i : -1 - i;
+  return up ? n - 1 : 0;
+}
+static Mapped bc_map(const P_t* P, const int c0[3]) {
+  Mapped m;
+  for (int a = 0; a < 3; a++) { m.c[a] = c0[a]; m.flip[a] = RL(1.0); m.side[a] = RL(0.0); }
+  for (int a = 0; a < 3; a++) {
+    if (c0[a] >= 0 && c0[a] < P->n[a]) continue;
+    int up = c0[a] >= P->n[a];
+    int k = up ? P->bc_hi[a] : P->bc_lo[a];
+    m.c[a] = bc_ghost_cell(c0[a], P->n[a], k);
+    if (bc_wall(k)) {
+      /* bc_vsign: line-tied and moving walls reverse all, reflecting the normal component */
+      if (k == 4 || k == 5) { m.flip[0] = -m.flip[0]; m.flip[1] = -m.flip[1]; m.flip[2] = -m.flip[2]; }
+      else m.flip[a] = -m.flip[a];
+    }
+    m.side[a] = (k == 3) ? (up ? RL(1.0) : RL(-1.0)) : RL(0.0);
+  }
+  return m;
+}
+/* bc_velocity (common.wgsl:536-539) without moving walls */
+static void bc_velocity(const Mapped* m, const real v[3], real out[3]) {
+  for (int a = 0; a < 3; a++) {
+    real w = v[a] * m->flip[a] + RL(0.0);
+    if (m->side[a] != RL(0.0)) w = rmax(w * m->side[a], RL(0.0)) * m->side[a];
+    out[a] = w;
+  }
+}
+static size_t cidx(const P_t* P, const int c[3]) {
+  return ((size_t)c[2] * P->n[1] + c[1]) * P->n[0] + c[0];
+}
+
+/* load_q (src/hydro/recon.wgsl:37-52) */
+static Q load_q(const P_t* P, const real* u1, const real* u2, const int c[3]) {
+  Mapped m = bc_map(P, c);
+  size_t o = cidx(P, m.c) * 4;
+  Q w = prim_from_cons(P, u1 + o, u2 + o);
+  Q q = w;
+  bc_velocity(&m, w.a + 1, q.a + 1);
+  return q;
+}
+/* load_pred (src/hydro/update.wgsl:47-63) */
+static Q load_pred(const P_t* P, const real* pa, const real* pb, const int c[3]) {
+  Mapped m = bc_map(P, c);
+  size_t o = cidx(P, m.c) * 4;
+  Q q;
+  q.a[0] = pa[o];
+  bc_velocity(&m, pa + o + 1, q.a + 1);
+  for (int i = 0; i < 4; i++) q.b[i] = pb[o + i];
+  return q;
+}
+
+/* ---------------- limiters: src/hydro/limiters/ *.wgsl ---------------- */
+#define SMOOTH RL(0.01)
+/* koren.wgsl:274-292 */
+static void edge4_koren(const real far[4], const real near[4], const real scale[4], real out[4]) {
+  real freev[4], s[4], s2[4], ss[4], q[4], qk[4];
+  int as_is[4];
+  int all_smooth = 1;
+  for (int i = 0; i < 4; i++) {
+    freev[i] = (far[i] + RL(2.0) * near[i]) / RL(3.0);
+    s[i] = SMOOTH * scale[i];
+    s2[i] = s[i] * s[i];
+    ss[i] = rmax(s2[i], RL(1e-30));
+    q[i] = far[i] * far[i] + near[i] * near[i];
+    as_is[i] = (s2[i] >= RL(1e-30) && s2[i] < RL(1e30)) || (s[i] == RL(0.0));
+    real k = RL(1.0) / rmax(rabs(s[i]), RL(1e-37));
+    real fk = far[i] * k, nk = near[i] * k;
+    qk[i] = fk * fk + nk * nk;
+    int smooth = as_is[i] ? (q[i] <= RL(0.5) * ss[i]) : (qk[i] <= RL(0.5));
+    all_smooth &= smooth;
+  }
+  if (all_smooth) { for (int i = 0; i < 4; i++) out[i] = freev[i]; return; }
+  for (int i = 0; i < 4; i++) {
+    real m = rmin(rmin(RL(2.0) * rabs(far[i]), RL(2.0) * rabs(near[i])), rabs(freev[i]));
+    real lim = (rsign(far[i]) * rsign(near[i]) > RL(0.0)) ? rsign(near[i]) * m : RL(0.0);
+    real eta = as_is[i] ? q[i] / ss[i] : qk[i];
+    out[i] = rmix(lim, freev[i], rclamp(RL(1.5) - eta, RL(0.0), RL(1.0)));
+  }
+}
+/* mc.wgsl:299-307 */
+static real limit_mc(real a, real b) {
+  real m = rmin(rmin(RL(2.0) * rabs(a), RL(2.0) * rabs(b)), RL(0.5) * rabs(a + b));
+  return (rsign(a) * rsign(b) > RL(0.0)) ? rsign(a) * m : RL(0.0);
+}
+/* minmod.wgsl:309-316 */
+static real limit_minmod(real a, real b) {
+  return (rsign(a) * rsign(b) > RL(0.0)) ? rsign(a) * rmin(rabs(a), rabs(b)) : RL(0.0);
+}
+/* vanleer.wgsl:318-325 */
+static real limit_vanleer(real a, real b) {
+  return (rsign(a) * rsign(b) > RL(0.0)) ? RL(2.0) * a * (b / (a + b)) : RL(0.0);
+}
+static void edge4(int lim, const real far[4], const real near[4], const real scale[4], real out[4]) {
+  if (lim == ORC_LIM_KOREN) { edge4_koren(far, near, scale, out); return; }
+  for (int i = 0; i < 4; i++) {
+    out[i] = lim == ORC_LIM_MC ? limit_mc(far[i], near[i])
+           : lim == ORC_LIM_MINMOD ? limit_minmod(far[i], near[i]) : limit_vanleer(far[i], near[i]);
+  }
+}
+static int lim_symmetric(int lim) { return lim != ORC_LIM_KOREN; }
+
+/* ---------------- reconstruction: src/hydro/q.wgsl ---------------- */
+#define ISEN_SMOOTH RL(0.2)
+#define ISEN_ADIABAT RL(0.01)
+#define ISEN_SHEAR RL(0.1)
+#define ISEN_CAP RL(0.25)
+#define ISEN_ROOM RL(0.01)
+static real len3(real x, real y, real z) { return rsqrt_(x * x + y * y + z * z); }
+/* isen_inc (q.wgsl:183-209) with no reference state and no potential (g_q* = 0, df = 0) */
+static void isen_inc(const P_t* P, const Q* l, const Q* c, const Q* r, real bx, real sgn, real out[2]) {
+  real gi = RL(1.0) / P->gamma;
+  real wl0 = l->a[0], wl1 = l->b[0], wc0 = c->a[0], wc1 = c->b[0], wr0 = r->a[0], wr1 = r->b[0];
+  if (!(wl0 > 0 && wl1 > 0 && wc0 > 0 && wc1 > 0 && wr0 > 0 && wr1 > 0)) { out[0] = 0; out[1] = 0; return; }
+  real tl = wl0 * rpow(wc1 / wl1, gi) - wc0;
+  real tr = wr0 * rpow(wc1 / wr1, gi) - wc0;
+  real far = sgn > 0 ? -tl : tr;
+  real near = sgn > 0 ? tr : -tl;
+  real f4[4] = {far, 0, 0, 0}, n4[4] = {near, 0, 0, 0}, s4[4] = {wc0, 1, 1, 1}, d4[4];
+  edge4(P->limiter, f4, n4, s4, d4);
+  real d = d4[0];
+  real pf = wc1 + RL(0.5) * sgn * (bx - RL(0.0) * RL(0.0));
+  real wn = sgn > 0 ? wr0 : wl0;
+  real rf = rclamp((wc0 + RL(0.5) * sgn * d) * rpow(rmax(pf, RL(1e-30)) / wc1, gi), rmin(wc0, wn), rmax(wc0, wn));
+  real rs = RL(0.0);
+  real ks = rclamp(RL(2.0) - rmax(rabs(wl1 - wc1), rabs(wr1 - wc1)) / (ISEN_SMOOTH * wc1), RL(0.0), RL(1.0));
+  real ka = rclamp(RL(2.0) - rmax(rabs(tl), rabs(tr)) / (ISEN_ADIABAT * wc0), RL(0.0), RL(1.0));
+  real dvr = len3(r->a[1] - c->a[1], r->a[2] - c->a[2], r->a[3] - c->a[3]);
+  real dvl = len3(c->a[1] - l->a[1], c->a[2] - l->a[2], c->a[3] - l->a[3]);
+  real kv = rclamp(RL(2.0) - rmax(dvr, dvl) / (ISEN_SHEAR * rmax(c->b[2], RL(1e-30))), RL(0.0), RL(1.0));
+  real bound = rmax(ISEN_CAP * rmin(rabs(wc0 - wl0), rabs(wr0 - wc0)), ISEN_ROOM * wc0 * ks * ka * kv);
+  out[0] = RL(2.0) * sgn * (rf - wc0) - rs;
+  out[1] = bound;
+}
+static real isen_blend(real x, const real i[2]) { return x + rclamp(i[0] - x, -i[1], i[1]); }
+
+/* edge_up / edge_dn (q.wgsl:214-239) with no potential (dl = dr = df = 0). up != 0: twice the
+ * increment from the cell's center to its upper face; else from its lower face to the center. */
+static Q edge(const P_t* P, const Q* l, const Q* c, const Q* r, int up) {
+  real bl[4], br[4], al[4], ar[4];
+  for (int i = 0; i < 4; i++) {
+    bl[i] = c->b[i] - l->b[i]; br[i] = r->b[i] - c->b[i];      /* wb_below, wb_above */
+    al[i] = c->a[i] - l->a[i]; ar[i] = r->a[i] - c->a[i];
+  }
+  bl[0] = bl[0] + RL(0.5) * (l->a[0] + c->a[0]) * RL(0.0);
+  br[0] = br[0] + RL(0.5) * (c->a[0] + r->a[0]) * RL(0.0);
+  real bscale[4] = {c->b[0] + RL(0.0), RL(0.0), c->b[2], RL(0.0)};      /* p_scale (q.wgsl:106-109) */
+  real ascale[4] = {c->a[0] + RL(0.0), c->b[2], c->b[2], c->b[2]};
+  Q o;
+  int sym = lim_symmetric(P->limiter);
+  if (up || sym) { edge4(P->limiter, bl, br, bscale, o.b); edge4(P->limiter, al, ar, ascale, o.a); }
+  else { edge4(P->limiter, br, bl, bscale, o.b); edge4(P->limiter, ar, al, ascale, o.a); }
+  o.b[0] = o.b[0] - c->a[0] * RL(0.0);
+  if (P->isen) {
+    real ii[2];
+    isen_inc(P, l, c, r, o.b[0], up ? RL(1.0) : RL(-1.0), ii);
+    o.a[0] = isen_blend(o.a[0], ii);
+  }
+  return o;
+}
+
+/* ---------------- predictor: src/hydro/predictor.wgsl:46-209 ---------------- */
+/* nb = {xm, xp, ym, yp, zm, zp} */
+static Q predictor(const P_t* P, const Q* q, const Q nb[6], real dt) {
+  Q s[3];
+  real big = RL(0.0), fall = RL(0.0);
+  int sym = lim_symmetric(P->limiter);
+  for (int a = 0; a < 3; a++) {
+    const Q* m = &nb[2 * a];
+    const Q* p = &nb[2 * a + 1];
+    Q u = edge(P, m, q, p, 1);
+    Q sa = u;
+    big = rmax(big, rabs(u.b[0] + RL(0.0)));
+    if (sym) {
+      fall = rmin(fall, RL(-0.5) * rabs(u.a[0] + RL(0.0)));
+    } else {
+      Q d = edge(P, m, q, p, 0);
+      for (int i = 0; i < 4; i++) { sa.a[i] = RL(0.5) * (u.a[i] + d.a[i]); sa.b[i] = RL(0.5) * (u.b[i] + d.b[i]); }
+      big = rmax(big, rabs(d.b[0] + RL(0.0)));
+      fall = rmin(fall, rmin(RL(0.5) * (u.a[0] + RL(0.0)), RL(-0.5) * (d.a[0] + RL(0.0))));
+    }
+    for (int i = 0; i < 4; i++) { s[a].a[i] = sa.a[i] * RL(1.0); s[a].b[i] = sa.b[i] * RL(1.0); }
+  }
+  const Q *xm = &nb[0], *xp = &nb[1], *ym = &nb[2], *yp = &nb[3], *zm = &nb[4], *zp = &nb[5];
+  real conv = (xp->a[1] - xm->a[1]) + (yp->a[2] - ym->a[2]) + (zp->a[3] - zm->a[3]);
+  real hsx = RL(0.5) * ((xm->a[0] + q->a[0]) * RL(0.0) + (q->a[0] + xp->a[0]) * RL(0.0));
+  real hsy = RL(0.5) * ((ym->a[0] + q->a[0]) * RL(0.0) + (q->a[0] + yp->a[0]) * RL(0.0));
+  real hsz = RL(0.5) * ((zm->a[0] + q->a[0]) * RL(0.0) + (q->a[0] + zp->a[0]) * RL(0.0));
+  real pq = RL(0.0);
+  real jump = rmax(rabs(xp->b[0] - xm->b[0] + hsx) / (rmin(xp->b[0], xm->b[0]) + pq),
+                   rmax(rabs(yp->b[0] - ym->b[0] + hsy) / (rmin(yp->b[0], ym->b[0]) + pq),
+                        rabs(zp->b[0] - zm->b[0] + hsz) / (rmin(zp->b[0], zm->b[0]) + pq)));
+  real shock = (conv < RL(0.0) && jump > P->shock_thresh) ? RL(1.0) : RL(0.0);
+
+  real h = RL(0.5) * dt / P->dx;
+  real rho = q->a[0] + RL(0.0);
+  real v[3] = {q->a[1], q->a[2], q->a[3]};
+  real cs = q->b[2];
+  real dv = s[0].a[1] + s[1].a[2] + s[2].a[3];
+  real da[4], db[4];
+  for (int i = 0; i < 4; i++) {
+    da[i] = v[0] * s[0].a[i] + v[1] * s[1].a[i] + v[2] * s[2].a[i];
+    db[i] = v[0] * s[0].b[i] + v[1] * s[1].b[i] + v[2] * s[2].b[i];
+  }
+  real gradp[3] = {s[0].b[0], s[1].b[0], s[2].b[0]};
+  real acoustic = RL(1.0);
+  Q o;
+  o.a[0] = rho - h * (da[0] + rho * dv);
+  for (int i = 0; i < 3; i++) o.a[1 + i] = v[i] - h * (da[1 + i] + gradp[i] / rho) + RL(0.5) * dt * RL(0.0);
+  real b0 = q->b[0] - h * (db[0] + acoustic * rho * cs * cs * dv);
+  real b1 = q->b[1] - h * db[1];
+  real rmin_ = o.a[0] + RL(0.0) + fall;
+  real pmin_ = b0 + RL(0.0) - RL(0.5) * big;
+  real first = RL(0.0);
+  if (!(rmin_ > RL(0.0) && (pmin_ > RL(0.0)))) {
+    for (int i = 0; i < 4; i++) o.a[i] = q->a[i];
+    b0 = q->b[0]; b1 = q->b[1];
+    first = RL(1.0);
+  }
+  o.b[0] = b0; o.b[1] = b1; o.b[2] = shock; o.b[3] = first;
+  return o;
+}
+
+/* ---------------- faces: src/hydro/flux.wgsl ---------------- */
+typedef struct { real rho, v[3], p, s, cs, eint, pd; } Side;
+typedef struct { real m, mom[3], e, ei, s, vn; } Flux;
+
+static void rot3(const real v[3], int axis, real o[3]) {
+  if (axis == 1) { o[0] = v[1]; o[1] = v[2]; o[2] = v[0]; }        /* v.yzx */
+  else if (axis == 2) { o[0] = v[2]; o[1] = v[0]; o[2] = v[1]; }   /* v.zxy */
+  else { o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; }
+}
+static void unrot3(const real v[3], int axis, real o[3]) {
+  if (axis == 1) { o[0] = v[2]; o[1] = v[0]; o[2] = v[1]; }        /* v.zxy */
+  else if (axis == 2) { o[0] = v[1]; o[1] = v[2]; o[2] = v[0]; }   /* v.yzx */
+  else { o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; }
+}
+
+/* to_side (flux.wgsl:60-85), ideal gas, no reference state */
+static Side to_side(const P_t* P, const Q* pr, const Q* s, real sgn, int axis, real lo0, real lo1) {
+  real hs = RL(0.5) * sgn * (RL(1.0) - pr->b[3]);
+  real a[4], b[2];
+  for (int i = 0; i < 4; i++) a[i] = pr->a[i] + hs * s->a[i];
+  for (int i = 0; i < 2; i++) b[i] = pr->b[i] + hs * s->b[i];
+  Side o;
+  o.rho = rmax(a[0] + RL(0.0), rmax(lo0, P->rho_floor));
+  rot3(a + 1, axis, o.v);
+  o.s = b[1];
+  o.p = rmax(b[0] + RL(0.0), rmax(lo1, P->p_floor));
+  o.pd = o.p;
+  o.cs = eos_sound_speed(P, o.rho, o.p);
+  o.eint = eos_eint_from_p(P, o.p);
+  return o;
+}
+
+/* riemann_flux, HLLC (riemann/hllc.wgsl:4-85) */
+static Flux riemann_hllc(const Side* l, const Side* r, real w_hll) {
+  real sl = rmin(l->v[0] - l->cs, r->v[0] - r->cs);
+  real sr = rmax(l->v[0] + l->cs, r->v[0] + r->cs);
+  real el = l->eint + RL(0.5) * l->rho * (l->v[0] * l->v[0] + l->v[1] * l->v[1] + l->v[2] * l->v[2]);
+  real er = r->eint + RL(0.5) * r->rho * (r->v[0] * r->v[0] + r->v[1] * r->v[1] + r->v[2] * r->v[2]);
+  real ml[3], mr[3], fpl[3], fpr[3];
+  for (int i = 0; i < 3; i++) { ml[i] = l->rho * l->v[i]; mr[i] = r->rho * r->v[i]; }
+  real fml = ml[0], fmr = mr[0];
+  for (int i = 0; i < 3; i++) { fpl[i] = ml[i] * l->v[0] + (i == 0 ? l->pd : RL(0.0)); fpr[i] = mr[i] * r->v[0] + (i == 0 ? r->pd : RL(0.0)); }
+  real fel = (el + l->p) * l->v[0];
+  real fer = (er + r->p) * r->v[0];
+  Flux out;
+  if (sl >= RL(0.0)) {
+    out.m = fml; for (int i = 0; i < 3; i++) out.mom[i] = fpl[i]; out.e = fel; out.vn = l->v[0];
+  } else if (sr <= RL(0.0)) {
+    out.m = fmr; for (int i = 0; i < 3; i++) out.mom[i] = fpr[i]; out.e = fer; out.vn = r->v[0];
+  } else {
+    real ql = l->rho * (sl - l->v[0]);
+    real qr = r->rho * (sr - r->v[0]);
+    real inv = RL(1.0) / (sr - sl);
+    real hm = (sr * fml - sl * fmr + sl * sr * (r->rho - l->rho)) * inv;
+    real hp[3];
+    for (int i = 0; i < 3; i++) hp[i] = (sr * fpl[i] - sl * fpr[i] + sl * sr * (mr[i] - ml[i])) * inv;
+    real he = (sr * fel - sl * fer + sl * sr * (er - el)) * inv;
+    if (!(ql < RL(0.0) && qr > RL(0.0))) {
+      out.m = hm; for (int i = 0; i < 3; i++) out.mom[i] = hp[i]; out.e = he;
+      out.vn = RL(0.5) * (l->v[0] + r->v[0]);
+      int up0 = out.m >= RL(0.0);
+      out.ei = out.m * (up0 ? l->eint / l->rho : r->eint / r->rho);
+      out.s = out.m * (up0 ? l->s : r->s);
+      return out;
+    }
+    real sm = (r->pd - l->pd + l->v[0] * ql - r->v[0] * qr) / (ql - qr);
+    real cm, cp[3], ce;
+    if (sm >= RL(0.0)) {
+      real k = (sm - l->v[0]) / (sl - sm);
+      real dr = l->rho * k;
+      real f = l->rho + dr;
+      cm = fml + sl * dr;
+      cp[0] = fpl[0] + sl * (sm * dr + l->rho * (sm - l->v[0]));
+      cp[1] = fpl[1] + sl * (dr * l->v[1]);
+      cp[2] = fpl[2] + sl * (dr * l->v[2]);
+      ce = fel + sl * (el * k + f * (sm - l->v[0]) * (sm + l->p / ql));
+    } else {
+      real k = (sm - r->v[0]) / (sr - sm);
+      real dr = r->rho * k;
+      real f = r->rho + dr;
+      cm = fmr + sr * dr;
+      cp[0] = fpr[0] + sr * (sm * dr + r->rho * (sm - r->v[0]));
+      cp[1] = fpr[1] + sr * (dr * r->v[1]);
+      cp[2] = fpr[2] + sr * (dr * r->v[2]);
+      ce = fer + sr * (er * k + f * (sm - r->v[0]) * (sm + r->p / qr));
+    }
+    out.m = rmix(cm, hm, w_hll);
+    for (int i = 0; i < 3; i++) out.mom[i] = rmix(cp[i], hp[i], w_hll);
+    out.e = rmix(ce, he, w_hll);
+    out.vn = sm;
+  }
+  int up = out.m >= RL(0.0);
+  out.ei = out.m * (up ? l->eint / l->rho : r->eint / r->rho);
+  out.s = out.m * (up ? l->s : r->s);
+  return out;
+}
+
+/* riemann_flux, HLL (riemann/hll.wgsl:87-121) */
+static Flux riemann_hll(const Side* l, const Side* r) {
+  real sl = rmin(l->v[0] - l->cs, r->v[0] - r->cs);
+  real sr = rmax(l->v[0] + l->cs, r->v[0] + r->cs);
+  real el = l->eint + RL(0.5) * l->rho * (l->v[0] * l->v[0] + l->v[1] * l->v[1] + l->v[2] * l->v[2]);
+  real er = r->eint + RL(0.5) * r->rho * (r->v[0] * r->v[0] + r->v[1] * r->v[1] + r->v[2] * r->v[2]);
+  real ml[3], mr[3], fpl[3], fpr[3];
+  for (int i = 0; i < 3; i++) { ml[i] = l->rho * l->v[i]; mr[i] = r->rho * r->v[i]; }
+  real fml = ml[0], fmr = mr[0];
+  for (int i = 0; i < 3; i++) { fpl[i] = ml[i] * l->v[0] + (i == 0 ? l->pd : RL(0.0)); fpr[i] = mr[i] * r->v[0] + (i == 0 ? r->pd : RL(0.0)); }
+  real fel = (el + l->p) * l->v[0];
+  real fer = (er + r->p) * r->v[0];
+  Flux out;
+  if (sl >= RL(0.0)) {
+    out.m = fml; for (int i = 0; i < 3; i++) out.mom[i] = fpl[i]; out.e = fel; out.vn = l->v[0];
+  } else if (sr <= RL(0.0)) {
+    out.m = fmr; for (int i = 0; i < 3; i++) out.mom[i] = fpr[i]; out.e = fer; out.vn = r->v[0];
+  } else {
+    real ql = l->rho * (sl - l->v[0]);
+    real qr = r->rho * (sr - r->v[0]);
+    real inv = RL(1.0) / (sr - sl);
+    out.m = (sr * fml - sl * fmr + sl * sr * (r->rho - l->rho)) * inv;
+    for (int i = 0; i < 3; i++) out.mom[i] = (sr * fpl[i] - sl * fpr[i] + sl * sr * (mr[i] - ml[i])) * inv;
+    out.e = (sr * fel - sl * fer + sl * sr * (er - el)) * inv;
+    out.vn = (r->pd - l->pd + l->v[0] * ql - r->v[0] * qr) / (ql - qr);
+  }
+  int up = out.m >= RL(0.0);
+  out.ei = out.m * (up ? l->eint / l->rho : r->eint / r->rho);
+  out.s = out.m * (up ? l->s : r->s);
+  return out;
+}
+
+/* face_flux (flux.wgsl:92-125) with no potential and no reference state */
+static Flux face_flux(const P_t* P, const Q* l, const Q* ls, const Q* r, const Q* rs, int axis) {
+  real w = rmax(l->b[2], r->b[2]);
+  real al0 = l->a[0] + RL(0.0), al1 = l->b[0] + RL(0.0);
+  real ar0 = r->a[0] + RL(0.0), ar1 = r->b[0] + RL(0.0);
+  real pb = rmin(al1, ar1);
+  {
+    real el = al1 - RL(0.5) * al0 * RL(0.0);
+    real er = ar1 + RL(0.5) * ar0 * RL(0.0);
+    pb = rmin(pb, rmin(el > RL(0.0) ? el : al1, er > RL(0.0) ? er : ar1));
+  }
+  real lo0 = RL(0.5) * rmin(al0, ar0), lo1 = RL(0.5) * pb;
+  Side sl = to_side(P, l, ls, RL(1.0), axis, lo0, lo1);
+  Side sr = to_side(P, r, rs, RL(-1.0), axis, lo0, lo1);
+  Flux F = P->riemann == ORC_RS_HLL ? riemann_hll(&sl, &sr) : riemann_hllc(&sl, &sr, w);
+  real mom[3];
+  unrot3(F.mom, axis, mom);
+  for (int i = 0; i < 3; i++) F.mom[i] = mom[i];
+  return F;
+}
+
+/* flux_between (flux.wgsl:254-347): the face between cell a and a + e */
+static Flux flux_between(const P_t* P, const real* u1, const real* u2, const real* pa, const real* pb,
+                         const int a[3], int axis) {
+  int ap[3] = {a[0], a[1], a[2]}, am[3] = {a[0], a[1], a[2]}, app[3] = {a[0], a[1], a[2]};
+  ap[axis] += 1; am[axis] -= 1; app[axis] += 2;
+  Q wl = load_q(P, u1, u2, a), wr = load_q(P, u1, u2, ap);
+  Q wll = load_q(P, u1, u2, am), wrr = load_q(P, u1, u2, app);
+  Q pl = load_pred(P, pa, pb, a), pr = load_pred(P, pa, pb, ap);
+  Q sl = edge(P, &wll, &wl, &wr, 1);
+  Q sr = edge(P, &wl, &wr, &wrr, 0);
+  return face_flux(P, &pl, &sl, &pr, &sr, axis);
+}
+
+/* ---------------- update: advance.wgsl:31-52, dual_energy.wgsl:73-87, fluff.wgsl:195-205 ------ */
+static void advance(const P_t* P, const real* u1, const real* u2, const real net[8], real p_half,
+                    real dt, real* o1, real* o2) {
+  real dtdx = dt / P->dx;
+  real rho = rmax(u1[0] + dtdx * net[0], P->rho_floor);
+  real mom[3];
+  for (int i = 0; i < 3; i++) mom[i] = (u1[1 + i] + RL(0.0)) + dtdx * net[1 + i];
+  real E = (u2[0] + RL(0.0)) + dtdx * (net[4] + RL(0.0));
+  real pdv = rmax(p_half, P->p_floor) * net[7];
+  real eint = u2[1] + dtdx * (net[5] + pdv);
+  real sc = u2[2] + dtdx * net[6];
+  real ke = RL(0.5) * (mom[0] * mom[0] + mom[1] * mom[1] + mom[2] * mom[2]) / rho;
+  real en0, en1;
+  {
+    real eth = E - ke;
+    real emin = eos_eint_from_p(P, P->p_floor);
+    if (eth > P->eta_dual * E && eth >= emin && eth >= RL(0.1) * RL(0.0)) { en0 = E; en1 = eth; }
+    else {
+      real ei = eint;
+      if (!(ei >= emin)) ei = emin;
+      en0 = ke + ei; en1 = ei;
+    }
+  }
+  real mout[3] = {mom[0], mom[1], mom[2]};
+  if (rho < P->fluff[0]) {
+    int changed = 0;
+    real te = en1;
+    real emax = rho * P->fluff[1];
+    if (te > emax) { te = emax; changed = 1; }
+    real pmax = rho * P->fluff[2];
+    real pm = len3(mom[0], mom[1], mom[2]);
+    if (pm > pmax) { for (int i = 0; i < 3; i++) mout[i] = mom[i] * (pmax / pm); changed = 1; }
+    if (changed) {
+      en0 = RL(0.5) * (mout[0] * mout[0] + mout[1] * mout[1] + mout[2] * mout[2]) / rho + te;
+      en1 = te;
+    }
+  }
+  o1[0] = rho; o1[1] = mout[0]; o1[2] = mout[1]; o1[3] = mout[2];
+  o2[0] = en0; o2[1] = en1; o2[2] = sc; o2[3] = RL(0.0);
+}
+
+/* ---------------- passes ---------------- */
+static size_t ncell(const P_t* P) { return (size_t)P->n[0] * P->n[1] * P->n[2]; }
+
+/* rates.wgsl:33-95 (cells marked solid are skipped; none are here) and the MAX reduction */
+real FN(max_rate)(const orc_params* prm, const real* u1, const real* u2) {
+  P_t P = load_params(prm);
+  real best = RL(0.0);
+  size_t nc = ncell(&P);
+  for (size_t o = 0; o < nc; o++) {
+    Q w = prim_from_cons(&P, u1 + 4 * o, u2 + 4 * o);
+    real signal = rsqrt_(w.b[2] * w.b[2] * RL(1.0) * RL(1.0) + RL(0.0));
+    real rate = rabs(w.a[1]) + rabs(w.a[2]) + rabs(w.a[3]) + (real)P.ndim * signal + RL(0.0);
+    if (u2[4 * o + 3] <= RL(0.5) && rate > best) best = rate;
+  }
+  return best;
+}
+
+/* clock.wgsl:132-242 with BANK, no gravity, radiation, cosmology, rotation or masses */
+void FN(clock)(const orc_params* prm, real clk[4], real max_rate) {
+  P_t P = load_params(prm);
+  real prev = clk[0];
+  if (prev > RL(0.0)) {
+    /* kahan1 (common.wgsl:366-370) */
+    real y = prev - clk[2];
+    real t = clk[1] + y;
+    clk[2] = (t - clk[1]) - y;
+    clk[1] = t;
+    clk[3] = clk[3] + RL(1.0);
+  }
+  real dt = P.cfl * P.dx / rmax(max_rate, RL(1.0e-30));
+  dt = rmin(dt, P.dt_max);
+  real remaining = P.t_end - clk[1];
+  if (remaining <= RL(1.0e-6) * rabs(P.t_end)) dt = RL(0.0);
+  else dt = rmin(dt, remaining / RL(1.0));
+  if (RL(1e30) - clk[1] < RL(0.5) * RL(1.0) * dt) dt = RL(0.0);
+  clk[0] = dt;
+}
+
+static void neighbors(const P_t* P, const real* u1, const real* u2, const int c[3], Q* q, Q nb[6]) {
+  *q = load_q(P, u1, u2, c);
+  int flat = P->n[2] == 1;
+  for (int a = 0; a < 3; a++) {
+    if (a == 2 && flat) { nb[4] = *q; nb[5] = *q; continue; }   /* predict.wgsl:53-55 */
+    int m[3] = {c[0], c[1], c[2]}, p[3] = {c[0], c[1], c[2]};
+    m[a] -= 1; p[a] += 1;
+    nb[2 * a] = load_q(P, u1, u2, m);
+    nb[2 * a + 1] = load_q(P, u1, u2, p);
+  }
+}
+
+/* predict.wgsl:32-119 */
+void FN(predict)(const orc_params* prm, const real* u1, const real* u2, real dt, real* pa, real* pb) {
+  P_t P = load_params(prm);
+  for (int k = 0; k < P.n[2]; k++)
+    for (int j = 0; j < P.n[1]; j++)
+      for (int i = 0; i < P.n[0]; i++) {
+        int c[3] = {i, j, k};
+        Q q;
+        Q nb[6];
+        neighbors(&P, u1, u2, c, &q, nb);
+        Q pr = predictor(&P, &q, nb, dt);
+        size_t o = cidx(&P, c) * 4;
+        for (int t = 0; t < 4; t++) { pa[o + t] = pr.a[t]; pb[o + t] = pr.b[t]; }
+      }
+}
+
+static void flux_pack(const Flux* F, real out[8]) {
+  out[0] = F->m; out[1] = F->mom[0]; out[2] = F->mom[1]; out[3] = F->mom[2];
+  out[4] = F->e; out[5] = F->ei; out[6] = F->s; out[7] = F->vn;
+}
+
+/* update.wgsl:189-405 with the tiled faces (tile.wgsl) */
+void FN(update)(const orc_params* prm, const real* u1, const real* u2, const real* pa, const real* pb,
+                real dt, real* o1, real* o2) {
+  P_t P = load_params(prm);
+  int flat = P.n[2] == 1;
+  for (int k = 0; k < P.n[2]; k++)
+    for (int j = 0; j < P.n[1]; j++)
+      for (int i = 0; i < P.n[0]; i++) {
+        int c[3] = {i, j, k};
+        real lo[3][8], hi[3][8];
+        for (int a = 0; a < 3; a++) {
+          if (a == 2 && flat) { for (int t = 0; t < 8; t++) { lo[a][t] = 0; hi[a][t] = 0; } continue; }
+          int m[3] = {i, j, k};
+          m[a] -= 1;
+          Flux fl = flux_between(&P, u1, u2, pa, pb, m, a);
+          Flux fh = flux_between(&P, u1, u2, pa, pb, c, a);
+          flux_pack(&fl, lo[a]);
+          flux_pack(&fh, hi[a]);
+        }
+        real net[8];
+        for (int t = 0; t < 8; t++)
+          net[t] = ((lo[0][t] - hi[0][t]) + (lo[1][t] - hi[1][t])) + (lo[2][t] - hi[2][t]);
+        size_t o = cidx(&P, c) * 4;
+        advance(&P, u1 + o, u2 + o, net, pb[o], dt, o1 + o, o2 + o);
+      }
+}
+
+void FN(face_mass)(const orc_params* prm, const real* u1, const real* u2, const real* pa, const real* pb,
+                   real* out6) {
+  P_t P = load_params(prm);
+  int flat = P.n[2] == 1;
+  for (int k = 0; k < P.n[2]; k++)
+    for (int j = 0; j < P.n[1]; j++)
+      for (int i = 0; i < P.n[0]; i++) {
+        int c[3] = {i, j, k};
+        size_t o = cidx(&P, c) * 6;
+        for (int a = 0; a < 3; a++) {
+          if (a == 2 && flat) { out6[o + 4] = 0; out6[o + 5] = 0; continue; }
+          int m[3] = {i, j, k};
+          m[a] -= 1;
+          out6[o + 2 * a] = flux_between(&P, u1, u2, pa, pb, m, a).m;
+          out6[o + 2 * a + 1] = flux_between(&P, u1, u2, pa, pb, c, a).m;
+        }
+      }
+}
+
+void FN(step)(const orc_params* prm, real* u1, real* u2, real clk[4], real* work) {
+  P_t P = load_params(prm);
+  size_t nc = ncell(&P);
+  real* pa = work;
+  real* pb = work + 4 * nc;
+  real* o1 = work + 8 * nc;
+  real* o2 = work + 12 * nc;
+  real rate = FN(max_rate)(prm, u1, u2);
+  FN(clock)(prm, clk, rate);
+  FN(predict)(prm, u1, u2, clk[0], pa, pb);
+  FN(update)(prm, u1, u2, pa, pb, clk[0], o1, o2);
+  memcpy(u1, o1, 4 * nc * sizeof(real));
+  memcpy(u2, o2, 4 * nc * sizeof(real));
+}
+
+/* ---------------- unit-level entry points ---------------- */
+void FN(edge4)(int limiter, const real far[4], const real near[4], const real scale[4], real out[4]) {
+  edge4(limiter, far, near, scale, out);
+}
+void FN(prim)(const orc_params* prm, const real u1[4], const real u2[4], real a[4], real b[4]) {
+  P_t P = load_params(prm);
+  Q q = prim_from_cons(&P, u1, u2);
+  for (int i = 0; i < 4; i++) { a[i] = q.a[i]; b[i] = q.b[i]; }
+}
+void FN(riemann)(const orc_params* prm, const real l[6], const real r[6], real w_hll, real out[8]) {
+  P_t P = load_params(prm);
+  Side sl, sr;
+  sl.rho = l[0]; sl.v[0] = l[1]; sl.v[1] = l[2]; sl.v[2] = l[3]; sl.p = l[4]; sl.s = l[5];
+  sr.rho = r[0]; sr.v[0] = r[1]; sr.v[1] = r[2]; sr.v[2] = r[3]; sr.p = r[4]; sr.s = r[5];
+  sl.pd = sl.p; sr.pd = sr.p;
+  sl.cs = eos_sound_speed(&P, sl.rho, sl.p); sl.eint = eos_eint_from_p(&P, sl.p);
+  sr.cs = eos_sound_speed(&P, sr.rho, sr.p); sr.eint = eos_eint_from_p(&P, sr.p);
+  Flux F = P.riemann == ORC_RS_HLL ? riemann_hll(&sl, &sr) : riemann_hllc(&sl, &sr, w_hll);
+  flux_pack(&F, out);
+}
+void FN(predictor)(const orc_params* prm, const real* st, real dt, real out[8]) {
+  P_t P = load_params(prm);
+  Q q;
+  Q nb[6];
+  for (int i = 0; i < 4; i++) { q.a[i] = st[i]; q.b[i] = st[4 + i]; }
+  for (int n = 0; n < 6; n++)
+    for (int i = 0; i < 4; i++) { nb[n].a[i] = st[8 * (n + 1) + i]; nb[n].b[i] = st[8 * (n + 1) + 4 + i]; }
+  Q o = predictor(&P, &q, nb, dt);
+  for (int i = 0; i < 4; i++) { out[i] = o.a[i]; out[4 + i] = o.b[i]; }
+}
