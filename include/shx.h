@@ -1,0 +1,127 @@
+/*
+ * libshx: the MI355X (gfx950) HIP backend of Shaderous's Godunov hydro update, behind a C ABI.
+ *
+ * The entry points replace the hydro step interface of the reference's uniform-grid simulation
+ * (`Sim`, src/sim.ts) and its block grid (`AmrSim`, src/amr/amr_sim.ts): a caller binds them
+ * through an FFI (an N-API addon for the TypeScript `Simulation` API; ctypes in Python). Each
+ * declaration names the reference member it stands in for.
+ *
+ * Conventions:
+ *  - Every entry returns an int status: SHX_OK (0) or a negative SHX_E* code; no C++ exception
+ *    crosses the ABI. shx_last_error() returns the calling thread's last message.
+ *  - The sim owns its device memory and its HIP stream; shx_destroy frees them
+ *    (Sim.destroy, src/sim.ts:3852-3876). Host buffers are copied in and out, never aliased.
+ *  - State layout (Sim.readState / setState, src/sim.ts:3031-3065, 3790-3807): two arrays of
+ *    4 floats per cell, cell ((k*ny + j)*nx + i):
+ *      u1 = (rho, rho vx, rho vy, rho vz)      u2 = (E, rho e, rho s, mark)
+ *    With precision 64 the same layout in doubles (the *_f64 variants).
+ *  - Stepping never blocks: shx_step enqueues graph launches on the sim's stream. Reads
+ *    synchronize that stream.
+ */
+#ifndef SHX_H
+#define SHX_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHX_OK 0
+#define SHX_EINVAL -1     /* bad argument or unsupported configuration (Sim refusal(), sim.ts:839) */
+#define SHX_ENOMEM -2     /* device allocation failed (the OOM refusal, sim.ts:2302-2311) */
+#define SHX_EDEVICE -3    /* a HIP call failed, or no usable GPU */
+#define SHX_ESTATE -4     /* non-finite clock or state (runToEnd's NaN check, sim.ts:3708) */
+
+/* Limiter (SimConfig.limiter, src/problems/types.ts:81-84) and Riemann solver (SimConfig.riemann). */
+enum { SHX_LIM_KOREN = 0, SHX_LIM_MC = 1, SHX_LIM_VANLEER = 2, SHX_LIM_MINMOD = 3 };
+enum { SHX_RS_HLLC = 0, SHX_RS_HLL = 1 };
+/* Boundary codes (src/hydro/boundary.wgsl:18-23): periodic, outflow, reflecting, outflow only. */
+enum { SHX_BC_PERIODIC = 0, SHX_BC_OUTFLOW = 1, SHX_BC_REFLECT = 2, SHX_BC_OUTFLOW_ONLY = 3 };
+/* Uniform-grid kernel path: PLAIN keeps the reference's passes (rates, clock, predict, tiled
+ * update); FUSED is one pass per step (rates of the next step in the epilogue). */
+enum { SHX_PATH_FUSED = 0, SHX_PATH_PLAIN = 1 };
+
+/* The SimConfig fields the hydro path reads (src/sim.ts:48-264, defaults configFor :645-736). */
+typedef struct {
+  int n[3];            /* cells along x, y, z; n[2] = 1 for a 2D run */
+  int ndim;            /* SimConfig.ndim: the rate is sum|v| + ndim c_s (P.geom.y) */
+  int bc_lo[3];
+  int bc_hi[3];
+  int limiter;
+  int riemann;
+  int isentropic;      /* SimConfig.isentropicRecon (ideal gas) */
+  int precision;       /* 32 (production) or 64 (the all-FP64 precision twin) */
+  int path;            /* SHX_PATH_* */
+  int device;          /* HIP device ordinal */
+  double dx;           /* cell size */
+  double gamma;        /* eosParams.gamma (ideal gas) */
+  double cfl;
+  double rho_floor;
+  double p_floor;
+  double eta_dual;
+  double shock_thresh;
+  double t_end;        /* 1e30 (or +inf) for none */
+  double fluff[3];     /* SimConfig.fluff: (rho below which the limits apply, max e, max v); rho 0 = off */
+  int tile[3];         /* fused kernel's column tile (x, y) and z chunk; 0 = autotuned default */
+} shx_config;
+
+typedef struct shx_sim shx_sim;
+
+/* Defaults of configFor (src/sim.ts:645-736) for an n^ndim grid. */
+void shx_config_default(shx_config* cfg, int nx, int ny, int nz);
+
+/* Sim.create (src/sim.ts:2283-2409) / createSim (src/create.ts:58-61): allocate the state,
+ * scratch and clock on the device and build the step graphs. */
+int shx_create(const shx_config* cfg, shx_sim** out);
+/* Sim.destroy (src/sim.ts:3852-3876) */
+int shx_destroy(shx_sim* sim);
+
+/* Sim.setState (src/sim.ts:3031-3065): ncell = nx*ny*nz cells of u1 and u2. Resets the clock
+ * to (dt 0, t = time, steps = steps) as clockState (src/sim.ts:1588-1593). */
+int shx_set_state(shx_sim* sim, const float* u1, const float* u2, size_t ncell, double time, int steps);
+int shx_set_state_f64(shx_sim* sim, const double* u1, const double* u2, size_t ncell, double time, int steps);
+/* Sim.readState (src/sim.ts:3790-3807) */
+int shx_read_state(shx_sim* sim, float* u1, float* u2);
+int shx_read_state_f64(shx_sim* sim, double* u1, double* u2);
+
+/* Sim.step (src/sim.ts:3592) / encodeSteps (src/sim.ts:3187-3328): enqueue `count` steps
+ * (rates -> clock -> predict -> update each, dt computed and applied on the GPU). */
+int shx_step(shx_sim* sim, int count);
+/* Wait for every enqueued step. */
+int shx_sync(shx_sim* sim);
+/* Sim.runToEnd (src/sim.ts:3689-3718): step in batches of `batch` until the clock holds at
+ * t_end (dt = 0) or max_steps; *steps = steps taken. SHX_ESTATE on a NaN clock. */
+int shx_run_to_end(shx_sim* sim, int batch, int max_steps, int* steps);
+/* The Clock struct (src/hydro/common.wgsl:84-96): (dt, t, tc, steps, a, ah, a1, hub, shear,
+ * shear_c, hold), as the step about to be taken sees it. */
+int shx_read_clock(shx_sim* sim, double out[11]);
+/* Sim.diagnostics (src/sim.ts:1647-1673, hydro/diagnostics.wgsl:13-98), with the sums in FP64:
+ * out[0] mass, [1..3] momentum, [4] kinetic energy, [5] internal energy (E - KE),
+ * [6] total energy E, [7] max Mach, [8] cells at the pressure floor, [9] rho*s total,
+ * [10..11] rho min/max, [12..13] p min/max. Deterministic (fixed reduction order). */
+int shx_diagnostics(shx_sim* sim, double out[16]);
+/* Sim.fluxMismatch (src/sim.ts:3724-3787) with the DEBUG_FLUX override (hydro/update.wgsl:187):
+ * every face's mass flux as each of its two cells computed it; *mismatched = faces whose two
+ * values differ in any bit, *faces = faces compared. */
+int shx_flux_mismatch(shx_sim* sim, int64_t* mismatched, int64_t* faces);
+/* Sim.profile (src/sim.ts:3541-3590): time `steps` steps pass by pass with HIP events.
+ * ms[0] rates, [1] clock, [2] predict, [3] update (plain path) or ms[3] the fused pass;
+ * ms[4] whole step. */
+int shx_profile(shx_sim* sim, int steps, double ms[5]);
+/* The name of the step's dominant kernel ("k_fused" or "k_update"), for profiles. */
+int shx_kernel_name(shx_sim* sim, char* buf, size_t len);
+
+/* The autotuner: time the fused kernel's tile candidates on the current state (which it
+ * restores) and keep the fastest; *chosen = (tx, ty, zc). */
+int shx_autotune(shx_sim* sim, int reps, int chosen[3]);
+
+/* The calling thread's last error message ("" if none). */
+const char* shx_last_error(void);
+/* Library version and the device it sees: "shx <ver> gfx950 <name>" */
+const char* shx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

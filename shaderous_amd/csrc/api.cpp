@@ -1,0 +1,520 @@
+// C-ABI of libshx for the uniform grid (include/shx.h): the sim object, its device memory,
+// stream and captured step graphs. Every entry catches its own failures and returns a status;
+// nothing throws across the ABI.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/shx.h"
+#include "common.hpp"
+#include "uniform.hpp"
+
+using namespace shx;
+
+namespace shx {
+thread_local std::string g_err;
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+}  // namespace shx
+
+#define HIPCHK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) return fail(SHX_EDEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+struct shx_sim {
+  shx_config cfg;
+  int es;                    // element size: 4 or 8
+  size_t ncell;
+  hipStream_t stream = nullptr;
+  void* U1[2] = {nullptr, nullptr};
+  void* U2[2] = {nullptr, nullptr};
+  void* PA = nullptr;
+  void* PB = nullptr;
+  void* clk = nullptr;       // 16 T: the Clock (11) and spare
+  unsigned long long* slot = nullptr;   // reduction slot: max rate of the current state
+  double* diag_part = nullptr;
+  double* diag_out = nullptr;
+  int diag_blocks = 256;
+  int cur = 0;
+  bool fused = false;
+  int tile = FUSED_32x32;
+  int zc = 32;
+  Params<float> pf;
+  Params<double> pd;
+  UniformOps ops{};
+  hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [parity][1 or 2 steps]
+};
+
+namespace {
+
+template <typename T>
+Params<T> make_params(const shx_config& c) {
+  Params<T> P;
+  for (int a = 0; a < 3; a++) { P.n[a] = c.n[a]; P.bc_lo[a] = c.bc_lo[a]; P.bc_hi[a] = c.bc_hi[a]; }
+  P.flat = c.n[2] == 1;
+  P.ndim = T(c.ndim);
+  P.gamma = T(c.gamma);
+  P.gm1 = P.gamma - T(1);
+  P.dx = T(c.dx);
+  P.cfl = T(c.cfl);
+  P.rho_floor = T(c.rho_floor);
+  P.p_floor = T(c.p_floor);
+  P.eta_dual = T(c.eta_dual);
+  P.shock_thresh = T(c.shock_thresh);
+  double te = c.t_end;
+  if (!(te < 1e30)) te = 1e30;
+  P.t_end = T(te);
+  P.dt_max = T(1e30);
+  for (int a = 0; a < 3; a++) P.fluff[a] = T(c.fluff[a]);
+  if (!(c.fluff[0] > 0)) { P.fluff[0] = T(0); P.fluff[1] = T(0); P.fluff[2] = T(0); }
+  else {
+    if (!(c.fluff[1] > 0)) P.fluff[1] = T(1e30);
+    if (!(c.fluff[2] > 0)) P.fluff[2] = T(1e30);
+  }
+  return P;
+}
+
+Launch launch_of(shx_sim* s) {
+  Launch L;
+  L.stream = s->stream;
+  L.params = s->es == 8 ? (const void*)&s->pd : (const void*)&s->pf;
+  L.ncell = s->ncell;
+  return L;
+}
+
+// One step from parity p into 1 - p.
+void enqueue_step(shx_sim* s, int p, bool debug = false) {
+  Launch L = launch_of(s);
+  const UniformOps& o = s->ops;
+  if (s->fused) {
+    o.clock(L, s->clk, s->slot);
+    o.fused(L, s->tile, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, s->zc, s->slot, debug);
+  } else {
+    o.rates(L, s->U1[p], s->U2[p], s->slot);
+    o.clock(L, s->clk, s->slot);
+    o.predict(L, s->U1[p], s->U2[p], s->PA, s->PB, s->clk);
+    o.update(L, s->U1[p], s->U2[p], s->PA, s->PB, s->U1[1 - p], s->U2[1 - p], s->clk, debug);
+  }
+}
+
+void destroy_graphs(shx_sim* s) {
+  for (int p = 0; p < 2; p++)
+    for (int k = 0; k < 2; k++)
+      if (s->graph[p][k]) { (void)hipGraphExecDestroy(s->graph[p][k]); s->graph[p][k] = nullptr; }
+}
+
+int build_graphs(shx_sim* s) {
+  destroy_graphs(s);
+  for (int p = 0; p < 2; p++)
+    for (int k = 0; k < 2; k++) {
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_step(s, p);
+      if (k == 1) enqueue_step(s, 1 - p);
+      HIPCHK(hipStreamEndCapture(s->stream, &g));
+      HIPCHK(hipGraphInstantiate(&s->graph[p][k], g, nullptr, nullptr, 0));
+      HIPCHK(hipGraphDestroy(g));
+    }
+  return SHX_OK;
+}
+
+// The reduction slot holds the max rate of the current state: the plain path recomputes it each
+// step (k_rates), the fused path in the previous step's epilogue; after a state change it is
+// recomputed here.
+int reset_rates(shx_sim* s) {
+  HIPCHK(hipMemsetAsync(s->slot, 0, sizeof(unsigned long long), s->stream));
+  if (s->fused) {
+    Launch L = launch_of(s);
+    s->ops.rates(L, s->U1[s->cur], s->U2[s->cur], s->slot);
+  }
+  return SHX_OK;
+}
+
+bool fused_supported(const shx_config& c) {
+  // z boundaries periodic (the chunked march wraps planes); x/y: periodic, outflow, reflecting
+  if (c.bc_lo[2] != SHX_BC_PERIODIC || c.bc_hi[2] != SHX_BC_PERIODIC) return false;
+  for (int a = 0; a < 2; a++)
+    if (c.bc_lo[a] == SHX_BC_OUTFLOW_ONLY || c.bc_hi[a] == SHX_BC_OUTFLOW_ONLY) return false;
+  for (int a = 0; a < 2; a++)
+    if (c.n[a] < 2) return false;
+  return true;
+}
+
+template <typename T>
+int set_state_t(shx_sim* s, const T* u1, const T* u2, size_t ncell, double time, int steps) {
+  if (!s) return fail(SHX_EINVAL, "null sim");
+  if ((int)sizeof(T) != s->es) return fail(SHX_EINVAL, "state precision %d does not match the sim's %d", (int)sizeof(T) * 8, s->es * 8);
+  if (ncell != s->ncell) return fail(SHX_EINVAL, "ncell %zu != %zu", ncell, s->ncell);
+  size_t bytes = ncell * 4 * sizeof(T);
+  for (size_t o = 0; o < 4 * ncell; o++)
+    if (!isfinite((double)u1[o]) || !isfinite((double)u2[o])) return fail(SHX_ESTATE, "non-finite state at %zu", o / 4);
+  s->cur = 0;
+  HIPCHK(hipMemcpyAsync(s->U1[0], u1, bytes, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->U2[0], u2, bytes, hipMemcpyHostToDevice, s->stream));
+  T clk[16] = {0};
+  clk[1] = T(time);
+  clk[3] = T(steps);
+  clk[4] = clk[5] = clk[6] = T(1);
+  HIPCHK(hipMemcpyAsync(s->clk, clk, sizeof clk, hipMemcpyHostToDevice, s->stream));
+  int rc = reset_rates(s);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return SHX_OK;
+}
+
+template <typename T>
+int read_state_t(shx_sim* s, T* u1, T* u2) {
+  if (!s) return fail(SHX_EINVAL, "null sim");
+  if ((int)sizeof(T) != s->es) return fail(SHX_EINVAL, "state precision mismatch");
+  size_t bytes = s->ncell * 4 * sizeof(T);
+  HIPCHK(hipMemcpyAsync(u1, s->U1[s->cur], bytes, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(u2, s->U2[s->cur], bytes, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return SHX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* shx_last_error(void) { return g_err.c_str(); }
+
+const char* shx_version(void) {
+  static std::string v;
+  hipDeviceProp_t prop;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    v = std::string("shx 0.1 ") + prop.gcnArchName + " " + prop.name;
+  else
+    v = "shx 0.1 (no device)";
+  return v.c_str();
+}
+
+void shx_config_default(shx_config* c, int nx, int ny, int nz) {
+  memset(c, 0, sizeof *c);
+  c->n[0] = nx; c->n[1] = ny; c->n[2] = nz;
+  c->ndim = nz == 1 ? 2 : 3;
+  c->limiter = SHX_LIM_KOREN;
+  c->riemann = SHX_RS_HLLC;
+  c->isentropic = 1;
+  c->precision = 32;
+  c->path = SHX_PATH_FUSED;
+  c->dx = 1.0 / nx;
+  c->gamma = 1.4;
+  c->cfl = 0.8;
+  c->rho_floor = 1e-10;
+  c->p_floor = 1e-14;
+  c->eta_dual = 1e-3;
+  c->shock_thresh = 0.5;
+  c->t_end = 1e30;
+}
+
+int shx_create(const shx_config* cfg, shx_sim** out) {
+  if (!cfg || !out) return fail(SHX_EINVAL, "null argument");
+  *out = nullptr;
+  const shx_config& c = *cfg;
+  for (int a = 0; a < 3; a++) {
+    if (c.n[a] < 1) return fail(SHX_EINVAL, "n[%d] = %d", a, c.n[a]);
+    if (c.bc_lo[a] < 0 || c.bc_lo[a] > 3 || c.bc_hi[a] < 0 || c.bc_hi[a] > 3) return fail(SHX_EINVAL, "boundary code on axis %d", a);
+  }
+  if (c.n[2] == 1 && (c.bc_lo[2] != 0 || c.bc_hi[2] != 0)) return fail(SHX_EINVAL, "a single layer along z must be periodic");
+  if (c.precision != 32 && c.precision != 64) return fail(SHX_EINVAL, "precision %d", c.precision);
+  if (!(c.gamma > 1) || !(c.dx > 0) || !(c.cfl > 0)) return fail(SHX_EINVAL, "gamma, dx and cfl must be positive (gamma > 1)");
+  UniformOps ops = uniform_ops(c.precision, c.limiter, c.riemann, c.isentropic ? 1 : 0);
+  if (!ops.rates) return fail(SHX_EINVAL, "no kernels built for precision %d limiter %d riemann %d isentropic %d", c.precision, c.limiter, c.riemann, c.isentropic);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c.device) return fail(SHX_EDEVICE, "no HIP device %d", c.device);
+  HIPCHK(hipSetDevice(c.device));
+
+  shx_sim* s = new shx_sim();
+  s->cfg = c;
+  s->es = c.precision == 64 ? 8 : 4;
+  s->ncell = (size_t)c.n[0] * c.n[1] * c.n[2];
+  s->ops = ops;
+  s->pf = make_params<float>(c);
+  s->pd = make_params<double>(c);
+  s->fused = c.path == SHX_PATH_FUSED && fused_supported(c);
+  s->tile = s->es == 8 ? FUSED_20x20 : FUSED_32x32;
+  s->zc = 32;
+  if (c.tile[0] > 0) {
+    int t = c.tile[0];
+    s->tile = t == 16 ? FUSED_20x20 : (c.tile[1] == 12 ? FUSED_32x16 : FUSED_32x32);
+    if (s->es == 8) s->tile = FUSED_20x20;
+  }
+  if (c.tile[2] > 0) s->zc = c.tile[2];
+  if (s->zc > c.n[2]) s->zc = c.n[2];
+
+  size_t bytes = s->ncell * 4 * s->es;
+  auto alloc = [&](void** p, size_t b) { return hipMalloc(p, b) == hipSuccess; };
+  bool ok = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess;
+  for (int p = 0; ok && p < 2; p++) ok = alloc(&s->U1[p], bytes) && alloc(&s->U2[p], bytes);
+  if (ok && !s->fused) ok = alloc(&s->PA, bytes) && alloc(&s->PB, bytes);
+  ok = ok && alloc(&s->clk, 16 * 8) && alloc((void**)&s->slot, 64) && alloc((void**)&s->diag_part, s->diag_blocks * 16 * 8) &&
+       alloc((void**)&s->diag_out, 16 * 8);
+  if (!ok) {
+    shx_destroy(s);
+    return fail(SHX_ENOMEM, "device allocation failed for %zu cells (%.2f GB per state copy)", s->ncell, bytes * 2 / 1e9);
+  }
+  (void)hipMemset(s->clk, 0, 16 * 8);
+  (void)hipMemset(s->slot, 0, 64);
+  int rc = build_graphs(s);
+  if (rc) { shx_destroy(s); return rc; }
+  *out = s;
+  return SHX_OK;
+}
+
+int shx_destroy(shx_sim* s) {
+  if (!s) return SHX_OK;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  destroy_graphs(s);
+  for (int p = 0; p < 2; p++) { (void)hipFree(s->U1[p]); (void)hipFree(s->U2[p]); }
+  (void)hipFree(s->PA); (void)hipFree(s->PB); (void)hipFree(s->clk); (void)hipFree(s->slot);
+  (void)hipFree(s->diag_part); (void)hipFree(s->diag_out);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return SHX_OK;
+}
+
+int shx_set_state(shx_sim* s, const float* u1, const float* u2, size_t ncell, double time, int steps) {
+  return set_state_t<float>(s, u1, u2, ncell, time, steps);
+}
+int shx_set_state_f64(shx_sim* s, const double* u1, const double* u2, size_t ncell, double time, int steps) {
+  return set_state_t<double>(s, u1, u2, ncell, time, steps);
+}
+int shx_read_state(shx_sim* s, float* u1, float* u2) { return read_state_t<float>(s, u1, u2); }
+int shx_read_state_f64(shx_sim* s, double* u1, double* u2) { return read_state_t<double>(s, u1, u2); }
+
+int shx_step(shx_sim* s, int count) {
+  if (!s || count < 0) return fail(SHX_EINVAL, "bad argument");
+  while (count >= 2) {
+    HIPCHK(hipGraphLaunch(s->graph[s->cur][1], s->stream));
+    count -= 2;
+  }
+  if (count == 1) {
+    HIPCHK(hipGraphLaunch(s->graph[s->cur][0], s->stream));
+    s->cur = 1 - s->cur;
+  }
+  return SHX_OK;
+}
+
+int shx_sync(shx_sim* s) {
+  if (!s) return fail(SHX_EINVAL, "null sim");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return SHX_OK;
+}
+
+int shx_read_clock(shx_sim* s, double out[11]) {
+  if (!s || !out) return fail(SHX_EINVAL, "bad argument");
+  unsigned char buf[16 * 8];
+  HIPCHK(hipMemcpyAsync(buf, s->clk, sizeof buf, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (int i = 0; i < 11; i++) out[i] = s->es == 8 ? ((double*)buf)[i] : (double)((float*)buf)[i];
+  return SHX_OK;
+}
+
+int shx_run_to_end(shx_sim* s, int batch, int max_steps, int* steps) {
+  if (!s) return fail(SHX_EINVAL, "null sim");
+  if (batch < 1) batch = 64;
+  int taken = 0;
+  double clk[11];
+  int rc = shx_read_clock(s, clk);
+  if (rc) return rc;
+  int start = (int)clk[3];
+  while (taken < max_steps) {
+    int n = batch < max_steps - taken ? batch : max_steps - taken;
+    rc = shx_step(s, n);
+    if (rc) return rc;
+    rc = shx_read_clock(s, clk);
+    if (rc) return rc;
+    if (!isfinite(clk[0]) || !isfinite(clk[1])) return fail(SHX_ESTATE, "the clock went non-finite (t = %g, dt = %g)", clk[1], clk[0]);
+    taken += n;
+    // the clock banks a step when the next one starts: steps taken = banked + (dt > 0)
+    if (clk[0] == 0) break;
+  }
+  if (steps) *steps = (int)clk[3] + (clk[0] > 0 ? 1 : 0) - start;
+  return SHX_OK;
+}
+
+int shx_diagnostics(shx_sim* s, double out[16]) {
+  if (!s || !out) return fail(SHX_EINVAL, "bad argument");
+  Launch L = launch_of(s);
+  s->ops.diag(L, s->U1[s->cur], s->U2[s->cur], s->diag_part, s->diag_blocks, s->diag_out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->diag_out, 16 * 8, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return SHX_OK;
+}
+
+int shx_flux_mismatch(shx_sim* s, int64_t* mismatched, int64_t* faces) {
+  if (!s) return fail(SHX_EINVAL, "null sim");
+  // one step in DEBUG_FLUX mode: the face masses land in the other parity's state arrays; the
+  // clock advances as for a step, so a copy of it is restored afterwards
+  unsigned char saved[16 * 8];
+  HIPCHK(hipMemcpyAsync(saved, s->clk, sizeof saved, hipMemcpyDeviceToHost, s->stream));
+  unsigned long long slot_saved;
+  HIPCHK(hipMemcpyAsync(&slot_saved, s->slot, 8, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  int p = s->cur;
+  enqueue_step(s, p, true);
+  HIPCHK(hipGetLastError());
+  size_t nc = s->ncell;
+  std::vector<double> a(4 * nc), b(4 * nc);
+  if (s->es == 8) {
+    HIPCHK(hipMemcpyAsync(a.data(), s->U1[1 - p], 32 * nc, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(b.data(), s->U2[1 - p], 32 * nc, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  } else {
+    std::vector<float> fa(4 * nc), fb(4 * nc);
+    HIPCHK(hipMemcpyAsync(fa.data(), s->U1[1 - p], 16 * nc, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(fb.data(), s->U2[1 - p], 16 * nc, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < 4 * nc; i++) { a[i] = fa[i]; b[i] = fb[i]; }
+  }
+  HIPCHK(hipMemcpyAsync(s->clk, saved, sizeof saved, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->slot, &slot_saved, 8, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  const int* n = s->cfg.n;
+  int64_t mis = 0, tot = 0;
+  auto lo = [&](size_t o, int axis) { return axis < 2 ? a[4 * o + 2 * axis] : b[4 * o]; };
+  auto hi = [&](size_t o, int axis) { return axis < 2 ? a[4 * o + 2 * axis + 1] : b[4 * o + 1]; };
+  for (int k = 0; k < n[2]; k++)
+    for (int j = 0; j < n[1]; j++)
+      for (int i = 0; i < n[0]; i++) {
+        size_t o = ((size_t)k * n[1] + j) * n[0] + i;
+        for (int axis = 0; axis < (n[2] == 1 ? 2 : 3); axis++) {
+          int c[3] = {i, j, k};
+          c[axis] += 1;
+          if (c[axis] >= n[axis]) {
+            if (s->cfg.bc_hi[axis] != SHX_BC_PERIODIC) continue;
+            c[axis] = 0;
+          }
+          size_t q = ((size_t)c[2] * n[1] + c[1]) * n[0] + c[0];
+          double x = hi(o, axis), y = lo(q, axis);
+          tot++;
+          if (memcmp(&x, &y, sizeof x) != 0) mis++;
+        }
+      }
+  if (mismatched) *mismatched = mis;
+  if (faces) *faces = tot;
+  return SHX_OK;
+}
+
+int shx_profile(shx_sim* s, int steps, double ms[5]) {
+  if (!s || !ms || steps < 1) return fail(SHX_EINVAL, "bad argument");
+  hipEvent_t ev[6];
+  for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&ev[i]));
+  double acc[5] = {0, 0, 0, 0, 0};
+  Launch L = launch_of(s);
+  for (int n = 0; n < steps; n++) {
+    int p = s->cur;
+    HIPCHK(hipEventRecord(ev[0], s->stream));
+    if (s->fused) {
+      s->ops.clock(L, s->clk, s->slot);
+      HIPCHK(hipEventRecord(ev[1], s->stream));
+      HIPCHK(hipEventRecord(ev[2], s->stream));
+      HIPCHK(hipEventRecord(ev[3], s->stream));
+      s->ops.fused(L, s->tile, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, s->zc, s->slot, false);
+    } else {
+      s->ops.rates(L, s->U1[p], s->U2[p], s->slot);
+      HIPCHK(hipEventRecord(ev[1], s->stream));
+      s->ops.clock(L, s->clk, s->slot);
+      HIPCHK(hipEventRecord(ev[2], s->stream));
+      s->ops.predict(L, s->U1[p], s->U2[p], s->PA, s->PB, s->clk);
+      HIPCHK(hipEventRecord(ev[3], s->stream));
+      s->ops.update(L, s->U1[p], s->U2[p], s->PA, s->PB, s->U1[1 - p], s->U2[1 - p], s->clk, false);
+    }
+    HIPCHK(hipEventRecord(ev[4], s->stream));
+    HIPCHK(hipEventSynchronize(ev[4]));
+    s->cur = 1 - p;
+    float t;
+    for (int i = 0; i < 4; i++) { HIPCHK(hipEventElapsedTime(&t, ev[i], ev[i + 1])); acc[i] += t; }
+    HIPCHK(hipEventElapsedTime(&t, ev[0], ev[4]));
+    acc[4] += t;
+  }
+  for (int i = 0; i < 5; i++) ms[i] = acc[i] / steps;
+  for (int i = 0; i < 6; i++) (void)hipEventDestroy(ev[i]);
+  return SHX_OK;
+}
+
+int shx_kernel_name(shx_sim* s, char* buf, size_t len) {
+  if (!s || !buf || len == 0) return fail(SHX_EINVAL, "bad argument");
+  snprintf(buf, len, "%s", s->fused ? "k_fused" : "k_update");
+  return SHX_OK;
+}
+
+int shx_autotune(shx_sim* s, int reps, int chosen[3]) {
+  if (!s) return fail(SHX_EINVAL, "null sim");
+  if (!s->fused || s->es == 8) {
+    if (chosen) { chosen[0] = 0; chosen[1] = 0; chosen[2] = s->zc; }
+    return SHX_OK;
+  }
+  if (reps < 1) reps = 3;
+  // keep the state: the candidates are timed on the current state and the state restored
+  size_t bytes = s->ncell * 16;
+  void *b1 = nullptr, *b2 = nullptr;
+  HIPCHK(hipMalloc(&b1, bytes));
+  HIPCHK(hipMalloc(&b2, bytes));
+  HIPCHK(hipMemcpyAsync(b1, s->U1[s->cur], bytes, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(b2, s->U2[s->cur], bytes, hipMemcpyDeviceToDevice, s->stream));
+  unsigned char clk_saved[128];
+  HIPCHK(hipMemcpyAsync(clk_saved, s->clk, 128, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  const int tiles[] = {FUSED_32x32, FUSED_32x16, FUSED_20x20};
+  const int zcs[] = {8, 16, 32, 64};
+  double best = 1e30;
+  int bt = s->tile, bz = s->zc;
+  Launch L = launch_of(s);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  for (int ti : tiles)
+    for (int z : zcs) {
+      if (z > s->cfg.n[2]) continue;
+      int p = s->cur;
+      s->ops.fused(L, ti, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, z, s->slot, false);  // warm
+      HIPCHK(hipEventRecord(e0, s->stream));
+      for (int r = 0; r < reps; r++)
+        s->ops.fused(L, ti, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, z, s->slot, false);
+      HIPCHK(hipEventRecord(e1, s->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float t;
+      HIPCHK(hipEventElapsedTime(&t, e0, e1));
+      if (t < best) { best = t; bt = ti; bz = z; }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  s->tile = bt;
+  s->zc = bz;
+  HIPCHK(hipMemcpyAsync(s->U1[s->cur], b1, bytes, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->U2[s->cur], b2, bytes, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->clk, clk_saved, 128, hipMemcpyHostToDevice, s->stream));
+  int rc = reset_rates(s);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  (void)hipFree(b1);
+  (void)hipFree(b2);
+  rc = build_graphs(s);
+  if (rc) return rc;
+  if (chosen) {
+    chosen[0] = bt == FUSED_32x32 ? 28 : (bt == FUSED_32x16 ? 28 : 16);
+    chosen[1] = bt == FUSED_32x32 ? 28 : (bt == FUSED_32x16 ? 12 : 16);
+    chosen[2] = bz;
+  }
+  return SHX_OK;
+}
+
+}  // extern "C"

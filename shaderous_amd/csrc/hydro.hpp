@@ -1,0 +1,500 @@
+// Device core of the MUSCL-Hancock + HLLC hydro update: primitives, boundary mapping, limited
+// reconstruction, the Hancock predictor, the face Riemann solve and the conservative update.
+// Shared by every kernel (uniform plain, uniform fused, blocks, ensemble) so that all of them
+// compute each quantity with the same instructions. Templated on the arithmetic type T (float
+// for the production path, double for the all-FP64 precision twin), the limiter, the Riemann
+// solver and the density-along-the-adiabat option.
+//
+// The arithmetic follows the reference's WGSL with every optional module off; each function
+// cites the file and lines it follows (paths under the reference's src/).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace shx {
+
+enum { LIM_KOREN = 0, LIM_MC = 1, LIM_VANLEER = 2, LIM_MINMOD = 3 };
+enum { RS_HLLC = 0, RS_HLL = 1 };
+enum { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECT = 2, BC_OUTFLOW_ONLY = 3 };
+
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<double> { using type = double4; };
+template <typename T> using T4 = typename Vec4<T>::type;
+
+// The Params values the hydro path reads (hydro/common.wgsl:4-79), in the run's precision.
+template <typename T>
+struct Params {
+  int n[3];
+  int bc_lo[3];
+  int bc_hi[3];
+  int flat;            // one layer along z (2D): z neighbours are the cell itself
+  T ndim;              // P.geom.y
+  T gamma, gm1;        // P.eos.x, P.eos.x - 1
+  T dx, cfl, rho_floor, p_floor, eta_dual, shock_thresh, t_end, dt_max;
+  T fluff[3];
+};
+
+template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+template <typename T> __device__ __forceinline__ T tabs(T a) { return a < T(0) ? -a : a; }
+template <typename T> __device__ __forceinline__ T tsign(T a) { return a > T(0) ? T(1) : (a < T(0) ? T(-1) : T(0)); }
+template <typename T> __device__ __forceinline__ T tclamp(T x, T lo, T hi) { return tmin(tmax(x, lo), hi); }
+__device__ __forceinline__ float tsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float tpow(float a, float b) { return powf(a, b); }
+__device__ __forceinline__ double tpow(double a, double b) { return pow(a, b); }
+// WGSL mix(x, y, a) = x (1 - a) + y a
+template <typename T> __device__ __forceinline__ T tmix(T x, T y, T a) { return x * (T(1) - a) + y * a; }
+// sign(a) * sign(b) > 0, formed from the signs so that small products do not underflow
+// (limiters/koren.wgsl:287-289)
+template <typename T> __device__ __forceinline__ bool same_sign(T a, T b) {
+  return (a > T(0) && b > T(0)) || (a < T(0) && b < T(0));
+}
+
+// Primitive record (hydro/q.wgsl:56): rho, v, p, scalar s, sound speed c.
+template <typename T>
+struct Prim {
+  T rho, vx, vy, vz, p, s, cs;
+};
+
+// prim_from_cons (hydro/prim.wgsl:131-156), ideal gas (eos/ideal.wgsl:235-267), pressure floor
+// without a Jeans floor (hydro/pfloor.wgsl:169-171).
+template <typename T>
+__device__ __forceinline__ Prim<T> prim_from_cons(const Params<T>& P, T4<T> u1, T4<T> u2) {
+  Prim<T> q;
+  T rho = tmax(u1.x, P.rho_floor);
+  T p = P.gm1 * u2.y;
+  if (!(p >= P.p_floor)) p = P.p_floor;
+  T ir = T(1) / rho;
+  q.rho = rho;
+  q.vx = u1.y * ir;
+  q.vy = u1.z * ir;
+  q.vz = u1.w * ir;
+  q.p = p;
+  q.s = u2.z * ir;
+  q.cs = tsqrt(P.gamma * p * ir);
+  return q;
+}
+
+// ---- boundary mapping: hydro/common.wgsl:433-539, hydro/boundary.wgsl:47-84 ----
+// One axis: the in-range index standing for i, the velocity sign along that axis and the
+// outflow-only side (+-1, 0 for none).
+struct AxisMap { int i; int flip; int side; };
+__device__ __forceinline__ AxisMap map_axis(int i, int n, int lo, int hi) {
+  AxisMap m{i, 1, 0};
+  if (i >= 0 && i < n) return m;
+  bool up = i >= n;
+  int k = up ? hi : lo;
+  if (k == BC_PERIODIC) {
+    int v = up ? i - n : i + n;
+    m.i = v < 0 ? 0 : (v > n - 1 ? n - 1 : v);
+  } else if (k == BC_REFLECT) {
+    m.i = up ? 2 * n - 1 - i : -1 - i;
+    m.flip = -1;
+  } else {
+    m.i = up ? n - 1 : 0;
+    if (k == BC_OUTFLOW_ONLY) m.side = up ? 1 : -1;
+  }
+  // (a reflecting axis of one or two cells, or a ragged tile's rim, can reach further than n
+  // beyond the face: keep every index in range)
+  m.i = m.i < 0 ? 0 : (m.i > n - 1 ? n - 1 : m.i);
+  return m;
+}
+// bc_velocity (common.wgsl:536-539) on one component
+template <typename T>
+__device__ __forceinline__ T bc_vel(T v, int flip, int side) {
+  T w = flip < 0 ? -v : v;
+  if (side != 0) {
+    T sd = T(side);
+    w = tmax(w * sd, T(0)) * sd;
+  }
+  return w;
+}
+
+// ---- limiters: hydro/limiters/*.wgsl, per channel ----
+// Twice the edge increment from the differences (far, near) and the channel's natural scale.
+// Koren (koren.wgsl:274-292): its early return gives exactly the blend's value per channel
+// (the blend weight is 1 for a smooth channel), so each channel is formed on its own.
+template <typename T, int LIM>
+__device__ __forceinline__ T edge1(T far, T near, T scale) {
+  if constexpr (LIM == LIM_KOREN) {
+    const T SMOOTH = T(0.01);
+    T fr = (far + T(2) * near) / T(3);
+    T s = SMOOTH * scale;
+    T s2 = s * s;
+    T ss = tmax(s2, T(1e-30));
+    T q = far * far + near * near;
+    bool as_is = (s2 >= T(1e-30) && s2 < T(1e30)) || (s == T(0));
+    T eta;
+    if (as_is) {
+      eta = q / ss;
+    } else {
+      T k = T(1) / tmax(tabs(s), T(1e-37));
+      T fk = far * k, nk = near * k;
+      eta = fk * fk + nk * nk;
+    }
+    T m = tmin(tmin(T(2) * tabs(far), T(2) * tabs(near)), tabs(fr));
+    T lim = same_sign(far, near) ? (near > T(0) ? m : -m) : T(0);
+    T w = tclamp(T(1.5) - eta, T(0), T(1));
+    return (w == T(1)) ? fr : tmix(lim, fr, w);
+  } else if constexpr (LIM == LIM_MC) {
+    T m = tmin(tmin(T(2) * tabs(far), T(2) * tabs(near)), T(0.5) * tabs(far + near));
+    return same_sign(far, near) ? (far > T(0) ? m : -m) : T(0);
+  } else if constexpr (LIM == LIM_MINMOD) {
+    T m = tmin(tabs(far), tabs(near));
+    return same_sign(far, near) ? (far > T(0) ? m : -m) : T(0);
+  } else {
+    return same_sign(far, near) ? T(2) * far * (near / (far + near)) : T(0);
+  }
+}
+template <int LIM> constexpr bool lim_symmetric() { return LIM != LIM_KOREN; }
+
+// Twice the increment of each reconstructed channel (rho, vx, vy, vz, p, s) toward a face.
+template <typename T>
+struct Inc { T rho, vx, vy, vz, p, s; };
+
+// isen_inc / isen_blend (hydro/q.wgsl:183-213) with no reference state and no potential.
+template <typename T, int LIM>
+__device__ __forceinline__ T isen_blend(const Params<T>& P, const Prim<T>& l, const Prim<T>& c, const Prim<T>& r,
+                                        T x, T bx, T sgn) {
+  const T ISEN_SMOOTH = T(0.2), ISEN_ADIABAT = T(0.01), ISEN_SHEAR = T(0.1), ISEN_CAP = T(0.25), ISEN_ROOM = T(0.01);
+  if (!(l.rho > T(0) && l.p > T(0) && c.rho > T(0) && c.p > T(0) && r.rho > T(0) && r.p > T(0))) {
+    return x + tclamp(T(0) - x, T(0), T(0));
+  }
+  T gi = T(1) / P.gamma;
+  T tl = l.rho * tpow(c.p / l.p, gi) - c.rho;
+  T tr = r.rho * tpow(c.p / r.p, gi) - c.rho;
+  T far = sgn > T(0) ? -tl : tr;
+  T near = sgn > T(0) ? tr : -tl;
+  T d = edge1<T, LIM>(far, near, c.rho);
+  T pf = c.p + T(0.5) * sgn * bx;
+  T wn = sgn > T(0) ? r.rho : l.rho;
+  T rf = tclamp((c.rho + T(0.5) * sgn * d) * tpow(tmax(pf, T(1e-30)) / c.p, gi), tmin(c.rho, wn), tmax(c.rho, wn));
+  T ks = tclamp(T(2) - tmax(tabs(l.p - c.p), tabs(r.p - c.p)) / (ISEN_SMOOTH * c.p), T(0), T(1));
+  T ka = tclamp(T(2) - tmax(tabs(tl), tabs(tr)) / (ISEN_ADIABAT * c.rho), T(0), T(1));
+  T dxr = r.vx - c.vx, dyr = r.vy - c.vy, dzr = r.vz - c.vz;
+  T dxl = c.vx - l.vx, dyl = c.vy - l.vy, dzl = c.vz - l.vz;
+  T dvr = tsqrt(dxr * dxr + dyr * dyr + dzr * dzr);
+  T dvl = tsqrt(dxl * dxl + dyl * dyl + dzl * dzl);
+  T kv = tclamp(T(2) - tmax(dvr, dvl) / (ISEN_SHEAR * tmax(c.cs, T(1e-30))), T(0), T(1));
+  T bound = tmax(ISEN_CAP * tmin(tabs(c.rho - l.rho), tabs(r.rho - c.rho)), ISEN_ROOM * c.rho * ks * ka * kv);
+  T i0 = T(2) * sgn * (rf - c.rho);
+  return x + tclamp(i0 - x, -bound, bound);
+}
+
+// edge_up (UP) / edge_dn (hydro/q.wgsl:214-239), no potential.
+template <typename T, int LIM, bool ISEN, bool UP>
+__device__ __forceinline__ Inc<T> edge(const Params<T>& P, const Prim<T>& l, const Prim<T>& c, const Prim<T>& r) {
+  Inc<T> o;
+  constexpr bool fwd = UP || lim_symmetric<LIM>();
+  auto e = [&](T lo, T mid, T hi, T scale) {
+    T below = mid - lo, above = hi - mid;
+    return fwd ? edge1<T, LIM>(below, above, scale) : edge1<T, LIM>(above, below, scale);
+  };
+  o.rho = e(l.rho, c.rho, r.rho, c.rho);
+  o.vx = e(l.vx, c.vx, r.vx, c.cs);
+  o.vy = e(l.vy, c.vy, r.vy, c.cs);
+  o.vz = e(l.vz, c.vz, r.vz, c.cs);
+  o.p = e(l.p, c.p, r.p, c.p);
+  o.s = e(l.s, c.s, r.s, T(0));
+  if constexpr (ISEN) o.rho = isen_blend<T, LIM>(P, l, c, r, o.rho, o.p, UP ? T(1) : T(-1));
+  return o;
+}
+
+// Half-step state (hydro/predictor.wgsl:46-209): rho, v, p, s at t + dt/2, the shock flag and
+// the first-order flag.
+template <typename T>
+struct Half { T rho, vx, vy, vz, p, s, shock, fo; };
+
+// One axis's contribution to the predictor: the face-to-face slope and the positivity terms.
+template <typename T>
+struct AxisSlope { Inc<T> s; T big; T fall; };
+
+template <typename T, int LIM, bool ISEN>
+__device__ __forceinline__ AxisSlope<T> axis_slope(const Params<T>& P, const Prim<T>& m, const Prim<T>& q,
+                                                   const Prim<T>& p, Inc<T>* up_out, Inc<T>* dn_out) {
+  AxisSlope<T> r;
+  Inc<T> u = edge<T, LIM, ISEN, true>(P, m, q, p);
+  if constexpr (lim_symmetric<LIM>()) {
+    r.s = u;
+    r.big = tabs(u.p);
+    r.fall = T(-0.5) * tabs(u.rho);
+    if (up_out) *up_out = u;
+    if (dn_out) *dn_out = u;
+  } else {
+    Inc<T> d = edge<T, LIM, ISEN, false>(P, m, q, p);
+    r.s.rho = T(0.5) * (u.rho + d.rho);
+    r.s.vx = T(0.5) * (u.vx + d.vx);
+    r.s.vy = T(0.5) * (u.vy + d.vy);
+    r.s.vz = T(0.5) * (u.vz + d.vz);
+    r.s.p = T(0.5) * (u.p + d.p);
+    r.s.s = T(0.5) * (u.s + d.s);
+    r.big = tmax(tabs(u.p), tabs(d.p));
+    r.fall = tmin(T(0.5) * u.rho, T(-0.5) * d.rho);
+    if (up_out) *up_out = u;
+    if (dn_out) *dn_out = d;
+  }
+  return r;
+}
+
+// The Hancock half step from the cell, its six neighbours' pressures and normal velocities
+// (for the shock detector) and the three axis slopes.
+template <typename T>
+__device__ __forceinline__ Half<T> hancock(const Params<T>& P, const Prim<T>& q, const AxisSlope<T> sl[3],
+                                           T conv, T jump, T dt) {
+  Half<T> o;
+  T big = tmax(tmax(tmax(T(0), sl[0].big), sl[1].big), sl[2].big);
+  T fall = tmin(tmin(tmin(T(0), sl[0].fall), sl[1].fall), sl[2].fall);
+  T shock = (conv < T(0) && jump > P.shock_thresh) ? T(1) : T(0);
+  T h = T(0.5) * dt / P.dx;
+  T rho = q.rho;
+  const Inc<T>& sx = sl[0].s;
+  const Inc<T>& sy = sl[1].s;
+  const Inc<T>& sz = sl[2].s;
+  T dv = sx.vx + sy.vy + sz.vz;
+  T da0 = q.vx * sx.rho + q.vy * sy.rho + q.vz * sz.rho;
+  T da1 = q.vx * sx.vx + q.vy * sy.vx + q.vz * sz.vx;
+  T da2 = q.vx * sx.vy + q.vy * sy.vy + q.vz * sz.vy;
+  T da3 = q.vx * sx.vz + q.vy * sy.vz + q.vz * sz.vz;
+  T db0 = q.vx * sx.p + q.vy * sy.p + q.vz * sz.p;
+  T db1 = q.vx * sx.s + q.vy * sy.s + q.vz * sz.s;
+  T ir = T(1) / rho;
+  o.rho = rho - h * (da0 + rho * dv);
+  o.vx = q.vx - h * (da1 + sx.p * ir);
+  o.vy = q.vy - h * (da2 + sy.p * ir);
+  o.vz = q.vz - h * (da3 + sz.p * ir);
+  o.p = q.p - h * (db0 + rho * q.cs * q.cs * dv);
+  o.s = q.s - h * db1;
+  o.shock = shock;
+  T rmin = o.rho + fall;
+  T pmin = o.p - T(0.5) * big;
+  o.fo = T(0);
+  if (!(rmin > T(0) && pmin > T(0))) {
+    o.rho = q.rho; o.vx = q.vx; o.vy = q.vy; o.vz = q.vz; o.p = q.p; o.s = q.s;
+    o.fo = T(1);
+  }
+  return o;
+}
+
+// Shock detector (predictor.wgsl:114-134): converging flow and the largest relative pressure
+// jump across the cell.
+template <typename T>
+__device__ __forceinline__ T jump_of(T pm, T pp) {
+  return tabs(pp - pm) / tmin(pp, pm);
+}
+
+// ---- face: hydro/flux.wgsl:60-125, riemann/hllc.wgsl, riemann/hll.wgsl ----
+template <typename T>
+struct Flux { T m, mx, my, mz, e, ei, s, vn; };
+
+// One side of a face (to_side, flux.wgsl:60-85) in the face frame: n normal, t1/t2 the
+// cyclically following components (rot, flux.wgsl:37-41).
+template <typename T>
+struct Side { T rho, vn, vt1, vt2, p, s, cs, eint; };
+
+template <typename T>
+__device__ __forceinline__ Side<T> make_side(const Params<T>& P, T rho, T vn, T vt1, T vt2, T p, T s, T lo_rho, T lo_p) {
+  Side<T> o;
+  o.rho = tmax(rho, tmax(lo_rho, P.rho_floor));
+  o.vn = vn; o.vt1 = vt1; o.vt2 = vt2;
+  o.s = s;
+  o.p = tmax(p, tmax(lo_p, P.p_floor));
+  o.cs = tsqrt(P.gamma * o.p / o.rho);
+  o.eint = o.p / P.gm1;
+  return o;
+}
+
+// HLLC blended toward HLL by w (riemann/hllc.wgsl:4-85), or HLL (riemann/hll.wgsl:87-121).
+template <typename T, int RS>
+__device__ __forceinline__ Flux<T> riemann(const Side<T>& l, const Side<T>& r, T w_hll) {
+  T sl = tmin(l.vn - l.cs, r.vn - r.cs);
+  T sr = tmax(l.vn + l.cs, r.vn + r.cs);
+  T el = l.eint + T(0.5) * l.rho * (l.vn * l.vn + l.vt1 * l.vt1 + l.vt2 * l.vt2);
+  T er = r.eint + T(0.5) * r.rho * (r.vn * r.vn + r.vt1 * r.vt1 + r.vt2 * r.vt2);
+  T ml0 = l.rho * l.vn, ml1 = l.rho * l.vt1, ml2 = l.rho * l.vt2;
+  T mr0 = r.rho * r.vn, mr1 = r.rho * r.vt1, mr2 = r.rho * r.vt2;
+  T fpl0 = ml0 * l.vn + l.p, fpl1 = ml1 * l.vn, fpl2 = ml2 * l.vn;
+  T fpr0 = mr0 * r.vn + r.p, fpr1 = mr1 * r.vn, fpr2 = mr2 * r.vn;
+  T fel = (el + l.p) * l.vn;
+  T fer = (er + r.p) * r.vn;
+  Flux<T> o;
+  if (sl >= T(0)) {
+    o.m = ml0; o.mx = fpl0; o.my = fpl1; o.mz = fpl2; o.e = fel; o.vn = l.vn;
+  } else if (sr <= T(0)) {
+    o.m = mr0; o.mx = fpr0; o.my = fpr1; o.mz = fpr2; o.e = fer; o.vn = r.vn;
+  } else {
+    T ql = l.rho * (sl - l.vn);
+    T qr = r.rho * (sr - r.vn);
+    T inv = T(1) / (sr - sl);
+    T slsr = sl * sr;
+    T hm = (sr * ml0 - sl * mr0 + slsr * (r.rho - l.rho)) * inv;
+    T hp0 = (sr * fpl0 - sl * fpr0 + slsr * (mr0 - ml0)) * inv;
+    T hp1 = (sr * fpl1 - sl * fpr1 + slsr * (mr1 - ml1)) * inv;
+    T hp2 = (sr * fpl2 - sl * fpr2 + slsr * (mr2 - ml2)) * inv;
+    T he = (sr * fel - sl * fer + slsr * (er - el)) * inv;
+    if constexpr (RS == RS_HLL) {
+      o.m = hm; o.mx = hp0; o.my = hp1; o.mz = hp2; o.e = he;
+      o.vn = (r.p - l.p + l.vn * ql - r.vn * qr) / (ql - qr);
+    } else {
+      if (!(ql < T(0) && qr > T(0))) {
+        o.m = hm; o.mx = hp0; o.my = hp1; o.mz = hp2; o.e = he;
+        o.vn = T(0.5) * (l.vn + r.vn);
+      } else {
+        T sm = (r.p - l.p + l.vn * ql - r.vn * qr) / (ql - qr);
+        // the star state as differences from the outer state (hllc.wgsl:52-74)
+        bool left = sm >= T(0);
+        T S = left ? sl : sr;
+        T rho = left ? l.rho : r.rho;
+        T un = left ? l.vn : r.vn;
+        T u1 = left ? l.vt1 : r.vt1;
+        T u2 = left ? l.vt2 : r.vt2;
+        T p = left ? l.p : r.p;
+        T q = left ? ql : qr;
+        T E = left ? el : er;
+        T fm = left ? ml0 : mr0;
+        T f0 = left ? fpl0 : fpr0;
+        T f1 = left ? fpl1 : fpr1;
+        T f2 = left ? fpl2 : fpr2;
+        T fe = left ? fel : fer;
+        T k = (sm - un) / (S - sm);
+        T dr = rho * k;
+        T f = rho + dr;
+        T cm = fm + S * dr;
+        T c0 = f0 + S * (sm * dr + rho * (sm - un));
+        T c1 = f1 + S * (dr * u1);
+        T c2 = f2 + S * (dr * u2);
+        T ce = fe + S * (E * k + f * (sm - un) * (sm + p / q));
+        o.m = tmix(cm, hm, w_hll);
+        o.mx = tmix(c0, hp0, w_hll);
+        o.my = tmix(c1, hp1, w_hll);
+        o.mz = tmix(c2, hp2, w_hll);
+        o.e = tmix(ce, he, w_hll);
+        o.vn = sm;
+      }
+    }
+  }
+  // internal energy and the scalar ride on the mass flux, upwinded
+  bool up = o.m >= T(0);
+  o.ei = o.m * (up ? l.eint / l.rho : r.eint / r.rho);
+  o.s = o.m * (up ? l.s : r.s);
+  return o;
+}
+
+// face_flux (flux.wgsl:92-125) between the half-step states hl (left) and hr (right) along
+// `axis`, given twice the increments from each cell toward the face (il: the left cell's up
+// increment, ir: the right cell's down increment). Momentum comes back in grid components.
+template <typename T, int RS>
+__device__ __forceinline__ Flux<T> face_flux(const Params<T>& P, const Half<T>& hl, const Inc<T>& il,
+                                             const Half<T>& hr, const Inc<T>& ir, int axis) {
+  T w = tmax(hl.shock, hr.shock);
+  T lo_rho = T(0.5) * tmin(hl.rho, hr.rho);
+  T lo_p = T(0.5) * tmin(hl.p, hr.p);
+  T hsl = T(0.5) * (T(1) - hl.fo);
+  T hsr = T(-0.5) * (T(1) - hr.fo);
+  // rot (flux.wgsl:37-41): axis 0 -> (x, y, z), 1 -> (y, z, x), 2 -> (z, x, y)
+  T lv[3] = {hl.vx + hsl * il.vx, hl.vy + hsl * il.vy, hl.vz + hsl * il.vz};
+  T rv[3] = {hr.vx + hsr * ir.vx, hr.vy + hsr * ir.vy, hr.vz + hsr * ir.vz};
+  int a1 = axis == 2 ? 0 : axis + 1;
+  int a2 = axis == 0 ? 2 : axis - 1;
+  Side<T> sl = make_side(P, hl.rho + hsl * il.rho, lv[axis], lv[a1], lv[a2], hl.p + hsl * il.p, hl.s + hsl * il.s,
+                         lo_rho, lo_p);
+  Side<T> sr = make_side(P, hr.rho + hsr * ir.rho, rv[axis], rv[a1], rv[a2], hr.p + hsr * ir.p, hr.s + hsr * ir.s,
+                         lo_rho, lo_p);
+  Flux<T> F = riemann<T, RS>(sl, sr, w);
+  // unrot: the face frame's (n, t1, t2) back to grid components
+  T g[3];
+  g[axis] = F.mx; g[a1] = F.my; g[a2] = F.mz;
+  F.mx = g[0]; F.my = g[1]; F.mz = g[2];
+  return F;
+}
+
+// advance (hydro/advance.wgsl:31-52), dual_energy (hydro/dual_energy.wgsl:73-87) and the
+// tenuous-gas limits (hydro/fluff.wgsl:195-205). net is the net inflow per unit dt/dx.
+template <typename T>
+__device__ __forceinline__ void advance(const Params<T>& P, T4<T> u1, T4<T> u2, const Flux<T>& net, T p_half, T dtdx,
+                                        T4<T>& o1, T4<T>& o2) {
+  T rho = tmax(u1.x + dtdx * net.m, P.rho_floor);
+  T mx = u1.y + dtdx * net.mx;
+  T my = u1.z + dtdx * net.my;
+  T mz = u1.w + dtdx * net.mz;
+  T E = u2.x + dtdx * net.e;
+  T pdv = tmax(p_half, P.p_floor) * net.vn;
+  T eint = u2.y + dtdx * (net.ei + pdv);
+  T sc = u2.z + dtdx * net.s;
+  T ke = T(0.5) * (mx * mx + my * my + mz * mz) / rho;
+  T eth = E - ke;
+  T emin = P.p_floor / P.gm1;
+  T en0, en1;
+  if (eth > P.eta_dual * E && eth >= emin && eth >= T(0)) {
+    en0 = E; en1 = eth;
+  } else {
+    T ei = eint;
+    if (!(ei >= emin)) ei = emin;
+    en0 = ke + ei; en1 = ei;
+  }
+  if (rho < P.fluff[0]) {
+    bool changed = false;
+    T emax = rho * P.fluff[1];
+    if (en1 > emax) { en1 = emax; changed = true; }
+    T pmax = rho * P.fluff[2];
+    T pm = tsqrt(mx * mx + my * my + mz * mz);
+    if (pm > pmax) { T f = pmax / pm; mx *= f; my *= f; mz *= f; changed = true; }
+    if (changed) en0 = T(0.5) * (mx * mx + my * my + mz * mz) / rho + en1;
+  }
+  o1.x = rho; o1.y = mx; o1.z = my; o1.w = mz;
+  o2.x = en0; o2.y = en1; o2.z = sc; o2.w = T(0);
+}
+
+// Net inflow (flux.wgsl:385-394) from the lower and upper face of each axis.
+template <typename T>
+__device__ __forceinline__ Flux<T> net_of(const Flux<T>& xl, const Flux<T>& xh, const Flux<T>& yl, const Flux<T>& yh,
+                                          const Flux<T>& zl, const Flux<T>& zh) {
+  Flux<T> n;
+  n.m = ((xl.m - xh.m) + (yl.m - yh.m)) + (zl.m - zh.m);
+  n.mx = ((xl.mx - xh.mx) + (yl.mx - yh.mx)) + (zl.mx - zh.mx);
+  n.my = ((xl.my - xh.my) + (yl.my - yh.my)) + (zl.my - zh.my);
+  n.mz = ((xl.mz - xh.mz) + (yl.mz - yh.mz)) + (zl.mz - zh.mz);
+  n.e = ((xl.e - xh.e) + (yl.e - yh.e)) + (zl.e - zh.e);
+  n.ei = ((xl.ei - xh.ei) + (yl.ei - yh.ei)) + (zl.ei - zh.ei);
+  n.s = ((xl.s - xh.s) + (yl.s - yh.s)) + (zl.s - zh.s);
+  n.vn = ((xl.vn - xh.vn) + (yl.vn - yh.vn)) + (zl.vn - zh.vn);
+  return n;
+}
+
+// The signal rate of a cell (hydro/rates.wgsl:60-62).
+template <typename T>
+__device__ __forceinline__ T rate_of(const Params<T>& P, const Prim<T>& w) {
+  T signal = tsqrt(w.cs * w.cs);
+  return tabs(w.vx) + tabs(w.vy) + tabs(w.vz) + P.ndim * signal;
+}
+
+// Non-negative floats order as their bit patterns: the reductions compare those.
+__device__ __forceinline__ uint32_t order_bits(float x) { return __float_as_uint(x); }
+__device__ __forceinline__ unsigned long long order_bits(double x) { return (unsigned long long)__double_as_longlong(x); }
+
+// The clock pass (hydro/clock.wgsl:132-242) on one thread: bank the step just taken
+// (Kahan-summed t), then dt = cfl dx / max rate, clamped by dt_max and the end time.
+// clk = (dt, t, tc, steps, ...11 entries).
+template <typename T>
+__device__ __forceinline__ void clock_pass(const Params<T>& P, T* clk, T max_rate) {
+  T prev = clk[0];
+  if (prev > T(0)) {
+    T y = prev - clk[2];
+    T t = clk[1] + y;
+    T d = t - clk[1];
+    clk[2] = d - y;
+    clk[1] = t;
+    clk[3] = clk[3] + T(1);
+  }
+  T dt = P.cfl * P.dx / tmax(max_rate, T(1e-30));
+  dt = tmin(dt, P.dt_max);
+  T remaining = P.t_end - clk[1];
+  if (remaining <= T(1e-6) * tabs(P.t_end)) dt = T(0);
+  else dt = tmin(dt, remaining);
+  if (T(1e30) - clk[1] < T(0.5) * dt) dt = T(0);
+  if (clk[10] != T(0)) dt = T(0);
+  clk[0] = dt;
+}
+
+}  // namespace shx

@@ -1,0 +1,43 @@
+// Interface between the C-ABI layer and the uniform-grid kernels: type-erased launchers, one
+// table per (precision, limiter, Riemann solver, isentropic) combination.
+#pragma once
+#include <algorithm>
+#include "hydro.hpp"
+
+namespace shx {
+
+struct Launch {
+  hipStream_t stream;
+  const void* params;   // Params<T> of the sim's precision
+  size_t ncell;
+};
+
+// Column tiles of the fused kernel (block BX x BY threads, tile (BX - 4) x (BY - 4) cells).
+enum { FUSED_20x20 = 0, FUSED_32x16 = 1, FUSED_32x32 = 2, FUSED_NTILES = 3 };
+
+struct UniformOps {
+  void (*rates)(const Launch&, const void* u1, const void* u2, unsigned long long* slot);
+  void (*clock)(const Launch&, void* clk, unsigned long long* slot);
+  void (*predict)(const Launch&, const void* u1, const void* u2, void* pa, void* pb, const void* clk);
+  void (*update)(const Launch&, const void* u1, const void* u2, const void* pa, const void* pb, void* o1, void* o2,
+                 const void* clk, bool debug);
+  void (*fused)(const Launch&, int tile, const void* u1, const void* u2, void* o1, void* o2, const void* clk, int zc,
+                unsigned long long* next_slot, bool debug);
+  void (*diag)(const Launch&, const void* u1, const void* u2, double* part, int nblocks, double* out);
+};
+
+// Each translation unit instantiates a subset; a null table means "not built".
+UniformOps uniform_ops_koren32(int rs, int isen);
+UniformOps uniform_ops_mc32(int rs, int isen);
+UniformOps uniform_ops_other32(int lim, int rs);
+UniformOps uniform_ops_f64(int lim, int rs, int isen);
+
+inline UniformOps uniform_ops(int precision, int lim, int rs, int isen) {
+  if (precision == 64) return uniform_ops_f64(lim, rs, isen);
+  if (lim == LIM_KOREN) return uniform_ops_koren32(rs, isen);
+  if (lim == LIM_MC) return uniform_ops_mc32(rs, isen);
+  if (isen) return UniformOps{};
+  return uniform_ops_other32(lim, rs);
+}
+
+}  // namespace shx

@@ -1,0 +1,633 @@
+// Uniform-grid hydro step kernels for gfx950.
+//
+// PLAIN path, one kernel per reference pass (src/sim.ts encodeSteps :3187-3328):
+//   k_rates   hydro/rates.wgsl:33-95 + the MAX reduction (gpu/reduce.wgsl): wave and block
+//             reductions, one atomicMax per block on the ordered bit pattern
+//   k_clock   hydro/clock.wgsl:132-242 on one thread; clears the reduction slot
+//   k_predict hydro/predict.wgsl:32-119: the Hancock half step per cell -> PA, PB
+//   k_update  hydro/update.wgsl:189-405 with the face tile (hydro/tile.wgsl:166-213): every
+//             face of the block's box computed once into LDS, then each cell's update
+//
+// FUSED path, one kernel per step (k_fused): a block owns a column tile of (BX - 4) x (BY - 4)
+// cells plus a two-cell halo ring and marches along z through a chunk of planes. Each plane's
+// primitives are converted once, staged in LDS for the x/y neighbours and kept in registers for
+// the z neighbours; each cell's half step and edge increments are computed once; each face is
+// solved once; the conservative update of the plane behind is written, and the next step's
+// signal rate is folded into a per-block maximum in the same pass. HBM traffic is one read and
+// one write of the state per step plus the halo re-reads the caches do not absorb.
+//
+// Both paths evaluate the same device functions (hydro.hpp) in the same order on the same
+// inputs, so their results agree bit for bit.
+#pragma once
+#include "uniform.hpp"
+
+namespace shx {
+
+// ---------------- reductions ----------------
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+  for (int o = 32; o > 0; o >>= 1) v = tmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Max over the block, then one atomicMax on the slot (ordered bit patterns of non-negative T).
+template <typename T>
+__device__ __forceinline__ void block_max_to(T v, unsigned long long* slot) {
+  __shared__ T red[16];
+  v = wave_max(v);
+  int lane = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  int nthreads = blockDim.x * blockDim.y * blockDim.z;
+  if ((lane & 63) == 0) red[lane >> 6] = v;
+  __syncthreads();
+  if (lane == 0) {
+    T m = T(0);
+    for (int w = 0; w < (nthreads + 63) / 64; w++) m = tmax(m, red[w]);
+    atomicMax(slot, (unsigned long long)order_bits(m));
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T from_bits(unsigned long long b);
+template <> __device__ __forceinline__ float from_bits<float>(unsigned long long b) { return __uint_as_float((uint32_t)b); }
+template <> __device__ __forceinline__ double from_bits<double>(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// ---------------- plain path ----------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_rates(Params<T> P, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2,
+                                               int ncell, unsigned long long* slot) {
+  T m = T(0);
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < ncell; o += gridDim.x * blockDim.x) {
+    T4<T> u2 = U2[o];
+    Prim<T> w = prim_from_cons(P, U1[o], u2);
+    T r = rate_of(P, w);
+    if (u2.w <= T(0.5)) m = tmax(m, r);
+  }
+  block_max_to(m, slot);
+}
+
+template <typename T>
+__global__ void k_clock(Params<T> P, T* clk, unsigned long long* slot) {
+  T rate = from_bits<T>(*slot);
+  clock_pass(P, clk, rate);
+  *slot = 0ull;
+}
+
+template <typename T>
+__device__ __forceinline__ size_t cell_index(const Params<T>& P, int i, int j, int k) {
+  return ((size_t)k * P.n[1] + j) * P.n[0] + i;
+}
+
+// load_q (hydro/recon.wgsl:37-52): primitives of a possibly out-of-range cell
+template <typename T>
+__device__ __forceinline__ Prim<T> load_q(const Params<T>& P, const T4<T>* U1, const T4<T>* U2, int i, int j, int k) {
+  AxisMap mx = map_axis(i, P.n[0], P.bc_lo[0], P.bc_hi[0]);
+  AxisMap my = map_axis(j, P.n[1], P.bc_lo[1], P.bc_hi[1]);
+  AxisMap mz = map_axis(k, P.n[2], P.bc_lo[2], P.bc_hi[2]);
+  size_t o = cell_index(P, mx.i, my.i, mz.i);
+  Prim<T> q = prim_from_cons(P, U1[o], U2[o]);
+  q.vx = bc_vel(q.vx, mx.flip, mx.side);
+  q.vy = bc_vel(q.vy, my.flip, my.side);
+  q.vz = bc_vel(q.vz, mz.flip, mz.side);
+  return q;
+}
+
+// load_pred (hydro/update.wgsl:47-63): half-step state of a possibly out-of-range cell
+template <typename T>
+__device__ __forceinline__ Half<T> load_half(const Params<T>& P, const T4<T>* PA, const T4<T>* PB, int i, int j, int k) {
+  AxisMap mx = map_axis(i, P.n[0], P.bc_lo[0], P.bc_hi[0]);
+  AxisMap my = map_axis(j, P.n[1], P.bc_lo[1], P.bc_hi[1]);
+  AxisMap mz = map_axis(k, P.n[2], P.bc_lo[2], P.bc_hi[2]);
+  size_t o = cell_index(P, mx.i, my.i, mz.i);
+  T4<T> a = PA[o], b = PB[o];
+  Half<T> h;
+  h.rho = a.x;
+  h.vx = bc_vel(a.y, mx.flip, mx.side);
+  h.vy = bc_vel(a.z, my.flip, my.side);
+  h.vz = bc_vel(a.w, mz.flip, mz.side);
+  h.p = b.x; h.s = b.y; h.shock = b.z; h.fo = b.w;
+  return h;
+}
+
+template <typename T, int LIM, bool ISEN>
+__global__ __launch_bounds__(256) void k_predict(Params<T> P, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2,
+                                                 T4<T>* __restrict__ PA, T4<T>* __restrict__ PB, const T* clk) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z * blockDim.z + threadIdx.z;
+  if (i >= P.n[0] || j >= P.n[1] || k >= P.n[2]) return;
+  T dt = clk[0];
+  Prim<T> q = load_q(P, U1, U2, i, j, k);
+  Prim<T> xm = load_q(P, U1, U2, i - 1, j, k), xp = load_q(P, U1, U2, i + 1, j, k);
+  Prim<T> ym = load_q(P, U1, U2, i, j - 1, k), yp = load_q(P, U1, U2, i, j + 1, k);
+  Prim<T> zm = q, zp = q;
+  if (!P.flat) { zm = load_q(P, U1, U2, i, j, k - 1); zp = load_q(P, U1, U2, i, j, k + 1); }
+  AxisSlope<T> sl[3];
+  sl[0] = axis_slope<T, LIM, ISEN>(P, xm, q, xp, nullptr, nullptr);
+  sl[1] = axis_slope<T, LIM, ISEN>(P, ym, q, yp, nullptr, nullptr);
+  sl[2] = axis_slope<T, LIM, ISEN>(P, zm, q, zp, nullptr, nullptr);
+  T conv = (xp.vx - xm.vx) + (yp.vy - ym.vy) + (zp.vz - zm.vz);
+  T jump = tmax(jump_of(xm.p, xp.p), tmax(jump_of(ym.p, yp.p), jump_of(zm.p, zp.p)));
+  Half<T> h = hancock(P, q, sl, conv, jump, dt);
+  size_t o = cell_index(P, i, j, k);
+  T4<T> a, b;
+  a.x = h.rho; a.y = h.vx; a.z = h.vy; a.w = h.vz;
+  b.x = h.p; b.y = h.s; b.z = h.shock; b.w = h.fo;
+  PA[o] = a;
+  PB[o] = b;
+}
+
+// flux_between (hydro/flux.wgsl:254-347): the face between cell a and a + e_axis
+template <typename T, int LIM, int RS, bool ISEN>
+__device__ __forceinline__ Flux<T> flux_between(const Params<T>& P, const T4<T>* U1, const T4<T>* U2, const T4<T>* PA,
+                                                const T4<T>* PB, int i, int j, int k, int axis) {
+  int ex = axis == 0, ey = axis == 1, ez = axis == 2;
+  Prim<T> wll = load_q(P, U1, U2, i - ex, j - ey, k - ez);
+  Prim<T> wl = load_q(P, U1, U2, i, j, k);
+  Prim<T> wr = load_q(P, U1, U2, i + ex, j + ey, k + ez);
+  Prim<T> wrr = load_q(P, U1, U2, i + 2 * ex, j + 2 * ey, k + 2 * ez);
+  Half<T> hl = load_half(P, PA, PB, i, j, k);
+  Half<T> hr = load_half(P, PA, PB, i + ex, j + ey, k + ez);
+  Inc<T> il = edge<T, LIM, ISEN, true>(P, wll, wl, wr);
+  Inc<T> ir = edge<T, LIM, ISEN, false>(P, wl, wr, wrr);
+  return face_flux<T, RS>(P, hl, il, hr, ir, axis);
+}
+
+template <typename T>
+__device__ __forceinline__ void put_flux(T* tf, int nf, int f, const Flux<T>& F) {
+  tf[0 * nf + f] = F.m; tf[1 * nf + f] = F.mx; tf[2 * nf + f] = F.my; tf[3 * nf + f] = F.mz;
+  tf[4 * nf + f] = F.e; tf[5 * nf + f] = F.ei; tf[6 * nf + f] = F.s; tf[7 * nf + f] = F.vn;
+}
+template <typename T>
+__device__ __forceinline__ Flux<T> get_flux(const T* tf, int nf, int f) {
+  Flux<T> F;
+  F.m = tf[0 * nf + f]; F.mx = tf[1 * nf + f]; F.my = tf[2 * nf + f]; F.mz = tf[3 * nf + f];
+  F.e = tf[4 * nf + f]; F.ei = tf[5 * nf + f]; F.s = tf[6 * nf + f]; F.vn = tf[7 * nf + f];
+  return F;
+}
+template <typename T>
+__device__ __forceinline__ Flux<T> zero_flux() {
+  Flux<T> F;
+  F.m = F.mx = F.my = F.mz = F.e = F.ei = F.s = F.vn = T(0);
+  return F;
+}
+
+// The tiled update: block (WX, WY, WZ) cells; WZ = 1 with P.flat for 2D.
+template <typename T, int LIM, int RS, bool ISEN, int WX, int WY, int WZ, bool DEBUG>
+__global__ __launch_bounds__(WX* WY* WZ) void k_update(Params<T> P, const T4<T>* __restrict__ U1,
+                                                        const T4<T>* __restrict__ U2, const T4<T>* __restrict__ PA,
+                                                        const T4<T>* __restrict__ PB, T4<T>* __restrict__ O1,
+                                                        T4<T>* __restrict__ O2, const T* clk) {
+  constexpr int NFX = (WX + 1) * WY * WZ;
+  constexpr int NFY = WX * (WY + 1) * WZ;
+  constexpr int NFZ = WX * WY * (WZ + 1);
+  constexpr int NF = NFX + NFY + NFZ;
+  constexpr int NT = WX * WY * WZ;
+  __shared__ T tf[8 * NF];
+  const int x0 = blockIdx.x * WX, y0 = blockIdx.y * WY, z0 = blockIdx.z * WZ;
+  const int li = threadIdx.x + WX * (threadIdx.y + WY * threadIdx.z);
+  const int nfz = P.flat ? 0 : NFZ;
+  for (int f = li; f < NFX + NFY + nfz; f += NT) {
+    int axis, fx, fy, fz;
+    if (f < NFX) {
+      axis = 0; fx = f % (WX + 1); fy = (f / (WX + 1)) % WY; fz = f / ((WX + 1) * WY);
+    } else if (f < NFX + NFY) {
+      int g = f - NFX;
+      axis = 1; fx = g % WX; fy = (g / WX) % (WY + 1); fz = g / (WX * (WY + 1));
+    } else {
+      int g = f - NFX - NFY;
+      axis = 2; fx = g % WX; fy = (g / WX) % WY; fz = g / (WX * WY);
+    }
+    // the face between cell a and a + e (tile.wgsl:186-191)
+    int ai = x0 + fx - (axis == 0), aj = y0 + fy - (axis == 1), ak = z0 + fz - (axis == 2);
+    Flux<T> F = flux_between<T, LIM, RS, ISEN>(P, U1, U2, PA, PB, ai, aj, ak, axis);
+    put_flux(tf, NF, f, F);
+  }
+  __syncthreads();
+  int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
+  int i = x0 + tx, j = y0 + ty, k = z0 + tz;
+  if (i >= P.n[0] || j >= P.n[1] || k >= P.n[2]) return;
+  Flux<T> xl = get_flux(tf, NF, tx + (WX + 1) * (ty + WY * tz));
+  Flux<T> xh = get_flux(tf, NF, tx + 1 + (WX + 1) * (ty + WY * tz));
+  Flux<T> yl = get_flux(tf, NF, NFX + tx + WX * (ty + (WY + 1) * tz));
+  Flux<T> yh = get_flux(tf, NF, NFX + tx + WX * (ty + 1 + (WY + 1) * tz));
+  Flux<T> zl = zero_flux<T>(), zh = zero_flux<T>();
+  if (!P.flat) {
+    zl = get_flux(tf, NF, NFX + NFY + tx + WX * (ty + WY * tz));
+    zh = get_flux(tf, NF, NFX + NFY + tx + WX * (ty + WY * (tz + 1)));
+  }
+  size_t o = cell_index(P, i, j, k);
+  if constexpr (DEBUG) {
+    // DEBUG_FLUX (hydro/update.wgsl:313-317)
+    T4<T> a, b;
+    a.x = xl.m; a.y = xh.m; a.z = yl.m; a.w = yh.m;
+    b.x = zl.m; b.y = zh.m; b.z = T(0); b.w = T(0);
+    O1[o] = a;
+    O2[o] = b;
+    return;
+  }
+  Flux<T> net = net_of(xl, xh, yl, yh, zl, zh);
+  T dtdx = clk[0] / P.dx;
+  T4<T> o1, o2;
+  advance(P, U1[o], U2[o], net, PB[o].x, dtdx, o1, o2);
+  O1[o] = o1;
+  O2[o] = o2;
+}
+
+// ---------------- fused path ----------------
+// LDS layout of the fused kernel, structure of arrays over the block's NT = BX x BY columns:
+//   q[6]  primitives of the centre plane (rho, vx, vy, vz, p, s)
+//   h[8]  half step (rho, vx, vy, vz, p, s, shock, first-order)
+//   ix[6], iy[6]  twice the increments toward the upper x and y faces
+//   fx[8], fy[8]  the x and y face fluxes below each column
+// When the separate layout would exceed the CU's LDS, fx/fy alias h/ix (one more barrier).
+template <typename T, int NT>
+struct FusedLayout {
+  static constexpr bool ALIAS = (size_t)NT * sizeof(T) * 42 > 150 * 1024;
+  static constexpr int Q = 0, H = 6 * NT, IX = 14 * NT, IY = 20 * NT;
+  static constexpr int FX = ALIAS ? 6 * NT : 26 * NT;
+  static constexpr int FY = ALIAS ? 14 * NT : 34 * NT;
+  static constexpr int SIZE = ALIAS ? 26 * NT : 42 * NT;
+};
+
+template <typename T, int NT>
+__device__ __forceinline__ Prim<T> lds_prim(const T* L, int t) {
+  Prim<T> p;
+  p.rho = L[0 * NT + t]; p.vx = L[1 * NT + t]; p.vy = L[2 * NT + t]; p.vz = L[3 * NT + t];
+  p.p = L[4 * NT + t]; p.s = L[5 * NT + t];
+  p.cs = T(0);
+  return p;
+}
+template <typename T, int NT>
+__device__ __forceinline__ void lds_put_flux(T* L, int base, int t, const Flux<T>& F) {
+  L[base + 0 * NT + t] = F.m; L[base + 1 * NT + t] = F.mx; L[base + 2 * NT + t] = F.my; L[base + 3 * NT + t] = F.mz;
+  L[base + 4 * NT + t] = F.e; L[base + 5 * NT + t] = F.ei; L[base + 6 * NT + t] = F.s; L[base + 7 * NT + t] = F.vn;
+}
+template <typename T, int NT>
+__device__ __forceinline__ Flux<T> lds_get_flux(const T* L, int base, int t) {
+  Flux<T> F;
+  F.m = L[base + 0 * NT + t]; F.mx = L[base + 1 * NT + t]; F.my = L[base + 2 * NT + t]; F.mz = L[base + 3 * NT + t];
+  F.e = L[base + 4 * NT + t]; F.ei = L[base + 5 * NT + t]; F.s = L[base + 6 * NT + t]; F.vn = L[base + 7 * NT + t];
+  return F;
+}
+template <typename T, int NT>
+__device__ __forceinline__ Half<T> lds_get_half(const T* L, int n) {
+  Half<T> h;
+  constexpr int H = FusedLayout<T, NT>::H;
+  h.rho = L[H + 0 * NT + n]; h.vx = L[H + 1 * NT + n]; h.vy = L[H + 2 * NT + n]; h.vz = L[H + 3 * NT + n];
+  h.p = L[H + 4 * NT + n]; h.s = L[H + 5 * NT + n]; h.shock = L[H + 6 * NT + n]; h.fo = L[H + 7 * NT + n];
+  return h;
+}
+template <typename T, int NT>
+__device__ __forceinline__ Inc<T> lds_get_inc(const T* L, int base, int n) {
+  Inc<T> i;
+  i.rho = L[base + 0 * NT + n]; i.vx = L[base + 1 * NT + n]; i.vy = L[base + 2 * NT + n];
+  i.vz = L[base + 3 * NT + n]; i.p = L[base + 4 * NT + n]; i.s = L[base + 5 * NT + n];
+  return i;
+}
+
+// The fused step. Block BX x BY threads; thread (tx, ty) owns column (x0 + tx - 2, y0 + ty - 2).
+// Grid (ceil(nx / TX), ceil(ny / TY), ceil(nz / zc)). Planes [z0, z0 + zc) are updated.
+// Boundary conditions along x and y: periodic, outflow or reflecting; along z: periodic (the
+// host routes other z boundaries to the plain path).
+template <typename T, int LIM, int RS, bool ISEN, int BX, int BY, bool DEBUG>
+__global__ __launch_bounds__(BX* BY) void k_fused(Params<T> P, const T4<T>* __restrict__ U1,
+                                                   const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1,
+                                                   T4<T>* __restrict__ O2, const T* __restrict__ clk, int zc,
+                                                   unsigned long long* next_slot) {
+  constexpr int NT = BX * BY;
+  constexpr int TX = BX - 4, TY = BY - 4;
+  using LY = FusedLayout<T, NT>;
+  __shared__ T L[LY::SIZE];
+  __shared__ T red[NT / 64 + 1];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int t = tx + BX * ty;
+  const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
+  const int x = blockIdx.x * TX + tx - 2;
+  const int y = blockIdx.y * TY + ty - 2;
+  const int z0 = blockIdx.z * zc;
+  const int z1 = min(z0 + zc, nz);
+  const T dt = clk[0];
+  const T dtdx = dt / P.dx;
+
+  // this column's source through the x/y boundaries (clamped in range for the ragged rim,
+  // whose results are never used)
+  AxisMap mx = map_axis(x, nx, P.bc_lo[0], P.bc_hi[0]);
+  AxisMap my = map_axis(y, ny, P.bc_lo[1], P.bc_hi[1]);
+  mx.i = min(max(mx.i, 0), nx - 1);
+  my.i = min(max(my.i, 0), ny - 1);
+  const size_t col = (size_t)my.i * nx + mx.i;
+  const size_t plane = (size_t)nx * ny;
+
+  const bool in_r1 = tx >= 1 && tx <= BX - 2 && ty >= 1 && ty <= BY - 2;
+  const bool real_x = x >= 0 && x < nx, real_y = y >= 0 && y < ny;
+  const bool tile = tx >= 2 && tx <= BX - 3 && ty >= 2 && ty <= BY - 3 && real_x && real_y;
+  const bool xface = tx >= 2 && tx <= BX - 2 && ty >= 2 && ty <= BY - 3 && x <= nx && real_y;
+  const bool yface = ty >= 2 && ty <= BY - 2 && tx >= 2 && tx <= BX - 3 && y <= ny && real_x;
+  // A ghost column beyond a non-periodic face takes the half step of the cell it maps to
+  // (load_pred, hydro/update.wgsl:47-63): that cell's stencil sits one column inward.
+  int sx = tx, sy = ty;
+  if (!real_x && P.bc_lo[0] != BC_PERIODIC && x < 0) sx = tx + (mx.i - x);
+  if (!real_x && P.bc_hi[0] != BC_PERIODIC && x >= nx) sx = tx + (mx.i - x);
+  if (!real_y && P.bc_lo[1] != BC_PERIODIC && y < 0) sy = ty + (my.i - y);
+  if (!real_y && P.bc_hi[1] != BC_PERIODIC && y >= ny) sy = ty + (my.i - y);
+  const bool mapped = (sx != tx) || (sy != ty);
+  const int ts = min(max(sx, 1), BX - 2) + BX * min(max(sy, 1), BY - 2);
+
+  auto load_plane = [&](int z, T4<T>& u1, T4<T>& u2) {
+    int zz = ((z % nz) + nz) % nz;
+    size_t o = (size_t)zz * plane + col;
+    u1 = U1[o];
+    u2 = U2[o];
+  };
+  auto prim_of = [&](T4<T> u1, T4<T> u2) {
+    Prim<T> q = prim_from_cons(P, u1, u2);
+    q.vx = bc_vel(q.vx, mx.flip, mx.side);
+    q.vy = bc_vel(q.vy, my.flip, my.side);
+    return q;
+  };
+
+  T4<T> a1, a2, b1, b2;
+  load_plane(z0 - 2, a1, a2);
+  Prim<T> pm = prim_of(a1, a2);
+  load_plane(z0 - 1, a1, a2);
+  Prim<T> pc = prim_of(a1, a2);
+  T4<T> uc1 = a1, uc2 = a2;           // the state of plane c (for the update one plane later)
+  load_plane(z0, b1, b2);              // plane c + 1, converted at the top of the loop
+
+  // carried from the previous plane
+  Half<T> h_prev;                      // half step of plane c - 1
+  Inc<T> iz_prev;                      // its twice increment toward the upper z face
+  Flux<T> fz_lo = zero_flux<T>();      // the z face below plane c - 1
+  Flux<T> axy_prev = zero_flux<T>();   // its (x_lo - x_hi) + (y_lo - y_hi)
+  T4<T> uprev1, uprev2;                // its state
+  T rate_max = T(0);
+  uprev1 = uc1; uprev2 = uc2;
+
+  for (int c = z0 - 1; c <= z1; c++) {
+    Prim<T> pp = prim_of(b1, b2);
+    T4<T> up1 = b1, up2 = b2;
+    if (c + 2 <= z1 + 1) load_plane(c + 2, b1, b2);   // prefetch plane c + 2
+
+    // 1. the centre plane's primitives into LDS
+    L[0 * NT + t] = pc.rho; L[1 * NT + t] = pc.vx; L[2 * NT + t] = pc.vy; L[3 * NT + t] = pc.vz;
+    L[4 * NT + t] = pc.p; L[5 * NT + t] = pc.s;
+    __syncthreads();
+
+    // 2. half step and increments of plane c (hydro/predictor.wgsl:46-209)
+    Half<T> hc;
+    Inc<T> ixu, ixd, iyu, iyd, izu, izd;
+    if (in_r1) {
+      Prim<T> xm = lds_prim<T, NT>(L, t - 1), xp = lds_prim<T, NT>(L, t + 1);
+      Prim<T> ym = lds_prim<T, NT>(L, t - BX), yp = lds_prim<T, NT>(L, t + BX);
+      AxisSlope<T> sl[3];
+      sl[0] = axis_slope<T, LIM, ISEN>(P, xm, pc, xp, &ixu, &ixd);
+      sl[1] = axis_slope<T, LIM, ISEN>(P, ym, pc, yp, &iyu, &iyd);
+      sl[2] = axis_slope<T, LIM, ISEN>(P, pm, pc, pp, &izu, &izd);
+      if (!mapped) {
+        T conv = (xp.vx - xm.vx) + (yp.vy - ym.vy) + (pp.vz - pm.vz);
+        T jump = tmax(jump_of(xm.p, xp.p), tmax(jump_of(ym.p, yp.p), jump_of(pm.p, pp.p)));
+        hc = hancock(P, pc, sl, conv, jump, dt);
+      } else {
+        // the mapped cell's own stencil: its centre and z neighbours are this column's
+        // primitives seen unflipped; its x/y neighbours are in LDS
+        Prim<T> qc = pc, qm = pm, qp = pp;
+        qc.vx = bc_vel(qc.vx, mx.flip, 0); qc.vy = bc_vel(qc.vy, my.flip, 0);
+        qm.vx = bc_vel(qm.vx, mx.flip, 0); qm.vy = bc_vel(qm.vy, my.flip, 0);
+        qp.vx = bc_vel(qp.vx, mx.flip, 0); qp.vy = bc_vel(qp.vy, my.flip, 0);
+        Prim<T> sxm = lds_prim<T, NT>(L, ts - 1), sxp = lds_prim<T, NT>(L, ts + 1);
+        Prim<T> sym = lds_prim<T, NT>(L, ts - BX), syp = lds_prim<T, NT>(L, ts + BX);
+        AxisSlope<T> ms[3];
+        ms[0] = axis_slope<T, LIM, ISEN>(P, sxm, qc, sxp, nullptr, nullptr);
+        ms[1] = axis_slope<T, LIM, ISEN>(P, sym, qc, syp, nullptr, nullptr);
+        ms[2] = axis_slope<T, LIM, ISEN>(P, qm, qc, qp, nullptr, nullptr);
+        T conv = (sxp.vx - sxm.vx) + (syp.vy - sym.vy) + (qp.vz - qm.vz);
+        T jump = tmax(jump_of(sxm.p, sxp.p), tmax(jump_of(sym.p, syp.p), jump_of(qm.p, qp.p)));
+        hc = hancock(P, qc, ms, conv, jump, dt);
+        hc.vx = bc_vel(hc.vx, mx.flip, mx.side);
+        hc.vy = bc_vel(hc.vy, my.flip, my.side);
+      }
+      // 3. what the neighbours above need
+      constexpr int H = LY::H, IX = LY::IX, IY = LY::IY;
+      L[H + 0 * NT + t] = hc.rho; L[H + 1 * NT + t] = hc.vx; L[H + 2 * NT + t] = hc.vy; L[H + 3 * NT + t] = hc.vz;
+      L[H + 4 * NT + t] = hc.p; L[H + 5 * NT + t] = hc.s; L[H + 6 * NT + t] = hc.shock; L[H + 7 * NT + t] = hc.fo;
+      L[IX + 0 * NT + t] = ixu.rho; L[IX + 1 * NT + t] = ixu.vx; L[IX + 2 * NT + t] = ixu.vy;
+      L[IX + 3 * NT + t] = ixu.vz; L[IX + 4 * NT + t] = ixu.p; L[IX + 5 * NT + t] = ixu.s;
+      L[IY + 0 * NT + t] = iyu.rho; L[IY + 1 * NT + t] = iyu.vx; L[IY + 2 * NT + t] = iyu.vy;
+      L[IY + 3 * NT + t] = iyu.vz; L[IY + 4 * NT + t] = iyu.p; L[IY + 5 * NT + t] = iyu.s;
+    }
+    __syncthreads();
+
+    // 4. x and y faces of plane c: the face below this column, from the neighbour's upper side
+    const bool live = c >= z0 && c < z1;
+    Flux<T> Fx = zero_flux<T>(), Fy = zero_flux<T>();
+    if (live && xface)
+      Fx = face_flux<T, RS>(P, lds_get_half<T, NT>(L, t - 1), lds_get_inc<T, NT>(L, LY::IX, t - 1), hc, ixd, 0);
+    if (live && yface)
+      Fy = face_flux<T, RS>(P, lds_get_half<T, NT>(L, t - BX), lds_get_inc<T, NT>(L, LY::IY, t - BX), hc, iyd, 1);
+    if constexpr (LY::ALIAS) __syncthreads();
+    if (live && xface) lds_put_flux<T, NT>(L, LY::FX, t, Fx);
+    if (live && yface) lds_put_flux<T, NT>(L, LY::FY, t, Fy);
+    // z face between planes c - 1 and c
+    Flux<T> fz = zero_flux<T>();
+    if (tile && c >= z0) fz = face_flux<T, RS>(P, h_prev, iz_prev, hc, izd, 2);
+    __syncthreads();
+
+    // 5. this plane's x/y net inflow, and the update of plane c - 1
+    Flux<T> axy = zero_flux<T>();
+    if (live && tile) {
+      Flux<T> xl = lds_get_flux<T, NT>(L, LY::FX, t), xh = lds_get_flux<T, NT>(L, LY::FX, t + 1);
+      Flux<T> yl = lds_get_flux<T, NT>(L, LY::FY, t), yh = lds_get_flux<T, NT>(L, LY::FY, t + BX);
+      if constexpr (DEBUG) {
+        axy.m = xl.m; axy.mx = xh.m; axy.my = yl.m; axy.mz = yh.m;
+      } else {
+        axy.m = (xl.m - xh.m) + (yl.m - yh.m);
+        axy.mx = (xl.mx - xh.mx) + (yl.mx - yh.mx);
+        axy.my = (xl.my - xh.my) + (yl.my - yh.my);
+        axy.mz = (xl.mz - xh.mz) + (yl.mz - yh.mz);
+        axy.e = (xl.e - xh.e) + (yl.e - yh.e);
+        axy.ei = (xl.ei - xh.ei) + (yl.ei - yh.ei);
+        axy.s = (xl.s - xh.s) + (yl.s - yh.s);
+        axy.vn = (xl.vn - xh.vn) + (yl.vn - yh.vn);
+      }
+    }
+    if (tile && c - 1 >= z0 && c - 1 < z1) {
+      size_t o = (size_t)(c - 1) * plane + col;
+      T4<T> o1, o2;
+      if constexpr (DEBUG) {
+        o1.x = axy_prev.m; o1.y = axy_prev.mx; o1.z = axy_prev.my; o1.w = axy_prev.mz;
+        o2.x = fz_lo.m; o2.y = fz.m; o2.z = T(0); o2.w = T(0);
+      } else {
+        Flux<T> net;
+        net.m = axy_prev.m + (fz_lo.m - fz.m);
+        net.mx = axy_prev.mx + (fz_lo.mx - fz.mx);
+        net.my = axy_prev.my + (fz_lo.my - fz.my);
+        net.mz = axy_prev.mz + (fz_lo.mz - fz.mz);
+        net.e = axy_prev.e + (fz_lo.e - fz.e);
+        net.ei = axy_prev.ei + (fz_lo.ei - fz.ei);
+        net.s = axy_prev.s + (fz_lo.s - fz.s);
+        net.vn = axy_prev.vn + (fz_lo.vn - fz.vn);
+        advance(P, uprev1, uprev2, net, h_prev.p, dtdx, o1, o2);
+        // the next step's signal rate (hydro/rates.wgsl:60-62), from the state just made
+        Prim<T> w = prim_from_cons(P, o1, o2);
+        rate_max = tmax(rate_max, rate_of(P, w));
+      }
+      O1[o] = o1;
+      O2[o] = o2;
+    }
+    // rotate
+    h_prev = hc;
+    iz_prev = izu;
+    fz_lo = fz;
+    axy_prev = axy;
+    uprev1 = uc1; uprev2 = uc2;
+    uc1 = up1; uc2 = up2;
+    pm = pc;
+    pc = pp;
+  }
+  if constexpr (!DEBUG) {
+    // block maximum of the next step's rate (order-independent: max is exact)
+    T v = wave_max(rate_max);
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) {
+      T m = T(0);
+      for (int w = 0; w < (NT + 63) / 64; w++) m = tmax(m, red[w]);
+      atomicMax(next_slot, (unsigned long long)order_bits(m));
+    }
+  }
+}
+
+// ---------------- diagnostics (FP64 sums) ----------------
+// Per block: mass, momentum (3), kinetic, internal (E - KE), total E, max Mach, floored cells,
+// rho s, rho min/max, p min/max. Sums in double; a second pass adds the blocks in order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag(Params<T> P, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2,
+                                              int ncell, double* part) {
+  __shared__ double sh[16][256];
+  double v[16];
+  for (int c = 0; c < 16; c++) v[c] = 0.0;
+  v[10] = 1e300; v[11] = -1e300; v[12] = 1e300; v[13] = -1e300;
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < ncell; o += gridDim.x * blockDim.x) {
+    T4<T> u1 = U1[o], u2 = U2[o];
+    double rho = (double)u1.x, mx = (double)u1.y, my = (double)u1.z, mz = (double)u1.w, E = (double)u2.x;
+    double ke = 0.5 * (mx * mx + my * my + mz * mz) / rho;
+    Prim<T> w = prim_from_cons(P, u1, u2);
+    double vv = sqrt((double)w.vx * w.vx + (double)w.vy * w.vy + (double)w.vz * w.vz);
+    v[0] += rho; v[1] += mx; v[2] += my; v[3] += mz; v[4] += ke; v[5] += E - ke; v[6] += E;
+    v[7] = fmax(v[7], vv / (double)w.cs);
+    v[8] += (P.gm1 * u2.y <= P.p_floor) ? 1.0 : 0.0;
+    v[9] += (double)u2.z;
+    v[10] = fmin(v[10], (double)w.rho); v[11] = fmax(v[11], (double)w.rho);
+    v[12] = fmin(v[12], (double)w.p); v[13] = fmax(v[13], (double)w.p);
+  }
+  for (int c = 0; c < 16; c++) sh[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      for (int c = 0; c < 16; c++) {
+        double a = sh[c][threadIdx.x], b = sh[c][threadIdx.x + s];
+        sh[c][threadIdx.x] = (c == 7 || c == 11 || c == 13) ? fmax(a, b) : (c == 10 || c == 12) ? fmin(a, b) : a + b;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 16) part[blockIdx.x * 16 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+template <typename T>
+__global__ void k_diag_final(const double* part, int nblocks, double* out) {
+  int c = threadIdx.x;
+  if (c >= 16) return;
+  double a = (c == 10 || c == 12) ? 1e300 : (c == 11 || c == 13) ? -1e300 : 0.0;
+  for (int b = 0; b < nblocks; b++) {
+    double x = part[b * 16 + c];
+    a = (c == 7 || c == 11 || c == 13) ? fmax(a, x) : (c == 10 || c == 12) ? fmin(a, x) : a + x;
+  }
+  out[c] = a;
+}
+
+// ---------------- launchers ----------------
+template <typename T, int LIM, int RS, bool ISEN>
+struct UniformImpl {
+  static void rates(const Launch& L, const void* u1, const void* u2, unsigned long long* slot) {
+    const Params<T>& P = *static_cast<const Params<T>*>(L.params);
+    int blocks = (int)std::min<size_t>((L.ncell + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_rates<T>, dim3(blocks), dim3(256), 0, L.stream, P, (const T4<T>*)u1, (const T4<T>*)u2,
+                       (int)L.ncell, slot);
+  }
+  static void clock(const Launch& L, void* clk, unsigned long long* slot) {
+    const Params<T>& P = *static_cast<const Params<T>*>(L.params);
+    hipLaunchKernelGGL(k_clock<T>, dim3(1), dim3(1), 0, L.stream, P, (T*)clk, slot);
+  }
+  static void predict(const Launch& L, const void* u1, const void* u2, void* pa, void* pb, const void* clk) {
+    const Params<T>& P = *static_cast<const Params<T>*>(L.params);
+    dim3 b = P.flat ? dim3(16, 16, 1) : dim3(8, 8, 4);
+    dim3 g((P.n[0] + b.x - 1) / b.x, (P.n[1] + b.y - 1) / b.y, (P.n[2] + b.z - 1) / b.z);
+    hipLaunchKernelGGL((k_predict<T, LIM, ISEN>), g, b, 0, L.stream, P, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)pa,
+                       (T4<T>*)pb, (const T*)clk);
+  }
+  static void update(const Launch& L, const void* u1, const void* u2, const void* pa, const void* pb, void* o1,
+                     void* o2, const void* clk, bool debug) {
+    const Params<T>& P = *static_cast<const Params<T>*>(L.params);
+    if (P.flat) {
+      dim3 b(16, 16, 1), g((P.n[0] + 15) / 16, (P.n[1] + 15) / 16, P.n[2]);
+      if (debug)
+        hipLaunchKernelGGL((k_update<T, LIM, RS, ISEN, 16, 16, 1, true>), g, b, 0, L.stream, P, (const T4<T>*)u1,
+                           (const T4<T>*)u2, (const T4<T>*)pa, (const T4<T>*)pb, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk);
+      else
+        hipLaunchKernelGGL((k_update<T, LIM, RS, ISEN, 16, 16, 1, false>), g, b, 0, L.stream, P, (const T4<T>*)u1,
+                           (const T4<T>*)u2, (const T4<T>*)pa, (const T4<T>*)pb, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk);
+    } else {
+      dim3 b(8, 4, 4), g((P.n[0] + 7) / 8, (P.n[1] + 3) / 4, (P.n[2] + 3) / 4);
+      if (debug)
+        hipLaunchKernelGGL((k_update<T, LIM, RS, ISEN, 8, 4, 4, true>), g, b, 0, L.stream, P, (const T4<T>*)u1,
+                           (const T4<T>*)u2, (const T4<T>*)pa, (const T4<T>*)pb, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk);
+      else
+        hipLaunchKernelGGL((k_update<T, LIM, RS, ISEN, 8, 4, 4, false>), g, b, 0, L.stream, P, (const T4<T>*)u1,
+                           (const T4<T>*)u2, (const T4<T>*)pa, (const T4<T>*)pb, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk);
+    }
+  }
+  template <int BX, int BY>
+  static void fused_tile(const Launch& L, const void* u1, const void* u2, void* o1, void* o2, const void* clk, int zc,
+                         unsigned long long* next_slot, bool debug) {
+    const Params<T>& P = *static_cast<const Params<T>*>(L.params);
+    constexpr int TX = BX - 4, TY = BY - 4;
+    dim3 g((P.n[0] + TX - 1) / TX, (P.n[1] + TY - 1) / TY, (P.n[2] + zc - 1) / zc);
+    if (debug)
+      hipLaunchKernelGGL((k_fused<T, LIM, RS, ISEN, BX, BY, true>), g, dim3(BX, BY), 0, L.stream, P, (const T4<T>*)u1,
+                         (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, zc, next_slot);
+    else
+      hipLaunchKernelGGL((k_fused<T, LIM, RS, ISEN, BX, BY, false>), g, dim3(BX, BY), 0, L.stream, P, (const T4<T>*)u1,
+                         (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, zc, next_slot);
+  }
+  static void fused(const Launch& L, int tile, const void* u1, const void* u2, void* o1, void* o2, const void* clk,
+                    int zc, unsigned long long* next_slot, bool debug) {
+    if constexpr (sizeof(T) == 8) {
+      fused_tile<20, 20>(L, u1, u2, o1, o2, clk, zc, next_slot, debug);
+    } else {
+      switch (tile) {
+        case FUSED_32x32: fused_tile<32, 32>(L, u1, u2, o1, o2, clk, zc, next_slot, debug); break;
+        case FUSED_32x16: fused_tile<32, 16>(L, u1, u2, o1, o2, clk, zc, next_slot, debug); break;
+        default: fused_tile<20, 20>(L, u1, u2, o1, o2, clk, zc, next_slot, debug); break;
+      }
+    }
+  }
+  static void diag(const Launch& L, const void* u1, const void* u2, double* part, int nblocks, double* out) {
+    const Params<T>& P = *static_cast<const Params<T>*>(L.params);
+    hipLaunchKernelGGL(k_diag<T>, dim3(nblocks), dim3(256), 0, L.stream, P, (const T4<T>*)u1, (const T4<T>*)u2,
+                       (int)L.ncell, part);
+    hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(64), 0, L.stream, (const double*)part, nblocks, out);
+  }
+  static UniformOps ops() {
+    UniformOps o;
+    o.rates = rates;
+    o.clock = clock;
+    o.predict = predict;
+    o.update = update;
+    o.fused = fused;
+    o.diag = diag;
+    return o;
+  }
+};
+
+}  // namespace shx
