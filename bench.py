@@ -71,6 +71,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--paths", default="fused,plain", help="which uniform-grid paths to time")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run a device copy of MIB MiB three times (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only: the hydro path is built and measured "
@@ -88,35 +91,55 @@ def main():
     log(f"[bench] initial conditions {n}^3 in {time.perf_counter() - t0:.1f} s")
     ncell = pb.ncell
 
-    # fused one-pass path
-    sim = Sim(config_for(pb, path="fused"))
-    sim.set_state(u1, u2)
+    paths = args.paths.split(",")
+    if args.copy_probe:
+        import torch
+        nb = args.copy_probe << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: 3 x {args.copy_probe} MiB device copies")
+
+    value = plain_value = None
+    kernel_ms = achieved = None
     tile = None
-    if not args.no_tune:
-        tile = sim.autotune(reps=3)
-        log(f"[bench] autotuned fused tile {tile}")
-    sim.step(args.warmup)
-    sim.sync()
-    log("[bench] fused warmup done")
-    t = time_steps(sim, args.steps)
-    value = ncell * args.steps / t
-    prof = sim.profile(5)
-    kernel_ms = prof["update"]
-    achieved = BYTES_PER_UPDATE * ncell / (kernel_ms * 1e-3)
-    log(f"[bench] fused: {t / args.steps * 1e3:.3f} ms/step, kernel {kernel_ms:.3f} ms, {value:.3e} upd/s")
-    diag = sim.diagnostics()
-    sim.destroy()
+    diag = None
+    t = tp = None
+    pprof = None
+    # fused one-pass path
+    if "fused" in paths:
+        sim = Sim(config_for(pb, path="fused"))
+        sim.set_state(u1, u2)
+        if not args.no_tune:
+            tile = sim.autotune(reps=3)
+            log(f"[bench] autotuned fused tile {tile}")
+        sim.step(args.warmup)
+        sim.sync()
+        log("[bench] fused warmup done")
+        t = time_steps(sim, args.steps)
+        value = ncell * args.steps / t
+        prof = sim.profile(5)
+        kernel_ms = prof["update"]
+        achieved = BYTES_PER_UPDATE * ncell / (kernel_ms * 1e-3)
+        log(f"[bench] fused: {t / args.steps * 1e3:.3f} ms/step, kernel {kernel_ms:.3f} ms, {value:.3e} upd/s")
+        diag = sim.diagnostics()
+        sim.destroy()
 
     # plain reference-structured path (rates, clock, predict, tiled update)
-    sim = Sim(config_for(pb, path="plain"))
-    sim.set_state(u1, u2)
-    sim.step(args.warmup)
-    sim.sync()
-    tp = time_steps(sim, args.steps)
-    plain_value = ncell * args.steps / tp
-    pprof = sim.profile(5)
-    sim.destroy()
-    log(f"[bench] plain: {tp / args.steps * 1e3:.3f} ms/step, {plain_value:.3e} upd/s, passes {pprof}")
+    if "plain" in paths:
+        sim = Sim(config_for(pb, path="plain"))
+        sim.set_state(u1, u2)
+        sim.step(args.warmup)
+        sim.sync()
+        tp = time_steps(sim, args.steps)
+        plain_value = ncell * args.steps / tp
+        pprof = sim.profile(5)
+        sim.destroy()
+        log(f"[bench] plain: {tp / args.steps * 1e3:.3f} ms/step, {plain_value:.3e} upd/s, passes {pprof}")
+    if value is None:
+        return
 
     cpu = None
     if not args.no_cpu:
@@ -143,8 +166,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
                      "kernel": "k_fused", "kernel_ms": kernel_ms, "bytes_per_update": BYTES_PER_UPDATE},
-        "plain": {"value": plain_value, "ms_per_step": tp / args.steps * 1e3, "passes_ms": pprof},
-        "fused_over_plain": value / plain_value,
+        "plain": {"value": plain_value, "ms_per_step": tp / args.steps * 1e3 if tp else None, "passes_ms": pprof},
+        "fused_over_plain": value / plain_value if plain_value else None,
         "conserved": {"mass": diag.mass, "e_tot": diag.e_tot},
         "cpu_baseline": cpu,
     }

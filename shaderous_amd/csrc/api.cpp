@@ -51,7 +51,7 @@ struct shx_sim {
   int diag_blocks = 256;
   int cur = 0;
   bool fused = false;
-  int tile = FUSED_32x32;
+  int tile = FUSED_32x16;
   int zc = 32;
   Params<float> pf;
   Params<double> pd;
@@ -69,6 +69,7 @@ Params<T> make_params(const shx_config& c) {
   P.ndim = T(c.ndim);
   P.gamma = T(c.gamma);
   P.gm1 = P.gamma - T(1);
+  P.igm1 = T(1) / P.gm1;
   P.dx = T(c.dx);
   P.cfl = T(c.cfl);
   P.rho_floor = T(c.rho_floor);
@@ -248,7 +249,7 @@ int shx_create(const shx_config* cfg, shx_sim** out) {
   s->pf = make_params<float>(c);
   s->pd = make_params<double>(c);
   s->fused = c.path == SHX_PATH_FUSED && fused_supported(c);
-  s->tile = s->es == 8 ? FUSED_20x20 : FUSED_32x32;
+  s->tile = s->es == 8 ? FUSED_20x20 : FUSED_32x16;
   s->zc = 32;
   if (c.tile[0] > 0) {
     int t = c.tile[0];

@@ -30,17 +30,28 @@ struct Params {
   int bc_hi[3];
   int flat;            // one layer along z (2D): z neighbours are the cell itself
   T ndim;              // P.geom.y
-  T gamma, gm1;        // P.eos.x, P.eos.x - 1
+  T gamma, gm1, igm1;  // P.eos.x, P.eos.x - 1, 1 / (P.eos.x - 1)
   T dx, cfl, rho_floor, p_floor, eta_dual, shock_thresh, t_end, dt_max;
   T fluff[3];
 };
 
-template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
-template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
-template <typename T> __device__ __forceinline__ T tabs(T a) { return a < T(0) ? -a : a; }
+// min/max/abs as single instructions (v_min_f32, v_max_f32, the |x| source modifier); a NaN
+// operand yields the other one.
+__device__ __forceinline__ float tmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double tmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float tmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double tmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ float tabs(float a) { return __builtin_fabsf(a); }
+__device__ __forceinline__ double tabs(double a) { return __builtin_fabs(a); }
 template <typename T> __device__ __forceinline__ T tsign(T a) { return a > T(0) ? T(1) : (a < T(0) ? T(-1) : T(0)); }
 template <typename T> __device__ __forceinline__ T tclamp(T x, T lo, T hi) { return tmin(tmax(x, lo), hi); }
-__device__ __forceinline__ float tsqrt(float x) { return __builtin_sqrtf(x); }
+// Single precision: the hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32, ~1 ulp;
+// WGSL allows 2.5 ulp for division). Double precision: correctly rounded.
+__device__ __forceinline__ float trcp(float b) { return __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double trcp(double b) { return 1.0 / b; }
+__device__ __forceinline__ float tdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double tdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ float tpow(float a, float b) { return powf(a, b); }
 __device__ __forceinline__ double tpow(double a, double b) { return pow(a, b); }
@@ -66,7 +77,7 @@ __device__ __forceinline__ Prim<T> prim_from_cons(const Params<T>& P, T4<T> u1, 
   T rho = tmax(u1.x, P.rho_floor);
   T p = P.gm1 * u2.y;
   if (!(p >= P.p_floor)) p = P.p_floor;
-  T ir = T(1) / rho;
+  T ir = trcp(rho);
   q.rho = rho;
   q.vx = u1.y * ir;
   q.vy = u1.z * ir;
@@ -120,7 +131,7 @@ template <typename T, int LIM>
 __device__ __forceinline__ T edge1(T far, T near, T scale) {
   if constexpr (LIM == LIM_KOREN) {
     const T SMOOTH = T(0.01);
-    T fr = (far + T(2) * near) / T(3);
+    T fr = (far + T(2) * near) * T(1.0 / 3.0);
     T s = SMOOTH * scale;
     T s2 = s * s;
     T ss = tmax(s2, T(1e-30));
@@ -128,9 +139,9 @@ __device__ __forceinline__ T edge1(T far, T near, T scale) {
     bool as_is = (s2 >= T(1e-30) && s2 < T(1e30)) || (s == T(0));
     T eta;
     if (as_is) {
-      eta = q / ss;
+      eta = tdiv(q, ss);
     } else {
-      T k = T(1) / tmax(tabs(s), T(1e-37));
+      T k = trcp(tmax(tabs(s), T(1e-37)));
       T fk = far * k, nk = near * k;
       eta = fk * fk + nk * nk;
     }
@@ -145,7 +156,7 @@ __device__ __forceinline__ T edge1(T far, T near, T scale) {
     T m = tmin(tabs(far), tabs(near));
     return same_sign(far, near) ? (far > T(0) ? m : -m) : T(0);
   } else {
-    return same_sign(far, near) ? T(2) * far * (near / (far + near)) : T(0);
+    return same_sign(far, near) ? T(2) * far * tdiv(near, far + near) : T(0);
   }
 }
 template <int LIM> constexpr bool lim_symmetric() { return LIM != LIM_KOREN; }
@@ -259,7 +270,7 @@ __device__ __forceinline__ Half<T> hancock(const Params<T>& P, const Prim<T>& q,
   T da3 = q.vx * sx.vz + q.vy * sy.vz + q.vz * sz.vz;
   T db0 = q.vx * sx.p + q.vy * sy.p + q.vz * sz.p;
   T db1 = q.vx * sx.s + q.vy * sy.s + q.vz * sz.s;
-  T ir = T(1) / rho;
+  T ir = trcp(rho);
   o.rho = rho - h * (da0 + rho * dv);
   o.vx = q.vx - h * (da1 + sx.p * ir);
   o.vy = q.vy - h * (da2 + sy.p * ir);
@@ -281,7 +292,7 @@ __device__ __forceinline__ Half<T> hancock(const Params<T>& P, const Prim<T>& q,
 // jump across the cell.
 template <typename T>
 __device__ __forceinline__ T jump_of(T pm, T pp) {
-  return tabs(pp - pm) / tmin(pp, pm);
+  return tdiv(tabs(pp - pm), tmin(pp, pm));
 }
 
 // ---- face: hydro/flux.wgsl:60-125, riemann/hllc.wgsl, riemann/hll.wgsl ----
@@ -300,8 +311,8 @@ __device__ __forceinline__ Side<T> make_side(const Params<T>& P, T rho, T vn, T 
   o.vn = vn; o.vt1 = vt1; o.vt2 = vt2;
   o.s = s;
   o.p = tmax(p, tmax(lo_p, P.p_floor));
-  o.cs = tsqrt(P.gamma * o.p / o.rho);
-  o.eint = o.p / P.gm1;
+  o.cs = tsqrt(tdiv(P.gamma * o.p, o.rho));
+  o.eint = o.p * P.igm1;
   return o;
 }
 
@@ -326,7 +337,7 @@ __device__ __forceinline__ Flux<T> riemann(const Side<T>& l, const Side<T>& r, T
   } else {
     T ql = l.rho * (sl - l.vn);
     T qr = r.rho * (sr - r.vn);
-    T inv = T(1) / (sr - sl);
+    T inv = trcp(sr - sl);
     T slsr = sl * sr;
     T hm = (sr * ml0 - sl * mr0 + slsr * (r.rho - l.rho)) * inv;
     T hp0 = (sr * fpl0 - sl * fpr0 + slsr * (mr0 - ml0)) * inv;
@@ -335,13 +346,13 @@ __device__ __forceinline__ Flux<T> riemann(const Side<T>& l, const Side<T>& r, T
     T he = (sr * fel - sl * fer + slsr * (er - el)) * inv;
     if constexpr (RS == RS_HLL) {
       o.m = hm; o.mx = hp0; o.my = hp1; o.mz = hp2; o.e = he;
-      o.vn = (r.p - l.p + l.vn * ql - r.vn * qr) / (ql - qr);
+      o.vn = tdiv(r.p - l.p + l.vn * ql - r.vn * qr, ql - qr);
     } else {
       if (!(ql < T(0) && qr > T(0))) {
         o.m = hm; o.mx = hp0; o.my = hp1; o.mz = hp2; o.e = he;
         o.vn = T(0.5) * (l.vn + r.vn);
       } else {
-        T sm = (r.p - l.p + l.vn * ql - r.vn * qr) / (ql - qr);
+        T sm = tdiv(r.p - l.p + l.vn * ql - r.vn * qr, ql - qr);
         // the star state as differences from the outer state (hllc.wgsl:52-74)
         bool left = sm >= T(0);
         T S = left ? sl : sr;
@@ -357,14 +368,14 @@ __device__ __forceinline__ Flux<T> riemann(const Side<T>& l, const Side<T>& r, T
         T f1 = left ? fpl1 : fpr1;
         T f2 = left ? fpl2 : fpr2;
         T fe = left ? fel : fer;
-        T k = (sm - un) / (S - sm);
+        T k = tdiv(sm - un, S - sm);
         T dr = rho * k;
         T f = rho + dr;
         T cm = fm + S * dr;
         T c0 = f0 + S * (sm * dr + rho * (sm - un));
         T c1 = f1 + S * (dr * u1);
         T c2 = f2 + S * (dr * u2);
-        T ce = fe + S * (E * k + f * (sm - un) * (sm + p / q));
+        T ce = fe + S * (E * k + f * (sm - un) * (sm + tdiv(p, q)));
         o.m = tmix(cm, hm, w_hll);
         o.mx = tmix(c0, hp0, w_hll);
         o.my = tmix(c1, hp1, w_hll);
@@ -376,37 +387,66 @@ __device__ __forceinline__ Flux<T> riemann(const Side<T>& l, const Side<T>& r, T
   }
   // internal energy and the scalar ride on the mass flux, upwinded
   bool up = o.m >= T(0);
-  o.ei = o.m * (up ? l.eint / l.rho : r.eint / r.rho);
+  o.ei = o.m * (up ? tdiv(l.eint, l.rho) : tdiv(r.eint, r.rho));
   o.s = o.m * (up ? l.s : r.s);
   return o;
 }
 
-// face_flux (flux.wgsl:92-125) between the half-step states hl (left) and hr (right) along
-// `axis`, given twice the increments from each cell toward the face (il: the left cell's up
-// increment, ir: the right cell's down increment). Momentum comes back in grid components.
+// One cell's side of a face before the face's own lower bound and floors: the half-step state
+// plus half the edge increment toward the face (to_side, flux.wgsl:60-63; zero increment for a
+// first-order cell), with the cell's half-step density and pressure (for the face's lower bound,
+// flux.wgsl:109-117) and its shock flag (flux.wgsl:93).
+template <typename T>
+struct SideData { T rho, vx, vy, vz, p, s, hrho, hp, shock; };
+
+// sgn = +1 for the cell's upper face (il its up increment), -1 for its lower face.
+template <typename T>
+__device__ __forceinline__ SideData<T> side_of(const Half<T>& h, const Inc<T>& inc, T sgn) {
+  T hs = T(0.5) * sgn * (T(1) - h.fo);
+  SideData<T> o;
+  o.rho = h.rho + hs * inc.rho;
+  o.vx = h.vx + hs * inc.vx;
+  o.vy = h.vy + hs * inc.vy;
+  o.vz = h.vz + hs * inc.vz;
+  o.p = h.p + hs * inc.p;
+  o.s = h.s + hs * inc.s;
+  o.hrho = h.rho;
+  o.hp = h.p;
+  o.shock = h.shock;
+  return o;
+}
+
+// face_flux (flux.wgsl:92-125) from the left cell's upper side and the right cell's lower side
+// along `axis`. Momentum comes back in grid components.
+template <typename T, int RS>
+__device__ __forceinline__ Flux<T> face_from_sides(const Params<T>& P, const SideData<T>& L, const SideData<T>& R,
+                                                   int axis) {
+  T w = tmax(L.shock, R.shock);
+  T lo_rho = T(0.5) * tmin(L.hrho, R.hrho);
+  T lo_p = T(0.5) * tmin(L.hp, R.hp);
+  // rot (flux.wgsl:37-41): axis 0 -> (x, y, z), 1 -> (y, z, x), 2 -> (z, x, y)
+  T ln, l1, l2, rn, r1, r2;
+  if (axis == 0) { ln = L.vx; l1 = L.vy; l2 = L.vz; rn = R.vx; r1 = R.vy; r2 = R.vz; }
+  else if (axis == 1) { ln = L.vy; l1 = L.vz; l2 = L.vx; rn = R.vy; r1 = R.vz; r2 = R.vx; }
+  else { ln = L.vz; l1 = L.vx; l2 = L.vy; rn = R.vz; r1 = R.vx; r2 = R.vy; }
+  Side<T> sl = make_side(P, L.rho, ln, l1, l2, L.p, L.s, lo_rho, lo_p);
+  Side<T> sr = make_side(P, R.rho, rn, r1, r2, R.p, R.s, lo_rho, lo_p);
+  Flux<T> F = riemann<T, RS>(sl, sr, w);
+  // unrot: the face frame's (n, t1, t2) back to grid components
+  T n = F.mx, t1 = F.my, t2 = F.mz;
+  if (axis == 0) { F.mx = n; F.my = t1; F.mz = t2; }
+  else if (axis == 1) { F.mx = t2; F.my = n; F.mz = t1; }
+  else { F.mx = t1; F.my = t2; F.mz = n; }
+  return F;
+}
+
+// face_flux between the half-step states hl (left) and hr (right), given twice the increments
+// from each cell toward the face (il: the left cell's up increment, ir: the right cell's down
+// increment).
 template <typename T, int RS>
 __device__ __forceinline__ Flux<T> face_flux(const Params<T>& P, const Half<T>& hl, const Inc<T>& il,
                                              const Half<T>& hr, const Inc<T>& ir, int axis) {
-  T w = tmax(hl.shock, hr.shock);
-  T lo_rho = T(0.5) * tmin(hl.rho, hr.rho);
-  T lo_p = T(0.5) * tmin(hl.p, hr.p);
-  T hsl = T(0.5) * (T(1) - hl.fo);
-  T hsr = T(-0.5) * (T(1) - hr.fo);
-  // rot (flux.wgsl:37-41): axis 0 -> (x, y, z), 1 -> (y, z, x), 2 -> (z, x, y)
-  T lv[3] = {hl.vx + hsl * il.vx, hl.vy + hsl * il.vy, hl.vz + hsl * il.vz};
-  T rv[3] = {hr.vx + hsr * ir.vx, hr.vy + hsr * ir.vy, hr.vz + hsr * ir.vz};
-  int a1 = axis == 2 ? 0 : axis + 1;
-  int a2 = axis == 0 ? 2 : axis - 1;
-  Side<T> sl = make_side(P, hl.rho + hsl * il.rho, lv[axis], lv[a1], lv[a2], hl.p + hsl * il.p, hl.s + hsl * il.s,
-                         lo_rho, lo_p);
-  Side<T> sr = make_side(P, hr.rho + hsr * ir.rho, rv[axis], rv[a1], rv[a2], hr.p + hsr * ir.p, hr.s + hsr * ir.s,
-                         lo_rho, lo_p);
-  Flux<T> F = riemann<T, RS>(sl, sr, w);
-  // unrot: the face frame's (n, t1, t2) back to grid components
-  T g[3];
-  g[axis] = F.mx; g[a1] = F.my; g[a2] = F.mz;
-  F.mx = g[0]; F.my = g[1]; F.mz = g[2];
-  return F;
+  return face_from_sides<T, RS>(P, side_of(hl, il, T(1)), side_of(hr, ir, T(-1)), axis);
 }
 
 // advance (hydro/advance.wgsl:31-52), dual_energy (hydro/dual_energy.wgsl:73-87) and the
@@ -422,9 +462,9 @@ __device__ __forceinline__ void advance(const Params<T>& P, T4<T> u1, T4<T> u2, 
   T pdv = tmax(p_half, P.p_floor) * net.vn;
   T eint = u2.y + dtdx * (net.ei + pdv);
   T sc = u2.z + dtdx * net.s;
-  T ke = T(0.5) * (mx * mx + my * my + mz * mz) / rho;
+  T ke = tdiv(T(0.5) * (mx * mx + my * my + mz * mz), rho);
   T eth = E - ke;
-  T emin = P.p_floor / P.gm1;
+  T emin = P.p_floor * P.igm1;
   T en0, en1;
   if (eth > P.eta_dual * E && eth >= emin && eth >= T(0)) {
     en0 = E; en1 = eth;
@@ -439,8 +479,8 @@ __device__ __forceinline__ void advance(const Params<T>& P, T4<T> u1, T4<T> u2, 
     if (en1 > emax) { en1 = emax; changed = true; }
     T pmax = rho * P.fluff[2];
     T pm = tsqrt(mx * mx + my * my + mz * mz);
-    if (pm > pmax) { T f = pmax / pm; mx *= f; my *= f; mz *= f; changed = true; }
-    if (changed) en0 = T(0.5) * (mx * mx + my * my + mz * mz) / rho + en1;
+    if (pm > pmax) { T f = tdiv(pmax, pm); mx *= f; my *= f; mz *= f; changed = true; }
+    if (changed) en0 = tdiv(T(0.5) * (mx * mx + my * my + mz * mz), rho) + en1;
   }
   o1.x = rho; o1.y = mx; o1.z = my; o1.w = mz;
   o2.x = en0; o2.y = en1; o2.z = sc; o2.w = T(0);

@@ -234,19 +234,17 @@ __global__ __launch_bounds__(WX* WY* WZ) void k_update(Params<T> P, const T4<T>*
 }
 
 // ---------------- fused path ----------------
-// LDS layout of the fused kernel, structure of arrays over the block's NT = BX x BY columns:
-//   q[6]  primitives of the centre plane (rho, vx, vy, vz, p, s)
-//   h[8]  half step (rho, vx, vy, vz, p, s, shock, first-order)
-//   ix[6], iy[6]  twice the increments toward the upper x and y faces
-//   fx[8], fy[8]  the x and y face fluxes below each column
-// When the separate layout would exceed the CU's LDS, fx/fy alias h/ix (one more barrier).
+// LDS of the fused kernel, structure of arrays over the block's NT = BX x BY columns:
+//   q[2][7]        primitives (rho, vx, vy, vz, p, s, c) in two alternating slots: plane c (the
+//                  x/y neighbours and the column's own centre) and plane c - 1 (the column's z - 1
+//                  neighbour), which each column then overwrites with its plane c + 1
+//   sx[9], sy[9]   each column's upper-x and upper-y face side (SideData), for its neighbours
+//   fx[8], fy[8]   the x and y fluxes of the face below each column; they reuse sx/sy's space
+//                  once every face has read its sides (a barrier apart)
 template <typename T, int NT>
 struct FusedLayout {
-  static constexpr bool ALIAS = (size_t)NT * sizeof(T) * 42 > 150 * 1024;
-  static constexpr int Q = 0, H = 6 * NT, IX = 14 * NT, IY = 20 * NT;
-  static constexpr int FX = ALIAS ? 6 * NT : 26 * NT;
-  static constexpr int FY = ALIAS ? 14 * NT : 34 * NT;
-  static constexpr int SIZE = ALIAS ? 26 * NT : 42 * NT;
+  static constexpr int Q0 = 0, Q1 = 7 * NT, SX = 14 * NT, SY = 23 * NT, FX = 14 * NT, FY = 22 * NT;
+  static constexpr int SIZE = 32 * NT;
 };
 
 template <typename T, int NT>
@@ -256,6 +254,31 @@ __device__ __forceinline__ Prim<T> lds_prim(const T* L, int t) {
   p.p = L[4 * NT + t]; p.s = L[5 * NT + t];
   p.cs = T(0);
   return p;
+}
+template <typename T, int NT>
+__device__ __forceinline__ void lds_put_prim(T* L, int t, const Prim<T>& q) {
+  L[0 * NT + t] = q.rho; L[1 * NT + t] = q.vx; L[2 * NT + t] = q.vy; L[3 * NT + t] = q.vz;
+  L[4 * NT + t] = q.p; L[5 * NT + t] = q.s; L[6 * NT + t] = q.cs;
+}
+template <typename T, int NT>
+__device__ __forceinline__ Prim<T> lds_prim_c(const T* L, int t) {
+  Prim<T> p = lds_prim<T, NT>(L, t);
+  p.cs = L[6 * NT + t];
+  return p;
+}
+template <typename T, int NT>
+__device__ __forceinline__ void lds_put_side(T* L, int base, int t, const SideData<T>& d) {
+  L[base + 0 * NT + t] = d.rho; L[base + 1 * NT + t] = d.vx; L[base + 2 * NT + t] = d.vy;
+  L[base + 3 * NT + t] = d.vz; L[base + 4 * NT + t] = d.p; L[base + 5 * NT + t] = d.s;
+  L[base + 6 * NT + t] = d.hrho; L[base + 7 * NT + t] = d.hp; L[base + 8 * NT + t] = d.shock;
+}
+template <typename T, int NT>
+__device__ __forceinline__ SideData<T> lds_get_side(const T* L, int base, int t) {
+  SideData<T> d;
+  d.rho = L[base + 0 * NT + t]; d.vx = L[base + 1 * NT + t]; d.vy = L[base + 2 * NT + t];
+  d.vz = L[base + 3 * NT + t]; d.p = L[base + 4 * NT + t]; d.s = L[base + 5 * NT + t];
+  d.hrho = L[base + 6 * NT + t]; d.hp = L[base + 7 * NT + t]; d.shock = L[base + 8 * NT + t];
+  return d;
 }
 template <typename T, int NT>
 __device__ __forceinline__ void lds_put_flux(T* L, int base, int t, const Flux<T>& F) {
@@ -269,31 +292,29 @@ __device__ __forceinline__ Flux<T> lds_get_flux(const T* L, int base, int t) {
   F.e = L[base + 4 * NT + t]; F.ei = L[base + 5 * NT + t]; F.s = L[base + 6 * NT + t]; F.vn = L[base + 7 * NT + t];
   return F;
 }
-template <typename T, int NT>
-__device__ __forceinline__ Half<T> lds_get_half(const T* L, int n) {
-  Half<T> h;
-  constexpr int H = FusedLayout<T, NT>::H;
-  h.rho = L[H + 0 * NT + n]; h.vx = L[H + 1 * NT + n]; h.vy = L[H + 2 * NT + n]; h.vz = L[H + 3 * NT + n];
-  h.p = L[H + 4 * NT + n]; h.s = L[H + 5 * NT + n]; h.shock = L[H + 6 * NT + n]; h.fo = L[H + 7 * NT + n];
-  return h;
-}
-template <typename T, int NT>
-__device__ __forceinline__ Inc<T> lds_get_inc(const T* L, int base, int n) {
-  Inc<T> i;
-  i.rho = L[base + 0 * NT + n]; i.vx = L[base + 1 * NT + n]; i.vy = L[base + 2 * NT + n];
-  i.vz = L[base + 3 * NT + n]; i.p = L[base + 4 * NT + n]; i.s = L[base + 5 * NT + n];
-  return i;
-}
+
 
 // The fused step. Block BX x BY threads; thread (tx, ty) owns column (x0 + tx - 2, y0 + ty - 2).
 // Grid (ceil(nx / TX), ceil(ny / TY), ceil(nz / zc)). Planes [z0, z0 + zc) are updated.
 // Boundary conditions along x and y: periodic, outflow or reflecting; along z: periodic (the
 // host routes other z boundaries to the plain path).
-template <typename T, int LIM, int RS, bool ISEN, int BX, int BY, bool DEBUG>
-__global__ __launch_bounds__(BX* BY) void k_fused(Params<T> P, const T4<T>* __restrict__ U1,
-                                                   const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1,
-                                                   T4<T>* __restrict__ O2, const T* __restrict__ clk, int zc,
-                                                   unsigned long long* next_slot) {
+//
+// Launch bound: in FP32 at least four waves per SIMD (two 512-thread blocks or one of 1024 per
+// CU), which caps the kernel at 128 VGPRs; the FP64 twin, whose values take two registers each,
+// is left uncapped.
+//
+// Per plane c of the march (4 barriers):
+//   1. plane c's primitives (converted last iteration) into LDS                      | barrier
+//   2. half step of plane c from the LDS x/y neighbours and the register z neighbours;
+//      its upper x and y sides into LDS                                              | barrier
+//   3. the x and y faces below this column and the z face below plane c             | barrier
+//   4. those x/y fluxes into LDS (over the sides)                                    | barrier
+//   5. plane c's x/y net inflow; the update of plane c - 1 (its state re-read, from L2) and
+//      the next step's signal rate
+template <typename T, int LIM, int RS, bool ISEN, int BX, int BY, bool DEBUG, bool MAPBC>
+__global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 4 : 1)) void k_fused(
+    Params<T> P, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1,
+    T4<T>* __restrict__ O2, const T* __restrict__ clk, int zc, unsigned long long* next_slot) {
   constexpr int NT = BX * BY;
   constexpr int TX = BX - 4, TY = BY - 4;
   using LY = FusedLayout<T, NT>;
@@ -313,8 +334,6 @@ __global__ __launch_bounds__(BX* BY) void k_fused(Params<T> P, const T4<T>* __re
   // whose results are never used)
   AxisMap mx = map_axis(x, nx, P.bc_lo[0], P.bc_hi[0]);
   AxisMap my = map_axis(y, ny, P.bc_lo[1], P.bc_hi[1]);
-  mx.i = min(max(mx.i, 0), nx - 1);
-  my.i = min(max(my.i, 0), ny - 1);
   const size_t col = (size_t)my.i * nx + mx.i;
   const size_t plane = (size_t)nx * ny;
 
@@ -326,19 +345,14 @@ __global__ __launch_bounds__(BX* BY) void k_fused(Params<T> P, const T4<T>* __re
   // A ghost column beyond a non-periodic face takes the half step of the cell it maps to
   // (load_pred, hydro/update.wgsl:47-63): that cell's stencil sits one column inward.
   int sx = tx, sy = ty;
-  if (!real_x && P.bc_lo[0] != BC_PERIODIC && x < 0) sx = tx + (mx.i - x);
-  if (!real_x && P.bc_hi[0] != BC_PERIODIC && x >= nx) sx = tx + (mx.i - x);
-  if (!real_y && P.bc_lo[1] != BC_PERIODIC && y < 0) sy = ty + (my.i - y);
-  if (!real_y && P.bc_hi[1] != BC_PERIODIC && y >= ny) sy = ty + (my.i - y);
-  const bool mapped = (sx != tx) || (sy != ty);
+  if (!real_x && ((x < 0 && P.bc_lo[0] != BC_PERIODIC) || (x >= nx && P.bc_hi[0] != BC_PERIODIC))) sx = tx + (mx.i - x);
+  if (!real_y && ((y < 0 && P.bc_lo[1] != BC_PERIODIC) || (y >= ny && P.bc_hi[1] != BC_PERIODIC))) sy = ty + (my.i - y);
+  // (MAPBC: the grid has a non-periodic x or y face; without one no column is mapped and the
+  // kernel is compiled without that path)
+  const bool mapped = MAPBC && ((sx != tx) || (sy != ty));
   const int ts = min(max(sx, 1), BX - 2) + BX * min(max(sy, 1), BY - 2);
 
-  auto load_plane = [&](int z, T4<T>& u1, T4<T>& u2) {
-    int zz = ((z % nz) + nz) % nz;
-    size_t o = (size_t)zz * plane + col;
-    u1 = U1[o];
-    u2 = U2[o];
-  };
+  auto plane_of = [&](int z) { return (size_t)(((z % nz) + nz) % nz) * plane + col; };
   auto prim_of = [&](T4<T> u1, T4<T> u2) {
     Prim<T> q = prim_from_cons(P, u1, u2);
     q.vx = bc_vel(q.vx, mx.flip, mx.side);
@@ -346,44 +360,50 @@ __global__ __launch_bounds__(BX* BY) void k_fused(Params<T> P, const T4<T>* __re
     return q;
   };
 
-  T4<T> a1, a2, b1, b2;
-  load_plane(z0 - 2, a1, a2);
-  Prim<T> pm = prim_of(a1, a2);
-  load_plane(z0 - 1, a1, a2);
-  Prim<T> pc = prim_of(a1, a2);
-  T4<T> uc1 = a1, uc2 = a2;           // the state of plane c (for the update one plane later)
-  load_plane(z0, b1, b2);              // plane c + 1, converted at the top of the loop
+  // prologue: planes z0 - 2 and z0 - 1 into the slots that iteration c = z0 - 1 reads as its
+  // c - 1 and c; plane z0 in registers
+  T4<T> b1, b2;
+  size_t o0 = plane_of(z0 - 2);
+  lds_put_prim<T, NT>(L + LY::Q0, t, prim_of(U1[o0], U2[o0]));
+  o0 = plane_of(z0 - 1);
+  lds_put_prim<T, NT>(L + LY::Q1, t, prim_of(U1[o0], U2[o0]));
+  o0 = plane_of(z0);
+  b1 = U1[o0]; b2 = U2[o0];
 
-  // carried from the previous plane
-  Half<T> h_prev;                      // half step of plane c - 1
-  Inc<T> iz_prev;                      // its twice increment toward the upper z face
+  SideData<T> zs_prev;                 // the upper z side of plane c - 1
   Flux<T> fz_lo = zero_flux<T>();      // the z face below plane c - 1
-  Flux<T> axy_prev = zero_flux<T>();   // its (x_lo - x_hi) + (y_lo - y_hi)
-  T4<T> uprev1, uprev2;                // its state
+  Flux<T> axy_prev = zero_flux<T>();   // plane c - 1's (x_lo - x_hi) + (y_lo - y_hi)
   T rate_max = T(0);
-  uprev1 = uc1; uprev2 = uc2;
 
   for (int c = z0 - 1; c <= z1; c++) {
     Prim<T> pp = prim_of(b1, b2);
-    T4<T> up1 = b1, up2 = b2;
-    if (c + 2 <= z1 + 1) load_plane(c + 2, b1, b2);   // prefetch plane c + 2
-
-    // 1. the centre plane's primitives into LDS
-    L[0 * NT + t] = pc.rho; L[1 * NT + t] = pc.vx; L[2 * NT + t] = pc.vy; L[3 * NT + t] = pc.vz;
-    L[4 * NT + t] = pc.p; L[5 * NT + t] = pc.s;
+    if (c + 2 <= z1 + 1) {             // prefetch plane c + 2
+      size_t o = plane_of(c + 2);
+      b1 = U1[o];
+      b2 = U2[o];
+    }
+    // 1. plane c is in slot Lq (written last iteration), plane c - 1 in Lm; the barrier also
+    //    keeps this iteration's side writes behind the last iteration's flux reads (they share
+    //    LDS)
+    const bool odd = ((c - z0 + 1) & 1) != 0;
+    const T* Lq = L + (odd ? LY::Q0 : LY::Q1);
+    T* Lm = L + (odd ? LY::Q1 : LY::Q0);
     __syncthreads();
 
-    // 2. half step and increments of plane c (hydro/predictor.wgsl:46-209)
-    Half<T> hc;
-    Inc<T> ixu, ixd, iyu, iyd, izu, izd;
+    // 2. half step and sides of plane c (hydro/predictor.wgsl:46-209)
+    SideData<T> sxl, syl, szl, szu;
     if (in_r1) {
-      Prim<T> xm = lds_prim<T, NT>(L, t - 1), xp = lds_prim<T, NT>(L, t + 1);
-      Prim<T> ym = lds_prim<T, NT>(L, t - BX), yp = lds_prim<T, NT>(L, t + BX);
+      Prim<T> pc = lds_prim_c<T, NT>(Lq, t);
+      Prim<T> pm = lds_prim<T, NT>(Lm, t);
+      Prim<T> xm = lds_prim<T, NT>(Lq, t - 1), xp = lds_prim<T, NT>(Lq, t + 1);
+      Prim<T> ym = lds_prim<T, NT>(Lq, t - BX), yp = lds_prim<T, NT>(Lq, t + BX);
+      Inc<T> ixu, ixd, iyu, iyd, izu, izd;
       AxisSlope<T> sl[3];
       sl[0] = axis_slope<T, LIM, ISEN>(P, xm, pc, xp, &ixu, &ixd);
       sl[1] = axis_slope<T, LIM, ISEN>(P, ym, pc, yp, &iyu, &iyd);
       sl[2] = axis_slope<T, LIM, ISEN>(P, pm, pc, pp, &izu, &izd);
-      if (!mapped) {
+      Half<T> hc;
+      if (!MAPBC || !mapped) {
         T conv = (xp.vx - xm.vx) + (yp.vy - ym.vy) + (pp.vz - pm.vz);
         T jump = tmax(jump_of(xm.p, xp.p), tmax(jump_of(ym.p, yp.p), jump_of(pm.p, pp.p)));
         hc = hancock(P, pc, sl, conv, jump, dt);
@@ -394,8 +414,8 @@ __global__ __launch_bounds__(BX* BY) void k_fused(Params<T> P, const T4<T>* __re
         qc.vx = bc_vel(qc.vx, mx.flip, 0); qc.vy = bc_vel(qc.vy, my.flip, 0);
         qm.vx = bc_vel(qm.vx, mx.flip, 0); qm.vy = bc_vel(qm.vy, my.flip, 0);
         qp.vx = bc_vel(qp.vx, mx.flip, 0); qp.vy = bc_vel(qp.vy, my.flip, 0);
-        Prim<T> sxm = lds_prim<T, NT>(L, ts - 1), sxp = lds_prim<T, NT>(L, ts + 1);
-        Prim<T> sym = lds_prim<T, NT>(L, ts - BX), syp = lds_prim<T, NT>(L, ts + BX);
+        Prim<T> sxm = lds_prim<T, NT>(Lq, ts - 1), sxp = lds_prim<T, NT>(Lq, ts + 1);
+        Prim<T> sym = lds_prim<T, NT>(Lq, ts - BX), syp = lds_prim<T, NT>(Lq, ts + BX);
         AxisSlope<T> ms[3];
         ms[0] = axis_slope<T, LIM, ISEN>(P, sxm, qc, sxp, nullptr, nullptr);
         ms[1] = axis_slope<T, LIM, ISEN>(P, sym, qc, syp, nullptr, nullptr);
@@ -406,48 +426,46 @@ __global__ __launch_bounds__(BX* BY) void k_fused(Params<T> P, const T4<T>* __re
         hc.vx = bc_vel(hc.vx, mx.flip, mx.side);
         hc.vy = bc_vel(hc.vy, my.flip, my.side);
       }
-      // 3. what the neighbours above need
-      constexpr int H = LY::H, IX = LY::IX, IY = LY::IY;
-      L[H + 0 * NT + t] = hc.rho; L[H + 1 * NT + t] = hc.vx; L[H + 2 * NT + t] = hc.vy; L[H + 3 * NT + t] = hc.vz;
-      L[H + 4 * NT + t] = hc.p; L[H + 5 * NT + t] = hc.s; L[H + 6 * NT + t] = hc.shock; L[H + 7 * NT + t] = hc.fo;
-      L[IX + 0 * NT + t] = ixu.rho; L[IX + 1 * NT + t] = ixu.vx; L[IX + 2 * NT + t] = ixu.vy;
-      L[IX + 3 * NT + t] = ixu.vz; L[IX + 4 * NT + t] = ixu.p; L[IX + 5 * NT + t] = ixu.s;
-      L[IY + 0 * NT + t] = iyu.rho; L[IY + 1 * NT + t] = iyu.vx; L[IY + 2 * NT + t] = iyu.vy;
-      L[IY + 3 * NT + t] = iyu.vz; L[IY + 4 * NT + t] = iyu.p; L[IY + 5 * NT + t] = iyu.s;
+      sxl = side_of(hc, ixd, T(-1));
+      syl = side_of(hc, iyd, T(-1));
+      szl = side_of(hc, izd, T(-1));
+      szu = side_of(hc, izu, T(1));
+      // what the neighbours above need
+      lds_put_side<T, NT>(L, LY::SX, t, side_of(hc, ixu, T(1)));
+      lds_put_side<T, NT>(L, LY::SY, t, side_of(hc, iyu, T(1)));
     }
+    // plane c + 1 into the slot plane c - 1 leaves (only this column reads its own c - 1 entry)
+    lds_put_prim<T, NT>(Lm, t, pp);
     __syncthreads();
 
-    // 4. x and y faces of plane c: the face below this column, from the neighbour's upper side
+    // 3. the x and y faces below this column, and the z face below plane c
     const bool live = c >= z0 && c < z1;
-    Flux<T> Fx = zero_flux<T>(), Fy = zero_flux<T>();
-    if (live && xface)
-      Fx = face_flux<T, RS>(P, lds_get_half<T, NT>(L, t - 1), lds_get_inc<T, NT>(L, LY::IX, t - 1), hc, ixd, 0);
-    if (live && yface)
-      Fy = face_flux<T, RS>(P, lds_get_half<T, NT>(L, t - BX), lds_get_inc<T, NT>(L, LY::IY, t - BX), hc, iyd, 1);
-    if constexpr (LY::ALIAS) __syncthreads();
+    Flux<T> Fx = zero_flux<T>(), Fy = zero_flux<T>(), fz = zero_flux<T>();
+    if (live && xface) Fx = face_from_sides<T, RS>(P, lds_get_side<T, NT>(L, LY::SX, t - 1), sxl, 0);
+    if (live && yface) Fy = face_from_sides<T, RS>(P, lds_get_side<T, NT>(L, LY::SY, t - BX), syl, 1);
+    if (tile && c >= z0) fz = face_from_sides<T, RS>(P, zs_prev, szl, 2);
+    __syncthreads();
+    // 4. the x/y fluxes into LDS
     if (live && xface) lds_put_flux<T, NT>(L, LY::FX, t, Fx);
     if (live && yface) lds_put_flux<T, NT>(L, LY::FY, t, Fy);
-    // z face between planes c - 1 and c
-    Flux<T> fz = zero_flux<T>();
-    if (tile && c >= z0) fz = face_flux<T, RS>(P, h_prev, iz_prev, hc, izd, 2);
     __syncthreads();
 
     // 5. this plane's x/y net inflow, and the update of plane c - 1
     Flux<T> axy = zero_flux<T>();
     if (live && tile) {
-      Flux<T> xl = lds_get_flux<T, NT>(L, LY::FX, t), xh = lds_get_flux<T, NT>(L, LY::FX, t + 1);
-      Flux<T> yl = lds_get_flux<T, NT>(L, LY::FY, t), yh = lds_get_flux<T, NT>(L, LY::FY, t + BX);
+      Flux<T> xh = lds_get_flux<T, NT>(L, LY::FX, t + 1);
+      Flux<T> yh = lds_get_flux<T, NT>(L, LY::FY, t + BX);
       if constexpr (DEBUG) {
-        axy.m = xl.m; axy.mx = xh.m; axy.my = yl.m; axy.mz = yh.m;
+        axy.m = Fx.m; axy.mx = xh.m; axy.my = Fy.m; axy.mz = yh.m;
       } else {
-        axy.m = (xl.m - xh.m) + (yl.m - yh.m);
-        axy.mx = (xl.mx - xh.mx) + (yl.mx - yh.mx);
-        axy.my = (xl.my - xh.my) + (yl.my - yh.my);
-        axy.mz = (xl.mz - xh.mz) + (yl.mz - yh.mz);
-        axy.e = (xl.e - xh.e) + (yl.e - yh.e);
-        axy.ei = (xl.ei - xh.ei) + (yl.ei - yh.ei);
-        axy.s = (xl.s - xh.s) + (yl.s - yh.s);
-        axy.vn = (xl.vn - xh.vn) + (yl.vn - yh.vn);
+        axy.m = (Fx.m - xh.m) + (Fy.m - yh.m);
+        axy.mx = (Fx.mx - xh.mx) + (Fy.mx - yh.mx);
+        axy.my = (Fx.my - xh.my) + (Fy.my - yh.my);
+        axy.mz = (Fx.mz - xh.mz) + (Fy.mz - yh.mz);
+        axy.e = (Fx.e - xh.e) + (Fy.e - yh.e);
+        axy.ei = (Fx.ei - xh.ei) + (Fy.ei - yh.ei);
+        axy.s = (Fx.s - xh.s) + (Fy.s - yh.s);
+        axy.vn = (Fx.vn - xh.vn) + (Fy.vn - yh.vn);
       }
     }
     if (tile && c - 1 >= z0 && c - 1 < z1) {
@@ -466,7 +484,7 @@ __global__ __launch_bounds__(BX* BY) void k_fused(Params<T> P, const T4<T>* __re
         net.ei = axy_prev.ei + (fz_lo.ei - fz.ei);
         net.s = axy_prev.s + (fz_lo.s - fz.s);
         net.vn = axy_prev.vn + (fz_lo.vn - fz.vn);
-        advance(P, uprev1, uprev2, net, h_prev.p, dtdx, o1, o2);
+        advance(P, U1[o], U2[o], net, zs_prev.hp, dtdx, o1, o2);
         // the next step's signal rate (hydro/rates.wgsl:60-62), from the state just made
         Prim<T> w = prim_from_cons(P, o1, o2);
         rate_max = tmax(rate_max, rate_of(P, w));
@@ -475,14 +493,9 @@ __global__ __launch_bounds__(BX* BY) void k_fused(Params<T> P, const T4<T>* __re
       O2[o] = o2;
     }
     // rotate
-    h_prev = hc;
-    iz_prev = izu;
+    zs_prev = szu;
     fz_lo = fz;
     axy_prev = axy;
-    uprev1 = uc1; uprev2 = uc2;
-    uc1 = up1; uc2 = up2;
-    pm = pc;
-    pc = pp;
   }
   if constexpr (!DEBUG) {
     // block maximum of the next step's rate (order-independent: max is exact)
@@ -593,12 +606,14 @@ struct UniformImpl {
     const Params<T>& P = *static_cast<const Params<T>*>(L.params);
     constexpr int TX = BX - 4, TY = BY - 4;
     dim3 g((P.n[0] + TX - 1) / TX, (P.n[1] + TY - 1) / TY, (P.n[2] + zc - 1) / zc);
-    if (debug)
-      hipLaunchKernelGGL((k_fused<T, LIM, RS, ISEN, BX, BY, true>), g, dim3(BX, BY), 0, L.stream, P, (const T4<T>*)u1,
-                         (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, zc, next_slot);
-    else
-      hipLaunchKernelGGL((k_fused<T, LIM, RS, ISEN, BX, BY, false>), g, dim3(BX, BY), 0, L.stream, P, (const T4<T>*)u1,
-                         (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, zc, next_slot);
+    bool mapbc = false;
+    for (int a = 0; a < 2; a++) mapbc |= P.bc_lo[a] != BC_PERIODIC || P.bc_hi[a] != BC_PERIODIC;
+#define SHX_FUSED(D, M)                                                                                          \
+  hipLaunchKernelGGL((k_fused<T, LIM, RS, ISEN, BX, BY, D, M>), g, dim3(BX, BY), 0, L.stream, P, (const T4<T>*)u1, \
+                     (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, zc, next_slot)
+    if (debug) { if (mapbc) SHX_FUSED(true, true); else SHX_FUSED(true, false); }
+    else { if (mapbc) SHX_FUSED(false, true); else SHX_FUSED(false, false); }
+#undef SHX_FUSED
   }
   static void fused(const Launch& L, int tile, const void* u1, const void* u2, void* o1, void* o2, const void* clk,
                     int zc, unsigned long long* next_slot, bool debug) {
