@@ -5,7 +5,9 @@ this package. It wraps ``oracle/_build/liboracle.so`` (built by ``oracle/Makefil
 
 - ``wgsl32`` / ``wgsl64``: the WGSL GPU scheme (src/hydro/*.wgsl) in FP32 and FP64;
 - ``RefHydro``: the reference's float64 CPU scheme (src/reference/hydro.ts:172-458);
-- ``exact_riemann``: Toro's exact solver (src/reference/riemann_exact.ts:9-71).
+- ``exact_riemann``: Toro's exact solver (src/reference/riemann_exact.ts:9-71);
+- ``AmrTable`` / ``Wgsl.amr_step`` / ``Wgsl.amr_prolong``: the block table of a fixed block tree
+  (src/amr/tree.ts) and one block-AMR step (the src/amr WGSL passes), 3D.
 
 Parity status: every restated function is pinned by tests/test_oracle.py (known answers,
 closed forms, invariants, and cross-checks between the two independent restatements).
@@ -36,6 +38,16 @@ class OrcParams(C.Structure):
     ]
 
 
+class OrcAmr(C.Structure):
+    _fields_ = [("nblocks", C.c_int), ("root", C.c_int * 3), ("bc_lo", C.c_int * 3), ("bc_hi", C.c_int * 3),
+                ("blk", C.POINTER(C.c_int)), ("tab", C.POINTER(C.c_int))]
+
+
+BT_STRIDE = 32
+BS = 16
+PAD = 20
+
+
 def build() -> None:
     """Compile the oracle with its Makefile (gcc, no GPU)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -63,6 +75,16 @@ def lib():
             getattr(_lib, pfx + "prim").argtypes = [C.POINTER(OrcParams), P, P, P, P]
             getattr(_lib, pfx + "riemann").argtypes = [C.POINTER(OrcParams), P, P, ct, P]
             getattr(_lib, pfx + "predictor").argtypes = [C.POINTER(OrcParams), P, ct, P]
+            getattr(_lib, pfx + "amr_step").argtypes = [C.POINTER(OrcParams), C.POINTER(OrcAmr), C.c_double, C.c_int,
+                                                        P, P, P]
+            getattr(_lib, pfx + "amr_assemble").argtypes = [C.POINTER(OrcParams), C.POINTER(OrcAmr), C.c_double,
+                                                            P, P, P]
+            getattr(_lib, pfx + "amr_assemble").restype = ct
+            getattr(_lib, pfx + "amr_prolong").argtypes = [C.POINTER(OrcParams), P, C.POINTER(C.c_int),
+                                                           C.POINTER(C.c_int), P]
+        I3 = C.c_int * 3
+        _lib.orc_amr_build.argtypes = [C.POINTER(OrcAmr), C.POINTER(C.c_int), C.c_int, I3, I3, I3]
+        _lib.orc_amr_free.argtypes = [C.POINTER(OrcAmr)]
         _lib.orc_ref_create.argtypes = [C.POINTER(OrcParams)]
         _lib.orc_ref_create.restype = C.c_void_p
         _lib.orc_ref_destroy.argtypes = [C.c_void_p]
@@ -178,6 +200,31 @@ class Wgsl:
         self.fn("predictor")(C.byref(prm), _ptr(st, self.ct), dt, _ptr(out, self.ct))
         return out
 
+    def amr_step(self, prm, table, dx0, u1, u2, clk, compensated=True):
+        """One block-AMR step in place. u1, u2: (nblocks * 4096, 4) leaf interiors, block by block,
+        x fastest; clk: 4-array (dt, t, tc, steps)."""
+        assert u1.dtype == self.dtype and u1.flags.c_contiguous and u1.shape == (table.nblocks * BS ** 3, 4)
+        assert u2.dtype == self.dtype and u2.flags.c_contiguous and u2.shape == u1.shape
+        self.fn("amr_step")(C.byref(prm), C.byref(table.t), dx0, int(bool(compensated)), _ptr(u1, self.ct),
+                            _ptr(u2, self.ct), _ptr(clk, self.ct))
+
+    def amr_assemble(self, prm, table, dx0, u1, u2):
+        """The assemble pass: (W of shape (nblocks, 20, 20, 20, 8) [z, y, x], largest leaf rate/dx)."""
+        W = np.zeros((table.nblocks, PAD, PAD, PAD, 8), self.dtype)
+        r = self.fn("amr_assemble")(C.byref(prm), C.byref(table.t), dx0, _ptr(self.arr(u1), self.ct),
+                                    _ptr(self.arr(u2), self.ct), _ptr(W, self.ct))
+        return W, r
+
+    def amr_prolong(self, prm, stencil, ok, par):
+        """The fine child of parity par (3 ints of 0/1) of a coarse cell, from its 7-cell stencil
+        (centre, -x, +x, -y, +y, -z, +z) of conserved states (7, 8); ok drops missing cells."""
+        st = self.arr(stencil).reshape(-1)
+        okc = (C.c_int * 7)(*[int(bool(v)) for v in ok])
+        pc = (C.c_int * 3)(*[int(v) for v in par])
+        out = np.empty(8, self.dtype)
+        self.fn("amr_prolong")(C.byref(prm), _ptr(st, self.ct), okc, pc, _ptr(out, self.ct))
+        return out
+
     def run(self, prm, u1, u2, steps, clk=None):
         """Advance `steps` steps in place; returns the clock."""
         if clk is None:
@@ -243,6 +290,28 @@ class RefHydro:
     @staticmethod
     def set_threads(n):
         lib().orc_ref_set_threads(int(n))
+
+
+class AmrTable:
+    """The block table of a fixed tree (src/amr/tree.ts:642-675): blocks (nblocks, 4) of
+    (level, ix, iy, iz), parents and leaves; boundary codes per axis (0 periodic)."""
+
+    def __init__(self, blocks, root, bc_lo=(0, 0, 0), bc_hi=(0, 0, 0)):
+        self.blocks = np.ascontiguousarray(blocks, dtype=np.int32)
+        self.nblocks = len(self.blocks)
+        self.t = OrcAmr()
+        rc = lib().orc_amr_build(C.byref(self.t), _ptr(self.blocks, C.c_int), self.nblocks, (C.c_int * 3)(*root),
+                                 (C.c_int * 3)(*bc_lo), (C.c_int * 3)(*bc_hi))
+        if rc != 0:
+            lib().orc_amr_free(C.byref(self.t))
+            raise ValueError(f"block {-1 - rc} breaks 2:1 balance")
+
+    def table(self):
+        return np.ctypeslib.as_array(self.t.tab, shape=(self.nblocks, BT_STRIDE)).copy()
+
+    def __del__(self):
+        if getattr(self, "t", None) is not None and self.t.tab:
+            lib().orc_amr_free(C.byref(self.t))
 
 
 def exact_riemann(L, R, gamma, xi):

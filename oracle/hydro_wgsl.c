@@ -265,8 +265,9 @@ static Q edge(const P_t* P, const Q* l, const Q* c, const Q* r, int up) {
 }
 
 /* ---------------- predictor: src/hydro/predictor.wgsl:46-209 ---------------- */
-/* nb = {xm, xp, ym, yp, zm, zp} */
-static Q predictor(const P_t* P, const Q* q, const Q nb[6], real dt) {
+/* nb = {xm, xp, ym, yp, zm, zp}; mask multiplies each axis's slopes (1 except at a block's rim
+   next to a coarser block, src/amr/predict.wgsl:205-213) */
+static Q predictor_masked(const P_t* P, const Q* q, const Q nb[6], real dt, const real mask[3]) {
   Q s[3];
   real big = RL(0.0), fall = RL(0.0);
   int sym = lim_symmetric(P->limiter);
@@ -284,7 +285,7 @@ static Q predictor(const P_t* P, const Q* q, const Q nb[6], real dt) {
       big = rmax(big, rabs(d.b[0] + RL(0.0)));
       fall = rmin(fall, rmin(RL(0.5) * (u.a[0] + RL(0.0)), RL(-0.5) * (d.a[0] + RL(0.0))));
     }
-    for (int i = 0; i < 4; i++) { s[a].a[i] = sa.a[i] * RL(1.0); s[a].b[i] = sa.b[i] * RL(1.0); }
+    for (int i = 0; i < 4; i++) { s[a].a[i] = sa.a[i] * mask[a]; s[a].b[i] = sa.b[i] * mask[a]; }
   }
   const Q *xm = &nb[0], *xp = &nb[1], *ym = &nb[2], *yp = &nb[3], *zm = &nb[4], *zp = &nb[5];
   real conv = (xp->a[1] - xm->a[1]) + (yp->a[2] - ym->a[2]) + (zp->a[3] - zm->a[3]);
@@ -324,6 +325,10 @@ static Q predictor(const P_t* P, const Q* q, const Q nb[6], real dt) {
   }
   o.b[0] = b0; o.b[1] = b1; o.b[2] = shock; o.b[3] = first;
   return o;
+}
+static Q predictor(const P_t* P, const Q* q, const Q nb[6], real dt) {
+  static const real one[3] = {1, 1, 1};
+  return predictor_masked(P, q, nb, dt, one);
 }
 
 /* ---------------- faces: src/hydro/flux.wgsl ---------------- */
@@ -690,5 +695,367 @@ void FN(predictor)(const orc_params* prm, const real* st, real dt, real out[8]) 
   for (int n = 0; n < 6; n++)
     for (int i = 0; i < 4; i++) { nb[n].a[i] = st[8 * (n + 1) + i]; nb[n].b[i] = st[8 * (n + 1) + 4 + i]; }
   Q o = predictor(&P, &q, nb, dt);
+  for (int i = 0; i < 4; i++) { out[i] = o.a[i]; out[4 + i] = o.b[i]; }
+}
+
+/* ====================== block AMR: the src/amr WGSL passes ======================
+ * Blocks of 16^3 cells with 2 ghost layers (BS = 16, NGH = 2, PADW = 20: src/amr/grid.wgsl:7-9).
+ * Per block: U (interior, leaves only), W (primitives of every texel but edges and corners),
+ * PA/PB (half steps of leaves' interiors and first ghost layer). Edge and corner texels of W are
+ * never written and stay zero, as in a fresh atlas (src/amr/grid.wgsl:4-5). One global dt. */
+#define BS_ ORC_BS
+#define PD_ ORC_PAD
+#define NG_ 2
+
+typedef struct { real a[4]; real b[4]; } Cons;
+
+static int bt_(const orc_amr* t, int s, int k) { return t->tab[ORC_BT_STRIDE * s + k]; }
+static int leaf_(const orc_amr* t, int s) { return (bt_(t, s, 4) & 3) == 3; }
+static size_t pidx(int s, const int l[3]) {
+  return (size_t)s * PD_ * PD_ * PD_ + ((size_t)(l[2] + NG_) * PD_ + (l[1] + NG_)) * PD_ + (l[0] + NG_);
+}
+static size_t iidx(int s, const int l[3]) {
+  return (size_t)s * BS_ * BS_ * BS_ + ((size_t)l[2] * BS_ + l[1]) * BS_ + l[0];
+}
+
+/* ghost_src (src/amr/grid.wgsl:184-222) */
+typedef struct { int kind, slot, l[3]; real flip[3], side[3]; } Src;
+static Src ghost_src(const orc_amr* t, int slot, const int l[3]) {
+  Src s;
+  s.kind = 0; s.slot = slot;
+  for (int a = 0; a < 3; a++) { s.l[a] = l[a]; s.flip[a] = RL(1.0); s.side[a] = RL(0.0); }
+  int count = 0, axis = 0;
+  for (int a = 0; a < 3; a++) if (l[a] < 0 || l[a] >= BS_) { count++; axis = a; }
+  if (count == 0) return s;
+  if (count > 1) { s.kind = 2; return s; }
+  int upper = l[axis] >= BS_;
+  int nb = bt_(t, slot, 6 + 2 * axis + upper);
+  if (nb >= 0) {
+    s.slot = nb;
+    s.l[axis] = l[axis] + (upper ? -BS_ : BS_);
+  } else if (nb == -1) {
+    s.kind = 1;
+    s.slot = bt_(t, slot, 20 + 2 * axis + upper);
+  } else {
+    int k = -1 - nb;
+    s.l[axis] = bc_ghost_cell(l[axis], BS_, k);
+    if (bc_wall(k)) {
+      if (k == 4 || k == 5) { s.flip[0] = -s.flip[0]; s.flip[1] = -s.flip[1]; s.flip[2] = -s.flip[2]; }
+      else s.flip[axis] = -s.flip[axis];
+    }
+    s.side[axis] = (k == 3) ? (upper ? RL(1.0) : RL(-1.0)) : RL(0.0);
+  }
+  return s;
+}
+
+static Cons u_leaf(const orc_amr* t, const real* u1, const real* u2, int slot, const int l[3]) {
+  (void)t;
+  Cons c;
+  size_t o = iidx(slot, l) * 4;
+  for (int i = 0; i < 4; i++) { c.a[i] = u1[o + i]; c.b[i] = u2[o + i]; }
+  return c;
+}
+/* u_at (src/amr/cons.wgsl:19-43): a parent's cell is the mean of its children's */
+static Cons u_at(const orc_amr* t, const real* u1, const real* u2, int slot, const int l[3]) {
+  if (leaf_(t, slot)) return u_leaf(t, u1, u2, slot, l);
+  int h[3] = {l[0] >= BS_ / 2, l[1] >= BS_ / 2, l[2] >= BS_ / 2};
+  int ch = bt_(t, slot, 12 + (h[0] | (h[1] << 1) | (h[2] << 2)));
+  Cons c;
+  if (ch < 0 || !leaf_(t, ch)) {
+    for (int i = 0; i < 4; i++) { c.a[i] = 0; c.b[i] = 0; }
+    c.a[0] = 1; c.b[0] = 1; c.b[1] = 1;
+    return c;
+  }
+  int f[3] = {2 * (l[0] - (BS_ / 2) * h[0]), 2 * (l[1] - (BS_ / 2) * h[1]), 2 * (l[2] - (BS_ / 2) * h[2])};
+  for (int i = 0; i < 4; i++) { c.a[i] = 0; c.b[i] = 0; }
+  for (int k = 0; k < 2; k++)
+    for (int j = 0; j < 2; j++)
+      for (int i = 0; i < 2; i++) {
+        int g[3] = {f[0] + i, f[1] + j, f[2] + k};
+        Cons u = u_leaf(t, u1, u2, ch, g);
+        for (int q = 0; q < 4; q++) { c.a[q] = c.a[q] + RL(1.0) * u.a[q]; c.b[q] = c.b[q] + RL(1.0) * u.b[q]; }
+      }
+  for (int q = 0; q < 4; q++) { c.a[q] = c.a[q] * RL(0.125); c.b[q] = c.b[q] * RL(0.125); }
+  return c;
+}
+/* u_near (src/amr/cons.wgsl:49-55) */
+static int u_near(const orc_amr* t, const real* u1, const real* u2, int slot, const int l[3], Cons* out) {
+  Src s = ghost_src(t, slot, l);
+  if (s.kind != 0) { for (int i = 0; i < 4; i++) { out->a[i] = 0; out->b[i] = 0; } return 0; }
+  Cons u = u_at(t, u1, u2, s.slot, s.l);
+  for (int i = 0; i < 3; i++) u.a[1 + i] = u.a[1 + i] * s.flip[i] + u.a[0] * RL(0.0);
+  *out = u;
+  return 1;
+}
+/* limit4 of the limiter slot (MC for mc and koren: koren.wgsl:294-297) */
+static real limit1(int lim, real a, real b) {
+  if (lim == ORC_LIM_MINMOD) return limit_minmod(a, b);
+  if (lim == ORC_LIM_VANLEER) return limit_vanleer(a, b);
+  return limit_mc(a, b);
+}
+/* mom_slope / slope_between (src/amr/cons.wgsl:65-74) */
+static Cons slope_between(const P_t* P, const Cons* uc, const Cons* m, const Cons* p) {
+  Cons s;
+  for (int i = 0; i < 4; i++) { s.a[i] = limit1(P->limiter, uc->a[i] - m->a[i], p->a[i] - uc->a[i]); s.b[i] = limit1(P->limiter, uc->b[i] - m->b[i], p->b[i] - uc->b[i]); }
+  real vc[3], vm[3], vp[3], sv[3];
+  for (int i = 0; i < 3; i++) {
+    vc[i] = uc->a[1 + i] / rmax(uc->a[0], P->rho_floor);
+    vm[i] = m->a[1 + i] / rmax(m->a[0], P->rho_floor);
+    vp[i] = p->a[1 + i] / rmax(p->a[0], P->rho_floor);
+  }
+  for (int i = 0; i < 3; i++) sv[i] = limit1(P->limiter, vc[i] - vm[i], vp[i] - vc[i]);
+  for (int i = 0; i < 3; i++) s.a[1 + i] = vc[i] * s.a[0] + uc->a[0] * sv[i];
+  return s;
+}
+/* prolong_slopes / prolong_child (src/amr/cons.wgsl:83-124) */
+static Cons prolong(const P_t* P, const Cons v[7], const int ok[7], const int par[3]) {
+  const Cons* uc = &v[0];
+  Cons s[3];
+  for (int a = 0; a < 3; a++) {
+    for (int i = 0; i < 4; i++) { s[a].a[i] = 0; s[a].b[i] = 0; }
+    if (ok[1 + 2 * a] && ok[2 + 2 * a]) s[a] = slope_between(P, uc, &v[1 + 2 * a], &v[2 + 2 * a]);
+  }
+  real drop[3], room[3], fit = RL(1.0);
+  for (int q = 0; q < 3; q++) {
+    int ai = q == 0 ? 0 : -1, bi = q == 1 ? 0 : (q == 2 ? 1 : -1);
+    real d = RL(0.0);
+    for (int a = 0; a < 3; a++) d = d + rabs(ai >= 0 ? s[a].a[ai] : s[a].b[bi]);
+    drop[q] = RL(0.25) * d;
+    room[q] = RL(0.5) * rmax(ai >= 0 ? uc->a[ai] : uc->b[bi], RL(0.0));
+  }
+  for (int q = 0; q < 3; q++) fit = rmin(fit, rmin(RL(1.0), room[q] / rmax(drop[q], RL(1.0e-37))));
+  real off[3];
+  for (int a = 0; a < 3; a++) off[a] = fit * ((real)par[a] - RL(0.5)) * RL(0.5);
+  Cons o;
+  for (int i = 0; i < 4; i++) {
+    o.a[i] = ((uc->a[i] + off[0] * s[0].a[i]) + off[1] * s[1].a[i]) + off[2] * s[2].a[i];
+    o.b[i] = ((uc->b[i] + off[0] * s[0].b[i]) + off[1] * s[1].b[i]) + off[2] * s[2].b[i];
+  }
+  return o;
+}
+/* from_coarse (src/amr/assemble.wgsl:47-75) */
+static Cons from_coarse(const orc_amr* t, const P_t* P, const real* u1, const real* u2, int slot, const int l[3], int cs) {
+  int gf[3], lc[3], par[3];
+  int Lc = bt_(t, cs, 0);
+  for (int a = 0; a < 3; a++) {
+    gf[a] = bt_(t, slot, 1 + a) * BS_ + l[a];
+    int nc = t->root[a] * BS_ * (1 << Lc);
+    int gc = gf[a] >> 1;
+    gc = ((gc % nc) + nc) % nc;
+    lc[a] = gc - bt_(t, cs, 1 + a) * BS_;
+    par[a] = gf[a] & 1;
+  }
+  Cons v[7];
+  int ok[7];
+  for (int k = 0; k < 7; k++) {
+    int at[3] = {lc[0], lc[1], lc[2]};
+    if (k > 0) at[(k - 1) / 2] += (k & 1) == 0 ? 1 : -1;
+    ok[k] = u_near(t, u1, u2, cs, at, &v[k]);
+  }
+  return prolong(P, v, ok, par);
+}
+
+/* assemble (src/amr/assemble.wgsl:77-154) into W (8 reals per texel: rho, vx, vy, vz, p, cs, s, e);
+   returns the largest rate / dx over leaf interiors */
+static real amr_assemble(const orc_amr* t, const P_t* P, double dx0, const real* u1, const real* u2, real* W) {
+  real best = RL(0.0);
+  for (int s = 0; s < t->nblocks; s++) {
+    real dx = (real)(dx0 / (double)(1 << bt_(t, s, 0)));
+    for (int z = -NG_; z < BS_ + NG_; z++)
+      for (int y = -NG_; y < BS_ + NG_; y++)
+        for (int x = -NG_; x < BS_ + NG_; x++) {
+          int l[3] = {x, y, z};
+          Src g = ghost_src(t, s, l);
+          if (g.kind == 2) continue;
+          Cons u = g.kind == 0 ? u_at(t, u1, u2, g.slot, g.l) : from_coarse(t, P, u1, u2, s, l, g.slot);
+          Q w = prim_from_cons(P, u.a, u.b);
+          Mapped m;
+          for (int a = 0; a < 3; a++) { m.flip[a] = g.flip[a]; m.side[a] = g.side[a]; }
+          real v[3];
+          bc_velocity(&m, w.a + 1, v);
+          real* o = W + pidx(s, l) * 8;
+          o[0] = w.a[0]; o[1] = v[0]; o[2] = v[1]; o[3] = v[2];
+          o[4] = w.b[0]; o[5] = w.b[2]; o[6] = w.b[1]; o[7] = RL(0.0);
+          int inside = x >= 0 && y >= 0 && z >= 0 && x < BS_ && y < BS_ && z < BS_;
+          if (inside && leaf_(t, s)) {
+            real signal = rsqrt_(w.b[2] * w.b[2] * RL(1.0) * RL(1.0) + RL(0.0));
+            real rate = rabs(v[0]) + rabs(v[1]) + rabs(v[2]) + (real)P->ndim * signal + RL(0.0);
+            real r = rate / dx;
+            if (r > best) best = r;
+          }
+        }
+  }
+  return best;
+}
+
+static Q load_w(const real* W, int s, const int l[3]) {
+  const real* o = W + pidx(s, l) * 8;
+  Q q;
+  q.a[0] = rabs(o[0]); q.a[1] = o[1]; q.a[2] = o[2]; q.a[3] = o[3];
+  q.b[0] = o[4]; q.b[1] = o[6]; q.b[2] = o[5]; q.b[3] = o[0] < 0 ? RL(1.0) : RL(0.0);
+  return q;
+}
+/* predict_cell (src/amr/predict.wgsl:90-181): mask multiplies each axis's slopes */
+static Q predict_cell(const P_t* P, const real* W, int s, const int l[3], const real mask[3], real dt, real dx) {
+  Q q = load_w(W, s, l), nb[6];
+  for (int a = 0; a < 3; a++) {
+    int m[3] = {l[0], l[1], l[2]}, p[3] = {l[0], l[1], l[2]};
+    m[a] -= 1; p[a] += 1;
+    nb[2 * a] = load_w(W, s, m);
+    nb[2 * a + 1] = load_w(W, s, p);
+  }
+  P_t Pl = *P;
+  Pl.dx = dx;
+  return predictor_masked(&Pl, &q, nb, dt, mask);
+}
+
+/* predict (src/amr/predict.wgsl:183-221) over leaves' interiors and first ghost layer */
+static void amr_predict(const orc_amr* t, const P_t* P, double dx0, const real* W, real dt, real* PA, real* PB) {
+  for (int s = 0; s < t->nblocks; s++) {
+    if (!leaf_(t, s)) continue;
+    for (int z = -1; z < BS_ + 1; z++)
+      for (int y = -1; y < BS_ + 1; y++)
+        for (int x = -1; x < BS_ + 1; x++) {
+          int l[3] = {x, y, z};
+          Src g = ghost_src(t, s, l);
+          if (g.kind == 2) continue;
+          int at_source = g.kind == 0;
+          int slot = at_source ? g.slot : s;
+          int cl[3] = {at_source ? g.l[0] : l[0], at_source ? g.l[1] : l[1], at_source ? g.l[2] : l[2]};
+          real mask[3] = {1, 1, 1};
+          if (!at_source)
+            for (int a = 0; a < 3; a++) if (l[a] == 0 || l[a] == BS_ - 1) mask[a] = 0;
+          real dx = (real)(dx0 / (double)(1 << bt_(t, slot, 0)));
+          Q pr = predict_cell(P, W, slot, cl, mask, dt, dx);
+          if (at_source) {
+            Mapped m;
+            for (int a = 0; a < 3; a++) { m.flip[a] = g.flip[a]; m.side[a] = g.side[a]; }
+            real v[3];
+            bc_velocity(&m, pr.a + 1, v);
+            pr.a[1] = v[0]; pr.a[2] = v[1]; pr.a[3] = v[2];
+          }
+          size_t o = pidx(s, l) * 4;
+          for (int i = 0; i < 4; i++) { PA[o + i] = pr.a[i]; PB[o + i] = pr.b[i]; }
+        }
+  }
+}
+
+/* flux_between on block s (src/hydro/flux.wgsl:254-347 with stored ghosts: amr/update.wgsl:50-58) */
+static Flux amr_flux_between(const P_t* P, const real* W, const real* PA, const real* PB, int s, const int a[3], int axis) {
+  int ap[3] = {a[0], a[1], a[2]}, am[3] = {a[0], a[1], a[2]}, app[3] = {a[0], a[1], a[2]};
+  ap[axis] += 1; am[axis] -= 1; app[axis] += 2;
+  Q wl = load_w(W, s, a), wr = load_w(W, s, ap), wll = load_w(W, s, am), wrr = load_w(W, s, app);
+  Q pl, pr;
+  size_t ol = pidx(s, a) * 4, orr = pidx(s, ap) * 4;
+  for (int i = 0; i < 4; i++) { pl.a[i] = PA[ol + i]; pl.b[i] = PB[ol + i]; pr.a[i] = PA[orr + i]; pr.b[i] = PB[orr + i]; }
+  Q sl = edge(P, &wll, &wl, &wr, 1);
+  Q sr = edge(P, &wl, &wr, &wrr, 0);
+  return face_flux(P, &pl, &sl, &pr, &sr, axis);
+}
+
+/* TwoSum: s + e = a + b exactly */
+static void two_sum(real a, real b, real* s, real* e) {
+  real x = a + b;
+  real bv = x - a;
+  real av = x - bv;
+  *e = (a - av) + (b - bv);
+  *s = x;
+}
+
+/* update (src/amr/update.wgsl:326-560) with match_faces (:207-290) */
+static void amr_update(const orc_amr* t, const P_t* P, double dx0, int comp, const real* u1, const real* u2,
+                       const real* W, const real* PA, const real* PB, real dt, real* o1, real* o2) {
+  for (int s = 0; s < t->nblocks; s++) {
+    if (!leaf_(t, s)) continue;
+    real dx = (real)(dx0 / (double)(1 << bt_(t, s, 0)));
+    for (int z = 0; z < BS_; z++)
+      for (int y = 0; y < BS_; y++)
+        for (int x = 0; x < BS_; x++) {
+          int l[3] = {x, y, z};
+          real fl[3][8], fh[3][8];
+          for (int axis = 0; axis < 3; axis++) {
+            int m[3] = {x, y, z};
+            m[axis] -= 1;
+            Flux a = amr_flux_between(P, W, PA, PB, s, m, axis);
+            Flux b = amr_flux_between(P, W, PA, PB, s, l, axis);
+            flux_pack(&a, fl[axis]);
+            flux_pack(&b, fh[axis]);
+          }
+          /* faces next to a refined neighbour: the mean of the fine fluxes */
+          for (int k = 0; k < 6; k++) {
+            int axis = k / 2, hi = k & 1;
+            if (l[axis] != (hi ? BS_ - 1 : 0)) continue;
+            int np = bt_(t, s, 6 + k);
+            if (np < 0 || leaf_(t, np)) continue;
+            int upper = !hi;
+            int h[3] = {l[0] >= BS_ / 2, l[1] >= BS_ / 2, l[2] >= BS_ / 2};
+            int bits[3] = {h[0], h[1], h[2]};
+            bits[axis] = upper;
+            int ch = bt_(t, np, 12 + (bits[0] | (bits[1] << 1) | (bits[2] << 2)));
+            int a0[3] = {2 * (l[0] - (BS_ / 2) * h[0]), 2 * (l[1] - (BS_ / 2) * h[1]), 2 * (l[2] - (BS_ / 2) * h[2])};
+            a0[axis] = upper ? BS_ - 1 : -1;
+            int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
+            real sum[8] = {0, 0, 0, 0, 0, 0, 0, 0}, err[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int j = 0; j < 2; j++)
+              for (int i = 0; i < 2; i++) {
+                int a[3] = {a0[0], a0[1], a0[2]};
+                a[t1] += i;
+                a[t2] += j;
+                Flux f = amr_flux_between(P, W, PA, PB, ch, a, axis);
+                real fv[8];
+                flux_pack(&f, fv);
+                for (int q = 0; q < 8; q++) {
+                  if (comp) { real e; two_sum(sum[q], fv[q], &sum[q], &e); err[q] = err[q] + e; }
+                  else sum[q] = sum[q] + fv[q];
+                }
+              }
+            real* dst = hi ? fh[axis] : fl[axis];
+            for (int q = 0; q < 8; q++) dst[q] = (sum[q] + err[q]) * RL(0.25);
+          }
+          real net[8];
+          for (int q = 0; q < 8; q++) net[q] = ((fl[0][q] - fh[0][q]) + (fl[1][q] - fh[1][q])) + (fl[2][q] - fh[2][q]);
+          size_t o = iidx(s, l) * 4;
+          P_t Pl = *P;
+          Pl.dx = dx;
+          advance(&Pl, u1 + o, u2 + o, net, PB[pidx(s, l) * 4], dt, o1 + o, o2 + o);
+        }
+  }
+}
+
+void FN(amr_step)(const orc_params* prm, const orc_amr* t, double dx0, int compensated, real* u1, real* u2,
+                  real clk[4]) {
+  P_t P = load_params(prm);
+  size_t np = (size_t)t->nblocks * PD_ * PD_ * PD_;
+  size_t ni = (size_t)t->nblocks * BS_ * BS_ * BS_;
+  real* W = (real*)calloc(np * 8, sizeof(real));
+  real* PA = (real*)calloc(np * 4, sizeof(real));
+  real* PB = (real*)calloc(np * 4, sizeof(real));
+  real* o1 = (real*)malloc(ni * 4 * sizeof(real));
+  real* o2 = (real*)malloc(ni * 4 * sizeof(real));
+  memcpy(o1, u1, ni * 4 * sizeof(real));
+  memcpy(o2, u2, ni * 4 * sizeof(real));
+  real rate = amr_assemble(t, &P, dx0, u1, u2, W);
+  /* the clock with PER_DX (src/hydro/clock.wgsl:160-161): dx taken as 1, rates per unit dx */
+  orc_params pc = *prm;
+  pc.dx = 1.0;
+  FN(clock)(&pc, clk, rate);
+  amr_predict(t, &P, dx0, W, clk[0], PA, PB);
+  amr_update(t, &P, dx0, compensated, u1, u2, W, PA, PB, clk[0], o1, o2);
+  memcpy(u1, o1, ni * 4 * sizeof(real));
+  memcpy(u2, o2, ni * 4 * sizeof(real));
+  free(W); free(PA); free(PB); free(o1); free(o2);
+}
+
+real FN(amr_assemble)(const orc_params* prm, const orc_amr* t, double dx0, const real* u1, const real* u2, real* W) {
+  P_t P = load_params(prm);
+  return amr_assemble(t, &P, dx0, u1, u2, W);
+}
+
+void FN(amr_prolong)(const orc_params* prm, const real* cons56, const int ok[7], const int par[3], real out[8]) {
+  P_t P = load_params(prm);
+  Cons v[7];
+  for (int k = 0; k < 7; k++)
+    for (int i = 0; i < 4; i++) { v[k].a[i] = cons56[8 * k + i]; v[k].b[i] = cons56[8 * k + 4 + i]; }
+  Cons o = prolong(&P, v, ok, par);
   for (int i = 0; i < 4; i++) { out[i] = o.a[i]; out[4 + i] = o.b[i]; }
 }

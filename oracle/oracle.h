@@ -88,8 +88,43 @@ typedef struct {
      q, xm, xp, ym, yp, zm, zp, each 8 reals; out = (a, b) 8 reals */                            \
   void P##predictor(const orc_params* prm, const T* stencil56, T dt, T out[8]);
 
+/* ---- block AMR (src/amr): the block table (amr_tree.c) ---- */
+#define ORC_BT_STRIDE 32   /* ints per block, as src/amr/tree.ts:42-53 */
+#define ORC_BS 16          /* cells per block side */
+#define ORC_PAD 20         /* stored texels per block side: 2 ghost layers each way */
+typedef struct {
+  int nblocks;
+  int root[3];
+  int bc_lo[3], bc_hi[3];
+  int* blk;   /* per block (level, ix, iy, iz) */
+  int* tab;   /* per block ORC_BT_STRIDE ints: [0] level [1..3] coords [4] flags (1 active, 2 leaf)
+                 [5] parent [6..11] face neighbours [12..19] children [20..25] coarser leaves */
+} orc_amr;
+/* 0, or -1 - s where block s breaks 2:1 balance */
+int orc_amr_build(orc_amr* t, const int* blocks, int nblocks, const int root[3], const int bc_lo[3], const int bc_hi[3]);
+void orc_amr_free(orc_amr* t);
+const int* orc_amr_table(const orc_amr* t);
+
+#define ORC_AMR_DECLARE(T, P)                                                                    \
+  /* one AMR step (assemble -> clock -> predict -> update, src/amr/amr_sim.ts:2850-2929):      \
+     u1/u2 hold 16^3 cells of 4 values per block (x fastest), in place; dx0 is the level-0     \
+     cell size; compensated != 0 sums the fine fluxes of a coarse face with TwoSum. */         \
+  void P##amr_step(const orc_params* prm, const orc_amr* t, double dx0, int compensated, T* u1, \
+                   T* u2, T clk[4]);                                                           \
+  /* the assemble pass alone: W (nblocks * 20^3 texels of 8 reals: rho, vx, vy, vz, p, cs, s, 0;  \
+     edges and corners left as given); returns the largest leaf rate / dx */                    \
+  T P##amr_assemble(const orc_params* prm, const orc_amr* t, double dx0, const T* u1, const T* u2, \
+                    T* W);                                                                      \
+  /* prolongation of one coarse cell (src/amr/cons.wgsl:76-124) from its 7-cell stencil of    \
+     conserved states (centre, -x, +x, -y, +y, -z, +z; 8 reals each; ok[k] = 0 drops it),     \
+     to the fine child of parity par[3] */                                                     \
+  void P##amr_prolong(const orc_params* prm, const T* cons56, const int ok[7], const int par[3], \
+                      T out[8]);
+
 ORC_DECLARE(float, orc_f32_)
 ORC_DECLARE(double, orc_f64_)
+ORC_AMR_DECLARE(float, orc_f32_)
+ORC_AMR_DECLARE(double, orc_f64_)
 
 /* ---- RefHydro restatement (float64) ---- */
 typedef struct orc_ref orc_ref;

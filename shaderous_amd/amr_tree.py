@@ -1,0 +1,139 @@
+"""Block trees for the AMR hydro path: 16^3-cell blocks, each refined into 8 children.
+
+A tree is the list of its blocks as (level, ix, iy, iz), parents and leaves, in the reference's
+convention (src/amr/tree.ts:14-53): block coordinates count blocks of that level from the
+domain's lower corner, the root grid has `root` blocks per side, and a block of level L covers
+cells [16 c 2^-L dx0 ...). Trees here are built once on the host (regridding during a run is not
+part of this path) and must keep 2:1 balance across faces (tree.ts neighbor :293-324).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+BS = 16
+
+
+def _face_neighbors(level, c, root, periodic):
+    out = []
+    for axis in range(3):
+        n = root[axis] << level
+        for step in (-1, 1):
+            v = c[axis] + step
+            if v < 0 or v >= n:
+                if not periodic[axis]:
+                    continue
+                v %= n
+            d = list(c)
+            d[axis] = v
+            out.append(tuple(d))
+    return out
+
+
+def build_tree(root, max_level, refine, periodic=(True, True, True)):
+    """Refine root blocks while `refine(level, (ix, iy, iz))` says so, up to `max_level`, then
+    refine further wherever 2:1 face balance requires. Returns an int32 array (nblocks, 4) of
+    (level, ix, iy, iz): every block of the tree, parents first by level."""
+    root = tuple(int(r) for r in root)
+    leaves = {(0, (i, j, k)) for k in range(root[2]) for j in range(root[1]) for i in range(root[0])}
+    refined = set()
+    for level in range(max_level):
+        want = sorted(c for (L, c) in leaves if L == level and refine(level, c))
+        for c in want:
+            _split(leaves, refined, level, c)
+    # 2:1 balance: a leaf whose face neighbour region is covered by leaves two levels finer
+    changed = True
+    while changed:
+        changed = False
+        for (L, c) in sorted(leaves):
+            if L == 0:
+                continue
+            for d in _face_neighbors(L, c, root, periodic):
+                # the region beyond this face at level L must be a block (leaf or parent) of
+                # level L, or covered by a leaf of level L - 1
+                if (L, d) in leaves or (L, d) in refined:
+                    continue
+                pc = (L - 1, (d[0] >> 1, d[1] >> 1, d[2] >> 1))
+                if pc in leaves:
+                    continue
+                # covered by something coarser than L - 1: split that leaf
+                for up in range(2, L + 1):
+                    q = (L - up, (d[0] >> up, d[1] >> up, d[2] >> up))
+                    if q in leaves:
+                        _split(leaves, refined, q[0], q[1])
+                        changed = True
+                        break
+            if changed:
+                break
+    blocks = [(L, *c) for (L, c) in refined] + [(L, *c) for (L, c) in leaves]
+    blocks.sort(key=lambda b: (b[0], b[3], b[2], b[1]))
+    return np.asarray(blocks, dtype=np.int32)
+
+
+def _split(leaves, refined, level, c):
+    if (level, c) not in leaves:
+        return
+    leaves.discard((level, c))
+    refined.add((level, c))
+    for o in range(8):
+        leaves.add((level + 1, (2 * c[0] + (o & 1), 2 * c[1] + ((o >> 1) & 1), 2 * c[2] + ((o >> 2) & 1))))
+
+
+def leaf_mask(blocks):
+    """True for the blocks of `blocks` that have no children."""
+    keys = {tuple(b) for b in blocks.tolist()}
+    out = np.ones(len(blocks), bool)
+    for i, (L, x, y, z) in enumerate(blocks.tolist()):
+        for o in range(8):
+            if (L + 1, 2 * x + (o & 1), 2 * y + ((o >> 1) & 1), 2 * z + ((o >> 2) & 1)) in keys:
+                out[i] = False
+                break
+    return out
+
+
+def cell_centers(block, dx0, box_lo=-0.5):
+    """Centres (x, y, z), each (16, 16, 16) indexed [k, j, i], of a block's cells."""
+    L, bx, by, bz = (int(v) for v in block)
+    dx = dx0 / (1 << L)
+    ax = [(np.arange(BS) + 0.5 + BS * c) * dx + box_lo for c in (bx, by, bz)]
+    z, y, x = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    return x, y, z
+
+
+def blocks_state(blocks, dx0, prim_fn, gamma, dtype=np.float32):
+    """Leaf-interior state of every block (parents get zeros): (nblocks * 4096, 4) u1 and u2, in
+    the block order given, cells x fastest. prim_fn(x, y, z) -> dict of primitives."""
+    from .problems import cons_from_prims
+    n = len(blocks)
+    u1 = np.zeros((n * BS ** 3, 4), dtype)
+    u2 = np.zeros((n * BS ** 3, 4), dtype)
+    leaf = leaf_mask(blocks)
+    for i, b in enumerate(blocks):
+        if not leaf[i]:
+            continue
+        x, y, z = cell_centers(b, dx0)
+        a1, a2 = cons_from_prims(prim_fn(x, y, z), gamma, dtype)
+        u1[i * BS ** 3:(i + 1) * BS ** 3] = a1
+        u2[i * BS ** 3:(i + 1) * BS ** 3] = a2
+    return u1, u2
+
+
+def blast_tree(root=8, max_level=3, radius=0.1, center=(0.0, 0.0, 0.0), threshold=0.3, p_in=100.0, p_out=1.0,
+               buffer=0.5):
+    """The block-AMR blast workload's tree: root^3 blocks on the unit box, refined up to
+    `max_level` wherever the relative jump of the initial state across a block (the pressure's:
+    the initial density is uniform) exceeds `threshold` within `buffer` block widths of it: the
+    blocks that straddle the sphere's surface, grown by half a block on every side. With the
+    defaults this gives 5000 blocks (4439 leaves, 18.2 M leaf cells) for the seeded centre."""
+    dx0 = 1.0 / (root * BS)
+    c = np.asarray(center)
+
+    def refine(level, b):
+        size = BS * dx0 / (1 << level)
+        lo = np.asarray(b) * size - 0.5
+        # nearest and farthest distance of the block (grown by `buffer` blocks) from the centre
+        glo, ghi = lo - buffer * size, lo + (1 + buffer) * size
+        near = np.sqrt(np.sum(np.maximum(0.0, np.maximum(glo - c, c - ghi)) ** 2))
+        far = np.sqrt(np.sum(np.maximum(np.abs(glo - c), np.abs(ghi - c)) ** 2))
+        # the region straddles the sphere's surface: a pressure jump p_in/p_out there
+        return near <= radius <= far and abs(p_in - p_out) / min(p_in, p_out) > threshold
+    return build_tree((root, root, root), max_level, refine), dx0
