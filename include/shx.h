@@ -116,6 +116,58 @@ int shx_kernel_name(shx_sim* sim, char* buf, size_t len);
  * restores) and keep the fastest; *chosen = (tx, ty, zc). */
 int shx_autotune(shx_sim* sim, int reps, int chosen[3]);
 
+/* ---------------- block AMR (AmrSim, src/amr/amr_sim.ts) ----------------
+ * A fixed 2:1-balanced block tree (the reference builds and keeps it in BlockTree,
+ * src/amr/tree.ts:106-824; re-meshing is not part of this path): blocks of 16^3 cells with 2
+ * ghost layers (tree.ts:14-18), given as (level, ix, iy, iz) per slot, parents and leaves,
+ * where block coordinates count blocks of that level from the domain's lower corner.
+ * The shx_config describes the level-0 grid: n = 16 * root blocks per axis, dx = level-0 cell
+ * size, ndim = 3. State layout: per slot 4096 cells (x fastest) of u1 and u2 as above; only
+ * leaves' cells are read and written. */
+typedef struct shx_amr shx_amr;
+
+/* Kernel path of the AMR step. PASSES: the reference's passes (assemble -> clock -> predict ->
+ * update, amr_sim.ts:2862-2872), each launched once over every level from device slot lists;
+ * LEVELS: the same kernels launched level by level (the plain per-level form). Both give the
+ * same bits. */
+enum { SHX_AMR_PATH_PASSES = 1, SHX_AMR_PATH_LEVELS = 2 };
+
+typedef struct {
+  int root[3];         /* root blocks per axis (the BlockTree root grid) */
+  int path;            /* SHX_AMR_PATH_* */
+  int compensated;     /* 1: the four fine fluxes of a coarse face are summed with TwoSum before
+                          their mean; 0: plain FP32 sum, as match_faces (amr/update.wgsl:207-290) */
+} shx_amr_layout;
+
+/* AmrSim.create (amr_sim.ts:1924-1992 with uploadTree :2566-2625): validate the tree (levels,
+ * coordinates, parents, all-or-no children, 2:1 balance across faces), build the block table
+ * and slot lists, allocate state and atlases, capture the step graphs. */
+int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const int* blocks, int nblocks,
+                   shx_amr** out);
+int shx_amr_destroy(shx_amr* sim);
+/* BlockTree.table (tree.ts:642-675): nblocks rows of 32 ints: [0] level, [1..3] coordinates,
+ * [4] flags (1 active, 2 leaf), [5] parent, [6..11] face neighbours (-x, +x, -y, +y, -z, +z: a
+ * same-level slot, -1 where the region is one level coarser, -1 - boundary code at a closed
+ * face), [12..19] children, [20..25] the coarser leaf beyond each -1 face; -1 elsewhere. */
+int shx_amr_table(shx_amr* sim, int* out, size_t len);
+/* AmrSim.setState / readLeaves: ncell = nblocks * 4096. Resets the clock as shx_set_state. */
+int shx_amr_set_state(shx_amr* sim, const float* u1, const float* u2, size_t ncell, double time, int steps);
+int shx_amr_set_state_f64(shx_amr* sim, const double* u1, const double* u2, size_t ncell, double time, int steps);
+int shx_amr_read_state(shx_amr* sim, float* u1, float* u2);
+int shx_amr_read_state_f64(shx_amr* sim, double* u1, double* u2);
+/* AmrSim.step / encodeSteps (amr_sim.ts:2709-2947), single global dt (PER_DX clock). */
+int shx_amr_step(shx_amr* sim, int count);
+int shx_amr_sync(shx_amr* sim);
+int shx_amr_read_clock(shx_amr* sim, double out[11]);
+/* AmrSim.diagnostics: as shx_diagnostics over leaves, sums weighted by cell volume (FP64). */
+int shx_amr_diagnostics(shx_amr* sim, double out[16]);
+/* Time `steps` steps pass by pass with HIP events: ms[0] assemble, [1] clock, [2] predict,
+ * [3] update, [4] whole step. */
+int shx_amr_profile(shx_amr* sim, int steps, double ms[5]);
+/* Counts: out[0] blocks, [1] leaves, [2] leaves with a refined neighbour, [3] levels,
+ * [4] kernel launches per step. */
+int shx_amr_stats(shx_amr* sim, int out[5]);
+
 /* The calling thread's last error message ("" if none). */
 const char* shx_last_error(void);
 /* Library version and the device it sees: "shx <ver> gfx950 <name>" */

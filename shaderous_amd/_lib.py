@@ -14,6 +14,7 @@ SHX_OK, SHX_EINVAL, SHX_ENOMEM, SHX_EDEVICE, SHX_ESTATE = 0, -1, -2, -3, -4
 LIMITERS = {"koren": 0, "mc": 1, "vanleer": 2, "minmod": 3}
 RIEMANN = {"hllc": 0, "hll": 1}
 PATHS = {"fused": 0, "plain": 1}
+AMR_PATHS = {"passes": 1, "levels": 2}
 
 
 class ShxError(RuntimeError):
@@ -33,6 +34,10 @@ class ShxConfig(C.Structure):
         ("p_floor", C.c_double), ("eta_dual", C.c_double), ("shock_thresh", C.c_double), ("t_end", C.c_double),
         ("fluff", C.c_double * 3), ("tile", C.c_int * 3),
     ]
+
+
+class ShxAmrLayout(C.Structure):
+    _fields_ = [("root", C.c_int * 3), ("path", C.c_int), ("compensated", C.c_int)]
 
 
 _lib = None
@@ -70,6 +75,19 @@ def load():
         "shx_profile": (I, [P, I, D]),
         "shx_kernel_name": (I, [P, C.c_char_p, C.c_size_t]),
         "shx_autotune": (I, [P, I, C.POINTER(I)]),
+        "shx_amr_create": (I, [C.POINTER(ShxConfig), C.POINTER(ShxAmrLayout), C.POINTER(I), I, C.POINTER(P)]),
+        "shx_amr_destroy": (I, [P]),
+        "shx_amr_table": (I, [P, C.POINTER(I), C.c_size_t]),
+        "shx_amr_set_state": (I, [P, F, F, C.c_size_t, C.c_double, I]),
+        "shx_amr_set_state_f64": (I, [P, D, D, C.c_size_t, C.c_double, I]),
+        "shx_amr_read_state": (I, [P, F, F]),
+        "shx_amr_read_state_f64": (I, [P, D, D]),
+        "shx_amr_step": (I, [P, I]),
+        "shx_amr_sync": (I, [P]),
+        "shx_amr_read_clock": (I, [P, D]),
+        "shx_amr_diagnostics": (I, [P, D]),
+        "shx_amr_profile": (I, [P, I, D]),
+        "shx_amr_stats": (I, [P, C.POINTER(I)]),
         "shx_last_error": (C.c_char_p, []),
         "shx_version": (C.c_char_p, []),
     }

@@ -1,0 +1,158 @@
+"""The block-AMR simulation, mirroring the reference's `AmrSim` for the hydro path.
+
+The reference's AmrSim (src/amr/amr_sim.ts) runs the same hydro step on 16^3-cell blocks of a
+BlockTree (src/amr/tree.ts): ``create`` (:1924-1992, with uploadTree :2566-2625), ``step`` /
+encodeSteps (:2709-2947), ``readLeaves``, ``diagnostics``, ``destroy``. The tree here is fixed at
+creation (re-meshing is not part of this path) and is given as the list of its blocks, parents and
+leaves, as (level, ix, iy, iz). Every call goes through the C ABI of libshx (include/shx.h,
+shx_amr_*); there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import replace
+
+import numpy as np
+
+from . import _lib
+from ._lib import ShxAmrLayout, check
+from .sim import Diagnostics, SimConfig
+
+BS = 16
+CELLS = BS ** 3
+
+
+class AmrSim:
+    """A block-AMR hydro run on one GPU. `cfg` describes the level-0 grid (n = 16 x root blocks,
+    dx = level-0 cell size); `blocks` is (nblocks, 4) of (level, ix, iy, iz)."""
+
+    def __init__(self, cfg: SimConfig, blocks, root, path: str = "passes", compensated: bool = True):
+        lib = _lib.load()
+        root = tuple(int(r) for r in root)
+        self.cfg = replace(cfg, n=tuple(BS * r for r in root), ndim=3)
+        self.root = root
+        self.blocks = np.ascontiguousarray(blocks, dtype=np.int32).reshape(-1, 4)
+        self.nblocks = len(self.blocks)
+        self.ncell = self.nblocks * CELLS
+        self.dtype = np.float64 if cfg.precision == 64 else np.float32
+        lay = ShxAmrLayout()
+        lay.root[:] = list(root)
+        lay.path = _lib.AMR_PATHS[path]
+        lay.compensated = int(bool(compensated))
+        self._h = C.c_void_p()
+        c = self.cfg.to_c()
+        check(lib.shx_amr_create(C.byref(c), C.byref(lay), self.blocks.ctypes.data_as(C.POINTER(C.c_int)),
+                                 self.nblocks, C.byref(self._h)))
+
+    # -- tree --
+    def table(self) -> np.ndarray:
+        out = np.empty((self.nblocks, 32), np.int32)
+        check(_lib.load().shx_amr_table(self._h, out.ctypes.data_as(C.POINTER(C.c_int)), out.size))
+        return out
+
+    def leaf_mask(self) -> np.ndarray:
+        return (self.table()[:, 4] & 2) != 0
+
+    def stats(self) -> dict:
+        out = (C.c_int * 5)()
+        check(_lib.load().shx_amr_stats(self._h, out))
+        return {"blocks": out[0], "leaves": out[1], "match_leaves": out[2], "levels": out[3],
+                "launches_per_step": out[4]}
+
+    # -- state --
+    def set_state(self, u1, u2, time: float = 0.0, steps: int = 0):
+        u1 = np.ascontiguousarray(u1, self.dtype).reshape(-1)
+        u2 = np.ascontiguousarray(u2, self.dtype).reshape(-1)
+        if u1.size != 4 * self.ncell or u2.size != 4 * self.ncell:
+            raise ValueError(f"state must hold {self.nblocks} blocks x 4096 cells of 4 values")
+        lib = _lib.load()
+        if self.dtype == np.float64:
+            P = C.POINTER(C.c_double)
+            check(lib.shx_amr_set_state_f64(self._h, u1.ctypes.data_as(P), u2.ctypes.data_as(P), self.ncell, time,
+                                            steps))
+        else:
+            P = C.POINTER(C.c_float)
+            check(lib.shx_amr_set_state(self._h, u1.ctypes.data_as(P), u2.ctypes.data_as(P), self.ncell, time, steps))
+
+    def read_state(self):
+        """(u1, u2) as (nblocks * 4096, 4) arrays, block by block; parents' cells are unused."""
+        u1 = np.empty((self.ncell, 4), self.dtype)
+        u2 = np.empty((self.ncell, 4), self.dtype)
+        lib = _lib.load()
+        if self.dtype == np.float64:
+            P = C.POINTER(C.c_double)
+            check(lib.shx_amr_read_state_f64(self._h, u1.ctypes.data_as(P), u2.ctypes.data_as(P)))
+        else:
+            P = C.POINTER(C.c_float)
+            check(lib.shx_amr_read_state(self._h, u1.ctypes.data_as(P), u2.ctypes.data_as(P)))
+        return u1, u2
+
+    def read_leaves(self, box_lo: float = -0.5):
+        """AmrSim.readLeaves: the leaves' cells as a dict of x, y, z (centres), dx, u1, u2."""
+        u1, u2 = self.read_state()
+        leaf = self.leaf_mask()
+        dx0 = self.cfg.dx
+        xs, ys, zs, dxs, sel = [], [], [], [], []
+        ax = np.arange(BS) + 0.5
+        for i in np.flatnonzero(leaf):
+            L, bx, by, bz = (int(v) for v in self.blocks[i])
+            h = dx0 / (1 << L)
+            z, y, x = np.meshgrid((ax + BS * bz) * h + box_lo, (ax + BS * by) * h + box_lo,
+                                  (ax + BS * bx) * h + box_lo, indexing="ij")
+            xs.append(x.ravel()); ys.append(y.ravel()); zs.append(z.ravel())
+            dxs.append(np.full(CELLS, h))
+            sel.append(np.arange(i * CELLS, (i + 1) * CELLS))
+        sel = np.concatenate(sel)
+        return {"x": np.concatenate(xs), "y": np.concatenate(ys), "z": np.concatenate(zs), "dx": np.concatenate(dxs),
+                "u1": u1[sel], "u2": u2[sel], "count": sel.size}
+
+    # -- stepping --
+    def step(self, count: int = 1):
+        check(_lib.load().shx_amr_step(self._h, int(count)))
+
+    def sync(self):
+        check(_lib.load().shx_amr_sync(self._h))
+
+    def clock(self) -> np.ndarray:
+        out = np.zeros(11)
+        check(_lib.load().shx_amr_read_clock(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def run_to(self, t_end: float, max_steps: int = 100000, batch: int = 16) -> int:
+        """Step until the clock holds (dt = 0 at t_end, set through cfg.t_end) or max_steps."""
+        taken = 0
+        while taken < max_steps:
+            n = min(batch, max_steps - taken)
+            self.step(n)
+            taken += n
+            ck = self.clock()
+            if not np.isfinite(ck[0]) or not np.isfinite(ck[1]):
+                raise _lib.ShxError(_lib.SHX_ESTATE, f"the clock went non-finite (t = {ck[1]}, dt = {ck[0]})")
+            if ck[0] == 0:
+                break
+        return taken
+
+    def diagnostics(self) -> Diagnostics:
+        out = np.zeros(16)
+        check(_lib.load().shx_amr_diagnostics(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        ck = self.clock()
+        return Diagnostics(t=ck[1] + ck[0], dt=ck[0], steps=int(ck[3] + (1 if ck[0] > 0 else 0)), mass=out[0],
+                           momentum=tuple(out[1:4]), e_kin=out[4], e_int=out[5], e_tot=out[6], max_mach=out[7],
+                           floored=out[8], scalar=out[9], rho_range=(out[10], out[11]), p_range=(out[12], out[13]),
+                           raw=out)
+
+    def profile(self, steps: int = 5) -> dict:
+        ms = np.zeros(5)
+        check(_lib.load().shx_amr_profile(self._h, steps, ms.ctypes.data_as(C.POINTER(C.c_double))))
+        return {"assemble": ms[0], "clock": ms[1], "predict": ms[2], "update": ms[3], "step": ms[4]}
+
+    def destroy(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.load().shx_amr_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

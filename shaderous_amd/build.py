@@ -1,7 +1,9 @@
-"""Build libshx.so (the HIP kernels and the C-ABI of include/shx.h) in-tree for gfx950.
+"""Build libshx.so (the HIP kernels and the C-ABI of include/shx.h) in-tree for gfx950, and the
+N-API addon shaderous_amd/node/shx.node that binds it for Node.
 
 ``python -m shaderous_amd.build`` compiles every translation unit under shaderous_amd/csrc with
-hipcc (in parallel) and links shaderous_amd/libshx.so. No GPU is needed to build.
+hipcc (in parallel), links shaderous_amd/libshx.so, then compiles the addon with gcc against the
+system's Node headers (skipped, with a message, where they are absent). No GPU is needed to build.
 """
 from __future__ import annotations
 
@@ -14,6 +16,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libshx.so")
+NODE_DIR = os.path.join(HERE, "node")
+ADDON = os.path.join(NODE_DIR, "shx.node")
+NODE_INCLUDE = "/usr/include/node"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 DEVICE_FLAGS = [
@@ -59,7 +64,24 @@ def build(force: bool = False, jobs: int | None = None) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    build_addon()
     return LIB
+
+
+def build_addon() -> str | None:
+    """shx.node: gcc, the system Node headers, linked to libshx.so beside it (rpath $ORIGIN/..)."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        print(f"[build] no {NODE_INCLUDE}/node_api.h: the Node addon is not built", file=sys.stderr)
+        return None
+    src = os.path.join(NODE_DIR, "shx_napi.c")
+    tmp = ADDON + ".tmp"
+    cmd = ["gcc", "-shared", "-fPIC", "-O2", "-Wall", "-Wno-unused-parameter", f"-I{NODE_INCLUDE}",
+           "-DNODE_GYP_MODULE_NAME=shx", src, "-o", tmp, f"-L{HERE}", "-lshx", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"addon build failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, ADDON)
+    return ADDON
 
 
 if __name__ == "__main__":

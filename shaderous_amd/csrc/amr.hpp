@@ -1,0 +1,66 @@
+// Interface between the C-ABI layer and the block-AMR kernels: type-erased launchers, one table
+// per (precision, limiter, Riemann solver, isentropic) combination.
+//
+// Blocks are 16^3 cells with 2 ghost layers (20^3 texels), as the reference's block atlas
+// (src/amr/grid.wgsl:7-9). Device data of a block grid of S slots:
+//   tab      S x 32 ints: the block table (src/amr/tree.ts:42-53)
+//   U1, U2   S x 4096 cells of 4 values, x fastest: the state of leaves (parents unused)
+//   W1, W2   S x 8000 texels: primitives (rho, vx, vy, vz) and (p, c_s, s, 0) of every texel but
+//            the edges and corners, which stay zero (the assembled W atlas)
+//   PA, PB   S x 8000 texels: half-step states of leaves' cells and first ghost layer
+//   lists    slot lists (all blocks, leaves, leaves without / with a refined neighbour), each
+//            also split by level
+#pragma once
+#include "hydro.hpp"
+
+namespace shx {
+
+constexpr int AMR_BS = 16;       // cells per block side
+constexpr int AMR_NG = 2;        // ghost layers
+constexpr int AMR_PAD = 20;      // texels per block side
+constexpr int AMR_BT = 32;       // ints per block-table row
+constexpr int AMR_MAXLEV = 8;
+constexpr int AMR_CELLS = AMR_BS * AMR_BS * AMR_BS;
+constexpr int AMR_TEXELS = AMR_PAD * AMR_PAD * AMR_PAD;
+
+// Block-table row (tree.ts writeRow :652-675)
+enum { BT_LEVEL = 0, BT_X = 1, BT_FLAGS = 4, BT_PARENT = 5, BT_NB = 6, BT_CHILD = 12, BT_COARSE = 20 };
+
+// Cell size per level, in the run's precision (rounded once from the FP64 value).
+template <typename T>
+struct Levels {
+  T dx[AMR_MAXLEV];
+};
+
+struct AmrLaunch {
+  hipStream_t stream;
+  const void* params;   // Params<T>: n = level-0 cells, dx = level-0 cell size
+  const void* levels;   // Levels<T>
+  const int* tab;       // device block table
+  double dx0;
+};
+
+// The fused block kernel's work: per listed leaf, its coarse-fine faces.
+struct AmrOps {
+  void (*assemble)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2, void* w1, void* w2,
+                   unsigned long long* slot);
+  void (*clock)(const AmrLaunch&, void* clk, unsigned long long* slot);
+  void (*predict)(const AmrLaunch&, const int* list, int count, const void* w1, const void* w2, void* pa, void* pb,
+                  const void* clk);
+  void (*update)(const AmrLaunch&, const int* list, int count, bool match, bool compensated, const void* u1,
+                 const void* u2, const void* w1, const void* w2, const void* pa, const void* pb, void* o1, void* o2,
+                 const void* clk);
+  void (*rates)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2,
+                unsigned long long* slot);
+  void (*diag)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2, double* part,
+               double* out);
+};
+
+AmrOps amr_ops_f32(int lim, int rs, int isen);
+AmrOps amr_ops_f64(int lim, int rs, int isen);
+
+inline AmrOps amr_ops(int precision, int lim, int rs, int isen) {
+  return precision == 64 ? amr_ops_f64(lim, rs, isen) : amr_ops_f32(lim, rs, isen);
+}
+
+}  // namespace shx

@@ -1,0 +1,616 @@
+// Block-AMR hydro kernels for gfx950, one kernel per reference pass (src/amr/amr_sim.ts
+// encodeSteps :2850-2929):
+//   k_amr_assemble  amr/assemble.wgsl:77-154: every texel of every listed block, ghosts computed
+//                   on load (same-level copy, boundary image, prolongation from a coarser leaf,
+//                   restriction of finer children) -> W; the largest rate / dx of leaf cells
+//   k_clock         hydro/clock.wgsl with PER_DX (dx taken as 1: the rates are per unit dx)
+//   k_amr_predict   amr/predict.wgsl:183-221: half steps of leaves' cells and first ghost layer
+//   k_amr_update    amr/update.wgsl:326-560 with the face tile (amr/tile.wgsl:59-104) and, for
+//                   leaves with a refined neighbour, match_faces (amr/update.wgsl:207-290): a
+//                   face on a refined neighbour takes the mean of the four fine fluxes,
+//                   optionally summed with TwoSum (error-free FP32 addition) before the mean
+// Each kernel takes a slot list: the host launches them once per level (the plain form) or once
+// over every level (batched); the results are the same bits either way.
+#pragma once
+#include "amr.hpp"
+#include "uniform_impl.hpp"
+
+namespace shx {
+
+__device__ __forceinline__ int bt(const int* __restrict__ tab, int s, int k) { return tab[s * AMR_BT + k]; }
+__device__ __forceinline__ bool is_leaf(const int* __restrict__ tab, int s) { return (bt(tab, s, BT_FLAGS) & 3) == 3; }
+// texel (x, y, z), each in [-2, 18), of a block's 20^3 atlas region; cell (x, y, z) in [0, 16)
+__device__ __forceinline__ int tex(int x, int y, int z) {
+  return ((z + AMR_NG) * AMR_PAD + (y + AMR_NG)) * AMR_PAD + (x + AMR_NG);
+}
+__device__ __forceinline__ int cel(int x, int y, int z) { return (z * AMR_BS + y) * AMR_BS + x; }
+
+// ghost_src (amr/grid.wgsl:184-222): where texel (x, y, z) of block `slot` takes its state.
+// kind 0: cell (x, y, z) of block `slot` (itself, a same-level neighbour or a refined one, or the
+// boundary image in itself, with velocity flips and outflow-only sides); kind 1: interpolated
+// from the coarser leaf `slot`; kind 2: an edge or corner texel, never assembled.
+struct GSrc {
+  int kind, slot, x, y, z;
+  int fx, fy, fz;   // velocity sign per axis
+  int sx, sy, sz;   // outflow-only side per axis
+};
+__device__ __forceinline__ GSrc ghost_src(const int* __restrict__ tab, int slot, int x, int y, int z) {
+  GSrc g{0, slot, x, y, z, 1, 1, 1, 0, 0, 0};
+  const bool ox = x < 0 || x >= AMR_BS, oy = y < 0 || y >= AMR_BS, oz = z < 0 || z >= AMR_BS;
+  const int count = (int)ox + (int)oy + (int)oz;
+  if (count == 0) return g;
+  if (count > 1) { g.kind = 2; return g; }
+  const int axis = ox ? 0 : (oy ? 1 : 2);
+  int li = ox ? x : (oy ? y : z);
+  const int upper = li >= AMR_BS;
+  const int nb = bt(tab, slot, BT_NB + 2 * axis + upper);
+  int flip = 1, side = 0;
+  if (nb >= 0) {
+    g.slot = nb;
+    li += upper ? -AMR_BS : AMR_BS;
+  } else if (nb == -1) {
+    g.kind = 1;
+    g.slot = bt(tab, slot, BT_COARSE + 2 * axis + upper);
+    return g;
+  } else {
+    const int k = -1 - nb;
+    if (k == BC_REFLECT) {
+      li = upper ? 2 * AMR_BS - 1 - li : -1 - li;
+      flip = -1;
+    } else {
+      li = upper ? AMR_BS - 1 : 0;
+    }
+    if (k == BC_OUTFLOW_ONLY) side = upper ? 1 : -1;
+  }
+  if (axis == 0) { g.x = li; g.fx = flip; g.sx = side; }
+  else if (axis == 1) { g.y = li; g.fy = flip; g.sy = side; }
+  else { g.z = li; g.fz = flip; g.sz = side; }
+  return g;
+}
+
+template <typename T>
+struct Cons {
+  T4<T> a, b;
+};
+template <typename T>
+__device__ __forceinline__ T4<T> v4(T x, T y, T z, T w) {
+  T4<T> r;
+  r.x = x; r.y = y; r.z = z; r.w = w;
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ Cons<T> u_leaf(const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, int slot, int x,
+                                          int y, int z) {
+  size_t o = (size_t)slot * AMR_CELLS + cel(x, y, z);
+  return Cons<T>{U1[o], U2[o]};
+}
+
+// u_at (amr/cons.wgsl:19-43): a leaf's cell, or a parent's cell as the mean of its 8 children's
+template <typename T>
+__device__ __forceinline__ Cons<T> u_at(const int* __restrict__ tab, const T4<T>* __restrict__ U1,
+                                        const T4<T>* __restrict__ U2, int slot, int x, int y, int z) {
+  if (is_leaf(tab, slot)) return u_leaf<T>(U1, U2, slot, x, y, z);
+  const int hx = x >= AMR_BS / 2, hy = y >= AMR_BS / 2, hz = z >= AMR_BS / 2;
+  const int ch = bt(tab, slot, BT_CHILD + (hx | (hy << 1) | (hz << 2)));
+  if (ch < 0 || !is_leaf(tab, ch)) return Cons<T>{v4<T>(1, 0, 0, 0), v4<T>(1, 1, 0, 0)};
+  const int fx = 2 * (x - (AMR_BS / 2) * hx), fy = 2 * (y - (AMR_BS / 2) * hy), fz = 2 * (z - (AMR_BS / 2) * hz);
+  Cons<T> c{v4<T>(0, 0, 0, 0), v4<T>(0, 0, 0, 0)};
+#pragma unroll
+  for (int k = 0; k < 2; k++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        Cons<T> u = u_leaf<T>(U1, U2, ch, fx + i, fy + j, fz + k);
+        c.a.x += u.a.x; c.a.y += u.a.y; c.a.z += u.a.z; c.a.w += u.a.w;
+        c.b.x += u.b.x; c.b.y += u.b.y; c.b.z += u.b.z; c.b.w += u.b.w;
+      }
+  const T e = T(0.125);
+  c.a.x *= e; c.a.y *= e; c.a.z *= e; c.a.w *= e;
+  c.b.x *= e; c.b.y *= e; c.b.z *= e; c.b.w *= e;
+  return c;
+}
+
+// u_near (amr/cons.wgsl:49-55): the state at a coarse block's texel, when it is a cell
+template <typename T>
+__device__ __forceinline__ bool u_near(const int* __restrict__ tab, const T4<T>* __restrict__ U1,
+                                       const T4<T>* __restrict__ U2, int cs, int x, int y, int z, Cons<T>& out) {
+  GSrc g = ghost_src(tab, cs, x, y, z);
+  if (g.kind != 0) {
+    out = Cons<T>{v4<T>(0, 0, 0, 0), v4<T>(0, 0, 0, 0)};
+    return false;
+  }
+  Cons<T> u = u_at<T>(tab, U1, U2, g.slot, g.x, g.y, g.z);
+  u.a.y = u.a.y * T(g.fx) + u.a.x * T(0);
+  u.a.z = u.a.z * T(g.fy) + u.a.x * T(0);
+  u.a.w = u.a.w * T(g.fz) + u.a.x * T(0);
+  out = u;
+  return true;
+}
+
+// The limiter of the prolongation slopes (amr/cons.wgsl:65-74: MC for mc and koren)
+template <int LIM>
+constexpr int prolong_lim() { return (LIM == LIM_MINMOD || LIM == LIM_VANLEER) ? LIM : LIM_MC; }
+
+// slope_between with mom_slope (amr/cons.wgsl:57-74): limited slopes of every conserved value,
+// momentum by the product rule from the density and velocity slopes
+template <typename T, int LIM>
+__device__ __forceinline__ Cons<T> slope_between(const Params<T>& P, const Cons<T>& c, const Cons<T>& m,
+                                                 const Cons<T>& p) {
+  constexpr int PL = prolong_lim<LIM>();
+  Cons<T> s;
+  s.a.x = edge1<T, PL>(c.a.x - m.a.x, p.a.x - c.a.x, T(0));
+  s.a.y = edge1<T, PL>(c.a.y - m.a.y, p.a.y - c.a.y, T(0));
+  s.a.z = edge1<T, PL>(c.a.z - m.a.z, p.a.z - c.a.z, T(0));
+  s.a.w = edge1<T, PL>(c.a.w - m.a.w, p.a.w - c.a.w, T(0));
+  s.b.x = edge1<T, PL>(c.b.x - m.b.x, p.b.x - c.b.x, T(0));
+  s.b.y = edge1<T, PL>(c.b.y - m.b.y, p.b.y - c.b.y, T(0));
+  s.b.z = edge1<T, PL>(c.b.z - m.b.z, p.b.z - c.b.z, T(0));
+  s.b.w = edge1<T, PL>(c.b.w - m.b.w, p.b.w - c.b.w, T(0));
+  const T dc = tmax(c.a.x, P.rho_floor), dm = tmax(m.a.x, P.rho_floor), dp = tmax(p.a.x, P.rho_floor);
+  const T vcx = tdiv(c.a.y, dc), vcy = tdiv(c.a.z, dc), vcz = tdiv(c.a.w, dc);
+  const T svx = edge1<T, PL>(vcx - tdiv(m.a.y, dm), tdiv(p.a.y, dp) - vcx, T(0));
+  const T svy = edge1<T, PL>(vcy - tdiv(m.a.z, dm), tdiv(p.a.z, dp) - vcy, T(0));
+  const T svz = edge1<T, PL>(vcz - tdiv(m.a.w, dm), tdiv(p.a.w, dp) - vcz, T(0));
+  s.a.y = vcx * s.a.x + c.a.x * svx;
+  s.a.z = vcy * s.a.x + c.a.x * svy;
+  s.a.w = vcz * s.a.x + c.a.x * svz;
+  return s;
+}
+
+// prolong_slopes / prolong_child (amr/cons.wgsl:83-124): the fine child of parity (px, py, pz)
+// of a coarse cell, from the 7-cell stencil (centre, -x, +x, -y, +y, -z, +z); one common
+// scale-back keeps rho, E and e of every child at least half the coarse cell's
+template <typename T, int LIM>
+__device__ __forceinline__ Cons<T> prolong(const Params<T>& P, const Cons<T> v[7], const bool ok[7], int px, int py,
+                                           int pz) {
+  const Cons<T> z{v4<T>(0, 0, 0, 0), v4<T>(0, 0, 0, 0)};
+  const Cons<T> sx = (ok[1] && ok[2]) ? slope_between<T, LIM>(P, v[0], v[1], v[2]) : z;
+  const Cons<T> sy = (ok[3] && ok[4]) ? slope_between<T, LIM>(P, v[0], v[3], v[4]) : z;
+  const Cons<T> sz = (ok[5] && ok[6]) ? slope_between<T, LIM>(P, v[0], v[5], v[6]) : z;
+  const Cons<T>& c = v[0];
+  T fit = T(1);
+  {
+    T d0 = ((T(0) + tabs(sx.a.x)) + tabs(sy.a.x)) + tabs(sz.a.x);
+    T d1 = ((T(0) + tabs(sx.b.x)) + tabs(sy.b.x)) + tabs(sz.b.x);
+    T d2 = ((T(0) + tabs(sx.b.y)) + tabs(sy.b.y)) + tabs(sz.b.y);
+    T r0 = T(0.5) * tmax(c.a.x, T(0)), r1 = T(0.5) * tmax(c.b.x, T(0)), r2 = T(0.5) * tmax(c.b.y, T(0));
+    fit = tmin(fit, tmin(T(1), tdiv(r0, tmax(T(0.25) * d0, T(1e-37)))));
+    fit = tmin(fit, tmin(T(1), tdiv(r1, tmax(T(0.25) * d1, T(1e-37)))));
+    fit = tmin(fit, tmin(T(1), tdiv(r2, tmax(T(0.25) * d2, T(1e-37)))));
+  }
+  const T ox = fit * (T(px) - T(0.5)) * T(0.5);
+  const T oy = fit * (T(py) - T(0.5)) * T(0.5);
+  const T oz = fit * (T(pz) - T(0.5)) * T(0.5);
+  Cons<T> o;
+#define SHX_PRO(F) o.F = ((c.F + ox * sx.F) + oy * sy.F) + oz * sz.F
+  SHX_PRO(a.x); SHX_PRO(a.y); SHX_PRO(a.z); SHX_PRO(a.w);
+  SHX_PRO(b.x); SHX_PRO(b.y); SHX_PRO(b.z); SHX_PRO(b.w);
+#undef SHX_PRO
+  return o;
+}
+
+// from_coarse (amr/assemble.wgsl:47-75): texel (x, y, z) of block `slot` interpolated from the
+// coarser leaf cs. n0 = level-0 cells per axis.
+template <typename T, int LIM>
+__device__ __forceinline__ Cons<T> from_coarse(const Params<T>& P, const int* __restrict__ tab,
+                                               const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, int slot,
+                                               int x, int y, int z, int cs) {
+  const int lc = bt(tab, cs, BT_LEVEL);
+  int c[3], par[3];
+  const int l[3] = {x, y, z};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const int gf = bt(tab, slot, BT_X + a) * AMR_BS + l[a];
+    const int nc = P.n[a] << lc;
+    int gc = gf >> 1;
+    gc = ((gc % nc) + nc) % nc;
+    c[a] = gc - bt(tab, cs, BT_X + a) * AMR_BS;
+    par[a] = gf & 1;
+  }
+  Cons<T> v[7];
+  bool ok[7];
+  ok[0] = u_near<T>(tab, U1, U2, cs, c[0], c[1], c[2], v[0]);
+  ok[1] = u_near<T>(tab, U1, U2, cs, c[0] - 1, c[1], c[2], v[1]);
+  ok[2] = u_near<T>(tab, U1, U2, cs, c[0] + 1, c[1], c[2], v[2]);
+  ok[3] = u_near<T>(tab, U1, U2, cs, c[0], c[1] - 1, c[2], v[3]);
+  ok[4] = u_near<T>(tab, U1, U2, cs, c[0], c[1] + 1, c[2], v[4]);
+  ok[5] = u_near<T>(tab, U1, U2, cs, c[0], c[1], c[2] - 1, v[5]);
+  ok[6] = u_near<T>(tab, U1, U2, cs, c[0], c[1], c[2] + 1, v[6]);
+  return prolong<T, LIM>(P, v, ok, par[0], par[1], par[2]);
+}
+
+// The assembled primitives of texel (x, y, z) of block `slot`; false for edges and corners.
+template <typename T, int LIM>
+__device__ __forceinline__ bool amr_texel(const Params<T>& P, const int* __restrict__ tab,
+                                          const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, int slot, int x,
+                                          int y, int z, Prim<T>& w) {
+  const GSrc g = ghost_src(tab, slot, x, y, z);
+  if (g.kind == 2) return false;
+  const Cons<T> u = g.kind == 0 ? u_at<T>(tab, U1, U2, g.slot, g.x, g.y, g.z)
+                                : from_coarse<T, LIM>(P, tab, U1, U2, slot, x, y, z, g.slot);
+  w = prim_from_cons(P, u.a, u.b);
+  w.vx = bc_vel(w.vx, g.fx, g.sx);
+  w.vy = bc_vel(w.vy, g.fy, g.sy);
+  w.vz = bc_vel(w.vz, g.fz, g.sz);
+  return true;
+}
+
+// ---------------- assemble ----------------
+constexpr int ASM_CHUNKS = (AMR_TEXELS + 255) / 256;   // 32 workgroups of 256 per block
+
+template <typename T, int LIM>
+__global__ __launch_bounds__(256) void k_amr_assemble(Params<T> P, Levels<T> LV, const int* __restrict__ tab,
+                                                      const int* __restrict__ list, const T4<T>* __restrict__ U1,
+                                                      const T4<T>* __restrict__ U2, T4<T>* __restrict__ W1,
+                                                      T4<T>* __restrict__ W2, unsigned long long* slot) {
+  const int b = blockIdx.x / ASM_CHUNKS;
+  const int t = (blockIdx.x % ASM_CHUNKS) * 256 + threadIdx.x;
+  T r = T(0);
+  if (t < AMR_TEXELS) {
+    const int x = t % AMR_PAD - AMR_NG, y = (t / AMR_PAD) % AMR_PAD - AMR_NG, z = t / (AMR_PAD * AMR_PAD) - AMR_NG;
+    const int s = list[b];
+    Prim<T> w;
+    if (amr_texel<T, LIM>(P, tab, U1, U2, s, x, y, z, w)) {
+      const size_t o = (size_t)s * AMR_TEXELS + t;
+      W1[o] = v4<T>(w.rho, w.vx, w.vy, w.vz);
+      W2[o] = v4<T>(w.p, w.cs, w.s, T(0));
+      const bool inside = x >= 0 && y >= 0 && z >= 0 && x < AMR_BS && y < AMR_BS && z < AMR_BS;
+      if (inside && is_leaf(tab, s)) r = rate_of(P, w) / LV.dx[bt(tab, s, BT_LEVEL)];
+    }
+  }
+  block_max_to(r, slot);
+}
+
+// The rates of the listed leaves' cells alone (the fused form's first step)
+template <typename T>
+__global__ __launch_bounds__(256) void k_amr_rates(Params<T> P, Levels<T> LV, const int* __restrict__ tab,
+                                                   const int* __restrict__ list, const T4<T>* __restrict__ U1,
+                                                   const T4<T>* __restrict__ U2, unsigned long long* slot) {
+  const int s = list[blockIdx.x];
+  const T dx = LV.dx[bt(tab, s, BT_LEVEL)];
+  T r = T(0);
+  for (int c = threadIdx.x; c < AMR_CELLS; c += 256) {
+    const size_t o = (size_t)s * AMR_CELLS + c;
+    r = tmax(r, rate_of(P, prim_from_cons(P, U1[o], U2[o])) / dx);
+  }
+  block_max_to(r, slot);
+}
+
+// ---------------- predict ----------------
+constexpr int PRED_SIDE = AMR_BS + 2;
+constexpr int PRED_TEXELS = PRED_SIDE * PRED_SIDE * PRED_SIDE;
+constexpr int PRED_CHUNKS = (PRED_TEXELS + 255) / 256;   // 23
+
+template <typename T>
+__device__ __forceinline__ Prim<T> load_w(const T4<T>* __restrict__ W1, const T4<T>* __restrict__ W2, size_t o) {
+  const T4<T> a = W1[o], b = W2[o];
+  Prim<T> q;
+  q.rho = tabs(a.x); q.vx = a.y; q.vy = a.z; q.vz = a.w;
+  q.p = b.x; q.cs = b.y; q.s = b.z;
+  return q;
+}
+
+// predict_cell (amr/predict.wgsl:90-181) on the atlas of block `slot` at texel (x, y, z); a zero
+// in mask drops that axis's slopes (ghosts next to a coarser block whose transverse neighbour
+// is an unassembled edge texel)
+template <typename T, int LIM, bool ISEN>
+__device__ __forceinline__ Half<T> predict_texel(const Params<T>& P, const T4<T>* __restrict__ W1,
+                                                 const T4<T>* __restrict__ W2, int slot, int x, int y, int z, T mx,
+                                                 T my, T mz, T dt) {
+  const size_t base = (size_t)slot * AMR_TEXELS;
+  const int c = tex(x, y, z);
+  const Prim<T> q = load_w<T>(W1, W2, base + c);
+  const Prim<T> xm = load_w<T>(W1, W2, base + c - 1), xp = load_w<T>(W1, W2, base + c + 1);
+  const Prim<T> ym = load_w<T>(W1, W2, base + c - AMR_PAD), yp = load_w<T>(W1, W2, base + c + AMR_PAD);
+  const Prim<T> zm = load_w<T>(W1, W2, base + c - AMR_PAD * AMR_PAD), zp = load_w<T>(W1, W2, base + c + AMR_PAD * AMR_PAD);
+  AxisSlope<T> sl[3];
+  sl[0] = axis_slope<T, LIM, ISEN>(P, xm, q, xp, nullptr, nullptr);
+  sl[1] = axis_slope<T, LIM, ISEN>(P, ym, q, yp, nullptr, nullptr);
+  sl[2] = axis_slope<T, LIM, ISEN>(P, zm, q, zp, nullptr, nullptr);
+  const T mk[3] = {mx, my, mz};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    Inc<T>& s = sl[a].s;
+    s.rho *= mk[a]; s.vx *= mk[a]; s.vy *= mk[a]; s.vz *= mk[a]; s.p *= mk[a]; s.s *= mk[a];
+  }
+  const T conv = (xp.vx - xm.vx) + (yp.vy - ym.vy) + (zp.vz - zm.vz);
+  const T jump = tmax(jump_of(xm.p, xp.p), tmax(jump_of(ym.p, yp.p), jump_of(zm.p, zp.p)));
+  return hancock(P, q, sl, conv, jump, dt);
+}
+
+template <typename T, int LIM, bool ISEN>
+__global__ __launch_bounds__(256) void k_amr_predict(Params<T> P, Levels<T> LV, const int* __restrict__ tab,
+                                                     const int* __restrict__ list, const T4<T>* __restrict__ W1,
+                                                     const T4<T>* __restrict__ W2, T4<T>* __restrict__ PA,
+                                                     T4<T>* __restrict__ PB, const T* __restrict__ clk) {
+  const int b = blockIdx.x / PRED_CHUNKS;
+  const int t = (blockIdx.x % PRED_CHUNKS) * 256 + threadIdx.x;
+  if (t >= PRED_TEXELS) return;
+  const int x = t % PRED_SIDE - 1, y = (t / PRED_SIDE) % PRED_SIDE - 1, z = t / (PRED_SIDE * PRED_SIDE) - 1;
+  const int s = list[b];
+  const GSrc g = ghost_src(tab, s, x, y, z);
+  if (g.kind == 2) return;
+  const bool at = g.kind == 0;
+  const int src = at ? g.slot : s;
+  Params<T> Pl = P;
+  Pl.dx = LV.dx[bt(tab, src, BT_LEVEL)];
+  const T mx = (!at && (x == 0 || x == AMR_BS - 1)) ? T(0) : T(1);
+  const T my = (!at && (y == 0 || y == AMR_BS - 1)) ? T(0) : T(1);
+  const T mz = (!at && (z == 0 || z == AMR_BS - 1)) ? T(0) : T(1);
+  Half<T> h = at ? predict_texel<T, LIM, ISEN>(Pl, W1, W2, src, g.x, g.y, g.z, T(1), T(1), T(1), clk[0])
+                 : predict_texel<T, LIM, ISEN>(Pl, W1, W2, src, x, y, z, mx, my, mz, clk[0]);
+  if (at) {
+    h.vx = bc_vel(h.vx, g.fx, g.sx);
+    h.vy = bc_vel(h.vy, g.fy, g.sy);
+    h.vz = bc_vel(h.vz, g.fz, g.sz);
+  }
+  const size_t o = (size_t)s * AMR_TEXELS + tex(x, y, z);
+  PA[o] = v4<T>(h.rho, h.vx, h.vy, h.vz);
+  PB[o] = v4<T>(h.p, h.s, h.shock, h.fo);
+}
+
+// ---------------- update ----------------
+template <typename T>
+__device__ __forceinline__ Half<T> load_half_at(const T4<T>* __restrict__ PA, const T4<T>* __restrict__ PB, size_t o) {
+  const T4<T> a = PA[o], b = PB[o];
+  Half<T> h;
+  h.rho = a.x; h.vx = a.y; h.vy = a.z; h.vz = a.w;
+  h.p = b.x; h.s = b.y; h.shock = b.z; h.fo = b.w;
+  return h;
+}
+
+// flux_between on block s's atlas (hydro/flux.wgsl:254-347 with stored ghosts, amr/update.wgsl:
+// 50-58): the face between texel (x, y, z) and the next one along axis
+template <typename T, int LIM, int RS, bool ISEN>
+__device__ __forceinline__ Flux<T> amr_flux(const Params<T>& P, const T4<T>* __restrict__ W1,
+                                            const T4<T>* __restrict__ W2, const T4<T>* __restrict__ PA,
+                                            const T4<T>* __restrict__ PB, int s, int x, int y, int z, int axis) {
+  const size_t o = (size_t)s * AMR_TEXELS + tex(x, y, z);
+  const int e = axis == 0 ? 1 : (axis == 1 ? AMR_PAD : AMR_PAD * AMR_PAD);
+  const Prim<T> wll = load_w<T>(W1, W2, o - e), wl = load_w<T>(W1, W2, o);
+  const Prim<T> wr = load_w<T>(W1, W2, o + e), wrr = load_w<T>(W1, W2, o + 2 * e);
+  const Half<T> hl = load_half_at<T>(PA, PB, o), hr = load_half_at<T>(PA, PB, o + e);
+  const Inc<T> il = edge<T, LIM, ISEN, true>(P, wll, wl, wr);
+  const Inc<T> ir = edge<T, LIM, ISEN, false>(P, wl, wr, wrr);
+  return face_flux<T, RS>(P, hl, il, hr, ir, axis);
+}
+
+// TwoSum: s + e = a + b exactly (Knuth)
+template <typename T>
+__device__ __forceinline__ void two_sum(T a, T b, T& s, T& e) {
+  const T x = a + b;
+  const T bv = x - a;
+  const T av = x - bv;
+  e = (a - av) + (b - bv);
+  s = x;
+}
+
+// The face of cell (x, y, z) of block s on side k (axis k / 2, upper if k odd), whose neighbour
+// across it is refined: the mean of the four fine fluxes through it (match_faces,
+// amr/update.wgsl:207-290), summed in a fixed order, with TwoSum when COMP.
+template <typename T, int LIM, int RS, bool ISEN, bool COMP>
+__device__ __forceinline__ Flux<T> fine_face(const Params<T>& P, const int* __restrict__ tab,
+                                             const T4<T>* __restrict__ W1, const T4<T>* __restrict__ W2,
+                                             const T4<T>* __restrict__ PA, const T4<T>* __restrict__ PB, int np, int k,
+                                             int x, int y, int z) {
+  const int axis = k >> 1, hi = k & 1, upper = !hi;
+  const int hx = x >= AMR_BS / 2, hy = y >= AMR_BS / 2, hz = z >= AMR_BS / 2;
+  const int bx = axis == 0 ? upper : hx, by = axis == 1 ? upper : hy, bz = axis == 2 ? upper : hz;
+  const int ch = bt(tab, np, BT_CHILD + (bx | (by << 1) | (bz << 2)));
+  int ax = 2 * (x - (AMR_BS / 2) * hx), ay = 2 * (y - (AMR_BS / 2) * hy), az = 2 * (z - (AMR_BS / 2) * hz);
+  const int an = upper ? AMR_BS - 1 : -1;
+  if (axis == 0) ax = an; else if (axis == 1) ay = an; else az = an;
+  const int t1 = axis == 2 ? 0 : axis + 1, t2 = axis == 0 ? 2 : axis - 1;   // (axis + 1) % 3, (axis + 2) % 3
+  T sum[8], err[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) { sum[q] = T(0); err[q] = T(0); }
+#pragma unroll
+  for (int j = 0; j < 2; j++)
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const int fx = ax + (t1 == 0 ? i : 0) + (t2 == 0 ? j : 0);
+      const int fy = ay + (t1 == 1 ? i : 0) + (t2 == 1 ? j : 0);
+      const int fz = az + (t1 == 2 ? i : 0) + (t2 == 2 ? j : 0);
+      const Flux<T> f = amr_flux<T, LIM, RS, ISEN>(P, W1, W2, PA, PB, ch, fx, fy, fz, axis);
+      const T fv[8] = {f.m, f.mx, f.my, f.mz, f.e, f.ei, f.s, f.vn};
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        if constexpr (COMP) {
+          T e;
+          two_sum(sum[q], fv[q], sum[q], e);
+          err[q] = err[q] + e;
+        } else {
+          sum[q] = sum[q] + fv[q];
+        }
+      }
+    }
+  Flux<T> F;
+  F.m = (sum[0] + err[0]) * T(0.25); F.mx = (sum[1] + err[1]) * T(0.25);
+  F.my = (sum[2] + err[2]) * T(0.25); F.mz = (sum[3] + err[3]) * T(0.25);
+  F.e = (sum[4] + err[4]) * T(0.25); F.ei = (sum[5] + err[5]) * T(0.25);
+  F.s = (sum[6] + err[6]) * T(0.25); F.vn = (sum[7] + err[7]) * T(0.25);
+  return F;
+}
+
+// The block update: workgroup 8 x 8 x 4 cells (a quarter-plane slab; 16 per block), every face
+// of its box solved once into LDS (896 faces, 28 KB), then each cell's update.
+constexpr int UPD_WX = 8, UPD_WY = 8, UPD_WZ = 4, UPD_TILES = 16;
+
+template <typename T, int LIM, int RS, bool ISEN, bool MATCH, bool COMP>
+__global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
+    Params<T> P, Levels<T> LV, const int* __restrict__ tab, const int* __restrict__ list,
+    const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, const T4<T>* __restrict__ W1,
+    const T4<T>* __restrict__ W2, const T4<T>* __restrict__ PA, const T4<T>* __restrict__ PB, T4<T>* __restrict__ O1,
+    T4<T>* __restrict__ O2, const T* __restrict__ clk) {
+  constexpr int WX = UPD_WX, WY = UPD_WY, WZ = UPD_WZ;
+  constexpr int NFX = (WX + 1) * WY * WZ, NFY = WX * (WY + 1) * WZ, NFZ = WX * WY * (WZ + 1);
+  constexpr int NF = NFX + NFY + NFZ, NT = WX * WY * WZ;
+  __shared__ T tf[8 * NF];
+  const int s = list[blockIdx.x / UPD_TILES];
+  const int tile = blockIdx.x % UPD_TILES;
+  const int x0 = (tile & 1) * WX, y0 = ((tile >> 1) & 1) * WY, z0 = (tile >> 2) * WZ;
+  const int li = threadIdx.x;
+  for (int f = li; f < NF; f += NT) {
+    int axis, fx, fy, fz;
+    if (f < NFX) {
+      axis = 0; fx = f % (WX + 1); fy = (f / (WX + 1)) % WY; fz = f / ((WX + 1) * WY);
+    } else if (f < NFX + NFY) {
+      const int g = f - NFX;
+      axis = 1; fx = g % WX; fy = (g / WX) % (WY + 1); fz = g / (WX * (WY + 1));
+    } else {
+      const int g = f - NFX - NFY;
+      axis = 2; fx = g % WX; fy = (g / WX) % WY; fz = g / (WX * WY);
+    }
+    const Flux<T> F = amr_flux<T, LIM, RS, ISEN>(P, W1, W2, PA, PB, s, x0 + fx - (axis == 0), y0 + fy - (axis == 1),
+                                                 z0 + fz - (axis == 2), axis);
+    put_flux(tf, NF, f, F);
+  }
+  __syncthreads();
+  const int tx = li % WX, ty = (li / WX) % WY, tz = li / (WX * WY);
+  const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
+  Flux<T> xl = get_flux(tf, NF, tx + (WX + 1) * (ty + WY * tz));
+  Flux<T> xh = get_flux(tf, NF, tx + 1 + (WX + 1) * (ty + WY * tz));
+  Flux<T> yl = get_flux(tf, NF, NFX + tx + WX * (ty + (WY + 1) * tz));
+  Flux<T> yh = get_flux(tf, NF, NFX + tx + WX * (ty + 1 + (WY + 1) * tz));
+  Flux<T> zl = get_flux(tf, NF, NFX + NFY + tx + WX * (ty + WY * tz));
+  Flux<T> zh = get_flux(tf, NF, NFX + NFY + tx + WX * (ty + WY * (tz + 1)));
+  if constexpr (MATCH) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const int axis = k >> 1, hi = k & 1;
+      const int l = axis == 0 ? x : (axis == 1 ? y : z);
+      if (l != (hi ? AMR_BS - 1 : 0)) continue;
+      const int np = bt(tab, s, BT_NB + k);
+      if (np < 0 || is_leaf(tab, np)) continue;
+      const Flux<T> F = fine_face<T, LIM, RS, ISEN, COMP>(P, tab, W1, W2, PA, PB, np, k, x, y, z);
+      switch (k) {
+        case 0: xl = F; break;
+        case 1: xh = F; break;
+        case 2: yl = F; break;
+        case 3: yh = F; break;
+        case 4: zl = F; break;
+        default: zh = F; break;
+      }
+    }
+  }
+  const Flux<T> net = net_of(xl, xh, yl, yh, zl, zh);
+  const T dx = LV.dx[bt(tab, s, BT_LEVEL)];
+  Params<T> Pl = P;
+  Pl.dx = dx;
+  const size_t o = (size_t)s * AMR_CELLS + cel(x, y, z);
+  T4<T> o1, o2;
+  advance(Pl, U1[o], U2[o], net, PB[(size_t)s * AMR_TEXELS + tex(x, y, z)].x, clk[0] / dx, o1, o2);
+  O1[o] = o1;
+  O2[o] = o2;
+}
+
+// ---------------- diagnostics (FP64, volume-weighted over leaves) ----------------
+// Channels as the uniform grid's (mass, momentum, kinetic, internal, E, max Mach, floored cells,
+// rho s, rho min/max, p min/max); one workgroup per listed leaf, parts summed in list order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_amr_diag(Params<T> P, const int* __restrict__ tab,
+                                                  const int* __restrict__ list, double dx0,
+                                                  const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2,
+                                                  double* part) {
+  __shared__ double sh[16][256];
+  const int s = list[blockIdx.x];
+  const double h = dx0 / (double)(1 << bt(tab, s, BT_LEVEL));
+  const double vol = h * h * h;
+  double v[16];
+  for (int c = 0; c < 16; c++) v[c] = 0.0;
+  v[10] = 1e300; v[11] = -1e300; v[12] = 1e300; v[13] = -1e300;
+  for (int c = threadIdx.x; c < AMR_CELLS; c += 256) {
+    const size_t o = (size_t)s * AMR_CELLS + c;
+    T4<T> u1 = U1[o], u2 = U2[o];
+    double rho = (double)u1.x, mx = (double)u1.y, my = (double)u1.z, mz = (double)u1.w, E = (double)u2.x;
+    double ke = 0.5 * (mx * mx + my * my + mz * mz) / rho;
+    Prim<T> w = prim_from_cons(P, u1, u2);
+    double vv = sqrt((double)w.vx * w.vx + (double)w.vy * w.vy + (double)w.vz * w.vz);
+    v[0] += rho; v[1] += mx; v[2] += my; v[3] += mz; v[4] += ke; v[5] += E - ke; v[6] += E;
+    v[7] = fmax(v[7], vv / (double)w.cs);
+    v[8] += (P.gm1 * u2.y <= P.p_floor) ? 1.0 : 0.0;
+    v[9] += (double)u2.z;
+    v[10] = fmin(v[10], (double)w.rho); v[11] = fmax(v[11], (double)w.rho);
+    v[12] = fmin(v[12], (double)w.p); v[13] = fmax(v[13], (double)w.p);
+  }
+  for (int c = 0; c < 16; c++) sh[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      for (int c = 0; c < 16; c++) {
+        double a = sh[c][threadIdx.x], b = sh[c][threadIdx.x + st];
+        sh[c][threadIdx.x] = (c == 7 || c == 11 || c == 13) ? fmax(a, b) : (c == 10 || c == 12) ? fmin(a, b) : a + b;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 16) {
+    const int c = threadIdx.x;
+    const bool summed = c <= 6 || c == 9;
+    part[blockIdx.x * 16 + c] = summed ? sh[c][0] * vol : sh[c][0];
+  }
+}
+
+// ---------------- launchers ----------------
+template <typename T, int LIM, int RS, bool ISEN>
+struct AmrImpl {
+  static const Params<T>& P(const AmrLaunch& L) { return *static_cast<const Params<T>*>(L.params); }
+  static const Levels<T>& LV(const AmrLaunch& L) { return *static_cast<const Levels<T>*>(L.levels); }
+
+  static void assemble(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2, void* w1,
+                       void* w2, unsigned long long* slot) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL((k_amr_assemble<T, LIM>), dim3(count * ASM_CHUNKS), dim3(256), 0, L.stream, P(L), LV(L), L.tab,
+                       list, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)w1, (T4<T>*)w2, slot);
+  }
+  static void clock(const AmrLaunch& L, void* clk, unsigned long long* slot) {
+    Params<T> P1 = P(L);
+    P1.dx = T(1);   // PER_DX: the reduced rates are already per unit length
+    hipLaunchKernelGGL(k_clock<T>, dim3(1), dim3(1), 0, L.stream, P1, (T*)clk, slot);
+  }
+  static void predict(const AmrLaunch& L, const int* list, int count, const void* w1, const void* w2, void* pa,
+                      void* pb, const void* clk) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL((k_amr_predict<T, LIM, ISEN>), dim3(count * PRED_CHUNKS), dim3(256), 0, L.stream, P(L), LV(L),
+                       L.tab, list, (const T4<T>*)w1, (const T4<T>*)w2, (T4<T>*)pa, (T4<T>*)pb, (const T*)clk);
+  }
+  static void update(const AmrLaunch& L, const int* list, int count, bool match, bool comp, const void* u1,
+                     const void* u2, const void* w1, const void* w2, const void* pa, const void* pb, void* o1, void* o2,
+                     const void* clk) {
+    if (count <= 0) return;
+#define SHX_UPD(M, C)                                                                                                  \
+  hipLaunchKernelGGL((k_amr_update<T, LIM, RS, ISEN, M, C>), dim3(count * UPD_TILES), dim3(UPD_WX * UPD_WY * UPD_WZ), \
+                     0, L.stream, P(L), LV(L), L.tab, list, (const T4<T>*)u1, (const T4<T>*)u2, (const T4<T>*)w1,      \
+                     (const T4<T>*)w2, (const T4<T>*)pa, (const T4<T>*)pb, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk)
+    if (!match) SHX_UPD(false, false);
+    else if (comp) SHX_UPD(true, true);
+    else SHX_UPD(true, false);
+#undef SHX_UPD
+  }
+  static void rates(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2,
+                    unsigned long long* slot) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_amr_rates<T>, dim3(count), dim3(256), 0, L.stream, P(L), LV(L), L.tab, list,
+                       (const T4<T>*)u1, (const T4<T>*)u2, slot);
+  }
+  static void diag(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2, double* part,
+                   double* out) {
+    hipLaunchKernelGGL(k_amr_diag<T>, dim3(count), dim3(256), 0, L.stream, P(L), L.tab, list, L.dx0,
+                       (const T4<T>*)u1, (const T4<T>*)u2, part);
+    hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(64), 0, L.stream, (const double*)part, count, out);
+  }
+  static AmrOps ops() {
+    AmrOps o;
+    o.assemble = assemble;
+    o.clock = clock;
+    o.predict = predict;
+    o.update = update;
+    o.rates = rates;
+    o.diag = diag;
+    return o;
+  }
+};
+
+}  // namespace shx

@@ -29,12 +29,6 @@ int fail(int code, const char* fmt, ...) {
 }
 }  // namespace shx
 
-#define HIPCHK(x)                                                                         \
-  do {                                                                                    \
-    hipError_t e_ = (x);                                                                  \
-    if (e_ != hipSuccess) return fail(SHX_EDEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
-  } while (0)
-
 struct shx_sim {
   shx_config cfg;
   int es;                    // element size: 4 or 8
@@ -60,34 +54,6 @@ struct shx_sim {
 };
 
 namespace {
-
-template <typename T>
-Params<T> make_params(const shx_config& c) {
-  Params<T> P;
-  for (int a = 0; a < 3; a++) { P.n[a] = c.n[a]; P.bc_lo[a] = c.bc_lo[a]; P.bc_hi[a] = c.bc_hi[a]; }
-  P.flat = c.n[2] == 1;
-  P.ndim = T(c.ndim);
-  P.gamma = T(c.gamma);
-  P.gm1 = P.gamma - T(1);
-  P.igm1 = T(1) / P.gm1;
-  P.dx = T(c.dx);
-  P.cfl = T(c.cfl);
-  P.rho_floor = T(c.rho_floor);
-  P.p_floor = T(c.p_floor);
-  P.eta_dual = T(c.eta_dual);
-  P.shock_thresh = T(c.shock_thresh);
-  double te = c.t_end;
-  if (!(te < 1e30)) te = 1e30;
-  P.t_end = T(te);
-  P.dt_max = T(1e30);
-  for (int a = 0; a < 3; a++) P.fluff[a] = T(c.fluff[a]);
-  if (!(c.fluff[0] > 0)) { P.fluff[0] = T(0); P.fluff[1] = T(0); P.fluff[2] = T(0); }
-  else {
-    if (!(c.fluff[1] > 0)) P.fluff[1] = T(1e30);
-    if (!(c.fluff[2] > 0)) P.fluff[2] = T(1e30);
-  }
-  return P;
-}
 
 Launch launch_of(shx_sim* s) {
   Launch L;

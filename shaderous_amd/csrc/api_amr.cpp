@@ -1,0 +1,473 @@
+// C-ABI of libshx for the block grid (include/shx.h, shx_amr_*): tree validation, the block
+// table and slot lists (built on the host once, as the reference's BlockTree and uploadTree do),
+// device state and atlases, and the captured step graphs.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "amr.hpp"
+#include "common.hpp"
+
+using namespace shx;
+
+namespace {
+
+struct Range {
+  int off = 0, count = 0;
+};
+
+}  // namespace
+
+struct shx_amr {
+  shx_config cfg;
+  shx_amr_layout lay;
+  int es = 4;
+  int nblocks = 0;
+  int nlevels = 0;
+  size_t ncell = 0;
+  std::vector<int> blocks;   // nblocks x 4
+  std::vector<int> tab;      // nblocks x AMR_BT
+  hipStream_t stream = nullptr;
+  int* d_tab = nullptr;
+  int* d_lists = nullptr;
+  // ranges into d_lists: every block, leaves, leaves without / with a refined neighbour; the
+  // same split per level
+  Range all, leaf, plain, match;
+  Range all_lv[AMR_MAXLEV], leaf_lv[AMR_MAXLEV], plain_lv[AMR_MAXLEV], match_lv[AMR_MAXLEV];
+  void* U1[2] = {nullptr, nullptr};
+  void* U2[2] = {nullptr, nullptr};
+  void *W1 = nullptr, *W2 = nullptr, *PA = nullptr, *PB = nullptr;
+  void* clk = nullptr;
+  unsigned long long* slot = nullptr;
+  double* diag_part = nullptr;
+  double* diag_out = nullptr;
+  int cur = 0;
+  Params<float> pf;
+  Params<double> pd;
+  Levels<float> lf;
+  Levels<double> ld;
+  AmrOps ops{};
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int launches = 0;
+};
+
+namespace {
+
+uint64_t key_of(int L, int x, int y, int z) {
+  return ((uint64_t)L << 60) | ((uint64_t)x << 40) | ((uint64_t)y << 20) | (uint64_t)z;
+}
+
+// Morton code of a block's lower corner at the finest level, for list order (neighbours close
+// in time share cache lines of each other's rims)
+uint64_t morton_of(const int* b, int maxlev) {
+  uint64_t c[3];
+  for (int a = 0; a < 3; a++) c[a] = (uint64_t)b[1 + a] << (maxlev - b[0]);
+  uint64_t m = 0;
+  for (int bit = 0; bit < 20; bit++)
+    for (int a = 0; a < 3; a++) m |= ((c[a] >> bit) & 1ull) << (3 * bit + a);
+  return m;
+}
+
+// The block table (tree.ts neighbor :293-324, writeRow :652-675); returns SHX_OK or SHX_EINVAL.
+int build_table(shx_amr* s) {
+  const int n = s->nblocks;
+  const int* root = s->lay.root;
+  std::unordered_map<uint64_t, int> at;
+  at.reserve(2 * n);
+  for (int i = 0; i < n; i++) {
+    const int* b = &s->blocks[4 * i];
+    if (b[0] < 0 || b[0] >= AMR_MAXLEV) return fail(SHX_EINVAL, "block %d: level %d outside [0, %d)", i, b[0], AMR_MAXLEV);
+    for (int a = 0; a < 3; a++)
+      if (b[1 + a] < 0 || b[1 + a] >= (root[a] << b[0])) return fail(SHX_EINVAL, "block %d: coordinate %d out of range", i, a);
+    if (!at.emplace(key_of(b[0], b[1], b[2], b[3]), i).second) return fail(SHX_EINVAL, "block %d repeats an earlier block", i);
+  }
+  auto find = [&](int L, int x, int y, int z) {
+    auto it = at.find(key_of(L, x, y, z));
+    return it == at.end() ? -1 : it->second;
+  };
+  s->tab.assign((size_t)n * AMR_BT, -1);
+  int maxlev = 0;
+  for (int i = 0; i < n; i++) {
+    const int* b = &s->blocks[4 * i];
+    int* r = &s->tab[(size_t)i * AMR_BT];
+    const int L = b[0];
+    maxlev = std::max(maxlev, L);
+    r[BT_LEVEL] = L; r[1] = b[1]; r[2] = b[2]; r[3] = b[3];
+    r[BT_PARENT] = L > 0 ? find(L - 1, b[1] >> 1, b[2] >> 1, b[3] >> 1) : -1;
+    if (L > 0 && r[BT_PARENT] < 0) return fail(SHX_EINVAL, "block %d (level %d) has no parent", i, L);
+    int nch = 0;
+    for (int o = 0; o < 8; o++) {
+      int ch = find(L + 1, 2 * b[1] + (o & 1), 2 * b[2] + ((o >> 1) & 1), 2 * b[3] + ((o >> 2) & 1));
+      r[BT_CHILD + o] = ch;
+      nch += ch >= 0;
+    }
+    if (nch != 0 && nch != 8) return fail(SHX_EINVAL, "block %d has %d of its 8 children", i, nch);
+    r[BT_FLAGS] = 1 | (nch == 0 ? 2 : 0);
+  }
+  int nroot = 0;
+  for (int i = 0; i < n; i++) nroot += s->blocks[4 * i] == 0;
+  if (nroot != root[0] * root[1] * root[2]) return fail(SHX_EINVAL, "%d root blocks listed, the root grid has %d", nroot, root[0] * root[1] * root[2]);
+  for (int i = 0; i < n; i++) {
+    const int* b = &s->blocks[4 * i];
+    int* r = &s->tab[(size_t)i * AMR_BT];
+    const int L = b[0];
+    for (int f = 0; f < 6; f++) {
+      const int axis = f >> 1, up = f & 1;
+      int c[3] = {b[1], b[2], b[3]};
+      const int nn = root[axis] << L;
+      int v = c[axis] + (up ? 1 : -1);
+      if (v < 0 || v >= nn) {
+        const int type = up ? s->cfg.bc_hi[axis] : s->cfg.bc_lo[axis];
+        if (type != SHX_BC_PERIODIC) { r[BT_NB + f] = -1 - type; continue; }
+        v = (v + nn) % nn;
+      }
+      c[axis] = v;
+      const int same = find(L, c[0], c[1], c[2]);
+      if (same >= 0) { r[BT_NB + f] = same; continue; }
+      const int coarse = L > 0 ? find(L - 1, c[0] >> 1, c[1] >> 1, c[2] >> 1) : -1;
+      // the face region is covered at this level or one coarser (2:1 balance); with children
+      // all-or-none, a block found one level coarser is a leaf
+      if (coarse < 0) return fail(SHX_EINVAL, "block %d breaks 2:1 balance across face %d", i, f);
+      r[BT_NB + f] = -1;
+      r[BT_COARSE + f] = coarse;
+    }
+  }
+  s->nlevels = maxlev + 1;
+  return SHX_OK;
+}
+
+int build_lists(shx_amr* s, std::vector<int>& lists) {
+  const int n = s->nblocks;
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  std::vector<uint64_t> mort(n);
+  for (int i = 0; i < n; i++) mort[i] = morton_of(&s->blocks[4 * i], s->nlevels - 1);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    int la = s->blocks[4 * a], lb = s->blocks[4 * b];
+    return la != lb ? la > lb : mort[a] < mort[b];   // finest level first, Morton order within
+  });
+  auto is_leaf = [&](int i) { return (s->tab[(size_t)i * AMR_BT + BT_FLAGS] & 2) != 0; };
+  auto has_fine = [&](int i) {
+    for (int f = 0; f < 6; f++) {
+      int nb = s->tab[(size_t)i * AMR_BT + BT_NB + f];
+      if (nb >= 0 && !is_leaf(nb)) return true;
+    }
+    return false;
+  };
+  auto add = [&](Range& r, auto pred) {
+    r.off = (int)lists.size();
+    for (int i : order)
+      if (pred(i)) lists.push_back(i);
+    r.count = (int)lists.size() - r.off;
+  };
+  add(s->all, [&](int) { return true; });
+  add(s->leaf, [&](int i) { return is_leaf(i); });
+  add(s->plain, [&](int i) { return is_leaf(i) && !has_fine(i); });
+  add(s->match, [&](int i) { return is_leaf(i) && has_fine(i); });
+  for (int L = 0; L < s->nlevels; L++) {
+    auto lv = [&](int i) { return s->blocks[4 * i] == L; };
+    add(s->all_lv[L], lv);
+    add(s->leaf_lv[L], [&](int i) { return lv(i) && is_leaf(i); });
+    add(s->plain_lv[L], [&](int i) { return lv(i) && is_leaf(i) && !has_fine(i); });
+    add(s->match_lv[L], [&](int i) { return lv(i) && is_leaf(i) && has_fine(i); });
+  }
+  return SHX_OK;
+}
+
+AmrLaunch launch_of(shx_amr* s) {
+  AmrLaunch L;
+  L.stream = s->stream;
+  L.params = s->es == 8 ? (const void*)&s->pd : (const void*)&s->pf;
+  L.levels = s->es == 8 ? (const void*)&s->ld : (const void*)&s->lf;
+  L.tab = s->d_tab;
+  L.dx0 = s->cfg.dx;
+  return L;
+}
+
+const int* lp(shx_amr* s, const Range& r) { return s->d_lists + r.off; }
+
+// Pass groups of one step from parity p into 1 - p; `stage` selects one group (-1: all):
+// 0 assemble, 1 clock, 2 predict, 3 update. Returns the launches enqueued.
+int enqueue_step(shx_amr* s, int p, int stage = -1) {
+  const AmrLaunch L = launch_of(s);
+  const AmrOps& o = s->ops;
+  const bool levels = s->lay.path == SHX_AMR_PATH_LEVELS;
+  const bool comp = s->lay.compensated != 0;
+  int n = 0;
+  auto run = [&](int g) { return stage < 0 || stage == g; };
+  if (run(0)) {
+    if (levels) {
+      for (int l = 0; l < s->nlevels; l++)
+        if (s->all_lv[l].count) { o.assemble(L, lp(s, s->all_lv[l]), s->all_lv[l].count, s->U1[p], s->U2[p], s->W1, s->W2, s->slot); n++; }
+    } else {
+      o.assemble(L, lp(s, s->all), s->all.count, s->U1[p], s->U2[p], s->W1, s->W2, s->slot);
+      n++;
+    }
+  }
+  if (run(1)) { o.clock(L, s->clk, s->slot); n++; }
+  if (run(2)) {
+    if (levels) {
+      for (int l = 0; l < s->nlevels; l++)
+        if (s->leaf_lv[l].count) { o.predict(L, lp(s, s->leaf_lv[l]), s->leaf_lv[l].count, s->W1, s->W2, s->PA, s->PB, s->clk); n++; }
+    } else {
+      o.predict(L, lp(s, s->leaf), s->leaf.count, s->W1, s->W2, s->PA, s->PB, s->clk);
+      n++;
+    }
+  }
+  if (run(3)) {
+    auto upd = [&](const Range& r, bool match) {
+      if (!r.count) return;
+      o.update(L, lp(s, r), r.count, match, comp, s->U1[p], s->U2[p], s->W1, s->W2, s->PA, s->PB, s->U1[1 - p],
+               s->U2[1 - p], s->clk);
+      n++;
+    };
+    if (levels) {
+      for (int l = 0; l < s->nlevels; l++) { upd(s->plain_lv[l], false); upd(s->match_lv[l], true); }
+    } else {
+      upd(s->plain, false);
+      upd(s->match, true);
+    }
+  }
+  return n;
+}
+
+void destroy_graphs(shx_amr* s) {
+  for (int p = 0; p < 2; p++)
+    if (s->graph[p]) { (void)hipGraphExecDestroy(s->graph[p]); s->graph[p] = nullptr; }
+}
+
+int build_graphs(shx_amr* s) {
+  destroy_graphs(s);
+  for (int p = 0; p < 2; p++) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    s->launches = enqueue_step(s, p);
+    HIPCHK(hipStreamEndCapture(s->stream, &g));
+    HIPCHK(hipGraphInstantiate(&s->graph[p], g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+  }
+  return SHX_OK;
+}
+
+template <typename T>
+Levels<T> levels_of(double dx0) {
+  Levels<T> lv;
+  for (int l = 0; l < AMR_MAXLEV; l++) lv.dx[l] = T(dx0 / (double)(1 << l));
+  return lv;
+}
+
+template <typename T>
+int set_state_t(shx_amr* s, const T* u1, const T* u2, size_t ncell, double time, int steps) {
+  if (!s) return fail(SHX_EINVAL, "null sim");
+  if ((int)sizeof(T) != s->es) return fail(SHX_EINVAL, "state precision %d does not match the sim's %d", (int)sizeof(T) * 8, s->es * 8);
+  if (ncell != s->ncell) return fail(SHX_EINVAL, "ncell %zu != %zu (blocks x 4096)", ncell, s->ncell);
+  for (int i = 0; i < s->nblocks; i++) {
+    if (!(s->tab[(size_t)i * AMR_BT + BT_FLAGS] & 2)) continue;
+    for (size_t o = (size_t)i * AMR_CELLS * 4; o < (size_t)(i + 1) * AMR_CELLS * 4; o++)
+      if (!isfinite((double)u1[o]) || !isfinite((double)u2[o])) return fail(SHX_ESTATE, "non-finite state in leaf %d", i);
+  }
+  const size_t bytes = ncell * 4 * sizeof(T);
+  s->cur = 0;
+  HIPCHK(hipMemcpyAsync(s->U1[0], u1, bytes, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->U2[0], u2, bytes, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->U1[1], u1, bytes, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->U2[1], u2, bytes, hipMemcpyHostToDevice, s->stream));
+  T clk[16] = {0};
+  clk[1] = T(time);
+  clk[3] = T(steps);
+  clk[4] = clk[5] = clk[6] = T(1);
+  HIPCHK(hipMemcpyAsync(s->clk, clk, sizeof clk, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemsetAsync(s->slot, 0, sizeof(unsigned long long), s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return SHX_OK;
+}
+
+template <typename T>
+int read_state_t(shx_amr* s, T* u1, T* u2) {
+  if (!s) return fail(SHX_EINVAL, "null sim");
+  if ((int)sizeof(T) != s->es) return fail(SHX_EINVAL, "state precision mismatch");
+  const size_t bytes = s->ncell * 4 * sizeof(T);
+  HIPCHK(hipMemcpyAsync(u1, s->U1[s->cur], bytes, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(u2, s->U2[s->cur], bytes, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return SHX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const int* blocks, int nblocks,
+                   shx_amr** out) {
+  if (!cfg || !layout || !blocks || !out || nblocks < 1) return fail(SHX_EINVAL, "null argument or no blocks");
+  *out = nullptr;
+  const shx_config& c = *cfg;
+  for (int a = 0; a < 3; a++) {
+    if (layout->root[a] < 1) return fail(SHX_EINVAL, "root[%d] = %d", a, layout->root[a]);
+    if (c.n[a] != AMR_BS * layout->root[a]) return fail(SHX_EINVAL, "n[%d] = %d must be 16 x root[%d] = %d", a, c.n[a], a, AMR_BS * layout->root[a]);
+    if (c.bc_lo[a] < 0 || c.bc_lo[a] > 3 || c.bc_hi[a] < 0 || c.bc_hi[a] > 3) return fail(SHX_EINVAL, "boundary code on axis %d", a);
+  }
+  if (c.ndim != 3) return fail(SHX_EINVAL, "the block grid is 3D (ndim %d)", c.ndim);
+  if (c.precision != 32 && c.precision != 64) return fail(SHX_EINVAL, "precision %d", c.precision);
+  if (!(c.gamma > 1) || !(c.dx > 0) || !(c.cfl > 0)) return fail(SHX_EINVAL, "gamma, dx and cfl must be positive (gamma > 1)");
+  if (layout->path != SHX_AMR_PATH_PASSES && layout->path != SHX_AMR_PATH_LEVELS) return fail(SHX_EINVAL, "AMR path %d", layout->path);
+  AmrOps ops = amr_ops(c.precision, c.limiter, c.riemann, c.isentropic ? 1 : 0);
+  if (!ops.assemble) return fail(SHX_EINVAL, "no block kernels built for precision %d limiter %d riemann %d isentropic %d", c.precision, c.limiter, c.riemann, c.isentropic);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c.device) return fail(SHX_EDEVICE, "no HIP device %d", c.device);
+
+  shx_amr* s = new shx_amr();
+  s->cfg = c;
+  s->lay = *layout;
+  s->es = c.precision == 64 ? 8 : 4;
+  s->nblocks = nblocks;
+  s->ncell = (size_t)nblocks * AMR_CELLS;
+  s->blocks.assign(blocks, blocks + 4 * (size_t)nblocks);
+  int rc = build_table(s);
+  if (rc) { delete s; return rc; }
+  std::vector<int> lists;
+  build_lists(s, lists);
+  s->ops = ops;
+  s->pf = make_params<float>(c);
+  s->pd = make_params<double>(c);
+  s->lf = levels_of<float>(c.dx);
+  s->ld = levels_of<double>(c.dx);
+  if (hipSetDevice(c.device) != hipSuccess) { delete s; return fail(SHX_EDEVICE, "hipSetDevice(%d)", c.device); }
+
+  const size_t ubytes = s->ncell * 4 * s->es;
+  const size_t abytes = (size_t)nblocks * AMR_TEXELS * 4 * s->es;
+  auto alloc = [&](void** p, size_t b) { return hipMalloc(p, b) == hipSuccess; };
+  bool ok = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess;
+  for (int p = 0; ok && p < 2; p++) ok = alloc(&s->U1[p], ubytes) && alloc(&s->U2[p], ubytes);
+  ok = ok && alloc(&s->W1, abytes) && alloc(&s->W2, abytes) && alloc(&s->PA, abytes) && alloc(&s->PB, abytes);
+  ok = ok && alloc((void**)&s->d_tab, s->tab.size() * sizeof(int)) && alloc((void**)&s->d_lists, lists.size() * sizeof(int));
+  ok = ok && alloc(&s->clk, 16 * 8) && alloc((void**)&s->slot, 64) && alloc((void**)&s->diag_part, (size_t)nblocks * 16 * 8) &&
+       alloc((void**)&s->diag_out, 16 * 8);
+  if (!ok) {
+    shx_amr_destroy(s);
+    return fail(SHX_ENOMEM, "device allocation failed for %d blocks (%.2f GB of state and atlases)", nblocks,
+                (2 * ubytes * 2 + 4 * abytes) / 1e9);
+  }
+  // edge and corner texels of W are never assembled: zero, as a fresh atlas
+  bool zeroed = hipMemset(s->W1, 0, abytes) == hipSuccess && hipMemset(s->W2, 0, abytes) == hipSuccess &&
+                hipMemset(s->PA, 0, abytes) == hipSuccess && hipMemset(s->PB, 0, abytes) == hipSuccess &&
+                hipMemset(s->U1[0], 0, ubytes) == hipSuccess && hipMemset(s->U2[0], 0, ubytes) == hipSuccess &&
+                hipMemset(s->U1[1], 0, ubytes) == hipSuccess && hipMemset(s->U2[1], 0, ubytes) == hipSuccess &&
+                hipMemset(s->clk, 0, 16 * 8) == hipSuccess && hipMemset(s->slot, 0, 64) == hipSuccess;
+  bool copied = hipMemcpy(s->d_tab, s->tab.data(), s->tab.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+                hipMemcpy(s->d_lists, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+  if (!zeroed || !copied || hipDeviceSynchronize() != hipSuccess) {
+    shx_amr_destroy(s);
+    return fail(SHX_EDEVICE, "initialising the block grid's device memory failed");
+  }
+  rc = build_graphs(s);
+  if (rc) { shx_amr_destroy(s); return rc; }
+  *out = s;
+  return SHX_OK;
+}
+
+int shx_amr_destroy(shx_amr* s) {
+  if (!s) return SHX_OK;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  destroy_graphs(s);
+  for (int p = 0; p < 2; p++) { (void)hipFree(s->U1[p]); (void)hipFree(s->U2[p]); }
+  (void)hipFree(s->W1); (void)hipFree(s->W2); (void)hipFree(s->PA); (void)hipFree(s->PB);
+  (void)hipFree(s->d_tab); (void)hipFree(s->d_lists); (void)hipFree(s->clk); (void)hipFree(s->slot);
+  (void)hipFree(s->diag_part); (void)hipFree(s->diag_out);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return SHX_OK;
+}
+
+int shx_amr_table(shx_amr* s, int* out, size_t len) {
+  if (!s || !out) return fail(SHX_EINVAL, "bad argument");
+  if (len < s->tab.size()) return fail(SHX_EINVAL, "table needs %zu ints", s->tab.size());
+  memcpy(out, s->tab.data(), s->tab.size() * sizeof(int));
+  return SHX_OK;
+}
+
+int shx_amr_set_state(shx_amr* s, const float* u1, const float* u2, size_t ncell, double time, int steps) {
+  return set_state_t<float>(s, u1, u2, ncell, time, steps);
+}
+int shx_amr_set_state_f64(shx_amr* s, const double* u1, const double* u2, size_t ncell, double time, int steps) {
+  return set_state_t<double>(s, u1, u2, ncell, time, steps);
+}
+int shx_amr_read_state(shx_amr* s, float* u1, float* u2) { return read_state_t<float>(s, u1, u2); }
+int shx_amr_read_state_f64(shx_amr* s, double* u1, double* u2) { return read_state_t<double>(s, u1, u2); }
+
+int shx_amr_step(shx_amr* s, int count) {
+  if (!s || count < 0) return fail(SHX_EINVAL, "bad argument");
+  for (int i = 0; i < count; i++) {
+    HIPCHK(hipGraphLaunch(s->graph[s->cur], s->stream));
+    s->cur = 1 - s->cur;
+  }
+  return SHX_OK;
+}
+
+int shx_amr_sync(shx_amr* s) {
+  if (!s) return fail(SHX_EINVAL, "null sim");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return SHX_OK;
+}
+
+int shx_amr_read_clock(shx_amr* s, double out[11]) {
+  if (!s || !out) return fail(SHX_EINVAL, "bad argument");
+  unsigned char buf[16 * 8];
+  HIPCHK(hipMemcpyAsync(buf, s->clk, sizeof buf, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (int i = 0; i < 11; i++) out[i] = s->es == 8 ? ((double*)buf)[i] : (double)((float*)buf)[i];
+  return SHX_OK;
+}
+
+int shx_amr_diagnostics(shx_amr* s, double out[16]) {
+  if (!s || !out) return fail(SHX_EINVAL, "bad argument");
+  const AmrLaunch L = launch_of(s);
+  s->ops.diag(L, lp(s, s->leaf), s->leaf.count, s->U1[s->cur], s->U2[s->cur], s->diag_part, s->diag_out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->diag_out, 16 * 8, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return SHX_OK;
+}
+
+int shx_amr_profile(shx_amr* s, int steps, double ms[5]) {
+  if (!s || !ms || steps < 1) return fail(SHX_EINVAL, "bad argument");
+  hipEvent_t ev[5];
+  for (int i = 0; i < 5; i++) HIPCHK(hipEventCreate(&ev[i]));
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int n = 0; n < steps; n++) {
+    const int p = s->cur;
+    HIPCHK(hipEventRecord(ev[0], s->stream));
+    for (int g = 0; g < 4; g++) {
+      enqueue_step(s, p, g);
+      HIPCHK(hipEventRecord(ev[g + 1], s->stream));
+    }
+    HIPCHK(hipEventSynchronize(ev[4]));
+    HIPCHK(hipGetLastError());
+    s->cur = 1 - p;
+    float t;
+    for (int i = 0; i < 4; i++) { HIPCHK(hipEventElapsedTime(&t, ev[i], ev[i + 1])); acc[i] += t; }
+    HIPCHK(hipEventElapsedTime(&t, ev[0], ev[4]));
+    acc[4] += t;
+  }
+  for (int i = 0; i < 5; i++) ms[i] = acc[i] / steps;
+  for (int i = 0; i < 5; i++) (void)hipEventDestroy(ev[i]);
+  return SHX_OK;
+}
+
+int shx_amr_stats(shx_amr* s, int out[5]) {
+  if (!s || !out) return fail(SHX_EINVAL, "bad argument");
+  out[0] = s->nblocks;
+  out[1] = s->leaf.count;
+  out[2] = s->match.count;
+  out[3] = s->nlevels;
+  out[4] = s->launches;
+  return SHX_OK;
+}
+
+}  // extern "C"
