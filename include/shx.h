@@ -126,11 +126,14 @@ int shx_autotune(shx_sim* sim, int reps, int chosen[3]);
  * leaves' cells are read and written. */
 typedef struct shx_amr shx_amr;
 
-/* Kernel path of the AMR step. PASSES: the reference's passes (assemble -> clock -> predict ->
- * update, amr_sim.ts:2862-2872), each launched once over every level from device slot lists;
- * LEVELS: the same kernels launched level by level (the plain per-level form). Both give the
- * same bits. */
-enum { SHX_AMR_PATH_PASSES = 1, SHX_AMR_PATH_LEVELS = 2 };
+/* Kernel path of the AMR step. FUSED (the batched form): a clock pass and one persistent kernel
+ * over a device list of leaves that forms ghosts, half steps, faces and the update of each block
+ * in one march, coarse-fine faces through a flux register (launched once for the leaves without a
+ * refined neighbour, then once per level for the others, finest first). PASSES: the reference's
+ * passes (assemble -> clock -> predict -> update, amr_sim.ts:2862-2872) with W, PA and PB atlases
+ * in HBM, each launched once over every level from device slot lists. LEVELS: those kernels
+ * launched level by level (the plain per-level form). All three give the same results. */
+enum { SHX_AMR_PATH_FUSED = 0, SHX_AMR_PATH_PASSES = 1, SHX_AMR_PATH_LEVELS = 2 };
 
 typedef struct {
   int root[3];         /* root blocks per axis (the BlockTree root grid) */

@@ -14,7 +14,7 @@ SHX_OK, SHX_EINVAL, SHX_ENOMEM, SHX_EDEVICE, SHX_ESTATE = 0, -1, -2, -3, -4
 LIMITERS = {"koren": 0, "mc": 1, "vanleer": 2, "minmod": 3}
 RIEMANN = {"hllc": 0, "hll": 1}
 PATHS = {"fused": 0, "plain": 1}
-AMR_PATHS = {"passes": 1, "levels": 2}
+AMR_PATHS = {"fused": 0, "passes": 1, "levels": 2}
 
 
 class ShxError(RuntimeError):

@@ -26,7 +26,7 @@ class AmrSim:
     """A block-AMR hydro run on one GPU. `cfg` describes the level-0 grid (n = 16 x root blocks,
     dx = level-0 cell size); `blocks` is (nblocks, 4) of (level, ix, iy, iz)."""
 
-    def __init__(self, cfg: SimConfig, blocks, root, path: str = "passes", compensated: bool = True):
+    def __init__(self, cfg: SimConfig, blocks, root, path: str = "fused", compensated: bool = True):
         lib = _lib.load()
         root = tuple(int(r) for r in root)
         self.cfg = replace(cfg, n=tuple(BS * r for r in root), ndim=3)

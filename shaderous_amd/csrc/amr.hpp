@@ -40,7 +40,6 @@ struct AmrLaunch {
   double dx0;
 };
 
-// The fused block kernel's work: per listed leaf, its coarse-fine faces.
 struct AmrOps {
   void (*assemble)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2, void* w1, void* w2,
                    unsigned long long* slot);
@@ -54,6 +53,10 @@ struct AmrOps {
                 unsigned long long* slot);
   void (*diag)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2, double* part,
                double* out);
+  // the fused block kernel over `count` listed leaves with `grid` persistent workgroups; reg_of:
+  // per (slot, face) the flux-register index of a face on a coarser block (-1 elsewhere)
+  void (*fused)(const AmrLaunch&, const int* list, int count, int grid, bool compensated, const int* reg_of, void* reg,
+                const void* u1, const void* u2, void* o1, void* o2, const void* clk, unsigned long long* slot);
 };
 
 AmrOps amr_ops_f32(int lim, int rs, int isen);

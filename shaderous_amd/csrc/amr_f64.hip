@@ -1,5 +1,5 @@
 // FP64 block-AMR kernels: the all-FP64 twin of the production path (precision study).
-#include "amr_impl.hpp"
+#include "amr_fused.hpp"
 namespace shx {
 AmrOps amr_ops_f64(int lim, int rs, int isen) {
   if (rs != RS_HLLC) return AmrOps{};

@@ -1,0 +1,365 @@
+// The fused block-AMR step for gfx950: one persistent kernel per launch consumes a device list
+// of leaves; a workgroup owns one 16^3 block at a time and computes, in one march along z, what
+// the reference's assemble, predict and update passes compute for it (src/amr/assemble.wgsl,
+// predict.wgsl, update.wgsl), with no W, PA or PB atlas in HBM:
+//
+//  - the block's 20^3 texels are formed on load from U (same-level neighbours, boundary images,
+//    prolongation from a coarser leaf, restriction of refined children: amr_texel), each once;
+//  - the half step of every cell and of the first ghost layer: cells and ghosts beside a coarser
+//    block from the block's own texels (the latter with masked transverse slopes), ghosts whose
+//    state is another block's cell ("at source", amr/predict.wgsl:183-221) from that block's
+//    texels, formed directly;
+//  - every face once; the conservative update of each cell; the next step's rate / dx;
+//  - coarse-fine faces through a flux register: a leaf writes the fluxes of its faces on a
+//    coarser block (as it computed them), and the coarser leaf, launched after it, reads the
+//    four fine fluxes of each of its faces there and takes their mean (TwoSum-compensated when
+//    asked), which is what match_faces (amr/update.wgsl:207-290) recomputes from the finer
+//    block's atlas. Launch order: leaves with no refined neighbour first, then the others level
+//    by level from the finest.
+//
+// Thread layout: 448 threads (7 waves) own the 400 columns of the block's 20 x 20 texel plane:
+// threads 0-255 the 16 x 16 cell columns, 256-319 the 64 face-ghost columns of ring 1,
+// 320-323 ring 1's corners, 324-399 ring 2, 400-447 idle. So the cell waves never diverge on
+// ghost work, and the one wave of face ghosts does the at-source half steps together. LDS holds
+// the plane data in raster order of the 20 x 20 columns (the uniform fused kernel's layout).
+#pragma once
+#include "amr_impl.hpp"
+
+namespace shx {
+
+constexpr int AF_SIDE = AMR_PAD;              // 20 columns per side
+constexpr int AF_NT = AF_SIDE * AF_SIDE;      // 400 columns
+constexpr int AF_THREADS = 448;
+
+// thread -> column (x, y) in block coordinates, [-2, 18); idle threads get x = y = -100
+__device__ __forceinline__ void af_column(int t, int& x, int& y) {
+  if (t < 256) { x = t & 15; y = t >> 4; return; }
+  t -= 256;
+  if (t < 64) {
+    const int side = t >> 4, i = t & 15;
+    x = side == 0 ? -1 : (side == 1 ? 16 : i);
+    y = side == 2 ? -1 : (side == 3 ? 16 : i);
+    return;
+  }
+  t -= 64;
+  if (t < 4) { x = (t & 1) ? 16 : -1; y = (t & 2) ? 16 : -1; return; }
+  t -= 4;
+  if (t < 20) { x = t - 2; y = -2; return; }
+  t -= 20;
+  if (t < 20) { x = t - 2; y = 17; return; }
+  t -= 20;
+  if (t < 18) { x = -2; y = t - 1; return; }
+  t -= 18;
+  if (t < 18) { x = 17; y = t - 1; return; }
+  x = y = -100;
+}
+
+// Assembled primitives of a texel of block `slot` (zero for edges and corners, as the atlas).
+// LEAF: the caller knows `slot` is a leaf, whose cells are read directly (a parent's cells are
+// the means of its children's, u_at).
+template <typename T, int LIM, bool LEAF>
+__device__ __forceinline__ Prim<T> texel_of(const Params<T>& P, const int* __restrict__ tab,
+                                            const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, int slot,
+                                            int x, int y, int z) {
+  if (x >= 0 && x < AMR_BS && y >= 0 && y < AMR_BS && z >= 0 && z < AMR_BS && (LEAF || is_leaf(tab, slot))) {
+    const size_t o = (size_t)slot * AMR_CELLS + cel(x, y, z);
+    return prim_from_cons(P, U1[o], U2[o]);
+  }
+  Prim<T> w;
+  if (!amr_texel<T, LIM>(P, tab, U1, U2, slot, x, y, z, w)) {
+    w.rho = w.vx = w.vy = w.vz = w.p = w.s = w.cs = T(0);
+  }
+  return w;
+}
+
+// The half step of texel (x, y, z) of block `slot` from that block's texels formed directly
+// (predict_cell on the source block, amr/predict.wgsl:90-181); P carries the block's dx
+template <typename T, int LIM, bool ISEN>
+__device__ __forceinline__ Half<T> half_at_source(const Params<T>& P, const int* __restrict__ tab,
+                                               const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, int slot,
+                                               int x, int y, int z, T dt) {
+  const Prim<T> q = texel_of<T, LIM, false>(P, tab, U1, U2, slot, x, y, z);
+  const Prim<T> xm = texel_of<T, LIM, false>(P, tab, U1, U2, slot, x - 1, y, z);
+  const Prim<T> xp = texel_of<T, LIM, false>(P, tab, U1, U2, slot, x + 1, y, z);
+  const Prim<T> ym = texel_of<T, LIM, false>(P, tab, U1, U2, slot, x, y - 1, z);
+  const Prim<T> yp = texel_of<T, LIM, false>(P, tab, U1, U2, slot, x, y + 1, z);
+  const Prim<T> zm = texel_of<T, LIM, false>(P, tab, U1, U2, slot, x, y, z - 1);
+  const Prim<T> zp = texel_of<T, LIM, false>(P, tab, U1, U2, slot, x, y, z + 1);
+  AxisSlope<T> sl[3];
+  sl[0] = axis_slope<T, LIM, ISEN>(P, xm, q, xp, nullptr, nullptr);
+  sl[1] = axis_slope<T, LIM, ISEN>(P, ym, q, yp, nullptr, nullptr);
+  sl[2] = axis_slope<T, LIM, ISEN>(P, zm, q, zp, nullptr, nullptr);
+  const T conv = (xp.vx - xm.vx) + (yp.vy - ym.vy) + (zp.vz - zm.vz);
+  const T jump = tmax(jump_of(xm.p, xp.p), tmax(jump_of(ym.p, yp.p), jump_of(zm.p, zp.p)));
+  return hancock(P, q, sl, conv, jump, dt);
+}
+
+// Flux register: per coarse-fine face of a leaf, its 16 x 16 faces' fluxes (8 values each), in
+// the face's transverse order (t2 * 16 + t1, t1 = (axis + 1) % 3, t2 = (axis + 2) % 3)
+template <typename T>
+__device__ __forceinline__ void reg_put(T* __restrict__ R, int reg, int idx, const Flux<T>& F) {
+  T4<T>* p = reinterpret_cast<T4<T>*>(R + ((size_t)reg * 256 + idx) * 8);
+  p[0] = v4<T>(F.m, F.mx, F.my, F.mz);
+  p[1] = v4<T>(F.e, F.ei, F.s, F.vn);
+}
+
+// The mean of the four fine fluxes through the face of cell (x, y, z) on side k, read from the
+// register of the refined neighbour's child; summed in match_faces' order (t2 outer, t1 inner)
+template <typename T, bool COMP>
+__device__ __forceinline__ Flux<T> reg_mean(const int* __restrict__ tab, const int* __restrict__ reg_of,
+                                            const T* __restrict__ R, int np, int k, int x, int y, int z,
+                                            const Flux<T>& fallback) {
+  const int axis = k >> 1, upper = !(k & 1);
+  const int hx = x >= AMR_BS / 2, hy = y >= AMR_BS / 2, hz = z >= AMR_BS / 2;
+  const int bx = axis == 0 ? upper : hx, by = axis == 1 ? upper : hy, bz = axis == 2 ? upper : hz;
+  const int ch = bt(tab, np, BT_CHILD + (bx | (by << 1) | (bz << 2)));
+  const int reg = ch >= 0 ? reg_of[ch * 6 + (k ^ 1)] : -1;
+  if (reg < 0) return fallback;   // not reached in a 2:1-balanced tree (checked at creation)
+  // transverse fine coordinates of the first of the four faces
+  const int fx = 2 * (x - (AMR_BS / 2) * hx), fy = 2 * (y - (AMR_BS / 2) * hy), fz = 2 * (z - (AMR_BS / 2) * hz);
+  const int u0 = axis == 0 ? fy : (axis == 1 ? fz : fx);   // t1
+  const int v0 = axis == 0 ? fz : (axis == 1 ? fx : fy);   // t2
+  T sum[8], err[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) { sum[q] = T(0); err[q] = T(0); }
+#pragma unroll
+  for (int j = 0; j < 2; j++)
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const T4<T>* p = reinterpret_cast<const T4<T>*>(R + ((size_t)reg * 256 + (v0 + j) * 16 + (u0 + i)) * 8);
+      const T4<T> a = p[0], b = p[1];
+      const T fv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        if constexpr (COMP) {
+          T e;
+          two_sum(sum[q], fv[q], sum[q], e);
+          err[q] = err[q] + e;
+        } else {
+          sum[q] = sum[q] + fv[q];
+        }
+      }
+    }
+  Flux<T> F;
+  F.m = (sum[0] + err[0]) * T(0.25); F.mx = (sum[1] + err[1]) * T(0.25);
+  F.my = (sum[2] + err[2]) * T(0.25); F.mz = (sum[3] + err[3]) * T(0.25);
+  F.e = (sum[4] + err[4]) * T(0.25); F.ei = (sum[5] + err[5]) * T(0.25);
+  F.s = (sum[6] + err[6]) * T(0.25); F.vn = (sum[7] + err[7]) * T(0.25);
+  return F;
+}
+
+// A face of the block on side k: computed flux F, written to the register when the neighbour
+// there is coarser, replaced by the register mean when it is refined
+template <typename T, bool COMP>
+__device__ __forceinline__ Flux<T> boundary_face(const int* __restrict__ tab, const int* __restrict__ reg_of,
+                                                 T* __restrict__ R, int s, int k, int x, int y, int z, int idx,
+                                                 Flux<T> F) {
+  const int nb = bt(tab, s, BT_NB + k);
+  if (nb == -1) {
+    const int reg = reg_of[s * 6 + k];
+    if (reg >= 0) reg_put(R, reg, idx, F);
+  } else if (nb >= 0 && !is_leaf(tab, nb)) {
+    F = reg_mean<T, COMP>(tab, reg_of, R, nb, k, x, y, z, F);
+  }
+  return F;
+}
+
+// Launch bound: seven waves per workgroup, at least two workgroups per CU in FP32 (128 VGPRs)
+template <typename T, int LIM, int RS, bool ISEN, bool COMP>
+__global__ __launch_bounds__(AF_THREADS, (sizeof(T) == 4 ? 4 : 1)) void k_amr_fused(
+    Params<T> P, Levels<T> LV, const int* __restrict__ tab, const int* __restrict__ list, int count,
+    const int* __restrict__ reg_of, T* __restrict__ R, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2,
+    T4<T>* __restrict__ O1, T4<T>* __restrict__ O2, const T* __restrict__ clk, unsigned long long* next_slot) {
+  constexpr int NT = AF_NT, BX = AF_SIDE;
+  using LY = FusedLayout<T, NT>;
+  __shared__ T L[LY::SIZE];
+  __shared__ T red[AF_THREADS / 64];
+  const int tid = threadIdx.x;
+  int x, y;
+  af_column(tid, x, y);
+  const bool valid = x > -100;
+  const int t = valid ? (y + 2) * BX + (x + 2) : 0;    // raster index of the column in LDS
+  const bool ox = x < 0 || x >= AMR_BS, oy = y < 0 || y >= AMR_BS;
+  const bool cell_col = valid && !ox && !oy;             // a 16 x 16 cell column
+  const bool r1 = valid && x >= -1 && x <= AMR_BS && y >= -1 && y <= AMR_BS;
+  const bool face_col = r1 && (ox != oy);                // ring-1 face ghost column
+  // x faces below columns x = 0..16 and y faces below y = 0..16, within the cell rows
+  const bool xface = valid && x >= 0 && x <= AMR_BS && !oy;
+  const bool yface = valid && y >= 0 && y <= AMR_BS && !ox;
+  const T dt = clk[0];
+  T rate_max = T(0);
+
+  for (int wi = blockIdx.x; wi < count; wi += gridDim.x) {
+    const int s = list[wi];
+    const T dx = LV.dx[bt(tab, s, BT_LEVEL)];
+    Params<T> Pl = P;
+    Pl.dx = dx;
+    const T dtdx = dt / dx;
+    // a face-ghost column's source (constant over the cell planes)
+    int gk = 0;
+    if (face_col) gk = ghost_src(tab, s, x, y, 0).kind;
+    __syncthreads();   // the previous block's last LDS reads are done
+
+    auto texel = [&](int z) { return texel_of<T, LIM, true>(P, tab, U1, U2, s, x, y, z); };
+    if (valid) {
+      lds_put_prim<T, NT>(L + LY::Q0, t, texel(-2));
+      lds_put_prim<T, NT>(L + LY::Q1, t, texel(-1));
+    }
+    Prim<T> nxt;
+    if (valid) nxt = texel(0);
+
+    SideData<T> zs_prev;
+    Flux<T> fz_lo = zero_flux<T>();
+    Flux<T> axy_prev = zero_flux<T>();
+
+    for (int c = -1; c <= AMR_BS; c++) {
+      const Prim<T> pp = nxt;             // plane c + 1
+      if (valid && c + 2 <= AMR_BS + 1) nxt = texel(c + 2);
+      const bool odd = ((c + 1) & 1) != 0;
+      const T* Lq = L + (odd ? LY::Q0 : LY::Q1);
+      T* Lm = L + (odd ? LY::Q1 : LY::Q0);
+      __syncthreads();
+
+      // 2. the half step of texel (x, y, c) and its sides
+      const bool oz = c < 0 || c >= AMR_BS;
+      const int nout = (int)ox + (int)oy + (int)oz;
+      SideData<T> sxl, syl, szl, szu;
+      if (r1 && nout <= 1) {
+        const Prim<T> pc = lds_prim_c<T, NT>(Lq, t);
+        const Prim<T> pm = lds_prim<T, NT>(Lm, t);
+        const Prim<T> xm = lds_prim<T, NT>(Lq, t - 1), xp = lds_prim<T, NT>(Lq, t + 1);
+        const Prim<T> ym = lds_prim<T, NT>(Lq, t - BX), yp = lds_prim<T, NT>(Lq, t + BX);
+        Inc<T> ixu, ixd, iyu, iyd, izu, izd;
+        AxisSlope<T> sl[3];
+        sl[0] = axis_slope<T, LIM, ISEN>(Pl, xm, pc, xp, &ixu, &ixd);
+        sl[1] = axis_slope<T, LIM, ISEN>(Pl, ym, pc, yp, &iyu, &iyd);
+        sl[2] = axis_slope<T, LIM, ISEN>(Pl, pm, pc, pp, &izu, &izd);
+        // the ghost's source: a face-ghost column's, or a ghost plane's (c = -1 or 16)
+        int kind = 0;
+        if (nout == 1) kind = oz ? (bt(tab, s, BT_NB + (c < 0 ? 4 : 5)) == -1 ? 1 : 0) : gk;
+        Half<T> hc;
+        if (nout == 0 || kind == 1) {
+          if (nout == 1) {
+            // beside a coarser block: predicted in place, transverse slopes dropped at the rim
+            // where their neighbour is an unassembled edge texel (amr/predict.wgsl:183-221)
+            const int cx = x, cy = y, cz = c;
+            const T mx = (!ox && (cx == 0 || cx == AMR_BS - 1)) ? T(0) : T(1);
+            const T my = (!oy && (cy == 0 || cy == AMR_BS - 1)) ? T(0) : T(1);
+            const T mz = (!oz && (cz == 0 || cz == AMR_BS - 1)) ? T(0) : T(1);
+            const T mk[3] = {mx, my, mz};
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+              Inc<T>& q = sl[a].s;
+              q.rho *= mk[a]; q.vx *= mk[a]; q.vy *= mk[a]; q.vz *= mk[a]; q.p *= mk[a]; q.s *= mk[a];
+            }
+          }
+          const T conv = (xp.vx - xm.vx) + (yp.vy - ym.vy) + (pp.vz - pm.vz);
+          const T jump = tmax(jump_of(xm.p, xp.p), tmax(jump_of(ym.p, yp.p), jump_of(pm.p, pp.p)));
+          hc = hancock(Pl, pc, sl, conv, jump, dt);
+        } else {
+          // another block's cell (same level, refined, or this block's boundary image)
+          const GSrc g = ghost_src(tab, s, x, y, c);
+          hc = half_at_source<T, LIM, ISEN>(Pl, tab, U1, U2, g.slot, g.x, g.y, g.z, dt);
+          hc.vx = bc_vel(hc.vx, g.fx, g.sx);
+          hc.vy = bc_vel(hc.vy, g.fy, g.sy);
+          hc.vz = bc_vel(hc.vz, g.fz, g.sz);
+        }
+        sxl = side_of(hc, ixd, T(-1));
+        syl = side_of(hc, iyd, T(-1));
+        szl = side_of(hc, izd, T(-1));
+        szu = side_of(hc, izu, T(1));
+        lds_put_side<T, NT>(L, LY::SX, t, side_of(hc, ixu, T(1)));
+        lds_put_side<T, NT>(L, LY::SY, t, side_of(hc, iyu, T(1)));
+      }
+      // plane c + 1 into the slot plane c - 1 leaves
+      if (valid) lds_put_prim<T, NT>(Lm, t, pp);
+      __syncthreads();
+
+      // 3. the x and y faces below this column, and the z face below plane c
+      const bool live = c >= 0 && c < AMR_BS;
+      Flux<T> Fx = zero_flux<T>(), Fy = zero_flux<T>(), fz = zero_flux<T>();
+      if (live && xface) {
+        Fx = face_from_sides<T, RS>(Pl, lds_get_side<T, NT>(L, LY::SX, t - 1), sxl, 0);
+        if (x == 0) Fx = boundary_face<T, COMP>(tab, reg_of, R, s, 0, x, y, c, c * 16 + y, Fx);
+        else if (x == AMR_BS) Fx = boundary_face<T, COMP>(tab, reg_of, R, s, 1, x - 1, y, c, c * 16 + y, Fx);
+      }
+      if (live && yface) {
+        Fy = face_from_sides<T, RS>(Pl, lds_get_side<T, NT>(L, LY::SY, t - BX), syl, 1);
+        if (y == 0) Fy = boundary_face<T, COMP>(tab, reg_of, R, s, 2, x, y, c, x * 16 + c, Fy);
+        else if (y == AMR_BS) Fy = boundary_face<T, COMP>(tab, reg_of, R, s, 3, x, y - 1, c, x * 16 + c, Fy);
+      }
+      if (cell_col && c >= 0) {
+        fz = face_from_sides<T, RS>(Pl, zs_prev, szl, 2);
+        if (c == 0) fz = boundary_face<T, COMP>(tab, reg_of, R, s, 4, x, y, c, y * 16 + x, fz);
+        else if (c == AMR_BS) fz = boundary_face<T, COMP>(tab, reg_of, R, s, 5, x, y, c - 1, y * 16 + x, fz);
+      }
+      __syncthreads();
+      // 4. the x/y fluxes into LDS
+      if (live && xface) lds_put_flux<T, NT>(L, LY::FX, t, Fx);
+      if (live && yface) lds_put_flux<T, NT>(L, LY::FY, t, Fy);
+      __syncthreads();
+
+      // 5. this plane's x/y net inflow, and the update of plane c - 1
+      Flux<T> axy = zero_flux<T>();
+      if (live && cell_col) {
+        const Flux<T> xh = lds_get_flux<T, NT>(L, LY::FX, t + 1);
+        const Flux<T> yh = lds_get_flux<T, NT>(L, LY::FY, t + BX);
+        axy.m = (Fx.m - xh.m) + (Fy.m - yh.m);
+        axy.mx = (Fx.mx - xh.mx) + (Fy.mx - yh.mx);
+        axy.my = (Fx.my - xh.my) + (Fy.my - yh.my);
+        axy.mz = (Fx.mz - xh.mz) + (Fy.mz - yh.mz);
+        axy.e = (Fx.e - xh.e) + (Fy.e - yh.e);
+        axy.ei = (Fx.ei - xh.ei) + (Fy.ei - yh.ei);
+        axy.s = (Fx.s - xh.s) + (Fy.s - yh.s);
+        axy.vn = (Fx.vn - xh.vn) + (Fy.vn - yh.vn);
+      }
+      if (cell_col && c - 1 >= 0 && c - 1 < AMR_BS) {
+        const size_t o = (size_t)s * AMR_CELLS + cel(x, y, c - 1);
+        Flux<T> net;
+        net.m = axy_prev.m + (fz_lo.m - fz.m);
+        net.mx = axy_prev.mx + (fz_lo.mx - fz.mx);
+        net.my = axy_prev.my + (fz_lo.my - fz.my);
+        net.mz = axy_prev.mz + (fz_lo.mz - fz.mz);
+        net.e = axy_prev.e + (fz_lo.e - fz.e);
+        net.ei = axy_prev.ei + (fz_lo.ei - fz.ei);
+        net.s = axy_prev.s + (fz_lo.s - fz.s);
+        net.vn = axy_prev.vn + (fz_lo.vn - fz.vn);
+        T4<T> o1, o2;
+        advance(Pl, U1[o], U2[o], net, zs_prev.hp, dtdx, o1, o2);
+        O1[o] = o1;
+        O2[o] = o2;
+        // the next step's rate / dx (assemble's leaf-cell rate, amr/assemble.wgsl:77-154)
+        rate_max = tmax(rate_max, rate_of(P, prim_from_cons(P, o1, o2)) / dx);
+      }
+      zs_prev = szu;
+      fz_lo = fz;
+      axy_prev = axy;
+    }
+  }
+  // workgroup maximum of the next step's rate, one atomicMax
+  T v = wave_max(rate_max);
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  if (tid == 0) {
+    T m = T(0);
+    for (int w = 0; w < AF_THREADS / 64; w++) m = tmax(m, red[w]);
+    atomicMax(next_slot, (unsigned long long)order_bits(m));
+  }
+}
+
+template <typename T, int LIM, int RS, bool ISEN>
+void AmrImpl<T, LIM, RS, ISEN>::fused(const AmrLaunch& L, const int* list, int count, int grid, bool comp,
+                                      const int* reg_of, void* reg, const void* u1, const void* u2, void* o1, void* o2,
+                                      const void* clk, unsigned long long* slot) {
+  if (count <= 0) return;
+  if (grid > count) grid = count;
+#define SHX_AF(C)                                                                                                  \
+  hipLaunchKernelGGL((k_amr_fused<T, LIM, RS, ISEN, C>), dim3(grid), dim3(AF_THREADS), 0, L.stream, P(L), LV(L),    \
+                     L.tab, list, count, reg_of, (T*)reg, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)o1,            \
+                     (T4<T>*)o2, (const T*)clk, slot)
+  if (comp) SHX_AF(true);
+  else SHX_AF(false);
+#undef SHX_AF
+}
+
+}  // namespace shx

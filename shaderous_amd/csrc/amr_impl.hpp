@@ -601,8 +601,11 @@ struct AmrImpl {
                        (const T4<T>*)u1, (const T4<T>*)u2, part);
     hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(64), 0, L.stream, (const double*)part, count, out);
   }
+  static void fused(const AmrLaunch& L, const int* list, int count, int grid, bool comp, const int* reg_of, void* reg,
+                    const void* u1, const void* u2, void* o1, void* o2, const void* clk, unsigned long long* slot);
   static AmrOps ops() {
     AmrOps o;
+    o.fused = fused;
     o.assemble = assemble;
     o.clock = clock;
     o.predict = predict;

@@ -43,6 +43,11 @@ struct shx_amr {
   void* U1[2] = {nullptr, nullptr};
   void* U2[2] = {nullptr, nullptr};
   void *W1 = nullptr, *W2 = nullptr, *PA = nullptr, *PB = nullptr;
+  // fused path: flux-register index per (slot, face) and the register
+  int* d_reg_of = nullptr;
+  void* reg = nullptr;
+  int nreg = 0;
+  int grid = 256;            // persistent workgroups of the fused kernel
   void* clk = nullptr;
   unsigned long long* slot = nullptr;
   double* diag_part = nullptr;
@@ -201,6 +206,20 @@ int enqueue_step(shx_amr* s, int p, int stage = -1) {
   const bool comp = s->lay.compensated != 0;
   int n = 0;
   auto run = [&](int g) { return stage < 0 || stage == g; };
+  if (s->lay.path == SHX_AMR_PATH_FUSED) {
+    if (run(1)) { o.clock(L, s->clk, s->slot); n++; }
+    if (run(3)) {
+      auto fz = [&](const Range& r) {
+        if (!r.count) return;
+        o.fused(L, lp(s, r), r.count, s->grid, comp, s->d_reg_of, s->reg, s->U1[p], s->U2[p], s->U1[1 - p],
+                s->U2[1 - p], s->clk, s->slot);
+        n++;
+      };
+      fz(s->plain);
+      for (int l = s->nlevels - 1; l >= 0; l--) fz(s->match_lv[l]);
+    }
+    return n;
+  }
   if (run(0)) {
     if (levels) {
       for (int l = 0; l < s->nlevels; l++)
@@ -284,6 +303,11 @@ int set_state_t(shx_amr* s, const T* u1, const T* u2, size_t ncell, double time,
   clk[4] = clk[5] = clk[6] = T(1);
   HIPCHK(hipMemcpyAsync(s->clk, clk, sizeof clk, hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipMemsetAsync(s->slot, 0, sizeof(unsigned long long), s->stream));
+  if (s->lay.path == SHX_AMR_PATH_FUSED) {
+    // the fused kernel leaves the next step's rate in the slot; the first step's comes from here
+    s->ops.rates(launch_of(s), lp(s, s->leaf), s->leaf.count, s->U1[0], s->U2[0], s->slot);
+    HIPCHK(hipGetLastError());
+  }
   HIPCHK(hipStreamSynchronize(s->stream));
   return SHX_OK;
 }
@@ -316,7 +340,8 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   if (c.ndim != 3) return fail(SHX_EINVAL, "the block grid is 3D (ndim %d)", c.ndim);
   if (c.precision != 32 && c.precision != 64) return fail(SHX_EINVAL, "precision %d", c.precision);
   if (!(c.gamma > 1) || !(c.dx > 0) || !(c.cfl > 0)) return fail(SHX_EINVAL, "gamma, dx and cfl must be positive (gamma > 1)");
-  if (layout->path != SHX_AMR_PATH_PASSES && layout->path != SHX_AMR_PATH_LEVELS) return fail(SHX_EINVAL, "AMR path %d", layout->path);
+  if (layout->path != SHX_AMR_PATH_FUSED && layout->path != SHX_AMR_PATH_PASSES && layout->path != SHX_AMR_PATH_LEVELS)
+    return fail(SHX_EINVAL, "AMR path %d", layout->path);
   AmrOps ops = amr_ops(c.precision, c.limiter, c.riemann, c.isentropic ? 1 : 0);
   if (!ops.assemble) return fail(SHX_EINVAL, "no block kernels built for precision %d limiter %d riemann %d isentropic %d", c.precision, c.limiter, c.riemann, c.isentropic);
   int ndev = 0;
@@ -340,12 +365,28 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   s->ld = levels_of<double>(c.dx);
   if (hipSetDevice(c.device) != hipSuccess) { delete s; return fail(SHX_EDEVICE, "hipSetDevice(%d)", c.device); }
 
+  const bool fused = layout->path == SHX_AMR_PATH_FUSED;
+  // flux registers: every leaf face on a coarser block
+  std::vector<int> reg_of((size_t)nblocks * 6, -1);
+  for (int i = 0; i < nblocks; i++) {
+    if (!(s->tab[(size_t)i * AMR_BT + BT_FLAGS] & 2)) continue;
+    for (int f = 0; f < 6; f++)
+      if (s->tab[(size_t)i * AMR_BT + BT_NB + f] == -1) reg_of[(size_t)i * 6 + f] = s->nreg++;
+  }
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess || cus < 1) cus = 256;
+    // FP32: two 7-wave workgroups per CU (128 VGPRs, 51 KB of LDS each); FP64: one
+    s->grid = (s->es == 8 ? 1 : 2) * cus;
+  }
   const size_t ubytes = s->ncell * 4 * s->es;
-  const size_t abytes = (size_t)nblocks * AMR_TEXELS * 4 * s->es;
+  const size_t abytes = fused ? 0 : (size_t)nblocks * AMR_TEXELS * 4 * s->es;
   auto alloc = [&](void** p, size_t b) { return hipMalloc(p, b) == hipSuccess; };
   bool ok = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess;
   for (int p = 0; ok && p < 2; p++) ok = alloc(&s->U1[p], ubytes) && alloc(&s->U2[p], ubytes);
-  ok = ok && alloc(&s->W1, abytes) && alloc(&s->W2, abytes) && alloc(&s->PA, abytes) && alloc(&s->PB, abytes);
+  if (!fused) ok = ok && alloc(&s->W1, abytes) && alloc(&s->W2, abytes) && alloc(&s->PA, abytes) && alloc(&s->PB, abytes);
+  ok = ok && alloc((void**)&s->d_reg_of, reg_of.size() * sizeof(int)) &&
+       alloc(&s->reg, (size_t)(s->nreg > 0 ? s->nreg : 1) * 256 * 8 * s->es);
   ok = ok && alloc((void**)&s->d_tab, s->tab.size() * sizeof(int)) && alloc((void**)&s->d_lists, lists.size() * sizeof(int));
   ok = ok && alloc(&s->clk, 16 * 8) && alloc((void**)&s->slot, 64) && alloc((void**)&s->diag_part, (size_t)nblocks * 16 * 8) &&
        alloc((void**)&s->diag_out, 16 * 8);
@@ -355,12 +396,13 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
                 (2 * ubytes * 2 + 4 * abytes) / 1e9);
   }
   // edge and corner texels of W are never assembled: zero, as a fresh atlas
-  bool zeroed = hipMemset(s->W1, 0, abytes) == hipSuccess && hipMemset(s->W2, 0, abytes) == hipSuccess &&
-                hipMemset(s->PA, 0, abytes) == hipSuccess && hipMemset(s->PB, 0, abytes) == hipSuccess &&
+  bool zeroed = (fused || (hipMemset(s->W1, 0, abytes) == hipSuccess && hipMemset(s->W2, 0, abytes) == hipSuccess &&
+                            hipMemset(s->PA, 0, abytes) == hipSuccess && hipMemset(s->PB, 0, abytes) == hipSuccess)) &&
                 hipMemset(s->U1[0], 0, ubytes) == hipSuccess && hipMemset(s->U2[0], 0, ubytes) == hipSuccess &&
                 hipMemset(s->U1[1], 0, ubytes) == hipSuccess && hipMemset(s->U2[1], 0, ubytes) == hipSuccess &&
                 hipMemset(s->clk, 0, 16 * 8) == hipSuccess && hipMemset(s->slot, 0, 64) == hipSuccess;
   bool copied = hipMemcpy(s->d_tab, s->tab.data(), s->tab.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+                hipMemcpy(s->d_reg_of, reg_of.data(), reg_of.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
                 hipMemcpy(s->d_lists, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
   if (!zeroed || !copied || hipDeviceSynchronize() != hipSuccess) {
     shx_amr_destroy(s);
@@ -378,6 +420,7 @@ int shx_amr_destroy(shx_amr* s) {
   destroy_graphs(s);
   for (int p = 0; p < 2; p++) { (void)hipFree(s->U1[p]); (void)hipFree(s->U2[p]); }
   (void)hipFree(s->W1); (void)hipFree(s->W2); (void)hipFree(s->PA); (void)hipFree(s->PB);
+  (void)hipFree(s->d_reg_of); (void)hipFree(s->reg);
   (void)hipFree(s->d_tab); (void)hipFree(s->d_lists); (void)hipFree(s->clk); (void)hipFree(s->slot);
   (void)hipFree(s->diag_part); (void)hipFree(s->diag_out);
   if (s->stream) (void)hipStreamDestroy(s->stream);

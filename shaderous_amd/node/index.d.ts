@@ -48,7 +48,7 @@ export interface Diagnostics {
 export interface AmrTree {
   root: [number, number, number];
   blocks: Int32Array;
-  path?: 'passes' | 'levels';
+  path?: 'fused' | 'passes' | 'levels';
   compensated?: boolean;
 }
 

@@ -14,7 +14,7 @@ const LIMITERS = { koren: 0, mc: 1, vanleer: 2, minmod: 3 };
 const RIEMANN = { hllc: 0, hll: 1 };
 const BC = { periodic: 0, outflow: 1, reflect: 2, reflecting: 2, 'outflow-only': 3 };
 const PATHS = { fused: 0, plain: 1 };
-const AMR_PATHS = { passes: 1, levels: 2 };
+const AMR_PATHS = { fused: 0, passes: 1, levels: 2 };
 
 function code(table, v, what) {
   if (typeof v === 'number') return v;
@@ -178,7 +178,7 @@ class Sim {
  * Int32Array of (level, ix, iy, iz) rows, parents and leaves; `root` the root blocks per axis.
  */
 class AmrSim {
-  constructor(cfg, { root, blocks, path = 'passes', compensated = true }) {
+  constructor(cfg, { root, blocks, path = 'fused', compensated = true }) {
     this.cfg = cfg;
     const n = root.map((r) => 16 * r);
     const ncfg = toNative({ ...cfg, n, ndim: 3 });

@@ -484,7 +484,7 @@ static napi_value js_amr_create(napi_env env, napi_callback_info info) {
   memset(&lay, 0, sizeof lay);
   lay.root[0] = lay.root[1] = lay.root[2] = 1;
   int3_prop(env, argv[1], "root", lay.root);
-  lay.path = (int)num_prop(env, argv[1], "path", SHX_AMR_PATH_PASSES);
+  lay.path = (int)num_prop(env, argv[1], "path", SHX_AMR_PATH_FUSED);
   lay.compensated = (int)num_prop(env, argv[1], "compensated", 1);
   bool is = false;
   napi_is_typedarray(env, argv[2], &is);

@@ -109,42 +109,46 @@ def test_amr_table_matches_oracle_blast():
         sim.destroy()
 
 
+@pytest.mark.parametrize("path", ["fused", "passes"])
 @pytest.mark.parametrize("compensated", [True, False])
 @pytest.mark.parametrize("lim,isen", [("mc", False), ("koren", True)])
-def test_amr_fp32_matches_oracle_refined(lim, isen, compensated):
+def test_amr_fp32_matches_oracle_refined(lim, isen, compensated, path):
     blocks = refined_tree()
     cfg = cfg_for((2, 2, 2), limiter=lim, isentropic=isen)
-    worst, _, _ = run_pair(blocks, (2, 2, 2), cfg, 4, compensated=compensated)
+    worst, _, _ = run_pair(blocks, (2, 2, 2), cfg, 4, path=path, compensated=compensated)
     print("L1", worst)
     assert max(worst.values()) <= L1_TOL, worst
 
 
-def test_amr_fp32_matches_oracle_blast_walls():
+@pytest.mark.parametrize("path", ["fused", "passes"])
+def test_amr_fp32_matches_oracle_blast_walls(path):
     blocks = small_blast()
     cfg = cfg_for((4, 4, 4), bc_lo=(2, 1, 0), bc_hi=(1, 2, 0))
 
     def ic(x, y, z):
         r = np.sqrt((x - 0.03) ** 2 + (y + 0.02) ** 2 + (z - 0.05) ** 2)
-        return {"rho": np.ones_like(x), "p": np.where(r < 0.2, 10.0, 1.0)}
-    worst, _, _ = run_pair(blocks, (4, 4, 4), cfg, 3, ic=ic)
+        return {"rho": np.ones_like(x), "p": np.where(r < 0.2, 10.0, 1.0), "vx": 0.3 * np.sin(6 * y)}
+    worst, _, _ = run_pair(blocks, (4, 4, 4), cfg, 3, path=path, ic=ic)
     print("L1", worst)
     assert max(worst.values()) <= L1_TOL, worst
 
 
-def test_amr_fp64_matches_oracle():
+@pytest.mark.parametrize("path", ["fused", "passes"])
+def test_amr_fp64_matches_oracle(path):
     blocks = refined_tree()
     cfg = cfg_for((2, 2, 2), precision=64)
-    worst, _, _ = run_pair(blocks, (2, 2, 2), cfg, 3)
+    worst, _, _ = run_pair(blocks, (2, 2, 2), cfg, 3, path=path)
     print("L1", worst)
     assert max(worst.values()) <= 1e-12, worst
 
 
-def test_amr_levels_equal_passes_bitwise():
+def test_amr_paths_equal_bitwise():
+    """The fused form, the batched passes and the per-level passes give the same bits."""
     blocks = small_blast()
     cfg = cfg_for((4, 4, 4))
     u1, u2 = amr_tree.blocks_state(blocks, cfg.dx, blob, cfg.gamma)
     out = {}
-    for path in ("passes", "levels"):
+    for path in ("passes", "levels", "fused"):
         sim = AmrSim(cfg, blocks, (4, 4, 4), path=path)
         try:
             sim.set_state(u1, u2)
@@ -157,6 +161,9 @@ def test_amr_levels_equal_passes_bitwise():
     sel = leaf_rows(blocks)
     assert np.array_equal(out["passes"][0][sel], out["levels"][0][sel])
     assert np.array_equal(out["passes"][1][sel], out["levels"][1][sel])
+    d = l1rel(out["fused"], out["passes"], sel)
+    print("fused vs passes", d, "bitwise", np.array_equal(out["fused"][0][sel], out["passes"][0][sel]))
+    assert max(d.values()) < 1e-7
 
 
 @pytest.mark.parametrize("precision,tol", [(32, 2e-6), (64, 1e-13)])
