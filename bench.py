@@ -11,7 +11,12 @@ zone-update: one read and one write of the 8-float state) timed with HIP events 
 stream, and the reference CPU scheme (the C restatement of RefHydro) timed on a bounded sample
 on this host's cores.
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--n 256] [--no-cpu]
+The block-AMR blast (root 8^3 blocks of 16^3 cells, up to 3 levels around a seeded-offset sphere
+of radius 0.1 holding P = 100 against rho = 1, P = 1, gamma 1.4; 5024 blocks, 4460 leaves, 18.3 M
+leaf cells; MC + HLLC, CFL 0.4) is timed beside it: the fused batched path (one persistent kernel
+over device leaf lists, graph-captured) against the plain per-level passes, reported under "amr".
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--n 256] [--no-cpu] [--amr-steps 20]
 """
 from __future__ import annotations
 
@@ -63,6 +68,54 @@ def cpu_baseline(pb, budget_planes=48):
                       f"of the same 256^3 state ({cells} cells, {t:.2f} s)"}
 
 
+def amr_bench(args):
+    """The AMR blast: fused batched path vs the plain per-level passes (and the batched passes if
+    asked), leaf zone-updates/s and the fused kernel's HBM roofline fraction (64 B per leaf
+    zone-update)."""
+    from shaderous_amd import amr_tree, problems
+    from shaderous_amd.amr import AmrSim
+    from shaderous_amd.sim import SimConfig
+    t0 = time.perf_counter()
+    blocks, dx0, prims, centre = problems.amr_blast(seed=7)
+    u1, u2 = amr_tree.blocks_state(blocks, dx0, prims, 1.4)
+    leaves = int(amr_tree.leaf_mask(blocks).sum())
+    cells = leaves * 4096
+    log(f"[bench] AMR blast: {len(blocks)} blocks, {leaves} leaves ({cells} cells), IC in {time.perf_counter() - t0:.1f} s")
+    cfg = SimConfig(n=(128, 128, 128), ndim=3, dx=dx0, gamma=1.4, limiter="mc", isentropic=False, cfl=0.4)
+    res = {}
+    for path in args.amr_paths.split(","):
+        sim = AmrSim(cfg, blocks, (8, 8, 8), path=path)
+        try:
+            sim.set_state(u1, u2)
+            sim.step(3)
+            sim.sync()
+            t0 = time.perf_counter()
+            sim.step(args.amr_steps)
+            sim.sync()
+            t = time.perf_counter() - t0
+            prof = sim.profile(3)
+            st = sim.stats()
+            d = sim.diagnostics()
+        finally:
+            sim.destroy()
+        res[path] = {"value": cells * args.amr_steps / t, "ms_per_step": t / args.amr_steps * 1e3, "passes_ms": prof,
+                     "launches_per_step": st["launches_per_step"], "mass": d.mass, "e_tot": d.e_tot}
+        log(f"[bench] AMR {path}: {t / args.amr_steps * 1e3:.3f} ms/step, {cells * args.amr_steps / t:.3e} upd/s, {prof}")
+    out = {"workload": "3D blast on 16^3 blocks: root 8^3, 3 levels, sphere r 0.1 at a seeded offset, P 100 vs 1, "
+                       "rho 1, gamma 1.4, PLM (MC) + HLLC, CFL 0.4, static mesh",
+           "blocks": len(blocks), "leaves": leaves, "leaf_cells": cells, "steps": args.amr_steps, "paths": res}
+    if "fused" in res:
+        f = res["fused"]
+        kernel_ms = f["passes_ms"]["update"]
+        ach = BYTES_PER_UPDATE * cells / (kernel_ms * 1e-3)
+        out["value"] = f["value"]
+        out["roofline"] = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                           "frac": ach / HBM_PEAK, "kernel": "k_amr_fused", "kernel_ms": kernel_ms}
+        if "levels" in res:
+            out["fused_over_plain_levels"] = f["value"] / res["levels"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +125,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--paths", default="fused,plain", help="which uniform-grid paths to time")
+    ap.add_argument("--amr-steps", type=int, default=20, help="timed steps of the AMR blast (0: skip it)")
+    ap.add_argument("--amr-paths", default="fused,levels", help="AMR paths to time (fused, passes, levels)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a device copy of MIB MiB three times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -138,7 +193,12 @@ def main():
         pprof = sim.profile(5)
         sim.destroy()
         log(f"[bench] plain: {tp / args.steps * 1e3:.3f} ms/step, {plain_value:.3e} upd/s, passes {pprof}")
+    amr = None
+    if args.amr_steps > 0:
+        amr = amr_bench(args)
     if value is None:
+        if amr:
+            print(json.dumps({"amr": amr}))
         return
 
     cpu = None
@@ -170,6 +230,7 @@ def main():
         "fused_over_plain": value / plain_value if plain_value else None,
         "conserved": {"mass": diag.mass, "e_tot": diag.e_tot},
         "cpu_baseline": cpu,
+        "amr": amr,
     }
     print(json.dumps(out))
 

@@ -99,21 +99,26 @@ def cell_centers(block, dx0, box_lo=-0.5):
     return x, y, z
 
 
-def blocks_state(blocks, dx0, prim_fn, gamma, dtype=np.float32):
+def blocks_state(blocks, dx0, prim_fn, gamma, dtype=np.float32, box_lo=-0.5):
     """Leaf-interior state of every block (parents get zeros): (nblocks * 4096, 4) u1 and u2, in
-    the block order given, cells x fastest. prim_fn(x, y, z) -> dict of primitives."""
+    the block order given, cells x fastest. prim_fn(x, y, z) -> dict of primitives (vectorised)."""
     from .problems import cons_from_prims
     n = len(blocks)
     u1 = np.zeros((n * BS ** 3, 4), dtype)
     u2 = np.zeros((n * BS ** 3, 4), dtype)
-    leaf = leaf_mask(blocks)
-    for i, b in enumerate(blocks):
-        if not leaf[i]:
-            continue
-        x, y, z = cell_centers(b, dx0)
-        a1, a2 = cons_from_prims(prim_fn(x, y, z), gamma, dtype)
-        u1[i * BS ** 3:(i + 1) * BS ** 3] = a1
-        u2[i * BS ** 3:(i + 1) * BS ** 3] = a2
+    leaf = np.flatnonzero(leaf_mask(blocks))
+    ax = np.arange(BS, dtype=np.float64) + 0.5
+    k, j, i = np.meshgrid(ax, ax, ax, indexing="ij")
+    cell = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1)          # (4096, 3), x fastest
+    for lo in range(0, len(leaf), 512):                                  # bounded memory
+        sel = leaf[lo:lo + 512]
+        b = blocks[sel].astype(np.float64)
+        h = dx0 / np.exp2(b[:, 0])[:, None, None]                        # (m, 1, 1)
+        pos = (cell[None, :, :] + BS * b[:, None, 1:4]) * h + box_lo     # (m, 4096, 3)
+        a1, a2 = cons_from_prims(prim_fn(pos[..., 0], pos[..., 1], pos[..., 2]), gamma, dtype)
+        rows = (sel[:, None] * BS ** 3 + np.arange(BS ** 3)[None, :]).ravel()
+        u1[rows] = a1
+        u2[rows] = a2
     return u1, u2
 
 

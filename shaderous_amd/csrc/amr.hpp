@@ -53,10 +53,15 @@ struct AmrOps {
                 unsigned long long* slot);
   void (*diag)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2, double* part,
                double* out);
-  // the fused block kernel over `count` listed leaves with `grid` persistent workgroups; reg_of:
-  // per (slot, face) the flux-register index of a face on a coarser block (-1 elsewhere)
-  void (*fused)(const AmrLaunch&, const int* list, int count, int grid, bool compensated, const int* reg_of, void* reg,
-                const void* u1, const void* u2, void* o1, void* o2, const void* clk, unsigned long long* slot);
+  // the fused form: the ghost shells of the listed leaves (shell i for list entry i), then the
+  // march over `count` listed leaves with `grid` persistent workgroups. shell_of: per slot, its
+  // shell; reg_of: per (slot, face) the flux-register index of a face on a coarser block (-1
+  // elsewhere)
+  void (*shell)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2, void* sh,
+                const void* clk);
+  void (*fused)(const AmrLaunch&, const int* list, int count, int grid, bool compensated, const int* shell_of,
+                const void* sh, const int* reg_of, void* reg, const void* u1, const void* u2, void* o1, void* o2,
+                const void* clk, unsigned long long* slot);
 };
 
 AmrOps amr_ops_f32(int lim, int rs, int isen);

@@ -1,14 +1,14 @@
-// The fused block-AMR step for gfx950: one persistent kernel per launch consumes a device list
-// of leaves; a workgroup owns one 16^3 block at a time and computes, in one march along z, what
-// the reference's assemble, predict and update passes compute for it (src/amr/assemble.wgsl,
-// predict.wgsl, update.wgsl), with no W, PA or PB atlas in HBM:
+// The fused block-AMR step for gfx950: two batched kernels over device lists of leaves compute
+// what the reference's assemble, predict and update passes compute (src/amr/assemble.wgsl,
+// predict.wgsl, update.wgsl), with no W, PA or PB atlas of the blocks' cells in HBM:
 //
-//  - the block's 20^3 texels are formed on load from U (same-level neighbours, boundary images,
-//    prolongation from a coarser leaf, restriction of refined children: amr_texel), each once;
-//  - the half step of every cell and of the first ghost layer: cells and ghosts beside a coarser
-//    block from the block's own texels (the latter with masked transverse slopes), ghosts whose
-//    state is another block's cell ("at source", amr/predict.wgsl:183-221) from that block's
-//    texels, formed directly;
+//  - k_amr_shell: per leaf, the ghost texels of its 20^3 region, formed from U (same-level
+//    neighbours, boundary images, prolongation from a coarser leaf, restriction of refined
+//    children: amr_texel), and the half steps of its first ghost layer where those are another
+//    block's cells ("at source", amr/predict.wgsl:183-221), all lanes busy;
+//  - k_amr_fused, one persistent kernel per launch: a workgroup owns one 16^3 block at a time
+//    and marches along z: its cells from U, its ghosts from the shell; the half step of every
+//    cell and of the ghosts beside a coarser block (masked transverse slopes), in LDS;
 //  - every face once; the conservative update of each cell; the next step's rate / dx;
 //  - coarse-fine faces through a flux register: a leaf writes the fluxes of its faces on a
 //    coarser block (as it computed them), and the coarser leaf, launched after it, reads the
@@ -17,11 +17,13 @@
 //    block's atlas. Launch order: leaves with no refined neighbour first, then the others level
 //    by level from the finest.
 //
-// Thread layout: 448 threads (7 waves) own the 400 columns of the block's 20 x 20 texel plane:
-// threads 0-255 the 16 x 16 cell columns, 256-319 the 64 face-ghost columns of ring 1,
-// 320-323 ring 1's corners, 324-399 ring 2, 400-447 idle. So the cell waves never diverge on
-// ghost work, and the one wave of face ghosts does the at-source half steps together. LDS holds
-// the plane data in raster order of the 20 x 20 columns (the uniform fused kernel's layout).
+// Thread layout: 384 threads (6 waves) own the columns of the block's 20 x 20 texel plane that
+// ever hold a value: threads 0-255 the 16 x 16 cell columns, 256-319 the 64 face-ghost columns
+// of ring 1, 320-383 the 64 of ring 2 (x or y = -2 or 17, the other in range). The 16 remaining
+// columns (both x and y outside the cells) are edges or corners at every plane: zero, as in the
+// atlas. So the cell waves never diverge on ghost work. LDS holds the plane data in raster order
+// of the 20 x 20 columns (the uniform fused kernel's layout). Six waves at up to 168 VGPRs keep
+// two workgroups per CU.
 #pragma once
 #include "amr_impl.hpp"
 
@@ -29,29 +31,18 @@ namespace shx {
 
 constexpr int AF_SIDE = AMR_PAD;              // 20 columns per side
 constexpr int AF_NT = AF_SIDE * AF_SIDE;      // 400 columns
-constexpr int AF_THREADS = 448;
+constexpr int AF_THREADS = 384;
+constexpr int AF_RING = 128;                  // ghost columns with a value: 64 of ring 1, 64 of ring 2
 
-// thread -> column (x, y) in block coordinates, [-2, 18); idle threads get x = y = -100
+// thread -> column (x, y) in block coordinates, [-2, 18)
 __device__ __forceinline__ void af_column(int t, int& x, int& y) {
   if (t < 256) { x = t & 15; y = t >> 4; return; }
-  t -= 256;
-  if (t < 64) {
-    const int side = t >> 4, i = t & 15;
-    x = side == 0 ? -1 : (side == 1 ? 16 : i);
-    y = side == 2 ? -1 : (side == 3 ? 16 : i);
-    return;
-  }
-  t -= 64;
-  if (t < 4) { x = (t & 1) ? 16 : -1; y = (t & 2) ? 16 : -1; return; }
-  t -= 4;
-  if (t < 20) { x = t - 2; y = -2; return; }
-  t -= 20;
-  if (t < 20) { x = t - 2; y = 17; return; }
-  t -= 20;
-  if (t < 18) { x = -2; y = t - 1; return; }
-  t -= 18;
-  if (t < 18) { x = 17; y = t - 1; return; }
-  x = y = -100;
+  const int ring = t < 320 ? 1 : 2;
+  t = (t - 256) & 63;
+  const int side = t >> 4, i = t & 15;
+  const int lo = -ring, hi = AMR_BS - 1 + ring;
+  x = side == 0 ? lo : (side == 1 ? hi : i);
+  y = side == 2 ? lo : (side == 3 ? hi : i);
 }
 
 // Assembled primitives of a texel of block `slot` (zero for edges and corners, as the atlas).
@@ -164,12 +155,83 @@ __device__ __forceinline__ Flux<T> boundary_face(const int* __restrict__ tab, co
   return F;
 }
 
-// Launch bound: seven waves per workgroup, at least two workgroups per CU in FP32 (128 VGPRs)
+template <typename T>
+__device__ __forceinline__ void scale_inc(Inc<T>& q, T m) {
+  q.rho *= m; q.vx *= m; q.vy *= m; q.vz *= m; q.p *= m; q.s *= m;
+}
+
+// ---------------- the ghost shell ----------------
+// Per leaf, what its march needs from outside its own cells, formed in one batched pass with
+// every lane busy: the ghost texels of its 20^3 region (the four ghost planes z = -2, -1, 16, 17
+// whole, then the 128 ghost columns with a value over the 16 cell planes) and the half steps of its first
+// ghost layer where they are another block's cells (64 face-ghost columns x 16 planes, then the
+// two z-face ghost planes). 32 bytes per item.
+constexpr int SH_TEXELS = 4 * AF_NT + AF_RING * AMR_BS;                     // 1600 + 128 * 16 = 3648
+constexpr int SH_HALVES = 64 * AMR_BS + 2 * AMR_BS * AMR_BS;                 // 1024 + 512 = 1536
+constexpr int SH_ITEMS = SH_TEXELS + SH_HALVES;                               // 5184
+constexpr int SH_CHUNKS = (SH_ITEMS + 255) / 256;
+
+__device__ __forceinline__ int ghost_plane_slot(int z) { return z == -2 ? 0 : (z == -1 ? 1 : (z == AMR_BS ? 2 : 3)); }
+
+template <typename T, int LIM, bool ISEN>
+__global__ __launch_bounds__(256) void k_amr_shell(Params<T> P, Levels<T> LV, const int* __restrict__ tab,
+                                                   const int* __restrict__ list, const T4<T>* __restrict__ U1,
+                                                   const T4<T>* __restrict__ U2, T4<T>* __restrict__ SH,
+                                                   const T* __restrict__ clk) {
+  const int li = blockIdx.x / SH_CHUNKS;
+  const int item = (blockIdx.x % SH_CHUNKS) * 256 + threadIdx.x;
+  if (item >= SH_ITEMS) return;
+  const int s = list[li];
+  T4<T>* out = SH + ((size_t)li * SH_ITEMS + item) * 2;
+  if (item < SH_TEXELS) {
+    int x, y, z;
+    if (item < 4 * AF_NT) {
+      const int pl = item / AF_NT, r = item % AF_NT;
+      z = pl == 0 ? -2 : (pl == 1 ? -1 : (pl == 2 ? AMR_BS : AMR_BS + 1));
+      x = r % AF_SIDE - 2;
+      y = r / AF_SIDE - 2;
+    } else {
+      const int g = item - 4 * AF_NT;
+      z = g / AF_RING;
+      af_column(256 + g % AF_RING, x, y);
+    }
+    const Prim<T> w = texel_of<T, LIM, true>(P, tab, U1, U2, s, x, y, z);
+    out[0] = v4<T>(w.rho, w.vx, w.vy, w.vz);
+    out[1] = v4<T>(w.p, w.s, w.cs, T(0));
+    return;
+  }
+  const int h = item - SH_TEXELS;
+  int x, y, z;
+  if (h < 64 * AMR_BS) {
+    af_column(256 + h / AMR_BS, x, y);
+    z = h % AMR_BS;
+  } else {
+    const int q = h - 64 * AMR_BS;
+    z = q < AMR_BS * AMR_BS ? -1 : AMR_BS;
+    x = (q % (AMR_BS * AMR_BS)) % AMR_BS;
+    y = (q % (AMR_BS * AMR_BS)) / AMR_BS;
+  }
+  const GSrc g = ghost_src(tab, s, x, y, z);
+  if (g.kind != 0) return;   // beside a coarser block: predicted in place by the march
+  Params<T> Pl = P;
+  Pl.dx = level_dx(P, bt(tab, g.slot, BT_LEVEL));
+  Half<T> hc = half_at_source<T, LIM, ISEN>(Pl, tab, U1, U2, g.slot, g.x, g.y, g.z, clk[0]);
+  hc.vx = bc_vel(hc.vx, g.fx, g.sx);
+  hc.vy = bc_vel(hc.vy, g.fy, g.sy);
+  hc.vz = bc_vel(hc.vz, g.fz, g.sz);
+  out[0] = v4<T>(hc.rho, hc.vx, hc.vy, hc.vz);
+  out[1] = v4<T>(hc.p, hc.s, hc.shock, hc.fo);
+}
+
+// ---------------- the march ----------------
+// Launch bound: six waves per workgroup, two workgroups per CU in FP32 (three waves per SIMD:
+// up to 168 VGPRs)
 template <typename T, int LIM, int RS, bool ISEN, bool COMP>
-__global__ __launch_bounds__(AF_THREADS, (sizeof(T) == 4 ? 4 : 1)) void k_amr_fused(
-    Params<T> P, Levels<T> LV, const int* __restrict__ tab, const int* __restrict__ list, int count,
-    const int* __restrict__ reg_of, T* __restrict__ R, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2,
-    T4<T>* __restrict__ O1, T4<T>* __restrict__ O2, const T* __restrict__ clk, unsigned long long* next_slot) {
+__global__ __launch_bounds__(AF_THREADS, (sizeof(T) == 4 ? 3 : 1)) void k_amr_fused(
+    Params<T> P, const int* __restrict__ tab, const int* __restrict__ list, int count,
+    const int* __restrict__ shell_of, const T4<T>* __restrict__ SH, const int* __restrict__ reg_of,
+    T* __restrict__ R, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1,
+    T4<T>* __restrict__ O2, const T* __restrict__ clk, unsigned long long* next_slot) {
   constexpr int NT = AF_NT, BX = AF_SIDE;
   using LY = FusedLayout<T, NT>;
   __shared__ T L[LY::SIZE];
@@ -177,36 +239,49 @@ __global__ __launch_bounds__(AF_THREADS, (sizeof(T) == 4 ? 4 : 1)) void k_amr_fu
   const int tid = threadIdx.x;
   int x, y;
   af_column(tid, x, y);
-  const bool valid = x > -100;
-  const int t = valid ? (y + 2) * BX + (x + 2) : 0;    // raster index of the column in LDS
+  const int t = (y + 2) * BX + (x + 2);                  // raster index of the column in LDS
+  // the columns no thread owns are edges at every plane: zero in both plane slots, for good
+  for (int i = tid; i < 2 * 7 * NT; i += AF_THREADS) L[i] = T(0);
   const bool ox = x < 0 || x >= AMR_BS, oy = y < 0 || y >= AMR_BS;
-  const bool cell_col = valid && !ox && !oy;             // a 16 x 16 cell column
-  const bool r1 = valid && x >= -1 && x <= AMR_BS && y >= -1 && y <= AMR_BS;
-  const bool face_col = r1 && (ox != oy);                // ring-1 face ghost column
+  const bool cell_col = !ox && !oy;             // a 16 x 16 cell column (tid < 256)
+  const bool r1 = x >= -1 && x <= AMR_BS && y >= -1 && y <= AMR_BS;
+  const bool face_col = r1 && (ox != oy);                // ring-1 face ghost column (tid 256..319)
   // x faces below columns x = 0..16 and y faces below y = 0..16, within the cell rows
-  const bool xface = valid && x >= 0 && x <= AMR_BS && !oy;
-  const bool yface = valid && y >= 0 && y <= AMR_BS && !ox;
+  const bool xface = x >= 0 && x <= AMR_BS && !oy;
+  const bool yface = y >= 0 && y <= AMR_BS && !ox;
   const T dt = clk[0];
+  const T dx0 = P.dx;
   T rate_max = T(0);
 
   for (int wi = blockIdx.x; wi < count; wi += gridDim.x) {
     const int s = list[wi];
-    const T dx = LV.dx[bt(tab, s, BT_LEVEL)];
-    Params<T> Pl = P;
-    Pl.dx = dx;
+    const T4<T>* sh = SH + (size_t)shell_of[s] * SH_ITEMS * 2;
+    // this block's cell size (P is the kernel's own copy: its dx is the block's from here on)
+    const T dx = dx0 * (T(1) / T(1 << bt(tab, s, BT_LEVEL)));
+    P.dx = dx;
     const T dtdx = dt / dx;
     // a face-ghost column's source (constant over the cell planes)
     int gk = 0;
     if (face_col) gk = ghost_src(tab, s, x, y, 0).kind;
+    const int kz_lo = bt(tab, s, BT_NB + 4) == -1 ? 1 : 0, kz_hi = bt(tab, s, BT_NB + 5) == -1 ? 1 : 0;
     __syncthreads();   // the previous block's last LDS reads are done
 
-    auto texel = [&](int z) { return texel_of<T, LIM, true>(P, tab, U1, U2, s, x, y, z); };
-    if (valid) {
-      lds_put_prim<T, NT>(L + LY::Q0, t, texel(-2));
-      lds_put_prim<T, NT>(L + LY::Q1, t, texel(-1));
-    }
-    Prim<T> nxt;
-    if (valid) nxt = texel(0);
+    // texel (x, y, z) of this block: its own cells from U, ghosts from the shell
+    auto texel = [&](int z) -> Prim<T> {
+      Prim<T> w;
+      if (cell_col && z >= 0 && z < AMR_BS) {
+        const size_t o = (size_t)s * AMR_CELLS + cel(x, y, z);
+        return prim_from_cons(P, U1[o], U2[o]);
+      }
+      const int g = (z < 0 || z >= AMR_BS) ? ghost_plane_slot(z) * AF_NT + t : 4 * AF_NT + z * AF_RING + (tid - 256);
+      const T4<T> a = sh[2 * g], b = sh[2 * g + 1];
+      w.rho = a.x; w.vx = a.y; w.vy = a.z; w.vz = a.w;
+      w.p = b.x; w.s = b.y; w.cs = b.z;
+      return w;
+    };
+    lds_put_prim<T, NT>(L + LY::Q0, t, texel(-2));
+    lds_put_prim<T, NT>(L + LY::Q1, t, texel(-1));
+    Prim<T> nxt = texel(0);
 
     SideData<T> zs_prev;
     Flux<T> fz_lo = zero_flux<T>();
@@ -214,7 +289,7 @@ __global__ __launch_bounds__(AF_THREADS, (sizeof(T) == 4 ? 4 : 1)) void k_amr_fu
 
     for (int c = -1; c <= AMR_BS; c++) {
       const Prim<T> pp = nxt;             // plane c + 1
-      if (valid && c + 2 <= AMR_BS + 1) nxt = texel(c + 2);
+      if (c + 2 <= AMR_BS + 1) nxt = texel(c + 2);
       const bool odd = ((c + 1) & 1) != 0;
       const T* Lq = L + (odd ? LY::Q0 : LY::Q1);
       T* Lm = L + (odd ? LY::Q1 : LY::Q0);
@@ -231,38 +306,34 @@ __global__ __launch_bounds__(AF_THREADS, (sizeof(T) == 4 ? 4 : 1)) void k_amr_fu
         const Prim<T> ym = lds_prim<T, NT>(Lq, t - BX), yp = lds_prim<T, NT>(Lq, t + BX);
         Inc<T> ixu, ixd, iyu, iyd, izu, izd;
         AxisSlope<T> sl[3];
-        sl[0] = axis_slope<T, LIM, ISEN>(Pl, xm, pc, xp, &ixu, &ixd);
-        sl[1] = axis_slope<T, LIM, ISEN>(Pl, ym, pc, yp, &iyu, &iyd);
-        sl[2] = axis_slope<T, LIM, ISEN>(Pl, pm, pc, pp, &izu, &izd);
+        sl[0] = axis_slope<T, LIM, ISEN>(P, xm, pc, xp, &ixu, &ixd);
+        sl[1] = axis_slope<T, LIM, ISEN>(P, ym, pc, yp, &iyu, &iyd);
+        sl[2] = axis_slope<T, LIM, ISEN>(P, pm, pc, pp, &izu, &izd);
         // the ghost's source: a face-ghost column's, or a ghost plane's (c = -1 or 16)
         int kind = 0;
-        if (nout == 1) kind = oz ? (bt(tab, s, BT_NB + (c < 0 ? 4 : 5)) == -1 ? 1 : 0) : gk;
+        if (nout == 1) kind = oz ? (c < 0 ? kz_lo : kz_hi) : gk;
         Half<T> hc;
         if (nout == 0 || kind == 1) {
           if (nout == 1) {
             // beside a coarser block: predicted in place, transverse slopes dropped at the rim
             // where their neighbour is an unassembled edge texel (amr/predict.wgsl:183-221)
-            const int cx = x, cy = y, cz = c;
-            const T mx = (!ox && (cx == 0 || cx == AMR_BS - 1)) ? T(0) : T(1);
-            const T my = (!oy && (cy == 0 || cy == AMR_BS - 1)) ? T(0) : T(1);
-            const T mz = (!oz && (cz == 0 || cz == AMR_BS - 1)) ? T(0) : T(1);
-            const T mk[3] = {mx, my, mz};
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-              Inc<T>& q = sl[a].s;
-              q.rho *= mk[a]; q.vx *= mk[a]; q.vy *= mk[a]; q.vz *= mk[a]; q.p *= mk[a]; q.s *= mk[a];
-            }
+            const T mx = (!ox && (x == 0 || x == AMR_BS - 1)) ? T(0) : T(1);
+            const T my = (!oy && (y == 0 || y == AMR_BS - 1)) ? T(0) : T(1);
+            const T mz = (!oz && (c == 0 || c == AMR_BS - 1)) ? T(0) : T(1);
+            scale_inc(sl[0].s, mx);
+            scale_inc(sl[1].s, my);
+            scale_inc(sl[2].s, mz);
           }
           const T conv = (xp.vx - xm.vx) + (yp.vy - ym.vy) + (pp.vz - pm.vz);
           const T jump = tmax(jump_of(xm.p, xp.p), tmax(jump_of(ym.p, yp.p), jump_of(pm.p, pp.p)));
-          hc = hancock(Pl, pc, sl, conv, jump, dt);
+          hc = hancock(P, pc, sl, conv, jump, dt);
         } else {
-          // another block's cell (same level, refined, or this block's boundary image)
-          const GSrc g = ghost_src(tab, s, x, y, c);
-          hc = half_at_source<T, LIM, ISEN>(Pl, tab, U1, U2, g.slot, g.x, g.y, g.z, dt);
-          hc.vx = bc_vel(hc.vx, g.fx, g.sx);
-          hc.vy = bc_vel(hc.vy, g.fy, g.sy);
-          hc.vz = bc_vel(hc.vz, g.fz, g.sz);
+          // another block's cell (same level, refined, or this block's boundary image): its half
+          // step was formed in the shell pass
+          const int h = oz ? 64 * AMR_BS + (c < 0 ? 0 : AMR_BS * AMR_BS) + tid : (tid - 256) * AMR_BS + c;
+          const T4<T> a = sh[2 * (SH_TEXELS + h)], b = sh[2 * (SH_TEXELS + h) + 1];
+          hc.rho = a.x; hc.vx = a.y; hc.vy = a.z; hc.vz = a.w;
+          hc.p = b.x; hc.s = b.y; hc.shock = b.z; hc.fo = b.w;
         }
         sxl = side_of(hc, ixd, T(-1));
         syl = side_of(hc, iyd, T(-1));
@@ -272,24 +343,24 @@ __global__ __launch_bounds__(AF_THREADS, (sizeof(T) == 4 ? 4 : 1)) void k_amr_fu
         lds_put_side<T, NT>(L, LY::SY, t, side_of(hc, iyu, T(1)));
       }
       // plane c + 1 into the slot plane c - 1 leaves
-      if (valid) lds_put_prim<T, NT>(Lm, t, pp);
+      lds_put_prim<T, NT>(Lm, t, pp);
       __syncthreads();
 
       // 3. the x and y faces below this column, and the z face below plane c
       const bool live = c >= 0 && c < AMR_BS;
       Flux<T> Fx = zero_flux<T>(), Fy = zero_flux<T>(), fz = zero_flux<T>();
       if (live && xface) {
-        Fx = face_from_sides<T, RS>(Pl, lds_get_side<T, NT>(L, LY::SX, t - 1), sxl, 0);
+        Fx = face_from_sides<T, RS>(P, lds_get_side<T, NT>(L, LY::SX, t - 1), sxl, 0);
         if (x == 0) Fx = boundary_face<T, COMP>(tab, reg_of, R, s, 0, x, y, c, c * 16 + y, Fx);
         else if (x == AMR_BS) Fx = boundary_face<T, COMP>(tab, reg_of, R, s, 1, x - 1, y, c, c * 16 + y, Fx);
       }
       if (live && yface) {
-        Fy = face_from_sides<T, RS>(Pl, lds_get_side<T, NT>(L, LY::SY, t - BX), syl, 1);
+        Fy = face_from_sides<T, RS>(P, lds_get_side<T, NT>(L, LY::SY, t - BX), syl, 1);
         if (y == 0) Fy = boundary_face<T, COMP>(tab, reg_of, R, s, 2, x, y, c, x * 16 + c, Fy);
         else if (y == AMR_BS) Fy = boundary_face<T, COMP>(tab, reg_of, R, s, 3, x, y - 1, c, x * 16 + c, Fy);
       }
       if (cell_col && c >= 0) {
-        fz = face_from_sides<T, RS>(Pl, zs_prev, szl, 2);
+        fz = face_from_sides<T, RS>(P, zs_prev, szl, 2);
         if (c == 0) fz = boundary_face<T, COMP>(tab, reg_of, R, s, 4, x, y, c, y * 16 + x, fz);
         else if (c == AMR_BS) fz = boundary_face<T, COMP>(tab, reg_of, R, s, 5, x, y, c - 1, y * 16 + x, fz);
       }
@@ -325,7 +396,7 @@ __global__ __launch_bounds__(AF_THREADS, (sizeof(T) == 4 ? 4 : 1)) void k_amr_fu
         net.s = axy_prev.s + (fz_lo.s - fz.s);
         net.vn = axy_prev.vn + (fz_lo.vn - fz.vn);
         T4<T> o1, o2;
-        advance(Pl, U1[o], U2[o], net, zs_prev.hp, dtdx, o1, o2);
+        advance(P, U1[o], U2[o], net, zs_prev.hp, dtdx, o1, o2);
         O1[o] = o1;
         O2[o] = o2;
         // the next step's rate / dx (assemble's leaf-cell rate, amr/assemble.wgsl:77-154)
@@ -348,15 +419,24 @@ __global__ __launch_bounds__(AF_THREADS, (sizeof(T) == 4 ? 4 : 1)) void k_amr_fu
 }
 
 template <typename T, int LIM, int RS, bool ISEN>
+void AmrImpl<T, LIM, RS, ISEN>::shell(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2,
+                                      void* sh, const void* clk) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL((k_amr_shell<T, LIM, ISEN>), dim3(count * SH_CHUNKS), dim3(256), 0, L.stream, P(L), LV(L), L.tab,
+                     list, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)sh, (const T*)clk);
+}
+
+template <typename T, int LIM, int RS, bool ISEN>
 void AmrImpl<T, LIM, RS, ISEN>::fused(const AmrLaunch& L, const int* list, int count, int grid, bool comp,
-                                      const int* reg_of, void* reg, const void* u1, const void* u2, void* o1, void* o2,
-                                      const void* clk, unsigned long long* slot) {
+                                      const int* shell_of, const void* sh, const int* reg_of, void* reg,
+                                      const void* u1, const void* u2, void* o1, void* o2, const void* clk,
+                                      unsigned long long* slot) {
   if (count <= 0) return;
   if (grid > count) grid = count;
 #define SHX_AF(C)                                                                                                  \
-  hipLaunchKernelGGL((k_amr_fused<T, LIM, RS, ISEN, C>), dim3(grid), dim3(AF_THREADS), 0, L.stream, P(L), LV(L),    \
-                     L.tab, list, count, reg_of, (T*)reg, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)o1,            \
-                     (T4<T>*)o2, (const T*)clk, slot)
+  hipLaunchKernelGGL((k_amr_fused<T, LIM, RS, ISEN, C>), dim3(grid), dim3(AF_THREADS), 0, L.stream, P(L), L.tab,   \
+                     list, count, shell_of, (const T4<T>*)sh, reg_of, (T*)reg, (const T4<T>*)u1,                    \
+                     (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, slot)
   if (comp) SHX_AF(true);
   else SHX_AF(false);
 #undef SHX_AF

@@ -24,6 +24,13 @@ __device__ __forceinline__ int tex(int x, int y, int z) {
   return ((z + AMR_NG) * AMR_PAD + (y + AMR_NG)) * AMR_PAD + (x + AMR_NG);
 }
 __device__ __forceinline__ int cel(int x, int y, int z) { return (z * AMR_BS + y) * AMR_BS + x; }
+// The cell size of level L: the level-0 size times 2^-L, exact (the same value as rounding
+// dx0 / 2^L once); formed arithmetically, as an indexed kernel-argument array would be copied
+// to scratch memory
+template <typename T>
+__device__ __forceinline__ T level_dx(const Params<T>& P, int L) {
+  return P.dx * (T(1) / T(1 << L));
+}
 
 // ghost_src (amr/grid.wgsl:184-222): where texel (x, y, z) of block `slot` takes its state.
 // kind 0: cell (x, y, z) of block `slot` (itself, a same-level neighbour or a refined one, or the
@@ -257,7 +264,7 @@ __global__ __launch_bounds__(256) void k_amr_assemble(Params<T> P, Levels<T> LV,
       W1[o] = v4<T>(w.rho, w.vx, w.vy, w.vz);
       W2[o] = v4<T>(w.p, w.cs, w.s, T(0));
       const bool inside = x >= 0 && y >= 0 && z >= 0 && x < AMR_BS && y < AMR_BS && z < AMR_BS;
-      if (inside && is_leaf(tab, s)) r = rate_of(P, w) / LV.dx[bt(tab, s, BT_LEVEL)];
+      if (inside && is_leaf(tab, s)) r = rate_of(P, w) / level_dx(P, bt(tab, s, BT_LEVEL));
     }
   }
   block_max_to(r, slot);
@@ -269,7 +276,7 @@ __global__ __launch_bounds__(256) void k_amr_rates(Params<T> P, Levels<T> LV, co
                                                    const int* __restrict__ list, const T4<T>* __restrict__ U1,
                                                    const T4<T>* __restrict__ U2, unsigned long long* slot) {
   const int s = list[blockIdx.x];
-  const T dx = LV.dx[bt(tab, s, BT_LEVEL)];
+  const T dx = level_dx(P, bt(tab, s, BT_LEVEL));
   T r = T(0);
   for (int c = threadIdx.x; c < AMR_CELLS; c += 256) {
     const size_t o = (size_t)s * AMR_CELLS + c;
@@ -335,7 +342,7 @@ __global__ __launch_bounds__(256) void k_amr_predict(Params<T> P, Levels<T> LV, 
   const bool at = g.kind == 0;
   const int src = at ? g.slot : s;
   Params<T> Pl = P;
-  Pl.dx = LV.dx[bt(tab, src, BT_LEVEL)];
+  Pl.dx = level_dx(P, bt(tab, src, BT_LEVEL));
   const T mx = (!at && (x == 0 || x == AMR_BS - 1)) ? T(0) : T(1);
   const T my = (!at && (y == 0 || y == AMR_BS - 1)) ? T(0) : T(1);
   const T mz = (!at && (z == 0 || z == AMR_BS - 1)) ? T(0) : T(1);
@@ -496,7 +503,7 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
     }
   }
   const Flux<T> net = net_of(xl, xh, yl, yh, zl, zh);
-  const T dx = LV.dx[bt(tab, s, BT_LEVEL)];
+  const T dx = level_dx(P, bt(tab, s, BT_LEVEL));
   Params<T> Pl = P;
   Pl.dx = dx;
   const size_t o = (size_t)s * AMR_CELLS + cel(x, y, z);
@@ -601,10 +608,14 @@ struct AmrImpl {
                        (const T4<T>*)u1, (const T4<T>*)u2, part);
     hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(64), 0, L.stream, (const double*)part, count, out);
   }
-  static void fused(const AmrLaunch& L, const int* list, int count, int grid, bool comp, const int* reg_of, void* reg,
-                    const void* u1, const void* u2, void* o1, void* o2, const void* clk, unsigned long long* slot);
+  static void shell(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2, void* sh,
+                    const void* clk);
+  static void fused(const AmrLaunch& L, const int* list, int count, int grid, bool comp, const int* shell_of,
+                    const void* sh, const int* reg_of, void* reg, const void* u1, const void* u2, void* o1, void* o2,
+                    const void* clk, unsigned long long* slot);
   static AmrOps ops() {
     AmrOps o;
+    o.shell = shell;
     o.fused = fused;
     o.assemble = assemble;
     o.clock = clock;

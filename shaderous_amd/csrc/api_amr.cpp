@@ -43,7 +43,10 @@ struct shx_amr {
   void* U1[2] = {nullptr, nullptr};
   void* U2[2] = {nullptr, nullptr};
   void *W1 = nullptr, *W2 = nullptr, *PA = nullptr, *PB = nullptr;
-  // fused path: flux-register index per (slot, face) and the register
+  // fused path: ghost shells (one per leaf, in leaf-list order) and each slot's shell index;
+  // flux-register index per (slot, face) and the register
+  void* shell = nullptr;
+  int* d_shell_of = nullptr;
   int* d_reg_of = nullptr;
   void* reg = nullptr;
   int nreg = 0;
@@ -208,11 +211,12 @@ int enqueue_step(shx_amr* s, int p, int stage = -1) {
   auto run = [&](int g) { return stage < 0 || stage == g; };
   if (s->lay.path == SHX_AMR_PATH_FUSED) {
     if (run(1)) { o.clock(L, s->clk, s->slot); n++; }
+    if (run(2)) { o.shell(L, lp(s, s->leaf), s->leaf.count, s->U1[p], s->U2[p], s->shell, s->clk); n++; }
     if (run(3)) {
       auto fz = [&](const Range& r) {
         if (!r.count) return;
-        o.fused(L, lp(s, r), r.count, s->grid, comp, s->d_reg_of, s->reg, s->U1[p], s->U2[p], s->U1[1 - p],
-                s->U2[1 - p], s->clk, s->slot);
+        o.fused(L, lp(s, r), r.count, s->grid, comp, s->d_shell_of, s->shell, s->d_reg_of, s->reg, s->U1[p],
+                s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, s->slot);
         n++;
       };
       fz(s->plain);
@@ -376,7 +380,7 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess || cus < 1) cus = 256;
-    // FP32: two 7-wave workgroups per CU (128 VGPRs, 51 KB of LDS each); FP64: one
+    // FP32: two 6-wave workgroups per CU (51 KB of LDS each); FP64: one
     s->grid = (s->es == 8 ? 1 : 2) * cus;
   }
   const size_t ubytes = s->ncell * 4 * s->es;
@@ -387,6 +391,11 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   if (!fused) ok = ok && alloc(&s->W1, abytes) && alloc(&s->W2, abytes) && alloc(&s->PA, abytes) && alloc(&s->PB, abytes);
   ok = ok && alloc((void**)&s->d_reg_of, reg_of.size() * sizeof(int)) &&
        alloc(&s->reg, (size_t)(s->nreg > 0 ? s->nreg : 1) * 256 * 8 * s->es);
+  // ghost shells: 5184 items of 8 values per leaf (the fused path)
+  std::vector<int> shell_of(nblocks, -1);
+  for (int i = 0; i < s->leaf.count; i++) shell_of[lists[s->leaf.off + i]] = i;
+  if (fused) ok = ok && alloc(&s->shell, (size_t)s->leaf.count * 5184 * 8 * s->es);
+  ok = ok && alloc((void**)&s->d_shell_of, shell_of.size() * sizeof(int));
   ok = ok && alloc((void**)&s->d_tab, s->tab.size() * sizeof(int)) && alloc((void**)&s->d_lists, lists.size() * sizeof(int));
   ok = ok && alloc(&s->clk, 16 * 8) && alloc((void**)&s->slot, 64) && alloc((void**)&s->diag_part, (size_t)nblocks * 16 * 8) &&
        alloc((void**)&s->diag_out, 16 * 8);
@@ -403,6 +412,7 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
                 hipMemset(s->clk, 0, 16 * 8) == hipSuccess && hipMemset(s->slot, 0, 64) == hipSuccess;
   bool copied = hipMemcpy(s->d_tab, s->tab.data(), s->tab.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
                 hipMemcpy(s->d_reg_of, reg_of.data(), reg_of.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+                hipMemcpy(s->d_shell_of, shell_of.data(), shell_of.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
                 hipMemcpy(s->d_lists, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
   if (!zeroed || !copied || hipDeviceSynchronize() != hipSuccess) {
     shx_amr_destroy(s);
@@ -420,7 +430,7 @@ int shx_amr_destroy(shx_amr* s) {
   destroy_graphs(s);
   for (int p = 0; p < 2; p++) { (void)hipFree(s->U1[p]); (void)hipFree(s->U2[p]); }
   (void)hipFree(s->W1); (void)hipFree(s->W2); (void)hipFree(s->PA); (void)hipFree(s->PB);
-  (void)hipFree(s->d_reg_of); (void)hipFree(s->reg);
+  (void)hipFree(s->d_reg_of); (void)hipFree(s->reg); (void)hipFree(s->shell); (void)hipFree(s->d_shell_of);
   (void)hipFree(s->d_tab); (void)hipFree(s->d_lists); (void)hipFree(s->clk); (void)hipFree(s->slot);
   (void)hipFree(s->diag_part); (void)hipFree(s->diag_out);
   if (s->stream) (void)hipStreamDestroy(s->stream);

@@ -87,6 +87,11 @@ function toNative(cfg) {
   };
 }
 
+function live(h) {
+  if (!h) throw new Error('the simulation was destroyed');
+  return h;
+}
+
 /** diagnosticsFrom (src/sim.ts:1647-1673): the Diagnostics record, sums in FP64. */
 function diagnosticsFrom(r, clock) {
   return {
@@ -121,49 +126,49 @@ class Sim {
   }
 
   setState(u1, u2, time = 0, steps = 0) {
-    native.setState(this.h, u1, u2, time, steps);
+    native.setState(live(this.h), u1, u2, time, steps);
   }
 
   readState() {
-    return native.readState(this.h);
+    return native.readState(live(this.h));
   }
 
   readStateAsync() {
-    return native.readStateAsync(this.h);
+    return native.readStateAsync(live(this.h));
   }
 
   step(count = 1) {
-    native.step(this.h, count);
+    native.step(live(this.h), count);
   }
 
   submitted() {
-    native.sync(this.h);
+    native.sync(live(this.h));
   }
 
   runToEnd(batch = 64, maxSteps = 1 << 30) {
-    return native.runToEnd(this.h, batch, maxSteps);
+    return native.runToEnd(live(this.h), batch, maxSteps);
   }
 
   clock() {
-    return native.readClock(this.h);
+    return native.readClock(live(this.h));
   }
 
   diagnostics() {
-    return diagnosticsFrom(native.diagnostics(this.h), native.readClock(this.h));
+    return diagnosticsFrom(native.diagnostics(live(this.h)), native.readClock(live(this.h)));
   }
 
   async readDiagnostics() {
-    const r = await native.diagnosticsAsync(this.h);
-    return diagnosticsFrom(r, native.readClock(this.h));
+    const r = await native.diagnosticsAsync(live(this.h));
+    return diagnosticsFrom(r, native.readClock(live(this.h)));
   }
 
   fluxMismatch() {
-    const v = native.fluxMismatch(this.h);
+    const v = native.fluxMismatch(live(this.h));
     return { mismatched: v[0], faces: v[1] };
   }
 
   profile(steps = 10) {
-    const ms = native.profile(this.h, steps);
+    const ms = native.profile(live(this.h), steps);
     return { rates: ms[0], clock: ms[1], predict: ms[2], update: ms[3], step: ms[4] };
   }
 
@@ -193,44 +198,44 @@ class AmrSim {
   }
 
   table() {
-    return native.amrTable(this.h);
+    return native.amrTable(live(this.h));
   }
 
   stats() {
-    const v = native.amrStats(this.h);
+    const v = native.amrStats(live(this.h));
     return { blocks: v[0], leaves: v[1], matchLeaves: v[2], levels: v[3], launchesPerStep: v[4] };
   }
 
   setState(u1, u2, time = 0, steps = 0) {
-    native.amrSetState(this.h, u1, u2, time, steps);
+    native.amrSetState(live(this.h), u1, u2, time, steps);
   }
 
   readState() {
-    return native.amrReadState(this.h);
+    return native.amrReadState(live(this.h));
   }
 
   readStateAsync() {
-    return native.readStateAsync(this.h);
+    return native.readStateAsync(live(this.h));
   }
 
   step(count = 1) {
-    native.amrStep(this.h, count);
+    native.amrStep(live(this.h), count);
   }
 
   submitted() {
-    native.amrSync(this.h);
+    native.amrSync(live(this.h));
   }
 
   clock() {
-    return native.amrReadClock(this.h);
+    return native.amrReadClock(live(this.h));
   }
 
   diagnostics() {
-    return diagnosticsFrom(native.amrDiagnostics(this.h), native.amrReadClock(this.h));
+    return diagnosticsFrom(native.amrDiagnostics(live(this.h)), native.amrReadClock(live(this.h)));
   }
 
   profile(steps = 5) {
-    const ms = native.amrProfile(this.h, steps);
+    const ms = native.amrProfile(live(this.h), steps);
     return { assemble: ms[0], clock: ms[1], predict: ms[2], update: ms[3], step: ms[4] };
   }
 

@@ -221,3 +221,21 @@ def ensemble_sizes(count: int = 64, seed: int = 42, choices=(64, 128, 256, 512))
     """Seeded grid sizes for the uneven-size ensemble."""
     rng = np.random.default_rng(seed)
     return [int(c) for c in rng.choice(np.asarray(choices), size=count)]
+
+
+def amr_blast(seed: int = 7, root: int = 8, max_level: int = 3, radius: float = 0.1, p_in: float = 100.0,
+              p_out: float = 1.0, gamma: float = 1.4):
+    """The block-AMR blast workload: the unit box, root^3 blocks of 16^3 cells, up to `max_level`
+    refinements around a sphere of `radius` at a seeded offset centre (uniform in +-0.05 per axis)
+    holding P = p_in against rho = 1, P = p_out. The initial density is uniform, so the refinement
+    indicator (a relative jump above 0.3 across a block) is evaluated on the pressure: blocks
+    straddling the sphere, grown by half a block (amr_tree.blast_tree). Returns (blocks, dx0,
+    prim_fn, centre)."""
+    from .amr_tree import blast_tree
+    c = np.random.default_rng(seed).uniform(-0.05, 0.05, 3)
+    blocks, dx0 = blast_tree(root=root, max_level=max_level, radius=radius, center=tuple(c), p_in=p_in, p_out=p_out)
+
+    def prims(x, y, z):
+        r2 = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2
+        return {"rho": np.ones_like(x), "p": np.where(r2 < radius * radius, p_in, p_out)}
+    return blocks, dx0, prims, c
