@@ -31,6 +31,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 BYTES_PER_UPDATE = 64      # read 8 FP32 conserved values, write 8
+# HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (scaled by the counters' reading of a 512 MiB device copy: 2.0 and 1.0), for the
+# configuration they were measured on: profiles/k_fused_turbulence_256.md
+TRAFFIC_MEASURED = {("k_fused", 256, (28, 12, 32)): (1801676 + 524343) * 1024}
 
 
 def log(msg):
@@ -224,7 +228,7 @@ def main():
                                f"MUSCL-Hancock, CFL 0.4", "cells": ncell, "path": "fused one-pass",
                    "tile": tile},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": TRAFFIC_MEASURED.get(("k_fused", n, tuple(tile or ()))),
                      "kernel": "k_fused", "kernel_ms": kernel_ms, "bytes_per_update": BYTES_PER_UPDATE},
         "plain": {"value": plain_value, "ms_per_step": tp / args.steps * 1e3 if tp else None, "passes_ms": pprof},
         "fused_over_plain": value / plain_value if plain_value else None,
