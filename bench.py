@@ -120,6 +120,43 @@ def amr_bench(args):
     return out
 
 
+def ensemble_bench(args):
+    """64 seeded 2D four-quadrant Riemann problems, sizes drawn from {64, 128, 256, 512}^2, gamma
+    1.4, outflow, MC + HLLC, CFL 0.4, each with its own dt, advanced together for `ens_steps`
+    steps by one persistent launch per step (autotuned tile)."""
+    from shaderous_amd import problems
+    from shaderous_amd.ensemble import Ensemble
+    from shaderous_amd.sim import SimConfig
+    ns = problems.ensemble_sizes(64, seed=42)
+    cells = sum(n * n for n in ns)
+    cfg = SimConfig(n=(64, 64, 1), ndim=2, dx=1.0 / 64, gamma=1.4, bc_lo=(1, 1, 0), bc_hi=(1, 1, 0), limiter="mc",
+                    isentropic=False, cfl=0.4)
+    ens = Ensemble(cfg, [(n, n) for n in ns])
+    try:
+        for i, n in enumerate(ns):
+            ens.set_state(i, *problems.riemann_quadrants(n, seed=1000 + i).state())
+        tile = ens.autotune(reps=3)
+        ens.step(5)
+        ens.sync()
+        t0 = time.perf_counter()
+        ens.step(args.ens_steps)
+        ens.sync()
+        t = time.perf_counter() - t0
+        prof = ens.profile(5)
+    finally:
+        ens.destroy()
+    value = cells * args.ens_steps / t
+    log(f"[bench] ensemble: 64 instances, {cells} cells, tile {tile}: {t / args.ens_steps * 1e3:.3f} ms/step, "
+        f"{value:.3e} upd/s, {prof}")
+    ach = BYTES_PER_UPDATE * cells / (prof["fused"] * 1e-3)
+    return {"workload": "64 2D four-quadrant Riemann problems, sizes from {64,128,256,512}^2 (seed 42), gamma 1.4, "
+                        "outflow, PLM (MC) + HLLC, CFL 0.4, per-instance dt",
+            "cells": cells, "steps": args.ens_steps, "value": value, "ms_per_step": t / args.ens_steps * 1e3,
+            "tile": tile, "passes_ms": prof,
+            "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK, "kernel": "k_ens_fused", "kernel_ms": prof["fused"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +168,7 @@ def main():
     ap.add_argument("--paths", default="fused,plain", help="which uniform-grid paths to time")
     ap.add_argument("--amr-steps", type=int, default=20, help="timed steps of the AMR blast (0: skip it)")
     ap.add_argument("--amr-paths", default="fused,levels", help="AMR paths to time (fused, passes, levels)")
+    ap.add_argument("--ens-steps", type=int, default=200, help="steps of the 2D ensemble (0: skip it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a device copy of MIB MiB three times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -200,9 +238,12 @@ def main():
     amr = None
     if args.amr_steps > 0:
         amr = amr_bench(args)
+    ens = None
+    if args.ens_steps > 0:
+        ens = ensemble_bench(args)
     if value is None:
-        if amr:
-            print(json.dumps({"amr": amr}))
+        if amr or ens:
+            print(json.dumps({"amr": amr, "ensemble": ens}))
         return
 
     cpu = None
@@ -235,6 +276,7 @@ def main():
         "conserved": {"mass": diag.mass, "e_tot": diag.e_tot},
         "cpu_baseline": cpu,
         "amr": amr,
+        "ensemble": ens,
     }
     print(json.dumps(out))
 

@@ -171,6 +171,30 @@ int shx_amr_profile(shx_amr* sim, int steps, double ms[5]);
  * [4] kernel launches per step. */
 int shx_amr_stats(shx_amr* sim, int out[5]);
 
+/* ---------------- ensembles of independent 2D grids ----------------
+ * The reference runs one Sim per device (src/sim.ts); an ensemble advances `count` independent
+ * 2D uniform grids, each its own size and clock (dt per instance), in one persistent launch per
+ * step over a device list of column tiles from every instance. Each instance is the run a Sim
+ * with the same configuration would make (the plain path's arithmetic); cfg gives what they
+ * share (limiter, solver, gamma, cfl, boundaries, floors, precision); sizes[2i], sizes[2i+1] are
+ * instance i's nx, ny on the unit box (dx = 1 / nx). */
+typedef struct shx_ens shx_ens;
+int shx_ens_create(const shx_config* cfg, int count, const int* sizes, shx_ens** out);
+int shx_ens_destroy(shx_ens* ens);
+/* Instance i's state, nx*ny cells of u1 and u2 in the Sim layout; resets its clock. */
+int shx_ens_set_state(shx_ens* ens, int i, const float* u1, const float* u2, size_t ncell);
+int shx_ens_set_state_f64(shx_ens* ens, int i, const double* u1, const double* u2, size_t ncell);
+int shx_ens_read_state(shx_ens* ens, int i, float* u1, float* u2);
+int shx_ens_read_state_f64(shx_ens* ens, int i, double* u1, double* u2);
+/* `count` steps of every instance (graph launches). */
+int shx_ens_step(shx_ens* ens, int count);
+int shx_ens_sync(shx_ens* ens);
+int shx_ens_read_clock(shx_ens* ens, int i, double out[11]);
+/* ms[0] clock pass, ms[1] the batched kernel, ms[2] whole step (HIP events, averaged). */
+int shx_ens_profile(shx_ens* ens, int steps, double ms[3]);
+/* Time the tile candidates on the current states (which it restores); chosen = (tx, ty). */
+int shx_ens_autotune(shx_ens* ens, int reps, int chosen[2]);
+
 /* The calling thread's last error message ("" if none). */
 const char* shx_last_error(void);
 /* Library version and the device it sees: "shx <ver> gfx950 <name>" */

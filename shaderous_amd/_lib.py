@@ -88,6 +88,17 @@ def load():
         "shx_amr_diagnostics": (I, [P, D]),
         "shx_amr_profile": (I, [P, I, D]),
         "shx_amr_stats": (I, [P, C.POINTER(I)]),
+        "shx_ens_create": (I, [C.POINTER(ShxConfig), I, C.POINTER(I), C.POINTER(P)]),
+        "shx_ens_destroy": (I, [P]),
+        "shx_ens_set_state": (I, [P, I, F, F, C.c_size_t]),
+        "shx_ens_set_state_f64": (I, [P, I, D, D, C.c_size_t]),
+        "shx_ens_read_state": (I, [P, I, F, F]),
+        "shx_ens_read_state_f64": (I, [P, I, D, D]),
+        "shx_ens_step": (I, [P, I]),
+        "shx_ens_sync": (I, [P]),
+        "shx_ens_read_clock": (I, [P, I, D]),
+        "shx_ens_profile": (I, [P, I, D]),
+        "shx_ens_autotune": (I, [P, I, C.POINTER(I)]),
         "shx_last_error": (C.c_char_p, []),
         "shx_version": (C.c_char_p, []),
     }

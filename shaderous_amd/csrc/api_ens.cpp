@@ -1,0 +1,334 @@
+// C-ABI of libshx for ensembles of independent 2D grids (include/shx.h, shx_ens_*): per-instance
+// parameters and offsets, the tile work list, concatenated state, per-instance clocks, graphs.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+#include "ens_impl.hpp"
+
+using namespace shx;
+
+struct shx_ens {
+  shx_config cfg;
+  int es = 4;
+  int count = 0;
+  std::vector<int> nx, ny;
+  std::vector<long long> off;
+  size_t ncell = 0;
+  hipStream_t stream = nullptr;
+  void* inst = nullptr;        // EnsInst<T>[count]
+  void* tiles[ENS_NTILES] = {nullptr, nullptr, nullptr};   // EnsTile lists per tile shape
+  int ntiles[ENS_NTILES] = {0, 0, 0};
+  void* U1[2] = {nullptr, nullptr};
+  void* U2[2] = {nullptr, nullptr};
+  void* clk = nullptr;         // 16 T per instance
+  unsigned long long* slot = nullptr;
+  int cur = 0;
+  int tile = ENS_32x16;
+  int grid = 512;
+  bool rates_stale = true;
+  EnsOps ops{};
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+void enqueue_step(shx_ens* e, int p) {
+  e->ops.clock(e->stream, e->inst, e->count, e->clk, e->slot);
+  e->ops.fused(e->stream, e->tile, e->grid, e->inst, e->tiles[e->tile], e->ntiles[e->tile], e->U1[p], e->U2[p],
+               e->U1[1 - p], e->U2[1 - p], e->clk, e->slot);
+}
+
+void destroy_graphs(shx_ens* e) {
+  for (int p = 0; p < 2; p++)
+    if (e->graph[p]) { (void)hipGraphExecDestroy(e->graph[p]); e->graph[p] = nullptr; }
+}
+
+int build_graphs(shx_ens* e) {
+  destroy_graphs(e);
+  for (int p = 0; p < 2; p++) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(e, p);
+    HIPCHK(hipStreamEndCapture(e->stream, &g));
+    HIPCHK(hipGraphInstantiate(&e->graph[p], g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+  }
+  return SHX_OK;
+}
+
+// The slots hold each instance's largest rate of its current state; recomputed before the first
+// step after a state change (outside the graphs).
+int refresh_rates(shx_ens* e) {
+  if (!e->rates_stale) return SHX_OK;
+  HIPCHK(hipMemsetAsync(e->slot, 0, sizeof(unsigned long long) * e->count, e->stream));
+  e->ops.rates(e->stream, e->inst, e->tiles[e->tile], e->ntiles[e->tile], e->tile, e->U1[e->cur], e->U2[e->cur], e->slot);
+  HIPCHK(hipGetLastError());
+  e->rates_stale = false;
+  return SHX_OK;
+}
+
+template <typename T>
+int set_state_t(shx_ens* e, int i, const T* u1, const T* u2, size_t ncell) {
+  if (!e || i < 0 || i >= e->count) return fail(SHX_EINVAL, "bad instance");
+  if ((int)sizeof(T) != e->es) return fail(SHX_EINVAL, "state precision mismatch");
+  if (ncell != (size_t)e->nx[i] * e->ny[i]) return fail(SHX_EINVAL, "instance %d holds %d x %d cells", i, e->nx[i], e->ny[i]);
+  for (size_t o = 0; o < 4 * ncell; o++)
+    if (!isfinite((double)u1[o]) || !isfinite((double)u2[o])) return fail(SHX_ESTATE, "non-finite state at %zu", o / 4);
+  // the instance's cells in both parities (the other instances keep theirs)
+  for (int p = 0; p < 2; p++) {
+    HIPCHK(hipMemcpyAsync((char*)e->U1[p] + e->off[i] * 4 * sizeof(T), u1, ncell * 4 * sizeof(T), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync((char*)e->U2[p] + e->off[i] * 4 * sizeof(T), u2, ncell * 4 * sizeof(T), hipMemcpyHostToDevice, e->stream));
+  }
+  T clk[16] = {0};
+  clk[4] = clk[5] = clk[6] = T(1);
+  HIPCHK(hipMemcpyAsync((char*)e->clk + (size_t)i * 16 * sizeof(T), clk, sizeof clk, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  e->rates_stale = true;
+  return SHX_OK;
+}
+
+template <typename T>
+int read_state_t(shx_ens* e, int i, T* u1, T* u2) {
+  if (!e || i < 0 || i >= e->count) return fail(SHX_EINVAL, "bad instance");
+  if ((int)sizeof(T) != e->es) return fail(SHX_EINVAL, "state precision mismatch");
+  const size_t n = (size_t)e->nx[i] * e->ny[i];
+  HIPCHK(hipMemcpyAsync(u1, (char*)e->U1[e->cur] + e->off[i] * 4 * sizeof(T), n * 4 * sizeof(T), hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipMemcpyAsync(u2, (char*)e->U2[e->cur] + e->off[i] * 4 * sizeof(T), n * 4 * sizeof(T), hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return SHX_OK;
+}
+
+template <typename T>
+std::vector<EnsInst<T>> instances(const shx_ens* e) {
+  std::vector<EnsInst<T>> v(e->count);
+  for (int i = 0; i < e->count; i++) {
+    shx_config c = e->cfg;
+    c.n[0] = e->nx[i]; c.n[1] = e->ny[i]; c.n[2] = 1;
+    c.ndim = 2;
+    c.dx = 1.0 / e->nx[i];
+    v[i].P = make_params<T>(c);
+    v[i].off = e->off[i];
+  }
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int shx_ens_create(const shx_config* cfg, int count, const int* sizes, shx_ens** out) {
+  if (!cfg || !sizes || !out || count < 1) return fail(SHX_EINVAL, "null argument or empty ensemble");
+  *out = nullptr;
+  const shx_config& c = *cfg;
+  if (c.precision != 32 && c.precision != 64) return fail(SHX_EINVAL, "precision %d", c.precision);
+  for (int a = 0; a < 2; a++)
+    if (c.bc_lo[a] < 0 || c.bc_lo[a] > 3 || c.bc_hi[a] < 0 || c.bc_hi[a] > 3) return fail(SHX_EINVAL, "boundary code on axis %d", a);
+  if (!(c.gamma > 1) || !(c.cfl > 0)) return fail(SHX_EINVAL, "gamma > 1 and cfl > 0 required");
+  for (int i = 0; i < count; i++)
+    if (sizes[2 * i] < 4 || sizes[2 * i + 1] < 4) return fail(SHX_EINVAL, "instance %d: %d x %d (at least 4 x 4)", i, sizes[2 * i], sizes[2 * i + 1]);
+  EnsOps ops = c.precision == 64 ? ens_ops_f64(c.limiter, c.riemann, c.isentropic ? 1 : 0)
+                                 : ens_ops_f32(c.limiter, c.riemann, c.isentropic ? 1 : 0);
+  if (!ops.fused) return fail(SHX_EINVAL, "no ensemble kernels built for precision %d limiter %d riemann %d isentropic %d", c.precision, c.limiter, c.riemann, c.isentropic);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c.device) return fail(SHX_EDEVICE, "no HIP device %d", c.device);
+  if (hipSetDevice(c.device) != hipSuccess) return fail(SHX_EDEVICE, "hipSetDevice(%d)", c.device);
+
+  shx_ens* e = new shx_ens();
+  e->cfg = c;
+  e->es = c.precision == 64 ? 8 : 4;
+  e->count = count;
+  e->ops = ops;
+  e->tile = e->es == 8 ? ENS_20x20 : ENS_32x16;
+  for (int i = 0; i < count; i++) {
+    e->nx.push_back(sizes[2 * i]);
+    e->ny.push_back(sizes[2 * i + 1]);
+    e->off.push_back((long long)e->ncell);
+    e->ncell += (size_t)sizes[2 * i] * sizes[2 * i + 1];
+  }
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess || cus < 1) cus = 256;
+    e->grid = 4 * cus;
+  }
+  auto alloc = [&](void** p, size_t b) { return hipMalloc(p, b) == hipSuccess; };
+  bool ok = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess;
+  const size_t bytes = e->ncell * 4 * e->es;
+  for (int p = 0; ok && p < 2; p++) ok = alloc(&e->U1[p], bytes) && alloc(&e->U2[p], bytes);
+  ok = ok && alloc(&e->clk, (size_t)count * 16 * 8) && alloc((void**)&e->slot, (size_t)count * 8);
+  const size_t isz = e->es == 8 ? sizeof(EnsInst<double>) : sizeof(EnsInst<float>);
+  ok = ok && alloc(&e->inst, (size_t)count * isz);
+  // tile lists, largest instances first (their tiles spread over the device first)
+  std::vector<int> order(count);
+  for (int i = 0; i < count; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return (long long)e->nx[a] * e->ny[a] > (long long)e->nx[b] * e->ny[b]; });
+  std::vector<EnsTile> lists[ENS_NTILES];
+  for (int k = 0; k < ENS_NTILES; k++) {
+    int tx, ty;
+    ens_tile_cells(k, tx, ty);
+    for (int i : order)
+      for (int y0 = 0; y0 < e->ny[i]; y0 += ty)
+        for (int x0 = 0; x0 < e->nx[i]; x0 += tx) lists[k].push_back(EnsTile{i, x0, y0, 0});
+    e->ntiles[k] = (int)lists[k].size();
+    ok = ok && alloc(&e->tiles[k], lists[k].size() * sizeof(EnsTile));
+  }
+  if (!ok) {
+    shx_ens_destroy(e);
+    return fail(SHX_ENOMEM, "device allocation failed for %zu cells", e->ncell);
+  }
+  bool copied = true;
+  if (e->es == 8) {
+    auto v = instances<double>(e);
+    copied = hipMemcpy(e->inst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice) == hipSuccess;
+  } else {
+    auto v = instances<float>(e);
+    copied = hipMemcpy(e->inst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice) == hipSuccess;
+  }
+  for (int k = 0; k < ENS_NTILES; k++)
+    copied = copied && hipMemcpy(e->tiles[k], lists[k].data(), lists[k].size() * sizeof(EnsTile), hipMemcpyHostToDevice) == hipSuccess;
+  copied = copied && hipMemset(e->clk, 0, (size_t)count * 16 * 8) == hipSuccess && hipMemset(e->slot, 0, (size_t)count * 8) == hipSuccess &&
+           hipMemset(e->U1[0], 0, bytes) == hipSuccess && hipMemset(e->U2[0], 0, bytes) == hipSuccess &&
+           hipMemset(e->U1[1], 0, bytes) == hipSuccess && hipMemset(e->U2[1], 0, bytes) == hipSuccess &&
+           hipDeviceSynchronize() == hipSuccess;
+  if (!copied) {
+    shx_ens_destroy(e);
+    return fail(SHX_EDEVICE, "initialising the ensemble's device memory failed");
+  }
+  int rc = build_graphs(e);
+  if (rc) { shx_ens_destroy(e); return rc; }
+  *out = e;
+  return SHX_OK;
+}
+
+int shx_ens_destroy(shx_ens* e) {
+  if (!e) return SHX_OK;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  destroy_graphs(e);
+  for (int p = 0; p < 2; p++) { (void)hipFree(e->U1[p]); (void)hipFree(e->U2[p]); }
+  for (int k = 0; k < ENS_NTILES; k++) (void)hipFree(e->tiles[k]);
+  (void)hipFree(e->inst); (void)hipFree(e->clk); (void)hipFree(e->slot);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return SHX_OK;
+}
+
+int shx_ens_set_state(shx_ens* e, int i, const float* u1, const float* u2, size_t ncell) {
+  return set_state_t<float>(e, i, u1, u2, ncell);
+}
+int shx_ens_set_state_f64(shx_ens* e, int i, const double* u1, const double* u2, size_t ncell) {
+  return set_state_t<double>(e, i, u1, u2, ncell);
+}
+int shx_ens_read_state(shx_ens* e, int i, float* u1, float* u2) { return read_state_t<float>(e, i, u1, u2); }
+int shx_ens_read_state_f64(shx_ens* e, int i, double* u1, double* u2) { return read_state_t<double>(e, i, u1, u2); }
+
+int shx_ens_step(shx_ens* e, int count) {
+  if (!e || count < 0) return fail(SHX_EINVAL, "bad argument");
+  int rc = refresh_rates(e);
+  if (rc) return rc;
+  for (int n = 0; n < count; n++) {
+    HIPCHK(hipGraphLaunch(e->graph[e->cur], e->stream));
+    e->cur = 1 - e->cur;
+  }
+  return SHX_OK;
+}
+
+int shx_ens_sync(shx_ens* e) {
+  if (!e) return fail(SHX_EINVAL, "null ensemble");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return SHX_OK;
+}
+
+int shx_ens_read_clock(shx_ens* e, int i, double out[11]) {
+  if (!e || !out || i < 0 || i >= e->count) return fail(SHX_EINVAL, "bad argument");
+  unsigned char buf[16 * 8];
+  HIPCHK(hipMemcpyAsync(buf, (char*)e->clk + (size_t)i * 16 * e->es, 16 * e->es, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  for (int k = 0; k < 11; k++) out[k] = e->es == 8 ? ((double*)buf)[k] : (double)((float*)buf)[k];
+  return SHX_OK;
+}
+
+int shx_ens_profile(shx_ens* e, int steps, double ms[3]) {
+  if (!e || !ms || steps < 1) return fail(SHX_EINVAL, "bad argument");
+  int rc = refresh_rates(e);
+  if (rc) return rc;
+  hipEvent_t ev[3];
+  for (int i = 0; i < 3; i++) HIPCHK(hipEventCreate(&ev[i]));
+  double acc[3] = {0, 0, 0};
+  for (int n = 0; n < steps; n++) {
+    const int p = e->cur;
+    HIPCHK(hipEventRecord(ev[0], e->stream));
+    e->ops.clock(e->stream, e->inst, e->count, e->clk, e->slot);
+    HIPCHK(hipEventRecord(ev[1], e->stream));
+    e->ops.fused(e->stream, e->tile, e->grid, e->inst, e->tiles[e->tile], e->ntiles[e->tile], e->U1[p], e->U2[p],
+                 e->U1[1 - p], e->U2[1 - p], e->clk, e->slot);
+    HIPCHK(hipEventRecord(ev[2], e->stream));
+    HIPCHK(hipEventSynchronize(ev[2]));
+    HIPCHK(hipGetLastError());
+    e->cur = 1 - p;
+    float t;
+    HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1])); acc[0] += t;
+    HIPCHK(hipEventElapsedTime(&t, ev[1], ev[2])); acc[1] += t;
+    HIPCHK(hipEventElapsedTime(&t, ev[0], ev[2])); acc[2] += t;
+  }
+  for (int i = 0; i < 3; i++) ms[i] = acc[i] / steps;
+  for (int i = 0; i < 3; i++) (void)hipEventDestroy(ev[i]);
+  return SHX_OK;
+}
+
+int shx_ens_autotune(shx_ens* e, int reps, int chosen[2]) {
+  if (!e) return fail(SHX_EINVAL, "null ensemble");
+  if (reps < 1) reps = 3;
+  int rc = refresh_rates(e);
+  if (rc) return rc;
+  const size_t bytes = e->ncell * 4 * e->es;
+  void *b1 = nullptr, *b2 = nullptr, *cs = nullptr, *ss = nullptr;
+  HIPCHK(hipMalloc(&b1, bytes));
+  HIPCHK(hipMalloc(&b2, bytes));
+  HIPCHK(hipMalloc(&cs, (size_t)e->count * 16 * 8));
+  HIPCHK(hipMalloc(&ss, (size_t)e->count * 8));
+  HIPCHK(hipMemcpyAsync(b1, e->U1[e->cur], bytes, hipMemcpyDeviceToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(b2, e->U2[e->cur], bytes, hipMemcpyDeviceToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(cs, e->clk, (size_t)e->count * 16 * 8, hipMemcpyDeviceToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(ss, e->slot, (size_t)e->count * 8, hipMemcpyDeviceToDevice, e->stream));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  double best = 1e30;
+  int bt = e->tile;
+  const int cands[] = {ENS_32x32, ENS_32x16, ENS_20x20};
+  for (int k : cands) {
+    if (e->es == 8 && k != ENS_20x20) continue;
+    const int p = e->cur;
+    e->ops.fused(e->stream, k, e->grid, e->inst, e->tiles[k], e->ntiles[k], e->U1[p], e->U2[p], e->U1[1 - p],
+                 e->U2[1 - p], e->clk, e->slot);   // warm
+    HIPCHK(hipEventRecord(e0, e->stream));
+    for (int r = 0; r < reps; r++)
+      e->ops.fused(e->stream, k, e->grid, e->inst, e->tiles[k], e->ntiles[k], e->U1[p], e->U2[p], e->U1[1 - p],
+                   e->U2[1 - p], e->clk, e->slot);
+    HIPCHK(hipEventRecord(e1, e->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipGetLastError());
+    float t;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    if (t < best) { best = t; bt = k; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  e->tile = bt;
+  HIPCHK(hipMemcpyAsync(e->U1[e->cur], b1, bytes, hipMemcpyDeviceToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(e->U2[e->cur], b2, bytes, hipMemcpyDeviceToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(e->clk, cs, (size_t)e->count * 16 * 8, hipMemcpyDeviceToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(e->slot, ss, (size_t)e->count * 8, hipMemcpyDeviceToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  (void)hipFree(b1); (void)hipFree(b2); (void)hipFree(cs); (void)hipFree(ss);
+  rc = build_graphs(e);
+  if (rc) return rc;
+  if (chosen) ens_tile_cells(bt, chosen[0], chosen[1]);
+  return SHX_OK;
+}
+
+}  // extern "C"

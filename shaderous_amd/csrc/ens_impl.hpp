@@ -1,0 +1,248 @@
+// Batched ensembles of independent 2D grids for gfx950: every instance of an ensemble (each its
+// own size, boundaries and clock) advances in one persistent launch per step that consumes a
+// device list of column tiles drawn from all instances. Per step:
+//   k_ens_clock   one thread per instance: the clock pass (hydro/clock.wgsl:132-242) with that
+//                 instance's largest rate, so each instance takes its own dt
+//   k_ens_fused   per tile (BX x BY columns, (BX - 4) x (BY - 4) cells), one plane: primitives
+//                 into LDS, the Hancock half step of the cells and the one-column ring, each face
+//                 once, the conservative update, and the next step's rate of the instance
+//                 (4 barriers per tile)
+// The arithmetic is the uniform grid's, in the order of its plain 2D passes (a flat grid's z
+// neighbours are the cell itself and its z fluxes zero), so an instance gives the same bits as a
+// single 2D run of the plain path.
+#pragma once
+#include "uniform_impl.hpp"
+
+namespace shx {
+
+// One tile of the work list: instance, and the tile's lower corner in cells
+struct EnsTile {
+  int inst, x0, y0, pad;
+};
+
+// Per instance: its parameters, and where its cells start in the concatenated state
+template <typename T>
+struct EnsInst {
+  Params<T> P;
+  long long off;
+};
+
+template <typename T>
+__global__ void k_ens_clock(const EnsInst<T>* __restrict__ inst, int count, T* clk, unsigned long long* slot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const T rate = from_bits<T>(slot[i]);
+  clock_pass(inst[i].P, clk + 16 * i, rate);
+  slot[i] = 0ull;
+}
+
+// The largest rate of every instance's cells (the first step after a state change)
+template <typename T>
+__global__ __launch_bounds__(256) void k_ens_rates(const EnsInst<T>* __restrict__ inst, const EnsTile* __restrict__ tiles,
+                                                   int ntiles, int tx, int ty, const T4<T>* __restrict__ U1,
+                                                   const T4<T>* __restrict__ U2, unsigned long long* slot) {
+  const EnsTile w = tiles[blockIdx.x];
+  const Params<T>& P = inst[w.inst].P;
+  const long long off = inst[w.inst].off;
+  T m = T(0);
+  for (int k = threadIdx.x; k < tx * ty; k += 256) {
+    const int i = w.x0 + k % tx, j = w.y0 + k / tx;
+    if (i >= P.n[0] || j >= P.n[1]) continue;
+    const size_t o = off + (size_t)j * P.n[0] + i;
+    const T4<T> u2 = U2[o];
+    if (u2.w <= T(0.5)) m = tmax(m, rate_of(P, prim_from_cons(P, U1[o], u2)));
+  }
+  block_max_to(m, slot + w.inst);
+}
+
+// LDS of the 2D tile kernel (structure of arrays over the NT columns): q (7), the upper x and y
+// sides (9 each); the x and y fluxes of the face below each column (8 each) reuse the sides'
+// space once every face has read its sides (a barrier apart)
+template <typename T, int NT>
+struct Ens2dLayout {
+  static constexpr int Q = 0, SX = 7 * NT, SY = 16 * NT, FX = 7 * NT, FY = 15 * NT, SIZE = 25 * NT;
+};
+
+template <typename T, int LIM, int RS, bool ISEN, int BX, int BY>
+__global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 2 : 1)) void k_ens_fused(
+    const EnsInst<T>* __restrict__ inst, const EnsTile* __restrict__ tiles, int ntiles, const T4<T>* __restrict__ U1,
+    const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1, T4<T>* __restrict__ O2, const T* __restrict__ clk,
+    unsigned long long* slot) {
+  constexpr int NT = BX * BY;
+  using LY = Ens2dLayout<T, NT>;
+  __shared__ T L[LY::SIZE];
+  __shared__ T red[NT / 64 + 1];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int t = tx + BX * ty;
+  const bool in_r1 = tx >= 1 && tx <= BX - 2 && ty >= 1 && ty <= BY - 2;
+
+  for (int wi = blockIdx.x; wi < ntiles; wi += gridDim.x) {
+    const EnsTile w = tiles[wi];
+    const Params<T> P = inst[w.inst].P;
+    const long long off = inst[w.inst].off;
+    const int nx = P.n[0], ny = P.n[1];
+    const T dt = clk[16 * w.inst];
+    const T dtdx = dt / P.dx;
+    const int x = w.x0 + tx - 2, y = w.y0 + ty - 2;
+    const AxisMap mx = map_axis(x, nx, P.bc_lo[0], P.bc_hi[0]);
+    const AxisMap my = map_axis(y, ny, P.bc_lo[1], P.bc_hi[1]);
+    const bool real_x = x >= 0 && x < nx, real_y = y >= 0 && y < ny;
+    const bool cell = tx >= 2 && tx <= BX - 3 && ty >= 2 && ty <= BY - 3 && real_x && real_y;
+    const bool xface = tx >= 2 && tx <= BX - 2 && ty >= 2 && ty <= BY - 3 && x <= nx && real_y;
+    const bool yface = ty >= 2 && ty <= BY - 2 && tx >= 2 && tx <= BX - 3 && y <= ny && real_x;
+    // a ghost column beyond a non-periodic face takes the half step of the cell it maps to
+    // (load_pred, hydro/update.wgsl:47-63): that cell's stencil sits inward in the tile
+    int sx = tx, sy = ty;
+    if (!real_x && ((x < 0 && P.bc_lo[0] != BC_PERIODIC) || (x >= nx && P.bc_hi[0] != BC_PERIODIC))) sx = tx + (mx.i - x);
+    if (!real_y && ((y < 0 && P.bc_lo[1] != BC_PERIODIC) || (y >= ny && P.bc_hi[1] != BC_PERIODIC))) sy = ty + (my.i - y);
+    const bool mapped = (sx != tx) || (sy != ty);
+    const int ts = min(max(sx, 1), BX - 2) + BX * min(max(sy, 1), BY - 2);
+
+    __syncthreads();   // the previous tile's last LDS reads are done
+    const size_t o = off + (size_t)my.i * nx + mx.i;
+    const T4<T> u1 = U1[o], u2 = U2[o];
+    Prim<T> q = prim_from_cons(P, u1, u2);
+    q.vx = bc_vel(q.vx, mx.flip, mx.side);
+    q.vy = bc_vel(q.vy, my.flip, my.side);
+    lds_put_prim<T, NT>(L + LY::Q, t, q);
+    __syncthreads();
+
+    SideData<T> sxl, syl;
+    T hp = T(0);
+    if (in_r1) {
+      const Prim<T> pc = lds_prim_c<T, NT>(L + LY::Q, t);
+      const Prim<T> xm = lds_prim<T, NT>(L + LY::Q, t - 1), xp = lds_prim<T, NT>(L + LY::Q, t + 1);
+      const Prim<T> ym = lds_prim<T, NT>(L + LY::Q, t - BX), yp = lds_prim<T, NT>(L + LY::Q, t + BX);
+      Inc<T> ixu, ixd, iyu, iyd;
+      AxisSlope<T> sl[3];
+      sl[0] = axis_slope<T, LIM, ISEN>(P, xm, pc, xp, &ixu, &ixd);
+      sl[1] = axis_slope<T, LIM, ISEN>(P, ym, pc, yp, &iyu, &iyd);
+      Half<T> hc;
+      if (!mapped) {
+        sl[2] = axis_slope<T, LIM, ISEN>(P, pc, pc, pc, nullptr, nullptr);
+        const T conv = (xp.vx - xm.vx) + (yp.vy - ym.vy) + (pc.vz - pc.vz);
+        const T jump = tmax(jump_of(xm.p, xp.p), tmax(jump_of(ym.p, yp.p), jump_of(pc.p, pc.p)));
+        hc = hancock(P, pc, sl, conv, jump, dt);
+      } else {
+        // the mapped cell's own stencil: its centre is this column's primitives seen unflipped
+        Prim<T> qc = pc;
+        qc.vx = bc_vel(qc.vx, mx.flip, 0);
+        qc.vy = bc_vel(qc.vy, my.flip, 0);
+        const Prim<T> sxm = lds_prim<T, NT>(L + LY::Q, ts - 1), sxp = lds_prim<T, NT>(L + LY::Q, ts + 1);
+        const Prim<T> sym = lds_prim<T, NT>(L + LY::Q, ts - BX), syp = lds_prim<T, NT>(L + LY::Q, ts + BX);
+        AxisSlope<T> ms[3];
+        ms[0] = axis_slope<T, LIM, ISEN>(P, sxm, qc, sxp, nullptr, nullptr);
+        ms[1] = axis_slope<T, LIM, ISEN>(P, sym, qc, syp, nullptr, nullptr);
+        ms[2] = axis_slope<T, LIM, ISEN>(P, qc, qc, qc, nullptr, nullptr);
+        const T conv = (sxp.vx - sxm.vx) + (syp.vy - sym.vy) + (qc.vz - qc.vz);
+        const T jump = tmax(jump_of(sxm.p, sxp.p), tmax(jump_of(sym.p, syp.p), jump_of(qc.p, qc.p)));
+        hc = hancock(P, qc, ms, conv, jump, dt);
+        hc.vx = bc_vel(hc.vx, mx.flip, mx.side);
+        hc.vy = bc_vel(hc.vy, my.flip, my.side);
+      }
+      hp = hc.p;
+      sxl = side_of(hc, ixd, T(-1));
+      syl = side_of(hc, iyd, T(-1));
+      lds_put_side<T, NT>(L, LY::SX, t, side_of(hc, ixu, T(1)));
+      lds_put_side<T, NT>(L, LY::SY, t, side_of(hc, iyu, T(1)));
+    }
+    __syncthreads();
+
+    Flux<T> Fx = zero_flux<T>(), Fy = zero_flux<T>();
+    if (xface) Fx = face_from_sides<T, RS>(P, lds_get_side<T, NT>(L, LY::SX, t - 1), sxl, 0);
+    if (yface) Fy = face_from_sides<T, RS>(P, lds_get_side<T, NT>(L, LY::SY, t - BX), syl, 1);
+    __syncthreads();
+    if (xface) lds_put_flux<T, NT>(L, LY::FX, t, Fx);
+    if (yface) lds_put_flux<T, NT>(L, LY::FY, t, Fy);
+    __syncthreads();
+
+    T r = T(0);
+    if (cell) {
+      const Flux<T> xh = lds_get_flux<T, NT>(L, LY::FX, t + 1);
+      const Flux<T> yh = lds_get_flux<T, NT>(L, LY::FY, t + BX);
+      const Flux<T> net = net_of(Fx, xh, Fy, yh, zero_flux<T>(), zero_flux<T>());
+      T4<T> o1, o2;
+      advance(P, u1, u2, net, hp, dtdx, o1, o2);
+      const size_t oc = off + (size_t)y * nx + x;
+      O1[oc] = o1;
+      O2[oc] = o2;
+      r = rate_of(P, prim_from_cons(P, o1, o2));
+    }
+    // the instance's next-step rate: tile maximum, one atomicMax
+    T v = wave_max(r);
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) {
+      T m = T(0);
+      for (int k = 0; k < (NT + 63) / 64; k++) m = tmax(m, red[k]);
+      atomicMax(slot + w.inst, (unsigned long long)order_bits(m));
+    }
+  }
+}
+
+}  // namespace shx
+
+namespace shx {
+
+// Column tiles of the ensemble kernel (threads BX x BY, cells (BX - 4) x (BY - 4))
+enum { ENS_20x20 = 0, ENS_32x16 = 1, ENS_32x32 = 2, ENS_NTILES = 3 };
+inline void ens_tile_cells(int tile, int& tx, int& ty) {
+  tx = tile == ENS_20x20 ? 16 : 28;
+  ty = tile == ENS_20x20 ? 16 : (tile == ENS_32x16 ? 12 : 28);
+}
+
+struct EnsOps {
+  void (*clock)(hipStream_t, const void* inst, int count, void* clk, unsigned long long* slot);
+  void (*rates)(hipStream_t, const void* inst, const void* tiles, int ntiles, int tile, const void* u1, const void* u2,
+                unsigned long long* slot);
+  void (*fused)(hipStream_t, int tile, int grid, const void* inst, const void* tiles, int ntiles, const void* u1,
+                const void* u2, void* o1, void* o2, const void* clk, unsigned long long* slot);
+};
+
+template <typename T, int LIM, int RS, bool ISEN>
+struct EnsImpl {
+  static void clock(hipStream_t st, const void* inst, int count, void* clk, unsigned long long* slot) {
+    hipLaunchKernelGGL(k_ens_clock<T>, dim3((count + 63) / 64), dim3(64), 0, st, (const EnsInst<T>*)inst, count,
+                       (T*)clk, slot);
+  }
+  static void rates(hipStream_t st, const void* inst, const void* tiles, int ntiles, int tile, const void* u1,
+                    const void* u2, unsigned long long* slot) {
+    int tx, ty;
+    ens_tile_cells(tile, tx, ty);
+    hipLaunchKernelGGL(k_ens_rates<T>, dim3(ntiles), dim3(256), 0, st, (const EnsInst<T>*)inst,
+                       (const EnsTile*)tiles, ntiles, tx, ty, (const T4<T>*)u1, (const T4<T>*)u2, slot);
+  }
+  template <int BX, int BY>
+  static void fused_tile(hipStream_t st, int grid, const void* inst, const void* tiles, int ntiles, const void* u1,
+                         const void* u2, void* o1, void* o2, const void* clk, unsigned long long* slot) {
+    hipLaunchKernelGGL((k_ens_fused<T, LIM, RS, ISEN, BX, BY>), dim3(grid), dim3(BX, BY), 0, st,
+                       (const EnsInst<T>*)inst, (const EnsTile*)tiles, ntiles, (const T4<T>*)u1, (const T4<T>*)u2,
+                       (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, slot);
+  }
+  static void fused(hipStream_t st, int tile, int grid, const void* inst, const void* tiles, int ntiles,
+                    const void* u1, const void* u2, void* o1, void* o2, const void* clk, unsigned long long* slot) {
+    if (grid > ntiles) grid = ntiles;
+    if (grid < 1) return;
+    if constexpr (sizeof(T) == 8) {
+      fused_tile<20, 20>(st, grid, inst, tiles, ntiles, u1, u2, o1, o2, clk, slot);
+    } else {
+      switch (tile) {
+        case ENS_32x32: fused_tile<32, 32>(st, grid, inst, tiles, ntiles, u1, u2, o1, o2, clk, slot); break;
+        case ENS_32x16: fused_tile<32, 16>(st, grid, inst, tiles, ntiles, u1, u2, o1, o2, clk, slot); break;
+        default: fused_tile<20, 20>(st, grid, inst, tiles, ntiles, u1, u2, o1, o2, clk, slot); break;
+      }
+    }
+  }
+  static EnsOps ops() {
+    EnsOps o;
+    o.clock = clock;
+    o.rates = rates;
+    o.fused = fused;
+    return o;
+  }
+};
+
+EnsOps ens_ops_f32(int lim, int rs, int isen);
+EnsOps ens_ops_f64(int lim, int rs, int isen);
+
+}  // namespace shx
