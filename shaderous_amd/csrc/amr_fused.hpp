@@ -31,7 +31,10 @@ namespace shx {
 
 constexpr int AF_SIDE = AMR_PAD;              // 20 columns per side
 constexpr int AF_NT = AF_SIDE * AF_SIDE;      // 400 columns
-constexpr int AF_THREADS = 384;
+constexpr int AF_THREADS = 384;             // one block's march
+// blocks marched side by side per workgroup: two in FP32 (2 x 51 KB of LDS), one in FP64
+template <typename T>
+constexpr int af_pair() { return sizeof(T) == 4 ? 2 : 1; }
 constexpr int AF_RING = 128;                  // ghost columns with a value: 64 of ring 1, 64 of ring 2
 
 // thread -> column (x, y) in block coordinates, [-2, 18)
@@ -139,20 +142,47 @@ __device__ __forceinline__ Flux<T> reg_mean(const int* __restrict__ tab, const i
   return F;
 }
 
-// A face of the block on side k: computed flux F, written to the register when the neighbour
-// there is coarser, replaced by the register mean when it is refined
-template <typename T, bool COMP>
-__device__ __forceinline__ Flux<T> boundary_face(const int* __restrict__ tab, const int* __restrict__ reg_of,
-                                                 T* __restrict__ R, int s, int k, int x, int y, int z, int idx,
+// Coarse-fine faces of the march, by a per-(slot, face) action: act >= 0 even: the neighbour is
+// coarser, write this face's fluxes to register act / 2; odd: the neighbour is refined, take
+// the face's fluxes from coarse-face buffer act / 2 (the means formed by k_amr_cfmean); -1:
+// neither.
+template <typename T>
+__device__ __forceinline__ Flux<T> boundary_face(int act, T* __restrict__ R, const T* __restrict__ CF, int idx,
                                                  Flux<T> F) {
-  const int nb = bt(tab, s, BT_NB + k);
-  if (nb == -1) {
-    const int reg = reg_of[s * 6 + k];
-    if (reg >= 0) reg_put(R, reg, idx, F);
-  } else if (nb >= 0 && !is_leaf(tab, nb)) {
-    F = reg_mean<T, COMP>(tab, reg_of, R, nb, k, x, y, z, F);
+  if (act < 0) return F;
+  if (act & 1) {
+    const T4<T>* p = reinterpret_cast<const T4<T>*>(CF + ((size_t)(act >> 1) * 256 + idx) * 8);
+    const T4<T> a = p[0], b = p[1];
+    F.m = a.x; F.mx = a.y; F.my = a.z; F.mz = a.w;
+    F.e = b.x; F.ei = b.y; F.s = b.z; F.vn = b.w;
+  } else {
+    reg_put(R, act >> 1, idx, F);
   }
   return F;
+}
+
+// The coarse sides of coarse-fine faces: for each listed (slot, face) whose neighbour is refined,
+// the mean of the four fine fluxes of each of its 256 faces, from the finer leaves' registers
+// (written by their march, launched before); match_faces' order and arithmetic
+// (amr/update.wgsl:207-290).
+template <typename T, bool COMP>
+__global__ __launch_bounds__(256) void k_amr_cfmean(const int* __restrict__ tab, const int* __restrict__ reg_of,
+                                                    const T* __restrict__ R, const int2* __restrict__ faces,
+                                                    int first, T* __restrict__ CF) {
+  const int2 f = faces[blockIdx.x];
+  const int s = f.x, k = f.y, axis = k >> 1;
+  const int idx = threadIdx.x;            // t2 * 16 + t1 over the face
+  const int t1 = idx & 15, t2 = idx >> 4;
+  const int n = (k & 1) ? AMR_BS - 1 : 0;
+  const int x = axis == 0 ? n : (axis == 1 ? t2 : t1);
+  const int y = axis == 1 ? n : (axis == 2 ? t2 : t1);
+  const int z = axis == 2 ? n : (axis == 0 ? t2 : t1);
+  const int np = bt(tab, s, BT_NB + k);
+  Flux<T> F = zero_flux<T>();
+  F = reg_mean<T, COMP>(tab, reg_of, R, np, k, x, y, z, F);
+  T4<T>* p = reinterpret_cast<T4<T>*>(CF + ((size_t)(first + blockIdx.x) * 256 + idx) * 8);
+  p[0] = v4<T>(F.m, F.mx, F.my, F.mz);
+  p[1] = v4<T>(F.e, F.ei, F.s, F.vn);
 }
 
 template <typename T>
@@ -224,52 +254,58 @@ __global__ __launch_bounds__(256) void k_amr_shell(Params<T> P, Levels<T> LV, co
 }
 
 // ---------------- the march ----------------
-// Launch bound: six waves per workgroup, two workgroups per CU in FP32 (three waves per SIMD:
-// up to 168 VGPRs)
-template <typename T, int LIM, int RS, bool ISEN, bool COMP>
-__global__ __launch_bounds__(AF_THREADS, (sizeof(T) == 4 ? 3 : 1)) void k_amr_fused(
+// Launch bound: twelve waves per workgroup (two blocks), one workgroup per CU in FP32 (three
+// waves per SIMD: up to 168 VGPRs); six waves (one block) in FP64
+template <typename T, int LIM, int RS, bool ISEN>
+__global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) == 4 ? 3 : 1)) void k_amr_fused(
     Params<T> P, const int* __restrict__ tab, const int* __restrict__ list, int count,
-    const int* __restrict__ shell_of, const T4<T>* __restrict__ SH, const int* __restrict__ reg_of,
-    T* __restrict__ R, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1,
-    T4<T>* __restrict__ O2, const T* __restrict__ clk, unsigned long long* next_slot) {
-  constexpr int NT = AF_NT, BX = AF_SIDE;
+    const int* __restrict__ shell_of, const T4<T>* __restrict__ SH, const int* __restrict__ act,
+    T* __restrict__ R, const T* __restrict__ CF, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2,
+    T4<T>* __restrict__ O1, T4<T>* __restrict__ O2, const T* __restrict__ clk, unsigned long long* next_slot) {
+  constexpr int NT = AF_NT, BX = AF_SIDE, AF_PAIR = af_pair<T>();
   using LY = FusedLayout<T, NT>;
-  __shared__ T L[LY::SIZE];
-  __shared__ T red[AF_THREADS / 64];
-  const int tid = threadIdx.x;
+  __shared__ T LL[AF_PAIR * LY::SIZE];
+  __shared__ T red[AF_PAIR * AF_THREADS / 64];
+  // which of the workgroup's two blocks this thread marches, and its thread index there
+  const int half = threadIdx.x >= AF_THREADS;
+  const int tid = threadIdx.x - AF_THREADS * half;
+  T* L = LL + half * LY::SIZE;
   int x, y;
   af_column(tid, x, y);
   const int t = (y + 2) * BX + (x + 2);                  // raster index of the column in LDS
   // the columns no thread owns are edges at every plane: zero in both plane slots, for good
   for (int i = tid; i < 2 * 7 * NT; i += AF_THREADS) L[i] = T(0);
-  const bool ox = x < 0 || x >= AMR_BS, oy = y < 0 || y >= AMR_BS;
-  const bool cell_col = !ox && !oy;             // a 16 x 16 cell column (tid < 256)
-  const bool r1 = x >= -1 && x <= AMR_BS && y >= -1 && y <= AMR_BS;
-  const bool face_col = r1 && (ox != oy);                // ring-1 face ghost column (tid 256..319)
-  // x faces below columns x = 0..16 and y faces below y = 0..16, within the cell rows
-  const bool xface = x >= 0 && x <= AMR_BS && !oy;
-  const bool yface = y >= 0 && y <= AMR_BS && !ox;
+  // roles by wave (uniform branches): waves 0-3 the cell columns, wave 4 the face ghosts of
+  // ring 1, wave 5 ring 2 (loads only)
+  const int wave = __builtin_amdgcn_readfirstlane(tid) >> 6;
+  const bool cells0 = wave < 4, ghosts0 = wave == 4;
+  // a face-ghost lane's side: 0 -x, 1 +x, 2 -y, 3 +y
+  const int side = (tid >> 4) & 3;
   const T dt = clk[0];
   const T dx0 = P.dx;
   T rate_max = T(0);
 
-  for (int wi = blockIdx.x; wi < count; wi += gridDim.x) {
-    const int s = list[wi];
+  for (int wp = blockIdx.x; AF_PAIR * wp < count; wp += gridDim.x) {
+    // the pair's second block may be missing: that half takes part in the barriers only
+    const int wi = AF_PAIR * wp + half;
+    const bool act_blk = wi < count;
+    const int s = list[act_blk ? wi : AF_PAIR * wp];
     const T4<T>* sh = SH + (size_t)shell_of[s] * SH_ITEMS * 2;
     // this block's cell size (P is the kernel's own copy: its dx is the block's from here on)
     const T dx = dx0 * (T(1) / T(1 << bt(tab, s, BT_LEVEL)));
     P.dx = dx;
     const T dtdx = dt / dx;
-    // a face-ghost column's source (constant over the cell planes)
-    int gk = 0;
-    if (face_col) gk = ghost_src(tab, s, x, y, 0).kind;
-    const int kz_lo = bt(tab, s, BT_NB + 4) == -1 ? 1 : 0, kz_hi = bt(tab, s, BT_NB + 5) == -1 ? 1 : 0;
+    const bool cells = cells0 && act_blk, ghosts = ghosts0 && act_blk;
+    // beside a coarser block (-1): its ghosts are predicted in place; elsewhere they are another
+    // block's cells, whose half steps come from the shell
+    const bool gk = ghosts && bt(tab, s, BT_NB + side) == -1;
+    const bool kz_lo = bt(tab, s, BT_NB + 4) == -1, kz_hi = bt(tab, s, BT_NB + 5) == -1;
     __syncthreads();   // the previous block's last LDS reads are done
 
     // texel (x, y, z) of this block: its own cells from U, ghosts from the shell
     auto texel = [&](int z) -> Prim<T> {
       Prim<T> w;
-      if (cell_col && z >= 0 && z < AMR_BS) {
+      if (cells && z >= 0 && z < AMR_BS) {
         const size_t o = (size_t)s * AMR_CELLS + cel(x, y, z);
         return prim_from_cons(P, U1[o], U2[o]);
       }
@@ -295,11 +331,11 @@ __global__ __launch_bounds__(AF_THREADS, (sizeof(T) == 4 ? 3 : 1)) void k_amr_fu
       T* Lm = L + (odd ? LY::Q1 : LY::Q0);
       __syncthreads();
 
-      // 2. the half step of texel (x, y, c) and its sides
+      // 2. the half step of texel (x, y, c) and its sides: every cell; a ghost plane's texels
+      //    (c = -1, 16) in the cell waves; the face ghosts of the cell planes in the ghost wave
       const bool oz = c < 0 || c >= AMR_BS;
-      const int nout = (int)ox + (int)oy + (int)oz;
       SideData<T> sxl, syl, szl, szu;
-      if (r1 && nout <= 1) {
+      if (cells || (ghosts && !oz)) {
         const Prim<T> pc = lds_prim_c<T, NT>(Lq, t);
         const Prim<T> pm = lds_prim<T, NT>(Lm, t);
         const Prim<T> xm = lds_prim<T, NT>(Lq, t - 1), xp = lds_prim<T, NT>(Lq, t + 1);
@@ -309,20 +345,17 @@ __global__ __launch_bounds__(AF_THREADS, (sizeof(T) == 4 ? 3 : 1)) void k_amr_fu
         sl[0] = axis_slope<T, LIM, ISEN>(P, xm, pc, xp, &ixu, &ixd);
         sl[1] = axis_slope<T, LIM, ISEN>(P, ym, pc, yp, &iyu, &iyd);
         sl[2] = axis_slope<T, LIM, ISEN>(P, pm, pc, pp, &izu, &izd);
-        // the ghost's source: a face-ghost column's, or a ghost plane's (c = -1 or 16)
-        int kind = 0;
-        if (nout == 1) kind = oz ? (c < 0 ? kz_lo : kz_hi) : gk;
+        const bool ghost = ghosts || oz;
+        const bool in_place = ghosts ? gk : (oz && (c < 0 ? kz_lo : kz_hi));
         Half<T> hc;
-        if (nout == 0 || kind == 1) {
-          if (nout == 1) {
-            // beside a coarser block: predicted in place, transverse slopes dropped at the rim
-            // where their neighbour is an unassembled edge texel (amr/predict.wgsl:183-221)
-            const T mx = (!ox && (x == 0 || x == AMR_BS - 1)) ? T(0) : T(1);
-            const T my = (!oy && (y == 0 || y == AMR_BS - 1)) ? T(0) : T(1);
-            const T mz = (!oz && (c == 0 || c == AMR_BS - 1)) ? T(0) : T(1);
-            scale_inc(sl[0].s, mx);
-            scale_inc(sl[1].s, my);
-            scale_inc(sl[2].s, mz);
+        if (!ghost || in_place) {
+          if (ghost) {
+            // beside a coarser block: predicted in place, the slopes along the ghost's rim axes
+            // dropped where their neighbour is an unassembled edge texel (amr/predict.wgsl:183-221)
+            const bool ox = x < 0 || x >= AMR_BS, oy = y < 0 || y >= AMR_BS;
+            scale_inc(sl[0].s, (!ox && (x == 0 || x == AMR_BS - 1)) ? T(0) : T(1));
+            scale_inc(sl[1].s, (!oy && (y == 0 || y == AMR_BS - 1)) ? T(0) : T(1));
+            scale_inc(sl[2].s, (!oz && (c == 0 || c == AMR_BS - 1)) ? T(0) : T(1));
           }
           const T conv = (xp.vx - xm.vx) + (yp.vy - ym.vy) + (pp.vz - pm.vz);
           const T jump = tmax(jump_of(xm.p, xp.p), tmax(jump_of(ym.p, yp.p), jump_of(pm.p, pp.p)));
@@ -346,74 +379,77 @@ __global__ __launch_bounds__(AF_THREADS, (sizeof(T) == 4 ? 3 : 1)) void k_amr_fu
       lds_put_prim<T, NT>(Lm, t, pp);
       __syncthreads();
 
-      // 3. the x and y faces below this column, and the z face below plane c
+      // 3. the x and y faces below this column (cells; the +x and +y ghosts' lower faces), and
+      //    the z face below plane c
       const bool live = c >= 0 && c < AMR_BS;
       Flux<T> Fx = zero_flux<T>(), Fy = zero_flux<T>(), fz = zero_flux<T>();
-      if (live && xface) {
+      if (live && (cells || (ghosts && side == 1))) {
         Fx = face_from_sides<T, RS>(P, lds_get_side<T, NT>(L, LY::SX, t - 1), sxl, 0);
-        if (x == 0) Fx = boundary_face<T, COMP>(tab, reg_of, R, s, 0, x, y, c, c * 16 + y, Fx);
-        else if (x == AMR_BS) Fx = boundary_face<T, COMP>(tab, reg_of, R, s, 1, x - 1, y, c, c * 16 + y, Fx);
+        if (x == 0) Fx = boundary_face<T>(act[s * 6 + 0], R, CF, c * 16 + y, Fx);
+        else if (x == AMR_BS) Fx = boundary_face<T>(act[s * 6 + 1], R, CF, c * 16 + y, Fx);
       }
-      if (live && yface) {
+      if (live && (cells || (ghosts && side == 3))) {
         Fy = face_from_sides<T, RS>(P, lds_get_side<T, NT>(L, LY::SY, t - BX), syl, 1);
-        if (y == 0) Fy = boundary_face<T, COMP>(tab, reg_of, R, s, 2, x, y, c, x * 16 + c, Fy);
-        else if (y == AMR_BS) Fy = boundary_face<T, COMP>(tab, reg_of, R, s, 3, x, y - 1, c, x * 16 + c, Fy);
+        if (y == 0) Fy = boundary_face<T>(act[s * 6 + 2], R, CF, x * 16 + c, Fy);
+        else if (y == AMR_BS) Fy = boundary_face<T>(act[s * 6 + 3], R, CF, x * 16 + c, Fy);
       }
-      if (cell_col && c >= 0) {
+      if (cells && c >= 0) {
         fz = face_from_sides<T, RS>(P, zs_prev, szl, 2);
-        if (c == 0) fz = boundary_face<T, COMP>(tab, reg_of, R, s, 4, x, y, c, y * 16 + x, fz);
-        else if (c == AMR_BS) fz = boundary_face<T, COMP>(tab, reg_of, R, s, 5, x, y, c - 1, y * 16 + x, fz);
+        if (c == 0) fz = boundary_face<T>(act[s * 6 + 4], R, CF, y * 16 + x, fz);
+        else if (c == AMR_BS) fz = boundary_face<T>(act[s * 6 + 5], R, CF, y * 16 + x, fz);
       }
       __syncthreads();
       // 4. the x/y fluxes into LDS
-      if (live && xface) lds_put_flux<T, NT>(L, LY::FX, t, Fx);
-      if (live && yface) lds_put_flux<T, NT>(L, LY::FY, t, Fy);
+      if (live && (cells || (ghosts && side == 1))) lds_put_flux<T, NT>(L, LY::FX, t, Fx);
+      if (live && (cells || (ghosts && side == 3))) lds_put_flux<T, NT>(L, LY::FY, t, Fy);
       __syncthreads();
 
       // 5. this plane's x/y net inflow, and the update of plane c - 1
-      Flux<T> axy = zero_flux<T>();
-      if (live && cell_col) {
-        const Flux<T> xh = lds_get_flux<T, NT>(L, LY::FX, t + 1);
-        const Flux<T> yh = lds_get_flux<T, NT>(L, LY::FY, t + BX);
-        axy.m = (Fx.m - xh.m) + (Fy.m - yh.m);
-        axy.mx = (Fx.mx - xh.mx) + (Fy.mx - yh.mx);
-        axy.my = (Fx.my - xh.my) + (Fy.my - yh.my);
-        axy.mz = (Fx.mz - xh.mz) + (Fy.mz - yh.mz);
-        axy.e = (Fx.e - xh.e) + (Fy.e - yh.e);
-        axy.ei = (Fx.ei - xh.ei) + (Fy.ei - yh.ei);
-        axy.s = (Fx.s - xh.s) + (Fy.s - yh.s);
-        axy.vn = (Fx.vn - xh.vn) + (Fy.vn - yh.vn);
+      if (cells) {
+        Flux<T> axy = zero_flux<T>();
+        if (live) {
+          const Flux<T> xh = lds_get_flux<T, NT>(L, LY::FX, t + 1);
+          const Flux<T> yh = lds_get_flux<T, NT>(L, LY::FY, t + BX);
+          axy.m = (Fx.m - xh.m) + (Fy.m - yh.m);
+          axy.mx = (Fx.mx - xh.mx) + (Fy.mx - yh.mx);
+          axy.my = (Fx.my - xh.my) + (Fy.my - yh.my);
+          axy.mz = (Fx.mz - xh.mz) + (Fy.mz - yh.mz);
+          axy.e = (Fx.e - xh.e) + (Fy.e - yh.e);
+          axy.ei = (Fx.ei - xh.ei) + (Fy.ei - yh.ei);
+          axy.s = (Fx.s - xh.s) + (Fy.s - yh.s);
+          axy.vn = (Fx.vn - xh.vn) + (Fy.vn - yh.vn);
+        }
+        if (c - 1 >= 0 && c - 1 < AMR_BS) {
+          const size_t o = (size_t)s * AMR_CELLS + cel(x, y, c - 1);
+          Flux<T> net;
+          net.m = axy_prev.m + (fz_lo.m - fz.m);
+          net.mx = axy_prev.mx + (fz_lo.mx - fz.mx);
+          net.my = axy_prev.my + (fz_lo.my - fz.my);
+          net.mz = axy_prev.mz + (fz_lo.mz - fz.mz);
+          net.e = axy_prev.e + (fz_lo.e - fz.e);
+          net.ei = axy_prev.ei + (fz_lo.ei - fz.ei);
+          net.s = axy_prev.s + (fz_lo.s - fz.s);
+          net.vn = axy_prev.vn + (fz_lo.vn - fz.vn);
+          T4<T> o1, o2;
+          advance(P, U1[o], U2[o], net, zs_prev.hp, dtdx, o1, o2);
+          O1[o] = o1;
+          O2[o] = o2;
+          // the next step's rate / dx (assemble's leaf-cell rate, amr/assemble.wgsl:77-154)
+          rate_max = tmax(rate_max, rate_of(P, prim_from_cons(P, o1, o2)) / dx);
+        }
+        zs_prev = szu;
+        fz_lo = fz;
+        axy_prev = axy;
       }
-      if (cell_col && c - 1 >= 0 && c - 1 < AMR_BS) {
-        const size_t o = (size_t)s * AMR_CELLS + cel(x, y, c - 1);
-        Flux<T> net;
-        net.m = axy_prev.m + (fz_lo.m - fz.m);
-        net.mx = axy_prev.mx + (fz_lo.mx - fz.mx);
-        net.my = axy_prev.my + (fz_lo.my - fz.my);
-        net.mz = axy_prev.mz + (fz_lo.mz - fz.mz);
-        net.e = axy_prev.e + (fz_lo.e - fz.e);
-        net.ei = axy_prev.ei + (fz_lo.ei - fz.ei);
-        net.s = axy_prev.s + (fz_lo.s - fz.s);
-        net.vn = axy_prev.vn + (fz_lo.vn - fz.vn);
-        T4<T> o1, o2;
-        advance(P, U1[o], U2[o], net, zs_prev.hp, dtdx, o1, o2);
-        O1[o] = o1;
-        O2[o] = o2;
-        // the next step's rate / dx (assemble's leaf-cell rate, amr/assemble.wgsl:77-154)
-        rate_max = tmax(rate_max, rate_of(P, prim_from_cons(P, o1, o2)) / dx);
-      }
-      zs_prev = szu;
-      fz_lo = fz;
-      axy_prev = axy;
     }
   }
   // workgroup maximum of the next step's rate, one atomicMax
   T v = wave_max(rate_max);
-  if ((tid & 63) == 0) red[tid >> 6] = v;
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (tid == 0) {
+  if (threadIdx.x == 0) {
     T m = T(0);
-    for (int w = 0; w < AF_THREADS / 64; w++) m = tmax(m, red[w]);
+    for (int w = 0; w < AF_PAIR * AF_THREADS / 64; w++) m = tmax(m, red[w]);
     atomicMax(next_slot, (unsigned long long)order_bits(m));
   }
 }
@@ -427,19 +463,28 @@ void AmrImpl<T, LIM, RS, ISEN>::shell(const AmrLaunch& L, const int* list, int c
 }
 
 template <typename T, int LIM, int RS, bool ISEN>
-void AmrImpl<T, LIM, RS, ISEN>::fused(const AmrLaunch& L, const int* list, int count, int grid, bool comp,
-                                      const int* shell_of, const void* sh, const int* reg_of, void* reg,
-                                      const void* u1, const void* u2, void* o1, void* o2, const void* clk,
-                                      unsigned long long* slot) {
+void AmrImpl<T, LIM, RS, ISEN>::fused(const AmrLaunch& L, const int* list, int count, int grid, const int* shell_of,
+                                      const void* sh, const int* act, void* reg, const void* cf, const void* u1,
+                                      const void* u2, void* o1, void* o2, const void* clk, unsigned long long* slot) {
   if (count <= 0) return;
-  if (grid > count) grid = count;
-#define SHX_AF(C)                                                                                                  \
-  hipLaunchKernelGGL((k_amr_fused<T, LIM, RS, ISEN, C>), dim3(grid), dim3(AF_THREADS), 0, L.stream, P(L), L.tab,   \
-                     list, count, shell_of, (const T4<T>*)sh, reg_of, (T*)reg, (const T4<T>*)u1,                    \
-                     (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, slot)
-  if (comp) SHX_AF(true);
-  else SHX_AF(false);
-#undef SHX_AF
+  constexpr int np = af_pair<T>();
+  const int pairs = (count + np - 1) / np;
+  if (grid > pairs) grid = pairs;
+  hipLaunchKernelGGL((k_amr_fused<T, LIM, RS, ISEN>), dim3(grid), dim3(np * AF_THREADS), 0, L.stream, P(L), L.tab, list,
+                     count, shell_of, (const T4<T>*)sh, act, (T*)reg, (const T*)cf, (const T4<T>*)u1,
+                     (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, slot);
+}
+
+template <typename T, int LIM, int RS, bool ISEN>
+void AmrImpl<T, LIM, RS, ISEN>::cfmean(const AmrLaunch& L, const int* faces, int count, int first, bool comp,
+                                       const int* reg_of, const void* reg, void* cf) {
+  if (count <= 0) return;
+  if (comp)
+    hipLaunchKernelGGL((k_amr_cfmean<T, true>), dim3(count), dim3(256), 0, L.stream, L.tab, reg_of, (const T*)reg,
+                       (const int2*)faces, first, (T*)cf);
+  else
+    hipLaunchKernelGGL((k_amr_cfmean<T, false>), dim3(count), dim3(256), 0, L.stream, L.tab, reg_of, (const T*)reg,
+                       (const int2*)faces, first, (T*)cf);
 }
 
 }  // namespace shx

@@ -49,6 +49,13 @@ struct shx_amr {
   int* d_shell_of = nullptr;
   int* d_reg_of = nullptr;
   void* reg = nullptr;
+  // coarse-fine faces seen from the coarse side: (slot, face) pairs by level, their buffers, and
+  // the march's per-(slot, face) action (register write, coarse-face read, or none)
+  int* d_act = nullptr;
+  int* d_cf_faces = nullptr;
+  void* cf = nullptr;
+  int ncf = 0;
+  Range cf_lv[AMR_MAXLEV];
   int nreg = 0;
   int grid = 256;            // persistent workgroups of the fused kernel
   void* clk = nullptr;
@@ -215,12 +222,19 @@ int enqueue_step(shx_amr* s, int p, int stage = -1) {
     if (run(3)) {
       auto fz = [&](const Range& r) {
         if (!r.count) return;
-        o.fused(L, lp(s, r), r.count, s->grid, comp, s->d_shell_of, s->shell, s->d_reg_of, s->reg, s->U1[p],
-                s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, s->slot);
+        o.fused(L, lp(s, r), r.count, s->grid, s->d_shell_of, s->shell, s->d_act, s->reg, s->cf, s->U1[p], s->U2[p],
+                s->U1[1 - p], s->U2[1 - p], s->clk, s->slot);
         n++;
       };
       fz(s->plain);
-      for (int l = s->nlevels - 1; l >= 0; l--) fz(s->match_lv[l]);
+      for (int l = s->nlevels - 1; l >= 0; l--) {
+        if (s->cf_lv[l].count) {
+          o.cfmean(L, s->d_cf_faces + 2 * s->cf_lv[l].off, s->cf_lv[l].count, s->cf_lv[l].off, comp, s->d_reg_of,
+                   s->reg, s->cf);
+          n++;
+        }
+        fz(s->match_lv[l]);
+      }
     }
     return n;
   }
@@ -380,8 +394,8 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess || cus < 1) cus = 256;
-    // FP32: two 6-wave workgroups per CU (51 KB of LDS each); FP64: one
-    s->grid = (s->es == 8 ? 1 : 2) * cus;
+    // one 12-wave workgroup (two blocks marched side by side, 102 KB of LDS in FP32) per CU
+    s->grid = cus;
   }
   const size_t ubytes = s->ncell * 4 * s->es;
   const size_t abytes = fused ? 0 : (size_t)nblocks * AMR_TEXELS * 4 * s->es;
@@ -389,8 +403,31 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   bool ok = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess;
   for (int p = 0; ok && p < 2; p++) ok = alloc(&s->U1[p], ubytes) && alloc(&s->U2[p], ubytes);
   if (!fused) ok = ok && alloc(&s->W1, abytes) && alloc(&s->W2, abytes) && alloc(&s->PA, abytes) && alloc(&s->PB, abytes);
+  // coarse sides of coarse-fine faces, by level; the march's action per (slot, face)
+  std::vector<int> act((size_t)nblocks * 6, -1), cf_faces;
+  for (int f = 0; f < nblocks * 6; f++)
+    if (reg_of[f] >= 0) act[f] = 2 * reg_of[f];
+  for (int l = 0; l < s->nlevels; l++) {
+    s->cf_lv[l].off = s->ncf;
+    for (int j = 0; j < s->match_lv[l].count; j++) {
+      const int i = lists[s->match_lv[l].off + j];
+      for (int f = 0; f < 6; f++) {
+        const int nb = s->tab[(size_t)i * AMR_BT + BT_NB + f];
+        if (nb >= 0 && !(s->tab[(size_t)nb * AMR_BT + BT_FLAGS] & 2)) {
+          act[(size_t)i * 6 + f] = 2 * s->ncf + 1;
+          cf_faces.push_back(i);
+          cf_faces.push_back(f);
+          s->ncf++;
+        }
+      }
+    }
+    s->cf_lv[l].count = s->ncf - s->cf_lv[l].off;
+  }
   ok = ok && alloc((void**)&s->d_reg_of, reg_of.size() * sizeof(int)) &&
-       alloc(&s->reg, (size_t)(s->nreg > 0 ? s->nreg : 1) * 256 * 8 * s->es);
+       alloc(&s->reg, (size_t)(s->nreg > 0 ? s->nreg : 1) * 256 * 8 * s->es) &&
+       alloc((void**)&s->d_act, act.size() * sizeof(int)) &&
+       alloc((void**)&s->d_cf_faces, (cf_faces.size() > 0 ? cf_faces.size() : 2) * sizeof(int)) &&
+       alloc(&s->cf, (size_t)(s->ncf > 0 ? s->ncf : 1) * 256 * 8 * s->es);
   // ghost shells: 5184 items of 8 values per leaf (the fused path)
   std::vector<int> shell_of(nblocks, -1);
   for (int i = 0; i < s->leaf.count; i++) shell_of[lists[s->leaf.off + i]] = i;
@@ -412,6 +449,8 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
                 hipMemset(s->clk, 0, 16 * 8) == hipSuccess && hipMemset(s->slot, 0, 64) == hipSuccess;
   bool copied = hipMemcpy(s->d_tab, s->tab.data(), s->tab.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
                 hipMemcpy(s->d_reg_of, reg_of.data(), reg_of.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+                hipMemcpy(s->d_act, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+                (cf_faces.empty() || hipMemcpy(s->d_cf_faces, cf_faces.data(), cf_faces.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess) &&
                 hipMemcpy(s->d_shell_of, shell_of.data(), shell_of.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
                 hipMemcpy(s->d_lists, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
   if (!zeroed || !copied || hipDeviceSynchronize() != hipSuccess) {
@@ -431,6 +470,7 @@ int shx_amr_destroy(shx_amr* s) {
   for (int p = 0; p < 2; p++) { (void)hipFree(s->U1[p]); (void)hipFree(s->U2[p]); }
   (void)hipFree(s->W1); (void)hipFree(s->W2); (void)hipFree(s->PA); (void)hipFree(s->PB);
   (void)hipFree(s->d_reg_of); (void)hipFree(s->reg); (void)hipFree(s->shell); (void)hipFree(s->d_shell_of);
+  (void)hipFree(s->d_act); (void)hipFree(s->d_cf_faces); (void)hipFree(s->cf);
   (void)hipFree(s->d_tab); (void)hipFree(s->d_lists); (void)hipFree(s->clk); (void)hipFree(s->slot);
   (void)hipFree(s->diag_part); (void)hipFree(s->diag_out);
   if (s->stream) (void)hipStreamDestroy(s->stream);
