@@ -167,6 +167,11 @@ int shx_amr_diagnostics(shx_amr* sim, double out[16]);
 /* Time `steps` steps pass by pass with HIP events: ms[0] assemble, [1] clock, [2] predict,
  * [3] update, [4] whole step. */
 int shx_amr_profile(shx_amr* sim, int steps, double ms[5]);
+/* AmrSim.fluxMismatch (src/amr/amr_sim.ts:4078-4160) with DEBUG_FLUX (amr/update.wgsl:324): one
+ * step's face mass fluxes as each cell computed them (the state and clock are left as they
+ * were). counts[0] same-level faces between blocks, [1] of them differing in any bit, [2]
+ * coarse-fine faces; *worst_coarse_fine = the largest |coarse flux - mean of the fine fluxes|. */
+int shx_amr_flux_mismatch(shx_amr* sim, int64_t counts[3], double* worst_coarse_fine);
 /* Counts: out[0] blocks, [1] leaves, [2] leaves with a refined neighbour, [3] levels,
  * [4] kernel launches per step. */
 int shx_amr_stats(shx_amr* sim, int out[5]);

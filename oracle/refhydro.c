@@ -43,6 +43,7 @@ struct orc_ref {
   double* W;        /* per cell: rho, vx, vy, vz, p, s, cs */
   Pred* memo;       /* per cell, valid where have[o] */
   unsigned char* have;
+  double* slab_next[7];   /* the slab update's output, kept between calls */
   double t, dt;
   int steps;
 };
@@ -348,7 +349,7 @@ orc_ref* orc_ref_create(const orc_params* prm) {
 }
 void orc_ref_destroy(orc_ref* h) {
   if (!h) return;
-  for (int i = 0; i < 7; i++) free(h->U[i]);
+  for (int i = 0; i < 7; i++) { free(h->U[i]); free(h->slab_next[i]); }
   free(h->W); free(h->have); free(h->memo); free(h);
 }
 double* orc_ref_U(orc_ref* h, int comp) { return h->U[comp]; }
@@ -498,9 +499,8 @@ double orc_ref_step_slab(orc_ref* h, int k0, int k1) {
   double rate = primitives_slab(h, k0, k1);
   h->dt = (c->cfl * c->dx) / rate;
   predict_planes(h, k0, k1);
-  double* next[7];
-  for (int i = 0; i < 7; i++) next[i] = (double*)malloc(sizeof(double) * h->ncell);
-  update_cells(h, k0, k1, next);
-  for (int i = 0; i < 7; i++) free(next[i]);
+  for (int i = 0; i < 7; i++)
+    if (!h->slab_next[i]) h->slab_next[i] = (double*)malloc(sizeof(double) * h->ncell);
+  update_cells(h, k0, k1, h->slab_next);
   return h->dt;
 }

@@ -141,6 +141,14 @@ class AmrSim:
                            floored=out[8], scalar=out[9], rho_range=(out[10], out[11]), p_range=(out[12], out[13]),
                            raw=out)
 
+    def flux_mismatch(self) -> dict:
+        """AmrSim.fluxMismatch: same-level faces between blocks (and how many differ in any bit),
+        coarse-fine faces, and the worst |coarse flux - mean of the fine fluxes|."""
+        counts = (C.c_int64 * 3)()
+        worst = C.c_double(0)
+        check(_lib.load().shx_amr_flux_mismatch(self._h, counts, C.byref(worst)))
+        return {"faces": counts[0], "mismatched": counts[1], "coarse_fine": counts[2], "worst_coarse_fine": worst.value}
+
     def profile(self, steps: int = 5) -> dict:
         ms = np.zeros(5)
         check(_lib.load().shx_amr_profile(self._h, steps, ms.ctypes.data_as(C.POINTER(C.c_double))))

@@ -48,7 +48,7 @@ struct AmrOps {
                   const void* clk);
   void (*update)(const AmrLaunch&, const int* list, int count, bool match, bool compensated, const void* u1,
                  const void* u2, const void* w1, const void* w2, const void* pa, const void* pb, void* o1, void* o2,
-                 const void* clk);
+                 const void* clk, bool debug);
   void (*rates)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2,
                 unsigned long long* slot);
   void (*diag)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2, double* part,

@@ -445,7 +445,9 @@ __device__ __forceinline__ Flux<T> fine_face(const Params<T>& P, const int* __re
 // of its box solved once into LDS (896 faces, 28 KB), then each cell's update.
 constexpr int UPD_WX = 8, UPD_WY = 8, UPD_WZ = 4, UPD_TILES = 16;
 
-template <typename T, int LIM, int RS, bool ISEN, bool MATCH, bool COMP>
+// DEBUG (DEBUG_FLUX, amr/update.wgsl:324): instead of the update, each cell writes the mass
+// flux of its six faces, (x lo, x hi, y lo, y hi) to O1 and (z lo, z hi, 0, 0) to O2.
+template <typename T, int LIM, int RS, bool ISEN, bool MATCH, bool COMP, bool DEBUG = false>
 __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
     Params<T> P, Levels<T> LV, const int* __restrict__ tab, const int* __restrict__ list,
     const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, const T4<T>* __restrict__ W1,
@@ -501,6 +503,12 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
         default: zh = F; break;
       }
     }
+  }
+  if constexpr (DEBUG) {
+    const size_t o = (size_t)s * AMR_CELLS + cel(x, y, z);
+    O1[o] = v4<T>(xl.m, xh.m, yl.m, yh.m);
+    O2[o] = v4<T>(zl.m, zh.m, T(0), T(0));
+    return;
   }
   const Flux<T> net = net_of(xl, xh, yl, yh, zl, zh);
   const T dx = level_dx(P, bt(tab, s, BT_LEVEL));
@@ -585,15 +593,22 @@ struct AmrImpl {
   }
   static void update(const AmrLaunch& L, const int* list, int count, bool match, bool comp, const void* u1,
                      const void* u2, const void* w1, const void* w2, const void* pa, const void* pb, void* o1, void* o2,
-                     const void* clk) {
+                     const void* clk, bool debug) {
     if (count <= 0) return;
-#define SHX_UPD(M, C)                                                                                                  \
-  hipLaunchKernelGGL((k_amr_update<T, LIM, RS, ISEN, M, C>), dim3(count * UPD_TILES), dim3(UPD_WX * UPD_WY * UPD_WZ), \
-                     0, L.stream, P(L), LV(L), L.tab, list, (const T4<T>*)u1, (const T4<T>*)u2, (const T4<T>*)w1,      \
-                     (const T4<T>*)w2, (const T4<T>*)pa, (const T4<T>*)pb, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk)
-    if (!match) SHX_UPD(false, false);
-    else if (comp) SHX_UPD(true, true);
-    else SHX_UPD(true, false);
+#define SHX_UPD(M, C, D)                                                                                               \
+  hipLaunchKernelGGL((k_amr_update<T, LIM, RS, ISEN, M, C, D>), dim3(count * UPD_TILES),                              \
+                     dim3(UPD_WX * UPD_WY * UPD_WZ), 0, L.stream, P(L), LV(L), L.tab, list, (const T4<T>*)u1,           \
+                     (const T4<T>*)u2, (const T4<T>*)w1, (const T4<T>*)w2, (const T4<T>*)pa, (const T4<T>*)pb,         \
+                     (T4<T>*)o1, (T4<T>*)o2, (const T*)clk)
+    if (debug) {
+      if (!match) SHX_UPD(false, false, true);
+      else if (comp) SHX_UPD(true, true, true);
+      else SHX_UPD(true, false, true);
+    } else {
+      if (!match) SHX_UPD(false, false, false);
+      else if (comp) SHX_UPD(true, true, false);
+      else SHX_UPD(true, false, false);
+    }
 #undef SHX_UPD
   }
   static void rates(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2,

@@ -261,7 +261,7 @@ int enqueue_step(shx_amr* s, int p, int stage = -1) {
     auto upd = [&](const Range& r, bool match) {
       if (!r.count) return;
       o.update(L, lp(s, r), r.count, match, comp, s->U1[p], s->U2[p], s->W1, s->W2, s->PA, s->PB, s->U1[1 - p],
-               s->U2[1 - p], s->clk);
+               s->U2[1 - p], s->clk, false);
       n++;
     };
     if (levels) {
@@ -550,6 +550,116 @@ int shx_amr_profile(shx_amr* s, int steps, double ms[5]) {
   }
   for (int i = 0; i < 5; i++) ms[i] = acc[i] / steps;
   for (int i = 0; i < 5; i++) (void)hipEventDestroy(ev[i]);
+  return SHX_OK;
+}
+
+int shx_amr_flux_mismatch(shx_amr* s, int64_t counts[3], double* worst_coarse_fine) {
+  if (!s || !counts) return fail(SHX_EINVAL, "bad argument");
+  // one step of the reference's passes with the update in DEBUG_FLUX form (the fused path gives
+  // the same bits): atlases for a fused sim are made for the call and freed after it
+  const bool own = s->W1 == nullptr;
+  const size_t abytes = (size_t)s->nblocks * AMR_TEXELS * 4 * s->es;
+  if (own) {
+    void** at[4] = {&s->W1, &s->W2, &s->PA, &s->PB};
+    for (void** a : at) {
+      if (hipMalloc(a, abytes) != hipSuccess) {
+        for (void** b : at) { (void)hipFree(*b); *b = nullptr; }
+        return fail(SHX_ENOMEM, "flux check: atlas allocation failed");
+      }
+      HIPCHK(hipMemsetAsync(*a, 0, abytes, s->stream));
+    }
+  }
+  unsigned char saved[16 * 8];
+  unsigned long long slot_saved;
+  HIPCHK(hipMemcpyAsync(saved, s->clk, sizeof saved, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(&slot_saved, s->slot, 8, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  const AmrLaunch L = launch_of(s);
+  const int p = s->cur;
+  HIPCHK(hipMemsetAsync(s->slot, 0, 8, s->stream));
+  s->ops.assemble(L, lp(s, s->all), s->all.count, s->U1[p], s->U2[p], s->W1, s->W2, s->slot);
+  s->ops.clock(L, s->clk, s->slot);
+  s->ops.predict(L, lp(s, s->leaf), s->leaf.count, s->W1, s->W2, s->PA, s->PB, s->clk);
+  const bool comp = s->lay.compensated != 0;
+  s->ops.update(L, lp(s, s->plain), s->plain.count, false, comp, s->U1[p], s->U2[p], s->W1, s->W2, s->PA, s->PB,
+                s->U1[1 - p], s->U2[1 - p], s->clk, true);
+  s->ops.update(L, lp(s, s->match), s->match.count, true, comp, s->U1[p], s->U2[p], s->W1, s->W2, s->PA, s->PB,
+                s->U1[1 - p], s->U2[1 - p], s->clk, true);
+  HIPCHK(hipGetLastError());
+  const size_t nc = s->ncell;
+  std::vector<double> f1(4 * nc), f2(4 * nc);
+  if (s->es == 8) {
+    HIPCHK(hipMemcpyAsync(f1.data(), s->U1[1 - p], 32 * nc, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(f2.data(), s->U2[1 - p], 32 * nc, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  } else {
+    std::vector<float> a(4 * nc), b(4 * nc);
+    HIPCHK(hipMemcpyAsync(a.data(), s->U1[1 - p], 16 * nc, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(b.data(), s->U2[1 - p], 16 * nc, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < 4 * nc; i++) { f1[i] = a[i]; f2[i] = b[i]; }
+  }
+  HIPCHK(hipMemcpyAsync(s->clk, saved, sizeof saved, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->slot, &slot_saved, 8, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (own) {
+    void** at[4] = {&s->W1, &s->W2, &s->PA, &s->PB};
+    for (void** a : at) { (void)hipFree(*a); *a = nullptr; }
+  }
+  // AmrSim.fluxMismatch (amr_sim.ts:4078-4160): same-level faces between blocks compared bit for
+  // bit (once, from the lower block); a coarse face against the mean of the fine fluxes beyond it
+  auto flux = [&](int b, int x, int y, int z, int f) {
+    const size_t o = ((size_t)b * AMR_CELLS + ((size_t)z * AMR_BS + y) * AMR_BS + x) * 4;
+    return f < 4 ? f1[o + f] : f2[o + f - 4];
+  };
+  auto is_leaf = [&](int i) { return (s->tab[(size_t)i * AMR_BT + BT_FLAGS] & 2) != 0; };
+  int64_t faces = 0, mis = 0, cf = 0;
+  double worst = 0;
+  for (int b = 0; b < s->nblocks; b++) {
+    if (!is_leaf(b)) continue;
+    for (int axis = 0; axis < 3; axis++)
+      for (int side = 0; side < 2; side++) {
+        const int nb = s->tab[(size_t)b * AMR_BT + BT_NB + 2 * axis + side];
+        if (nb < 0) continue;
+        const bool refined = !is_leaf(nb);
+        if (!refined && side == 0) continue;
+        for (int k = 0; k < AMR_BS; k++)
+          for (int j = 0; j < AMR_BS; j++)
+            for (int i = 0; i < AMR_BS; i++) {
+              int l[3] = {i, j, k};
+              if (l[axis] != (side ? AMR_BS - 1 : 0)) continue;
+              const double mine = flux(b, i, j, k, 2 * axis + side);
+              if (!refined) {
+                int m[3] = {i, j, k};
+                m[axis] = 0;
+                const double other = flux(nb, m[0], m[1], m[2], 2 * axis);
+                faces++;
+                if (memcmp(&mine, &other, sizeof mine) != 0) mis++;
+                continue;
+              }
+              const int t1 = axis == 2 ? 0 : axis + 1, t2 = axis == 0 ? 2 : axis - 1;
+              double sum = 0;
+              for (int sb = 0; sb < 4; sb++) {
+                int bits[3] = {0, 0, 0}, m[3] = {0, 0, 0};
+                bits[axis] = side ? 0 : 1;
+                m[axis] = side ? 0 : AMR_BS - 1;
+                const int q[2] = {t1, t2};
+                for (int n = 0; n < 2; n++) {
+                  bits[q[n]] = l[q[n]] >= AMR_BS / 2;
+                  m[q[n]] = 2 * (l[q[n]] - (AMR_BS / 2) * bits[q[n]]) + ((sb >> n) & 1);
+                }
+                const int ch = s->tab[(size_t)nb * AMR_BT + BT_CHILD + (bits[0] | (bits[1] << 1) | (bits[2] << 2))];
+                sum += flux(ch, m[0], m[1], m[2], 2 * axis + (side ? 0 : 1));
+              }
+              cf++;
+              worst = fmax(worst, fabs(mine - sum / 4));
+            }
+      }
+  }
+  counts[0] = faces;
+  counts[1] = mis;
+  counts[2] = cf;
+  if (worst_coarse_fine) *worst_coarse_fine = worst;
   return SHX_OK;
 }
 

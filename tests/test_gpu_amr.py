@@ -249,6 +249,31 @@ def test_amr_sod_tube():
     assert c["count"] < 256 * 32 * 32
 
 
+@pytest.mark.parametrize("path", ["fused", "passes"])
+def test_amr_flux_mismatch(path):
+    """AmrSim.fluxMismatch (amr-shock-tube's criterion, src/validation/tests.ts:1230-1263): every
+    same-level face between blocks bit-identical from both sides, coarse-fine faces within 1e-6
+    of the mean of the fine fluxes; the state and clock are left as they were."""
+    blocks = small_blast()
+    cfg = cfg_for((4, 4, 4), bc_lo=(2, 1, 0), bc_hi=(1, 2, 0))
+    u1, u2 = amr_tree.blocks_state(blocks, cfg.dx, blob, cfg.gamma)
+    sim = AmrSim(cfg, blocks, (4, 4, 4), path=path)
+    try:
+        sim.set_state(u1, u2)
+        sim.step(3)
+        before = sim.read_state()
+        ck = sim.clock()
+        fm = sim.flux_mismatch()
+        after = sim.read_state()
+        assert np.array_equal(before[0], after[0]) and np.array_equal(sim.clock(), ck)
+        sim.step(1)
+    finally:
+        sim.destroy()
+    print("flux", fm)
+    assert fm["faces"] > 0 and fm["mismatched"] == 0
+    assert fm["coarse_fine"] > 0 and fm["worst_coarse_fine"] < 1e-6
+
+
 def test_amr_errors():
     cfg = cfg_for((2, 2, 2))
     bad = [(0, x, y, z) for z in range(2) for y in range(2) for x in range(2)]
