@@ -216,7 +216,7 @@ def main():
         sim = Sim(config_for(pb, path="fused"))
         sim.set_state(u1, u2)
         if not args.no_tune:
-            tile = sim.autotune(reps=3)
+            tile = sim.autotune(reps=5)
             log(f"[bench] autotuned fused tile {tile}")
         sim.step(args.warmup)
         sim.sync()

@@ -218,8 +218,11 @@ int shx_create(const shx_config* cfg, shx_sim** out) {
   s->tile = s->es == 8 ? FUSED_20x20 : FUSED_32x16;
   s->zc = 32;
   if (c.tile[0] > 0) {
-    int t = c.tile[0];
-    s->tile = t == 16 ? FUSED_20x20 : (c.tile[1] == 12 ? FUSED_32x16 : FUSED_32x32);
+    for (int k = 0; k < FUSED_NTILES; k++) {
+      int tx, ty;
+      fused_tile_cells(k, tx, ty);
+      if (tx == c.tile[0] && ty == c.tile[1]) s->tile = k;
+    }
     if (s->es == 8) s->tile = FUSED_20x20;
   }
   if (c.tile[2] > 0) s->zc = c.tile[2];
@@ -440,8 +443,8 @@ int shx_autotune(shx_sim* s, int reps, int chosen[3]) {
   unsigned char clk_saved[128];
   HIPCHK(hipMemcpyAsync(clk_saved, s->clk, 128, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
-  const int tiles[] = {FUSED_32x32, FUSED_32x16, FUSED_20x20};
-  const int zcs[] = {8, 16, 32, 64};
+  const int tiles[] = {FUSED_R32x16, FUSED_R32x8, FUSED_32x32, FUSED_32x16, FUSED_20x20};
+  const int zcs[] = {16, 32, 64, 128};
   double best = 1e30;
   int bt = s->tile, bz = s->zc;
   Launch L = launch_of(s);
@@ -477,8 +480,7 @@ int shx_autotune(shx_sim* s, int reps, int chosen[3]) {
   rc = build_graphs(s);
   if (rc) return rc;
   if (chosen) {
-    chosen[0] = bt == FUSED_32x32 ? 28 : (bt == FUSED_32x16 ? 28 : 16);
-    chosen[1] = bt == FUSED_32x32 ? 28 : (bt == FUSED_32x16 ? 12 : 16);
+    fused_tile_cells(bt, chosen[0], chosen[1]);
     chosen[2] = bz;
   }
   return SHX_OK;

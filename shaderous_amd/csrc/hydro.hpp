@@ -387,7 +387,7 @@ __device__ __forceinline__ Flux<T> riemann(const Side<T>& l, const Side<T>& r, T
   }
   // internal energy and the scalar ride on the mass flux, upwinded
   bool up = o.m >= T(0);
-  o.ei = o.m * (up ? tdiv(l.eint, l.rho) : tdiv(r.eint, r.rho));
+  o.ei = o.m * tdiv(up ? l.eint : r.eint, up ? l.rho : r.rho);
   o.s = o.m * (up ? l.s : r.s);
   return o;
 }

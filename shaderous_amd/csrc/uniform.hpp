@@ -12,8 +12,19 @@ struct Launch {
   size_t ncell;
 };
 
-// Column tiles of the fused kernel (block BX x BY threads, tile (BX - 4) x (BY - 4) cells).
-enum { FUSED_20x20 = 0, FUSED_32x16 = 1, FUSED_32x32 = 2, FUSED_NTILES = 3 };
+// Column tiles of the fused kernel: k_fused's (block BX x BY threads in raster order, tile
+// (BX - 4) x (BY - 4) cells) and k_fused_r's (threads by role, tiles of 32 x 16 or 32 x 8 cells).
+enum { FUSED_20x20 = 0, FUSED_32x16 = 1, FUSED_32x32 = 2, FUSED_R32x16 = 3, FUSED_R32x8 = 4, FUSED_NTILES = 5 };
+// cells per tile along x and y
+inline void fused_tile_cells(int tile, int& tx, int& ty) {
+  switch (tile) {
+    case FUSED_20x20: tx = 16; ty = 16; break;
+    case FUSED_32x16: tx = 28; ty = 12; break;
+    case FUSED_32x32: tx = 28; ty = 28; break;
+    case FUSED_R32x16: tx = 32; ty = 16; break;
+    default: tx = 32; ty = 8; break;
+  }
+}
 
 struct UniformOps {
   void (*rates)(const Launch&, const void* u1, const void* u2, unsigned long long* slot);
