@@ -34,7 +34,33 @@ BYTES_PER_UPDATE = 64      # read 8 FP32 conserved values, write 8
 # HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (scaled by the counters' reading of a 512 MiB device copy: 2.0 and 1.0), for the
 # configuration they were measured on: profiles/k_fused_turbulence_256.md
-TRAFFIC_MEASURED = {("k_fused", 256, (28, 12, 32)): (1801676 + 524343) * 1024}
+TRAFFIC_MEASURED = {
+    ("k_fused", 256, (28, 12, 32)): (2 * 900838 + 524343) * 1024,
+    ("k_fused_r", 256, (32, 8, 64)): (2 * 650672 + 524320) * 1024,
+    ("k_fused_r", 256, (32, 8, 128)): (2 * 630350 + 524304) * 1024,
+    ("k_fused_r", 256, (32, 16, 128)): (2 * 663997 + 524296) * 1024,
+}
+
+
+# VALU instructions (wave64) per launch of the dominant kernel, measured with rocprofv3 --pmc
+# SQ_INSTS_VALU for the configuration they were measured on (profiles/k_fused_turbulence_256.md)
+VALU_MEASURED = {
+    ("k_fused_r", 256, (32, 8, 64)): 376973268,
+    ("k_fused_r", 256, (32, 8, 128)): 373012398,
+    ("k_fused_r", 256, (32, 16, 128)): 329879703,
+}
+# MI355X VALU issue peak for FP32 without packed math: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+VALU_LANE_PEAK = 256 * 4 * 16 * 2.4e9
+
+
+def valu_issue(kname, n, tile, kernel_ms):
+    """Issued VALU lane-slots per second of the kernel against the issue peak (a wave64 FP32
+    instruction occupies a SIMD for 4 cycles), from the measured instruction count."""
+    insts = VALU_MEASURED.get((kname, n, tuple(tile or ())))
+    if insts is None or not kernel_ms:
+        return None
+    rate = insts * 64 / (kernel_ms * 1e-3)
+    return {"insts_per_launch": insts, "lane_ops_per_s": rate, "peak": VALU_LANE_PEAK, "frac": rate / VALU_LANE_PEAK}
 
 
 def log(msg):
@@ -171,6 +197,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tile", default=None, metavar="TX,TY,ZC",
+                    help="fused tile and z chunk to use instead of autotuning (e.g. 32,8,64)")
     ap.add_argument("--paths", default="fused,plain", help="which uniform-grid paths to time")
     ap.add_argument("--amr-steps", type=int, default=20, help="timed steps of the AMR blast (0: skip it)")
     ap.add_argument("--amr-paths", default="fused,levels", help="AMR paths to time (fused, passes, levels)")
@@ -213,9 +241,13 @@ def main():
     pprof = None
     # fused one-pass path
     if "fused" in paths:
-        sim = Sim(config_for(pb, path="fused"))
+        forced = tuple(int(v) for v in args.tile.split(",")) if args.tile else None
+        sim = Sim(config_for(pb, path="fused", tile=forced or (0, 0, 0)))
         sim.set_state(u1, u2)
-        if not args.no_tune:
+        if forced:
+            tile = forced
+            log(f"[bench] fused tile {tile}")
+        elif not args.no_tune:
             tile = sim.autotune(reps=5)
             log(f"[bench] autotuned fused tile {tile}")
         sim.step(args.warmup)
@@ -252,6 +284,8 @@ def main():
             print(json.dumps({"amr": amr, "ensemble": ens}))
         return
 
+    # threads by role for the 32-wide tiles (k_fused_r), raster order otherwise (k_fused)
+    kname = "k_fused_r" if tile and tile[0] == 32 else "k_fused"
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (RefHydro restatement) ...")
@@ -275,8 +309,9 @@ def main():
                                f"MUSCL-Hancock, CFL 0.4", "cells": ncell, "path": "fused one-pass",
                    "tile": tile},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": TRAFFIC_MEASURED.get(("k_fused", n, tuple(tile or ()))),
-                     "kernel": "k_fused", "kernel_ms": kernel_ms, "bytes_per_update": BYTES_PER_UPDATE},
+                     "frac": achieved / HBM_PEAK, "traffic": TRAFFIC_MEASURED.get((kname, n, tuple(tile or ()))),
+                     "kernel": kname, "kernel_ms": kernel_ms, "bytes_per_update": BYTES_PER_UPDATE,
+                     "valu": valu_issue(kname, n, tile, kernel_ms)},
         "plain": {"value": plain_value, "ms_per_step": tp / args.steps * 1e3 if tp else None, "passes_ms": pprof},
         "fused_over_plain": value / plain_value if plain_value else None,
         "conserved": {"mass": diag.mass, "e_tot": diag.e_tot},
