@@ -53,14 +53,15 @@ struct AmrOps {
                 unsigned long long* slot);
   void (*diag)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2, double* part,
                double* out);
-  // the fused form: the ghost shells of the listed leaves (shell i for list entry i); the march
+  // the fused form: the ghost shells (per leaf shell index, the waves of work listed as (shell,
+  // slot, face, chunk): work0 the ghost texels, work1 the half steps); the march
   // over `count` listed leaves with `grid` persistent workgroups (shell_of: per slot, its shell;
   // act: per (slot, face), 2 r for a face on a coarser block whose fluxes go to register r, 2 c + 1
   // for a face on a refined block whose fluxes come from coarse-face buffer c, -1 otherwise); and
   // the coarse-face means of `count` listed (slot, face) pairs into buffers first, first + 1, ...
   // (reg_of: per (slot, face) the register index, -1 elsewhere)
-  void (*shell)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2, void* sh,
-                const void* clk);
+  void (*shell)(const AmrLaunch&, const int* work0, int n0, const int* work1, int n1, const void* u1, const void* u2,
+                void* sh, const void* clk);
   void (*fused)(const AmrLaunch&, const int* list, int count, int grid, const int* shell_of, const void* sh,
                 const int* act, void* reg, const void* cf, const void* u1, const void* u2, void* o1, void* o2,
                 const void* clk, unsigned long long* slot);

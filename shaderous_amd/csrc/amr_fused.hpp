@@ -203,54 +203,104 @@ constexpr int SH_CHUNKS = (SH_ITEMS + 255) / 256;
 
 __device__ __forceinline__ int ghost_plane_slot(int z) { return z == -2 ? 0 : (z == -1 ? 1 : (z == AMR_BS ? 2 : 3)); }
 
-template <typename T, int LIM, bool ISEN>
-__global__ __launch_bounds__(256) void k_amr_shell(Params<T> P, Levels<T> LV, const int* __restrict__ tab,
-                                                   const int* __restrict__ list, const T4<T>* __restrict__ U1,
-                                                   const T4<T>* __restrict__ U2, T4<T>* __restrict__ SH,
-                                                   const T* __restrict__ clk) {
-  const int li = blockIdx.x / SH_CHUNKS;
-  const int item = (blockIdx.x % SH_CHUNKS) * 256 + threadIdx.x;
-  if (item >= SH_ITEMS) return;
-  const int s = list[li];
-  T4<T>* out = SH + ((size_t)li * SH_ITEMS + item) * 2;
-  if (item < SH_TEXELS) {
-    int x, y, z;
-    if (item < 4 * AF_NT) {
-      const int pl = item / AF_NT, r = item % AF_NT;
-      z = pl == 0 ? -2 : (pl == 1 ? -1 : (pl == 2 ? AMR_BS : AMR_BS + 1));
-      x = r % AF_SIDE - 2;
-      y = r / AF_SIDE - 2;
+// The same-level leaf across face k of block s (0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z), or -1. Its
+// cells are the block's ghost texels there as they are (no image, no interpolation), so the
+// march reads them directly, and the shell forms nothing for that face but the half steps of
+// its rim.
+__device__ __forceinline__ int same_leaf(const int* __restrict__ tab, int s, int k) {
+  const int nb = bt(tab, s, BT_NB + k);
+  return (nb >= 0 && is_leaf(tab, nb)) ? nb : -1;
+}
+// Away from the face's rim (both coordinates along the face in [1, 14]), the half step of a
+// first-layer ghost of a same-level leaf uses only texels the march holds, the same ones the
+// source block's own predictor uses: the march forms it in place, exactly as at its source.
+__device__ __forceinline__ bool off_rim(int i, int j) {
+  return i >= 1 && i <= AMR_BS - 2 && j >= 1 && j <= AMR_BS - 2;
+}
+
+// Work of the shell, one wave per item (shell index, slot, face k, chunk), listed on the host from
+// the static mesh: PART 0, the ghost texels of face k (two layers, 512 texels: chunks 0-7) where
+// its neighbour is not a same-level leaf; PART 1, the first layer's half steps there (chunks 0-3)
+// or, beside a same-level leaf, the 60 on the face's rim (chunk 4). Faces on a coarser block
+// have no PART 1 work (the march predicts those ghosts in place).
+constexpr int SH_RIM = 4;
+
+template <int PART>
+constexpr int sh_part_waves() { return PART == 0 ? 6 : 3; }
+
+// the face position (i, j) of rim lane r (r < 60): the rows j = 0 and 15, then the columns
+// i = 0 and 15 between them
+__device__ __forceinline__ void rim_position(int r, int& i, int& j) {
+  if (r < 16) { i = r; j = 0; }
+  else if (r < 32) { i = r - 16; j = AMR_BS - 1; }
+  else if (r < 46) { i = 0; j = r - 31; }
+  else { i = AMR_BS - 1; j = r - 45; }
+}
+
+template <typename T, int LIM, bool ISEN, int PART>
+__global__ __launch_bounds__(256, sh_part_waves<PART>()) void k_amr_shell(
+    Params<T> P, Levels<T> LV, const int* __restrict__ tab, const int4* __restrict__ work, int nwork,
+    const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, T4<T>* __restrict__ SH, const T* __restrict__ clk) {
+  const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wv >= nwork) return;
+  const int4 wk = work[wv];
+  const int li = wk.x, s = wk.y, k = wk.z, code = wk.w;
+  const int lane = threadIdx.x & 63;
+  int x, y, z, item;
+  if constexpr (PART == 0) {
+    const int e = code * 64 + lane, hi = e >> 8, rem = e & 255;
+    if (k < 4) {
+      // ring hi + 1 column k * 16 + i, plane rem / 16
+      const int col = hi * 64 + k * 16 + (rem & 15);
+      z = rem >> 4;
+      af_column(256 + col, x, y);
+      item = 4 * AF_NT + z * AF_RING + col;
     } else {
-      const int g = item - 4 * AF_NT;
-      z = g / AF_RING;
-      af_column(256 + g % AF_RING, x, y);
+      // ghost plane (k == 4: z = -2, -1; k == 5: z = 16, 17), cell column rem
+      const int pl = (k == 4 ? 0 : 2) + hi;
+      x = rem & 15;
+      y = rem >> 4;
+      z = pl == 0 ? -2 : (pl == 1 ? -1 : (pl == 2 ? AMR_BS : AMR_BS + 1));
+      item = pl * AF_NT + (y + 2) * AF_SIDE + (x + 2);
     }
     const Prim<T> w = texel_of<T, LIM, true>(P, tab, U1, U2, s, x, y, z);
+    T4<T>* out = SH + ((size_t)li * SH_ITEMS + item) * 2;
     out[0] = v4<T>(w.rho, w.vx, w.vy, w.vz);
     out[1] = v4<T>(w.p, w.s, w.cs, T(0));
-    return;
-  }
-  const int h = item - SH_TEXELS;
-  int x, y, z;
-  if (h < 64 * AMR_BS) {
-    af_column(256 + h / AMR_BS, x, y);
-    z = h % AMR_BS;
   } else {
-    const int q = h - 64 * AMR_BS;
-    z = q < AMR_BS * AMR_BS ? -1 : AMR_BS;
-    x = (q % (AMR_BS * AMR_BS)) % AMR_BS;
-    y = (q % (AMR_BS * AMR_BS)) / AMR_BS;
+    int i, j;
+    if (code == SH_RIM) {
+      if (lane >= 60) return;
+      rim_position(lane, i, j);
+    } else {
+      const int e = code * 64 + lane;
+      i = e & 15;
+      j = e >> 4;
+    }
+    int h;
+    if (k < 4) {
+      const int col = k * 16 + i;
+      af_column(256 + col, x, y);
+      z = j;
+      h = col * AMR_BS + z;
+    } else {
+      x = i;
+      y = j;
+      z = k == 4 ? -1 : AMR_BS;
+      h = 64 * AMR_BS + (k == 4 ? 0 : AMR_BS * AMR_BS) + y * AMR_BS + x;
+    }
+    const GSrc g = ghost_src(tab, s, x, y, z);
+    if (g.kind != 0) return;   // beside a coarser block: predicted in place by the march
+    Params<T> Pl = P;
+    Pl.dx = level_dx(P, bt(tab, g.slot, BT_LEVEL));
+    Half<T> hc = half_at_source<T, LIM, ISEN>(Pl, tab, U1, U2, g.slot, g.x, g.y, g.z, clk[0]);
+    hc.vx = bc_vel(hc.vx, g.fx, g.sx);
+    hc.vy = bc_vel(hc.vy, g.fy, g.sy);
+    hc.vz = bc_vel(hc.vz, g.fz, g.sz);
+    T4<T>* out = SH + ((size_t)li * SH_ITEMS + SH_TEXELS + h) * 2;
+    out[0] = v4<T>(hc.rho, hc.vx, hc.vy, hc.vz);
+    out[1] = v4<T>(hc.p, hc.s, hc.shock, hc.fo);
   }
-  const GSrc g = ghost_src(tab, s, x, y, z);
-  if (g.kind != 0) return;   // beside a coarser block: predicted in place by the march
-  Params<T> Pl = P;
-  Pl.dx = level_dx(P, bt(tab, g.slot, BT_LEVEL));
-  Half<T> hc = half_at_source<T, LIM, ISEN>(Pl, tab, U1, U2, g.slot, g.x, g.y, g.z, clk[0]);
-  hc.vx = bc_vel(hc.vx, g.fx, g.sx);
-  hc.vy = bc_vel(hc.vy, g.fy, g.sy);
-  hc.vz = bc_vel(hc.vz, g.fz, g.sz);
-  out[0] = v4<T>(hc.rho, hc.vx, hc.vy, hc.vz);
-  out[1] = v4<T>(hc.p, hc.s, hc.shock, hc.fo);
 }
 
 // ---------------- the march ----------------
@@ -300,16 +350,33 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
     // block's cells, whose half steps come from the shell
     const bool gk = ghosts && bt(tab, s, BT_NB + side) == -1;
     const bool kz_lo = bt(tab, s, BT_NB + 4) == -1, kz_hi = bt(tab, s, BT_NB + 5) == -1;
+    // same-level leaves: across this ring lane's side, and across the two z faces
+    const int nbs = (!cells0 && act_blk) ? same_leaf(tab, s, side) : -1;
+    const int nbz_lo = same_leaf(tab, s, 4), nbz_hi = same_leaf(tab, s, 5);
     __syncthreads();   // the previous block's last LDS reads are done
 
-    // texel (x, y, z) of this block: its own cells from U, ghosts from the shell
+    // texel (x, y, z) of this block: its own cells and a same-level leaf's from U, edges zero,
+    // other ghosts from the shell
     auto texel = [&](int z) -> Prim<T> {
       Prim<T> w;
-      if (cells && z >= 0 && z < AMR_BS) {
-        const size_t o = (size_t)s * AMR_CELLS + cel(x, y, z);
+      w.rho = w.vx = w.vy = w.vz = w.p = w.s = w.cs = T(0);
+      const bool oz = z < 0 || z >= AMR_BS;
+      if (!act_blk || (!cells0 && oz)) return w;
+      int src = -1;
+      if (cells0) {
+        if (!oz) src = s;
+        else if ((z < 0 ? nbz_lo : nbz_hi) >= 0) src = z < 0 ? nbz_lo : nbz_hi;
+      } else {
+        src = nbs;
+      }
+      if (src >= 0) {
+        const int xs = x < 0 ? x + AMR_BS : (x >= AMR_BS ? x - AMR_BS : x);
+        const int ys = y < 0 ? y + AMR_BS : (y >= AMR_BS ? y - AMR_BS : y);
+        const int zs = z < 0 ? z + AMR_BS : (z >= AMR_BS ? z - AMR_BS : z);
+        const size_t o = (size_t)src * AMR_CELLS + cel(xs, ys, zs);
         return prim_from_cons(P, U1[o], U2[o]);
       }
-      const int g = (z < 0 || z >= AMR_BS) ? ghost_plane_slot(z) * AF_NT + t : 4 * AF_NT + z * AF_RING + (tid - 256);
+      const int g = oz ? ghost_plane_slot(z) * AF_NT + t : 4 * AF_NT + z * AF_RING + (tid - 256);
       const T4<T> a = sh[2 * g], b = sh[2 * g + 1];
       w.rho = a.x; w.vx = a.y; w.vy = a.z; w.vz = a.w;
       w.p = b.x; w.s = b.y; w.cs = b.z;
@@ -347,9 +414,12 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
         sl[2] = axis_slope<T, LIM, ISEN>(P, pm, pc, pp, &izu, &izd);
         const bool ghost = ghosts || oz;
         const bool in_place = ghosts ? gk : (oz && (c < 0 ? kz_lo : kz_hi));
+        // a same-level leaf's cell away from the face's rim: as at its source, in place
+        const bool exact = ghosts ? (nbs >= 0 && off_rim(side < 2 ? y : x, c))
+                                  : (oz && (c < 0 ? nbz_lo : nbz_hi) >= 0 && off_rim(x, y));
         Half<T> hc;
-        if (!ghost || in_place) {
-          if (ghost) {
+        if (!ghost || in_place || exact) {
+          if (ghost && !exact) {
             // beside a coarser block: predicted in place, the slopes along the ghost's rim axes
             // dropped where their neighbour is an unassembled edge texel (amr/predict.wgsl:183-221)
             const bool ox = x < 0 || x >= AMR_BS, oy = y < 0 || y >= AMR_BS;
@@ -455,11 +525,14 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
 }
 
 template <typename T, int LIM, int RS, bool ISEN>
-void AmrImpl<T, LIM, RS, ISEN>::shell(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2,
-                                      void* sh, const void* clk) {
-  if (count <= 0) return;
-  hipLaunchKernelGGL((k_amr_shell<T, LIM, ISEN>), dim3(count * SH_CHUNKS), dim3(256), 0, L.stream, P(L), LV(L), L.tab,
-                     list, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)sh, (const T*)clk);
+void AmrImpl<T, LIM, RS, ISEN>::shell(const AmrLaunch& L, const int* work0, int n0, const int* work1, int n1,
+                                      const void* u1, const void* u2, void* sh, const void* clk) {
+  if (n0 > 0)
+    hipLaunchKernelGGL((k_amr_shell<T, LIM, ISEN, 0>), dim3((n0 + 3) / 4), dim3(256), 0, L.stream, P(L), LV(L), L.tab,
+                       (const int4*)work0, n0, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)sh, (const T*)clk);
+  if (n1 > 0)
+    hipLaunchKernelGGL((k_amr_shell<T, LIM, ISEN, 1>), dim3((n1 + 3) / 4), dim3(256), 0, L.stream, P(L), LV(L), L.tab,
+                       (const int4*)work1, n1, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)sh, (const T*)clk);
 }
 
 template <typename T, int LIM, int RS, bool ISEN>

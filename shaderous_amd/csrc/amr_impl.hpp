@@ -623,8 +623,8 @@ struct AmrImpl {
                        (const T4<T>*)u1, (const T4<T>*)u2, part);
     hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(64), 0, L.stream, (const double*)part, count, out);
   }
-  static void shell(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2, void* sh,
-                    const void* clk);
+  static void shell(const AmrLaunch& L, const int* work0, int n0, const int* work1, int n1, const void* u1,
+                    const void* u2, void* sh, const void* clk);
   static void fused(const AmrLaunch& L, const int* list, int count, int grid, const int* shell_of, const void* sh,
                     const int* act, void* reg, const void* cf, const void* u1, const void* u2, void* o1, void* o2,
                     const void* clk, unsigned long long* slot);

@@ -47,6 +47,9 @@ struct shx_amr {
   // flux-register index per (slot, face) and the register
   void* shell = nullptr;
   int* d_shell_of = nullptr;
+  // the shell's waves of work (shell index, slot, face, chunk): ghost texels, half steps
+  int* d_shw[2] = {nullptr, nullptr};
+  int nshw[2] = {0, 0};
   int* d_reg_of = nullptr;
   void* reg = nullptr;
   // coarse-fine faces seen from the coarse side: (slot, face) pairs by level, their buffers, and
@@ -218,7 +221,10 @@ int enqueue_step(shx_amr* s, int p, int stage = -1) {
   auto run = [&](int g) { return stage < 0 || stage == g; };
   if (s->lay.path == SHX_AMR_PATH_FUSED) {
     if (run(1)) { o.clock(L, s->clk, s->slot); n++; }
-    if (run(2)) { o.shell(L, lp(s, s->leaf), s->leaf.count, s->U1[p], s->U2[p], s->shell, s->clk); n++; }
+    if (run(2)) {
+      o.shell(L, s->d_shw[0], s->nshw[0], s->d_shw[1], s->nshw[1], s->U1[p], s->U2[p], s->shell, s->clk);
+      n += (s->nshw[0] > 0) + (s->nshw[1] > 0);
+    }
     if (run(3)) {
       auto fz = [&](const Range& r) {
         if (!r.count) return;
@@ -433,6 +439,32 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   for (int i = 0; i < s->leaf.count; i++) shell_of[lists[s->leaf.off + i]] = i;
   if (fused) ok = ok && alloc(&s->shell, (size_t)s->leaf.count * 5184 * 8 * s->es);
   ok = ok && alloc((void**)&s->d_shell_of, shell_of.size() * sizeof(int));
+  // the shell's work: a face whose neighbour is a same-level leaf needs only the half steps of
+  // its rim (the march reads that leaf's cells and predicts the rest in place); a face on a
+  // coarser block needs its ghost texels only; any other face (refined neighbour, boundary
+  // image) its texels and all its half steps
+  std::vector<int> shw[2];
+  for (int i = 0; fused && i < s->leaf.count; i++) {
+    const int b = lists[s->leaf.off + i];
+    for (int f = 0; f < 6; f++) {
+      const int nb = s->tab[(size_t)b * AMR_BT + BT_NB + f];
+      const bool same_leaf = nb >= 0 && (s->tab[(size_t)nb * AMR_BT + BT_FLAGS] & 3) == 3;
+      auto add = [&](int part, int chunk) {
+        shw[part].insert(shw[part].end(), {i, b, f, chunk});
+      };
+      if (same_leaf) {
+        add(1, 4);
+      } else {
+        for (int c = 0; c < 8; c++) add(0, c);
+        if (nb != -1)
+          for (int c = 0; c < 4; c++) add(1, c);
+      }
+    }
+  }
+  for (int part = 0; part < 2; part++) {
+    s->nshw[part] = (int)(shw[part].size() / 4);
+    ok = ok && alloc((void**)&s->d_shw[part], (shw[part].empty() ? 4 : shw[part].size()) * sizeof(int));
+  }
   ok = ok && alloc((void**)&s->d_tab, s->tab.size() * sizeof(int)) && alloc((void**)&s->d_lists, lists.size() * sizeof(int));
   ok = ok && alloc(&s->clk, 16 * 8) && alloc((void**)&s->slot, 64) && alloc((void**)&s->diag_part, (size_t)nblocks * 16 * 8) &&
        alloc((void**)&s->diag_out, 16 * 8);
@@ -452,6 +484,8 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
                 hipMemcpy(s->d_act, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
                 (cf_faces.empty() || hipMemcpy(s->d_cf_faces, cf_faces.data(), cf_faces.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess) &&
                 hipMemcpy(s->d_shell_of, shell_of.data(), shell_of.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+                (shw[0].empty() || hipMemcpy(s->d_shw[0], shw[0].data(), shw[0].size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess) &&
+                (shw[1].empty() || hipMemcpy(s->d_shw[1], shw[1].data(), shw[1].size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess) &&
                 hipMemcpy(s->d_lists, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
   if (!zeroed || !copied || hipDeviceSynchronize() != hipSuccess) {
     shx_amr_destroy(s);
@@ -470,6 +504,7 @@ int shx_amr_destroy(shx_amr* s) {
   for (int p = 0; p < 2; p++) { (void)hipFree(s->U1[p]); (void)hipFree(s->U2[p]); }
   (void)hipFree(s->W1); (void)hipFree(s->W2); (void)hipFree(s->PA); (void)hipFree(s->PB);
   (void)hipFree(s->d_reg_of); (void)hipFree(s->reg); (void)hipFree(s->shell); (void)hipFree(s->d_shell_of);
+  (void)hipFree(s->d_shw[0]); (void)hipFree(s->d_shw[1]);
   (void)hipFree(s->d_act); (void)hipFree(s->d_cf_faces); (void)hipFree(s->cf);
   (void)hipFree(s->d_tab); (void)hipFree(s->d_lists); (void)hipFree(s->clk); (void)hipFree(s->slot);
   (void)hipFree(s->diag_part); (void)hipFree(s->diag_out);
