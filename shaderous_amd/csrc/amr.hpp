@@ -40,6 +40,30 @@ struct AmrLaunch {
   double dx0;
 };
 
+// The fused march's queue and what its workgroups read besides the state: the queue of leaves
+// (pairs of consecutive entries, -1 pads), per slot its shell, per (slot, face) its action (2 r:
+// fluxes to register r on a coarser block; 2 c + 1: the means of the finer fluxes into
+// coarse-face buffer c; -1 neither) and register index, per slot the finer leaves it waits for
+// (CSR: dep_off, dep), the control words (qctl) and a done flag per slot.
+enum { AMR_Q_NEXT = 0, AMR_Q_EPOCH = 1, AMR_Q_ERROR = 2, AMR_Q_WORDS = 4 };
+// bound of one dependency wait, in sleeps of about 128 cycles (~0.2 s at 2.4 GHz)
+constexpr int AMR_WAIT_SPINS = 1 << 22;
+struct AmrQueue {
+  const int* list;
+  int count;
+  int grid;              // workgroups (one per CU)
+  const int* shell_of;
+  const int* act;
+  void* reg;
+  void* cf;
+  const int* reg_of;
+  int comp;              // TwoSum-compensated means
+  const int* dep_off;
+  const int* dep;
+  int* qctl;
+  int* done;
+};
+
 struct AmrOps {
   void (*assemble)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2, void* w1, void* w2,
                    unsigned long long* slot);
@@ -54,19 +78,12 @@ struct AmrOps {
   void (*diag)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2, double* part,
                double* out);
   // the fused form: the ghost shells (per leaf shell index, the waves of work listed as (shell,
-  // slot, face, chunk): work0 the ghost texels, work1 the half steps); the march
-  // over `count` listed leaves with `grid` persistent workgroups (shell_of: per slot, its shell;
-  // act: per (slot, face), 2 r for a face on a coarser block whose fluxes go to register r, 2 c + 1
-  // for a face on a refined block whose fluxes come from coarse-face buffer c, -1 otherwise); and
-  // the coarse-face means of `count` listed (slot, face) pairs into buffers first, first + 1, ...
-  // (reg_of: per (slot, face) the register index, -1 elsewhere)
+  // slot, face, chunk): work0 the ghost texels, work1 the half steps); the march of every leaf
+  // in one launch, from the queue Q
   void (*shell)(const AmrLaunch&, const int* work0, int n0, const int* work1, int n1, const void* u1, const void* u2,
                 void* sh, const void* clk);
-  void (*fused)(const AmrLaunch&, const int* list, int count, int grid, const int* shell_of, const void* sh,
-                const int* act, void* reg, const void* cf, const void* u1, const void* u2, void* o1, void* o2,
+  void (*fused)(const AmrLaunch&, const AmrQueue& Q, const void* sh, const void* u1, const void* u2, void* o1, void* o2,
                 const void* clk, unsigned long long* slot);
-  void (*cfmean)(const AmrLaunch&, const int* faces, int count, int first, bool compensated, const int* reg_of,
-                 const void* reg, void* cf);
 };
 
 AmrOps amr_ops_f32(int lim, int rs, int isen);

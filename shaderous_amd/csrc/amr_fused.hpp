@@ -11,11 +11,15 @@
 //    cell and of the ghosts beside a coarser block (masked transverse slopes), in LDS;
 //  - every face once; the conservative update of each cell; the next step's rate / dx;
 //  - coarse-fine faces through a flux register: a leaf writes the fluxes of its faces on a
-//    coarser block (as it computed them), and the coarser leaf, launched after it, reads the
-//    four fine fluxes of each of its faces there and takes their mean (TwoSum-compensated when
-//    asked), which is what match_faces (amr/update.wgsl:207-290) recomputes from the finer
-//    block's atlas. Launch order: leaves with no refined neighbour first, then the others level
-//    by level from the finest.
+//    coarser block (as it computed them), and the coarser leaf reads the four fine fluxes of
+//    each of its faces there and takes their mean (TwoSum-compensated when asked), which is
+//    what match_faces (amr/update.wgsl:207-290) recomputes from the finer block's atlas.
+//  - one launch for all leaves: workgroups take pairs from a device queue (leaves with no
+//    refined neighbour first, then the others level by level from the finest); a leaf with a
+//    refined neighbour first waits for the finer leaves across it (a done flag per slot, set
+//    with release order after their march). The queue hands out pairs in order and a leaf only
+//    waits for leaves earlier in it, all of which are held by running workgroups, so the waits
+//    end; each is bounded besides, and a wait that runs out sets an error flag the host reports.
 //
 // Thread layout: 384 threads (6 waves) own the columns of the block's 20 x 20 texel plane that
 // ever hold a value: threads 0-255 the 16 x 16 cell columns, 256-319 the 64 face-ghost columns
@@ -99,7 +103,7 @@ __device__ __forceinline__ void reg_put(T* __restrict__ R, int reg, int idx, con
 
 // The mean of the four fine fluxes through the face of cell (x, y, z) on side k, read from the
 // register of the refined neighbour's child; summed in match_faces' order (t2 outer, t1 inner)
-template <typename T, bool COMP>
+template <typename T, bool COMP, bool COHERENT>
 __device__ __forceinline__ Flux<T> reg_mean(const int* __restrict__ tab, const int* __restrict__ reg_of,
                                             const T* __restrict__ R, int np, int k, int x, int y, int z,
                                             const Flux<T>& fallback) {
@@ -120,9 +124,16 @@ __device__ __forceinline__ Flux<T> reg_mean(const int* __restrict__ tab, const i
   for (int j = 0; j < 2; j++)
 #pragma unroll
     for (int i = 0; i < 2; i++) {
-      const T4<T>* p = reinterpret_cast<const T4<T>*>(R + ((size_t)reg * 256 + (v0 + j) * 16 + (u0 + i)) * 8);
-      const T4<T> a = p[0], b = p[1];
-      const T fv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+      const T* p = R + ((size_t)reg * 256 + (v0 + j) * 16 + (u0 + i)) * 8;
+      T fv[8];
+      if constexpr (COHERENT) {
+        // device-coherent loads: written by another workgroup of this launch
+#pragma unroll
+        for (int q = 0; q < 8; q++) fv[q] = __hip_atomic_load(p + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        const T4<T> a = reinterpret_cast<const T4<T>*>(p)[0], b = reinterpret_cast<const T4<T>*>(p)[1];
+        fv[0] = a.x; fv[1] = a.y; fv[2] = a.z; fv[3] = a.w; fv[4] = b.x; fv[5] = b.y; fv[6] = b.z; fv[7] = b.w;
+      }
 #pragma unroll
       for (int q = 0; q < 8; q++) {
         if constexpr (COMP) {
@@ -144,7 +155,7 @@ __device__ __forceinline__ Flux<T> reg_mean(const int* __restrict__ tab, const i
 
 // Coarse-fine faces of the march, by a per-(slot, face) action: act >= 0 even: the neighbour is
 // coarser, write this face's fluxes to register act / 2; odd: the neighbour is refined, take
-// the face's fluxes from coarse-face buffer act / 2 (the means formed by k_amr_cfmean); -1:
+// the face's fluxes from coarse-face buffer act / 2 (the means its march formed first); -1:
 // neither.
 template <typename T>
 __device__ __forceinline__ Flux<T> boundary_face(int act, T* __restrict__ R, const T* __restrict__ CF, int idx,
@@ -159,30 +170,6 @@ __device__ __forceinline__ Flux<T> boundary_face(int act, T* __restrict__ R, con
     reg_put(R, act >> 1, idx, F);
   }
   return F;
-}
-
-// The coarse sides of coarse-fine faces: for each listed (slot, face) whose neighbour is refined,
-// the mean of the four fine fluxes of each of its 256 faces, from the finer leaves' registers
-// (written by their march, launched before); match_faces' order and arithmetic
-// (amr/update.wgsl:207-290).
-template <typename T, bool COMP>
-__global__ __launch_bounds__(256) void k_amr_cfmean(const int* __restrict__ tab, const int* __restrict__ reg_of,
-                                                    const T* __restrict__ R, const int2* __restrict__ faces,
-                                                    int first, T* __restrict__ CF) {
-  const int2 f = faces[blockIdx.x];
-  const int s = f.x, k = f.y, axis = k >> 1;
-  const int idx = threadIdx.x;            // t2 * 16 + t1 over the face
-  const int t1 = idx & 15, t2 = idx >> 4;
-  const int n = (k & 1) ? AMR_BS - 1 : 0;
-  const int x = axis == 0 ? n : (axis == 1 ? t2 : t1);
-  const int y = axis == 1 ? n : (axis == 2 ? t2 : t1);
-  const int z = axis == 2 ? n : (axis == 0 ? t2 : t1);
-  const int np = bt(tab, s, BT_NB + k);
-  Flux<T> F = zero_flux<T>();
-  F = reg_mean<T, COMP>(tab, reg_of, R, np, k, x, y, z, F);
-  T4<T>* p = reinterpret_cast<T4<T>*>(CF + ((size_t)(first + blockIdx.x) * 256 + idx) * 8);
-  p[0] = v4<T>(F.m, F.mx, F.my, F.mz);
-  p[1] = v4<T>(F.e, F.ei, F.s, F.vn);
 }
 
 template <typename T>
@@ -304,18 +291,30 @@ __global__ __launch_bounds__(256, sh_part_waves<PART>()) void k_amr_shell(
 }
 
 // ---------------- the march ----------------
+// Before each march: the queue starts at pair 0 and the step's epoch (the value done flags are
+// set to) moves on.
+template <typename T>
+__global__ void k_amr_queue_start(int* qctl) {
+  qctl[AMR_Q_NEXT] = 0;
+  qctl[AMR_Q_EPOCH] = qctl[AMR_Q_EPOCH] + 1;
+}
+
 // Launch bound: twelve waves per workgroup (two blocks), one workgroup per CU in FP32 (three
 // waves per SIMD: up to 168 VGPRs); six waves (one block) in FP64
 template <typename T, int LIM, int RS, bool ISEN>
 __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) == 4 ? 3 : 1)) void k_amr_fused(
     Params<T> P, const int* __restrict__ tab, const int* __restrict__ list, int count,
     const int* __restrict__ shell_of, const T4<T>* __restrict__ SH, const int* __restrict__ act,
-    T* __restrict__ R, const T* __restrict__ CF, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2,
-    T4<T>* __restrict__ O1, T4<T>* __restrict__ O2, const T* __restrict__ clk, unsigned long long* next_slot) {
+    T* __restrict__ R, T* __restrict__ CF, const int* __restrict__ reg_of, int comp, const int* __restrict__ dep_off,
+    const int* __restrict__ dep, int* __restrict__ qctl, int* __restrict__ done, const T4<T>* __restrict__ U1,
+    const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1, T4<T>* __restrict__ O2, const T* __restrict__ clk,
+    unsigned long long* next_slot) {
   constexpr int NT = AF_NT, BX = AF_SIDE, AF_PAIR = af_pair<T>();
   using LY = FusedLayout<T, NT>;
   __shared__ T LL[AF_PAIR * LY::SIZE];
   __shared__ T red[AF_PAIR * AF_THREADS / 64];
+  __shared__ int q_pair;
+  const int epoch = qctl[AMR_Q_EPOCH];
   // which of the workgroup's two blocks this thread marches, and its thread index there
   const int half = threadIdx.x >= AF_THREADS;
   const int tid = threadIdx.x - AF_THREADS * half;
@@ -335,11 +334,54 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
   const T dx0 = P.dx;
   T rate_max = T(0);
 
-  for (int wp = blockIdx.x; AF_PAIR * wp < count; wp += gridDim.x) {
-    // the pair's second block may be missing: that half takes part in the barriers only
+  for (;;) {
+    // the next pair from the queue (every thread has passed a barrier since it last read q_pair)
+    if (threadIdx.x == 0) q_pair = atomicAdd(&qctl[AMR_Q_NEXT], 1);
+    __syncthreads();
+    const int wp = q_pair;
+    __syncthreads();   // every thread has read q_pair before thread 0 takes the next ticket
+    if (AF_PAIR * wp >= count) break;
+    // -1 pads a queue part of odd length. With pairs it is always a pair's second entry (parts
+    // start at even positions): that half takes part in the barriers only. A one-block
+    // workgroup skips it.
+    if (list[AF_PAIR * wp] < 0) continue;
     const int wi = AF_PAIR * wp + half;
-    const bool act_blk = wi < count;
+    const bool act_blk = wi < count && list[wi] >= 0;
     const int s = list[act_blk ? wi : AF_PAIR * wp];
+    // a leaf with a refined neighbour: wait for the finer leaves across it (one lane each), then
+    // form the means of their fluxes on its coarse-fine faces
+    const int d0 = act_blk ? dep_off[s] : 0, nd = act_blk ? dep_off[s + 1] - d0 : 0;
+    // The flags and, after them, the finer leaves' register fluxes are read with device-coherent
+    // loads (no cache-wide invalidation; the writer wrote its registers back before its flag)
+    if (tid < nd) {
+      int* flag = done + dep[d0 + tid];
+      int spins = 0;
+      while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (++spins > AMR_WAIT_SPINS) {
+          __hip_atomic_store(&qctl[AMR_Q_ERROR], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __syncthreads();
+    if (nd > 0) {
+      for (int k = 0; k < 6; k++) {
+        const int a = act[s * 6 + k];
+        if (a < 0 || !(a & 1) || tid >= 256) continue;
+        const int axis = k >> 1, t1 = tid & 15, t2 = tid >> 4;
+        const int n = (k & 1) ? AMR_BS - 1 : 0;
+        const int fx = axis == 0 ? n : (axis == 1 ? t2 : t1);
+        const int fy = axis == 1 ? n : (axis == 2 ? t2 : t1);
+        const int fz = axis == 2 ? n : (axis == 0 ? t2 : t1);
+        const int np = bt(tab, s, BT_NB + k);
+        const Flux<T> F = comp ? reg_mean<T, true, true>(tab, reg_of, R, np, k, fx, fy, fz, zero_flux<T>())
+                               : reg_mean<T, false, true>(tab, reg_of, R, np, k, fx, fy, fz, zero_flux<T>());
+        T4<T>* q = reinterpret_cast<T4<T>*>(CF + ((size_t)(a >> 1) * 256 + tid) * 8);
+        q[0] = v4<T>(F.m, F.mx, F.my, F.mz);
+        q[1] = v4<T>(F.e, F.ei, F.s, F.vn);
+      }
+    }
     const T4<T>* sh = SH + (size_t)shell_of[s] * SH_ITEMS * 2;
     // this block's cell size (P is the kernel's own copy: its dx is the block's from here on)
     const T dx = dx0 * (T(1) / T(1 << bt(tab, s, BT_LEVEL)));
@@ -512,6 +554,19 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
         axy_prev = axy;
       }
     }
+    // a leaf on a coarser block: its register fluxes are written (the barrier waits for every
+    // wave's stores); one release fence writes them back to device-coherent memory, then its
+    // done flag
+    __syncthreads();
+    bool writes_reg = false;
+    for (int k = 0; k < 6; k++) {
+      const int a = act[s * 6 + k];
+      writes_reg = writes_reg || (a >= 0 && !(a & 1));
+    }
+    if (tid == 0 && act_blk && writes_reg) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_store(done + s, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
   // workgroup maximum of the next step's rate, one atomicMax
   T v = wave_max(rate_max);
@@ -536,28 +591,17 @@ void AmrImpl<T, LIM, RS, ISEN>::shell(const AmrLaunch& L, const int* work0, int 
 }
 
 template <typename T, int LIM, int RS, bool ISEN>
-void AmrImpl<T, LIM, RS, ISEN>::fused(const AmrLaunch& L, const int* list, int count, int grid, const int* shell_of,
-                                      const void* sh, const int* act, void* reg, const void* cf, const void* u1,
+void AmrImpl<T, LIM, RS, ISEN>::fused(const AmrLaunch& L, const AmrQueue& Q, const void* sh, const void* u1,
                                       const void* u2, void* o1, void* o2, const void* clk, unsigned long long* slot) {
-  if (count <= 0) return;
+  if (Q.count <= 0) return;
   constexpr int np = af_pair<T>();
-  const int pairs = (count + np - 1) / np;
-  if (grid > pairs) grid = pairs;
-  hipLaunchKernelGGL((k_amr_fused<T, LIM, RS, ISEN>), dim3(grid), dim3(np * AF_THREADS), 0, L.stream, P(L), L.tab, list,
-                     count, shell_of, (const T4<T>*)sh, act, (T*)reg, (const T*)cf, (const T4<T>*)u1,
-                     (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, slot);
-}
-
-template <typename T, int LIM, int RS, bool ISEN>
-void AmrImpl<T, LIM, RS, ISEN>::cfmean(const AmrLaunch& L, const int* faces, int count, int first, bool comp,
-                                       const int* reg_of, const void* reg, void* cf) {
-  if (count <= 0) return;
-  if (comp)
-    hipLaunchKernelGGL((k_amr_cfmean<T, true>), dim3(count), dim3(256), 0, L.stream, L.tab, reg_of, (const T*)reg,
-                       (const int2*)faces, first, (T*)cf);
-  else
-    hipLaunchKernelGGL((k_amr_cfmean<T, false>), dim3(count), dim3(256), 0, L.stream, L.tab, reg_of, (const T*)reg,
-                       (const int2*)faces, first, (T*)cf);
+  const int pairs = (Q.count + np - 1) / np;
+  const int grid = Q.grid < pairs ? Q.grid : pairs;
+  hipLaunchKernelGGL((k_amr_queue_start<T>), dim3(1), dim3(1), 0, L.stream, Q.qctl);
+  hipLaunchKernelGGL((k_amr_fused<T, LIM, RS, ISEN>), dim3(grid), dim3(np * AF_THREADS), 0, L.stream, P(L), L.tab,
+                     Q.list, Q.count, Q.shell_of, (const T4<T>*)sh, Q.act, (T*)Q.reg, (T*)Q.cf, Q.reg_of, Q.comp,
+                     Q.dep_off, Q.dep, Q.qctl, Q.done, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2,
+                     (const T*)clk, slot);
 }
 
 }  // namespace shx

@@ -625,15 +625,11 @@ struct AmrImpl {
   }
   static void shell(const AmrLaunch& L, const int* work0, int n0, const int* work1, int n1, const void* u1,
                     const void* u2, void* sh, const void* clk);
-  static void fused(const AmrLaunch& L, const int* list, int count, int grid, const int* shell_of, const void* sh,
-                    const int* act, void* reg, const void* cf, const void* u1, const void* u2, void* o1, void* o2,
-                    const void* clk, unsigned long long* slot);
-  static void cfmean(const AmrLaunch& L, const int* faces, int count, int first, bool comp, const int* reg_of,
-                     const void* reg, void* cf);
+  static void fused(const AmrLaunch& L, const AmrQueue& Q, const void* sh, const void* u1, const void* u2, void* o1,
+                    void* o2, const void* clk, unsigned long long* slot);
   static AmrOps ops() {
     AmrOps o;
     o.shell = shell;
-    o.cfmean = cfmean;
     o.fused = fused;
     o.assemble = assemble;
     o.clock = clock;

@@ -39,6 +39,7 @@ struct shx_amr {
   // ranges into d_lists: every block, leaves, leaves without / with a refined neighbour; the
   // same split per level
   Range all, leaf, plain, match;
+  Range queue;   // the fused march's queue (-1 pads)
   Range all_lv[AMR_MAXLEV], leaf_lv[AMR_MAXLEV], plain_lv[AMR_MAXLEV], match_lv[AMR_MAXLEV];
   void* U1[2] = {nullptr, nullptr};
   void* U2[2] = {nullptr, nullptr};
@@ -55,10 +56,14 @@ struct shx_amr {
   // coarse-fine faces seen from the coarse side: (slot, face) pairs by level, their buffers, and
   // the march's per-(slot, face) action (register write, coarse-face read, or none)
   int* d_act = nullptr;
-  int* d_cf_faces = nullptr;
   void* cf = nullptr;
   int ncf = 0;
-  Range cf_lv[AMR_MAXLEV];
+  // the march queue's dependencies (per slot, the finer leaves it waits for), control words and
+  // done flags
+  int* d_dep_off = nullptr;
+  int* d_dep = nullptr;
+  int* d_qctl = nullptr;
+  int* d_done = nullptr;
   int nreg = 0;
   int grid = 256;            // persistent workgroups of the fused kernel
   void* clk = nullptr;
@@ -188,6 +193,27 @@ int build_lists(shx_amr* s, std::vector<int>& lists) {
   add(s->leaf, [&](int i) { return is_leaf(i); });
   add(s->plain, [&](int i) { return is_leaf(i) && !has_fine(i); });
   add(s->match, [&](int i) { return is_leaf(i) && has_fine(i); });
+  // the fused march's queue, the chain of coarse-fine dependencies first: level by level from
+  // the finest, the leaves beside a coarser leaf (whose fluxes it waits for) and the leaves
+  // beside a refined block (which wait for the finer leaves, one part earlier); then every
+  // other leaf, which fills the CUs while the chain runs. Each part is padded to an even
+  // length, so a workgroup's pair never holds a leaf and a finer leaf it waits for.
+  auto has_coarse = [&](int i) {
+    for (int f = 0; f < 6; f++)
+      if (s->tab[(size_t)i * AMR_BT + BT_NB + f] == -1) return true;
+    return false;
+  };
+  s->queue.off = (int)lists.size();
+  auto pad = [&] { if (((int)lists.size() - s->queue.off) & 1) lists.push_back(-1); };
+  for (int L = s->nlevels - 1; L >= 0; L--) {
+    for (int i : order)
+      if (s->blocks[4 * i] == L && is_leaf(i) && (has_coarse(i) || has_fine(i))) lists.push_back(i);
+    pad();
+  }
+  for (int i : order)
+    if (is_leaf(i) && !has_coarse(i) && !has_fine(i)) lists.push_back(i);
+  pad();
+  s->queue.count = (int)lists.size() - s->queue.off;
   for (int L = 0; L < s->nlevels; L++) {
     auto lv = [&](int i) { return s->blocks[4 * i] == L; };
     add(s->all_lv[L], lv);
@@ -225,22 +251,11 @@ int enqueue_step(shx_amr* s, int p, int stage = -1) {
       o.shell(L, s->d_shw[0], s->nshw[0], s->d_shw[1], s->nshw[1], s->U1[p], s->U2[p], s->shell, s->clk);
       n += (s->nshw[0] > 0) + (s->nshw[1] > 0);
     }
-    if (run(3)) {
-      auto fz = [&](const Range& r) {
-        if (!r.count) return;
-        o.fused(L, lp(s, r), r.count, s->grid, s->d_shell_of, s->shell, s->d_act, s->reg, s->cf, s->U1[p], s->U2[p],
-                s->U1[1 - p], s->U2[1 - p], s->clk, s->slot);
-        n++;
-      };
-      fz(s->plain);
-      for (int l = s->nlevels - 1; l >= 0; l--) {
-        if (s->cf_lv[l].count) {
-          o.cfmean(L, s->d_cf_faces + 2 * s->cf_lv[l].off, s->cf_lv[l].count, s->cf_lv[l].off, comp, s->d_reg_of,
-                   s->reg, s->cf);
-          n++;
-        }
-        fz(s->match_lv[l]);
-      }
+    if (run(3) && s->queue.count > 0) {
+      AmrQueue Q{lp(s, s->queue), s->queue.count, s->grid, s->d_shell_of, s->d_act, s->reg, s->cf, s->d_reg_of,
+                 comp ? 1 : 0, s->d_dep_off, s->d_dep, s->d_qctl, s->d_done};
+      o.fused(L, Q, s->shell, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, s->slot);
+      n += 2;
     }
     return n;
   }
@@ -336,6 +351,20 @@ int set_state_t(shx_amr* s, const T* u1, const T* u2, size_t ncell, double time,
   return SHX_OK;
 }
 
+// After a synchronisation: a dependency wait of the fused march that ran out (the step's
+// coarse-fine means were formed without the finer fluxes) is an error, reported once.
+int check_queue(shx_amr* s) {
+  if (!s->d_qctl) return SHX_OK;
+  int w[AMR_Q_WORDS];
+  HIPCHK(hipMemcpy(w, s->d_qctl, sizeof w, hipMemcpyDeviceToHost));
+  if (w[AMR_Q_ERROR]) {
+    const int zero = 0;
+    HIPCHK(hipMemcpy(s->d_qctl + AMR_Q_ERROR, &zero, sizeof zero, hipMemcpyHostToDevice));
+    return fail(SHX_EDEVICE, "the block march's wait for finer leaves timed out; the state is unreliable");
+  }
+  return SHX_OK;
+}
+
 template <typename T>
 int read_state_t(shx_amr* s, T* u1, T* u2) {
   if (!s) return fail(SHX_EINVAL, "null sim");
@@ -344,7 +373,7 @@ int read_state_t(shx_amr* s, T* u1, T* u2) {
   HIPCHK(hipMemcpyAsync(u1, s->U1[s->cur], bytes, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipMemcpyAsync(u2, s->U2[s->cur], bytes, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
-  return SHX_OK;
+  return check_queue(s);
 }
 
 }  // namespace
@@ -409,30 +438,36 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   bool ok = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess;
   for (int p = 0; ok && p < 2; p++) ok = alloc(&s->U1[p], ubytes) && alloc(&s->U2[p], ubytes);
   if (!fused) ok = ok && alloc(&s->W1, abytes) && alloc(&s->W2, abytes) && alloc(&s->PA, abytes) && alloc(&s->PB, abytes);
-  // coarse sides of coarse-fine faces, by level; the march's action per (slot, face)
-  std::vector<int> act((size_t)nblocks * 6, -1), cf_faces;
+  // the march's action per (slot, face)
+  std::vector<int> act((size_t)nblocks * 6, -1);
   for (int f = 0; f < nblocks * 6; f++)
     if (reg_of[f] >= 0) act[f] = 2 * reg_of[f];
-  for (int l = 0; l < s->nlevels; l++) {
-    s->cf_lv[l].off = s->ncf;
-    for (int j = 0; j < s->match_lv[l].count; j++) {
-      const int i = lists[s->match_lv[l].off + j];
-      for (int f = 0; f < 6; f++) {
-        const int nb = s->tab[(size_t)i * AMR_BT + BT_NB + f];
-        if (nb >= 0 && !(s->tab[(size_t)nb * AMR_BT + BT_FLAGS] & 2)) {
-          act[(size_t)i * 6 + f] = 2 * s->ncf + 1;
-          cf_faces.push_back(i);
-          cf_faces.push_back(f);
-          s->ncf++;
-        }
+  // per leaf with a refined neighbour: a coarse-face buffer per such face, and the finer leaves
+  // across them (the four children of the neighbour on the face), which its march waits for
+  std::vector<int> dep_off(nblocks + 1, 0), dep;
+  for (int i = 0; i < nblocks; i++) {
+    dep_off[i] = (int)dep.size();
+    if (!(s->tab[(size_t)i * AMR_BT + BT_FLAGS] & 2)) continue;   // not a leaf
+    for (int f = 0; f < 6; f++) {
+      const int nb = s->tab[(size_t)i * AMR_BT + BT_NB + f];
+      if (nb < 0 || (s->tab[(size_t)nb * AMR_BT + BT_FLAGS] & 2)) continue;   // not a refined neighbour
+      act[(size_t)i * 6 + f] = 2 * s->ncf + 1;
+      s->ncf++;
+      const int axis = f >> 1, bit = (f & 1) ? 0 : 1;   // the neighbour's children on the shared face
+      for (int c = 0; c < 8; c++) {
+        if (((c >> axis) & 1) != bit) continue;
+        const int ch = s->tab[(size_t)nb * AMR_BT + BT_CHILD + c];
+        if (ch >= 0) dep.push_back(ch);
       }
     }
-    s->cf_lv[l].count = s->ncf - s->cf_lv[l].off;
   }
+  dep_off[nblocks] = (int)dep.size();
   ok = ok && alloc((void**)&s->d_reg_of, reg_of.size() * sizeof(int)) &&
        alloc(&s->reg, (size_t)(s->nreg > 0 ? s->nreg : 1) * 256 * 8 * s->es) &&
        alloc((void**)&s->d_act, act.size() * sizeof(int)) &&
-       alloc((void**)&s->d_cf_faces, (cf_faces.size() > 0 ? cf_faces.size() : 2) * sizeof(int)) &&
+       alloc((void**)&s->d_dep_off, dep_off.size() * sizeof(int)) &&
+       alloc((void**)&s->d_dep, (dep.empty() ? 1 : dep.size()) * sizeof(int)) &&
+       alloc((void**)&s->d_qctl, AMR_Q_WORDS * sizeof(int)) && alloc((void**)&s->d_done, nblocks * sizeof(int)) &&
        alloc(&s->cf, (size_t)(s->ncf > 0 ? s->ncf : 1) * 256 * 8 * s->es);
   // ghost shells: 5184 items of 8 values per leaf (the fused path)
   std::vector<int> shell_of(nblocks, -1);
@@ -482,7 +517,10 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   bool copied = hipMemcpy(s->d_tab, s->tab.data(), s->tab.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
                 hipMemcpy(s->d_reg_of, reg_of.data(), reg_of.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
                 hipMemcpy(s->d_act, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
-                (cf_faces.empty() || hipMemcpy(s->d_cf_faces, cf_faces.data(), cf_faces.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess) &&
+                hipMemcpy(s->d_dep_off, dep_off.data(), dep_off.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+                (dep.empty() || hipMemcpy(s->d_dep, dep.data(), dep.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess) &&
+                hipMemset(s->d_qctl, 0, AMR_Q_WORDS * sizeof(int)) == hipSuccess &&
+                hipMemset(s->d_done, 0, nblocks * sizeof(int)) == hipSuccess &&
                 hipMemcpy(s->d_shell_of, shell_of.data(), shell_of.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
                 (shw[0].empty() || hipMemcpy(s->d_shw[0], shw[0].data(), shw[0].size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess) &&
                 (shw[1].empty() || hipMemcpy(s->d_shw[1], shw[1].data(), shw[1].size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess) &&
@@ -505,7 +543,8 @@ int shx_amr_destroy(shx_amr* s) {
   (void)hipFree(s->W1); (void)hipFree(s->W2); (void)hipFree(s->PA); (void)hipFree(s->PB);
   (void)hipFree(s->d_reg_of); (void)hipFree(s->reg); (void)hipFree(s->shell); (void)hipFree(s->d_shell_of);
   (void)hipFree(s->d_shw[0]); (void)hipFree(s->d_shw[1]);
-  (void)hipFree(s->d_act); (void)hipFree(s->d_cf_faces); (void)hipFree(s->cf);
+  (void)hipFree(s->d_act); (void)hipFree(s->cf);
+  (void)hipFree(s->d_dep_off); (void)hipFree(s->d_dep); (void)hipFree(s->d_qctl); (void)hipFree(s->d_done);
   (void)hipFree(s->d_tab); (void)hipFree(s->d_lists); (void)hipFree(s->clk); (void)hipFree(s->slot);
   (void)hipFree(s->diag_part); (void)hipFree(s->diag_out);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -541,7 +580,7 @@ int shx_amr_step(shx_amr* s, int count) {
 int shx_amr_sync(shx_amr* s) {
   if (!s) return fail(SHX_EINVAL, "null sim");
   HIPCHK(hipStreamSynchronize(s->stream));
-  return SHX_OK;
+  return check_queue(s);
 }
 
 int shx_amr_read_clock(shx_amr* s, double out[11]) {
