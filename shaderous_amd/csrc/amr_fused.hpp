@@ -26,8 +26,9 @@
 // of ring 1, 320-383 the 64 of ring 2 (x or y = -2 or 17, the other in range). The 16 remaining
 // columns (both x and y outside the cells) are edges or corners at every plane: zero, as in the
 // atlas. So the cell waves never diverge on ghost work. LDS holds the plane data in raster order
-// of the 20 x 20 columns (the uniform fused kernel's layout). Six waves at up to 168 VGPRs keep
-// two workgroups per CU.
+// of the 20 x 20 columns, fluxes apart from sides (the role-tiled uniform kernel's layout, 73.6 KB
+// per block in FP32), so a plane needs two barriers: after the half steps, after the fluxes.
+// Two blocks share a 12-wave workgroup (one per CU at up to 168 VGPRs).
 #pragma once
 #include "amr_impl.hpp"
 
@@ -310,7 +311,7 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
     const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1, T4<T>* __restrict__ O2, const T* __restrict__ clk,
     unsigned long long* next_slot) {
   constexpr int NT = AF_NT, BX = AF_SIDE, AF_PAIR = af_pair<T>();
-  using LY = FusedLayout<T, NT>;
+  using LY = RoleLayout<NT>;
   __shared__ T LL[AF_PAIR * LY::SIZE];
   __shared__ T red[AF_PAIR * AF_THREADS / 64];
   __shared__ int q_pair;
@@ -323,7 +324,7 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
   af_column(tid, x, y);
   const int t = (y + 2) * BX + (x + 2);                  // raster index of the column in LDS
   // the columns no thread owns are edges at every plane: zero in both plane slots, for good
-  for (int i = tid; i < 2 * 7 * NT; i += AF_THREADS) L[i] = T(0);
+  for (int i = tid; i < 12 * NT; i += AF_THREADS) L[i] = T(0);
   // roles by wave (uniform branches): waves 0-3 the cell columns, wave 4 the face ghosts of
   // ring 1, wave 5 ring 2 (loads only)
   const int wave = __builtin_amdgcn_readfirstlane(tid) >> 6;
@@ -424,9 +425,15 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
       w.p = b.x; w.s = b.y; w.cs = b.z;
       return w;
     };
-    lds_put_prim<T, NT>(L + LY::Q0, t, texel(-2));
-    lds_put_prim<T, NT>(L + LY::Q1, t, texel(-1));
+    lds_put_prim6<T, NT>(L + LY::Q0, t, texel(-2));
+    T cs_c;   // this column's sound speed in plane c
+    {
+      const Prim<T> q = texel(-1);
+      lds_put_prim6<T, NT>(L + LY::Q1, t, q);
+      cs_c = q.cs;
+    }
     Prim<T> nxt = texel(0);
+    __syncthreads();
 
     SideData<T> zs_prev;
     Flux<T> fz_lo = zero_flux<T>();
@@ -438,14 +445,14 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
       const bool odd = ((c + 1) & 1) != 0;
       const T* Lq = L + (odd ? LY::Q0 : LY::Q1);
       T* Lm = L + (odd ? LY::Q1 : LY::Q0);
-      __syncthreads();
 
       // 2. the half step of texel (x, y, c) and its sides: every cell; a ghost plane's texels
       //    (c = -1, 16) in the cell waves; the face ghosts of the cell planes in the ghost wave
       const bool oz = c < 0 || c >= AMR_BS;
       SideData<T> sxl, syl, szl, szu;
       if (cells || (ghosts && !oz)) {
-        const Prim<T> pc = lds_prim_c<T, NT>(Lq, t);
+        Prim<T> pc = lds_prim<T, NT>(Lq, t);
+        pc.cs = cs_c;
         const Prim<T> pm = lds_prim<T, NT>(Lm, t);
         const Prim<T> xm = lds_prim<T, NT>(Lq, t - 1), xp = lds_prim<T, NT>(Lq, t + 1);
         const Prim<T> ym = lds_prim<T, NT>(Lq, t - BX), yp = lds_prim<T, NT>(Lq, t + BX);
@@ -488,7 +495,8 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
         lds_put_side<T, NT>(L, LY::SY, t, side_of(hc, iyu, T(1)));
       }
       // plane c + 1 into the slot plane c - 1 leaves
-      lds_put_prim<T, NT>(Lm, t, pp);
+      lds_put_prim6<T, NT>(Lm, t, pp);
+      cs_c = pp.cs;
       __syncthreads();
 
       // 3. the x and y faces below this column (cells; the +x and +y ghosts' lower faces), and
@@ -510,7 +518,6 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
         if (c == 0) fz = boundary_face<T>(act[s * 6 + 4], R, CF, y * 16 + x, fz);
         else if (c == AMR_BS) fz = boundary_face<T>(act[s * 6 + 5], R, CF, y * 16 + x, fz);
       }
-      __syncthreads();
       // 4. the x/y fluxes into LDS
       if (live && (cells || (ghosts && side == 1))) lds_put_flux<T, NT>(L, LY::FX, t, Fx);
       if (live && (cells || (ghosts && side == 3))) lds_put_flux<T, NT>(L, LY::FY, t, Fy);

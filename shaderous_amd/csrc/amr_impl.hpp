@@ -621,7 +621,7 @@ struct AmrImpl {
                    double* out) {
     hipLaunchKernelGGL(k_amr_diag<T>, dim3(count), dim3(256), 0, L.stream, P(L), L.tab, list, L.dx0,
                        (const T4<T>*)u1, (const T4<T>*)u2, part);
-    hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(64), 0, L.stream, (const double*)part, count, out);
+    hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(1024), 0, L.stream, (const double*)part, count, out);
   }
   static void shell(const AmrLaunch& L, const int* work0, int n0, const int* work1, int n1, const void* u1,
                     const void* u2, void* sh, const void* clk);

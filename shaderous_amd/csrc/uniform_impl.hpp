@@ -551,16 +551,27 @@ __global__ __launch_bounds__(256) void k_diag(Params<T> P, const T4<T>* __restri
   if (threadIdx.x < 16) part[blockIdx.x * 16 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
+// The 16 channels over the partial rows: 64 groups of consecutive rows per channel, each folded
+// in row order, then the 64 group results in group order (a fixed order: deterministic).
+// Launch: one workgroup of 1024 threads.
 template <typename T>
-__global__ void k_diag_final(const double* part, int nblocks, double* out) {
-  int c = threadIdx.x;
-  if (c >= 16) return;
-  double a = (c == 10 || c == 12) ? 1e300 : (c == 11 || c == 13) ? -1e300 : 0.0;
-  for (int b = 0; b < nblocks; b++) {
-    double x = part[b * 16 + c];
-    a = (c == 7 || c == 11 || c == 13) ? fmax(a, x) : (c == 10 || c == 12) ? fmin(a, x) : a + x;
+__global__ __launch_bounds__(1024) void k_diag_final(const double* part, int nblocks, double* out) {
+  __shared__ double grp[64][17];
+  const int c = threadIdx.x & 15, g = threadIdx.x >> 4;
+  auto fold = [c](double a, double x) {
+    return (c == 7 || c == 11 || c == 13) ? fmax(a, x) : (c == 10 || c == 12) ? fmin(a, x) : a + x;
+  };
+  const double id = (c == 10 || c == 12) ? 1e300 : (c == 11 || c == 13) ? -1e300 : 0.0;
+  const int per = (nblocks + 63) / 64, b0 = g * per, b1 = min(b0 + per, nblocks);
+  double a = id;
+  for (int b = b0; b < b1; b++) a = fold(a, part[b * 16 + c]);
+  grp[g][c] = a;
+  __syncthreads();
+  if (g == 0) {
+    double r = id;
+    for (int k = 0; k < 64; k++) r = fold(r, grp[k][c]);
+    out[c] = r;
   }
-  out[c] = a;
 }
 
 // ---------------- launchers ----------------
@@ -651,7 +662,7 @@ struct UniformImpl {
     const Params<T>& P = *static_cast<const Params<T>*>(L.params);
     hipLaunchKernelGGL(k_diag<T>, dim3(nblocks), dim3(256), 0, L.stream, P, (const T4<T>*)u1, (const T4<T>*)u2,
                        (int)L.ncell, part);
-    hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(64), 0, L.stream, (const double*)part, nblocks, out);
+    hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(1024), 0, L.stream, (const double*)part, nblocks, out);
   }
   static UniformOps ops() {
     UniformOps o;
