@@ -175,6 +175,18 @@ int shx_amr_flux_mismatch(shx_amr* sim, int64_t counts[3], double* worst_coarse_
 /* Counts: out[0] blocks, [1] leaves, [2] leaves with a refined neighbour, [3] levels,
  * [4] kernel launches per step. */
 int shx_amr_stats(shx_amr* sim, int out[5]);
+/* The refinement indicator of every leaf (AmrSim's flagging pass, src/amr/amr_sim.ts:3029-3060,
+ * with BASELINE-style density jumps in place of amr/flag.wgsl's Loehner estimate): out[slot] =
+ * the largest |rho_a - rho_b| / min(rho_a, rho_b) between face-adjacent cells of the leaf;
+ * NaN for parents. out holds one double per block. */
+int shx_amr_indicator(shx_amr* sim, double* out);
+/* Re-mesh (AmrSim.applyRegrid, src/amr/amr_sim.ts:2648-2697, amr/regrid.wgsl:40-139): replace
+ * the tree by `blocks` (nblocks x 4 ints as for shx_amr_create, 2:1 balanced), which must lie
+ * within one level of the current tree everywhere: each new leaf is a current leaf (copied), a
+ * child of one (prolonged: conservative, limited, the children's mean energy kept) or a current
+ * parent of eight leaves (their mean). The clock carries on; table, lists and graphs are rebuilt.
+ * On failure the sim is unchanged. */
+int shx_amr_regrid(shx_amr* sim, const int* blocks, int nblocks);
 
 /* ---------------- ensembles of independent 2D grids ----------------
  * The reference runs one Sim per device (src/sim.ts); an ensemble advances `count` independent

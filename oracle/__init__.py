@@ -82,6 +82,9 @@ def lib():
             getattr(_lib, pfx + "amr_assemble").restype = ct
             getattr(_lib, pfx + "amr_prolong").argtypes = [C.POINTER(OrcParams), P, C.POINTER(C.c_int),
                                                            C.POINTER(C.c_int), P]
+            getattr(_lib, pfx + "amr_regrid").argtypes = [C.POINTER(OrcParams), C.POINTER(OrcAmr), P, P, C.c_int,
+                                                          C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                          P, P]
         I3 = C.c_int * 3
         _lib.orc_amr_build.argtypes = [C.POINTER(OrcAmr), C.POINTER(C.c_int), C.c_int, I3, I3, I3]
         _lib.orc_amr_free.argtypes = [C.POINTER(OrcAmr)]
@@ -224,6 +227,19 @@ class Wgsl:
         out = np.empty(8, self.dtype)
         self.fn("amr_prolong")(C.byref(prm), _ptr(st, self.ct), okc, pc, _ptr(out, self.ct))
         return out
+
+    def amr_regrid(self, prm, table, u1, u2, kind, src, oct):
+        """Re-mesh: the new tree's leaf interiors from the old tree's (table, u1, u2), per new slot
+        kind (0 none, 1 copy, 2 prolong, 3 restrict), source old slot and child octant. Returns
+        (o1, o2) of len(kind) * 4096 rows (zero where kind is 0)."""
+        kind, src, oct = (np.ascontiguousarray(a, np.int32) for a in (kind, src, oct))
+        n = len(kind)
+        o1 = np.zeros((n * BS ** 3, 4), self.dtype)
+        o2 = np.zeros((n * BS ** 3, 4), self.dtype)
+        self.fn("amr_regrid")(C.byref(prm), C.byref(table.t), _ptr(self.arr(u1), self.ct), _ptr(self.arr(u2), self.ct),
+                              n, _ptr(kind, C.c_int), _ptr(src, C.c_int), _ptr(oct, C.c_int), _ptr(o1, self.ct),
+                              _ptr(o2, self.ct))
+        return o1, o2
 
     def run(self, prm, u1, u2, steps, clk=None):
         """Advance `steps` steps in place; returns the clock."""

@@ -1051,6 +1051,61 @@ real FN(amr_assemble)(const orc_params* prm, const orc_amr* t, double dx0, const
   return amr_assemble(t, &P, dx0, u1, u2, W);
 }
 
+/* Re-meshing (src/amr/regrid.wgsl:40-139): every leaf of a new tree from the old tree's state.
+   Per new slot: kind 0 not a leaf (left as given), 1 copy of old slot src, 2 prolongation of
+   old leaf src to its child of octant oct, 3 restriction of old parent src (the mean of its 8
+   leaf children). The prolongation's stencil is the old state's (u_near, as the ghost fill's
+   from_coarse); its total energy is each child's own kinetic energy plus its interpolated
+   internal energy plus the offset that keeps the children's mean the coarse cell's
+   (regrid.wgsl:86-97). */
+static real ke_of(const P_t* P, const Cons* u) {
+  return RL(0.5) * ((u->a[1] * u->a[1] + u->a[2] * u->a[2]) + u->a[3] * u->a[3]) / rmax(u->a[0], P->rho_floor);
+}
+void FN(amr_regrid)(const orc_params* prm, const orc_amr* told, const real* u1, const real* u2, int nnew,
+                    const int* kind, const int* src, const int* oct, real* o1, real* o2) {
+  P_t P = load_params(prm);
+  for (int b = 0; b < nnew; b++) {
+    if (kind[b] == 0) continue;
+    for (int z = 0; z < BS_; z++)
+      for (int y = 0; y < BS_; y++)
+        for (int x = 0; x < BS_; x++) {
+          int l[3] = {x, y, z};
+          Cons u;
+          if (kind[b] == 1) {
+            u = u_leaf(told, u1, u2, src[b], l);
+          } else if (kind[b] == 3) {
+            u = u_at(told, u1, u2, src[b], l);
+          } else {
+            int lc[3], par[3];
+            for (int a = 0; a < 3; a++) {
+              lc[a] = (l[a] >> 1) + (BS_ / 2) * ((oct[b] >> a) & 1);
+              par[a] = l[a] & 1;
+            }
+            Cons v[7];
+            int ok[7];
+            for (int k = 0; k < 7; k++) {
+              int at[3] = {lc[0], lc[1], lc[2]};
+              if (k > 0) at[(k - 1) / 2] += (k & 1) == 0 ? 1 : -1;
+              ok[k] = u_near(told, u1, u2, src[b], at, &v[k]);
+            }
+            u = prolong(&P, v, ok, par);
+            real ke = RL(0.0);
+            for (int k = 0; k < 2; k++)
+              for (int j = 0; j < 2; j++)
+                for (int i = 0; i < 2; i++) {
+                  int pc[3] = {i, j, k};
+                  Cons ch = prolong(&P, v, ok, pc);
+                  ke = ke + ke_of(&P, &ch);
+                }
+            ke = ke * RL(0.125);
+            u.b[0] = ke_of(&P, &u) + u.b[1] + ((v[0].b[0] - v[0].b[1]) - ke);
+          }
+          size_t o = ((size_t)b * BS_ * BS_ * BS_ + ((size_t)z * BS_ + y) * BS_ + x) * 4;
+          for (int i = 0; i < 4; i++) { o1[o + i] = u.a[i]; o2[o + i] = u.b[i]; }
+        }
+  }
+}
+
 void FN(amr_prolong)(const orc_params* prm, const real* cons56, const int ok[7], const int par[3], real out[8]) {
   P_t P = load_params(prm);
   Cons v[7];

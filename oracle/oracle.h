@@ -119,7 +119,12 @@ const int* orc_amr_table(const orc_amr* t);
      conserved states (centre, -x, +x, -y, +y, -z, +z; 8 reals each; ok[k] = 0 drops it),     \
      to the fine child of parity par[3] */                                                     \
   void P##amr_prolong(const orc_params* prm, const T* cons56, const int ok[7], const int par[3], \
-                      T out[8]);
+                      T out[8]);                                                                \
+  /* re-meshing (src/amr/regrid.wgsl:40-139): per new slot b, kind[b] 0 none, 1 copy of old     \
+     slot src[b], 2 prolongation of old leaf src[b] into its child oct[b], 3 restriction of old \
+     parent src[b]; o1/o2 hold nnew blocks */                                                   \
+  void P##amr_regrid(const orc_params* prm, const orc_amr* told, const T* u1, const T* u2, int nnew, \
+                     const int* kind, const int* src, const int* oct, T* o1, T* o2);
 
 ORC_DECLARE(float, orc_f32_)
 ORC_DECLARE(double, orc_f64_)

@@ -89,6 +89,8 @@ def load():
         "shx_amr_profile": (I, [P, I, D]),
         "shx_amr_stats": (I, [P, C.POINTER(I)]),
         "shx_amr_flux_mismatch": (I, [P, C.POINTER(C.c_int64), D]),
+        "shx_amr_indicator": (I, [P, C.POINTER(C.c_double)]),
+        "shx_amr_regrid": (I, [P, C.POINTER(I), I]),
         "shx_ens_create": (I, [C.POINTER(ShxConfig), I, C.POINTER(I), C.POINTER(P)]),
         "shx_ens_destroy": (I, [P]),
         "shx_ens_set_state": (I, [P, I, F, F, C.c_size_t]),

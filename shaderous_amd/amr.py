@@ -2,10 +2,11 @@
 
 The reference's AmrSim (src/amr/amr_sim.ts) runs the same hydro step on 16^3-cell blocks of a
 BlockTree (src/amr/tree.ts): ``create`` (:1924-1992, with uploadTree :2566-2625), ``step`` /
-encodeSteps (:2709-2947), ``readLeaves``, ``diagnostics``, ``destroy``. The tree here is fixed at
-creation (re-meshing is not part of this path) and is given as the list of its blocks, parents and
-leaves, as (level, ix, iy, iz). Every call goes through the C ABI of libshx (include/shx.h,
-shx_amr_*); there is no CPU path.
+encodeSteps (:2709-2947), ``readLeaves``, ``diagnostics``, re-meshing (the flagging pass
+:3029-3060 and applyRegrid :2648-2697), ``destroy``. The tree is given as the list of its blocks,
+parents and leaves, as (level, ix, iy, iz); ``regrid`` replaces it by one within a level of it
+everywhere and ``adapt`` chooses that tree from the refinement indicator. Every call goes through
+the C ABI of libshx (include/shx.h, shx_amr_*); there is no CPU path.
 """
 from __future__ import annotations
 
@@ -52,6 +53,36 @@ class AmrSim:
 
     def leaf_mask(self) -> np.ndarray:
         return (self.table()[:, 4] & 2) != 0
+
+    # -- re-meshing --
+    def indicator(self) -> np.ndarray:
+        """Per block the largest relative density jump between face-adjacent cells of the leaf
+        (NaN for parents): the refinement indicator."""
+        out = np.empty(self.nblocks, np.float64)
+        check(_lib.load().shx_amr_indicator(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def regrid(self, blocks):
+        """Replace the tree by `blocks` (within one level of the current tree everywhere): leaves
+        copied, prolonged from their parent or restricted from their children; the clock carries
+        on."""
+        blocks = np.ascontiguousarray(blocks, dtype=np.int32).reshape(-1, 4)
+        check(_lib.load().shx_amr_regrid(self._h, blocks.ctypes.data_as(C.POINTER(C.c_int)), len(blocks)))
+        self.blocks = blocks
+        self.nblocks = len(blocks)
+        self.ncell = self.nblocks * CELLS
+
+    def adapt(self, refine_above: float, coarsen_below: float, max_level: int, periodic=(True, True, True)) -> bool:
+        """One re-mesh from the indicator (amr_tree.adapt_tree): split leaves above
+        `refine_above`, merge parents of leaves all below `coarsen_below`. Returns whether the
+        tree changed."""
+        from . import amr_tree
+        new = amr_tree.adapt_tree(self.blocks, self.root, self.indicator(), refine_above, coarsen_below, max_level,
+                                  periodic)
+        if len(new) == len(self.blocks) and {tuple(b) for b in new.tolist()} == {tuple(b) for b in self.blocks.tolist()}:
+            return False
+        self.regrid(new)
+        return True
 
     def stats(self) -> dict:
         out = (C.c_int * 5)()

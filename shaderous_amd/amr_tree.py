@@ -3,8 +3,9 @@
 A tree is the list of its blocks as (level, ix, iy, iz), parents and leaves, in the reference's
 convention (src/amr/tree.ts:14-53): block coordinates count blocks of that level from the
 domain's lower corner, the root grid has `root` blocks per side, and a block of level L covers
-cells [16 c 2^-L dx0 ...). Trees here are built once on the host (regridding during a run is not
-part of this path) and must keep 2:1 balance across faces (tree.ts neighbor :293-324).
+cells [16 c 2^-L dx0 ...). Trees are built on the host and must keep 2:1 balance across faces
+(tree.ts neighbor :293-324). A run re-meshes by handing the sim a new tree that differs from the
+old one by at most one level everywhere (`adapt_tree`, `regrid_map`).
 """
 from __future__ import annotations
 
@@ -142,3 +143,60 @@ def blast_tree(root=8, max_level=3, radius=0.1, center=(0.0, 0.0, 0.0), threshol
         # the region straddles the sphere's surface: a pressure jump p_in/p_out there
         return near <= radius <= far and abs(p_in - p_out) / min(p_in, p_out) > threshold
     return build_tree((root, root, root), max_level, refine), dx0
+
+
+def regrid_map(old_blocks, new_blocks):
+    """What re-meshing does for each block of the new tree (src/amr/regrid.wgsl:40-139, the
+    table of tree.ts:681-720): kind 0 (not a leaf), 1 copy of an old leaf, 2 prolongation of an
+    old leaf into its child `oct`, 3 restriction of an old parent whose children are all old
+    leaves. Returns int32 arrays (kind, src, oct), src an old slot. Raises ValueError for a new
+    leaf more than one level from the old tree."""
+    old = {tuple(b): i for i, b in enumerate(np.asarray(old_blocks).tolist())}
+    old_leaf = leaf_mask(np.asarray(old_blocks))
+    new_blocks = np.asarray(new_blocks)
+    new_leaf = leaf_mask(new_blocks)
+    n = len(new_blocks)
+    kind = np.zeros(n, np.int32)
+    src = np.zeros(n, np.int32)
+    oct_ = np.zeros(n, np.int32)
+    for j, (L, x, y, z) in enumerate(new_blocks.tolist()):
+        if not new_leaf[j]:
+            continue
+        i = old.get((L, x, y, z))
+        if i is not None and old_leaf[i]:
+            kind[j], src[j] = 1, i
+            continue
+        if i is not None:
+            kids = [old.get((L + 1, 2 * x + (o & 1), 2 * y + ((o >> 1) & 1), 2 * z + ((o >> 2) & 1))) for o in range(8)]
+            if all(k is not None and old_leaf[k] for k in kids):
+                kind[j], src[j] = 3, i
+                continue
+        p = old.get((L - 1, x >> 1, y >> 1, z >> 1)) if L > 0 else None
+        if p is not None and old_leaf[p]:
+            kind[j], src[j], oct_[j] = 2, p, (x & 1) | ((y & 1) << 1) | ((z & 1) << 2)
+            continue
+        raise ValueError(f"new leaf {(L, x, y, z)} is more than one level from the old tree")
+    return kind, src, oct_
+
+
+def adapt_tree(blocks, root, indicator, refine_above, coarsen_below, max_level, periodic=(True, True, True)):
+    """The next tree from the current one and a per-block indicator (NaN for parents): leaves
+    above `refine_above` split (below `max_level`), parents whose eight children are leaves all
+    below `coarsen_below` merge, then 2:1 balance restored by splitting (which can undo a merge).
+    Every block stays within one level of the current tree."""
+    blocks = np.asarray(blocks)
+    leaf = leaf_mask(blocks)
+    keys = {tuple(b): i for i, b in enumerate(blocks.tolist())}
+    split = {(L, (x, y, z)) for (L, x, y, z), lf, v in zip(blocks.tolist(), leaf, indicator)
+             if lf and L < max_level and v > refine_above}
+    parents = {(L, (x, y, z)) for (L, x, y, z), lf in zip(blocks.tolist(), leaf) if not lf}
+    merge = set()
+    for (L, c) in parents:
+        kids = [keys.get((L + 1, 2 * c[0] + (o & 1), 2 * c[1] + ((o >> 1) & 1), 2 * c[2] + ((o >> 2) & 1)))
+                for o in range(8)]
+        if all(k is not None and leaf[k] and indicator[k] < coarsen_below for k in kids):
+            merge.add((L, c))
+
+    def refine(level, c):
+        return ((level, c) in parents and (level, c) not in merge) or (level, c) in split
+    return build_tree(root, max_level, refine, periodic)

@@ -84,6 +84,13 @@ struct AmrOps {
                 void* sh, const void* clk);
   void (*fused)(const AmrLaunch&, const AmrQueue& Q, const void* sh, const void* u1, const void* u2, void* o1, void* o2,
                 const void* clk, unsigned long long* slot);
+  // re-meshing: `count` new leaves listed as int4 (new slot, kind 1 copy / 2 prolong / 3
+  // restrict, old slot, child octant), from the old tree (L: its table, params, stream) and
+  // state into both buffers of the new state
+  void (*regrid)(const AmrLaunch& L, const int* map, int count, const void* u1, const void* u2, void* n1a, void* n2a,
+                 void* n1b, void* n2b);
+  // the refinement indicator of the listed leaves into out[slot] (precision of the state)
+  void (*indicator)(const AmrLaunch&, const int* list, int count, const void* u1, void* out);
 };
 
 AmrOps amr_ops_f32(int lim, int rs, int isen);

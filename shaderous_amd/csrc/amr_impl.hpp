@@ -569,6 +569,79 @@ __global__ __launch_bounds__(256) void k_amr_diag(Params<T> P, const int* __rest
 }
 
 // ---------------- launchers ----------------
+// ---------------- re-meshing ----------------
+// Every leaf of a new tree from the old tree's state (amr/regrid.wgsl:40-139), per listed new
+// leaf (new slot, kind, old source slot, child octant): 1 copy of an old leaf; 2 prolongation of
+// an old leaf into its child (the ghost fill's conservative limited stencil over the old state,
+// then the total energy as each child's kinetic energy plus its interpolated internal energy
+// plus the offset that keeps the children's mean the coarse cell's, regrid.wgsl:86-97); 3
+// restriction of an old parent (the mean of its leaf children, u_at). Written into both state
+// buffers, as a state upload is. told: the old block table.
+template <typename T>
+__device__ __forceinline__ T ke_of(const Params<T>& P, const Cons<T>& u) {
+  return tdiv(T(0.5) * ((u.a.y * u.a.y + u.a.z * u.a.z) + u.a.w * u.a.w), tmax(u.a.x, P.rho_floor));
+}
+
+template <typename T, int LIM>
+__global__ __launch_bounds__(256) void k_amr_regrid(Params<T> P, const int* __restrict__ told,
+                                                    const int4* __restrict__ map, const T4<T>* __restrict__ U1,
+                                                    const T4<T>* __restrict__ U2, T4<T>* __restrict__ N1a,
+                                                    T4<T>* __restrict__ N2a, T4<T>* __restrict__ N1b,
+                                                    T4<T>* __restrict__ N2b) {
+  const int4 m = map[blockIdx.x / (AMR_CELLS / 256)];
+  const int c = (blockIdx.x % (AMR_CELLS / 256)) * 256 + threadIdx.x;
+  const int x = c % AMR_BS, y = (c / AMR_BS) % AMR_BS, z = c / (AMR_BS * AMR_BS);
+  Cons<T> u;
+  if (m.y == 1) {
+    u = u_leaf<T>(U1, U2, m.z, x, y, z);
+  } else if (m.y == 3) {
+    u = u_at<T>(told, U1, U2, m.z, x, y, z);
+  } else {
+    const int lc[3] = {(x >> 1) + (AMR_BS / 2) * (m.w & 1), (y >> 1) + (AMR_BS / 2) * ((m.w >> 1) & 1),
+                       (z >> 1) + (AMR_BS / 2) * ((m.w >> 2) & 1)};
+    Cons<T> v[7];
+    bool ok[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      int at[3] = {lc[0], lc[1], lc[2]};
+      if (k > 0) at[(k - 1) / 2] += (k & 1) == 0 ? 1 : -1;
+      ok[k] = u_near<T>(told, U1, U2, m.z, at[0], at[1], at[2], v[k]);
+    }
+    u = prolong<T, LIM>(P, v, ok, x & 1, y & 1, z & 1);
+    T ke = T(0);
+    for (int k = 0; k < 2; k++)
+      for (int j = 0; j < 2; j++)
+        for (int i = 0; i < 2; i++) ke = ke + ke_of(P, prolong<T, LIM>(P, v, ok, i, j, k));
+    ke = ke * T(0.125);
+    u.b.x = ke_of(P, u) + u.b.y + ((v[0].b.x - v[0].b.y) - ke);
+  }
+  const size_t o = (size_t)m.x * AMR_CELLS + c;
+  N1a[o] = u.a; N2a[o] = u.b;
+  N1b[o] = u.a; N2b[o] = u.b;
+}
+
+// The refinement indicator of the listed leaves: the largest relative density jump
+// |rho_a - rho_b| / min(rho_a, rho_b) between face-adjacent cells of the block's interior.
+template <typename T>
+__global__ __launch_bounds__(256) void k_amr_indicator(const int* __restrict__ list, const T4<T>* __restrict__ U1,
+                                                       T* __restrict__ out) {
+  __shared__ T red[4];
+  const int s = list[blockIdx.x];
+  T r = T(0);
+  for (int c = threadIdx.x; c < AMR_CELLS; c += 256) {
+    const int x = c % AMR_BS, y = (c / AMR_BS) % AMR_BS, z = c / (AMR_BS * AMR_BS);
+    const size_t o = (size_t)s * AMR_CELLS + c;
+    const T a = U1[o].x;
+    if (x + 1 < AMR_BS) { const T b = U1[o + 1].x; r = tmax(r, tdiv(tabs(a - b), tmin(a, b))); }
+    if (y + 1 < AMR_BS) { const T b = U1[o + AMR_BS].x; r = tmax(r, tdiv(tabs(a - b), tmin(a, b))); }
+    if (z + 1 < AMR_BS) { const T b = U1[o + AMR_BS * AMR_BS].x; r = tmax(r, tdiv(tabs(a - b), tmin(a, b))); }
+  }
+  r = wave_max(r);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) out[s] = tmax(tmax(red[0], red[1]), tmax(red[2], red[3]));
+}
+
 template <typename T, int LIM, int RS, bool ISEN>
 struct AmrImpl {
   static const Params<T>& P(const AmrLaunch& L) { return *static_cast<const Params<T>*>(L.params); }
@@ -627,8 +700,21 @@ struct AmrImpl {
                     const void* u2, void* sh, const void* clk);
   static void fused(const AmrLaunch& L, const AmrQueue& Q, const void* sh, const void* u1, const void* u2, void* o1,
                     void* o2, const void* clk, unsigned long long* slot);
+  static void regrid(const AmrLaunch& Lold, const int* map, int count, const void* u1, const void* u2, void* n1a,
+                     void* n2a, void* n1b, void* n2b) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL((k_amr_regrid<T, LIM>), dim3(count * (AMR_CELLS / 256)), dim3(256), 0, Lold.stream, P(Lold),
+                       Lold.tab, (const int4*)map, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)n1a, (T4<T>*)n2a,
+                       (T4<T>*)n1b, (T4<T>*)n2b);
+  }
+  static void indicator(const AmrLaunch& L, const int* list, int count, const void* u1, void* out) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL((k_amr_indicator<T>), dim3(count), dim3(256), 0, L.stream, list, (const T4<T>*)u1, (T*)out);
+  }
   static AmrOps ops() {
     AmrOps o;
+    o.regrid = regrid;
+    o.indicator = indicator;
     o.shell = shell;
     o.fused = fused;
     o.assemble = assemble;

@@ -602,6 +602,100 @@ int shx_amr_diagnostics(shx_amr* s, double out[16]) {
   return SHX_OK;
 }
 
+int shx_amr_indicator(shx_amr* s, double* out) {
+  if (!s || !out) return fail(SHX_EINVAL, "bad argument");
+  void* d = nullptr;
+  HIPCHK(hipMalloc(&d, (size_t)s->nblocks * s->es));
+  const AmrLaunch L = launch_of(s);
+  s->ops.indicator(L, lp(s, s->leaf), s->leaf.count, s->U1[s->cur], d);
+  int rc = hipGetLastError() == hipSuccess ? SHX_OK : SHX_EDEVICE;
+  std::vector<unsigned char> h((size_t)s->nblocks * s->es);
+  if (rc == SHX_OK && (hipMemcpyAsync(h.data(), d, h.size(), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+                       hipStreamSynchronize(s->stream) != hipSuccess))
+    rc = SHX_EDEVICE;
+  (void)hipFree(d);
+  if (rc != SHX_OK) return fail(rc, "the indicator pass failed");
+  for (int i = 0; i < s->nblocks; i++) {
+    const bool leaf = (s->tab[(size_t)i * AMR_BT + BT_FLAGS] & 2) != 0;
+    const double v = s->es == 8 ? ((const double*)h.data())[i] : (double)((const float*)h.data())[i];
+    out[i] = leaf ? v : NAN;
+  }
+  return SHX_OK;
+}
+
+int shx_amr_regrid(shx_amr* s, const int* blocks, int nblocks) {
+  if (!s || !blocks || nblocks < 1) return fail(SHX_EINVAL, "bad argument");
+  // what each block of the new tree takes from the current one (the regrid table of
+  // src/amr/tree.ts:681-720): checked before anything is allocated
+  std::unordered_map<uint64_t, int> cur;
+  for (int i = 0; i < s->nblocks; i++) {
+    const int* b = &s->blocks[4 * i];
+    cur[key_of(b[0], b[1], b[2], b[3])] = i;
+  }
+  auto is_leaf = [&](int i) { return (s->tab[(size_t)i * AMR_BT + BT_FLAGS] & 2) != 0; };
+  auto find = [&](int L, int x, int y, int z) {
+    if (L < 0 || x < 0 || y < 0 || z < 0) return -1;
+    auto it = cur.find(key_of(L, x, y, z));
+    return it == cur.end() ? -1 : it->second;
+  };
+  std::unordered_map<uint64_t, int> nkeys;
+  for (int j = 0; j < nblocks; j++) nkeys[key_of(blocks[4 * j], blocks[4 * j + 1], blocks[4 * j + 2], blocks[4 * j + 3])] = j;
+  std::vector<int> map;   // (new slot, kind, old slot, octant) per new leaf
+  for (int j = 0; j < nblocks; j++) {
+    const int L = blocks[4 * j], x = blocks[4 * j + 1], y = blocks[4 * j + 2], z = blocks[4 * j + 3];
+    bool new_leaf = true;
+    for (int o = 0; o < 8 && new_leaf; o++)
+      if (nkeys.count(key_of(L + 1, 2 * x + (o & 1), 2 * y + ((o >> 1) & 1), 2 * z + ((o >> 2) & 1)))) new_leaf = false;
+    if (!new_leaf) continue;
+    const int i = find(L, x, y, z);
+    if (i >= 0 && is_leaf(i)) { map.insert(map.end(), {j, 1, i, 0}); continue; }
+    if (i >= 0) {
+      bool kids = true;
+      for (int o = 0; o < 8 && kids; o++) {
+        const int k = find(L + 1, 2 * x + (o & 1), 2 * y + ((o >> 1) & 1), 2 * z + ((o >> 2) & 1));
+        kids = k >= 0 && is_leaf(k);
+      }
+      if (kids) { map.insert(map.end(), {j, 3, i, 0}); continue; }
+    }
+    const int p = L > 0 ? find(L - 1, x >> 1, y >> 1, z >> 1) : -1;
+    if (p >= 0 && is_leaf(p)) { map.insert(map.end(), {j, 2, p, (x & 1) | ((y & 1) << 1) | ((z & 1) << 2)}); continue; }
+    return fail(SHX_EINVAL, "new leaf (%d, %d, %d, %d) is more than one level from the current tree", L, x, y, z);
+  }
+  // the new tree's sim (validates the tree and allocates its memory), then the state moved over
+  shx_amr* ns = nullptr;
+  int rc = shx_amr_create(&s->cfg, &s->lay, blocks, nblocks, &ns);
+  if (rc != SHX_OK) return rc;
+  int* d_map = nullptr;
+  const int count = (int)(map.size() / 4);
+  bool ok = hipStreamSynchronize(s->stream) == hipSuccess &&
+            hipMalloc((void**)&d_map, (map.empty() ? 4 : map.size()) * sizeof(int)) == hipSuccess &&
+            hipMemcpy(d_map, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+  if (ok) {
+    const AmrLaunch Lold = launch_of(s);
+    s->ops.regrid(Lold, d_map, count, s->U1[s->cur], s->U2[s->cur], ns->U1[0], ns->U2[0], ns->U1[1], ns->U2[1]);
+    ok = hipGetLastError() == hipSuccess &&
+         hipMemcpyAsync(ns->clk, s->clk, 16 * s->es, hipMemcpyDeviceToDevice, s->stream) == hipSuccess &&
+         hipStreamSynchronize(s->stream) == hipSuccess;
+  }
+  (void)hipFree(d_map);
+  if (!ok) {
+    shx_amr_destroy(ns);
+    return fail(SHX_EDEVICE, "moving the state to the new tree failed");
+  }
+  ns->cur = 0;
+  if (ns->lay.path == SHX_AMR_PATH_FUSED) {
+    // the fused kernel takes each step's rate from the previous one: seed it for the new state
+    ns->ops.rates(launch_of(ns), lp(ns, ns->leaf), ns->leaf.count, ns->U1[0], ns->U2[0], ns->slot);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ns->stream) != hipSuccess) {
+      shx_amr_destroy(ns);
+      return fail(SHX_EDEVICE, "seeding the new tree's rate failed");
+    }
+  }
+  std::swap(*s, *ns);
+  shx_amr_destroy(ns);
+  return SHX_OK;
+}
+
 int shx_amr_profile(shx_amr* s, int steps, double ms[5]) {
   if (!s || !ms || steps < 1) return fail(SHX_EINVAL, "bad argument");
   hipEvent_t ev[5];

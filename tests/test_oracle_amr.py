@@ -271,3 +271,103 @@ def test_amr_ghosts_of_linear_field():
                 assert np.allclose(W[s][..., 1][tuple(idx)], 0.1, atol=1e-13)
                 checked[kind] += 1
     assert checked["coarse"] > 0 and checked["fine"] > 0, checked
+
+
+# ---------------- re-meshing ----------------
+
+def remeshed(blocks):
+    """The refined tree re-meshed once: the level-1 parent at the corner merged back into a
+    leaf, two level-0 leaves split (within one level of the old tree everywhere)."""
+    ind = np.where(amr_tree.leaf_mask(blocks), 0.0, np.nan)
+    for i, b in enumerate(blocks.tolist()):
+        if b in ([0, 1, 1, 1], [0, 0, 1, 1]):
+            ind[i] = 1.0
+        if b[0] == 2:
+            ind[i] = 0.0
+    return amr_tree.adapt_tree(blocks, (2, 2, 2), ind, 0.5, 0.1, 2)
+
+
+def test_regrid_map_kinds():
+    blocks = refined_tree()
+    k, s, o = amr_tree.regrid_map(blocks, blocks)
+    leaf = amr_tree.leaf_mask(blocks)
+    assert np.all(k[leaf] == 1) and np.all(k[~leaf] == 0) and np.array_equal(s[leaf], np.flatnonzero(leaf))
+    new = remeshed(blocks)
+    k, s, o = amr_tree.regrid_map(blocks, new)
+    assert set(np.unique(k)) == {0, 1, 2, 3}
+    for j in np.flatnonzero(k == 2):                     # a prolonged child: its parent is an old leaf
+        L, x, y, z = new[j]
+        assert tuple(blocks[s[j]]) == (L - 1, x >> 1, y >> 1, z >> 1) and o[j] == (x & 1) | (y & 1) << 1 | (z & 1) << 2
+    for j in np.flatnonzero(k == 3):                     # a merged block was an old parent
+        assert tuple(blocks[s[j]]) == tuple(new[j]) and not leaf[s[j]]
+    # two levels at once is refused
+    with pytest.raises(ValueError):
+        amr_tree.regrid_map(root_blocks((2, 2, 2)), blocks)
+
+
+def test_adapt_tree_stays_within_a_level():
+    blocks, _ = small_blast_tree()
+    rng = np.random.default_rng(3)
+    leaf = amr_tree.leaf_mask(blocks)
+    for _ in range(3):
+        ind = np.where(leaf, rng.uniform(0, 1, len(blocks)), np.nan)
+        new = amr_tree.adapt_tree(blocks, (4, 4, 4), ind, 0.8, 0.3, 3)
+        AmrTable(new, (4, 4, 4))                          # 2:1 balanced (raises otherwise)
+        amr_tree.regrid_map(blocks, new)                  # within one level (raises otherwise)
+
+
+@pytest.mark.parametrize("impl,tol", [(wgsl64, 1e-13), (wgsl32, 2e-6)])
+def test_regrid_conserves_and_round_trips(impl, tol):
+    """Prolongation and restriction keep the totals to round-off; prolonging every leaf and
+    restricting back gives the state again (the children's mean is the coarse cell, energy
+    included)."""
+    blocks = refined_tree()
+    t = AmrTable(blocks, (2, 2, 2))
+    u1, u2 = amr_state(blocks, impl.dtype)
+    prm = params((32, 32, 32), ndim=3, gamma=1.4, cfl=0.4, limiter="mc")
+    new = remeshed(blocks)
+    k, s, o = amr_tree.regrid_map(blocks, new)
+    n1, n2 = impl.amr_regrid(prm, t, u1, u2, k, s, o)
+    c0, c1 = totals(blocks, u1, u2), totals(new, n1, n2)
+    scale = np.abs(c0) + np.array([0, 1, 1, 1, 0]) * c0[0]
+    assert np.all(np.abs(c1 - c0) <= tol * scale), (c1 - c0) / scale
+    # split every leaf below level 2, then merge the new parents back
+    leaf = amr_tree.leaf_mask(blocks)
+    up = amr_tree.adapt_tree(blocks, (2, 2, 2), np.where(leaf, 1.0, np.nan), 0.5, -1.0, 2)
+    k, s, o = amr_tree.regrid_map(blocks, up)
+    assert (k == 2).sum() > 0
+    f1, f2 = impl.amr_regrid(prm, t, u1, u2, k, s, o)
+    tu = AmrTable(up, (2, 2, 2))
+    kb, sb, ob = amr_tree.regrid_map(up, blocks)
+    b1, b2 = impl.amr_regrid(prm, tu, f1, f2, kb, sb, ob)
+    sel = leaf.repeat(BS ** 3)
+    for a, b in ((b1, u1), (b2, u2)):
+        err = np.abs(a[sel].astype(np.float64) - b[sel]).max(axis=0) / np.abs(b[sel]).max(axis=0).clip(1e-30)
+        assert err.max() <= 4 * tol, err
+
+
+def test_regrid_prolongs_linear_fields_exactly():
+    """A linear density and momentum field on a uniform root grid: every child of a split
+    block takes the linear field's value at its own centre (slopes within the limiter's
+    range), and restricting back returns it."""
+    root = (2, 2, 2)
+    blocks = root_blocks(root)
+    lin = lambda x, y, z: {"rho": 2.0 + 0.5 * x + 0.25 * y - 0.125 * z, "vx": 0.0 * x, "vy": 0.0 * x, "vz": 0.0 * x,
+                           "p": 1.0 + 0.0 * x}
+    u1, u2 = amr_tree.blocks_state(blocks, 1.0 / 32, lin, 1.4, np.float64)
+    t = AmrTable(blocks, root)
+    prm = params((32, 32, 32), ndim=3, gamma=1.4, cfl=0.4, limiter="mc")
+    up = amr_tree.adapt_tree(blocks, root, np.ones(len(blocks)), 0.5, -1.0, 1)
+    k, s, o = amr_tree.regrid_map(blocks, up)
+    n1, _ = wgsl64.amr_regrid(prm, t, u1, u2, k, s, o)
+    e1, _ = amr_tree.blocks_state(up, 1.0 / 32, lin, 1.4, np.float64)
+    # away from the periodic wrap (where the linear field jumps) the children are exact
+    inner = np.zeros(len(up) * BS ** 3, bool)
+    for j, (L, x, y, z) in enumerate(up.tolist()):
+        if L == 1:
+            kk, jj, ii = np.meshgrid(np.arange(BS), np.arange(BS), np.arange(BS), indexing="ij")
+            g = [BS * c + a for c, a in ((x, ii), (y, jj), (z, kk))]
+            ok = np.all([(gi >= 2) & (gi < 64 - 2) for gi in g], axis=0)
+            inner[j * BS ** 3:(j + 1) * BS ** 3] = ok.ravel()
+    assert inner.sum() > 0
+    assert np.abs(n1[inner, 0] - e1[inner, 0]).max() < 1e-13
