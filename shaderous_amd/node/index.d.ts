@@ -75,6 +75,8 @@ export declare class AmrSim {
   readonly ncell: number;
   table(): Int32Array;
   stats(): { blocks: number; leaves: number; matchLeaves: number; levels: number; launchesPerStep: number };
+  indicator(): Float64Array;
+  regrid(blocks: Int32Array): void;
   setState(u1: State, u2: State, time?: number, steps?: number): void;
   readState(): { u1: State; u2: State };
   readStateAsync(): Promise<{ u1: State; u2: State }>;

@@ -206,6 +206,18 @@ class AmrSim {
     return { blocks: v[0], leaves: v[1], matchLeaves: v[2], levels: v[3], launchesPerStep: v[4] };
   }
 
+  /** Per block the largest relative density jump between face-adjacent cells (NaN for parents). */
+  indicator() {
+    return native.amrIndicator(live(this.h));
+  }
+
+  /** Re-mesh to `blocks` (Int32Array rows), within one level of the current tree everywhere. */
+  regrid(blocks) {
+    native.amrRegrid(live(this.h), blocks);
+    this.nblocks = blocks.length / 4;
+    this.ncell = this.nblocks * 4096;
+  }
+
   setState(u1, u2, time = 0, steps = 0) {
     native.amrSetState(live(this.h), u1, u2, time, steps);
   }

@@ -618,6 +618,43 @@ static napi_value js_amr_stats(napi_env env, napi_callback_info info) {
   return f64_array(env, v, 5);
 }
 
+static napi_value js_amr_indicator(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  Handle* h = get_handle(env, argv[0], 1);
+  if (!h) return NULL;
+  double* v = (double*)malloc((size_t)h->nblocks * sizeof(double));
+  if (!v) { napi_throw_error(env, NULL, "out of memory"); return NULL; }
+  int rc = shx_amr_indicator((shx_amr*)h->p, v);
+  if (rc != SHX_OK) { free(v); return throw_status(env, rc); }
+  napi_value out = f64_array(env, v, h->nblocks);
+  free(v);
+  return out;
+}
+
+static napi_value js_amr_regrid(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Handle* h = get_handle(env, argv[0], 1);
+  if (!h) return NULL;
+  bool is = false;
+  napi_is_typedarray(env, argv[1], &is);
+  napi_typedarray_type t;
+  size_t len = 0;
+  void* data = NULL;
+  napi_value ab;
+  size_t off;
+  if (is) NAPI_CALL(env, napi_get_typedarray_info(env, argv[1], &t, &len, &data, &ab, &off));
+  if (!is || t != napi_int32_array || len % 4 != 0) {
+    napi_throw_type_error(env, NULL, "blocks is an Int32Array of (level, ix, iy, iz) rows");
+    return NULL;
+  }
+  SHX_CALL(env, shx_amr_regrid((shx_amr*)h->p, (const int*)data, (int)(len / 4)));
+  h->nblocks = (int)(len / 4);
+  h->ncell = (size_t)h->nblocks * 4096;
+  return undefined(env);
+}
+
 /* ---------------- module ---------------- */
 #define EXPORT(name, fn) {name, NULL, fn, NULL, NULL, NULL, (napi_property_attributes)(napi_enumerable | napi_writable), NULL}
 
@@ -647,6 +684,8 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT("amrDiagnostics", js_amr_diagnostics),
       EXPORT("amrProfile", js_amr_profile),
       EXPORT("amrStats", js_amr_stats),
+      EXPORT("amrIndicator", js_amr_indicator),
+      EXPORT("amrRegrid", js_amr_regrid),
   };
   if (napi_define_properties(env, exports, sizeof props / sizeof props[0], props) != napi_ok) return NULL;
   return exports;

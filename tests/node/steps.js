@@ -1,6 +1,7 @@
 'use strict';
 // Drives the hydro path through the Node binding, as a TypeScript caller of the reference's API
-// would: node steps.js <dir>. Reads <dir>/job.json ({kind: 'uniform'|'amr', cfg, steps, tree?})
+// would: node steps.js <dir>. Reads <dir>/job.json ({kind: 'uniform'|'amr', cfg, steps, tree?,
+// regrid?, after?})
 // and the initial state <dir>/u1.bin, <dir>/u2.bin (little-endian floats of the run's
 // precision), steps, and writes <dir>/o1.bin, <dir>/o2.bin, <dir>/out.json (clock, diagnostics).
 const fs = require('fs');
@@ -27,6 +28,11 @@ async function main() {
     sim.setState(load('u1.bin'), load('u2.bin'));
     const d0 = sim.diagnostics();
     sim.step(job.steps);
+    if (job.regrid) {
+      // re-mesh, then step on the new tree
+      sim.regrid(Int32Array.from(job.regrid));
+      sim.step(job.after || 0);
+    }
     const st = await sim.readStateAsync();
     const d1 = await (sim.readDiagnostics ? sim.readDiagnostics() : Promise.resolve(sim.diagnostics()));
     fs.writeFileSync(path.join(dir, 'o1.bin'), Buffer.from(st.u1.buffer));

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reaso
 EXPORTS = ["version", "create", "destroy", "setState", "readState", "readStateAsync", "step", "sync", "runToEnd",
            "readClock", "diagnostics", "diagnosticsAsync", "fluxMismatch", "profile", "amrCreate", "amrTable",
            "amrSetState", "amrReadState", "amrStep", "amrSync", "amrReadClock", "amrDiagnostics", "amrProfile",
-           "amrStats"]
+           "amrStats", "amrIndicator", "amrRegrid"]
 
 
 def node(code):
@@ -118,4 +118,35 @@ def test_node_amr_matches_python(tmp_path):
     finally:
         sim.destroy()
     leaf = amr_tree.leaf_mask(blocks).repeat(4096)
+    assert np.array_equal(o1[leaf], p1[leaf]) and np.array_equal(o2[leaf], p2[leaf])
+
+
+@pytest.mark.gpu
+def test_node_amr_regrid_matches_python(tmp_path):
+    """A re-mesh through the Node facade (amr.regrid) then two steps on the new tree give the
+    ctypes caller's bits."""
+    from shaderous_amd.amr import AmrSim
+    from shaderous_amd.sim import SimConfig
+    blocks = amr_tree.build_tree((2, 2, 2), 1, lambda L, c: c == (0, 0, 0))
+    new = amr_tree.build_tree((2, 2, 2), 1, lambda L, c: c in ((0, 0, 0), (1, 0, 0)))
+
+    def ic(x, y, z):
+        return {"rho": 1.0 + 0.5 * np.exp(-(x * x + y * y + z * z) / 0.02), "vx": 0.2, "p": 1.0}
+    u1, u2 = amr_tree.blocks_state(blocks, 1.0 / 32, ic, 1.4)
+    job = {"kind": "amr", "steps": 2, "after": 2, "regrid": new.reshape(-1).tolist(),
+           "cfg": {"n": 32, "ndim": 3, "limiter": "mc", "isentropicRecon": False, "cfl": 0.4, "box": 1.0},
+           "tree": {"root": [2, 2, 2], "blocks": blocks.reshape(-1).tolist(), "path": "fused"}}
+    o1, o2, out = _run(tmp_path, job, u1, u2)
+    cfg = SimConfig(n=(32, 32, 32), dx=1.0 / 32, limiter="mc", isentropic=False, cfl=0.4)
+    sim = AmrSim(cfg, blocks, (2, 2, 2))
+    try:
+        sim.set_state(u1, u2)
+        sim.step(2)
+        sim.regrid(new)
+        sim.step(2)
+        p1, p2 = sim.read_state()
+    finally:
+        sim.destroy()
+    leaf = amr_tree.leaf_mask(new).repeat(4096)
+    assert o1.shape == p1.shape
     assert np.array_equal(o1[leaf], p1[leaf]) and np.array_equal(o2[leaf], p2[leaf])
