@@ -45,7 +45,7 @@ TRAFFIC_MEASURED = {
 # VALU instructions (wave64) per launch of the dominant kernel, measured with rocprofv3 --pmc
 # SQ_INSTS_VALU for the configuration they were measured on (profiles/k_fused_turbulence_256.md)
 VALU_MEASURED = {
-    ("k_fused_r", 256, (32, 8, 64)): 376973268,
+    ("k_fused_r", 256, (32, 8, 64)): 347257461,
     ("k_fused_r", 256, (32, 8, 128)): 373012398,
     ("k_fused_r", 256, (32, 16, 128)): 329879703,
 }

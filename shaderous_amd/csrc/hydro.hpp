@@ -337,18 +337,25 @@ __device__ __forceinline__ Flux<T> riemann(const Side<T>& l, const Side<T>& r, T
   } else {
     T ql = l.rho * (sl - l.vn);
     T qr = r.rho * (sr - r.vn);
-    T inv = trcp(sr - sl);
-    T slsr = sl * sr;
-    T hm = (sr * ml0 - sl * mr0 + slsr * (r.rho - l.rho)) * inv;
-    T hp0 = (sr * fpl0 - sl * fpr0 + slsr * (mr0 - ml0)) * inv;
-    T hp1 = (sr * fpl1 - sl * fpr1 + slsr * (mr1 - ml1)) * inv;
-    T hp2 = (sr * fpl2 - sl * fpr2 + slsr * (mr2 - ml2)) * inv;
-    T he = (sr * fel - sl * fer + slsr * (er - el)) * inv;
+    const bool star = RS != RS_HLL && ql < T(0) && qr > T(0);
+    // The HLL state between the waves: the flux itself where the star densities are not
+    // positive (and for HLL), the blend's other end where the face is flagged; a subsonic
+    // unflagged face (w = 0: the blend is the star flux exactly) skips it.
+    T hm, hp0, hp1, hp2, he;
+    if (!star || w_hll != T(0)) {
+      T inv = trcp(sr - sl);
+      T slsr = sl * sr;
+      hm = (sr * ml0 - sl * mr0 + slsr * (r.rho - l.rho)) * inv;
+      hp0 = (sr * fpl0 - sl * fpr0 + slsr * (mr0 - ml0)) * inv;
+      hp1 = (sr * fpl1 - sl * fpr1 + slsr * (mr1 - ml1)) * inv;
+      hp2 = (sr * fpl2 - sl * fpr2 + slsr * (mr2 - ml2)) * inv;
+      he = (sr * fel - sl * fer + slsr * (er - el)) * inv;
+    }
     if constexpr (RS == RS_HLL) {
       o.m = hm; o.mx = hp0; o.my = hp1; o.mz = hp2; o.e = he;
       o.vn = tdiv(r.p - l.p + l.vn * ql - r.vn * qr, ql - qr);
     } else {
-      if (!(ql < T(0) && qr > T(0))) {
+      if (!star) {
         o.m = hm; o.mx = hp0; o.my = hp1; o.mz = hp2; o.e = he;
         o.vn = T(0.5) * (l.vn + r.vn);
       } else {
@@ -376,11 +383,15 @@ __device__ __forceinline__ Flux<T> riemann(const Side<T>& l, const Side<T>& r, T
         T c1 = f1 + S * (dr * u1);
         T c2 = f2 + S * (dr * u2);
         T ce = fe + S * (E * k + f * (sm - un) * (sm + tdiv(p, q)));
-        o.m = tmix(cm, hm, w_hll);
-        o.mx = tmix(c0, hp0, w_hll);
-        o.my = tmix(c1, hp1, w_hll);
-        o.mz = tmix(c2, hp2, w_hll);
-        o.e = tmix(ce, he, w_hll);
+        if (w_hll != T(0)) {
+          o.m = tmix(cm, hm, w_hll);
+          o.mx = tmix(c0, hp0, w_hll);
+          o.my = tmix(c1, hp1, w_hll);
+          o.mz = tmix(c2, hp2, w_hll);
+          o.e = tmix(ce, he, w_hll);
+        } else {
+          o.m = cm; o.mx = c0; o.my = c1; o.mz = c2; o.e = ce;
+        }
         o.vn = sm;
       }
     }
