@@ -46,8 +46,9 @@ TRAFFIC_MEASURED = {
 # SQ_INSTS_VALU for the configuration they were measured on (profiles/k_fused_turbulence_256.md)
 VALU_MEASURED = {
     ("k_fused_r", 256, (32, 8, 64)): 347257461,
-    ("k_fused_r", 256, (32, 8, 128)): 373012398,
-    ("k_fused_r", 256, (32, 16, 128)): 329879703,
+    ("k_fused_r", 256, (32, 8, 128)): 343363812,
+    ("k_fused_r", 256, (32, 16, 128)): 302267580,
+    ("k_fused_r", 256, (32, 16, 64)): 305729005,
 }
 # MI355X VALU issue peak for FP32 without packed math: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_LANE_PEAK = 256 * 4 * 16 * 2.4e9
