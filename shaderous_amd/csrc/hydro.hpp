@@ -317,23 +317,34 @@ __device__ __forceinline__ Side<T> make_side(const Params<T>& P, T rho, T vn, T 
 }
 
 // HLLC blended toward HLL by w (riemann/hllc.wgsl:4-85), or HLL (riemann/hll.wgsl:87-121).
+// A side's total energy, momentum and normal flux (hllc.wgsl:7-28), each from the side's own
+// primitives in one fixed expression
+template <typename T>
+struct SideFlux { T E, m0, m1, m2, f0, f1, f2, fe; };
+template <typename T>
+__device__ __forceinline__ SideFlux<T> side_flux(T rho, T vn, T vt1, T vt2, T p, T eint) {
+  SideFlux<T> o;
+  o.E = eint + T(0.5) * rho * (vn * vn + vt1 * vt1 + vt2 * vt2);
+  o.m0 = rho * vn; o.m1 = rho * vt1; o.m2 = rho * vt2;
+  o.f0 = o.m0 * vn + p; o.f1 = o.m1 * vn; o.f2 = o.m2 * vn;
+  o.fe = (o.E + p) * vn;
+  return o;
+}
+
+// HLLC blended toward HLL by w (riemann/hllc.wgsl:4-85), or HLL (riemann/hll.wgsl:87-121).
+// Each branch forms only the side quantities it uses: the upwind side's on a supersonic face,
+// the star side's on a subsonic one, both where the HLL state is needed.
 template <typename T, int RS>
 __device__ __forceinline__ Flux<T> riemann(const Side<T>& l, const Side<T>& r, T w_hll) {
   T sl = tmin(l.vn - l.cs, r.vn - r.cs);
   T sr = tmax(l.vn + l.cs, r.vn + r.cs);
-  T el = l.eint + T(0.5) * l.rho * (l.vn * l.vn + l.vt1 * l.vt1 + l.vt2 * l.vt2);
-  T er = r.eint + T(0.5) * r.rho * (r.vn * r.vn + r.vt1 * r.vt1 + r.vt2 * r.vt2);
-  T ml0 = l.rho * l.vn, ml1 = l.rho * l.vt1, ml2 = l.rho * l.vt2;
-  T mr0 = r.rho * r.vn, mr1 = r.rho * r.vt1, mr2 = r.rho * r.vt2;
-  T fpl0 = ml0 * l.vn + l.p, fpl1 = ml1 * l.vn, fpl2 = ml2 * l.vn;
-  T fpr0 = mr0 * r.vn + r.p, fpr1 = mr1 * r.vn, fpr2 = mr2 * r.vn;
-  T fel = (el + l.p) * l.vn;
-  T fer = (er + r.p) * r.vn;
   Flux<T> o;
-  if (sl >= T(0)) {
-    o.m = ml0; o.mx = fpl0; o.my = fpl1; o.mz = fpl2; o.e = fel; o.vn = l.vn;
-  } else if (sr <= T(0)) {
-    o.m = mr0; o.mx = fpr0; o.my = fpr1; o.mz = fpr2; o.e = fer; o.vn = r.vn;
+  if (sl >= T(0) || sr <= T(0)) {
+    // supersonic: the upwind side's own flux
+    const bool up_l = sl >= T(0);
+    const SideFlux<T> u = side_flux(up_l ? l.rho : r.rho, up_l ? l.vn : r.vn, up_l ? l.vt1 : r.vt1,
+                                    up_l ? l.vt2 : r.vt2, up_l ? l.p : r.p, up_l ? l.eint : r.eint);
+    o.m = u.m0; o.mx = u.f0; o.my = u.f1; o.mz = u.f2; o.e = u.fe; o.vn = up_l ? l.vn : r.vn;
   } else {
     T ql = l.rho * (sl - l.vn);
     T qr = r.rho * (sr - r.vn);
@@ -343,13 +354,15 @@ __device__ __forceinline__ Flux<T> riemann(const Side<T>& l, const Side<T>& r, T
     // unflagged face (w = 0: the blend is the star flux exactly) skips it.
     T hm, hp0, hp1, hp2, he;
     if (!star || w_hll != T(0)) {
+      const SideFlux<T> a = side_flux(l.rho, l.vn, l.vt1, l.vt2, l.p, l.eint);
+      const SideFlux<T> b = side_flux(r.rho, r.vn, r.vt1, r.vt2, r.p, r.eint);
       T inv = trcp(sr - sl);
       T slsr = sl * sr;
-      hm = (sr * ml0 - sl * mr0 + slsr * (r.rho - l.rho)) * inv;
-      hp0 = (sr * fpl0 - sl * fpr0 + slsr * (mr0 - ml0)) * inv;
-      hp1 = (sr * fpl1 - sl * fpr1 + slsr * (mr1 - ml1)) * inv;
-      hp2 = (sr * fpl2 - sl * fpr2 + slsr * (mr2 - ml2)) * inv;
-      he = (sr * fel - sl * fer + slsr * (er - el)) * inv;
+      hm = (sr * a.m0 - sl * b.m0 + slsr * (r.rho - l.rho)) * inv;
+      hp0 = (sr * a.f0 - sl * b.f0 + slsr * (b.m0 - a.m0)) * inv;
+      hp1 = (sr * a.f1 - sl * b.f1 + slsr * (b.m1 - a.m1)) * inv;
+      hp2 = (sr * a.f2 - sl * b.f2 + slsr * (b.m2 - a.m2)) * inv;
+      he = (sr * a.fe - sl * b.fe + slsr * (b.E - a.E)) * inv;
     }
     if constexpr (RS == RS_HLL) {
       o.m = hm; o.mx = hp0; o.my = hp1; o.mz = hp2; o.e = he;
@@ -369,20 +382,15 @@ __device__ __forceinline__ Flux<T> riemann(const Side<T>& l, const Side<T>& r, T
         T u2 = left ? l.vt2 : r.vt2;
         T p = left ? l.p : r.p;
         T q = left ? ql : qr;
-        T E = left ? el : er;
-        T fm = left ? ml0 : mr0;
-        T f0 = left ? fpl0 : fpr0;
-        T f1 = left ? fpl1 : fpr1;
-        T f2 = left ? fpl2 : fpr2;
-        T fe = left ? fel : fer;
+        const SideFlux<T> c = side_flux(rho, un, u1, u2, p, left ? l.eint : r.eint);
         T k = tdiv(sm - un, S - sm);
         T dr = rho * k;
         T f = rho + dr;
-        T cm = fm + S * dr;
-        T c0 = f0 + S * (sm * dr + rho * (sm - un));
-        T c1 = f1 + S * (dr * u1);
-        T c2 = f2 + S * (dr * u2);
-        T ce = fe + S * (E * k + f * (sm - un) * (sm + tdiv(p, q)));
+        T cm = c.m0 + S * dr;
+        T c0 = c.f0 + S * (sm * dr + rho * (sm - un));
+        T c1 = c.f1 + S * (dr * u1);
+        T c2 = c.f2 + S * (dr * u2);
+        T ce = c.fe + S * (c.E * k + f * (sm - un) * (sm + tdiv(p, q)));
         if (w_hll != T(0)) {
           o.m = tmix(cm, hm, w_hll);
           o.mx = tmix(c0, hp0, w_hll);
