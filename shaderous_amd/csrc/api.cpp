@@ -45,6 +45,7 @@ struct shx_sim {
   int diag_blocks = 256;
   int cur = 0;
   bool fused = false;
+  bool zbc = false;          // fused path with a non-periodic z face: PA, PB hold planes 0 and nz - 1
   int tile = FUSED_32x16;
   int zc = 32;
   Params<float> pf;
@@ -69,7 +70,9 @@ void enqueue_step(shx_sim* s, int p, bool debug = false) {
   const UniformOps& o = s->ops;
   if (s->fused) {
     o.clock(L, s->clk, s->slot);
-    o.fused(L, s->tile, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, s->zc, s->slot, debug);
+    if (s->zbc) o.zplanes(L, s->U1[p], s->U2[p], s->PA, s->PB, s->clk);
+    o.fused(L, s->tile, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, s->zbc ? s->PA : nullptr,
+            s->zbc ? s->PB : nullptr, s->zc, s->slot, debug);
   } else {
     o.rates(L, s->U1[p], s->U2[p], s->slot);
     o.clock(L, s->clk, s->slot);
@@ -112,8 +115,13 @@ int reset_rates(shx_sim* s) {
 }
 
 bool fused_supported(const shx_config& c) {
-  // z boundaries periodic (the chunked march wraps planes); x/y: periodic, outflow, reflecting
-  if (c.bc_lo[2] != SHX_BC_PERIODIC || c.bc_hi[2] != SHX_BC_PERIODIC) return false;
+  // z: any face of a 3D grid (a non-periodic one through the boundary planes' half steps), a
+  // flat grid periodic; x/y: periodic, outflow, reflecting
+  if (c.n[2] < 2 && (c.bc_lo[2] != SHX_BC_PERIODIC || c.bc_hi[2] != SHX_BC_PERIODIC)) return false;
+  // an x or y axis periodic on one face only: its ghost columns' half steps would need the
+  // wrapped cells' own stencils (the plain path takes them)
+  for (int a = 0; a < 2; a++)
+    if ((c.bc_lo[a] == SHX_BC_PERIODIC) != (c.bc_hi[a] == SHX_BC_PERIODIC)) return false;
   for (int a = 0; a < 2; a++)
     if (c.bc_lo[a] == SHX_BC_OUTFLOW_ONLY || c.bc_hi[a] == SHX_BC_OUTFLOW_ONLY) return false;
   for (int a = 0; a < 2; a++)
@@ -232,7 +240,12 @@ int shx_create(const shx_config* cfg, shx_sim** out) {
   auto alloc = [&](void** p, size_t b) { return hipMalloc(p, b) == hipSuccess; };
   bool ok = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess;
   for (int p = 0; ok && p < 2; p++) ok = alloc(&s->U1[p], bytes) && alloc(&s->U2[p], bytes);
+  s->zbc = s->fused && (c.bc_lo[2] != SHX_BC_PERIODIC || c.bc_hi[2] != SHX_BC_PERIODIC);
   if (ok && !s->fused) ok = alloc(&s->PA, bytes) && alloc(&s->PB, bytes);
+  if (ok && s->zbc) {
+    const size_t zb = (size_t)2 * c.n[0] * c.n[1] * 4 * s->es;
+    ok = alloc(&s->PA, zb) && alloc(&s->PB, zb);
+  }
   ok = ok && alloc(&s->clk, 16 * 8) && alloc((void**)&s->slot, 64) && alloc((void**)&s->diag_part, s->diag_blocks * 16 * 8) &&
        alloc((void**)&s->diag_out, 16 * 8);
   if (!ok) {
@@ -396,8 +409,10 @@ int shx_profile(shx_sim* s, int steps, double ms[5]) {
       s->ops.clock(L, s->clk, s->slot);
       HIPCHK(hipEventRecord(ev[1], s->stream));
       HIPCHK(hipEventRecord(ev[2], s->stream));
+      if (s->zbc) s->ops.zplanes(L, s->U1[p], s->U2[p], s->PA, s->PB, s->clk);
       HIPCHK(hipEventRecord(ev[3], s->stream));
-      s->ops.fused(L, s->tile, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, s->zc, s->slot, false);
+      s->ops.fused(L, s->tile, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, s->zbc ? s->PA : nullptr,
+                   s->zbc ? s->PB : nullptr, s->zc, s->slot, false);
     } else {
       s->ops.rates(L, s->U1[p], s->U2[p], s->slot);
       HIPCHK(hipEventRecord(ev[1], s->stream));
@@ -448,6 +463,10 @@ int shx_autotune(shx_sim* s, int reps, int chosen[3]) {
   double best = 1e30;
   int bt = s->tile, bz = s->zc;
   Launch L = launch_of(s);
+  // the boundary planes' half steps of the state every candidate starts from
+  const void* za = s->zbc ? s->PA : nullptr;
+  const void* zb = s->zbc ? s->PB : nullptr;
+  if (s->zbc) s->ops.zplanes(L, s->U1[s->cur], s->U2[s->cur], s->PA, s->PB, s->clk);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -455,10 +474,10 @@ int shx_autotune(shx_sim* s, int reps, int chosen[3]) {
     for (int z : zcs) {
       if (z > s->cfg.n[2]) continue;
       int p = s->cur;
-      s->ops.fused(L, ti, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, z, s->slot, false);  // warm
+      s->ops.fused(L, ti, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, za, zb, z, s->slot, false);  // warm
       HIPCHK(hipEventRecord(e0, s->stream));
       for (int r = 0; r < reps; r++)
-        s->ops.fused(L, ti, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, z, s->slot, false);
+        s->ops.fused(L, ti, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, za, zb, z, s->slot, false);
       HIPCHK(hipEventRecord(e1, s->stream));
       HIPCHK(hipEventSynchronize(e1));
       float t;

@@ -32,8 +32,11 @@ struct UniformOps {
   void (*predict)(const Launch&, const void* u1, const void* u2, void* pa, void* pb, const void* clk);
   void (*update)(const Launch&, const void* u1, const void* u2, const void* pa, const void* pb, void* o1, void* o2,
                  const void* clk, bool debug);
-  void (*fused)(const Launch&, int tile, const void* u1, const void* u2, void* o1, void* o2, const void* clk, int zc,
-                unsigned long long* next_slot, bool debug);
+  // the fused step; za, zb: the half steps of planes 0 and nz - 1 (zplanes) when a z face is not
+  // periodic, else null
+  void (*fused)(const Launch&, int tile, const void* u1, const void* u2, void* o1, void* o2, const void* clk,
+                const void* za, const void* zb, int zc, unsigned long long* next_slot, bool debug);
+  void (*zplanes)(const Launch&, const void* u1, const void* u2, void* za, void* zb, const void* clk);
   void (*diag)(const Launch&, const void* u1, const void* u2, double* part, int nblocks, double* out);
 };
 

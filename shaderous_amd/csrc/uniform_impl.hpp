@@ -108,14 +108,11 @@ __device__ __forceinline__ Half<T> load_half(const Params<T>& P, const T4<T>* PA
   return h;
 }
 
+// The half step of cell (i, j, k) (predict.wgsl:32-119): PA = (rho, v), PB = (p, s, shock, first order)
 template <typename T, int LIM, bool ISEN>
-__global__ __launch_bounds__(256) void k_predict(Params<T> P, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2,
-                                                 T4<T>* __restrict__ PA, T4<T>* __restrict__ PB, const T* clk) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int k = blockIdx.z * blockDim.z + threadIdx.z;
-  if (i >= P.n[0] || j >= P.n[1] || k >= P.n[2]) return;
-  T dt = clk[0];
+__device__ __forceinline__ void predict_cell(const Params<T>& P, const T4<T>* __restrict__ U1,
+                                             const T4<T>* __restrict__ U2, int i, int j, int k, T dt, T4<T>& a,
+                                             T4<T>& b) {
   Prim<T> q = load_q(P, U1, U2, i, j, k);
   Prim<T> xm = load_q(P, U1, U2, i - 1, j, k), xp = load_q(P, U1, U2, i + 1, j, k);
   Prim<T> ym = load_q(P, U1, U2, i, j - 1, k), yp = load_q(P, U1, U2, i, j + 1, k);
@@ -128,12 +125,40 @@ __global__ __launch_bounds__(256) void k_predict(Params<T> P, const T4<T>* __res
   T conv = (xp.vx - xm.vx) + (yp.vy - ym.vy) + (zp.vz - zm.vz);
   T jump = tmax(jump_of(xm.p, xp.p), tmax(jump_of(ym.p, yp.p), jump_of(zm.p, zp.p)));
   Half<T> h = hancock(P, q, sl, conv, jump, dt);
-  size_t o = cell_index(P, i, j, k);
-  T4<T> a, b;
   a.x = h.rho; a.y = h.vx; a.z = h.vy; a.w = h.vz;
   b.x = h.p; b.y = h.s; b.z = h.shock; b.w = h.fo;
+}
+
+template <typename T, int LIM, bool ISEN>
+__global__ __launch_bounds__(256) void k_predict(Params<T> P, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2,
+                                                 T4<T>* __restrict__ PA, T4<T>* __restrict__ PB, const T* clk) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z * blockDim.z + threadIdx.z;
+  if (i >= P.n[0] || j >= P.n[1] || k >= P.n[2]) return;
+  T4<T> a, b;
+  predict_cell<T, LIM, ISEN>(P, U1, U2, i, j, k, clk[0], a, b);
+  size_t o = cell_index(P, i, j, k);
   PA[o] = a;
   PB[o] = b;
+}
+
+// The half steps of the first and last planes (z = 0 and nz - 1) into ZA, ZB (two planes each):
+// what the fused march takes for its ghost planes beyond a non-periodic z face (load_pred,
+// hydro/update.wgsl:47-63, maps a ghost to one of these cells). Grid (x, y, 2).
+template <typename T, int LIM, bool ISEN>
+__global__ __launch_bounds__(256) void k_zplanes(Params<T> P, const T4<T>* __restrict__ U1,
+                                                 const T4<T>* __restrict__ U2, T4<T>* __restrict__ ZA,
+                                                 T4<T>* __restrict__ ZB, const T* clk) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= P.n[0] || j >= P.n[1]) return;
+  const int k = blockIdx.z == 0 ? 0 : P.n[2] - 1;
+  T4<T> a, b;
+  predict_cell<T, LIM, ISEN>(P, U1, U2, i, j, k, clk[0], a, b);
+  const size_t o = (size_t)blockIdx.z * P.n[0] * P.n[1] + (size_t)j * P.n[0] + i;
+  ZA[o] = a;
+  ZB[o] = b;
 }
 
 // flux_between (hydro/flux.wgsl:254-347): the face between cell a and a + e_axis
@@ -294,10 +319,30 @@ __device__ __forceinline__ Flux<T> lds_get_flux(const T* L, int base, int t) {
 }
 
 
+// With a non-periodic z face, a ghost plane (c < 0 or c >= nz) takes the half step of the cell it
+// maps to (load_pred, hydro/update.wgsl:47-63), with that cell's own stencil: plane 0 or nz - 1,
+// held in ZA/ZB (k_zplanes), with its normal velocity seen through the face. (Beyond a periodic
+// face of such a grid too: the wrapped cell's stencil meets the other, non-periodic face.)
+// Other planes keep h.
+template <typename T>
+__device__ __forceinline__ Half<T> zghost_half(const Params<T>& P, const T4<T>* __restrict__ ZA,
+                                               const T4<T>* __restrict__ ZB, int c, size_t col, const Half<T>& h) {
+  const int nz = P.n[2];
+  if (c >= 0 && c < nz) return h;
+  const AxisMap mz = map_axis(c, nz, P.bc_lo[2], P.bc_hi[2]);
+  const size_t o = (mz.i == 0 ? 0 : (size_t)P.n[0] * P.n[1]) + col;
+  const T4<T> a = ZA[o], b = ZB[o];
+  Half<T> g;
+  g.rho = a.x; g.vx = a.y; g.vy = a.z; g.vz = bc_vel(a.w, mz.flip, mz.side);
+  g.p = b.x; g.s = b.y; g.shock = b.z; g.fo = b.w;
+  return g;
+}
+
 // The fused step. Block BX x BY threads; thread (tx, ty) owns column (x0 + tx - 2, y0 + ty - 2).
 // Grid (ceil(nx / TX), ceil(ny / TY), ceil(nz / zc)). Planes [z0, z0 + zc) are updated.
-// Boundary conditions along x and y: periodic, outflow or reflecting; along z: periodic (the
-// host routes other z boundaries to the plain path).
+// Boundary conditions along x and y: periodic, outflow or reflecting (an axis periodic on one
+// face only goes to the plain path); along z: any, a non-periodic face through the boundary
+// planes' half steps (ZA, ZB: zghost_half).
 //
 // Launch bound: in FP32 at least four waves per SIMD (two 512-thread blocks or one of 1024 per
 // CU), which caps the kernel at 128 VGPRs; the FP64 twin, whose values take two registers each,
@@ -314,7 +359,8 @@ __device__ __forceinline__ Flux<T> lds_get_flux(const T* L, int base, int t) {
 template <typename T, int LIM, int RS, bool ISEN, int BX, int BY, bool DEBUG, bool MAPBC>
 __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 4 : 1)) void k_fused(
     Params<T> P, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1,
-    T4<T>* __restrict__ O2, const T* __restrict__ clk, int zc, unsigned long long* next_slot) {
+    T4<T>* __restrict__ O2, const T* __restrict__ clk, const T4<T>* __restrict__ ZA, const T4<T>* __restrict__ ZB,
+    int zc, unsigned long long* next_slot) {
   constexpr int NT = BX * BY;
   constexpr int TX = BX - 4, TY = BY - 4;
   using LY = FusedLayout<T, NT>;
@@ -352,11 +398,20 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 4 : 1)) void k_fused(
   const bool mapped = MAPBC && ((sx != tx) || (sy != ty));
   const int ts = min(max(sx, 1), BX - 2) + BX * min(max(sy, 1), BY - 2);
 
-  auto plane_of = [&](int z) { return (size_t)(((z % nz) + nz) % nz) * plane + col; };
-  auto prim_of = [&](T4<T> u1, T4<T> u2) {
+  // z: periodic, or (ZA given) a non-periodic face that ghost planes map through
+  const bool zbc = ZA != nullptr;
+  auto plane_of = [&](int z) {
+    const int zi = zbc ? map_axis(z, nz, P.bc_lo[2], P.bc_hi[2]).i : ((z % nz) + nz) % nz;
+    return (size_t)zi * plane + col;
+  };
+  auto prim_of = [&](T4<T> u1, T4<T> u2, int z) {
     Prim<T> q = prim_from_cons(P, u1, u2);
     q.vx = bc_vel(q.vx, mx.flip, mx.side);
     q.vy = bc_vel(q.vy, my.flip, my.side);
+    if (zbc && (z < 0 || z >= nz)) {
+      const AxisMap mz = map_axis(z, nz, P.bc_lo[2], P.bc_hi[2]);
+      q.vz = bc_vel(q.vz, mz.flip, mz.side);
+    }
     return q;
   };
 
@@ -364,9 +419,9 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 4 : 1)) void k_fused(
   // c - 1 and c; plane z0 in registers
   T4<T> b1, b2;
   size_t o0 = plane_of(z0 - 2);
-  lds_put_prim<T, NT>(L + LY::Q0, t, prim_of(U1[o0], U2[o0]));
+  lds_put_prim<T, NT>(L + LY::Q0, t, prim_of(U1[o0], U2[o0], z0 - 2));
   o0 = plane_of(z0 - 1);
-  lds_put_prim<T, NT>(L + LY::Q1, t, prim_of(U1[o0], U2[o0]));
+  lds_put_prim<T, NT>(L + LY::Q1, t, prim_of(U1[o0], U2[o0], z0 - 1));
   o0 = plane_of(z0);
   b1 = U1[o0]; b2 = U2[o0];
 
@@ -376,7 +431,7 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 4 : 1)) void k_fused(
   T rate_max = T(0);
 
   for (int c = z0 - 1; c <= z1; c++) {
-    Prim<T> pp = prim_of(b1, b2);
+    Prim<T> pp = prim_of(b1, b2, c + 1);
     if (c + 2 <= z1 + 1) {             // prefetch plane c + 2
       size_t o = plane_of(c + 2);
       b1 = U1[o];
@@ -426,6 +481,7 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 4 : 1)) void k_fused(
         hc.vx = bc_vel(hc.vx, mx.flip, mx.side);
         hc.vy = bc_vel(hc.vy, my.flip, my.side);
       }
+      if (zbc && tile) hc = zghost_half(P, ZA, ZB, c, (size_t)y * nx + x, hc);
       sxl = side_of(hc, ixd, T(-1));
       syl = side_of(hc, iyd, T(-1));
       szl = side_of(hc, izd, T(-1));
@@ -616,8 +672,8 @@ struct UniformImpl {
     }
   }
   template <int BX, int BY>
-  static void fused_tile(const Launch& L, const void* u1, const void* u2, void* o1, void* o2, const void* clk, int zc,
-                         unsigned long long* next_slot, bool debug) {
+  static void fused_tile(const Launch& L, const void* u1, const void* u2, void* o1, void* o2, const void* clk,
+                         const void* za, const void* zb, int zc, unsigned long long* next_slot, bool debug) {
     const Params<T>& P = *static_cast<const Params<T>*>(L.params);
     constexpr int TX = BX - 4, TY = BY - 4;
     dim3 g((P.n[0] + TX - 1) / TX, (P.n[1] + TY - 1) / TY, (P.n[2] + zc - 1) / zc);
@@ -625,38 +681,47 @@ struct UniformImpl {
     for (int a = 0; a < 2; a++) mapbc |= P.bc_lo[a] != BC_PERIODIC || P.bc_hi[a] != BC_PERIODIC;
 #define SHX_FUSED(D, M)                                                                                          \
   hipLaunchKernelGGL((k_fused<T, LIM, RS, ISEN, BX, BY, D, M>), g, dim3(BX, BY), 0, L.stream, P, (const T4<T>*)u1, \
-                     (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, zc, next_slot)
+                     (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, (const T4<T>*)za, (const T4<T>*)zb, zc,  \
+                     next_slot)
     if (debug) { if (mapbc) SHX_FUSED(true, true); else SHX_FUSED(true, false); }
     else { if (mapbc) SHX_FUSED(false, true); else SHX_FUSED(false, false); }
 #undef SHX_FUSED
   }
   template <int TY>
-  static void fused_roles(const Launch& L, const void* u1, const void* u2, void* o1, void* o2, const void* clk, int zc,
-                          unsigned long long* next_slot, bool debug) {
+  static void fused_roles(const Launch& L, const void* u1, const void* u2, void* o1, void* o2, const void* clk,
+                          const void* za, const void* zb, int zc, unsigned long long* next_slot, bool debug) {
     const Params<T>& P = *static_cast<const Params<T>*>(L.params);
     dim3 g((P.n[0] + 31) / 32, (P.n[1] + TY - 1) / TY, (P.n[2] + zc - 1) / zc);
     bool mapbc = false;
     for (int a = 0; a < 2; a++) mapbc |= P.bc_lo[a] != BC_PERIODIC || P.bc_hi[a] != BC_PERIODIC;
 #define SHX_FUSED_R(D, M)                                                                                            \
   hipLaunchKernelGGL((k_fused_r<T, LIM, RS, ISEN, TY, D, M>), g, dim3(RoleTile<32, TY>::NC), 0, L.stream, P,         \
-                     (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, zc, next_slot)
+                     (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, (const T4<T>*)za, \
+                     (const T4<T>*)zb, zc, next_slot)
     if (debug) { if (mapbc) SHX_FUSED_R(true, true); else SHX_FUSED_R(true, false); }
     else { if (mapbc) SHX_FUSED_R(false, true); else SHX_FUSED_R(false, false); }
 #undef SHX_FUSED_R
   }
+  // za, zb: the half steps of planes 0 and nz - 1 for a non-periodic z face (null: z periodic)
   static void fused(const Launch& L, int tile, const void* u1, const void* u2, void* o1, void* o2, const void* clk,
-                    int zc, unsigned long long* next_slot, bool debug) {
+                    const void* za, const void* zb, int zc, unsigned long long* next_slot, bool debug) {
     if constexpr (sizeof(T) == 8) {
-      fused_tile<20, 20>(L, u1, u2, o1, o2, clk, zc, next_slot, debug);
+      fused_tile<20, 20>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug);
     } else {
       switch (tile) {
-        case FUSED_R32x16: fused_roles<16>(L, u1, u2, o1, o2, clk, zc, next_slot, debug); break;
-        case FUSED_R32x8: fused_roles<8>(L, u1, u2, o1, o2, clk, zc, next_slot, debug); break;
-        case FUSED_32x32: fused_tile<32, 32>(L, u1, u2, o1, o2, clk, zc, next_slot, debug); break;
-        case FUSED_32x16: fused_tile<32, 16>(L, u1, u2, o1, o2, clk, zc, next_slot, debug); break;
-        default: fused_tile<20, 20>(L, u1, u2, o1, o2, clk, zc, next_slot, debug); break;
+        case FUSED_R32x16: fused_roles<16>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
+        case FUSED_R32x8: fused_roles<8>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
+        case FUSED_32x32: fused_tile<32, 32>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
+        case FUSED_32x16: fused_tile<32, 16>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
+        default: fused_tile<20, 20>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
       }
     }
+  }
+  static void zplanes(const Launch& L, const void* u1, const void* u2, void* za, void* zb, const void* clk) {
+    const Params<T>& P = *static_cast<const Params<T>*>(L.params);
+    dim3 b(16, 16, 1), g((P.n[0] + 15) / 16, (P.n[1] + 15) / 16, 2);
+    hipLaunchKernelGGL((k_zplanes<T, LIM, ISEN>), g, b, 0, L.stream, P, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)za,
+                       (T4<T>*)zb, (const T*)clk);
   }
   static void diag(const Launch& L, const void* u1, const void* u2, double* part, int nblocks, double* out) {
     const Params<T>& P = *static_cast<const Params<T>*>(L.params);
@@ -671,6 +736,7 @@ struct UniformImpl {
     o.predict = predict;
     o.update = update;
     o.fused = fused;
+    o.zplanes = zplanes;
     o.diag = diag;
     return o;
   }

@@ -64,6 +64,7 @@ __device__ __forceinline__ void role_column(int t, int& cx, int& cy) {
 
 // Launch bound: TY = 16, 720 threads (12 waves), one workgroup per CU at up to 168 VGPRs; TY = 8,
 // 432 threads (7 waves), two per CU at up to 128 VGPRs.
+// z: periodic, or through a non-periodic face as k_fused (ZA, ZB: the boundary planes' half steps).
 // Barriers: the half steps of plane c (sides, and plane c + 1's primitives) are visible after
 // the first, its x/y fluxes after the second. The sides of plane c + 1 overwrite those of plane c
 // after the second barrier of plane c (the last reads of sides were before it); the fluxes of
@@ -72,7 +73,8 @@ __device__ __forceinline__ void role_column(int t, int& cx, int& cy) {
 template <typename T, int LIM, int RS, bool ISEN, int TY, bool DEBUG, bool MAPBC>
 __global__ __launch_bounds__((RoleTile<32, TY>::NC), (TY == 16 ? 3 : 4)) void k_fused_r(
     Params<T> P, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1,
-    T4<T>* __restrict__ O2, const T* __restrict__ clk, int zc, unsigned long long* next_slot) {
+    T4<T>* __restrict__ O2, const T* __restrict__ clk, const T4<T>* __restrict__ ZA, const T4<T>* __restrict__ ZB, int zc,
+    unsigned long long* next_slot) {
   constexpr int TX = 32;
   using RT = RoleTile<TX, TY>;
   constexpr int NT = RT::NC, BX = RT::CX, BY = RT::CY;
@@ -110,21 +112,30 @@ __global__ __launch_bounds__((RoleTile<32, TY>::NC), (TY == 16 ? 3 : 4)) void k_
   const bool mapped = MAPBC && ((sx != cx) || (sy != cy));
   const int ts = (min(max(sy, -1), TY) + 2) * BX + (min(max(sx, -1), TX) + 2);
 
-  auto plane_of = [&](int z) { return (size_t)(((z % nz) + nz) % nz) * plane + col; };
-  auto prim_of = [&](T4<T> u1, T4<T> u2) {
+  // z: periodic, or (ZA given) a non-periodic face that ghost planes map through
+  const bool zbc = ZA != nullptr;
+  auto plane_of = [&](int z) {
+    const int zi = zbc ? map_axis(z, nz, P.bc_lo[2], P.bc_hi[2]).i : ((z % nz) + nz) % nz;
+    return (size_t)zi * plane + col;
+  };
+  auto prim_of = [&](T4<T> u1, T4<T> u2, int z) {
     Prim<T> q = prim_from_cons(P, u1, u2);
     q.vx = bc_vel(q.vx, mx.flip, mx.side);
     q.vy = bc_vel(q.vy, my.flip, my.side);
+    if (zbc && (z < 0 || z >= nz)) {
+      const AxisMap mz = map_axis(z, nz, P.bc_lo[2], P.bc_hi[2]);
+      q.vz = bc_vel(q.vz, mz.flip, mz.side);
+    }
     return q;
   };
 
   T4<T> b1, b2;
   size_t o0 = plane_of(z0 - 2);
-  lds_put_prim6<T, NT>(L + LY::Q0, t, prim_of(U1[o0], U2[o0]));
+  lds_put_prim6<T, NT>(L + LY::Q0, t, prim_of(U1[o0], U2[o0], z0 - 2));
   o0 = plane_of(z0 - 1);
   T cs_c;   // this column's sound speed in plane c
   {
-    Prim<T> q = prim_of(U1[o0], U2[o0]);
+    Prim<T> q = prim_of(U1[o0], U2[o0], z0 - 1);
     lds_put_prim6<T, NT>(L + LY::Q1, t, q);
     cs_c = q.cs;
   }
@@ -138,7 +149,7 @@ __global__ __launch_bounds__((RoleTile<32, TY>::NC), (TY == 16 ? 3 : 4)) void k_
   T rate_max = T(0);
 
   for (int c = z0 - 1; c <= z1; c++) {
-    Prim<T> pp = prim_of(b1, b2);
+    Prim<T> pp = prim_of(b1, b2, c + 1);
     if (c + 2 <= z1 + 1) {
       size_t o = plane_of(c + 2);
       b1 = U1[o];
@@ -183,6 +194,7 @@ __global__ __launch_bounds__((RoleTile<32, TY>::NC), (TY == 16 ? 3 : 4)) void k_
         hc.vx = bc_vel(hc.vx, mx.flip, mx.side);
         hc.vy = bc_vel(hc.vy, my.flip, my.side);
       }
+      if (zbc && upd) hc = zghost_half(P, ZA, ZB, c, (size_t)y * nx + x, hc);
       sxl = side_of(hc, ixd, T(-1));
       syl = side_of(hc, iyd, T(-1));
       szl = side_of(hc, izd, T(-1));

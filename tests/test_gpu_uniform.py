@@ -158,6 +158,41 @@ def test_every_fused_tile_equals_plain(tile):
         assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]), (tile, pb.id)
 
 
+@pytest.mark.parametrize("zbc", [(1, 1), (2, 2), (0, 2), (3, 1)])
+def test_fused_z_boundaries_equal_plain(zbc):
+    """Non-periodic z faces on the fused path (its ghost planes take the boundary planes' half
+    steps): every tile gives the plain path's bits, with outflow / reflecting x and y as well;
+    every face bit-identical from both sides."""
+    x, y, z = problems.centers((40, 36, 37), 1.0)
+    g = np.exp(-(x * x + y * y + (z - 0.3) ** 2) / 0.02)
+    pr = {"rho": 1.0 + 0.5 * g, "vx": 0.3 * np.sin(2 * np.pi * y), "vy": 0.2 * np.cos(2 * np.pi * z),
+          "vz": 0.6 * np.sin(2 * np.pi * x) - 1.5 * g, "p": 1.0 + 4.0 * g}
+    for bxy in (0, 2):
+        pb = problems.Problem(id="zbc", n=(40, 36, 37), box=1.0, gamma=5.0 / 3.0, limiter="mc", isentropic=False,
+                              cfl=0.4, prims=pr, bc_lo=(bxy, bxy, zbc[0]), bc_hi=(bxy, bxy, zbc[1]))
+        for tile in [(32, 8, 16), (28, 12, 8), (16, 16, 37)]:
+            out = []
+            print(f"\n[gpu] z bc {zbc} xy {bxy} tile {tile}", flush=True)
+            for path in ("plain", "fused"):
+                with Sim(config_for(pb, path=path, tile=tile)) as sim:
+                    sim.set_state(*pb.state())
+                    sim.step(4)
+                    out.append(sim.read_state())
+                    if path == "fused":
+                        mis, faces = sim.flux_mismatch()
+                        assert mis == 0, (zbc, bxy, tile, mis, faces)
+            assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]), (zbc, bxy, tile)
+
+
+def test_fused_outflow_z_matches_oracle():
+    pb = blob3d(24)
+    pb.bc_lo = (1, 2, 1)
+    pb.bc_hi = (1, 2, 2)
+    w = run_pair(pb, 5, path="fused")
+    print(w, flush=True)
+    assert max(w.values()) <= L1_TOL, w
+
+
 @pytest.mark.parametrize("path", ["plain", "fused"])
 def test_flux_determinism(path):
     """flux-determinism (src/validation/tests.ts:960-978): 0 mismatched faces."""
