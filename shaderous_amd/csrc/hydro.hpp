@@ -461,11 +461,25 @@ __device__ __forceinline__ Flux<T> face_from_sides(const Params<T>& P, const Sid
 
 // face_flux between the half-step states hl (left) and hr (right), given twice the increments
 // from each cell toward the face (il: the left cell's up increment, ir: the right cell's down
-// increment).
+// increment), for an axis known only at run time. The velocities are rotated into the face frame
+// here and the face solved as one along x, the same values in the same order as
+// face_from_sides(axis). The components are picked from registers (the empty asm makes them
+// opaque values): picking them from the side structs lets the compiler select by address, which
+// puts both sides in scratch memory.
 template <typename T, int RS>
 __device__ __forceinline__ Flux<T> face_flux(const Params<T>& P, const Half<T>& hl, const Inc<T>& il,
                                              const Half<T>& hr, const Inc<T>& ir, int axis) {
-  return face_from_sides<T, RS>(P, side_of(hl, il, T(1)), side_of(hr, ir, T(-1)), axis);
+  SideData<T> L = side_of(hl, il, T(1)), R = side_of(hr, ir, T(-1));
+  T lx = L.vx, ly = L.vy, lz = L.vz, rx = R.vx, ry = R.vy, rz = R.vz;
+  asm("" : "+v"(lx), "+v"(ly), "+v"(lz), "+v"(rx), "+v"(ry), "+v"(rz));
+  const bool a0 = axis == 0, a1 = axis == 1;
+  L.vx = a0 ? lx : (a1 ? ly : lz); L.vy = a0 ? ly : (a1 ? lz : lx); L.vz = a0 ? lz : (a1 ? lx : ly);
+  R.vx = a0 ? rx : (a1 ? ry : rz); R.vy = a0 ? ry : (a1 ? rz : rx); R.vz = a0 ? rz : (a1 ? rx : ry);
+  Flux<T> F = face_from_sides<T, RS>(P, L, R, 0);
+  T n = F.mx, t1 = F.my, t2 = F.mz;
+  asm("" : "+v"(n), "+v"(t1), "+v"(t2));
+  F.mx = a0 ? n : (a1 ? t2 : t1); F.my = a0 ? t1 : (a1 ? n : t2); F.mz = a0 ? t2 : (a1 ? t1 : n);
+  return F;
 }
 
 // advance (hydro/advance.wgsl:31-52), dual_energy (hydro/dual_energy.wgsl:73-87) and the

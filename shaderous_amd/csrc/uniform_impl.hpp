@@ -323,19 +323,19 @@ __device__ __forceinline__ Flux<T> lds_get_flux(const T* L, int base, int t) {
 // maps to (load_pred, hydro/update.wgsl:47-63), with that cell's own stencil: plane 0 or nz - 1,
 // held in ZA/ZB (k_zplanes), with its normal velocity seen through the face. (Beyond a periodic
 // face of such a grid too: the wrapped cell's stencil meets the other, non-periodic face.)
-// Other planes keep h.
+// Other planes keep h. The override is written field by field into h: a form that returns either
+// h or the loaded state lets the compiler select between their addresses, which puts h in scratch
+// memory on every plane.
 template <typename T>
-__device__ __forceinline__ Half<T> zghost_half(const Params<T>& P, const T4<T>* __restrict__ ZA,
-                                               const T4<T>* __restrict__ ZB, int c, size_t col, const Half<T>& h) {
+__device__ __forceinline__ void zghost_half(const Params<T>& P, const T4<T>* __restrict__ ZA,
+                                            const T4<T>* __restrict__ ZB, int c, size_t col, Half<T>& h) {
   const int nz = P.n[2];
-  if (c >= 0 && c < nz) return h;
+  if (c >= 0 && c < nz) return;
   const AxisMap mz = map_axis(c, nz, P.bc_lo[2], P.bc_hi[2]);
   const size_t o = (mz.i == 0 ? 0 : (size_t)P.n[0] * P.n[1]) + col;
   const T4<T> a = ZA[o], b = ZB[o];
-  Half<T> g;
-  g.rho = a.x; g.vx = a.y; g.vy = a.z; g.vz = bc_vel(a.w, mz.flip, mz.side);
-  g.p = b.x; g.s = b.y; g.shock = b.z; g.fo = b.w;
-  return g;
+  h.rho = a.x; h.vx = a.y; h.vy = a.z; h.vz = bc_vel(a.w, mz.flip, mz.side);
+  h.p = b.x; h.s = b.y; h.shock = b.z; h.fo = b.w;
 }
 
 // The fused step. Block BX x BY threads; thread (tx, ty) owns column (x0 + tx - 2, y0 + ty - 2).
@@ -393,13 +393,13 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 4 : 1)) void k_fused(
   int sx = tx, sy = ty;
   if (!real_x && ((x < 0 && P.bc_lo[0] != BC_PERIODIC) || (x >= nx && P.bc_hi[0] != BC_PERIODIC))) sx = tx + (mx.i - x);
   if (!real_y && ((y < 0 && P.bc_lo[1] != BC_PERIODIC) || (y >= ny && P.bc_hi[1] != BC_PERIODIC))) sy = ty + (my.i - y);
-  // (MAPBC: the grid has a non-periodic x or y face; without one no column is mapped and the
+  // (MAPBC: the grid has a non-periodic face; without one no column or plane is mapped and the
   // kernel is compiled without that path)
   const bool mapped = MAPBC && ((sx != tx) || (sy != ty));
   const int ts = min(max(sx, 1), BX - 2) + BX * min(max(sy, 1), BY - 2);
 
   // z: periodic, or (ZA given) a non-periodic face that ghost planes map through
-  const bool zbc = ZA != nullptr;
+  const bool zbc = MAPBC && ZA != nullptr;
   auto plane_of = [&](int z) {
     const int zi = zbc ? map_axis(z, nz, P.bc_lo[2], P.bc_hi[2]).i : ((z % nz) + nz) % nz;
     return (size_t)zi * plane + col;
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 4 : 1)) void k_fused(
         hc.vx = bc_vel(hc.vx, mx.flip, mx.side);
         hc.vy = bc_vel(hc.vy, my.flip, my.side);
       }
-      if (zbc && tile) hc = zghost_half(P, ZA, ZB, c, (size_t)y * nx + x, hc);
+      if (zbc && tile) zghost_half(P, ZA, ZB, c, (size_t)y * nx + x, hc);
       sxl = side_of(hc, ixd, T(-1));
       syl = side_of(hc, iyd, T(-1));
       szl = side_of(hc, izd, T(-1));
@@ -677,7 +677,7 @@ struct UniformImpl {
     const Params<T>& P = *static_cast<const Params<T>*>(L.params);
     constexpr int TX = BX - 4, TY = BY - 4;
     dim3 g((P.n[0] + TX - 1) / TX, (P.n[1] + TY - 1) / TY, (P.n[2] + zc - 1) / zc);
-    bool mapbc = false;
+    bool mapbc = za != nullptr;
     for (int a = 0; a < 2; a++) mapbc |= P.bc_lo[a] != BC_PERIODIC || P.bc_hi[a] != BC_PERIODIC;
 #define SHX_FUSED(D, M)                                                                                          \
   hipLaunchKernelGGL((k_fused<T, LIM, RS, ISEN, BX, BY, D, M>), g, dim3(BX, BY), 0, L.stream, P, (const T4<T>*)u1, \
@@ -692,7 +692,7 @@ struct UniformImpl {
                           const void* za, const void* zb, int zc, unsigned long long* next_slot, bool debug) {
     const Params<T>& P = *static_cast<const Params<T>*>(L.params);
     dim3 g((P.n[0] + 31) / 32, (P.n[1] + TY - 1) / TY, (P.n[2] + zc - 1) / zc);
-    bool mapbc = false;
+    bool mapbc = za != nullptr;
     for (int a = 0; a < 2; a++) mapbc |= P.bc_lo[a] != BC_PERIODIC || P.bc_hi[a] != BC_PERIODIC;
 #define SHX_FUSED_R(D, M)                                                                                            \
   hipLaunchKernelGGL((k_fused_r<T, LIM, RS, ISEN, TY, D, M>), g, dim3(RoleTile<32, TY>::NC), 0, L.stream, P,         \

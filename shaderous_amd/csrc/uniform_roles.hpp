@@ -113,7 +113,7 @@ __global__ __launch_bounds__((RoleTile<32, TY>::NC), (TY == 16 ? 3 : 4)) void k_
   const int ts = (min(max(sy, -1), TY) + 2) * BX + (min(max(sx, -1), TX) + 2);
 
   // z: periodic, or (ZA given) a non-periodic face that ghost planes map through
-  const bool zbc = ZA != nullptr;
+  const bool zbc = MAPBC && ZA != nullptr;
   auto plane_of = [&](int z) {
     const int zi = zbc ? map_axis(z, nz, P.bc_lo[2], P.bc_hi[2]).i : ((z % nz) + nz) % nz;
     return (size_t)zi * plane + col;
@@ -194,7 +194,7 @@ __global__ __launch_bounds__((RoleTile<32, TY>::NC), (TY == 16 ? 3 : 4)) void k_
         hc.vx = bc_vel(hc.vx, mx.flip, mx.side);
         hc.vy = bc_vel(hc.vy, my.flip, my.side);
       }
-      if (zbc && upd) hc = zghost_half(P, ZA, ZB, c, (size_t)y * nx + x, hc);
+      if (zbc && upd) zghost_half(P, ZA, ZB, c, (size_t)y * nx + x, hc);
       sxl = side_of(hc, ixd, T(-1));
       syl = side_of(hc, iyd, T(-1));
       szl = side_of(hc, izd, T(-1));
