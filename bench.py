@@ -35,20 +35,20 @@ BYTES_PER_UPDATE = 64      # read 8 FP32 conserved values, write 8
 # WRITE_SIZE (scaled by the counters' reading of a 512 MiB device copy: 2.0 and 1.0), for the
 # configuration they were measured on: profiles/k_fused_turbulence_256.md
 TRAFFIC_MEASURED = {
-    ("k_fused", 256, (28, 12, 32)): (2 * 900838 + 524343) * 1024,
-    ("k_fused_r", 256, (32, 8, 64)): (2 * 650672 + 524320) * 1024,
-    ("k_fused_r", 256, (32, 8, 128)): (2 * 630350 + 524304) * 1024,
-    ("k_fused_r", 256, (32, 16, 128)): (2 * 663997 + 524296) * 1024,
+    ("k_fused_r", 256, (32, 8, 64)): (2 * 656817 + 524320) * 1024,
+    ("k_fused_r", 256, (32, 8, 128)): (2 * 639784 + 524304) * 1024,
+    ("k_fused_r", 256, (32, 16, 64)): (2 * 670860 + 524304) * 1024,
+    ("k_fused_r", 256, (32, 16, 128)): (2 * 664321 + 524296) * 1024,
 }
 
 
 # VALU instructions (wave64) per launch of the dominant kernel, measured with rocprofv3 --pmc
 # SQ_INSTS_VALU for the configuration they were measured on (profiles/k_fused_turbulence_256.md)
 VALU_MEASURED = {
-    ("k_fused_r", 256, (32, 8, 64)): 347257461,
-    ("k_fused_r", 256, (32, 8, 128)): 343363812,
-    ("k_fused_r", 256, (32, 16, 128)): 302267580,
-    ("k_fused_r", 256, (32, 16, 64)): 305729005,
+    ("k_fused_r", 256, (32, 8, 64)): 321597499,
+    ("k_fused_r", 256, (32, 8, 128)): 317745278,
+    ("k_fused_r", 256, (32, 16, 128)): 279585481,
+    ("k_fused_r", 256, (32, 16, 64)): 282991080,
 }
 # MI355X VALU issue peak for FP32 without packed math: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_LANE_PEAK = 256 * 4 * 16 * 2.4e9
