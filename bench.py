@@ -45,10 +45,8 @@ TRAFFIC_MEASURED = {
 # VALU instructions (wave64) per launch of the dominant kernel, measured with rocprofv3 --pmc
 # SQ_INSTS_VALU for the configuration they were measured on (profiles/k_fused_turbulence_256.md)
 VALU_MEASURED = {
-    ("k_fused_r", 256, (32, 8, 64)): 321597499,
-    ("k_fused_r", 256, (32, 8, 128)): 317745278,
-    ("k_fused_r", 256, (32, 16, 128)): 279585481,
-    ("k_fused_r", 256, (32, 16, 64)): 282991080,
+    ("k_fused_r", 256, (32, 8, 64)): 285507643,
+    ("k_fused_r", 256, (32, 16, 128)): 249966281,
 }
 # MI355X VALU issue peak for FP32 without packed math: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_LANE_PEAK = 256 * 4 * 16 * 2.4e9

@@ -62,6 +62,12 @@ template <typename T> __device__ __forceinline__ T tmix(T x, T y, T a) { return 
 template <typename T> __device__ __forceinline__ bool same_sign(T a, T b) {
   return (a > T(0) && b > T(0)) || (a < T(0) && b < T(0));
 }
+// minmod(x, y) = sign(x) min(|x|, |y|) when sign(x) sign(y) > 0, else 0: in FP32 the median of
+// (0, x, y), one instruction (a zero result may carry the sign of x or y)
+__device__ __forceinline__ float minmod(float x, float y) { return __builtin_amdgcn_fmed3f(0.0f, x, y); }
+__device__ __forceinline__ double minmod(double x, double y) {
+  return same_sign(x, y) ? (x > 0.0 ? tmin(x, y) : tmax(x, y)) : 0.0;
+}
 
 // Primitive record (hydro/q.wgsl:56): rho, v, p, scalar s, sound speed c.
 template <typename T>
@@ -145,16 +151,15 @@ __device__ __forceinline__ T edge1(T far, T near, T scale) {
       T fk = far * k, nk = near * k;
       eta = fk * fk + nk * nk;
     }
-    T m = tmin(tmin(T(2) * tabs(far), T(2) * tabs(near)), tabs(fr));
-    T lim = same_sign(far, near) ? (near > T(0) ? m : -m) : T(0);
+    // sign(near) min(2|far|, 2|near|, |fr|) where far and near share a sign (fr then does too)
+    T lim = minmod(T(2) * far, minmod(T(2) * near, fr));
     T w = tclamp(T(1.5) - eta, T(0), T(1));
     return (w == T(1)) ? fr : tmix(lim, fr, w);
   } else if constexpr (LIM == LIM_MC) {
-    T m = tmin(tmin(T(2) * tabs(far), T(2) * tabs(near)), T(0.5) * tabs(far + near));
-    return same_sign(far, near) ? (far > T(0) ? m : -m) : T(0);
+    // sign(far) min(2|far|, 2|near|, |far + near| / 2) where far and near share a sign
+    return minmod(T(2) * far, minmod(T(2) * near, T(0.5) * (far + near)));
   } else if constexpr (LIM == LIM_MINMOD) {
-    T m = tmin(tabs(far), tabs(near));
-    return same_sign(far, near) ? (far > T(0) ? m : -m) : T(0);
+    return minmod(far, near);
   } else {
     return same_sign(far, near) ? T(2) * far * tdiv(near, far + near) : T(0);
   }
