@@ -45,8 +45,8 @@ TRAFFIC_MEASURED = {
 # VALU instructions (wave64) per launch of the dominant kernel, measured with rocprofv3 --pmc
 # SQ_INSTS_VALU for the configuration they were measured on (profiles/k_fused_turbulence_256.md)
 VALU_MEASURED = {
-    ("k_fused_r", 256, (32, 8, 64)): 285507643,
-    ("k_fused_r", 256, (32, 16, 128)): 249966281,
+    ("k_fused_r", 256, (32, 8, 64)): 286394543,
+    ("k_fused_r", 256, (16, 16, 64)): 264811530,
 }
 # MI355X VALU issue peak for FP32 without packed math: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_LANE_PEAK = 256 * 4 * 16 * 2.4e9
@@ -283,8 +283,8 @@ def main():
             print(json.dumps({"amr": amr, "ensemble": ens}))
         return
 
-    # threads by role for the 32-wide tiles (k_fused_r), raster order otherwise (k_fused)
-    kname = "k_fused_r" if tile and tile[0] == 32 else "k_fused"
+    # raster order for the 28-wide tiles (k_fused), threads by role otherwise (k_fused_r)
+    kname = "k_fused" if tile and tile[0] == 28 else "k_fused_r"
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (RefHydro restatement) ...")

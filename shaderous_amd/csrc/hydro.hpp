@@ -464,17 +464,13 @@ __device__ __forceinline__ Flux<T> face_from_sides(const Params<T>& P, const Sid
   return F;
 }
 
-// face_flux between the half-step states hl (left) and hr (right), given twice the increments
-// from each cell toward the face (il: the left cell's up increment, ir: the right cell's down
-// increment), for an axis known only at run time. The velocities are rotated into the face frame
-// here and the face solved as one along x, the same values in the same order as
+// face_from_sides for an axis known only at run time: the velocities are rotated into the face
+// frame here and the face solved as one along x, the same values in the same order as
 // face_from_sides(axis). The components are picked from registers (the empty asm makes them
 // opaque values): picking them from the side structs lets the compiler select by address, which
 // puts both sides in scratch memory.
 template <typename T, int RS>
-__device__ __forceinline__ Flux<T> face_flux(const Params<T>& P, const Half<T>& hl, const Inc<T>& il,
-                                             const Half<T>& hr, const Inc<T>& ir, int axis) {
-  SideData<T> L = side_of(hl, il, T(1)), R = side_of(hr, ir, T(-1));
+__device__ __forceinline__ Flux<T> face_from_sides_rt(const Params<T>& P, SideData<T> L, SideData<T> R, int axis) {
   T lx = L.vx, ly = L.vy, lz = L.vz, rx = R.vx, ry = R.vy, rz = R.vz;
   asm("" : "+v"(lx), "+v"(ly), "+v"(lz), "+v"(rx), "+v"(ry), "+v"(rz));
   const bool a0 = axis == 0, a1 = axis == 1;
@@ -485,6 +481,15 @@ __device__ __forceinline__ Flux<T> face_flux(const Params<T>& P, const Half<T>& 
   asm("" : "+v"(n), "+v"(t1), "+v"(t2));
   F.mx = a0 ? n : (a1 ? t2 : t1); F.my = a0 ? t1 : (a1 ? n : t2); F.mz = a0 ? t2 : (a1 ? t1 : n);
   return F;
+}
+
+// face_flux between the half-step states hl (left) and hr (right), given twice the increments
+// from each cell toward the face (il: the left cell's up increment, ir: the right cell's down
+// increment), along a run-time axis.
+template <typename T, int RS>
+__device__ __forceinline__ Flux<T> face_flux(const Params<T>& P, const Half<T>& hl, const Inc<T>& il,
+                                             const Half<T>& hr, const Inc<T>& ir, int axis) {
+  return face_from_sides_rt<T, RS>(P, side_of(hl, il, T(1)), side_of(hr, ir, T(-1)), axis);
 }
 
 // advance (hydro/advance.wgsl:31-52), dual_energy (hydro/dual_energy.wgsl:73-87) and the

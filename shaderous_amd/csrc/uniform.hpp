@@ -14,6 +14,7 @@ struct Launch {
 
 // Column tiles of the fused kernel: k_fused's (block BX x BY threads in raster order, tile
 // (BX - 4) x (BY - 4) cells) and k_fused_r's (threads by role, tiles of 32 x 16 or 32 x 8 cells).
+// FUSED_20x20 is a 16 x 16 cell tile: k_fused_r's in FP32, k_fused's (the only FP64 tile) in FP64.
 enum { FUSED_20x20 = 0, FUSED_32x16 = 1, FUSED_32x32 = 2, FUSED_R32x16 = 3, FUSED_R32x8 = 4, FUSED_NTILES = 5 };
 // cells per tile along x and y
 inline void fused_tile_cells(int tile, int& tx, int& ty) {

@@ -687,15 +687,15 @@ struct UniformImpl {
     else { if (mapbc) SHX_FUSED(false, true); else SHX_FUSED(false, false); }
 #undef SHX_FUSED
   }
-  template <int TY>
+  template <int TX, int TY>
   static void fused_roles(const Launch& L, const void* u1, const void* u2, void* o1, void* o2, const void* clk,
                           const void* za, const void* zb, int zc, unsigned long long* next_slot, bool debug) {
     const Params<T>& P = *static_cast<const Params<T>*>(L.params);
-    dim3 g((P.n[0] + 31) / 32, (P.n[1] + TY - 1) / TY, (P.n[2] + zc - 1) / zc);
+    dim3 g((P.n[0] + TX - 1) / TX, (P.n[1] + TY - 1) / TY, (P.n[2] + zc - 1) / zc);
     bool mapbc = za != nullptr;
     for (int a = 0; a < 2; a++) mapbc |= P.bc_lo[a] != BC_PERIODIC || P.bc_hi[a] != BC_PERIODIC;
 #define SHX_FUSED_R(D, M)                                                                                            \
-  hipLaunchKernelGGL((k_fused_r<T, LIM, RS, ISEN, TY, D, M>), g, dim3(RoleTile<32, TY>::NC), 0, L.stream, P,         \
+  hipLaunchKernelGGL((k_fused_r<T, LIM, RS, ISEN, TX, TY, D, M>), g, dim3(RoleTile<TX, TY>::NTH), 0, L.stream, P,     \
                      (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, (const T4<T>*)za, \
                      (const T4<T>*)zb, zc, next_slot)
     if (debug) { if (mapbc) SHX_FUSED_R(true, true); else SHX_FUSED_R(true, false); }
@@ -709,11 +709,12 @@ struct UniformImpl {
       fused_tile<20, 20>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug);
     } else {
       switch (tile) {
-        case FUSED_R32x16: fused_roles<16>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
-        case FUSED_R32x8: fused_roles<8>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
+        case FUSED_R32x16: fused_roles<32, 16>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
+        case FUSED_R32x8: fused_roles<32, 8>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
         case FUSED_32x32: fused_tile<32, 32>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
         case FUSED_32x16: fused_tile<32, 16>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
-        default: fused_tile<20, 20>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
+        // 16 x 16 cells: threads by role in FP32
+        default: fused_roles<16, 16>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
       }
     }
   }

@@ -1,23 +1,33 @@
 // The fused uniform-grid step with threads by role (k_fused_r): the same arithmetic as k_fused,
 // with a different map of threads to the tile's columns and two barriers per plane instead of
-// four. A tile is TX x TY cells (TX = 32; TY = 16 or 8, so 256^3 and other power-of-two grids
-// split into whole tiles); its (TX + 4) x (TY + 4) columns are owned by threads in this order:
-//   threads [0, TX TY)            the cell columns, x fastest: whole waves of cell work
-//   the next 2 TX + 2 TY + 4      ring 1 (half steps), the TX + TY columns whose faces with the
-//                                 tile's cells are solved here first, so one wave holds them all
-//   the last 2 TX + 2 TY + 12     ring 2 (loads only)
-// The cell waves never carry halo lanes, and only one ring wave solves faces. LDS holds the plane
-// data in raster order of the columns, with the x/y fluxes apart from the sides (RoleLayout), so
-// a plane needs one barrier after the half steps and one after the fluxes.
+// four. A tile is TX x TY cells (32 x 16, 32 x 8 or 16 x 16, so 256^3 and other power-of-two
+// grids split into whole tiles); of its (TX + 4) x (TY + 4) columns, threads own, in this order:
+//   [0, TX TY)               the cell columns, x fastest: whole waves of cell work
+//   the next 2 TX + 2 TY     ring 1 without its corners (half steps): the top row and the right
+//                            column first (they also solve the faces they share with the tile's
+//                            cells), then the bottom row and the left column
+//   the next 4               ring 1's corners (loads only: their primitives feed the transverse
+//                            slopes of ring 1's ends)
+//   the last 2 TX + 2 TY     ring 2 without its corners (loads only); ring 2's corners are never
+//                            read
+// The cell waves never carry halo lanes; at 16 x 16 the 64 half-step lanes of ring 1 are one
+// wave. LDS holds the plane data in raster order of the columns, with the x/y fluxes apart from
+// the sides (RoleLayout), so a plane needs one barrier after the half steps and one after the
+// fluxes.
 #pragma once
 
 namespace shx {
 
 template <int TX, int TY>
 struct RoleTile {
-  static constexpr int CX = TX + 4, CY = TY + 4, NC = CX * CY;
+  static constexpr int CX = TX + 4, CY = TY + 4, NC = CX * CY;   // LDS columns
   static constexpr int NCELL = TX * TY;
-  static constexpr int R1 = 2 * TX + 2 * TY + 4;
+  static constexpr int NHALF = NCELL + 2 * TX + 2 * TY;          // threads that form half steps
+  static constexpr int NFACE = NCELL + TX + TY;                  // threads that solve x/y faces
+  static constexpr int NTH = NHALF + 4 + 2 * TX + 2 * TY;        // threads
+  // waves per SIMD the launch bound asks for: one 12-wave workgroup per CU at 32 x 16 (168
+  // VGPRs), two 7-wave workgroups otherwise (128 VGPRs)
+  static constexpr int MINW = NCELL > 256 ? 3 : 4;
 };
 
 // LDS of k_fused_r, per column: primitives of two planes (rho, v, p, s: the centre's sound speed
@@ -41,41 +51,38 @@ __device__ __forceinline__ void role_column(int t, int& cx, int& cy) {
   using RT = RoleTile<TX, TY>;
   if (t < RT::NCELL) { cx = t % TX; cy = t / TX; return; }
   t -= RT::NCELL;
-  if (t < RT::R1) {
-    if (t < TX) { cx = t; cy = TY; return; }             // y faces on the top row
-    t -= TX;
-    if (t < TY) { cx = TX; cy = t; return; }             // x faces on the right column
-    t -= TY;
-    if (t < 2) { cx = t ? TX : -1; cy = TY; return; }    // the top corners
-    t -= 2;
-    if (t < TX + 2) { cx = t - 1; cy = -1; return; }     // the bottom row
-    t -= TX + 2;
-    cx = -1; cy = t;                                     // the left column
-    return;
-  }
-  t -= RT::R1;
-  if (t < TX + 4) { cx = t - 2; cy = -2; return; }
-  t -= TX + 4;
-  if (t < TX + 4) { cx = t - 2; cy = TY + 1; return; }
-  t -= TX + 4;
-  if (t < TY + 2) { cx = -2; cy = t - 1; return; }
-  cx = TX + 1; cy = t - (TY + 2) - 1;
+  if (t < TX) { cx = t; cy = TY; return; }             // ring 1: y faces on the top row
+  t -= TX;
+  if (t < TY) { cx = TX; cy = t; return; }             // x faces on the right column
+  t -= TY;
+  if (t < TX) { cx = t; cy = -1; return; }             // the bottom row
+  t -= TX;
+  if (t < TY) { cx = -1; cy = t; return; }             // the left column
+  t -= TY;
+  if (t < 4) { cx = (t & 1) ? TX : -1; cy = (t & 2) ? TY : -1; return; }   // the corners
+  t -= 4;
+  if (t < TX) { cx = t; cy = -2; return; }             // ring 2: bottom, top, left, right
+  t -= TX;
+  if (t < TX) { cx = t; cy = TY + 1; return; }
+  t -= TX;
+  if (t < TY) { cx = -2; cy = t; return; }
+  t -= TY;
+  cx = TX + 1; cy = t;
 }
 
-// Launch bound: TY = 16, 720 threads (12 waves), one workgroup per CU at up to 168 VGPRs; TY = 8,
-// 432 threads (7 waves), two per CU at up to 128 VGPRs.
+// Launch bound: 32 x 16, 708 threads (12 waves), one workgroup per CU at up to 168 VGPRs;
+// 32 x 8 (420 threads) and 16 x 16 (388), 7 waves, two per CU at up to 128 VGPRs.
 // z: periodic, or through a non-periodic face as k_fused (ZA, ZB: the boundary planes' half steps).
 // Barriers: the half steps of plane c (sides, and plane c + 1's primitives) are visible after
 // the first, its x/y fluxes after the second. The sides of plane c + 1 overwrite those of plane c
 // after the second barrier of plane c (the last reads of sides were before it); the fluxes of
 // plane c + 1 overwrite those of plane c after the first barrier of plane c + 1; each primitive
 // buffer is rewritten a plane after its neighbours were last read from it.
-template <typename T, int LIM, int RS, bool ISEN, int TY, bool DEBUG, bool MAPBC>
-__global__ __launch_bounds__((RoleTile<32, TY>::NC), (TY == 16 ? 3 : 4)) void k_fused_r(
+template <typename T, int LIM, int RS, bool ISEN, int TX, int TY, bool DEBUG, bool MAPBC>
+__global__ __launch_bounds__((RoleTile<TX, TY>::NTH), (RoleTile<TX, TY>::MINW)) void k_fused_r(
     Params<T> P, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1,
     T4<T>* __restrict__ O2, const T* __restrict__ clk, const T4<T>* __restrict__ ZA, const T4<T>* __restrict__ ZB, int zc,
     unsigned long long* next_slot) {
-  constexpr int TX = 32;
   using RT = RoleTile<TX, TY>;
   constexpr int NT = RT::NC, BX = RT::CX, BY = RT::CY;
   using LY = RoleLayout<NT>;
@@ -99,7 +106,8 @@ __global__ __launch_bounds__((RoleTile<32, TY>::NC), (TY == 16 ? 3 : 4)) void k_
   const size_t col = (size_t)my.i * nx + mx.i;
   const size_t plane = (size_t)nx * ny;
 
-  const bool in_r1 = cx >= -1 && cx <= TX && cy >= -1 && cy <= TY;
+  const bool in_r1 = tid < RT::NHALF;              // cells and ring 1 without its corners
+  const bool ring_face = !cell && tid < RT::NFACE; // ring 1's top row and right column
   const bool real_x = x >= 0 && x < nx, real_y = y >= 0 && y < ny;
   const bool upd = cell && real_x && real_y;
   const bool xface = cx >= 0 && cx <= TX && cy >= 0 && cy < TY && x <= nx && real_y;
@@ -209,12 +217,20 @@ __global__ __launch_bounds__((RoleTile<32, TY>::NC), (TY == 16 ? 3 : 4)) void k_
     // 3. the x and y faces below this column, and the z face below plane c
     const bool live = c >= z0 && c < z1;
     Flux<T> Fx = zero_flux<T>(), Fy = zero_flux<T>(), fz = zero_flux<T>();
-    if (live && xface) Fx = face_from_sides<T, RS>(P, lds_get_side<T, NT>(L, LY::SX, t - 1), sxl, 0);
-    if (live && yface) Fy = face_from_sides<T, RS>(P, lds_get_side<T, NT>(L, LY::SY, t - BX), syl, 1);
-    if (upd && c >= z0) fz = face_from_sides<T, RS>(P, zs_prev, szl, 2);
-    // 4. the x/y fluxes into LDS
-    if (live && xface) lds_put_flux<T, NT>(L, LY::FX, t, Fx);
-    if (live && yface) lds_put_flux<T, NT>(L, LY::FY, t, Fy);
+    if (cell) {
+      if (live && xface) Fx = face_from_sides<T, RS>(P, lds_get_side<T, NT>(L, LY::SX, t - 1), sxl, 0);
+      if (live && yface) Fy = face_from_sides<T, RS>(P, lds_get_side<T, NT>(L, LY::SY, t - BX), syl, 1);
+      if (upd && c >= z0) fz = face_from_sides<T, RS>(P, zs_prev, szl, 2);
+      // 4. the x/y fluxes into LDS
+      if (live && xface) lds_put_flux<T, NT>(L, LY::FX, t, Fx);
+      if (live && yface) lds_put_flux<T, NT>(L, LY::FY, t, Fy);
+    } else if (ring_face && live && (xface || yface)) {
+      // ring 1: the right column's x face or the top row's y face, one solve along the lane's axis
+      const bool ax = xface;
+      const Flux<T> F = face_from_sides_rt<T, RS>(
+          P, lds_get_side<T, NT>(L, ax ? LY::SX : LY::SY, ax ? t - 1 : t - BX), ax ? sxl : syl, ax ? 0 : 1);
+      lds_put_flux<T, NT>(L, ax ? LY::FX : LY::FY, t, F);
+    }
     __syncthreads();
 
     // 5. this plane's x/y net inflow, and the update of plane c - 1
