@@ -42,9 +42,11 @@ template <typename T>
 constexpr int af_pair() { return sizeof(T) == 4 ? 2 : 1; }
 constexpr int AF_RING = 128;                  // ghost columns with a value: 64 of ring 1, 64 of ring 2
 
-// thread -> column (x, y) in block coordinates, [-2, 18)
+// thread -> column (x, y) in block coordinates, [-2, 18). A cell wave holds a 4-wide strip of the
+// 16 rows: with the LDS rows 20 columns apart its lanes reach distinct banks (4 rows of 16 would
+// meet in a third of them).
 __device__ __forceinline__ void af_column(int t, int& x, int& y) {
-  if (t < 256) { x = t & 15; y = t >> 4; return; }
+  if (t < 256) { x = 4 * (t >> 6) + (t & 3); y = (t & 63) >> 2; return; }
   const int ring = t < 320 ? 1 : 2;
   t = (t - 256) & 63;
   const int side = t >> 4, i = t & 15;
@@ -482,7 +484,7 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
         } else {
           // another block's cell (same level, refined, or this block's boundary image): its half
           // step was formed in the shell pass
-          const int h = oz ? 64 * AMR_BS + (c < 0 ? 0 : AMR_BS * AMR_BS) + tid : (tid - 256) * AMR_BS + c;
+          const int h = oz ? 64 * AMR_BS + (c < 0 ? 0 : AMR_BS * AMR_BS) + y * AMR_BS + x : (tid - 256) * AMR_BS + c;
           const T4<T> a = sh[2 * (SH_TEXELS + h)], b = sh[2 * (SH_TEXELS + h) + 1];
           hc.rho = a.x; hc.vx = a.y; hc.vy = a.z; hc.vz = a.w;
           hc.p = b.x; hc.s = b.y; hc.shock = b.z; hc.fo = b.w;

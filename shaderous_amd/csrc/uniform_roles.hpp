@@ -49,7 +49,18 @@ __device__ __forceinline__ void lds_put_prim6(T* L, int t, const Prim<T>& q) {
 template <int TX, int TY>
 __device__ __forceinline__ void role_column(int t, int& cx, int& cy) {
   using RT = RoleTile<TX, TY>;
-  if (t < RT::NCELL) { cx = t % TX; cy = t / TX; return; }
+  if (t < RT::NCELL) {
+    if constexpr (TX == 16) {
+      // a wave holds a 4-wide strip of 16 rows: with the LDS rows 20 columns apart its lanes
+      // then reach distinct banks (4 rows of 16 would meet in a third of them)
+      cx = 4 * (t >> 6) + (t & 3);
+      cy = (t & 63) >> 2;
+    } else {
+      cx = t % TX;
+      cy = t / TX;
+    }
+    return;
+  }
   t -= RT::NCELL;
   if (t < TX) { cx = t; cy = TY; return; }             // ring 1: y faces on the top row
   t -= TX;
