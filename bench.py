@@ -35,10 +35,11 @@ BYTES_PER_UPDATE = 64      # read 8 FP32 conserved values, write 8
 # WRITE_SIZE (scaled by the counters' reading of a 512 MiB device copy: 2.0 and 1.0), for the
 # configuration they were measured on: profiles/k_fused_turbulence_256.md
 TRAFFIC_MEASURED = {
-    ("k_fused_r", 256, (32, 8, 64)): (2 * 656817 + 524320) * 1024,
-    ("k_fused_r", 256, (32, 8, 128)): (2 * 639784 + 524304) * 1024,
-    ("k_fused_r", 256, (32, 16, 64)): (2 * 670860 + 524304) * 1024,
-    ("k_fused_r", 256, (32, 16, 128)): (2 * 664321 + 524296) * 1024,
+    ("k_fused_r", 256, (32, 8, 64)): (2 * 653295 + 524320) * 1024,
+    ("k_fused_r", 256, (32, 8, 128)): (2 * 635980 + 524304) * 1024,
+    ("k_fused_r", 256, (16, 16, 64)): (2 * 801034 + 524323) * 1024,
+    ("k_fused_r", 256, (16, 16, 128)): (2 * 780350 + 524308) * 1024,
+    ("k_fused_r", 256, (32, 16, 128)): (2 * 664318 + 524296) * 1024,
 }
 
 
@@ -46,7 +47,10 @@ TRAFFIC_MEASURED = {
 # SQ_INSTS_VALU for the configuration they were measured on (profiles/k_fused_turbulence_256.md)
 VALU_MEASURED = {
     ("k_fused_r", 256, (32, 8, 64)): 286394543,
-    ("k_fused_r", 256, (16, 16, 64)): 264811530,
+    ("k_fused_r", 256, (32, 8, 128)): 282960418,
+    ("k_fused_r", 256, (16, 16, 64)): 265036818,
+    ("k_fused_r", 256, (16, 16, 128)): 261889406,
+    ("k_fused_r", 256, (32, 16, 128)): 253433258,
 }
 # MI355X VALU issue peak for FP32 without packed math: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_LANE_PEAK = 256 * 4 * 16 * 2.4e9
