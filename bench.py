@@ -237,7 +237,7 @@ def main():
         log(f"[bench] copy probe: 3 x {args.copy_probe} MiB device copies")
 
     value = plain_value = None
-    kernel_ms = achieved = None
+    kernel_ms = achieved = kname = None
     tile = None
     diag = None
     t = tp = None
@@ -260,6 +260,7 @@ def main():
         value = ncell * args.steps / t
         prof = sim.profile(5)
         kernel_ms = prof["update"]
+        kname = sim.kernel_name()
         achieved = BYTES_PER_UPDATE * ncell / (kernel_ms * 1e-3)
         log(f"[bench] fused: {t / args.steps * 1e3:.3f} ms/step, kernel {kernel_ms:.3f} ms, {value:.3e} upd/s")
         diag = sim.diagnostics()
@@ -287,8 +288,6 @@ def main():
             print(json.dumps({"amr": amr, "ensemble": ens}))
         return
 
-    # raster order for the 28-wide tiles (k_fused), threads by role otherwise (k_fused_r)
-    kname = "k_fused" if tile and tile[0] == 28 else "k_fused_r"
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (RefHydro restatement) ...")

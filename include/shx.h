@@ -109,7 +109,7 @@ int shx_flux_mismatch(shx_sim* sim, int64_t* mismatched, int64_t* faces);
  * ms[0] rates, [1] clock, [2] predict, [3] update (plain path) or ms[3] the fused pass;
  * ms[4] whole step. */
 int shx_profile(shx_sim* sim, int steps, double ms[5]);
-/* The name of the step's dominant kernel ("k_fused" or "k_update"), for profiles. */
+/* The name of the step's dominant kernel ("k_fused_r", "k_fused" or "k_update"), for profiles. */
 int shx_kernel_name(shx_sim* sim, char* buf, size_t len);
 
 /* The autotuner: time the fused kernel's tile candidates on the current state (which it
@@ -117,8 +117,8 @@ int shx_kernel_name(shx_sim* sim, char* buf, size_t len);
 int shx_autotune(shx_sim* sim, int reps, int chosen[3]);
 
 /* ---------------- block AMR (AmrSim, src/amr/amr_sim.ts) ----------------
- * A fixed 2:1-balanced block tree (the reference builds and keeps it in BlockTree,
- * src/amr/tree.ts:106-824; re-meshing is not part of this path): blocks of 16^3 cells with 2
+ * A 2:1-balanced block tree (the reference builds and keeps it in BlockTree,
+ * src/amr/tree.ts:106-824; shx_amr_regrid below moves a sim onto a new one): blocks of 16^3 cells with 2
  * ghost layers (tree.ts:14-18), given as (level, ix, iy, iz) per slot, parents and leaves,
  * where block coordinates count blocks of that level from the domain's lower corner.
  * The shx_config describes the level-0 grid: n = 16 * root blocks per axis, dx = level-0 cell
@@ -128,8 +128,8 @@ typedef struct shx_amr shx_amr;
 
 /* Kernel path of the AMR step. FUSED (the batched form): a clock pass and one persistent kernel
  * over a device list of leaves that forms ghosts, half steps, faces and the update of each block
- * in one march, coarse-fine faces through a flux register (launched once for the leaves without a
- * refined neighbour, then once per level for the others, finest first). PASSES: the reference's
+ * in one march, coarse-fine faces through a flux register (one launch: workgroups take leaves
+ * from a device queue, a leaf beside a refined block after the finer leaves across it). PASSES: the reference's
  * passes (assemble -> clock -> predict -> update, amr_sim.ts:2862-2872) with W, PA and PB atlases
  * in HBM, each launched once over every level from device slot lists. LEVELS: those kernels
  * launched level by level (the plain per-level form). All three give the same results. */

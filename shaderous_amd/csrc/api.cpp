@@ -437,7 +437,9 @@ int shx_profile(shx_sim* s, int steps, double ms[5]) {
 
 int shx_kernel_name(shx_sim* s, char* buf, size_t len) {
   if (!s || !buf || len == 0) return fail(SHX_EINVAL, "bad argument");
-  snprintf(buf, len, "%s", s->fused ? "k_fused" : "k_update");
+  // threads by role: the FP32 tiles 32 x 16, 32 x 8 and 16 x 16
+  const bool roles = s->es == 4 && (s->tile == FUSED_R32x16 || s->tile == FUSED_R32x8 || s->tile == FUSED_20x20);
+  snprintf(buf, len, "%s", !s->fused ? "k_update" : (roles ? "k_fused_r" : "k_fused"));
   return SHX_OK;
 }
 
