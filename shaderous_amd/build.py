@@ -8,7 +8,9 @@ system's Node headers (skipped, with a message, where they are absent). No GPU i
 from __future__ import annotations
 
 import concurrent.futures as cf
+import json
 import os
+import re
 import subprocess
 import sys
 
@@ -38,16 +40,55 @@ def sources():
     return sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
+# per kernel, from hipcc's resource-usage remarks: registers, scratch, occupancy, LDS
+_RESOURCE_KEYS = {"VGPRs": "vgprs", "AGPRs": "agprs", "ScratchSize [bytes/lane]": "scratch",
+                  "Occupancy [waves/SIMD]": "occupancy", "VGPRs Spill": "vgpr_spill",
+                  "LDS Size [bytes/block]": "lds"}
+
+
+def _resources(stderr: str) -> list:
+    out, cur = [], None
+    for line in stderr.splitlines():
+        m = re.search(r"remark:\s+(.*?)\s+\[-Rpass-analysis=kernel-resource-usage\]", line)
+        if not m:
+            continue
+        text = m.group(1)
+        if text.startswith("Function Name: "):
+            cur = {"kernel": text[len("Function Name: "):]}
+            out.append(cur)
+        elif cur is not None and ": " in text:
+            key, val = text.rsplit(": ", 1)
+            if key in _RESOURCE_KEYS:
+                cur[_RESOURCE_KEYS[key]] = int(val)
+    return out
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, src + ".o")
+    res = os.path.join(BUILD, src + ".resources.json")
     deps = [os.path.join(CSRC, f) for f in os.listdir(CSRC)]
-    if os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps):
+    if (os.path.exists(obj) and os.path.exists(res)
+            and os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps)):
         return obj
-    cmd = [HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *DEVICE_FLAGS, "-Rpass-analysis=kernel-resource-usage", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr[-4000:]}")
+    with open(res, "w") as f:
+        json.dump(_resources(r.stderr), f)
     return obj
+
+
+def kernel_resources() -> list:
+    """Registers, scratch bytes per lane, occupancy and LDS of every kernel of the last build
+    (hipcc's kernel-resource-usage remarks, recorded per translation unit)."""
+    out = []
+    for src in sources():
+        path = os.path.join(BUILD, src + ".resources.json")
+        if os.path.exists(path):
+            with open(path) as f:
+                out.extend(json.load(f))
+    return out
 
 
 def build(force: bool = False, jobs: int | None = None) -> str:
