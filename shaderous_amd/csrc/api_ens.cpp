@@ -144,6 +144,12 @@ int shx_ens_create(const shx_config* cfg, int count, const int* sizes, shx_ens**
   e->count = count;
   e->ops = ops;
   e->tile = e->es == 8 ? ENS_20x20 : ENS_32x16;
+  // a tile asked for by its cells (tile[0] x tile[1]); FP64 has the one
+  for (int k = 0; c.tile[0] > 0 && e->es == 4 && k < ENS_NTILES; k++) {
+    int tx, ty;
+    ens_tile_cells(k, tx, ty);
+    if (tx == c.tile[0] && ty == c.tile[1]) e->tile = k;
+  }
   for (int i = 0; i < count; i++) {
     e->nx.push_back(sizes[2 * i]);
     e->ny.push_back(sizes[2 * i + 1]);

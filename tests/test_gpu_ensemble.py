@@ -38,10 +38,15 @@ def l1(a, b):
             "E": float(np.abs(a2[:, 0] - b2[:, 0]).sum() / np.abs(b2[:, 0]).sum())}
 
 
-@pytest.mark.parametrize("lim,isen", [("mc", False), ("koren", True)])
-def test_ensemble_equals_standalone_runs(lim, isen):
+# the ensemble kernel's tiles, by cells (threads: 32 x 16, 32 x 32, 20 x 20)
+ENS_TILES = [(28, 12), (28, 28), (16, 16)]
+
+
+@pytest.mark.parametrize("lim,isen,tile", [("mc", False, t) for t in ENS_TILES] + [("koren", True, t) for t in ENS_TILES])
+def test_ensemble_equals_standalone_runs(lim, isen, tile):
     sizes = [(64, 64), (100, 37), (128, 128), (30, 90), (200, 64)]
     cfg = shared_cfg(limiter=lim, isentropic=isen)
+    cfg.tile = (tile[0], tile[1], 0)
     ens = Ensemble(cfg, sizes)
     pbs = []
     try:
