@@ -130,15 +130,23 @@ FUSED_TILES = [(16, 16), (28, 12), (28, 28), (32, 16), (32, 8)]
 @pytest.mark.parametrize("tile", FUSED_TILES)
 def test_every_fused_tile_equals_plain(tile):
     """Each column tile of the fused kernel gives the plain path's bits: a periodic 3D grid, a
-    ragged 3D grid (partial tiles on every axis, partial z chunks) and outflow / reflecting 2D
-    grids (mapped ghost columns); and every face bit-identical from both sides."""
+    ragged 3D grid (partial tiles on every axis, partial z chunks; with MC and with the reference's
+    default Koren + isentropic reconstruction) and outflow / reflecting 2D grids (mapped ghost
+    columns); and every face bit-identical from both sides."""
     x, y, z = problems.centers((45, 21, 13), 1.0)
     ragged = problems.Problem(id="ragged", n=(45, 21, 13), box=1.0, gamma=5.0 / 3.0, limiter="mc", isentropic=False,
                               cfl=0.4, prims={"rho": 1.0 + 0.5 * np.exp(-(x * x + y * y + z * z) / 0.02),
                                               "vx": 0.3 * np.sin(2 * np.pi * y), "vy": 0.2 * np.cos(2 * np.pi * z),
                                               "vz": -0.25 * np.sin(2 * np.pi * x),
                                               "p": 1.0 + 4.0 * np.exp(-(x * x + y * y) / 0.005)})
-    cases = [blob3d(32), ragged]
+    # the reference's default reconstruction (Koren, isentropic) on the ragged grid too
+    x2, y2, z2 = problems.centers((37, 19, 11), 1.0)
+    koren = problems.Problem(id="koren-isen", n=(37, 19, 11), box=1.0, gamma=1.4, limiter="koren", isentropic=True,
+                             cfl=0.4, prims={"rho": 1.0 + 0.5 * np.exp(-(x2 * x2 + y2 * y2 + z2 * z2) / 0.02),
+                                             "vx": 0.3 * np.sin(2 * np.pi * y2), "vy": 0.2 * np.cos(2 * np.pi * z2),
+                                             "vz": -0.25 * np.sin(2 * np.pi * x2),
+                                             "p": 1.0 + 4.0 * np.exp(-(x2 * x2 + y2 * y2) / 0.005)})
+    cases = [blob3d(32), ragged, koren]
     for bc in (1, 2):
         pb = problems.riemann_quadrants(50, seed=5)
         pb.bc_lo = (bc, bc, 0)
