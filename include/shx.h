@@ -194,7 +194,9 @@ int shx_amr_regrid(shx_amr* sim, const int* blocks, int nblocks);
  * step over a device list of column tiles from every instance. Each instance is the run a Sim
  * with the same configuration would make (the plain path's arithmetic); cfg gives what they
  * share (limiter, solver, gamma, cfl, boundaries, floors, precision); sizes[2i], sizes[2i+1] are
- * instance i's nx, ny on the unit box (dx = 1 / nx). */
+ * instance i's nx, ny on the unit box (dx = 1 / nx). Boundaries: each of x and y periodic on both
+ * faces, or outflow / reflecting (outflow-only faces and an axis periodic on one face only are
+ * refused with SHX_EINVAL: a Sim runs those on its plain path). */
 typedef struct shx_ens shx_ens;
 int shx_ens_create(const shx_config* cfg, int count, const int* sizes, shx_ens** out);
 int shx_ens_destroy(shx_ens* ens);

@@ -128,6 +128,16 @@ int shx_ens_create(const shx_config* cfg, int count, const int* sizes, shx_ens**
   if (c.precision != 32 && c.precision != 64) return fail(SHX_EINVAL, "precision %d", c.precision);
   for (int a = 0; a < 2; a++)
     if (c.bc_lo[a] < 0 || c.bc_lo[a] > 3 || c.bc_hi[a] < 0 || c.bc_hi[a] > 3) return fail(SHX_EINVAL, "boundary code on axis %d", a);
+  // the tile kernel forms a ghost column's half step from the cell it maps to (outflow,
+  // reflecting) or from its own stencil (periodic on both faces); outflow-only clamps and an axis
+  // periodic on one face need the cell's own half step as the plain path loads it (load_pred),
+  // which a standalone Sim then runs on its plain path
+  for (int a = 0; a < 2; a++) {
+    if (c.bc_lo[a] == SHX_BC_OUTFLOW_ONLY || c.bc_hi[a] == SHX_BC_OUTFLOW_ONLY)
+      return fail(SHX_EINVAL, "ensemble boundaries: outflow-only on axis %d is not supported (periodic, outflow or reflecting)", a);
+    if ((c.bc_lo[a] == SHX_BC_PERIODIC) != (c.bc_hi[a] == SHX_BC_PERIODIC))
+      return fail(SHX_EINVAL, "ensemble boundaries: axis %d is periodic on one face only", a);
+  }
   if (!(c.gamma > 1) || !(c.cfl > 0)) return fail(SHX_EINVAL, "gamma > 1 and cfl > 0 required");
   for (int i = 0; i < count; i++)
     if (sizes[2 * i] < 4 || sizes[2 * i + 1] < 4) return fail(SHX_EINVAL, "instance %d: %d x %d (at least 4 x 4)", i, sizes[2 * i], sizes[2 * i + 1]);

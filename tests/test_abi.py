@@ -37,3 +37,21 @@ def test_library_loads_and_reports_errors_without_a_gpu_call():
     h = ctypes.c_void_p()
     assert lib.shx_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.SHX_EINVAL
     assert b"n[0]" in lib.shx_last_error()
+
+
+def test_ensemble_refuses_boundaries_its_kernel_cannot_match():
+    """shx_ens_create refuses outflow-only faces and an axis periodic on one face only (before any
+    device call): the ensemble's tile kernel matches the plain path only on periodic, outflow and
+    reflecting boundaries."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libshx.so not built")
+    lib = _lib.load()
+    sizes = (ctypes.c_int * 2)(16, 16)
+    for lo, hi, what in [((3, 1, 0), (1, 1, 0), b"outflow-only"), ((1, 0, 0), (1, 1, 0), b"one face only")]:
+        cfg = _lib.ShxConfig()
+        lib.shx_config_default(ctypes.byref(cfg), 16, 16, 1)
+        cfg.bc_lo[:] = list(lo)
+        cfg.bc_hi[:] = list(hi)
+        h = ctypes.c_void_p()
+        assert lib.shx_ens_create(ctypes.byref(cfg), 1, sizes, ctypes.byref(h)) == _lib.SHX_EINVAL
+        assert what in lib.shx_last_error()
