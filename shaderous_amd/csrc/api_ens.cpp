@@ -21,8 +21,8 @@ struct shx_ens {
   size_t ncell = 0;
   hipStream_t stream = nullptr;
   void* inst = nullptr;        // EnsInst<T>[count]
-  void* tiles[ENS_NTILES] = {nullptr, nullptr, nullptr};   // EnsTile lists per tile shape
-  int ntiles[ENS_NTILES] = {0, 0, 0};
+  void* tiles[ENS_NTILES] = {};   // EnsTile lists per work shape
+  int ntiles[ENS_NTILES] = {};
   void* U1[2] = {nullptr, nullptr};
   void* U2[2] = {nullptr, nullptr};
   void* clk = nullptr;         // 16 T per instance
@@ -188,7 +188,7 @@ int shx_ens_create(const shx_config* cfg, int count, const int* sizes, shx_ens**
     ens_tile_cells(k, tx, ty);
     for (int i : order)
       for (int y0 = 0; y0 < e->ny[i]; y0 += ty)
-        for (int x0 = 0; x0 < e->nx[i]; x0 += tx) lists[k].push_back(EnsTile{i, x0, y0, 0});
+        for (int x0 = 0; x0 < e->nx[i]; x0 += tx) lists[k].push_back(EnsTile{i, x0, y0, std::min(y0 + ty, e->ny[i])});
     e->ntiles[k] = (int)lists[k].size();
     ok = ok && alloc(&e->tiles[k], lists[k].size() * sizeof(EnsTile));
   }
@@ -315,7 +315,7 @@ int shx_ens_autotune(shx_ens* e, int reps, int chosen[2]) {
   HIPCHK(hipEventCreate(&e1));
   double best = 1e30;
   int bt = e->tile;
-  const int cands[] = {ENS_32x32, ENS_32x16, ENS_20x20};
+  const int cands[] = {ENS_32x32, ENS_32x16, ENS_20x20, ENS_ROWS16, ENS_ROWS32};
   for (int k : cands) {
     if (e->es == 8 && k != ENS_20x20) continue;
     const int p = e->cur;

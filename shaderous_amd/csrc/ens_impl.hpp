@@ -17,7 +17,7 @@ namespace shx {
 
 // One tile of the work list: instance, and the tile's lower corner in cells
 struct EnsTile {
-  int inst, x0, y0, pad;
+  int inst, x0, y0, y1;   // y1: a strip's end row (k_ens_rows; the tiles ignore it)
 };
 
 // Per instance: its parameters, and where its cells start in the concatenated state
@@ -180,15 +180,197 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 2 : 1)) void k_ens_fused(
   }
 }
 
+// ---------------- the row march (FP32) ----------------
+// One wave per strip of 60 cells x a run of rows: lane l owns column x0 - 2 + l (lanes 2-61 the
+// cells, 0-1 and 62-63 the x halo) and marches up the rows; a column's x neighbours come from the
+// lanes beside it (DPP wave shifts: no LDS, no barrier), its y neighbours from the registers of
+// the rows below and above. The arithmetic is k_ens_fused's (the plain 2D passes' order), so the
+// bits are the same. Ghosts beyond an outflow or reflecting face take the half step of the cell
+// they map to: an x ghost from that cell's lane (a lane permutation), a ghost row below from the
+// first row's half step one row later, a ghost row above from the last row's (outflow-only faces
+// and axes periodic on one face only are refused when the ensemble is made).
+constexpr int ENS_STRIP = 60;
+
+// lane l receives lane l - 1's value (shr) or lane l + 1's (shl); the end lanes get 0
+__device__ __forceinline__ float dpp_shr1(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float dpp_shl1(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_get(float v, int src) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src << 2, __builtin_bit_cast(int, v)));
+}
+template <bool LEFT>
+__device__ __forceinline__ Prim<float> prim_beside(const Prim<float>& q) {
+  auto f = [](float v) { return LEFT ? dpp_shr1(v) : dpp_shl1(v); };
+  Prim<float> o;
+  o.rho = f(q.rho); o.vx = f(q.vx); o.vy = f(q.vy); o.vz = f(q.vz); o.p = f(q.p); o.s = f(q.s);
+  o.cs = 0.0f;   // as lds_prim: a neighbour's sound speed is not used
+  return o;
+}
+__device__ __forceinline__ SideData<float> side_from_left(const SideData<float>& d) {
+  SideData<float> o;
+  o.rho = dpp_shr1(d.rho); o.vx = dpp_shr1(d.vx); o.vy = dpp_shr1(d.vy); o.vz = dpp_shr1(d.vz);
+  o.p = dpp_shr1(d.p); o.s = dpp_shr1(d.s); o.hrho = dpp_shr1(d.hrho); o.hp = dpp_shr1(d.hp);
+  o.shock = dpp_shr1(d.shock);
+  return o;
+}
+__device__ __forceinline__ Flux<float> flux_from_right(const Flux<float>& F) {
+  Flux<float> o;
+  o.m = dpp_shl1(F.m); o.mx = dpp_shl1(F.mx); o.my = dpp_shl1(F.my); o.mz = dpp_shl1(F.mz);
+  o.e = dpp_shl1(F.e); o.ei = dpp_shl1(F.ei); o.s = dpp_shl1(F.s); o.vn = dpp_shl1(F.vn);
+  return o;
+}
+__device__ __forceinline__ Half<float> half_of_lane(const Half<float>& h, int src) {
+  Half<float> o;
+  o.rho = lane_get(h.rho, src); o.vx = lane_get(h.vx, src); o.vy = lane_get(h.vy, src); o.vz = lane_get(h.vz, src);
+  o.p = lane_get(h.p, src); o.s = lane_get(h.s, src); o.shock = lane_get(h.shock, src); o.fo = lane_get(h.fo, src);
+  return o;
+}
+
+// Work item: EnsTile{instance, x0, y0, y1} (rows [y0, y1)). Every lane runs every instruction
+// (the DPP and permute reads need all 64 lanes); lane masks only guard the stores.
+template <int LIM, int RS, bool ISEN>
+__global__ __launch_bounds__(64) void k_ens_rows(const EnsInst<float>* __restrict__ inst,
+                                                 const EnsTile* __restrict__ strips, int nstrips,
+                                                 const float4* __restrict__ U1, const float4* __restrict__ U2,
+                                                 float4* __restrict__ O1, float4* __restrict__ O2,
+                                                 const float* __restrict__ clk, unsigned long long* slot) {
+  using T = float;
+  const EnsTile w = strips[blockIdx.x];
+  const Params<T> P = inst[w.inst].P;
+  const long long off = inst[w.inst].off;
+  const int nx = P.n[0], ny = P.n[1];
+  const T dt = clk[16 * w.inst];
+  const T dtdx = dt / P.dx;
+  const int lane = threadIdx.x;
+  const int x = w.x0 - 2 + lane;
+  const int y0 = w.y0, y1 = w.y1;
+  const AxisMap mx = map_axis(x, nx, P.bc_lo[0], P.bc_hi[0]);
+  const bool real_x = x >= 0 && x < nx;
+  // an x ghost beyond an outflow / reflecting face: the half step of the cell it maps to, whose
+  // lane is in this strip (that cell lies within two columns of the face)
+  const bool mapped_x = !real_x && ((x < 0 && P.bc_lo[0] != BC_PERIODIC) || (x >= nx && P.bc_hi[0] != BC_PERIODIC));
+  const int src = mapped_x ? mx.i - (w.x0 - 2) : lane;
+  const bool cell_lane = lane >= 2 && lane < 2 + ENS_STRIP && real_x;
+  // ghost rows beyond an outflow / reflecting face: below the first row, above the last
+  const bool ymap_lo = y0 == 0 && P.bc_lo[1] != BC_PERIODIC;
+  const bool ymap_hi = y1 == ny && P.bc_hi[1] != BC_PERIODIC;
+
+  auto row = [&](int r) -> Prim<T> {
+    const AxisMap my = map_axis(r, ny, P.bc_lo[1], P.bc_hi[1]);
+    const size_t o = off + (size_t)my.i * nx + mx.i;
+    Prim<T> q = prim_from_cons(P, U1[o], U2[o]);
+    q.vx = bc_vel(q.vx, mx.flip, mx.side);
+    q.vy = bc_vel(q.vy, my.flip, my.side);
+    return q;
+  };
+
+  Prim<T> pm = row(y0 - 2), pc = row(y0 - 1);
+  SideData<T> syu_prev;
+  Inc<T> iyu_ghost;                 // the ghost row below's own y increment
+  Half<T> hc_prev;                  // the row below's half step (for a ghost row above)
+  Flux<T> fy_lo = zero_flux<T>(), axy_prev = zero_flux<T>();
+  T hp_prev = T(0);
+  T rate_max = T(0);
+
+  for (int c = y0 - 1; c <= y1; c++) {
+    const Prim<T> pp = row(c + 1);
+    const Prim<T> xm = prim_beside<true>(pc), xp = prim_beside<false>(pc);
+    Inc<T> ixu, ixd, iyu, iyd;
+    AxisSlope<T> sl[3];
+    sl[0] = axis_slope<T, LIM, ISEN>(P, xm, pc, xp, &ixu, &ixd);
+    sl[1] = axis_slope<T, LIM, ISEN>(P, pm, pc, pp, &iyu, &iyd);
+    sl[2] = axis_slope<T, LIM, ISEN>(P, pc, pc, pc, nullptr, nullptr);
+    const T conv = (xp.vx - xm.vx) + (pp.vy - pm.vy) + (pc.vz - pc.vz);
+    const T jump = tmax(jump_of(xm.p, xp.p), tmax(jump_of(pm.p, pp.p), jump_of(pc.p, pc.p)));
+    Half<T> hc = hancock(P, pc, sl, conv, jump, dt);
+    // a ghost row above: the last row's half step
+    const bool ghost_hi = ymap_hi && c == ny;
+    if (ghost_hi) {
+      const AxisMap my = map_axis(c, ny, P.bc_lo[1], P.bc_hi[1]);
+      hc = hc_prev;
+      hc.vy = bc_vel(hc.vy, my.flip, my.side);
+    }
+    // an x ghost: its cell's half step (all lanes take part in the permute)
+    {
+      const Half<T> hm = half_of_lane(hc, src);
+      if (mapped_x) {
+        hc = hm;
+        hc.vx = bc_vel(hc.vx, mx.flip, mx.side);
+      }
+    }
+    const SideData<T> sxl = side_of(hc, ixd, T(-1));
+    const SideData<T> syl = side_of(hc, iyd, T(-1));
+    const SideData<T> syu = side_of(hc, iyu, T(1));
+    // the x face on this lane's left (every lane, for the shifts)
+    const Flux<T> Fx = face_from_sides<T, RS>(P, side_from_left(side_of(hc, ixu, T(1))), sxl, 0);
+    const Flux<T> xh = flux_from_right(Fx);
+    Flux<T> axy;
+    axy.m = Fx.m - xh.m; axy.mx = Fx.mx - xh.mx; axy.my = Fx.my - xh.my; axy.mz = Fx.mz - xh.mz;
+    axy.e = Fx.e - xh.e; axy.ei = Fx.ei - xh.ei; axy.s = Fx.s - xh.s; axy.vn = Fx.vn - xh.vn;
+    if (c == y0 - 1 && ymap_lo) {
+      // the ghost row below the first: its half step is the first row's, formed next
+      iyu_ghost = iyu;
+    } else {
+      Flux<T> fy = zero_flux<T>();
+      if (c >= y0) {
+        SideData<T> lo = syu_prev;
+        if (c == 0 && ymap_lo) {
+          const AxisMap my = map_axis(-1, ny, P.bc_lo[1], P.bc_hi[1]);
+          Half<T> hg = hc;
+          hg.vy = bc_vel(hg.vy, my.flip, my.side);
+          lo = side_of(hg, iyu_ghost, T(1));
+        }
+        fy = face_from_sides<T, RS>(P, lo, syl, 1);
+      }
+      // the update of row c - 1 (net_of's order: x, then y, then the flat z)
+      if (c - 1 >= y0 && c - 1 < y1 && cell_lane) {
+        Flux<T> net;
+        net.m = (axy_prev.m + (fy_lo.m - fy.m)) + (T(0) - T(0));
+        net.mx = (axy_prev.mx + (fy_lo.mx - fy.mx)) + (T(0) - T(0));
+        net.my = (axy_prev.my + (fy_lo.my - fy.my)) + (T(0) - T(0));
+        net.mz = (axy_prev.mz + (fy_lo.mz - fy.mz)) + (T(0) - T(0));
+        net.e = (axy_prev.e + (fy_lo.e - fy.e)) + (T(0) - T(0));
+        net.ei = (axy_prev.ei + (fy_lo.ei - fy.ei)) + (T(0) - T(0));
+        net.s = (axy_prev.s + (fy_lo.s - fy.s)) + (T(0) - T(0));
+        net.vn = (axy_prev.vn + (fy_lo.vn - fy.vn)) + (T(0) - T(0));
+        const size_t o = off + (size_t)(c - 1) * nx + x;
+        T4<T> o1, o2;
+        advance(P, U1[o], U2[o], net, hp_prev, dtdx, o1, o2);
+        O1[o] = o1;
+        O2[o] = o2;
+        rate_max = tmax(rate_max, rate_of(P, prim_from_cons(P, o1, o2)));
+      }
+      fy_lo = fy;
+    }
+    syu_prev = syu;
+    axy_prev = axy;
+    hp_prev = hc.p;
+    hc_prev = hc;
+    pm = pc;
+    pc = pp;
+  }
+  const T v = wave_max(rate_max);
+  if (lane == 0) atomicMax(slot + w.inst, (unsigned long long)order_bits(v));
+}
+
 }  // namespace shx
 
 namespace shx {
 
-// Column tiles of the ensemble kernel (threads BX x BY, cells (BX - 4) x (BY - 4))
-enum { ENS_20x20 = 0, ENS_32x16 = 1, ENS_32x32 = 2, ENS_NTILES = 3 };
+// Work shapes of the ensemble: k_ens_fused's tiles (threads BX x BY, cells (BX - 4) x (BY - 4))
+// and k_ens_rows' strips (60 cells x 16 or 32 rows, FP32)
+enum { ENS_20x20 = 0, ENS_32x16 = 1, ENS_32x32 = 2, ENS_ROWS16 = 3, ENS_ROWS32 = 4, ENS_NTILES = 5 };
 inline void ens_tile_cells(int tile, int& tx, int& ty) {
-  tx = tile == ENS_20x20 ? 16 : 28;
-  ty = tile == ENS_20x20 ? 16 : (tile == ENS_32x16 ? 12 : 28);
+  switch (tile) {
+    case ENS_20x20: tx = 16; ty = 16; break;
+    case ENS_32x16: tx = 28; ty = 12; break;
+    case ENS_32x32: tx = 28; ty = 28; break;
+    case ENS_ROWS16: tx = ENS_STRIP; ty = 16; break;
+    default: tx = ENS_STRIP; ty = 32; break;
+  }
 }
 
 struct EnsOps {
@@ -229,6 +411,13 @@ struct EnsImpl {
       switch (tile) {
         case ENS_32x32: fused_tile<32, 32>(st, grid, inst, tiles, ntiles, u1, u2, o1, o2, clk, slot); break;
         case ENS_32x16: fused_tile<32, 16>(st, grid, inst, tiles, ntiles, u1, u2, o1, o2, clk, slot); break;
+        case ENS_ROWS16:
+        case ENS_ROWS32:
+          // one wave per strip, every strip its own workgroup
+          hipLaunchKernelGGL((k_ens_rows<LIM, RS, ISEN>), dim3(ntiles), dim3(64), 0, st,
+                             (const EnsInst<float>*)inst, (const EnsTile*)tiles, ntiles, (const float4*)u1,
+                             (const float4*)u2, (float4*)o1, (float4*)o2, (const float*)clk, slot);
+          break;
         default: fused_tile<20, 20>(st, grid, inst, tiles, ntiles, u1, u2, o1, o2, clk, slot); break;
       }
     }

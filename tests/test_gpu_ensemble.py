@@ -38,8 +38,9 @@ def l1(a, b):
             "E": float(np.abs(a2[:, 0] - b2[:, 0]).sum() / np.abs(b2[:, 0]).sum())}
 
 
-# the ensemble kernel's tiles, by cells (threads: 32 x 16, 32 x 32, 20 x 20)
-ENS_TILES = [(28, 12), (28, 28), (16, 16)]
+# the ensemble's work shapes, by cells: k_ens_fused's tiles (threads 32 x 16, 32 x 32, 20 x 20)
+# and k_ens_rows' strips (60 cells x 16 or 32 rows)
+ENS_TILES = [(28, 12), (28, 28), (16, 16), (60, 16), (60, 32)]
 
 
 @pytest.mark.parametrize("lim,isen,tile", [("mc", False, t) for t in ENS_TILES] + [("koren", True, t) for t in ENS_TILES])
