@@ -28,7 +28,7 @@ struct shx_ens {
   void* clk = nullptr;         // 16 T per instance
   unsigned long long* slot = nullptr;
   int cur = 0;
-  int tile = ENS_32x16;
+  int tile = ENS_ROWS16;
   int grid = 512;
   bool rates_stale = true;
   EnsOps ops{};
@@ -153,7 +153,7 @@ int shx_ens_create(const shx_config* cfg, int count, const int* sizes, shx_ens**
   e->es = c.precision == 64 ? 8 : 4;
   e->count = count;
   e->ops = ops;
-  e->tile = e->es == 8 ? ENS_20x20 : ENS_32x16;
+  e->tile = e->es == 8 ? ENS_20x20 : ENS_ROWS16;
   // a tile asked for by its cells (tile[0] x tile[1]); FP64 has the one
   for (int k = 0; c.tile[0] > 0 && e->es == 4 && k < ENS_NTILES; k++) {
     int tx, ty;
@@ -315,7 +315,7 @@ int shx_ens_autotune(shx_ens* e, int reps, int chosen[2]) {
   HIPCHK(hipEventCreate(&e1));
   double best = 1e30;
   int bt = e->tile;
-  const int cands[] = {ENS_32x32, ENS_32x16, ENS_20x20, ENS_ROWS16, ENS_ROWS32};
+  const int cands[] = {ENS_32x32, ENS_32x16, ENS_20x20, ENS_ROWS8, ENS_ROWS16, ENS_ROWS32};
   for (int k : cands) {
     if (e->es == 8 && k != ENS_20x20) continue;
     const int p = e->cur;

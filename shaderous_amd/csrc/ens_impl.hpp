@@ -361,14 +361,15 @@ __global__ __launch_bounds__(64) void k_ens_rows(const EnsInst<float>* __restric
 namespace shx {
 
 // Work shapes of the ensemble: k_ens_fused's tiles (threads BX x BY, cells (BX - 4) x (BY - 4))
-// and k_ens_rows' strips (60 cells x 16 or 32 rows, FP32)
-enum { ENS_20x20 = 0, ENS_32x16 = 1, ENS_32x32 = 2, ENS_ROWS16 = 3, ENS_ROWS32 = 4, ENS_NTILES = 5 };
+// and k_ens_rows' strips (60 cells x 8, 16 or 32 rows, FP32)
+enum { ENS_20x20 = 0, ENS_32x16 = 1, ENS_32x32 = 2, ENS_ROWS16 = 3, ENS_ROWS32 = 4, ENS_ROWS8 = 5, ENS_NTILES = 6 };
 inline void ens_tile_cells(int tile, int& tx, int& ty) {
   switch (tile) {
     case ENS_20x20: tx = 16; ty = 16; break;
     case ENS_32x16: tx = 28; ty = 12; break;
     case ENS_32x32: tx = 28; ty = 28; break;
     case ENS_ROWS16: tx = ENS_STRIP; ty = 16; break;
+    case ENS_ROWS8: tx = ENS_STRIP; ty = 8; break;
     default: tx = ENS_STRIP; ty = 32; break;
   }
 }
@@ -411,6 +412,7 @@ struct EnsImpl {
       switch (tile) {
         case ENS_32x32: fused_tile<32, 32>(st, grid, inst, tiles, ntiles, u1, u2, o1, o2, clk, slot); break;
         case ENS_32x16: fused_tile<32, 16>(st, grid, inst, tiles, ntiles, u1, u2, o1, o2, clk, slot); break;
+        case ENS_ROWS8:
         case ENS_ROWS16:
         case ENS_ROWS32:
           // one wave per strip, every strip its own workgroup

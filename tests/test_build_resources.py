@@ -14,6 +14,7 @@ NO_SCRATCH = {
     "k_update": "_ZN3shx8k_updateIfLi1ELi0ELb0E",
     "k_zplanes": "_ZN3shx9k_zplanesIfLi1ELb0E",
     "k_ens_fused": "_ZN3shx11k_ens_fusedIfLi1ELi0ELb0E",
+    "k_ens_rows": "_ZN3shx10k_ens_rowsILi1ELi0ELb0E",
     "k_amr_assemble": "_ZN3shx14k_amr_assembleIfLi1E",
     "k_amr_predict": "_ZN3shx13k_amr_predictIfLi1ELb0E",
     "k_amr_update": "_ZN3shx12k_amr_updateIfLi1ELi0ELb0E",

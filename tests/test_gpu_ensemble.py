@@ -39,8 +39,8 @@ def l1(a, b):
 
 
 # the ensemble's work shapes, by cells: k_ens_fused's tiles (threads 32 x 16, 32 x 32, 20 x 20)
-# and k_ens_rows' strips (60 cells x 16 or 32 rows)
-ENS_TILES = [(28, 12), (28, 28), (16, 16), (60, 16), (60, 32)]
+# and k_ens_rows' strips (60 cells x 8, 16 or 32 rows)
+ENS_TILES = [(28, 12), (28, 28), (16, 16), (60, 8), (60, 16), (60, 32)]
 
 
 @pytest.mark.parametrize("lim,isen,tile", [("mc", False, t) for t in ENS_TILES] + [("koren", True, t) for t in ENS_TILES])
