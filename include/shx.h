@@ -39,7 +39,9 @@ enum { SHX_RS_HLLC = 0, SHX_RS_HLL = 1 };
 /* Boundary codes (src/hydro/boundary.wgsl:18-23): periodic, outflow, reflecting, outflow only. */
 enum { SHX_BC_PERIODIC = 0, SHX_BC_OUTFLOW = 1, SHX_BC_REFLECT = 2, SHX_BC_OUTFLOW_ONLY = 3 };
 /* Uniform-grid kernel path: PLAIN keeps the reference's passes (rates, clock, predict, tiled
- * update); FUSED is one pass per step (rates of the next step in the epilogue). */
+ * update); FUSED is one pass per step (rates of the next step in the epilogue). A flat FP32 grid
+ * on the FUSED path runs the 2D kernels of the ensembles (as a batch of one grid): the row march
+ * over 60-cell strips or the LDS tile kernel. */
 enum { SHX_PATH_FUSED = 0, SHX_PATH_PLAIN = 1 };
 
 /* The SimConfig fields the hydro path reads (src/sim.ts:48-264, defaults configFor :645-736). */
@@ -63,7 +65,9 @@ typedef struct {
   double shock_thresh;
   double t_end;        /* 1e30 (or +inf) for none */
   double fluff[3];     /* SimConfig.fluff: (rho below which the limits apply, max e, max v); rho 0 = off */
-  int tile[3];         /* fused kernel's column tile (x, y) and z chunk; 0 = autotuned default */
+  int tile[3];         /* fused kernel's column tile (x, y) and z chunk; 0 = autotuned default.
+                        * A flat FP32 grid takes the 2D tiles (16,16), (28,12), (28,28) and the
+                        * strips (60,8), (60,16) (the default), (60,32) */
 } shx_config;
 
 typedef struct shx_sim shx_sim;
@@ -109,7 +113,8 @@ int shx_flux_mismatch(shx_sim* sim, int64_t* mismatched, int64_t* faces);
  * ms[0] rates, [1] clock, [2] predict, [3] update (plain path) or ms[3] the fused pass;
  * ms[4] whole step. */
 int shx_profile(shx_sim* sim, int steps, double ms[5]);
-/* The name of the step's dominant kernel ("k_fused_r", "k_fused" or "k_update"), for profiles. */
+/* The name of the step's dominant kernel ("k_fused_r", "k_fused", "k_ens_rows", "k_ens_fused" or
+ * "k_update"), for profiles. */
 int shx_kernel_name(shx_sim* sim, char* buf, size_t len);
 
 /* The autotuner: time the fused kernel's tile candidates on the current state (which it

@@ -7,11 +7,13 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
 #include "../../include/shx.h"
 #include "common.hpp"
+#include "ens_impl.hpp"
 #include "uniform.hpp"
 
 using namespace shx;
@@ -52,6 +54,16 @@ struct shx_sim {
   Params<double> pd;
   UniformOps ops{};
   hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [parity][1 or 2 steps]
+  // A flat FP32 grid on the fused path steps through the 2D kernels as a batch of one grid: the
+  // row march (k_ens_rows, one wave per 60-cell strip) or the LDS tile kernel (k_ens_fused), by
+  // tile. The 3D march (ops.fused) remains its flux-debug form.
+  bool flat2d = false;
+  EnsOps eops{};
+  void* einst = nullptr;                  // one EnsInst<float>: this grid's parameters, offset 0
+  void* etiles[ENS_NTILES] = {};          // the grid's tiles (strips) for every 2D tile shape
+  int entiles[ENS_NTILES] = {};
+  int etile = ENS_ROWS16;
+  int egrid = 1024;                       // persistent grid of the tile kernel
 };
 
 namespace {
@@ -64,6 +76,18 @@ Launch launch_of(shx_sim* s) {
   return L;
 }
 
+// The fused path's one-pass update from parity p into 1 - p (after the clock and the boundary
+// planes' half steps): the 2D kernels on a flat FP32 grid, else the 3D march with tile `tile`.
+void enqueue_fused(shx_sim* s, int p, int tile, int zc, bool debug) {
+  if (s->flat2d && !debug) {
+    s->eops.fused(s->stream, tile, s->egrid, s->einst, s->etiles[tile], s->entiles[tile], s->U1[p], s->U2[p],
+                  s->U1[1 - p], s->U2[1 - p], s->clk, s->slot);
+    return;
+  }
+  s->ops.fused(launch_of(s), tile, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk,
+               s->zbc ? s->PA : nullptr, s->zbc ? s->PB : nullptr, zc, s->slot, debug);
+}
+
 // One step from parity p into 1 - p.
 void enqueue_step(shx_sim* s, int p, bool debug = false) {
   Launch L = launch_of(s);
@@ -71,8 +95,7 @@ void enqueue_step(shx_sim* s, int p, bool debug = false) {
   if (s->fused) {
     o.clock(L, s->clk, s->slot);
     if (s->zbc) o.zplanes(L, s->U1[p], s->U2[p], s->PA, s->PB, s->clk);
-    o.fused(L, s->tile, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, s->zbc ? s->PA : nullptr,
-            s->zbc ? s->PB : nullptr, s->zc, s->slot, debug);
+    enqueue_fused(s, p, s->flat2d && !debug ? s->etile : s->tile, s->zc, debug);
   } else {
     o.rates(L, s->U1[p], s->U2[p], s->slot);
     o.clock(L, s->clk, s->slot);
@@ -235,6 +258,16 @@ int shx_create(const shx_config* cfg, shx_sim** out) {
   }
   if (c.tile[2] > 0) s->zc = c.tile[2];
   if (s->zc > c.n[2]) s->zc = c.n[2];
+  if (s->fused && s->es == 4 && c.n[2] == 1) {
+    s->eops = ens_ops_f32(c.limiter, c.riemann, c.isentropic ? 1 : 0);
+    s->flat2d = s->eops.fused != nullptr;
+  }
+  if (s->flat2d && c.tile[0] > 0)
+    for (int k = 0; k < ENS_NTILES; k++) {
+      int tx, ty;
+      ens_tile_cells(k, tx, ty);
+      if (tx == c.tile[0] && ty == c.tile[1]) s->etile = k;
+    }
 
   size_t bytes = s->ncell * 4 * s->es;
   auto alloc = [&](void** p, size_t b) { return hipMalloc(p, b) == hipSuccess; };
@@ -248,12 +281,37 @@ int shx_create(const shx_config* cfg, shx_sim** out) {
   }
   ok = ok && alloc(&s->clk, 16 * 8) && alloc((void**)&s->slot, 64) && alloc((void**)&s->diag_part, s->diag_blocks * 16 * 8) &&
        alloc((void**)&s->diag_out, 16 * 8);
+  std::vector<EnsTile> lists[ENS_NTILES];
+  if (s->flat2d) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess || cus < 1) cus = 256;
+    s->egrid = 4 * cus;
+    ok = ok && alloc(&s->einst, sizeof(EnsInst<float>));
+    for (int k = 0; k < ENS_NTILES; k++) {
+      int tx, ty;
+      ens_tile_cells(k, tx, ty);
+      for (int y0 = 0; y0 < c.n[1]; y0 += ty)
+        for (int x0 = 0; x0 < c.n[0]; x0 += tx) lists[k].push_back(EnsTile{0, x0, y0, std::min(y0 + ty, c.n[1])});
+      s->entiles[k] = (int)lists[k].size();
+      ok = ok && alloc(&s->etiles[k], lists[k].size() * sizeof(EnsTile));
+    }
+  }
   if (!ok) {
     shx_destroy(s);
     return fail(SHX_ENOMEM, "device allocation failed for %zu cells (%.2f GB per state copy)", s->ncell, bytes * 2 / 1e9);
   }
   (void)hipMemset(s->clk, 0, 16 * 8);
   (void)hipMemset(s->slot, 0, 64);
+  if (s->flat2d) {
+    const EnsInst<float> one{s->pf, 0};
+    bool copied = hipMemcpy(s->einst, &one, sizeof one, hipMemcpyHostToDevice) == hipSuccess;
+    for (int k = 0; k < ENS_NTILES; k++)
+      copied = copied && hipMemcpy(s->etiles[k], lists[k].data(), lists[k].size() * sizeof(EnsTile), hipMemcpyHostToDevice) == hipSuccess;
+    if (!copied) {
+      shx_destroy(s);
+      return fail(SHX_EDEVICE, "copying the grid's tile lists to the device failed");
+    }
+  }
   int rc = build_graphs(s);
   if (rc) { shx_destroy(s); return rc; }
   *out = s;
@@ -267,6 +325,8 @@ int shx_destroy(shx_sim* s) {
   for (int p = 0; p < 2; p++) { (void)hipFree(s->U1[p]); (void)hipFree(s->U2[p]); }
   (void)hipFree(s->PA); (void)hipFree(s->PB); (void)hipFree(s->clk); (void)hipFree(s->slot);
   (void)hipFree(s->diag_part); (void)hipFree(s->diag_out);
+  (void)hipFree(s->einst);
+  for (int k = 0; k < ENS_NTILES; k++) (void)hipFree(s->etiles[k]);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return SHX_OK;
@@ -411,8 +471,7 @@ int shx_profile(shx_sim* s, int steps, double ms[5]) {
       HIPCHK(hipEventRecord(ev[2], s->stream));
       if (s->zbc) s->ops.zplanes(L, s->U1[p], s->U2[p], s->PA, s->PB, s->clk);
       HIPCHK(hipEventRecord(ev[3], s->stream));
-      s->ops.fused(L, s->tile, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, s->zbc ? s->PA : nullptr,
-                   s->zbc ? s->PB : nullptr, s->zc, s->slot, false);
+      enqueue_fused(s, p, s->flat2d ? s->etile : s->tile, s->zc, false);
     } else {
       s->ops.rates(L, s->U1[p], s->U2[p], s->slot);
       HIPCHK(hipEventRecord(ev[1], s->stream));
@@ -439,7 +498,9 @@ int shx_kernel_name(shx_sim* s, char* buf, size_t len) {
   if (!s || !buf || len == 0) return fail(SHX_EINVAL, "bad argument");
   // threads by role: the FP32 tiles 32 x 16, 32 x 8 and 16 x 16
   const bool roles = s->es == 4 && (s->tile == FUSED_R32x16 || s->tile == FUSED_R32x8 || s->tile == FUSED_20x20);
-  snprintf(buf, len, "%s", !s->fused ? "k_update" : (roles ? "k_fused_r" : "k_fused"));
+  const bool rows = s->etile == ENS_ROWS8 || s->etile == ENS_ROWS16 || s->etile == ENS_ROWS32;
+  const char* fused = s->flat2d ? (rows ? "k_ens_rows" : "k_ens_fused") : (roles ? "k_fused_r" : "k_fused");
+  snprintf(buf, len, "%s", !s->fused ? "k_update" : fused);
   return SHX_OK;
 }
 
@@ -460,26 +521,27 @@ int shx_autotune(shx_sim* s, int reps, int chosen[3]) {
   unsigned char clk_saved[128];
   HIPCHK(hipMemcpyAsync(clk_saved, s->clk, 128, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
-  const int tiles[] = {FUSED_R32x16, FUSED_R32x8, FUSED_32x32, FUSED_32x16, FUSED_20x20};
+  const int tiles3[] = {FUSED_R32x16, FUSED_R32x8, FUSED_32x32, FUSED_32x16, FUSED_20x20};
+  const int tiles2[] = {ENS_32x32, ENS_32x16, ENS_20x20, ENS_ROWS8, ENS_ROWS16, ENS_ROWS32};
+  std::vector<int> tiles = s->flat2d ? std::vector<int>(tiles2, tiles2 + 6) : std::vector<int>(tiles3, tiles3 + 5);
   const int zcs[] = {16, 32, 64, 128};
   double best = 1e30;
-  int bt = s->tile, bz = s->zc;
+  int bt = s->flat2d ? s->etile : s->tile, bz = s->zc;
   Launch L = launch_of(s);
   // the boundary planes' half steps of the state every candidate starts from
-  const void* za = s->zbc ? s->PA : nullptr;
-  const void* zb = s->zbc ? s->PB : nullptr;
   if (s->zbc) s->ops.zplanes(L, s->U1[s->cur], s->U2[s->cur], s->PA, s->PB, s->clk);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   for (int ti : tiles)
     for (int z : zcs) {
-      if (z > s->cfg.n[2]) continue;
+      // the 2D kernels have no plane chunk: one candidate per tile; a chunk beyond the grid's
+      // planes is the whole grid, timed once
+      if (s->flat2d ? z != zcs[0] : (z > s->cfg.n[2] && z != zcs[0])) continue;
       int p = s->cur;
-      s->ops.fused(L, ti, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, za, zb, z, s->slot, false);  // warm
+      enqueue_fused(s, p, ti, z, false);  // warm
       HIPCHK(hipEventRecord(e0, s->stream));
-      for (int r = 0; r < reps; r++)
-        s->ops.fused(L, ti, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk, za, zb, z, s->slot, false);
+      for (int r = 0; r < reps; r++) enqueue_fused(s, p, ti, z, false);
       HIPCHK(hipEventRecord(e1, s->stream));
       HIPCHK(hipEventSynchronize(e1));
       float t;
@@ -488,8 +550,12 @@ int shx_autotune(shx_sim* s, int reps, int chosen[3]) {
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  s->tile = bt;
-  s->zc = bz;
+  if (s->flat2d) {
+    s->etile = bt;
+  } else {
+    s->tile = bt;
+    s->zc = std::min(bz, s->cfg.n[2]);
+  }
   HIPCHK(hipMemcpyAsync(s->U1[s->cur], b1, bytes, hipMemcpyDeviceToDevice, s->stream));
   HIPCHK(hipMemcpyAsync(s->U2[s->cur], b2, bytes, hipMemcpyDeviceToDevice, s->stream));
   HIPCHK(hipMemcpyAsync(s->clk, clk_saved, 128, hipMemcpyHostToDevice, s->stream));
@@ -501,8 +567,9 @@ int shx_autotune(shx_sim* s, int reps, int chosen[3]) {
   rc = build_graphs(s);
   if (rc) return rc;
   if (chosen) {
-    fused_tile_cells(bt, chosen[0], chosen[1]);
-    chosen[2] = bz;
+    if (s->flat2d) ens_tile_cells(bt, chosen[0], chosen[1]);
+    else fused_tile_cells(bt, chosen[0], chosen[1]);
+    chosen[2] = s->zc;
   }
   return SHX_OK;
 }

@@ -192,6 +192,62 @@ def test_fused_z_boundaries_equal_plain(zbc):
             assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]), (zbc, bxy, tile)
 
 
+FLAT_TILES = [(16, 16), (28, 12), (28, 28), (60, 8), (60, 16), (60, 32)]
+
+
+@pytest.mark.parametrize("tile", FLAT_TILES)
+def test_flat_grids_take_the_2d_kernels_with_the_plain_bits(tile):
+    """A flat FP32 grid on the fused path runs the 2D kernels (the row march over 60-cell strips,
+    or the LDS tile kernel): the plain path's bits on periodic, outflow, reflecting and mixed
+    faces, with ragged grids (partial strips and tiles on both axes, strips of one row), MC and
+    Koren + isentropic; every face bit-identical from both sides in the flux-debug form."""
+    cases = [problems.kelvin_helmholtz(64)]
+    for nx, ny, lim, isen, bxy in [(125, 67, "mc", False, (0, 0)), (61, 49, "koren", True, (0, 0)),
+                                   (70, 33, "mc", False, (0, 2)), (75, 17, "koren", True, (2, 1))]:
+        x, y, _ = problems.centers((nx, ny, 1), 1.0)
+        g = np.exp(-(x * x + y * y) / 0.02)
+        cases.append(problems.Problem(id=f"blob2d-{lim}", n=(nx, ny, 1), box=1.0, gamma=1.4, limiter=lim,
+                                      isentropic=isen, cfl=0.4, bc_lo=(bxy[0], bxy[1], 0), bc_hi=(bxy[0], bxy[1], 0),
+                                      prims={"rho": 1.0 + 0.8 * g, "vx": 0.4 * np.sin(2 * np.pi * y) - 1.5 * x * g,
+                                             "vy": 0.3 * np.cos(2 * np.pi * x) - 1.5 * y * g,
+                                             "p": 1.0 + 5.0 * np.exp(-(x * x + y * y) / 0.005)}))
+    for bc in (1, 2):
+        pb = problems.riemann_quadrants(50, seed=5)
+        pb.bc_lo = (bc, bc, 0)
+        pb.bc_hi = (bc, bc, 0)
+        cases.append(pb)
+    want_kernel = "k_ens_rows" if tile[0] == 60 else "k_ens_fused"
+    for pb in cases:
+        out = []
+        print(f"\n[gpu] flat tile {tile} {pb.id} {pb.n} bc {pb.bc_lo}", flush=True)
+        for path in ("plain", "fused"):
+            with Sim(config_for(pb, path=path, tile=(tile[0], tile[1], 1))) as sim:
+                sim.set_state(*pb.state())
+                sim.step(5)
+                out.append(sim.read_state())
+                if path == "fused":
+                    assert sim.kernel_name() == want_kernel
+                    mis, faces = sim.flux_mismatch()
+                    assert mis == 0, (tile, pb.id, mis, faces)
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]), (tile, pb.id)
+
+
+def test_flat_grid_autotune_keeps_the_bits():
+    pb = problems.kelvin_helmholtz(200)
+    out = []
+    for tune in (False, True):
+        with Sim(config_for(pb, path="fused")) as sim:
+            sim.set_state(*pb.state())
+            sim.step(2)
+            if tune:
+                chosen = sim.autotune(2)
+                print(f"\n[gpu] KH 200^2 autotuned tile {chosen}, kernel {sim.kernel_name()}", flush=True)
+                assert (chosen[0], chosen[1]) in FLAT_TILES
+            sim.step(4)
+            out.append(sim.read_state())
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_fused_outflow_z_matches_oracle():
     pb = blob3d(24)
     pb.bc_lo = (1, 2, 1)
