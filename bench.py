@@ -15,6 +15,8 @@ The block-AMR blast (root 8^3 blocks of 16^3 cells, up to 3 levels around a seed
 of radius 0.1 holding P = 100 against rho = 1, P = 1, gamma 1.4; 5024 blocks, 4460 leaves, 18.3 M
 leaf cells; MC + HLLC, CFL 0.4) is timed beside it: the fused batched path (one persistent kernel
 over device leaf lists, graph-captured) against the plain per-level passes, reported under "amr".
+The 2D ensemble (64 seeded Riemann quadrant problems) is reported under "ensemble", and 2D
+Kelvin-Helmholtz at 256^2 and 2048^2 under "kh", beside the reference's published figures.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--n 256] [--no-cpu] [--amr-steps 20]
 """
@@ -194,6 +196,39 @@ def ensemble_bench(args):
                          "frac": ach / HBM_PEAK, "kernel": kernel, "kernel_ms": prof["fused"]}}
 
 
+# The reference's published KH uniform throughputs (zone-updates/s, FP32, Koren + HLLC; other
+# hardware), paper/paper/figures/data/performance/throughput_h200.json:1 and throughput.json:1
+KH_PUBLISHED = {256: {"H200 NVL": 7.25e8, "RTX 3080": 5.04e8}, 2048: {"H200 NVL": 2.74e9, "RTX 3080": 1.85e9}}
+
+
+def kh_bench(args):
+    """2D Kelvin-Helmholtz on the unit square (periodic, gamma 5/3, the shear layer of the
+    256^2 CPU-runnable case) at 256^2 and 2048^2, MC + HLLC, CFL 0.4, on the fused path (a flat
+    FP32 grid steps through the 2D kernels, autotuned tile): zone-updates/s beside the
+    reference's published figures for the same grids on its own hardware."""
+    from shaderous_amd import problems
+    from shaderous_amd.sim import Sim, config_for
+    out = {}
+    for n in (256, 2048):
+        pb = problems.kelvin_helmholtz(n)
+        steps = args.kh_steps if n == 256 else max(args.kh_steps // 2, 1)
+        with Sim(config_for(pb, path="fused")) as sim:
+            sim.set_state(*pb.state())
+            tile = sim.autotune(reps=5)
+            sim.step(args.warmup)
+            t = time_steps(sim, steps)
+            prof = sim.profile(5)
+            kname = sim.kernel_name()
+        value = n * n * steps / t
+        log(f"[bench] KH {n}^2: tile {tile[:2]} ({kname}), {t / steps * 1e3:.4f} ms/step, {value:.3e} upd/s")
+        out[f"{n}^2"] = {"value": value, "ms_per_step": t / steps * 1e3, "steps": steps, "tile": tile[:2],
+                         "kernel": kname, "kernel_ms": prof["update"],
+                         "hbm_frac": BYTES_PER_UPDATE * n * n / (prof["update"] * 1e-3) / HBM_PEAK,
+                         "reference_published": KH_PUBLISHED[n]}
+    return {"workload": "2D Kelvin-Helmholtz, unit square, periodic, gamma 5/3, rho 2 / 1, vx +-0.5, P 2.5, "
+                        "seeded vy noise, PLM (MC) + HLLC, CFL 0.4", "grids": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +243,7 @@ def main():
     ap.add_argument("--amr-steps", type=int, default=20, help="timed steps of the AMR blast (0: skip it)")
     ap.add_argument("--amr-paths", default="fused,levels", help="AMR paths to time (fused, passes, levels)")
     ap.add_argument("--ens-steps", type=int, default=200, help="steps of the 2D ensemble (0: skip it)")
+    ap.add_argument("--kh-steps", type=int, default=100, help="steps of 2D KH at 256^2 (half at 2048^2; 0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a device copy of MIB MiB three times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -285,9 +321,12 @@ def main():
     ens = None
     if args.ens_steps > 0:
         ens = ensemble_bench(args)
+    kh = None
+    if args.kh_steps > 0:
+        kh = kh_bench(args)
     if value is None:
-        if amr or ens:
-            print(json.dumps({"amr": amr, "ensemble": ens}))
+        if amr or ens or kh:
+            print(json.dumps({"amr": amr, "ensemble": ens, "kh": kh}))
         return
 
     cpu = None
@@ -322,6 +361,7 @@ def main():
         "cpu_baseline": cpu,
         "amr": amr,
         "ensemble": ens,
+        "kh": kh,
     }
     print(json.dumps(out))
 
