@@ -27,6 +27,7 @@ import json
 import os
 import sys
 import time
+from dataclasses import replace
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -104,9 +105,48 @@ def cpu_baseline(pb, budget_s=10.0):
         t += time.perf_counter() - t0
         reps += 1
     cells = reps * pb.ncell
+    # one thread: slabs of 16 planes, in turn, until ~5 s
+    oracle.RefHydro.set_threads(1)
+    slabs, t1 = 0, 0.0
+    while t1 < budget_s / 2 and slabs < nz // 16:
+        k0 = 16 * slabs
+        t0 = time.perf_counter()
+        h.step_slab(k0, k0 + 16)
+        t1 += time.perf_counter() - t0
+        slabs += 1
+    cells1 = slabs * 16 * pb.n[0] * pb.n[1]
+    oracle.RefHydro.set_threads(cores)
     return {"value": cells / t, "unit": "zone-updates/s", "cores": cores, "kind": "port",
             "sample": f"{reps} whole-grid updates of RefHydro (C restatement, float64, {cores} OpenMP threads) on "
-                      f"the same {pb.n[0]}^3 state ({cells} zone-updates, {t:.1f} s)"}
+                      f"the same {pb.n[0]}^3 state ({cells} zone-updates, {t:.1f} s)",
+            "single_thread": {"value": cells1 / t1, "cores": 1,
+                              "sample": f"{slabs} slabs of 16 planes, one thread ({cells1} zone-updates, {t1:.1f} s)"}}
+
+
+def cpu_baseline_kh(pb, cores, steps=100):
+    """RefHydro on the 2D KH grid, run in full (`steps` steps with dt from the state's own
+    rates, src/reference/hydro.ts:172-458 restated in C, float64) at `cores` threads, and its
+    first fifth at one thread."""
+    import numpy as np
+    import oracle
+    out = {}
+    for threads, nsteps in ((cores, steps), (1, max(steps // 5, 1))):
+        oracle.RefHydro.set_threads(threads)
+        prm = oracle.params(pb.n, ndim=2, dx=pb.dx, gamma=pb.gamma, cfl=pb.cfl, limiter=pb.limiter,
+                            isentropic=pb.isentropic, riemann=pb.riemann)
+        h = oracle.RefHydro(prm)
+        h.set_cons(*pb.state(np.float64))
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            h.step()
+        t = time.perf_counter() - t0
+        out[threads] = {"value": pb.ncell * nsteps / t, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+                        "sample": f"{nsteps} steps of RefHydro (C restatement, float64) on the {pb.n[0]}^2 grid, "
+                                  f"{threads} thread(s), {t:.2f} s"}
+    oracle.RefHydro.set_threads(cores)
+    res = out[cores]
+    res["single_thread"] = out[1]
+    return res
 
 
 def amr_bench(args):
@@ -168,11 +208,13 @@ def ensemble_bench(args):
     cells = sum(n * n for n in ns)
     cfg = SimConfig(n=(64, 64, 1), ndim=2, dx=1.0 / 64, gamma=1.4, bc_lo=(1, 1, 0), bc_hi=(1, 1, 0), limiter="mc",
                     isentropic=False, cfl=0.4)
+    if args.tile2d:
+        cfg = replace(cfg, tile=(*args.tile2d, 1))
     ens = Ensemble(cfg, [(n, n) for n in ns])
     try:
         for i, n in enumerate(ns):
             ens.set_state(i, *problems.riemann_quadrants(n, seed=1000 + i).state())
-        tile = ens.autotune(reps=3)
+        tile = tuple(args.tile2d) if args.tile2d else ens.autotune(reps=3)
         ens.step(5)
         ens.sync()
         t0 = time.perf_counter()
@@ -212,9 +254,10 @@ def kh_bench(args):
     for n in (256, 2048):
         pb = problems.kelvin_helmholtz(n)
         steps = args.kh_steps if n == 256 else max(args.kh_steps // 2, 1)
-        with Sim(config_for(pb, path="fused")) as sim:
+        forced = (*args.tile2d, 1) if args.tile2d else (0, 0, 0)
+        with Sim(config_for(pb, path="fused", tile=forced)) as sim:
             sim.set_state(*pb.state())
-            tile = sim.autotune(reps=5)
+            tile = forced if args.tile2d else sim.autotune(reps=5)
             sim.step(args.warmup)
             t = time_steps(sim, steps)
             prof = sim.profile(5)
@@ -225,6 +268,9 @@ def kh_bench(args):
                          "kernel": kname, "kernel_ms": prof["update"],
                          "hbm_frac": BYTES_PER_UPDATE * n * n / (prof["update"] * 1e-3) / HBM_PEAK,
                          "reference_published": KH_PUBLISHED[n]}
+        if n == 256 and not args.no_cpu:
+            out[f"{n}^2"]["cpu_baseline"] = cpu_baseline_kh(pb, min(os.cpu_count() or 1, 16))
+            log(f"[bench] KH {n}^2 CPU: {out[f'{n}^2']['cpu_baseline']['value']:.3e} upd/s")
     return {"workload": "2D Kelvin-Helmholtz, unit square, periodic, gamma 5/3, rho 2 / 1, vx +-0.5, P 2.5, "
                         "seeded vy noise, PLM (MC) + HLLC, CFL 0.4", "grids": out}
 
@@ -244,6 +290,8 @@ def main():
     ap.add_argument("--amr-paths", default="fused,levels", help="AMR paths to time (fused, passes, levels)")
     ap.add_argument("--ens-steps", type=int, default=200, help="steps of the 2D ensemble (0: skip it)")
     ap.add_argument("--kh-steps", type=int, default=100, help="steps of 2D KH at 256^2 (half at 2048^2; 0: skip)")
+    ap.add_argument("--tile2d", default=None, metavar="TX,TY", type=lambda v: tuple(int(x) for x in v.split(",")),
+                    help="2D tile for the ensemble and KH instead of autotuning (e.g. 60,16)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a device copy of MIB MiB three times (calibrates HBM byte counters)")
     args = ap.parse_args()

@@ -8,8 +8,9 @@
 //                 once, the conservative update, and the next step's rate of the instance
 //                 (4 barriers per tile)
 // The arithmetic is the uniform grid's, in the order of its plain 2D passes (a flat grid's z
-// neighbours are the cell itself and its z fluxes zero), so an instance gives the same bits as a
-// single 2D run of the plain path.
+// neighbours are the cell itself: its half step leaves the zero z slope out, half_flat, as the
+// plain predictor does on a flat grid), so an instance gives the values of a single 2D run of
+// the plain path (zero fluxes added or left out change at most the sign of a zero).
 #pragma once
 #include "uniform_impl.hpp"
 
@@ -114,29 +115,15 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 2 : 1)) void k_ens_fused(
       const Prim<T> xm = lds_prim<T, NT>(L + LY::Q, t - 1), xp = lds_prim<T, NT>(L + LY::Q, t + 1);
       const Prim<T> ym = lds_prim<T, NT>(L + LY::Q, t - BX), yp = lds_prim<T, NT>(L + LY::Q, t + BX);
       Inc<T> ixu, ixd, iyu, iyd;
-      AxisSlope<T> sl[3];
-      sl[0] = axis_slope<T, LIM, ISEN>(P, xm, pc, xp, &ixu, &ixd);
-      sl[1] = axis_slope<T, LIM, ISEN>(P, ym, pc, yp, &iyu, &iyd);
-      Half<T> hc;
-      if (!mapped) {
-        sl[2] = axis_slope<T, LIM, ISEN>(P, pc, pc, pc, nullptr, nullptr);
-        const T conv = (xp.vx - xm.vx) + (yp.vy - ym.vy) + (pc.vz - pc.vz);
-        const T jump = tmax(jump_of(xm.p, xp.p), tmax(jump_of(ym.p, yp.p), jump_of(pc.p, pc.p)));
-        hc = hancock(P, pc, sl, conv, jump, dt);
-      } else {
+      Half<T> hc = half_flat<T, LIM, ISEN>(P, pc, xm, xp, ym, yp, dt, &ixu, &ixd, &iyu, &iyd);
+      if (mapped) {
         // the mapped cell's own stencil: its centre is this column's primitives seen unflipped
         Prim<T> qc = pc;
         qc.vx = bc_vel(qc.vx, mx.flip, 0);
         qc.vy = bc_vel(qc.vy, my.flip, 0);
         const Prim<T> sxm = lds_prim<T, NT>(L + LY::Q, ts - 1), sxp = lds_prim<T, NT>(L + LY::Q, ts + 1);
         const Prim<T> sym = lds_prim<T, NT>(L + LY::Q, ts - BX), syp = lds_prim<T, NT>(L + LY::Q, ts + BX);
-        AxisSlope<T> ms[3];
-        ms[0] = axis_slope<T, LIM, ISEN>(P, sxm, qc, sxp, nullptr, nullptr);
-        ms[1] = axis_slope<T, LIM, ISEN>(P, sym, qc, syp, nullptr, nullptr);
-        ms[2] = axis_slope<T, LIM, ISEN>(P, qc, qc, qc, nullptr, nullptr);
-        const T conv = (sxp.vx - sxm.vx) + (syp.vy - sym.vy) + (qc.vz - qc.vz);
-        const T jump = tmax(jump_of(sxm.p, sxp.p), tmax(jump_of(sym.p, syp.p), jump_of(qc.p, qc.p)));
-        hc = hancock(P, qc, ms, conv, jump, dt);
+        hc = half_flat<T, LIM, ISEN>(P, qc, sxm, sxp, sym, syp, dt, nullptr, nullptr, nullptr, nullptr);
         hc.vx = bc_vel(hc.vx, mx.flip, mx.side);
         hc.vy = bc_vel(hc.vy, my.flip, my.side);
       }
@@ -279,13 +266,7 @@ __global__ __launch_bounds__(64) void k_ens_rows(const EnsInst<float>* __restric
     const Prim<T> pp = row(c + 1);
     const Prim<T> xm = prim_beside<true>(pc), xp = prim_beside<false>(pc);
     Inc<T> ixu, ixd, iyu, iyd;
-    AxisSlope<T> sl[3];
-    sl[0] = axis_slope<T, LIM, ISEN>(P, xm, pc, xp, &ixu, &ixd);
-    sl[1] = axis_slope<T, LIM, ISEN>(P, pm, pc, pp, &iyu, &iyd);
-    sl[2] = axis_slope<T, LIM, ISEN>(P, pc, pc, pc, nullptr, nullptr);
-    const T conv = (xp.vx - xm.vx) + (pp.vy - pm.vy) + (pc.vz - pc.vz);
-    const T jump = tmax(jump_of(xm.p, xp.p), tmax(jump_of(pm.p, pp.p), jump_of(pc.p, pc.p)));
-    Half<T> hc = hancock(P, pc, sl, conv, jump, dt);
+    Half<T> hc = half_flat<T, LIM, ISEN>(P, pc, xm, xp, pm, pp, dt, &ixu, &ixd, &iyu, &iyd);
     // a ghost row above: the last row's half step
     const bool ghost_hi = ymap_hi && c == ny;
     if (ghost_hi) {
@@ -325,17 +306,17 @@ __global__ __launch_bounds__(64) void k_ens_rows(const EnsInst<float>* __restric
         }
         fy = face_from_sides<T, RS>(P, lo, syl, 1);
       }
-      // the update of row c - 1 (net_of's order: x, then y, then the flat z)
+      // the update of row c - 1 (net_of's order: x, then y; the flat z adds zeros)
       if (c - 1 >= y0 && c - 1 < y1 && cell_lane) {
         Flux<T> net;
-        net.m = (axy_prev.m + (fy_lo.m - fy.m)) + (T(0) - T(0));
-        net.mx = (axy_prev.mx + (fy_lo.mx - fy.mx)) + (T(0) - T(0));
-        net.my = (axy_prev.my + (fy_lo.my - fy.my)) + (T(0) - T(0));
-        net.mz = (axy_prev.mz + (fy_lo.mz - fy.mz)) + (T(0) - T(0));
-        net.e = (axy_prev.e + (fy_lo.e - fy.e)) + (T(0) - T(0));
-        net.ei = (axy_prev.ei + (fy_lo.ei - fy.ei)) + (T(0) - T(0));
-        net.s = (axy_prev.s + (fy_lo.s - fy.s)) + (T(0) - T(0));
-        net.vn = (axy_prev.vn + (fy_lo.vn - fy.vn)) + (T(0) - T(0));
+        net.m = axy_prev.m + (fy_lo.m - fy.m);
+        net.mx = axy_prev.mx + (fy_lo.mx - fy.mx);
+        net.my = axy_prev.my + (fy_lo.my - fy.my);
+        net.mz = axy_prev.mz + (fy_lo.mz - fy.mz);
+        net.e = axy_prev.e + (fy_lo.e - fy.e);
+        net.ei = axy_prev.ei + (fy_lo.ei - fy.ei);
+        net.s = axy_prev.s + (fy_lo.s - fy.s);
+        net.vn = axy_prev.vn + (fy_lo.vn - fy.vn);
         const size_t o = off + (size_t)(c - 1) * nx + x;
         T4<T> o1, o2;
         advance(P, U1[o], U2[o], net, hp_prev, dtdx, o1, o2);

@@ -300,6 +300,52 @@ __device__ __forceinline__ T jump_of(T pm, T pp) {
   return tdiv(tabs(pp - pm), tmin(pp, pm));
 }
 
+// The half step of a cell of a flat grid (one layer along z) from its four neighbours: hancock
+// without the z axis, whose neighbours are the cell itself (predictor.wgsl:46-209 with a zero
+// z slope, no z convergence and no z jump). The terms left out are zeros, so the result is
+// hancock's but for the sign of a zero. ixu .. iyd (may be null) receive the x and y edge
+// increments.
+template <typename T, int LIM, bool ISEN>
+__device__ __forceinline__ Half<T> half_flat(const Params<T>& P, const Prim<T>& q, const Prim<T>& xm,
+                                             const Prim<T>& xp, const Prim<T>& ym, const Prim<T>& yp, T dt,
+                                             Inc<T>* ixu, Inc<T>* ixd, Inc<T>* iyu, Inc<T>* iyd) {
+  const AxisSlope<T> ax = axis_slope<T, LIM, ISEN>(P, xm, q, xp, ixu, ixd);
+  const AxisSlope<T> ay = axis_slope<T, LIM, ISEN>(P, ym, q, yp, iyu, iyd);
+  const T conv = (xp.vx - xm.vx) + (yp.vy - ym.vy);
+  const T jump = tmax(jump_of(xm.p, xp.p), jump_of(ym.p, yp.p));
+  Half<T> o;
+  T big = tmax(tmax(T(0), ax.big), ay.big);
+  T fall = tmin(tmin(T(0), ax.fall), ay.fall);
+  T shock = (conv < T(0) && jump > P.shock_thresh) ? T(1) : T(0);
+  T h = T(0.5) * dt / P.dx;
+  T rho = q.rho;
+  const Inc<T>& sx = ax.s;
+  const Inc<T>& sy = ay.s;
+  T dv = sx.vx + sy.vy;
+  T da0 = q.vx * sx.rho + q.vy * sy.rho;
+  T da1 = q.vx * sx.vx + q.vy * sy.vx;
+  T da2 = q.vx * sx.vy + q.vy * sy.vy;
+  T da3 = q.vx * sx.vz + q.vy * sy.vz;
+  T db0 = q.vx * sx.p + q.vy * sy.p;
+  T db1 = q.vx * sx.s + q.vy * sy.s;
+  T ir = trcp(rho);
+  o.rho = rho - h * (da0 + rho * dv);
+  o.vx = q.vx - h * (da1 + sx.p * ir);
+  o.vy = q.vy - h * (da2 + sy.p * ir);
+  o.vz = q.vz - h * da3;
+  o.p = q.p - h * (db0 + rho * q.cs * q.cs * dv);
+  o.s = q.s - h * db1;
+  o.shock = shock;
+  T rmin = o.rho + fall;
+  T pmin = o.p - T(0.5) * big;
+  o.fo = T(0);
+  if (!(rmin > T(0) && pmin > T(0))) {
+    o.rho = q.rho; o.vx = q.vx; o.vy = q.vy; o.vz = q.vz; o.p = q.p; o.s = q.s;
+    o.fo = T(1);
+  }
+  return o;
+}
+
 // ---- face: hydro/flux.wgsl:60-125, riemann/hllc.wgsl, riemann/hll.wgsl ----
 template <typename T>
 struct Flux { T m, mx, my, mz, e, ei, s, vn; };

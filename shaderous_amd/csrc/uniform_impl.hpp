@@ -116,8 +116,13 @@ __device__ __forceinline__ void predict_cell(const Params<T>& P, const T4<T>* __
   Prim<T> q = load_q(P, U1, U2, i, j, k);
   Prim<T> xm = load_q(P, U1, U2, i - 1, j, k), xp = load_q(P, U1, U2, i + 1, j, k);
   Prim<T> ym = load_q(P, U1, U2, i, j - 1, k), yp = load_q(P, U1, U2, i, j + 1, k);
-  Prim<T> zm = q, zp = q;
-  if (!P.flat) { zm = load_q(P, U1, U2, i, j, k - 1); zp = load_q(P, U1, U2, i, j, k + 1); }
+  if (P.flat) {
+    const Half<T> h = half_flat<T, LIM, ISEN>(P, q, xm, xp, ym, yp, dt, nullptr, nullptr, nullptr, nullptr);
+    a.x = h.rho; a.y = h.vx; a.z = h.vy; a.w = h.vz;
+    b.x = h.p; b.y = h.s; b.z = h.shock; b.w = h.fo;
+    return;
+  }
+  Prim<T> zm = load_q(P, U1, U2, i, j, k - 1), zp = load_q(P, U1, U2, i, j, k + 1);
   AxisSlope<T> sl[3];
   sl[0] = axis_slope<T, LIM, ISEN>(P, xm, q, xp, nullptr, nullptr);
   sl[1] = axis_slope<T, LIM, ISEN>(P, ym, q, yp, nullptr, nullptr);
