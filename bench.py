@@ -15,8 +15,10 @@ The block-AMR blast (root 8^3 blocks of 16^3 cells, up to 3 levels around a seed
 of radius 0.1 holding P = 100 against rho = 1, P = 1, gamma 1.4; 5024 blocks, 4460 leaves, 18.3 M
 leaf cells; MC + HLLC, CFL 0.4) is timed beside it: the fused batched path (one persistent kernel
 over device leaf lists, graph-captured) against the plain per-level passes, reported under "amr".
-The 2D ensemble (64 seeded Riemann quadrant problems) is reported under "ensemble", and 2D
-Kelvin-Helmholtz at 256^2 and 2048^2 under "kh", beside the reference's published figures.
+The 2D ensemble (64 seeded Riemann quadrant problems) is reported under "ensemble", 2D
+Kelvin-Helmholtz 256^2 (with the CPU RefHydro on the same 100 steps) under "kh", and the
+reference's published uniform grids (KH 256^2 and 2048^2, Sedov 128^3 and 256^3) with its default
+scheme under "published_grids", beside its published figures.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--n 256] [--no-cpu] [--amr-steps 20]
 """
@@ -24,6 +26,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -238,41 +241,76 @@ def ensemble_bench(args):
                          "frac": ach / HBM_PEAK, "kernel": kernel, "kernel_ms": prof["fused"]}}
 
 
-# The reference's published KH uniform throughputs (zone-updates/s, FP32, Koren + HLLC; other
-# hardware), paper/paper/figures/data/performance/throughput_h200.json:1 and throughput.json:1
-KH_PUBLISHED = {256: {"H200 NVL": 7.25e8, "RTX 3080": 5.04e8}, 2048: {"H200 NVL": 2.74e9, "RTX 3080": 1.85e9}}
+# The reference's published uniform-grid throughputs (zone-updates/s; FP32, its default Koren
+# limiter and HLLC, one MUSCL-Hancock stage per step; its own hardware):
+# paper/paper/figures/data/performance/throughput_h200.json:1, throughput_h100.json:1,
+# throughput_a100.json:1 and throughput.json:1 (RTX 3080)
+PUBLISHED = {
+    ("kh", 256): {"H200 NVL": 7.25e8, "H100 SXM": 6.22e8, "A100 SXM4": 4.49e8, "RTX 3080": 5.04e8},
+    ("kh", 2048): {"H200 NVL": 2.74e9, "H100 SXM": 2.60e9, "A100 SXM4": 1.74e9, "RTX 3080": 1.85e9},
+    ("sedov", 128): {"H200 NVL": 1.42e9, "H100 SXM": 1.34e9, "A100 SXM4": 1.04e9, "RTX 3080": 1.26e9},
+    ("sedov", 256): {"H200 NVL": 1.67e9, "H100 SXM": 1.57e9, "A100 SXM4": 1.20e9, "RTX 3080": 1.39e9},
+}
+
+
+def uniform_run(args, pb, steps):
+    """Zone-updates/s of a uniform problem on the fused path with the autotuned tile (or
+    --tile2d on a flat grid), and the dominant kernel's time by HIP events."""
+    from shaderous_amd.sim import Sim, config_for
+    flat = pb.n[2] == 1
+    forced = (*args.tile2d, 1) if (args.tile2d and flat) else (0, 0, 0)
+    with Sim(config_for(pb, path="fused", tile=forced)) as sim:
+        sim.set_state(*pb.state())
+        tile = forced if forced[0] else sim.autotune(reps=5)
+        sim.step(args.warmup)
+        t = time_steps(sim, steps)
+        prof = sim.profile(5)
+        kname = sim.kernel_name()
+    value = pb.ncell * steps / t
+    return {"value": value, "ms_per_step": t / steps * 1e3, "steps": steps, "tile": list(tile[:2] if flat else tile),
+            "kernel": kname, "kernel_ms": prof["update"],
+            "hbm_frac": BYTES_PER_UPDATE * pb.ncell / (prof["update"] * 1e-3) / HBM_PEAK}
 
 
 def kh_bench(args):
-    """2D Kelvin-Helmholtz on the unit square (periodic, gamma 5/3, the shear layer of the
-    256^2 CPU-runnable case) at 256^2 and 2048^2, MC + HLLC, CFL 0.4, on the fused path (a flat
-    FP32 grid steps through the 2D kernels, autotuned tile): zone-updates/s beside the
-    reference's published figures for the same grids on its own hardware."""
+    """2D Kelvin-Helmholtz 256^2, the reference-CPU-runnable case: unit square, periodic, gamma
+    5/3, PLM (MC) + HLLC, CFL 0.4, on the fused path (a flat FP32 grid steps through the 2D
+    kernels), with the CPU RefHydro running the same 100 steps."""
     from shaderous_amd import problems
-    from shaderous_amd.sim import Sim, config_for
+    pb = problems.kelvin_helmholtz(256)
+    r = uniform_run(args, pb, args.kh_steps)
+    log(f"[bench] KH 256^2: tile {r['tile']} ({r['kernel']}), {r['ms_per_step']:.4f} ms/step, {r['value']:.3e} upd/s")
+    if not args.no_cpu:
+        r["cpu_baseline"] = cpu_baseline_kh(pb, min(os.cpu_count() or 1, 16))
+        log(f"[bench] KH 256^2 CPU: {r['cpu_baseline']['value']:.3e} upd/s")
+    return {"workload": "2D Kelvin-Helmholtz 256^2, unit square, periodic, gamma 5/3, rho 2 / 1, vx +-0.5, P 2.5, "
+                        "seeded vy noise, PLM (MC) + HLLC, CFL 0.4", **r}
+
+
+def published_bench(args):
+    """The reference's published uniform grids with its default scheme (Koren + isentropic
+    reconstruction, HLLC; CFL 0.4): 2D Kelvin-Helmholtz at 256^2 and 2048^2 (the shear layer
+    above) and the 3D Sedov blast at 128^3 and 256^3 (gamma 1.4, P 1e-5 around E = 1 within 3
+    cells, periodic), on the fused path with the autotuned tile, beside its published figures
+    for the same grids on its own hardware."""
+    from shaderous_amd import problems
     out = {}
-    for n in (256, 2048):
-        pb = problems.kelvin_helmholtz(n)
-        steps = args.kh_steps if n == 256 else max(args.kh_steps // 2, 1)
-        forced = (*args.tile2d, 1) if args.tile2d else (0, 0, 0)
-        with Sim(config_for(pb, path="fused", tile=forced)) as sim:
-            sim.set_state(*pb.state())
-            tile = forced if args.tile2d else sim.autotune(reps=5)
-            sim.step(args.warmup)
-            t = time_steps(sim, steps)
-            prof = sim.profile(5)
-            kname = sim.kernel_name()
-        value = n * n * steps / t
-        log(f"[bench] KH {n}^2: tile {tile[:2]} ({kname}), {t / steps * 1e3:.4f} ms/step, {value:.3e} upd/s")
-        out[f"{n}^2"] = {"value": value, "ms_per_step": t / steps * 1e3, "steps": steps, "tile": tile[:2],
-                         "kernel": kname, "kernel_ms": prof["update"],
-                         "hbm_frac": BYTES_PER_UPDATE * n * n / (prof["update"] * 1e-3) / HBM_PEAK,
-                         "reference_published": KH_PUBLISHED[n]}
-        if n == 256 and not args.no_cpu:
-            out[f"{n}^2"]["cpu_baseline"] = cpu_baseline_kh(pb, min(os.cpu_count() or 1, 16))
-            log(f"[bench] KH {n}^2 CPU: {out[f'{n}^2']['cpu_baseline']['value']:.3e} upd/s")
-    return {"workload": "2D Kelvin-Helmholtz, unit square, periodic, gamma 5/3, rho 2 / 1, vx +-0.5, P 2.5, "
-                        "seeded vy noise, PLM (MC) + HLLC, CFL 0.4", "grids": out}
+    for kind, n in (("kh", 256), ("kh", 2048), ("sedov", 128), ("sedov", 256)):
+        if kind == "kh":
+            pb = problems.kelvin_helmholtz(n, limiter="koren", isentropic=True)
+            steps = args.kh_steps if n == 256 else max(args.kh_steps // 2, 1)
+            name = f"KH {n}^2"
+        else:
+            pb = problems.sedov(n, limiter="koren", isentropic=True, t_end=math.inf)
+            steps = max(args.kh_steps // 2, 1) if n == 128 else max(args.kh_steps // 5, 1)
+            name = f"Sedov {n}^3"
+        r = uniform_run(args, pb, steps)
+        r["reference_published"] = PUBLISHED[(kind, n)]
+        r["over_h200"] = r["value"] / PUBLISHED[(kind, n)]["H200 NVL"]
+        log(f"[bench] {name} (Koren): tile {r['tile']} ({r['kernel']}), {r['ms_per_step']:.4f} ms/step, "
+            f"{r['value']:.3e} upd/s, {r['over_h200']:.1f}x the reference on an H200")
+        out[name] = r
+    return {"scheme": "PLM (Koren, isentropic reconstruction) + HLLC, CFL 0.4, FP32", "grids": out}
 
 
 def main():
@@ -289,7 +327,8 @@ def main():
     ap.add_argument("--amr-steps", type=int, default=20, help="timed steps of the AMR blast (0: skip it)")
     ap.add_argument("--amr-paths", default="fused,levels", help="AMR paths to time (fused, passes, levels)")
     ap.add_argument("--ens-steps", type=int, default=200, help="steps of the 2D ensemble (0: skip it)")
-    ap.add_argument("--kh-steps", type=int, default=100, help="steps of 2D KH at 256^2 (half at 2048^2; 0: skip)")
+    ap.add_argument("--kh-steps", type=int, default=100,
+                    help="steps of 2D KH at 256^2 (half at 2048^2 and Sedov 128^3, a fifth at 256^3; 0: skip)")
     ap.add_argument("--tile2d", default=None, metavar="TX,TY", type=lambda v: tuple(int(x) for x in v.split(",")),
                     help="2D tile for the ensemble and KH instead of autotuning (e.g. 60,16)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -369,12 +408,13 @@ def main():
     ens = None
     if args.ens_steps > 0:
         ens = ensemble_bench(args)
-    kh = None
+    kh = pub = None
     if args.kh_steps > 0:
         kh = kh_bench(args)
+        pub = published_bench(args)
     if value is None:
         if amr or ens or kh:
-            print(json.dumps({"amr": amr, "ensemble": ens, "kh": kh}))
+            print(json.dumps({"amr": amr, "ensemble": ens, "kh": kh, "published_grids": pub}))
         return
 
     cpu = None
@@ -410,6 +450,7 @@ def main():
         "amr": amr,
         "ensemble": ens,
         "kh": kh,
+        "published_grids": pub,
     }
     print(json.dumps(out))
 

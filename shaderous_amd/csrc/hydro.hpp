@@ -53,7 +53,10 @@ __device__ __forceinline__ float tdiv(float a, float b) { return a * __builtin_a
 __device__ __forceinline__ double tdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
-__device__ __forceinline__ float tpow(float a, float b) { return powf(a, b); }
+// Single-precision pow as WGSL defines it, exp2(b log2 a) (every base here is positive), by the
+// hardware's log2 and exp2 (v_log_f32, v_exp_f32): the library powf's exact-case handling costs
+// ~40 instructions a call, and the isentropic reconstruction makes six calls per axis.
+__device__ __forceinline__ float tpow(float a, float b) { return __builtin_amdgcn_exp2f(b * __builtin_amdgcn_logf(a)); }
 __device__ __forceinline__ double tpow(double a, double b) { return pow(a, b); }
 // WGSL mix(x, y, a) = x (1 - a) + y a
 template <typename T> __device__ __forceinline__ T tmix(T x, T y, T a) { return x * (T(1) - a) + y * a; }
@@ -170,38 +173,54 @@ template <int LIM> constexpr bool lim_symmetric() { return LIM != LIM_KOREN; }
 template <typename T>
 struct Inc { T rho, vx, vy, vz, p, s; };
 
-// isen_inc / isen_blend (hydro/q.wgsl:183-213) with no reference state and no potential.
-template <typename T, int LIM>
-__device__ __forceinline__ T isen_blend(const Params<T>& P, const Prim<T>& l, const Prim<T>& c, const Prim<T>& r,
-                                        T x, T bx, T sgn) {
-  const T ISEN_SMOOTH = T(0.2), ISEN_ADIABAT = T(0.01), ISEN_SHEAR = T(0.1), ISEN_CAP = T(0.25), ISEN_ROOM = T(0.01);
-  if (!(l.rho > T(0) && l.p > T(0) && c.rho > T(0) && c.p > T(0) && r.rho > T(0) && r.p > T(0))) {
-    return x + tclamp(T(0) - x, T(0), T(0));
-  }
-  T gi = T(1) / P.gamma;
-  T tl = l.rho * tpow(c.p / l.p, gi) - c.rho;
-  T tr = r.rho * tpow(c.p / r.p, gi) - c.rho;
-  T far = sgn > T(0) ? -tl : tr;
-  T near = sgn > T(0) ? tr : -tl;
-  T d = edge1<T, LIM>(far, near, c.rho);
-  T pf = c.p + T(0.5) * sgn * bx;
-  T wn = sgn > T(0) ? r.rho : l.rho;
-  T rf = tclamp((c.rho + T(0.5) * sgn * d) * tpow(tmax(pf, T(1e-30)) / c.p, gi), tmin(c.rho, wn), tmax(c.rho, wn));
-  T ks = tclamp(T(2) - tmax(tabs(l.p - c.p), tabs(r.p - c.p)) / (ISEN_SMOOTH * c.p), T(0), T(1));
-  T ka = tclamp(T(2) - tmax(tabs(tl), tabs(tr)) / (ISEN_ADIABAT * c.rho), T(0), T(1));
+// isen_inc / isen_blend (hydro/q.wgsl:183-213) with no reference state and no potential, in two
+// parts: what depends only on the three cells (the densities along the adiabat through the
+// centre, the smoothness, adiabat and shear switches), shared by a cell's up and down edges, and
+// the edge's own blend.
+template <typename T>
+struct IsenCommon { bool ok; T gi, tl, tr, room; };
+
+template <typename T>
+__device__ __forceinline__ IsenCommon<T> isen_common(const Params<T>& P, const Prim<T>& l, const Prim<T>& c,
+                                                     const Prim<T>& r) {
+  const T ISEN_SMOOTH = T(0.2), ISEN_ADIABAT = T(0.01), ISEN_SHEAR = T(0.1), ISEN_ROOM = T(0.01);
+  IsenCommon<T> k;
+  k.ok = l.rho > T(0) && l.p > T(0) && c.rho > T(0) && c.p > T(0) && r.rho > T(0) && r.p > T(0);
+  k.gi = trcp(P.gamma);
+  k.tl = l.rho * tpow(tdiv(c.p, l.p), k.gi) - c.rho;
+  k.tr = r.rho * tpow(tdiv(c.p, r.p), k.gi) - c.rho;
+  T ks = tclamp(T(2) - tdiv(tmax(tabs(l.p - c.p), tabs(r.p - c.p)), ISEN_SMOOTH * c.p), T(0), T(1));
+  T ka = tclamp(T(2) - tdiv(tmax(tabs(k.tl), tabs(k.tr)), ISEN_ADIABAT * c.rho), T(0), T(1));
   T dxr = r.vx - c.vx, dyr = r.vy - c.vy, dzr = r.vz - c.vz;
   T dxl = c.vx - l.vx, dyl = c.vy - l.vy, dzl = c.vz - l.vz;
   T dvr = tsqrt(dxr * dxr + dyr * dyr + dzr * dzr);
   T dvl = tsqrt(dxl * dxl + dyl * dyl + dzl * dzl);
-  T kv = tclamp(T(2) - tmax(dvr, dvl) / (ISEN_SHEAR * tmax(c.cs, T(1e-30))), T(0), T(1));
-  T bound = tmax(ISEN_CAP * tmin(tabs(c.rho - l.rho), tabs(r.rho - c.rho)), ISEN_ROOM * c.rho * ks * ka * kv);
+  T kv = tclamp(T(2) - tdiv(tmax(dvr, dvl), ISEN_SHEAR * tmax(c.cs, T(1e-30))), T(0), T(1));
+  k.room = ISEN_ROOM * c.rho * ks * ka * kv;
+  return k;
+}
+
+template <typename T, int LIM>
+__device__ __forceinline__ T isen_blend(const IsenCommon<T>& k, const Prim<T>& l, const Prim<T>& c, const Prim<T>& r,
+                                        T x, T bx, T sgn) {
+  const T ISEN_CAP = T(0.25);
+  if (!k.ok) return x + tclamp(T(0) - x, T(0), T(0));
+  T far = sgn > T(0) ? -k.tl : k.tr;
+  T near = sgn > T(0) ? k.tr : -k.tl;
+  T d = edge1<T, LIM>(far, near, c.rho);
+  T pf = c.p + T(0.5) * sgn * bx;
+  T wn = sgn > T(0) ? r.rho : l.rho;
+  T rf = tclamp((c.rho + T(0.5) * sgn * d) * tpow(tdiv(tmax(pf, T(1e-30)), c.p), k.gi), tmin(c.rho, wn),
+                tmax(c.rho, wn));
+  T bound = tmax(ISEN_CAP * tmin(tabs(c.rho - l.rho), tabs(r.rho - c.rho)), k.room);
   T i0 = T(2) * sgn * (rf - c.rho);
   return x + tclamp(i0 - x, -bound, bound);
 }
 
 // edge_up (UP) / edge_dn (hydro/q.wgsl:214-239), no potential.
 template <typename T, int LIM, bool ISEN, bool UP>
-__device__ __forceinline__ Inc<T> edge(const Params<T>& P, const Prim<T>& l, const Prim<T>& c, const Prim<T>& r) {
+__device__ __forceinline__ Inc<T> edge_k(const Params<T>& P, const Prim<T>& l, const Prim<T>& c, const Prim<T>& r,
+                                         const IsenCommon<T>& k) {
   Inc<T> o;
   constexpr bool fwd = UP || lim_symmetric<LIM>();
   auto e = [&](T lo, T mid, T hi, T scale) {
@@ -214,8 +233,14 @@ __device__ __forceinline__ Inc<T> edge(const Params<T>& P, const Prim<T>& l, con
   o.vz = e(l.vz, c.vz, r.vz, c.cs);
   o.p = e(l.p, c.p, r.p, c.p);
   o.s = e(l.s, c.s, r.s, T(0));
-  if constexpr (ISEN) o.rho = isen_blend<T, LIM>(P, l, c, r, o.rho, o.p, UP ? T(1) : T(-1));
+  if constexpr (ISEN) o.rho = isen_blend<T, LIM>(k, l, c, r, o.rho, o.p, UP ? T(1) : T(-1));
   return o;
+}
+template <typename T, int LIM, bool ISEN, bool UP>
+__device__ __forceinline__ Inc<T> edge(const Params<T>& P, const Prim<T>& l, const Prim<T>& c, const Prim<T>& r) {
+  IsenCommon<T> k{};
+  if constexpr (ISEN) k = isen_common(P, l, c, r);
+  return edge_k<T, LIM, ISEN, UP>(P, l, c, r, k);
 }
 
 // Half-step state (hydro/predictor.wgsl:46-209): rho, v, p, s at t + dt/2, the shock flag and
@@ -231,7 +256,10 @@ template <typename T, int LIM, bool ISEN>
 __device__ __forceinline__ AxisSlope<T> axis_slope(const Params<T>& P, const Prim<T>& m, const Prim<T>& q,
                                                    const Prim<T>& p, Inc<T>* up_out, Inc<T>* dn_out) {
   AxisSlope<T> r;
-  Inc<T> u = edge<T, LIM, ISEN, true>(P, m, q, p);
+  // a cell's up and down edges share the isentropic terms of its three cells
+  IsenCommon<T> k{};
+  if constexpr (ISEN) k = isen_common(P, m, q, p);
+  Inc<T> u = edge_k<T, LIM, ISEN, true>(P, m, q, p, k);
   if constexpr (lim_symmetric<LIM>()) {
     r.s = u;
     r.big = tabs(u.p);
@@ -239,7 +267,7 @@ __device__ __forceinline__ AxisSlope<T> axis_slope(const Params<T>& P, const Pri
     if (up_out) *up_out = u;
     if (dn_out) *dn_out = u;
   } else {
-    Inc<T> d = edge<T, LIM, ISEN, false>(P, m, q, p);
+    Inc<T> d = edge_k<T, LIM, ISEN, false>(P, m, q, p, k);
     r.s.rho = T(0.5) * (u.rho + d.rho);
     r.s.vx = T(0.5) * (u.vx + d.vx);
     r.s.vy = T(0.5) * (u.vy + d.vy);
