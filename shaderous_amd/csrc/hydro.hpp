@@ -133,25 +133,41 @@ __device__ __forceinline__ T bc_vel(T v, int flip, int side) {
 }
 
 // ---- limiters: hydro/limiters/*.wgsl, per channel ----
-// Twice the edge increment from the differences (far, near) and the channel's natural scale.
+// Koren's smoothness measure of a channel against its natural scale s = 0.01 scale: the
+// difference energy over s^2 (koren.wgsl:274-292), or, where s^2 over- or underflows, the
+// differences scaled by 1/|s| first. What depends on the scale alone is formed once per scale
+// (a cell's six channels have four: rho, c_s three times, p, none) and shared by its edges.
+template <typename T>
+struct EdgeScale { T s2inv; T k; bool as_is; };
+
+template <typename T>
+__device__ __forceinline__ EdgeScale<T> edge_scale(T scale) {
+  const T SMOOTH = T(0.01);
+  T s = SMOOTH * scale;
+  T s2 = s * s;
+  EdgeScale<T> e;
+  e.as_is = (s2 >= T(1e-30) && s2 < T(1e30)) || (s == T(0));
+  e.s2inv = tmax(s2, T(1e-30));
+  if constexpr (sizeof(T) == 4) e.s2inv = trcp(e.s2inv);   // tdiv(q, ss) = q rcp(ss) in FP32
+  e.k = trcp(tmax(tabs(s), T(1e-37)));
+  return e;
+}
+
+// Twice the edge increment from the differences (far, near) and the channel's scale.
 // Koren (koren.wgsl:274-292): its early return gives exactly the blend's value per channel
 // (the blend weight is 1 for a smooth channel), so each channel is formed on its own.
 template <typename T, int LIM>
-__device__ __forceinline__ T edge1(T far, T near, T scale) {
+__device__ __forceinline__ T edge1(T far, T near, const EdgeScale<T>& sc) {
   if constexpr (LIM == LIM_KOREN) {
-    const T SMOOTH = T(0.01);
     T fr = (far + T(2) * near) * T(1.0 / 3.0);
-    T s = SMOOTH * scale;
-    T s2 = s * s;
-    T ss = tmax(s2, T(1e-30));
     T q = far * far + near * near;
-    bool as_is = (s2 >= T(1e-30) && s2 < T(1e30)) || (s == T(0));
     T eta;
-    if (as_is) {
-      eta = tdiv(q, ss);
+    if (sc.as_is) {
+      // FP32: q times the reciprocal (tdiv's form); FP64: the quotient (s2inv holds ss)
+      if constexpr (sizeof(T) == 4) eta = q * sc.s2inv;
+      else eta = tdiv(q, sc.s2inv);
     } else {
-      T k = trcp(tmax(tabs(s), T(1e-37)));
-      T fk = far * k, nk = near * k;
+      T fk = far * sc.k, nk = near * sc.k;
       eta = fk * fk + nk * nk;
     }
     // sign(near) min(2|far|, 2|near|, |fr|) where far and near share a sign (fr then does too)
@@ -166,6 +182,13 @@ __device__ __forceinline__ T edge1(T far, T near, T scale) {
   } else {
     return same_sign(far, near) ? T(2) * far * tdiv(near, far + near) : T(0);
   }
+}
+// The same from the scale itself (callers with one channel per scale)
+template <typename T, int LIM>
+__device__ __forceinline__ T edge1(T far, T near, T scale) {
+  EdgeScale<T> sc{};
+  if constexpr (LIM == LIM_KOREN) sc = edge_scale(scale);
+  return edge1<T, LIM>(far, near, sc);
 }
 template <int LIM> constexpr bool lim_symmetric() { return LIM != LIM_KOREN; }
 
@@ -201,13 +224,13 @@ __device__ __forceinline__ IsenCommon<T> isen_common(const Params<T>& P, const P
 }
 
 template <typename T, int LIM>
-__device__ __forceinline__ T isen_blend(const IsenCommon<T>& k, const Prim<T>& l, const Prim<T>& c, const Prim<T>& r,
-                                        T x, T bx, T sgn) {
+__device__ __forceinline__ T isen_blend(const IsenCommon<T>& k, const EdgeScale<T>& srho, const Prim<T>& l,
+                                        const Prim<T>& c, const Prim<T>& r, T x, T bx, T sgn) {
   const T ISEN_CAP = T(0.25);
   if (!k.ok) return x + tclamp(T(0) - x, T(0), T(0));
   T far = sgn > T(0) ? -k.tl : k.tr;
   T near = sgn > T(0) ? k.tr : -k.tl;
-  T d = edge1<T, LIM>(far, near, c.rho);
+  T d = edge1<T, LIM>(far, near, srho);
   T pf = c.p + T(0.5) * sgn * bx;
   T wn = sgn > T(0) ? r.rho : l.rho;
   T rf = tclamp((c.rho + T(0.5) * sgn * d) * tpow(tdiv(tmax(pf, T(1e-30)), c.p), k.gi), tmin(c.rho, wn),
@@ -223,17 +246,21 @@ __device__ __forceinline__ Inc<T> edge_k(const Params<T>& P, const Prim<T>& l, c
                                          const IsenCommon<T>& k) {
   Inc<T> o;
   constexpr bool fwd = UP || lim_symmetric<LIM>();
-  auto e = [&](T lo, T mid, T hi, T scale) {
+  auto e = [&](T lo, T mid, T hi, const EdgeScale<T>& sc) {
     T below = mid - lo, above = hi - mid;
-    return fwd ? edge1<T, LIM>(below, above, scale) : edge1<T, LIM>(above, below, scale);
+    return fwd ? edge1<T, LIM>(below, above, sc) : edge1<T, LIM>(above, below, sc);
   };
-  o.rho = e(l.rho, c.rho, r.rho, c.rho);
-  o.vx = e(l.vx, c.vx, r.vx, c.cs);
-  o.vy = e(l.vy, c.vy, r.vy, c.cs);
-  o.vz = e(l.vz, c.vz, r.vz, c.cs);
-  o.p = e(l.p, c.p, r.p, c.p);
-  o.s = e(l.s, c.s, r.s, T(0));
-  if constexpr (ISEN) o.rho = isen_blend<T, LIM>(k, l, c, r, o.rho, o.p, UP ? T(1) : T(-1));
+  EdgeScale<T> srho{}, sv{}, sp{}, s0{};
+  if constexpr (LIM == LIM_KOREN) {
+    srho = edge_scale(c.rho); sv = edge_scale(c.cs); sp = edge_scale(c.p); s0 = edge_scale(T(0));
+  }
+  o.rho = e(l.rho, c.rho, r.rho, srho);
+  o.vx = e(l.vx, c.vx, r.vx, sv);
+  o.vy = e(l.vy, c.vy, r.vy, sv);
+  o.vz = e(l.vz, c.vz, r.vz, sv);
+  o.p = e(l.p, c.p, r.p, sp);
+  o.s = e(l.s, c.s, r.s, s0);
+  if constexpr (ISEN) o.rho = isen_blend<T, LIM>(k, srho, l, c, r, o.rho, o.p, UP ? T(1) : T(-1));
   return o;
 }
 template <typename T, int LIM, bool ISEN, bool UP>
