@@ -1,7 +1,8 @@
 """Kernel resources recorded by the build (hipcc's kernel-resource-usage remarks, per translation
-unit): the step kernels of the default scheme (MC + HLLC, FP32) use no scratch memory, whose
-stores go to HBM on every plane of a march, and the role-tiled kernels keep the occupancy and LDS
-their launch shapes are designed for. CPU only."""
+unit): the step kernels of the default scheme (MC + HLLC, FP32), and those the reference's own
+scheme (Koren + isentropic) runs on, use no scratch memory, whose stores go to HBM on every plane
+of a march, and the role-tiled kernels keep the occupancy and LDS their launch shapes are
+designed for. CPU only."""
 import pytest
 
 from shaderous_amd import build
@@ -22,6 +23,15 @@ NO_SCRATCH = {
 }
 
 
+# the reference's default scheme (Koren + isentropic reconstruction, HLLC; LIM 0, RS 0, ISEN
+# true) in the shapes its runs take: the 32 x 16 role tile on a grid periodic in x and y (the
+# autotuner's pick: 32 x 8 and 16 x 16 spill at 128 VGPRs with Koren) and the 2D row march
+NO_SCRATCH_KOREN = {
+    "k_fused_r Koren + isentropic 32 x 16": "_ZN3shx9k_fused_rIfLi0ELi0ELb1ELi32ELi16ELb0ELb0E",
+    "k_ens_rows Koren + isentropic": "_ZN3shx10k_ens_rowsILi0ELi0ELb1E",
+}
+
+
 @pytest.fixture(scope="module")
 def resources():
     res = build.kernel_resources()
@@ -31,7 +41,7 @@ def resources():
 
 
 def test_default_scheme_step_kernels_use_no_scratch(resources):
-    for what, prefix in NO_SCRATCH.items():
+    for what, prefix in {**NO_SCRATCH, **NO_SCRATCH_KOREN}.items():
         found = [k for k in resources if k["kernel"].startswith(prefix)]
         assert found, f"no {what} kernel in the build"
         bad = [(k["kernel"], k.get("scratch")) for k in found if k.get("scratch", 0) != 0]
