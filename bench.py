@@ -16,7 +16,7 @@ of radius 0.1 holding P = 100 against rho = 1, P = 1, gamma 1.4; 5024 blocks, 44
 leaf cells; MC + HLLC, CFL 0.4) is timed beside it: the fused batched path (one persistent kernel
 over device leaf lists, graph-captured) against the plain per-level passes, reported under "amr".
 The 2D ensemble (64 seeded Riemann quadrant problems) is reported under "ensemble", 2D
-Kelvin-Helmholtz 256^2 (with the CPU RefHydro on the same 100 steps) under "kh", and the
+Kelvin-Helmholtz at 256^2 (with the CPU RefHydro on the same 100 steps) and 2048^2 under "kh", and the
 reference's published uniform grids (KH 256^2 and 2048^2, Sedov 128^3 and 256^3) with its default
 scheme under "published_grids", beside its published figures.
 
@@ -273,18 +273,21 @@ def uniform_run(args, pb, steps):
 
 
 def kh_bench(args):
-    """2D Kelvin-Helmholtz 256^2, the reference-CPU-runnable case: unit square, periodic, gamma
-    5/3, PLM (MC) + HLLC, CFL 0.4, on the fused path (a flat FP32 grid steps through the 2D
-    kernels), with the CPU RefHydro running the same 100 steps."""
+    """2D Kelvin-Helmholtz at 256^2 (the reference-CPU-runnable case, with the CPU RefHydro
+    running the same 100 steps) and 2048^2: unit square, periodic, gamma 5/3, PLM (MC) + HLLC,
+    CFL 0.4, on the fused path (a flat FP32 grid steps through the 2D kernels)."""
     from shaderous_amd import problems
-    pb = problems.kelvin_helmholtz(256)
-    r = uniform_run(args, pb, args.kh_steps)
-    log(f"[bench] KH 256^2: tile {r['tile']} ({r['kernel']}), {r['ms_per_step']:.4f} ms/step, {r['value']:.3e} upd/s")
-    if not args.no_cpu:
-        r["cpu_baseline"] = cpu_baseline_kh(pb, min(os.cpu_count() or 1, 16))
-        log(f"[bench] KH 256^2 CPU: {r['cpu_baseline']['value']:.3e} upd/s")
-    return {"workload": "2D Kelvin-Helmholtz 256^2, unit square, periodic, gamma 5/3, rho 2 / 1, vx +-0.5, P 2.5, "
-                        "seeded vy noise, PLM (MC) + HLLC, CFL 0.4", **r}
+    out = {}
+    for n in (256, 2048):
+        pb = problems.kelvin_helmholtz(n)
+        r = uniform_run(args, pb, args.kh_steps if n == 256 else max(args.kh_steps // 2, 1))
+        log(f"[bench] KH {n}^2: tile {r['tile']} ({r['kernel']}), {r['ms_per_step']:.4f} ms/step, {r['value']:.3e} upd/s")
+        if n == 256 and not args.no_cpu:
+            r["cpu_baseline"] = cpu_baseline_kh(pb, min(os.cpu_count() or 1, 16))
+            log(f"[bench] KH 256^2 CPU: {r['cpu_baseline']['value']:.3e} upd/s")
+        out[f"{n}^2"] = r
+    return {"workload": "2D Kelvin-Helmholtz, unit square, periodic, gamma 5/3, rho 2 / 1, vx +-0.5, P 2.5, "
+                        "seeded vy noise, PLM (MC) + HLLC, CFL 0.4", "grids": out}
 
 
 def published_bench(args):
