@@ -39,9 +39,9 @@ enum { SHX_RS_HLLC = 0, SHX_RS_HLL = 1 };
 /* Boundary codes (src/hydro/boundary.wgsl:18-23): periodic, outflow, reflecting, outflow only. */
 enum { SHX_BC_PERIODIC = 0, SHX_BC_OUTFLOW = 1, SHX_BC_REFLECT = 2, SHX_BC_OUTFLOW_ONLY = 3 };
 /* Uniform-grid kernel path: PLAIN keeps the reference's passes (rates, clock, predict, tiled
- * update); FUSED is one pass per step (rates of the next step in the epilogue). A flat FP32 grid
- * on the FUSED path runs the 2D kernels of the ensembles (as a batch of one grid): the row march
- * over 60-cell strips or the LDS tile kernel. */
+ * update); FUSED is one pass per step (rates of the next step in the epilogue). A flat grid on
+ * the FUSED path runs the 2D kernels of the ensembles (as a batch of one grid): in FP32 the row
+ * march over 60-cell strips or the LDS tile kernel, in FP64 the tile kernel. */
 enum { SHX_PATH_FUSED = 0, SHX_PATH_PLAIN = 1 };
 
 /* The SimConfig fields the hydro path reads (src/sim.ts:48-264, defaults configFor :645-736). */

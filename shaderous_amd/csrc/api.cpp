@@ -54,12 +54,13 @@ struct shx_sim {
   Params<double> pd;
   UniformOps ops{};
   hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [parity][1 or 2 steps]
-  // A flat FP32 grid on the fused path steps through the 2D kernels as a batch of one grid: the
-  // row march (k_ens_rows, one wave per 60-cell strip) or the LDS tile kernel (k_ens_fused), by
-  // tile. The 3D march (ops.fused) remains its flux-debug form.
+  // A flat grid on the fused path steps through the 2D kernels as a batch of one grid: in FP32
+  // the row march (k_ens_rows, one wave per 60-cell strip) or the LDS tile kernel (k_ens_fused),
+  // by tile; in FP64 the tile kernel's 16 x 16 form. The 3D march (ops.fused) remains its
+  // flux-debug form.
   bool flat2d = false;
   EnsOps eops{};
-  void* einst = nullptr;                  // one EnsInst<float>: this grid's parameters, offset 0
+  void* einst = nullptr;                  // one EnsInst<T>: this grid's parameters, offset 0
   void* etiles[ENS_NTILES] = {};          // the grid's tiles (strips) for every 2D tile shape
   int entiles[ENS_NTILES] = {};
   int etile = ENS_ROWS16;
@@ -258,11 +259,13 @@ int shx_create(const shx_config* cfg, shx_sim** out) {
   }
   if (c.tile[2] > 0) s->zc = c.tile[2];
   if (s->zc > c.n[2]) s->zc = c.n[2];
-  if (s->fused && s->es == 4 && c.n[2] == 1) {
-    s->eops = ens_ops_f32(c.limiter, c.riemann, c.isentropic ? 1 : 0);
+  if (s->fused && c.n[2] == 1) {
+    const int isen = c.isentropic ? 1 : 0;
+    s->eops = s->es == 8 ? ens_ops_f64(c.limiter, c.riemann, isen) : ens_ops_f32(c.limiter, c.riemann, isen);
     s->flat2d = s->eops.fused != nullptr;
+    if (s->es == 8) s->etile = ENS_20x20;   // the FP64 2D kernel's one tile
   }
-  if (s->flat2d && c.tile[0] > 0)
+  if (s->flat2d && s->es == 4 && c.tile[0] > 0)
     for (int k = 0; k < ENS_NTILES; k++) {
       int tx, ty;
       ens_tile_cells(k, tx, ty);
@@ -286,7 +289,7 @@ int shx_create(const shx_config* cfg, shx_sim** out) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess || cus < 1) cus = 256;
     s->egrid = 4 * cus;
-    ok = ok && alloc(&s->einst, sizeof(EnsInst<float>));
+    ok = ok && alloc(&s->einst, s->es == 8 ? sizeof(EnsInst<double>) : sizeof(EnsInst<float>));
     for (int k = 0; k < ENS_NTILES; k++) {
       int tx, ty;
       ens_tile_cells(k, tx, ty);
@@ -304,7 +307,9 @@ int shx_create(const shx_config* cfg, shx_sim** out) {
   (void)hipMemset(s->slot, 0, 64);
   if (s->flat2d) {
     const EnsInst<float> one{s->pf, 0};
-    bool copied = hipMemcpy(s->einst, &one, sizeof one, hipMemcpyHostToDevice) == hipSuccess;
+    const EnsInst<double> one64{s->pd, 0};
+    bool copied = s->es == 8 ? hipMemcpy(s->einst, &one64, sizeof one64, hipMemcpyHostToDevice) == hipSuccess
+                             : hipMemcpy(s->einst, &one, sizeof one, hipMemcpyHostToDevice) == hipSuccess;
     for (int k = 0; k < ENS_NTILES; k++)
       copied = copied && hipMemcpy(s->etiles[k], lists[k].data(), lists[k].size() * sizeof(EnsTile), hipMemcpyHostToDevice) == hipSuccess;
     if (!copied) {

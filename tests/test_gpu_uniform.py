@@ -232,6 +232,34 @@ def test_flat_grids_take_the_2d_kernels_with_the_plain_bits(tile):
         assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]), (tile, pb.id)
 
 
+def test_flat_fp64_grids_take_the_2d_kernel_with_the_plain_bits():
+    """A flat FP64 grid on the fused path runs the FP64 2D tile kernel: the FP64 plain path's bits
+    (MC on periodic KH and outflow quadrants, Koren + isentropic on reflecting quadrants) and the
+    FP64 oracle's values to 1e-12."""
+    cases = [(problems.kelvin_helmholtz(48), {}), (problems.riemann_quadrants(50, seed=5), {"bc": 1}),
+             (problems.riemann_quadrants(37, seed=6, limiter="koren", isentropic=True), {"bc": 2})]
+    for pb, o in cases:
+        if "bc" in o:
+            pb.bc_lo = (o["bc"], o["bc"], 0)
+            pb.bc_hi = (o["bc"], o["bc"], 0)
+        out = []
+        print(f"\n[gpu] flat fp64 {pb.id} {pb.n} {pb.limiter} bc {pb.bc_lo}", flush=True)
+        for path in ("plain", "fused"):
+            cfg = config_for(pb, path=path, precision=64)
+            with Sim(cfg) as sim:
+                sim.set_state(*pb.state(np.float64))
+                sim.step(5)
+                out.append(sim.read_state())
+                if path == "fused":
+                    assert sim.kernel_name() == "k_ens_fused"
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]), pb.id
+        prm = orc_params(pb, cfg)
+        u1, u2 = pb.state(np.float64)
+        wgsl64.run(prm, u1, u2, 5, np.zeros(4))
+        d = l1rel(out[1], (u1, u2))
+        assert max(d.values()) < 1e-12, (pb.id, d)
+
+
 def test_flat_grid_autotune_keeps_the_bits():
     pb = problems.kelvin_helmholtz(200)
     out = []
