@@ -181,7 +181,7 @@ int shx_amr_flux_mismatch(shx_amr* sim, int64_t counts[3], double* worst_coarse_
  * [4] kernel launches per step. */
 int shx_amr_stats(shx_amr* sim, int out[5]);
 /* The refinement indicator of every leaf (AmrSim's flagging pass, src/amr/amr_sim.ts:3029-3060,
- * with BASELINE-style density jumps in place of amr/flag.wgsl's Loehner estimate): out[slot] =
+ * with relative density jumps in place of amr/flag.wgsl's Loehner estimate): out[slot] =
  * the largest |rho_a - rho_b| / min(rho_a, rho_b) between face-adjacent cells of the leaf;
  * NaN for parents. out holds one double per block. */
 int shx_amr_indicator(shx_amr* sim, double* out);

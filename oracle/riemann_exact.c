@@ -1,7 +1,7 @@
 /*
  * TEST INFRASTRUCTURE (the oracle): Toro's exact solution of the 1D ideal-gas Riemann problem,
  * restated from src/reference/riemann_exact.ts:9-71 (Newton iteration for p* from the
- * two-rarefaction guess, then sampling of the self-similar fan). Used to judge shock tubes.
+ * two-rarefaction guess, then sampling of the self-similar fan). Used to check shock tubes.
  *
  * Parity pins (tests/test_oracle.py): Toro's table 4.2 Sod star state p* = 0.30313,
  * u* = 0.92745, rho = 0.26557 between contact and shock; Toro test 3 p* = 460.894,

@@ -3,7 +3,8 @@ and an all-FP64 run of the same kernels on the GPU, per field (density, momentum
 the conserved totals of both. The Sedov 128^3 run to t = 0.05 is the precision study itself
 (gamma 1.4, ambient rho 1 with 1e-4 noise, P 1e-5, E = 1 in r < 3 cells).
 
-Each case records its numbers in gpurun_out/precision.json when that folder exists.
+Each case records its numbers in precision.json in the repository's results folder when that
+folder exists.
 """
 import json
 import os
