@@ -68,6 +68,9 @@ typedef struct {
   int tile[3];         /* fused kernel's column tile (x, y) and z chunk; 0 = autotuned default.
                         * A flat FP32 grid takes the 2D tiles (16,16), (28,12), (28,28) and the
                         * strips (60,8), (60,16) (the default), (60,32) */
+  double org[3];       /* the domain's lower corner (domainLo, src/domain.ts:17-26; Params.org):
+                        * cell (i, j, k) is centred at (i + 1/2) dx + org. Read by the
+                        * diagnostics' centre of mass. Default -n[0] dx / 2 on every axis */
 } shx_config;
 
 typedef struct shx_sim shx_sim;
@@ -89,22 +92,32 @@ int shx_set_state_f64(shx_sim* sim, const double* u1, const double* u2, size_t n
 int shx_read_state(shx_sim* sim, float* u1, float* u2);
 int shx_read_state_f64(shx_sim* sim, double* u1, double* u2);
 
-/* Sim.step (src/sim.ts:3592) / encodeSteps (src/sim.ts:3187-3328): enqueue `count` steps
- * (rates -> clock -> predict -> update each, dt computed and applied on the GPU). */
-int shx_step(shx_sim* sim, int count);
+/* Sim.step (src/sim.ts:3592) / encodeSteps (src/sim.ts:3187-3328, Simulation.encodeSteps
+ * :485): enqueue `count` steps (rates -> clock -> predict -> update each, dt computed and applied
+ * on the GPU) and return at once; *taken (may be null) = the steps enqueued, which is `count`
+ * (the uniform grid waits on no refinement check). */
+int shx_step(shx_sim* sim, int count, int* taken);
 /* Wait for every enqueued step. */
 int shx_sync(shx_sim* sim);
-/* Sim.runToEnd (src/sim.ts:3689-3718): step in batches of `batch` until the clock holds at
- * t_end (dt = 0) or max_steps; *steps = steps taken. SHX_ESTATE on a NaN clock. */
+/* Sim.runToEnd (src/sim.ts:3689-3718): step in batches (from `batch`, never far past the steps
+ * the clock says remain) until the clock holds at t_end (dt = 0, t > 0); *steps = the clock's
+ * banked step count then (clock[3], counted from the state's start, as the reference returns).
+ * SHX_EINVAL without a finite t_end; SHX_ESTATE on a NaN clock or when the count passes
+ * max_steps before t_end (the reference throws for both). */
 int shx_run_to_end(shx_sim* sim, int batch, int max_steps, int* steps);
 /* The Clock struct (src/hydro/common.wgsl:84-96): (dt, t, tc, steps, a, ah, a1, hub, shear,
  * shear_c, hold), as the step about to be taken sees it. */
 int shx_read_clock(shx_sim* sim, double out[11]);
-/* Sim.diagnostics (src/sim.ts:1647-1673, hydro/diagnostics.wgsl:13-98), with the sums in FP64:
- * out[0] mass, [1..3] momentum, [4] kinetic energy, [5] internal energy (E - KE),
- * [6] total energy E, [7] max Mach, [8] cells at the pressure floor, [9] rho*s total,
- * [10..11] rho min/max, [12..13] p min/max. Deterministic (fixed reduction order). */
-int shx_diagnostics(shx_sim* sim, double out[16]);
+/* Sim.diagnostics (src/sim.ts:1647-1673): the 24 channels of hydro/diagnostics.wgsl:13-98 in
+ * the reference's order (DIAG_OPS, src/sim.ts:1808), every cell weighted by its volume
+ * dv = dx^ndim (P.geom.x, sim.ts:1733), summed in FP64 in a fixed order (deterministic):
+ * out[0] mass, [1..3] momentum, [4] kinetic energy, [5] internal energy as the total holds it
+ * (E - KE), [6] gravitational energy (0 on this path), [7] max Mach, [8] floored cells
+ * (rho <= 1.0001 rho_floor or p <= 1.0001 p_floor), [9] Jeans-unresolved cells (0),
+ * [10, 11] rho min/max, [12, 13] temperature (gamma - 1) e / rho min/max, [14, 15] specific
+ * scalar min/max, [16, 17] p min/max, [18, 19] speed min/max, [20..22] sum of rho dv x (x the
+ * cell centre; z = 0 on a flat grid), [23] nuclear heating (0). */
+int shx_diagnostics(shx_sim* sim, double out[24]);
 /* Sim.fluxMismatch (src/sim.ts:3724-3787) with the DEBUG_FLUX override (hydro/update.wgsl:187):
  * every face's mass flux as each of its two cells computed it; *mismatched = faces whose two
  * values differ in any bit, *faces = faces compared. */
@@ -120,6 +133,35 @@ int shx_kernel_name(shx_sim* sim, char* buf, size_t len);
 /* The autotuner: time the fused kernel's tile candidates on the current state (which it
  * restores) and keep the fastest; *chosen = (tx, ty, zc). */
 int shx_autotune(shx_sim* sim, int reps, int chosen[3]);
+
+/* Sim.writeParams (src/sim.ts:2680-2700): take the run-time fields of *cfg (gamma, cfl, t_end,
+ * rho_floor, p_floor, eta_dual, shock_thresh, fluff, org) into the parameter block the steps
+ * read, from the next enqueued step on (the step graphs are rebuilt). The grid, boundaries,
+ * precision, path, device and scheme must equal the sim's (SHX_EINVAL otherwise: a new sim is
+ * needed for those, as the reference recreates its textures). The clock and state are kept. */
+int shx_write_params(shx_sim* sim, const shx_config* cfg);
+/* Sim.buildHydro (src/sim.ts:2434-2560): select the hydro kernels for the limiter, Riemann
+ * solver and reconstruction of *cfg (hydroSlots, sim.ts:1188-1232), with its run-time fields
+ * as shx_write_params; the other fields must equal the sim's. The state and clock are kept. */
+int shx_build_hydro(shx_sim* sim, const shx_config* cfg);
+
+/* ---------------- asynchronous readbacks ----------------
+ * The reference reads state and diagnostics back with mapAsync (Sim.readState / readDiagnostics,
+ * src/sim.ts:3598-3640, 3790-3807): the copy is enqueued when the call is made, and the caller
+ * waits on it later. A readback is that copy: begun on the sim's stream (after every step
+ * enqueued so far), into host memory the readback owns, so the sim may step, re-mesh or be
+ * destroyed meanwhile. `what`: bit 0 the state, bit 1 the diagnostics; the clock always. */
+typedef struct shx_readback shx_readback;
+enum { SHX_READ_STATE = 1, SHX_READ_DIAG = 2 };
+int shx_readback_begin(shx_sim* sim, int what, shx_readback** out);
+/* Wait until the readback's copies are done (any thread). */
+int shx_readback_wait(shx_readback* rb);
+/* After shx_readback_wait: its host data. u1, u2: ncell x 4 values each in the sim's precision
+ * (null without SHX_READ_STATE); clock: the 11 clock values; diag: the 24 diagnostics channels
+ * (null without SHX_READ_DIAG). Any out-pointer may be null. Valid until shx_readback_free. */
+int shx_readback_data(shx_readback* rb, const void** u1, const void** u2, size_t* ncell, int* precision,
+                      const double** clock, const double** diag);
+int shx_readback_free(shx_readback* rb);
 
 /* ---------------- block AMR (AmrSim, src/amr/amr_sim.ts) ----------------
  * A 2:1-balanced block tree (the reference builds and keeps it in BlockTree,
@@ -163,12 +205,19 @@ int shx_amr_set_state(shx_amr* sim, const float* u1, const float* u2, size_t nce
 int shx_amr_set_state_f64(shx_amr* sim, const double* u1, const double* u2, size_t ncell, double time, int steps);
 int shx_amr_read_state(shx_amr* sim, float* u1, float* u2);
 int shx_amr_read_state_f64(shx_amr* sim, double* u1, double* u2);
-/* AmrSim.step / encodeSteps (amr_sim.ts:2709-2947), single global dt (PER_DX clock). */
-int shx_amr_step(shx_amr* sim, int count);
+/* AmrSim.step / encodeSteps (amr_sim.ts:2709-2947), single global dt (PER_DX clock): enqueue
+ * `count` steps; *taken (may be null) = the steps enqueued (`count`: re-meshing is the caller's,
+ * through shx_amr_indicator / shx_amr_regrid, so no step waits on a refinement check). */
+int shx_amr_step(shx_amr* sim, int count, int* taken);
 int shx_amr_sync(shx_amr* sim);
 int shx_amr_read_clock(shx_amr* sim, double out[11]);
-/* AmrSim.diagnostics: as shx_diagnostics over leaves, sums weighted by cell volume (FP64). */
-int shx_amr_diagnostics(shx_amr* sim, double out[16]);
+/* AmrSim.diagnostics (amr/diagnostics.wgsl:13-117): the channels of shx_diagnostics over leaf
+ * cells, each weighted by its own volume h^3 and placed at its centre (blk_pos,
+ * amr/grid.wgsl:172-177). */
+int shx_amr_diagnostics(shx_amr* sim, double out[24]);
+/* A readback of the block grid (as shx_readback_begin): the leaves' state of every slot, the
+ * clock and the diagnostics. */
+int shx_amr_readback_begin(shx_amr* sim, int what, shx_readback** out);
 /* Time `steps` steps pass by pass with HIP events: ms[0] assemble, [1] clock, [2] predict,
  * [3] update, [4] whole step. */
 int shx_amr_profile(shx_amr* sim, int steps, double ms[5]);

@@ -32,7 +32,7 @@ class ShxConfig(C.Structure):
         ("path", C.c_int), ("device", C.c_int),
         ("dx", C.c_double), ("gamma", C.c_double), ("cfl", C.c_double), ("rho_floor", C.c_double),
         ("p_floor", C.c_double), ("eta_dual", C.c_double), ("shock_thresh", C.c_double), ("t_end", C.c_double),
-        ("fluff", C.c_double * 3), ("tile", C.c_int * 3),
+        ("fluff", C.c_double * 3), ("tile", C.c_int * 3), ("org", C.c_double * 3),
     ]
 
 
@@ -66,7 +66,7 @@ def load():
         "shx_set_state_f64": (I, [P, D, D, C.c_size_t, C.c_double, I]),
         "shx_read_state": (I, [P, F, F]),
         "shx_read_state_f64": (I, [P, D, D]),
-        "shx_step": (I, [P, I]),
+        "shx_step": (I, [P, I, C.POINTER(I)]),
         "shx_sync": (I, [P]),
         "shx_run_to_end": (I, [P, I, I, C.POINTER(I)]),
         "shx_read_clock": (I, [P, D]),
@@ -75,6 +75,14 @@ def load():
         "shx_profile": (I, [P, I, D]),
         "shx_kernel_name": (I, [P, C.c_char_p, C.c_size_t]),
         "shx_autotune": (I, [P, I, C.POINTER(I)]),
+        "shx_write_params": (I, [P, C.POINTER(ShxConfig)]),
+        "shx_build_hydro": (I, [P, C.POINTER(ShxConfig)]),
+        "shx_readback_begin": (I, [P, I, C.POINTER(P)]),
+        "shx_amr_readback_begin": (I, [P, I, C.POINTER(P)]),
+        "shx_readback_wait": (I, [P]),
+        "shx_readback_data": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(C.c_size_t), C.POINTER(I),
+                                  C.POINTER(D), C.POINTER(D)]),
+        "shx_readback_free": (I, [P]),
         "shx_amr_create": (I, [C.POINTER(ShxConfig), C.POINTER(ShxAmrLayout), C.POINTER(I), I, C.POINTER(P)]),
         "shx_amr_destroy": (I, [P]),
         "shx_amr_table": (I, [P, C.POINTER(I), C.c_size_t]),
@@ -82,7 +90,7 @@ def load():
         "shx_amr_set_state_f64": (I, [P, D, D, C.c_size_t, C.c_double, I]),
         "shx_amr_read_state": (I, [P, F, F]),
         "shx_amr_read_state_f64": (I, [P, D, D]),
-        "shx_amr_step": (I, [P, I]),
+        "shx_amr_step": (I, [P, I, C.POINTER(I)]),
         "shx_amr_sync": (I, [P]),
         "shx_amr_read_clock": (I, [P, D]),
         "shx_amr_diagnostics": (I, [P, D]),

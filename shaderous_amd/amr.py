@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import ShxAmrLayout, check
-from .sim import Diagnostics, SimConfig
+from .sim import Diagnostics, SimConfig, diagnostics_from
 
 BS = 16
 CELLS = BS ** 3
@@ -72,11 +72,14 @@ class AmrSim:
         self.nblocks = len(blocks)
         self.ncell = self.nblocks * CELLS
 
-    def adapt(self, refine_above: float, coarsen_below: float, max_level: int, periodic=(True, True, True)) -> bool:
+    def adapt(self, refine_above: float, coarsen_below: float, max_level: int, periodic=None) -> bool:
         """One re-mesh from the indicator (amr_tree.adapt_tree): split leaves above
         `refine_above`, merge parents of leaves all below `coarsen_below`. Returns whether the
-        tree changed."""
+        tree changed. `periodic` (per axis) defaults to the run's boundaries: an axis wraps for
+        the 2:1 balance only where both its faces are periodic."""
         from . import amr_tree
+        if periodic is None:
+            periodic = tuple(self.cfg.bc_lo[a] == 0 and self.cfg.bc_hi[a] == 0 for a in range(3))
         new = amr_tree.adapt_tree(self.blocks, self.root, self.indicator(), refine_above, coarsen_below, max_level,
                                   periodic)
         if len(new) == len(self.blocks) and {tuple(b) for b in new.tolist()} == {tuple(b) for b in self.blocks.tolist()}:
@@ -138,8 +141,11 @@ class AmrSim:
                 "u1": u1[sel], "u2": u2[sel], "count": sel.size}
 
     # -- stepping --
-    def step(self, count: int = 1):
-        check(_lib.load().shx_amr_step(self._h, int(count)))
+    def step(self, count: int = 1) -> int:
+        """Enqueue `count` steps (encodeSteps); returns the steps taken."""
+        n = C.c_int(0)
+        check(_lib.load().shx_amr_step(self._h, int(count), C.byref(n)))
+        return n.value
 
     def sync(self):
         check(_lib.load().shx_amr_sync(self._h))
@@ -164,13 +170,9 @@ class AmrSim:
         return taken
 
     def diagnostics(self) -> Diagnostics:
-        out = np.zeros(16)
+        out = np.zeros(24)
         check(_lib.load().shx_amr_diagnostics(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
-        ck = self.clock()
-        return Diagnostics(t=ck[1] + ck[0], dt=ck[0], steps=int(ck[3] + (1 if ck[0] > 0 else 0)), mass=out[0],
-                           momentum=tuple(out[1:4]), e_kin=out[4], e_int=out[5], e_tot=out[6], max_mach=out[7],
-                           floored=out[8], scalar=out[9], rho_range=(out[10], out[11]), p_range=(out[12], out[13]),
-                           raw=out)
+        return diagnostics_from(out, self.clock())
 
     def flux_mismatch(self) -> dict:
         """AmrSim.fluxMismatch: same-level faces between blocks (and how many differ in any bit),

@@ -22,6 +22,8 @@ NODE_DIR = os.path.join(HERE, "node")
 ADDON = os.path.join(NODE_DIR, "shx.node")
 NODE_INCLUDE = "/usr/include/node"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# the C ABI every translation unit includes
+HEADER = os.path.join(os.path.dirname(HERE), "include", "shx.h")
 
 DEVICE_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -66,7 +68,7 @@ def _resources(stderr: str) -> list:
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, src + ".o")
     res = os.path.join(BUILD, src + ".resources.json")
-    deps = [os.path.join(CSRC, f) for f in os.listdir(CSRC)]
+    deps = [os.path.join(CSRC, f) for f in os.listdir(CSRC)] + [HEADER]
     if (os.path.exists(obj) and os.path.exists(res)
             and os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps)):
         return obj

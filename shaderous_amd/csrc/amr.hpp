@@ -11,6 +11,7 @@
 //   lists    slot lists (all blocks, leaves, leaves without / with a refined neighbour), each
 //            also split by level
 #pragma once
+#include "diag.hpp"
 #include "hydro.hpp"
 
 namespace shx {
@@ -75,8 +76,9 @@ struct AmrOps {
                  const void* clk, bool debug);
   void (*rates)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2,
                 unsigned long long* slot);
-  void (*diag)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2, double* part,
-               double* out);
+  // the 24 diagnostics channels over the listed leaves into out (FP64), a record per leaf in part
+  void (*diag)(const AmrLaunch&, const DiagGeo&, const int* list, int count, const void* u1, const void* u2,
+               double* part, double* out);
   // the fused form: the ghost shells (per leaf shell index, the waves of work listed as (shell,
   // slot, face, chunk): work0 the ghost texels, work1 the half steps); the march of every leaf
   // in one launch, from the queue Q

@@ -354,12 +354,13 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
     // a leaf with a refined neighbour: wait for the finer leaves across it (one lane each), then
     // form the means of their fluxes on its coarse-fine faces
     const int d0 = act_blk ? dep_off[s] : 0, nd = act_blk ? dep_off[s + 1] - d0 : 0;
-    // The flags and, after them, the finer leaves' register fluxes are read with device-coherent
-    // loads (no cache-wide invalidation; the writer wrote its registers back before its flag)
+    // The flags are read with agent-scope acquire loads, pairing with the writers' release
+    // fence: what a finer leaf wrote before its flag (its register fluxes) is visible to the
+    // reads after the barrier below, which are device-coherent loads as well
     if (tid < nd) {
       int* flag = done + dep[d0 + tid];
       int spins = 0;
-      while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+      while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
         if (++spins > AMR_WAIT_SPINS) {
           __hip_atomic_store(&qctl[AMR_Q_ERROR], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;

@@ -522,50 +522,28 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
 }
 
 // ---------------- diagnostics (FP64, volume-weighted over leaves) ----------------
-// Channels as the uniform grid's (mass, momentum, kinetic, internal, E, max Mach, floored cells,
-// rho s, rho min/max, p min/max); one workgroup per listed leaf, parts summed in list order.
+// The 24 channels of amr/diagnostics.wgsl:13-117: one workgroup per listed leaf (its cells of
+// volume h^3 at blk_pos, amr/grid.wgsl:172-177), records folded by k_diag_final in list order.
 template <typename T>
-__global__ __launch_bounds__(256) void k_amr_diag(Params<T> P, const int* __restrict__ tab,
+__global__ __launch_bounds__(256) void k_amr_diag(Params<T> P, DiagGeo G, const int* __restrict__ tab,
                                                   const int* __restrict__ list, double dx0,
                                                   const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2,
                                                   double* part) {
-  __shared__ double sh[16][256];
+  __shared__ double sh[4 * DIAG_CH];
   const int s = list[blockIdx.x];
   const double h = dx0 / (double)(1 << bt(tab, s, BT_LEVEL));
-  const double vol = h * h * h;
-  double v[16];
-  for (int c = 0; c < 16; c++) v[c] = 0.0;
-  v[10] = 1e300; v[11] = -1e300; v[12] = 1e300; v[13] = -1e300;
+  const double dv = G.flat ? h * h : h * h * h;
+  const int bx = bt(tab, s, BT_X) * AMR_BS, by = bt(tab, s, BT_X + 1) * AMR_BS, bz = bt(tab, s, BT_X + 2) * AMR_BS;
+  double r[DIAG_CH];
+  diag_init(r);
   for (int c = threadIdx.x; c < AMR_CELLS; c += 256) {
+    const int i = c % AMR_BS, j = (c / AMR_BS) % AMR_BS, k = c / (AMR_BS * AMR_BS);
     const size_t o = (size_t)s * AMR_CELLS + c;
-    T4<T> u1 = U1[o], u2 = U2[o];
-    double rho = (double)u1.x, mx = (double)u1.y, my = (double)u1.z, mz = (double)u1.w, E = (double)u2.x;
-    double ke = 0.5 * (mx * mx + my * my + mz * mz) / rho;
-    Prim<T> w = prim_from_cons(P, u1, u2);
-    double vv = sqrt((double)w.vx * w.vx + (double)w.vy * w.vy + (double)w.vz * w.vz);
-    v[0] += rho; v[1] += mx; v[2] += my; v[3] += mz; v[4] += ke; v[5] += E - ke; v[6] += E;
-    v[7] = fmax(v[7], vv / (double)w.cs);
-    v[8] += (P.gm1 * u2.y <= P.p_floor) ? 1.0 : 0.0;
-    v[9] += (double)u2.z;
-    v[10] = fmin(v[10], (double)w.rho); v[11] = fmax(v[11], (double)w.rho);
-    v[12] = fmin(v[12], (double)w.p); v[13] = fmax(v[13], (double)w.p);
+    const double x = (bx + i + 0.5) * h + G.org[0], y = (by + j + 0.5) * h + G.org[1];
+    const double z = G.flat ? 0.0 : (bz + k + 0.5) * h + G.org[2];
+    diag_cell(P, U1[o], U2[o], dv, x, y, z, r);
   }
-  for (int c = 0; c < 16; c++) sh[c][threadIdx.x] = v[c];
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) {
-      for (int c = 0; c < 16; c++) {
-        double a = sh[c][threadIdx.x], b = sh[c][threadIdx.x + st];
-        sh[c][threadIdx.x] = (c == 7 || c == 11 || c == 13) ? fmax(a, b) : (c == 10 || c == 12) ? fmin(a, b) : a + b;
-      }
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x < 16) {
-    const int c = threadIdx.x;
-    const bool summed = c <= 6 || c == 9;
-    part[blockIdx.x * 16 + c] = summed ? sh[c][0] * vol : sh[c][0];
-  }
+  diag_block(r, sh, part + (size_t)blockIdx.x * DIAG_CH);
 }
 
 // ---------------- launchers ----------------
@@ -690,11 +668,12 @@ struct AmrImpl {
     hipLaunchKernelGGL(k_amr_rates<T>, dim3(count), dim3(256), 0, L.stream, P(L), LV(L), L.tab, list,
                        (const T4<T>*)u1, (const T4<T>*)u2, slot);
   }
-  static void diag(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2, double* part,
-                   double* out) {
-    hipLaunchKernelGGL(k_amr_diag<T>, dim3(count), dim3(256), 0, L.stream, P(L), L.tab, list, L.dx0,
-                       (const T4<T>*)u1, (const T4<T>*)u2, part);
-    hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(1024), 0, L.stream, (const double*)part, count, out);
+  static void diag(const AmrLaunch& L, const DiagGeo& G, const int* list, int count, const void* u1, const void* u2,
+                   double* part, double* out) {
+    if (count > 0)
+      hipLaunchKernelGGL(k_amr_diag<T>, dim3(count), dim3(256), 0, L.stream, P(L), G, L.tab, list, L.dx0,
+                         (const T4<T>*)u1, (const T4<T>*)u2, part);
+    hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(32 * DIAG_CH), 0, L.stream, (const double*)part, count, out);
   }
   static void shell(const AmrLaunch& L, const int* work0, int n0, const int* work1, int n1, const void* u1,
                     const void* u2, void* sh, const void* clk);

@@ -42,8 +42,8 @@ struct shx_sim {
   void* PB = nullptr;
   void* clk = nullptr;       // 16 T: the Clock (11) and spare
   unsigned long long* slot = nullptr;   // reduction slot: max rate of the current state
-  double* diag_part = nullptr;
-  double* diag_out = nullptr;
+  double* diag_part = nullptr;           // diag_blocks records of DIAG_CH channels
+  double* diag_out = nullptr;            // DIAG_CH channels
   int diag_blocks = 256;
   int cur = 0;
   bool fused = false;
@@ -68,6 +68,14 @@ struct shx_sim {
 };
 
 namespace {
+
+DiagGeo diag_geo(const shx_config& c) {
+  DiagGeo G;
+  G.dv = c.ndim == 1 ? c.dx : c.ndim == 2 ? c.dx * c.dx : c.dx * c.dx * c.dx;
+  for (int a = 0; a < 3; a++) G.org[a] = c.org[a];
+  G.flat = c.n[2] == 1;
+  return G;
+}
 
 Launch launch_of(shx_sim* s) {
   Launch L;
@@ -186,9 +194,160 @@ int read_state_t(shx_sim* s, T* u1, T* u2) {
   return SHX_OK;
 }
 
+// The kernels of a configuration's scheme (limiter, Riemann solver, reconstruction): the 3D
+// tables, and for a flat FP32 grid whose tile lists exist the 2D kernels. False if not built.
+bool select_kernels(shx_sim* s, const shx_config& c, bool lists) {
+  UniformOps ops = uniform_ops(c.precision, c.limiter, c.riemann, c.isentropic ? 1 : 0);
+  if (!ops.rates) return false;
+  s->ops = ops;
+  s->flat2d = false;
+  if (lists && c.n[2] == 1) {
+    const int isen = c.isentropic ? 1 : 0;
+    s->eops = s->es == 8 ? ens_ops_f64(c.limiter, c.riemann, isen) : ens_ops_f32(c.limiter, c.riemann, isen);
+    s->flat2d = s->eops.fused != nullptr;
+  }
+  return true;
+}
+
+// New parameter blocks from a configuration (run-time fields), on the device where a kernel
+// reads them from memory (the 2D kernels' instance record).
+int apply_params(shx_sim* s, const shx_config& c) {
+  s->cfg = c;
+  s->pf = make_params<float>(c);
+  s->pd = make_params<double>(c);
+  if (s->einst) {
+    const EnsInst<float> one{s->pf, 0};
+    const EnsInst<double> one64{s->pd, 0};
+    if (s->es == 8) HIPCHK(hipMemcpyAsync(s->einst, &one64, sizeof one64, hipMemcpyHostToDevice, s->stream));
+    else HIPCHK(hipMemcpyAsync(s->einst, &one, sizeof one, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  return SHX_OK;
+}
+
+// What a running sim cannot change: the grid and its storage, the boundaries, the precision,
+// the path and the device (and, unless `scheme`, the limiter, solver and reconstruction).
+bool same_layout(const shx_config& a, const shx_config& b, bool scheme) {
+  for (int k = 0; k < 3; k++)
+    if (a.n[k] != b.n[k] || a.bc_lo[k] != b.bc_lo[k] || a.bc_hi[k] != b.bc_hi[k]) return false;
+  if (a.ndim != b.ndim || a.precision != b.precision || a.path != b.path || a.device != b.device || a.dx != b.dx)
+    return false;
+  if (!scheme && (a.limiter != b.limiter || a.riemann != b.riemann || (a.isentropic != 0) != (b.isentropic != 0)))
+    return false;
+  return true;
+}
+
+int reconfigure(shx_sim* s, const shx_config* cfg, bool scheme) {
+  if (!s || !cfg) return fail(SHX_EINVAL, "null argument");
+  if (!same_layout(s->cfg, *cfg, scheme))
+    return fail(SHX_EINVAL, scheme ? "the grid, boundaries, precision, path and device of a sim are fixed"
+                                   : "the grid, boundaries, precision, path, device and scheme of a sim are fixed "
+                                     "(shx_build_hydro changes the scheme)");
+  if (!(cfg->gamma > 1) || !(cfg->cfl > 0)) return fail(SHX_EINVAL, "gamma must exceed 1 and cfl be positive");
+  shx_config c = *cfg;
+  for (int k = 0; k < 3; k++) c.tile[k] = s->cfg.tile[k];
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (scheme) {
+    shx_sim probe = {};
+    probe.es = s->es;
+    if (!select_kernels(&probe, c, false))
+      return fail(SHX_EINVAL, "no kernels built for precision %d limiter %d riemann %d isentropic %d", c.precision,
+                  c.limiter, c.riemann, c.isentropic);
+    select_kernels(s, c, s->etiles[0] != nullptr);
+  }
+  int rc = apply_params(s, c);
+  if (rc) return rc;
+  // the rate of the current state depends on gamma and ndim
+  rc = reset_rates(s);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return build_graphs(s);
+}
+
 }  // namespace
 
+namespace shx {
+int readback_begin(hipStream_t stream, int what, int es, size_t ncell, const void* U1, const void* U2, const void* clk,
+                   const double* diag, shx_readback** out) {
+  *out = nullptr;
+  shx_readback* r = new shx_readback();
+  r->ncell = (what & SHX_READ_STATE) ? ncell : 0;
+  r->es = es;
+  r->what = what;
+  const size_t sb = r->ncell * 4 * es;
+  const size_t bytes = 2 * sb + 16 * 8 + DIAG_CH * 8;
+  if (hipHostMalloc((void**)&r->host, bytes, hipHostMallocDefault) != hipSuccess) {
+    delete r;
+    return fail(SHX_ENOMEM, "pinned host allocation of %zu bytes for a readback failed", bytes);
+  }
+  hipError_t e = hipEventCreateWithFlags(&r->ev, hipEventDisableTiming);
+  if (e == hipSuccess && sb) e = hipMemcpyAsync(r->host, U1, sb, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess && sb) e = hipMemcpyAsync(r->host + sb, U2, sb, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(r->host + 2 * sb, clk, 16 * es, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess && (what & SHX_READ_DIAG))
+    e = hipMemcpyAsync(r->host + 2 * sb + 16 * 8, diag, DIAG_CH * 8, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipEventRecord(r->ev, stream);
+  if (e != hipSuccess) {
+    // nothing may still write into the host memory once it is freed
+    (void)hipStreamSynchronize(stream);
+    shx_readback_free(r);
+    return fail(SHX_EDEVICE, "enqueueing a readback: %s", hipGetErrorString(e));
+  }
+  *out = r;
+  return SHX_OK;
+}
+}  // namespace shx
+
 extern "C" {
+
+int shx_readback_wait(shx_readback* r) {
+  if (!r) return fail(SHX_EINVAL, "null readback");
+  if (r->done) return SHX_OK;
+  HIPCHK(hipEventSynchronize(r->ev));
+  const size_t sb = r->ncell * 4 * r->es;
+  const unsigned char* c = r->host + 2 * sb;
+  for (int i = 0; i < 11; i++) r->clock[i] = r->es == 8 ? ((const double*)c)[i] : (double)((const float*)c)[i];
+  if (r->what & SHX_READ_DIAG) memcpy(r->diag, r->host + 2 * sb + 16 * 8, sizeof r->diag);
+  r->done = true;
+  return SHX_OK;
+}
+
+int shx_readback_data(shx_readback* r, const void** u1, const void** u2, size_t* ncell, int* precision,
+                      const double** clock, const double** diag) {
+  if (!r) return fail(SHX_EINVAL, "null readback");
+  if (!r->done) return fail(SHX_EINVAL, "the readback has not been waited for");
+  const size_t sb = r->ncell * 4 * r->es;
+  if (u1) *u1 = r->ncell ? r->host : nullptr;
+  if (u2) *u2 = r->ncell ? r->host + sb : nullptr;
+  if (ncell) *ncell = r->ncell;
+  if (precision) *precision = r->es * 8;
+  if (clock) *clock = r->clock;
+  if (diag) *diag = (r->what & SHX_READ_DIAG) ? r->diag : nullptr;
+  return SHX_OK;
+}
+
+int shx_readback_free(shx_readback* r) {
+  if (!r) return SHX_OK;
+  // a readback freed before it was waited for: its copies still target the host memory
+  if (!r->done && r->ev) (void)hipEventSynchronize(r->ev);
+  if (r->ev) (void)hipEventDestroy(r->ev);
+  if (r->host) (void)hipHostFree(r->host);
+  delete r;
+  return SHX_OK;
+}
+
+int shx_readback_begin(shx_sim* s, int what, shx_readback** out) {
+  if (!s || !out) return fail(SHX_EINVAL, "null argument");
+  if (what & SHX_READ_DIAG) {
+    Launch L = launch_of(s);
+    s->ops.diag(L, diag_geo(s->cfg), s->U1[s->cur], s->U2[s->cur], s->diag_part, s->diag_blocks, s->diag_out);
+    HIPCHK(hipGetLastError());
+  }
+  return readback_begin(s->stream, what, s->es, s->ncell, s->U1[s->cur], s->U2[s->cur], s->clk, s->diag_out, out);
+}
+
+int shx_write_params(shx_sim* s, const shx_config* cfg) { return reconfigure(s, cfg, false); }
+int shx_build_hydro(shx_sim* s, const shx_config* cfg) { return reconfigure(s, cfg, true); }
 
 const char* shx_last_error(void) { return g_err.c_str(); }
 
@@ -220,6 +379,7 @@ void shx_config_default(shx_config* c, int nx, int ny, int nz) {
   c->eta_dual = 1e-3;
   c->shock_thresh = 0.5;
   c->t_end = 1e30;
+  for (int a = 0; a < 3; a++) c->org[a] = -0.5 * nx * c->dx;
 }
 
 int shx_create(const shx_config* cfg, shx_sim** out) {
@@ -282,8 +442,8 @@ int shx_create(const shx_config* cfg, shx_sim** out) {
     const size_t zb = (size_t)2 * c.n[0] * c.n[1] * 4 * s->es;
     ok = alloc(&s->PA, zb) && alloc(&s->PB, zb);
   }
-  ok = ok && alloc(&s->clk, 16 * 8) && alloc((void**)&s->slot, 64) && alloc((void**)&s->diag_part, s->diag_blocks * 16 * 8) &&
-       alloc((void**)&s->diag_out, 16 * 8);
+  ok = ok && alloc(&s->clk, 16 * 8) && alloc((void**)&s->slot, 64) && alloc((void**)&s->diag_part, (size_t)s->diag_blocks * DIAG_CH * 8) &&
+       alloc((void**)&s->diag_out, DIAG_CH * 8);
   std::vector<EnsTile> lists[ENS_NTILES];
   if (s->flat2d) {
     int cus = 0;
@@ -346,8 +506,10 @@ int shx_set_state_f64(shx_sim* s, const double* u1, const double* u2, size_t nce
 int shx_read_state(shx_sim* s, float* u1, float* u2) { return read_state_t<float>(s, u1, u2); }
 int shx_read_state_f64(shx_sim* s, double* u1, double* u2) { return read_state_t<double>(s, u1, u2); }
 
-int shx_step(shx_sim* s, int count) {
+int shx_step(shx_sim* s, int count, int* taken) {
+  if (taken) *taken = 0;
   if (!s || count < 0) return fail(SHX_EINVAL, "bad argument");
+  const int want = count;
   while (count >= 2) {
     HIPCHK(hipGraphLaunch(s->graph[s->cur][1], s->stream));
     count -= 2;
@@ -356,6 +518,7 @@ int shx_step(shx_sim* s, int count) {
     HIPCHK(hipGraphLaunch(s->graph[s->cur][0], s->stream));
     s->cur = 1 - s->cur;
   }
+  if (taken) *taken = want;
   return SHX_OK;
 }
 
@@ -376,33 +539,18 @@ int shx_read_clock(shx_sim* s, double out[11]) {
 
 int shx_run_to_end(shx_sim* s, int batch, int max_steps, int* steps) {
   if (!s) return fail(SHX_EINVAL, "null sim");
-  if (batch < 1) batch = 64;
-  int taken = 0;
-  double clk[11];
-  int rc = shx_read_clock(s, clk);
-  if (rc) return rc;
-  int start = (int)clk[3];
-  while (taken < max_steps) {
-    int n = batch < max_steps - taken ? batch : max_steps - taken;
-    rc = shx_step(s, n);
-    if (rc) return rc;
-    rc = shx_read_clock(s, clk);
-    if (rc) return rc;
-    if (!isfinite(clk[0]) || !isfinite(clk[1])) return fail(SHX_ESTATE, "the clock went non-finite (t = %g, dt = %g)", clk[1], clk[0]);
-    taken += n;
-    // the clock banks a step when the next one starts: steps taken = banked + (dt > 0)
-    if (clk[0] == 0) break;
-  }
-  if (steps) *steps = (int)clk[3] + (clk[0] > 0 ? 1 : 0) - start;
-  return SHX_OK;
+  if (!(s->cfg.t_end < 1e30)) return fail(SHX_EINVAL, "run_to_end needs a finite t_end");
+  return run_to_end_loop(
+      [s](int n) { return shx_step(s, n, nullptr); }, [s](double* c) { return shx_read_clock(s, c); }, s->cfg.t_end,
+      batch, max_steps, steps);
 }
 
-int shx_diagnostics(shx_sim* s, double out[16]) {
+int shx_diagnostics(shx_sim* s, double out[24]) {
   if (!s || !out) return fail(SHX_EINVAL, "bad argument");
   Launch L = launch_of(s);
-  s->ops.diag(L, s->U1[s->cur], s->U2[s->cur], s->diag_part, s->diag_blocks, s->diag_out);
+  s->ops.diag(L, diag_geo(s->cfg), s->U1[s->cur], s->U2[s->cur], s->diag_part, s->diag_blocks, s->diag_out);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, s->diag_out, 16 * 8, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(out, s->diag_out, DIAG_CH * 8, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return SHX_OK;
 }

@@ -353,6 +353,14 @@ int set_state_t(shx_amr* s, const T* u1, const T* u2, size_t ncell, double time,
 
 // After a synchronisation: a dependency wait of the fused march that ran out (the step's
 // coarse-fine means were formed without the finer fluxes) is an error, reported once.
+DiagGeo diag_geo(const shx_config& c) {
+  DiagGeo G;
+  G.dv = 0;   // per block: h^ndim
+  for (int a = 0; a < 3; a++) G.org[a] = c.org[a];
+  G.flat = c.n[2] == 1 || c.ndim < 3;
+  return G;
+}
+
 int check_queue(shx_amr* s) {
   if (!s->d_qctl) return SHX_OK;
   int w[AMR_Q_WORDS];
@@ -501,8 +509,8 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
     ok = ok && alloc((void**)&s->d_shw[part], (shw[part].empty() ? 4 : shw[part].size()) * sizeof(int));
   }
   ok = ok && alloc((void**)&s->d_tab, s->tab.size() * sizeof(int)) && alloc((void**)&s->d_lists, lists.size() * sizeof(int));
-  ok = ok && alloc(&s->clk, 16 * 8) && alloc((void**)&s->slot, 64) && alloc((void**)&s->diag_part, (size_t)nblocks * 16 * 8) &&
-       alloc((void**)&s->diag_out, 16 * 8);
+  ok = ok && alloc(&s->clk, 16 * 8) && alloc((void**)&s->slot, 64) && alloc((void**)&s->diag_part, (size_t)nblocks * DIAG_CH * 8) &&
+       alloc((void**)&s->diag_out, DIAG_CH * 8);
   if (!ok) {
     shx_amr_destroy(s);
     return fail(SHX_ENOMEM, "device allocation failed for %d blocks (%.2f GB of state and atlases)", nblocks,
@@ -568,11 +576,13 @@ int shx_amr_set_state_f64(shx_amr* s, const double* u1, const double* u2, size_t
 int shx_amr_read_state(shx_amr* s, float* u1, float* u2) { return read_state_t<float>(s, u1, u2); }
 int shx_amr_read_state_f64(shx_amr* s, double* u1, double* u2) { return read_state_t<double>(s, u1, u2); }
 
-int shx_amr_step(shx_amr* s, int count) {
+int shx_amr_step(shx_amr* s, int count, int* taken) {
+  if (taken) *taken = 0;
   if (!s || count < 0) return fail(SHX_EINVAL, "bad argument");
   for (int i = 0; i < count; i++) {
     HIPCHK(hipGraphLaunch(s->graph[s->cur], s->stream));
     s->cur = 1 - s->cur;
+    if (taken) *taken = i + 1;
   }
   return SHX_OK;
 }
@@ -592,14 +602,26 @@ int shx_amr_read_clock(shx_amr* s, double out[11]) {
   return SHX_OK;
 }
 
-int shx_amr_diagnostics(shx_amr* s, double out[16]) {
+int shx_amr_diagnostics(shx_amr* s, double out[24]) {
   if (!s || !out) return fail(SHX_EINVAL, "bad argument");
   const AmrLaunch L = launch_of(s);
-  s->ops.diag(L, lp(s, s->leaf), s->leaf.count, s->U1[s->cur], s->U2[s->cur], s->diag_part, s->diag_out);
+  s->ops.diag(L, diag_geo(s->cfg), lp(s, s->leaf), s->leaf.count, s->U1[s->cur], s->U2[s->cur], s->diag_part,
+              s->diag_out);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, s->diag_out, 16 * 8, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(out, s->diag_out, DIAG_CH * 8, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
-  return SHX_OK;
+  return check_queue(s);
+}
+
+int shx_amr_readback_begin(shx_amr* s, int what, shx_readback** out) {
+  if (!s || !out) return fail(SHX_EINVAL, "null argument");
+  if (what & SHX_READ_DIAG) {
+    const AmrLaunch L = launch_of(s);
+    s->ops.diag(L, diag_geo(s->cfg), lp(s, s->leaf), s->leaf.count, s->U1[s->cur], s->U2[s->cur], s->diag_part,
+                s->diag_out);
+    HIPCHK(hipGetLastError());
+  }
+  return readback_begin(s->stream, what, s->es, s->ncell, s->U1[s->cur], s->U2[s->cur], s->clk, s->diag_out, out);
 }
 
 int shx_amr_indicator(shx_amr* s, double* out) {
@@ -615,6 +637,8 @@ int shx_amr_indicator(shx_amr* s, double* out) {
     rc = SHX_EDEVICE;
   (void)hipFree(d);
   if (rc != SHX_OK) return fail(rc, "the indicator pass failed");
+  rc = check_queue(s);
+  if (rc != SHX_OK) return rc;
   for (int i = 0; i < s->nblocks; i++) {
     const bool leaf = (s->tab[(size_t)i * AMR_BT + BT_FLAGS] & 2) != 0;
     const double v = s->es == 8 ? ((const double*)h.data())[i] : (double)((const float*)h.data())[i];
@@ -661,9 +685,13 @@ int shx_amr_regrid(shx_amr* s, const int* blocks, int nblocks) {
     if (p >= 0 && is_leaf(p)) { map.insert(map.end(), {j, 2, p, (x & 1) | ((y & 1) << 1) | ((z & 1) << 2)}); continue; }
     return fail(SHX_EINVAL, "new leaf (%d, %d, %d, %d) is more than one level from the current tree", L, x, y, z);
   }
+  // a state the last march left unreliable is not carried onto the new tree
+  HIPCHK(hipStreamSynchronize(s->stream));
+  int rc = check_queue(s);
+  if (rc != SHX_OK) return rc;
   // the new tree's sim (validates the tree and allocates its memory), then the state moved over
   shx_amr* ns = nullptr;
-  int rc = shx_amr_create(&s->cfg, &s->lay, blocks, nblocks, &ns);
+  rc = shx_amr_create(&s->cfg, &s->lay, blocks, nblocks, &ns);
   if (rc != SHX_OK) return rc;
   int* d_map = nullptr;
   const int count = (int)(map.size() / 4);

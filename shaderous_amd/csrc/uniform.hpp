@@ -2,6 +2,7 @@
 // table per (precision, limiter, Riemann solver, isentropic) combination.
 #pragma once
 #include <algorithm>
+#include "diag.hpp"
 #include "hydro.hpp"
 
 namespace shx {
@@ -38,7 +39,8 @@ struct UniformOps {
   void (*fused)(const Launch&, int tile, const void* u1, const void* u2, void* o1, void* o2, const void* clk,
                 const void* za, const void* zb, int zc, unsigned long long* next_slot, bool debug);
   void (*zplanes)(const Launch&, const void* u1, const void* u2, void* za, void* zb, const void* clk);
-  void (*diag)(const Launch&, const void* u1, const void* u2, double* part, int nblocks, double* out);
+  // the 24 diagnostics channels of the state into out (FP64), per-workgroup records in part
+  void (*diag)(const Launch&, const DiagGeo&, const void* u1, const void* u2, double* part, int nblocks, double* out);
 };
 
 // Each translation unit instantiates a subset; a null table means "not built".

@@ -576,63 +576,23 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 4 : 1)) void k_fused(
 namespace shx {
 
 // ---------------- diagnostics (FP64 sums) ----------------
-// Per block: mass, momentum (3), kinetic, internal (E - KE), total E, max Mach, floored cells,
-// rho s, rho min/max, p min/max. Sums in double; a second pass adds the blocks in order.
+// One record of the 24 channels per workgroup over a fixed set of cells (a grid-stride loop),
+// folded by k_diag_final in workgroup order.
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag(Params<T> P, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2,
-                                              int ncell, double* part) {
-  __shared__ double sh[16][256];
-  double v[16];
-  for (int c = 0; c < 16; c++) v[c] = 0.0;
-  v[10] = 1e300; v[11] = -1e300; v[12] = 1e300; v[13] = -1e300;
+__global__ __launch_bounds__(256) void k_diag(Params<T> P, DiagGeo G, const T4<T>* __restrict__ U1,
+                                              const T4<T>* __restrict__ U2, int ncell, double* part) {
+  __shared__ double sh[4 * DIAG_CH];
+  double r[DIAG_CH];
+  diag_init(r);
+  const int nx = P.n[0], ny = P.n[1];
+  const double dx = (double)P.dx;
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < ncell; o += gridDim.x * blockDim.x) {
-    T4<T> u1 = U1[o], u2 = U2[o];
-    double rho = (double)u1.x, mx = (double)u1.y, my = (double)u1.z, mz = (double)u1.w, E = (double)u2.x;
-    double ke = 0.5 * (mx * mx + my * my + mz * mz) / rho;
-    Prim<T> w = prim_from_cons(P, u1, u2);
-    double vv = sqrt((double)w.vx * w.vx + (double)w.vy * w.vy + (double)w.vz * w.vz);
-    v[0] += rho; v[1] += mx; v[2] += my; v[3] += mz; v[4] += ke; v[5] += E - ke; v[6] += E;
-    v[7] = fmax(v[7], vv / (double)w.cs);
-    v[8] += (P.gm1 * u2.y <= P.p_floor) ? 1.0 : 0.0;
-    v[9] += (double)u2.z;
-    v[10] = fmin(v[10], (double)w.rho); v[11] = fmax(v[11], (double)w.rho);
-    v[12] = fmin(v[12], (double)w.p); v[13] = fmax(v[13], (double)w.p);
+    const int i = o % nx, j = (o / nx) % ny, k = o / (nx * ny);
+    const double x = (i + 0.5) * dx + G.org[0], y = (j + 0.5) * dx + G.org[1];
+    const double z = G.flat ? 0.0 : (k + 0.5) * dx + G.org[2];
+    diag_cell(P, U1[o], U2[o], G.dv, x, y, z, r);
   }
-  for (int c = 0; c < 16; c++) sh[c][threadIdx.x] = v[c];
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      for (int c = 0; c < 16; c++) {
-        double a = sh[c][threadIdx.x], b = sh[c][threadIdx.x + s];
-        sh[c][threadIdx.x] = (c == 7 || c == 11 || c == 13) ? fmax(a, b) : (c == 10 || c == 12) ? fmin(a, b) : a + b;
-      }
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x < 16) part[blockIdx.x * 16 + threadIdx.x] = sh[threadIdx.x][0];
-}
-
-// The 16 channels over the partial rows: 64 groups of consecutive rows per channel, each folded
-// in row order, then the 64 group results in group order (a fixed order: deterministic).
-// Launch: one workgroup of 1024 threads.
-template <typename T>
-__global__ __launch_bounds__(1024) void k_diag_final(const double* part, int nblocks, double* out) {
-  __shared__ double grp[64][17];
-  const int c = threadIdx.x & 15, g = threadIdx.x >> 4;
-  auto fold = [c](double a, double x) {
-    return (c == 7 || c == 11 || c == 13) ? fmax(a, x) : (c == 10 || c == 12) ? fmin(a, x) : a + x;
-  };
-  const double id = (c == 10 || c == 12) ? 1e300 : (c == 11 || c == 13) ? -1e300 : 0.0;
-  const int per = (nblocks + 63) / 64, b0 = g * per, b1 = min(b0 + per, nblocks);
-  double a = id;
-  for (int b = b0; b < b1; b++) a = fold(a, part[b * 16 + c]);
-  grp[g][c] = a;
-  __syncthreads();
-  if (g == 0) {
-    double r = id;
-    for (int k = 0; k < 64; k++) r = fold(r, grp[k][c]);
-    out[c] = r;
-  }
+  diag_block(r, sh, part + (size_t)blockIdx.x * DIAG_CH);
 }
 
 // ---------------- launchers ----------------
@@ -729,11 +689,12 @@ struct UniformImpl {
     hipLaunchKernelGGL((k_zplanes<T, LIM, ISEN>), g, b, 0, L.stream, P, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)za,
                        (T4<T>*)zb, (const T*)clk);
   }
-  static void diag(const Launch& L, const void* u1, const void* u2, double* part, int nblocks, double* out) {
+  static void diag(const Launch& L, const DiagGeo& G, const void* u1, const void* u2, double* part, int nblocks,
+                   double* out) {
     const Params<T>& P = *static_cast<const Params<T>*>(L.params);
-    hipLaunchKernelGGL(k_diag<T>, dim3(nblocks), dim3(256), 0, L.stream, P, (const T4<T>*)u1, (const T4<T>*)u2,
+    hipLaunchKernelGGL(k_diag<T>, dim3(nblocks), dim3(256), 0, L.stream, P, G, (const T4<T>*)u1, (const T4<T>*)u2,
                        (int)L.ncell, part);
-    hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(1024), 0, L.stream, (const double*)part, nblocks, out);
+    hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(32 * DIAG_CH), 0, L.stream, (const double*)part, nblocks, out);
   }
   static UniformOps ops() {
     UniformOps o;

@@ -26,6 +26,12 @@ export interface SimConfig {
   path: 'fused' | 'plain';
   device: number;
   tile: [number, number, number];
+  /** the domain's lower corner (default -box/2 on every axis) */
+  org?: [number, number, number];
+  /** the state reset() loads */
+  initialState?: { u1: State; u2: State };
+  /** the clock a reset() starts from */
+  restart?: { time: number; steps: number };
 }
 
 export interface Diagnostics {
@@ -41,7 +47,12 @@ export interface Diagnostics {
   eTot: number;
   maxMach: number;
   floored: number;
-  scalar: number;
+  jeansUnresolved: number;
+  nuclearHeating: number;
+  com: [number, number, number];
+  perturber: null;
+  masses: [];
+  /** (min, max) of rho, temperature, specific scalar, pressure and speed */
   ranges: [number, number][];
 }
 
@@ -52,39 +63,78 @@ export interface AmrTree {
   compensated?: boolean;
 }
 
-export declare class Sim {
-  static create(cfg: SimConfig): Sim;
+/** The members of the reference's `Simulation` (src/sim.ts:465-627) this path serves. */
+interface Simulation {
   readonly ncell: number;
-  setState(u1: State, u2: State, time?: number, steps?: number): void;
-  readState(): { u1: State; u2: State };
-  readStateAsync(): Promise<{ u1: State; u2: State }>;
-  step(count?: number): void;
+  readonly nz: number;
+  readonly preparing: boolean;
+  encodeSteps(enc: unknown, count: number, mayHold?: boolean, speculate?: number): number;
   submitted(): void;
-  runToEnd(batch?: number, maxSteps?: number): number;
+  stepsResume(): Promise<void> | null;
+  step(count?: number): void;
+  sync(): void;
   clock(): Float64Array;
-  diagnostics(): Diagnostics;
-  readDiagnostics(): Promise<Diagnostics>;
-  fluxMismatch(): { mismatched: number; faces: number };
-  profile(steps?: number): { rates: number; clock: number; predict: number; update: number; step: number };
+  setState(u1: State, u2: State, rhoMean?: number): void;
+  /** The state and the clock it starts from (a restart). */
+  load(u1: State, u2: State, time?: number, steps?: number): void;
+  reset(): void;
+  readState(): Promise<{ u1: State; u2: State }>;
+  readStateAsync(): Promise<{ u1: State; u2: State }>;
+  diagnostics(): Promise<Diagnostics>;
+  readDiagnostics(): Promise<Diagnostics | null>;
   destroy(): void;
 }
 
-export declare class AmrSim {
+export declare class Sim implements Simulation {
+  static create(cfg: SimConfig): Sim;
+  readonly ncell: number;
+  readonly nz: number;
+  readonly preparing: boolean;
+  encodeSteps(enc: unknown, count: number, mayHold?: boolean, speculate?: number): number;
+  submitted(): void;
+  stepsResume(): Promise<void> | null;
+  step(count?: number): void;
+  sync(): void;
+  clock(): Float64Array;
+  setState(u1: State, u2: State, rhoMean?: number): void;
+  load(u1: State, u2: State, time?: number, steps?: number): void;
+  reset(): void;
+  readState(): Promise<{ u1: State; u2: State }>;
+  readStateAsync(): Promise<{ u1: State; u2: State }>;
+  diagnostics(): Promise<Diagnostics>;
+  readDiagnostics(): Promise<Diagnostics | null>;
+  runToEnd(batch?: number, maxSteps?: number): Promise<number>;
+  writeParams(): void;
+  buildHydro(): Promise<void>;
+  fluxMismatch(): Promise<{ mismatched: number; faces: number }>;
+  profile(steps?: number): Promise<{ rates: number; clock: number; predict: number; update: number; step: number }>;
+  destroy(): void;
+}
+
+export declare class AmrSim implements Simulation {
   static create(cfg: SimConfig, tree: AmrTree): AmrSim;
   readonly nblocks: number;
   readonly ncell: number;
+  readonly nz: number;
+  readonly preparing: boolean;
   table(): Int32Array;
   stats(): { blocks: number; leaves: number; matchLeaves: number; levels: number; launchesPerStep: number };
   indicator(): Float64Array;
   regrid(blocks: Int32Array): void;
-  setState(u1: State, u2: State, time?: number, steps?: number): void;
-  readState(): { u1: State; u2: State };
-  readStateAsync(): Promise<{ u1: State; u2: State }>;
-  step(count?: number): void;
+  encodeSteps(enc: unknown, count: number, mayHold?: boolean, speculate?: number): number;
   submitted(): void;
+  stepsResume(): Promise<void> | null;
+  step(count?: number): void;
+  sync(): void;
   clock(): Float64Array;
-  diagnostics(): Diagnostics;
-  profile(steps?: number): { assemble: number; clock: number; predict: number; update: number; step: number };
+  setState(u1: State, u2: State, rhoMean?: number): void;
+  load(u1: State, u2: State, time?: number, steps?: number): void;
+  reset(): void;
+  readState(): Promise<{ u1: State; u2: State }>;
+  readStateAsync(): Promise<{ u1: State; u2: State }>;
+  diagnostics(): Promise<Diagnostics>;
+  readDiagnostics(): Promise<Diagnostics | null>;
+  profile(steps?: number): Promise<{ assemble: number; clock: number; predict: number; update: number; step: number }>;
   destroy(): void;
 }
 

@@ -5,9 +5,12 @@
  * throws a JS Error carrying shx_last_error() on a failure status (the reference throws Error for
  * refusals, src/sim.ts:2289-2311). Handles are napi externals whose finalizer destroys the sim if
  * JS drops it without destroy(). State arrays are Float32Array (precision 32) or Float64Array
- * (precision 64), copied in and out. readStateAsync / diagnosticsAsync run the blocking readback
- * on a worker thread (napi_create_async_work) and resolve a Promise, as the reference's mapAsync
- * readbacks do (src/sim.ts:3790-3807).
+ * (precision 64), copied in and out. readStateAsync / diagnosticsAsync enqueue the readback's
+ * copies on the sim's stream when they are called (shx_readback_begin: the state as the steps
+ * enqueued so far leave it, into host memory the readback owns), wait for them on a worker
+ * thread (napi_create_async_work) and resolve a Promise, as the reference's mapAsync readbacks
+ * do (src/sim.ts:3598-3640, 3790-3807). The worker touches only the readback, so the sim may
+ * step, re-mesh or be destroyed while one is pending.
  */
 #include <node_api.h>
 #include <stdio.h>
@@ -164,6 +167,7 @@ static void config_from(napi_env env, napi_value o, shx_config* c) {
   c->t_end = num_prop(env, o, "tEnd", c->t_end);
   dbl3_prop(env, o, "fluff", c->fluff);
   int3_prop(env, o, "tile", c->tile);
+  dbl3_prop(env, o, "org", c->org);
 }
 
 /* A typed array's data pointer, checked against the element type and length wanted. */
@@ -300,9 +304,27 @@ static napi_value js_step(napi_env env, napi_callback_info info) {
   if (!h) return NULL;
   int32_t count;
   NAPI_CALL(env, napi_get_value_int32(env, argv[1], &count));
-  SHX_CALL(env, shx_step((shx_sim*)h->p, count));
+  int taken = 0;
+  SHX_CALL(env, shx_step((shx_sim*)h->p, count, &taken));
+  napi_value out;
+  NAPI_CALL(env, napi_create_int32(env, taken, &out));
+  return out;
+}
+
+/* writeParams / buildHydro (src/sim.ts:2680-2700, 2434-2560): the facade's current config. */
+static napi_value reconfigure(napi_env env, napi_callback_info info, int scheme) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Handle* h = get_handle(env, argv[0], 0);
+  if (!h) return NULL;
+  shx_config c;
+  config_from(env, argv[1], &c);
+  if (scheme) SHX_CALL(env, shx_build_hydro((shx_sim*)h->p, &c));
+  else SHX_CALL(env, shx_write_params((shx_sim*)h->p, &c));
   return undefined(env);
 }
+static napi_value js_write_params(napi_env env, napi_callback_info info) { return reconfigure(env, info, 0); }
+static napi_value js_build_hydro(napi_env env, napi_callback_info info) { return reconfigure(env, info, 1); }
 
 static napi_value js_sync(napi_env env, napi_callback_info info) {
   napi_value argv[1];
@@ -343,9 +365,9 @@ static napi_value js_diagnostics(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 1, argv)) return NULL;
   Handle* h = get_handle(env, argv[0], 0);
   if (!h) return NULL;
-  double out[16];
+  double out[24];
   SHX_CALL(env, shx_diagnostics((shx_sim*)h->p, out));
-  return f64_array(env, out, 16);
+  return f64_array(env, out, 24);
 }
 
 static napi_value js_flux_mismatch(napi_env env, napi_callback_info info) {
@@ -375,82 +397,83 @@ static napi_value js_profile(napi_env env, napi_callback_info info) {
 typedef struct {
   napi_async_work work;
   napi_deferred deferred;
-  napi_ref keep;       /* keeps the handle alive while the worker runs */
-  Handle* h;
-  int kind;            /* 0 read state, 1 diagnostics */
+  shx_readback* rb;    /* begun on the JS thread; the worker only waits for it */
+  int kind;            /* 0 state, 1 diagnostics */
   int rc;
   char err[512];
-  void* d1;
-  void* d2;
-  double diag[16];
 } Async;
 
 static void async_execute(napi_env env, void* data) {
   (void)env;
   Async* a = (Async*)data;
-  if (a->kind == 0) {
-    if (a->h->amr)
-      a->rc = a->h->precision == 64 ? shx_amr_read_state_f64((shx_amr*)a->h->p, a->d1, a->d2)
-                                    : shx_amr_read_state((shx_amr*)a->h->p, a->d1, a->d2);
-    else
-      a->rc = a->h->precision == 64 ? shx_read_state_f64((shx_sim*)a->h->p, a->d1, a->d2)
-                                    : shx_read_state((shx_sim*)a->h->p, a->d1, a->d2);
-  } else {
-    a->rc = a->h->amr ? shx_amr_diagnostics((shx_amr*)a->h->p, a->diag) : shx_diagnostics((shx_sim*)a->h->p, a->diag);
-  }
+  a->rc = shx_readback_wait(a->rb);
   if (a->rc != SHX_OK) snprintf(a->err, sizeof a->err, "libshx error %d: %s", a->rc, shx_last_error());
+}
+
+/* {u1, u2, clock} or {diag, clock}: the clock read with the data, after the same steps. */
+static napi_value readback_value(napi_env env, Async* a) {
+  const void *u1, *u2;
+  const double *clock, *diag;
+  size_t ncell;
+  int precision;
+  if (shx_readback_data(a->rb, &u1, &u2, &ncell, &precision, &clock, &diag) != SHX_OK) return NULL;
+  napi_value o, ck;
+  if (napi_create_object(env, &o) != napi_ok) return NULL;
+  ck = f64_array(env, clock, 11);
+  if (!ck || napi_set_named_property(env, o, "clock", ck) != napi_ok) return NULL;
+  if (a->kind == 0) {
+    size_t n = 4 * ncell, es = precision == 64 ? 8 : 4;
+    void *p1, *p2;
+    napi_value x = new_typed(env, precision, n, &p1);
+    napi_value y = new_typed(env, precision, n, &p2);
+    if (!x || !y) return NULL;
+    memcpy(p1, u1, n * es);
+    memcpy(p2, u2, n * es);
+    if (napi_set_named_property(env, o, "u1", x) != napi_ok || napi_set_named_property(env, o, "u2", y) != napi_ok)
+      return NULL;
+  } else {
+    napi_value d = f64_array(env, diag, 24);
+    if (!d || napi_set_named_property(env, o, "diag", d) != napi_ok) return NULL;
+  }
+  return o;
 }
 
 static void async_complete(napi_env env, napi_status status, void* data) {
   Async* a = (Async*)data;
   napi_value result = NULL;
-  if (status != napi_ok || a->rc != SHX_OK) {
+  if (status == napi_ok && a->rc == SHX_OK) result = readback_value(env, a);
+  if (result) {
+    napi_resolve_deferred(env, a->deferred, result);
+  } else {
     napi_value msg, err;
-    napi_create_string_utf8(env, a->rc != SHX_OK ? a->err : "async readback cancelled", NAPI_AUTO_LENGTH, &msg);
+    const char* m = a->rc != SHX_OK ? a->err : status != napi_ok ? "async readback cancelled" : "out of memory";
+    napi_create_string_utf8(env, m, NAPI_AUTO_LENGTH, &msg);
     napi_create_error(env, NULL, msg, &err);
     napi_reject_deferred(env, a->deferred, err);
-  } else {
-    if (a->kind == 0) {
-      size_t n = 4 * a->h->ncell;
-      size_t es = a->h->precision == 64 ? 8 : 4;
-      void *p1, *p2;
-      napi_value x = new_typed(env, a->h->precision, n, &p1);
-      napi_value y = new_typed(env, a->h->precision, n, &p2);
-      memcpy(p1, a->d1, n * es);
-      memcpy(p2, a->d2, n * es);
-      result = state_pair(env, x, y);
-    } else {
-      result = f64_array(env, a->diag, 16);
-    }
-    napi_resolve_deferred(env, a->deferred, result);
   }
-  napi_delete_reference(env, a->keep);
   napi_delete_async_work(env, a->work);
-  free(a->d1);
-  free(a->d2);
+  shx_readback_free(a->rb);
   free(a);
 }
 
-static napi_value start_async(napi_env env, napi_value handle, Handle* h, int kind) {
+static napi_value start_async(napi_env env, Handle* h, int kind) {
   Async* a = (Async*)calloc(1, sizeof(Async));
-  a->h = h;
+  if (!a) { napi_throw_error(env, NULL, "out of memory"); return NULL; }
   a->kind = kind;
-  if (kind == 0) {
-    size_t bytes = 4 * h->ncell * (h->precision == 64 ? 8 : 4);
-    a->d1 = malloc(bytes);
-    a->d2 = malloc(bytes);
-    if (!a->d1 || !a->d2) {
-      free(a->d1); free(a->d2); free(a);
-      napi_throw_error(env, NULL, "out of memory");
-      return NULL;
-    }
-  }
+  const int what = kind == 0 ? SHX_READ_STATE : SHX_READ_DIAG;
+  int rc = h->amr ? shx_amr_readback_begin((shx_amr*)h->p, what, &a->rb)
+                  : shx_readback_begin((shx_sim*)h->p, what, &a->rb);
+  if (rc != SHX_OK) { free(a); return throw_status(env, rc); }
   napi_value promise, name;
-  NAPI_CALL(env, napi_create_promise(env, &a->deferred, &promise));
-  NAPI_CALL(env, napi_create_reference(env, handle, 1, &a->keep));
-  NAPI_CALL(env, napi_create_string_utf8(env, "shx.readback", NAPI_AUTO_LENGTH, &name));
-  NAPI_CALL(env, napi_create_async_work(env, NULL, name, async_execute, async_complete, a, &a->work));
-  NAPI_CALL(env, napi_queue_async_work(env, a->work));
+  if (napi_create_promise(env, &a->deferred, &promise) != napi_ok ||
+      napi_create_string_utf8(env, "shx.readback", NAPI_AUTO_LENGTH, &name) != napi_ok ||
+      napi_create_async_work(env, NULL, name, async_execute, async_complete, a, &a->work) != napi_ok ||
+      napi_queue_async_work(env, a->work) != napi_ok) {
+    shx_readback_free(a->rb);
+    free(a);
+    napi_throw_error(env, NULL, "N-API: could not queue the readback");
+    return NULL;
+  }
   return promise;
 }
 
@@ -461,7 +484,7 @@ static napi_value js_read_state_async(napi_env env, napi_callback_info info) {
   NAPI_CALL(env, napi_get_value_external(env, argv[0], &data));
   Handle* h = (Handle*)data;
   if (!h || !h->p) { napi_throw_error(env, NULL, "the simulation was destroyed"); return NULL; }
-  return start_async(env, argv[0], h, 0);
+  return start_async(env, h, 0);
 }
 
 static napi_value js_diagnostics_async(napi_env env, napi_callback_info info) {
@@ -471,7 +494,7 @@ static napi_value js_diagnostics_async(napi_env env, napi_callback_info info) {
   NAPI_CALL(env, napi_get_value_external(env, argv[0], &data));
   Handle* h = (Handle*)data;
   if (!h || !h->p) { napi_throw_error(env, NULL, "the simulation was destroyed"); return NULL; }
-  return start_async(env, argv[0], h, 1);
+  return start_async(env, h, 1);
 }
 
 /* ---------------- block grid ---------------- */
@@ -561,8 +584,11 @@ static napi_value js_amr_step(napi_env env, napi_callback_info info) {
   if (!h) return NULL;
   int32_t count;
   NAPI_CALL(env, napi_get_value_int32(env, argv[1], &count));
-  SHX_CALL(env, shx_amr_step((shx_amr*)h->p, count));
-  return undefined(env);
+  int taken = 0;
+  SHX_CALL(env, shx_amr_step((shx_amr*)h->p, count, &taken));
+  napi_value out;
+  NAPI_CALL(env, napi_create_int32(env, taken, &out));
+  return out;
 }
 
 static napi_value js_amr_sync(napi_env env, napi_callback_info info) {
@@ -589,9 +615,9 @@ static napi_value js_amr_diagnostics(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 1, argv)) return NULL;
   Handle* h = get_handle(env, argv[0], 1);
   if (!h) return NULL;
-  double out[16];
+  double out[24];
   SHX_CALL(env, shx_amr_diagnostics((shx_amr*)h->p, out));
-  return f64_array(env, out, 16);
+  return f64_array(env, out, 24);
 }
 
 static napi_value js_amr_profile(napi_env env, napi_callback_info info) {
@@ -667,6 +693,8 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT("readState", js_read_state),
       EXPORT("readStateAsync", js_read_state_async),
       EXPORT("step", js_step),
+      EXPORT("writeParams", js_write_params),
+      EXPORT("buildHydro", js_build_hydro),
       EXPORT("sync", js_sync),
       EXPORT("runToEnd", js_run_to_end),
       EXPORT("readClock", js_read_clock),

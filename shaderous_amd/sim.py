@@ -42,6 +42,7 @@ class SimConfig:
     path: str = "fused"
     device: int = 0
     tile: tuple = (0, 0, 0)
+    org: tuple = None       # the domain's lower corner; None: -n[0] dx / 2 on every axis (domainLo)
 
     def to_c(self) -> ShxConfig:
         c = ShxConfig()
@@ -63,6 +64,8 @@ class SimConfig:
         c.t_end = float(self.t_end) if math.isfinite(self.t_end) else 1e30
         c.fluff[:] = [float(v) for v in self.fluff]
         c.tile[:] = [int(v) for v in self.tile]
+        org = self.org if self.org is not None else (-0.5 * self.n[0] * self.dx,) * 3
+        c.org[:] = [float(v) for v in org]
         return c
 
 
@@ -77,22 +80,36 @@ def config_for(problem, **over) -> SimConfig:
 
 @dataclass
 class Diagnostics:
-    """Sim.diagnostics (src/sim.ts:1647-1673) with FP64 sums."""
+    """Sim.diagnostics (src/sim.ts:1647-1673): the 24 channels of hydro/diagnostics.wgsl, every
+    cell weighted by its volume, summed in FP64; `raw` holds the channels as the ABI returns them."""
 
     t: float
     dt: float
     steps: int
+    held: bool
     mass: float
     momentum: tuple
     e_kin: float
     e_int: float
+    e_grav: float
     e_tot: float
     max_mach: float
     floored: float
-    scalar: float
-    rho_range: tuple
-    p_range: tuple
+    jeans_unresolved: float
+    nuclear_heating: float
+    com: tuple
+    ranges: dict            # (min, max) of rho, T, s, p, speed
     raw: np.ndarray = field(repr=False, default=None)
+
+
+def diagnostics_from(r, ck) -> Diagnostics:
+    """diagnosticsFrom (src/sim.ts:1647-1673): the record from the channels and the clock."""
+    names = ("rho", "T", "s", "p", "speed")
+    return Diagnostics(t=ck[1] + ck[0], dt=ck[0], steps=int(ck[3] + (1 if ck[0] > 0 else 0)), held=ck[10] != 0,
+                       mass=r[0], momentum=tuple(r[1:4]), e_kin=r[4], e_int=r[5], e_grav=r[6],
+                       e_tot=r[4] + r[5] + r[6], max_mach=r[7], floored=r[8], jeans_unresolved=r[9],
+                       nuclear_heating=r[23], com=tuple(r[20:23] / r[0]) if r[0] else (0.0, 0.0, 0.0),
+                       ranges={k: (r[10 + 2 * i], r[11 + 2 * i]) for i, k in enumerate(names)}, raw=r)
 
 
 class Sim:
@@ -144,13 +161,17 @@ class Sim:
         self.set_state(u1, u2)
 
     # -- stepping --
-    def step(self, count: int = 1):
-        check(_lib.load().shx_step(self._h, int(count)))
+    def step(self, count: int = 1) -> int:
+        """Enqueue `count` steps (encodeSteps); returns the steps taken."""
+        n = C.c_int(0)
+        check(_lib.load().shx_step(self._h, int(count), C.byref(n)))
+        return n.value
 
     def sync(self):
         check(_lib.load().shx_sync(self._h))
 
-    def run_to_end(self, batch: int = 64, max_steps: int = 1 << 30) -> int:
+    def run_to_end(self, batch: int = 16, max_steps: int = 1 << 30) -> int:
+        """Sim.runToEnd: step until the clock holds at t_end; the clock's step count then."""
         n = C.c_int(0)
         check(_lib.load().shx_run_to_end(self._h, batch, max_steps, C.byref(n)))
         return n.value
@@ -161,13 +182,23 @@ class Sim:
         return out
 
     def diagnostics(self) -> Diagnostics:
-        out = np.zeros(16)
+        out = np.zeros(24)
         check(_lib.load().shx_diagnostics(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
-        ck = self.clock()
-        return Diagnostics(t=ck[1] + ck[0], dt=ck[0], steps=int(ck[3] + (1 if ck[0] > 0 else 0)), mass=out[0],
-                           momentum=tuple(out[1:4]), e_kin=out[4], e_int=out[5], e_tot=out[6], max_mach=out[7],
-                           floored=out[8], scalar=out[9], rho_range=(out[10], out[11]), p_range=(out[12], out[13]),
-                           raw=out)
+        return diagnostics_from(out, self.clock())
+
+    def write_params(self, cfg: SimConfig = None):
+        """Sim.writeParams: the run-time fields of `cfg` (default self.cfg) from the next step on."""
+        cfg = cfg or self.cfg
+        c = cfg.to_c()
+        check(_lib.load().shx_write_params(self._h, C.byref(c)))
+        self.cfg = cfg
+
+    def build_hydro(self, cfg: SimConfig = None):
+        """Sim.buildHydro: the kernels of `cfg`'s limiter, Riemann solver and reconstruction."""
+        cfg = cfg or self.cfg
+        c = cfg.to_c()
+        check(_lib.load().shx_build_hydro(self._h, C.byref(c)))
+        self.cfg = cfg
 
     def flux_mismatch(self):
         """(mismatched faces, faces compared) (Sim.fluxMismatch)."""

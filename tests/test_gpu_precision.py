@@ -79,7 +79,7 @@ def test_precision_turbulence_256():
 
 
 def test_precision_sedov_128():
-    data = compare_uniform("sedov_128^3_t0.05", problems.sedov(128), None, 1e-4)
+    data = compare_uniform("sedov_128^3_t0.05", problems.sedov(128), None, 1e-5)
     assert abs(data["t32"] - 0.05) < 1e-6 and abs(data["t64"] - 0.05) < 1e-12
 
 

@@ -314,6 +314,15 @@ def test_conservation_periodic(path):
         assert abs(d1.momentum[a] - d0.momentum[a]) < 2e-7 * d0.mass
 
 
+@pytest.mark.parametrize("path", ["fused", "plain"])
+def test_kelvin_helmholtz_256_full_run_matches_oracle(path):
+    """The KH workload at its full size (256^2, 100 steps, MC + HLLC, CFL 0.4) against the FP32
+    oracle after every step, on both paths."""
+    worst = run_pair(problems.kelvin_helmholtz(256), 100, path=path)
+    print(f"\n[gpu] KH 256^2 x 100 {path}: worst per-step L1 {worst}", flush=True)
+    assert max(worst.values()) <= L1_TOL, worst
+
+
 def test_sod_matches_reference_measurement():
     """The reference's sod-axes measurement (75 steps, L1 0.0078902) on the GPU, as runToEnd."""
     g = GOLD["sod_axes"]
