@@ -69,14 +69,6 @@ struct shx_sim {
 
 namespace {
 
-DiagGeo diag_geo(const shx_config& c) {
-  DiagGeo G;
-  G.dv = c.ndim == 1 ? c.dx : c.ndim == 2 ? c.dx * c.dx : c.dx * c.dx * c.dx;
-  for (int a = 0; a < 3; a++) G.org[a] = c.org[a];
-  G.flat = c.n[2] == 1;
-  return G;
-}
-
 Launch launch_of(shx_sim* s) {
   Launch L;
   L.stream = s->stream;
@@ -340,7 +332,7 @@ int shx_readback_begin(shx_sim* s, int what, shx_readback** out) {
   if (!s || !out) return fail(SHX_EINVAL, "null argument");
   if (what & SHX_READ_DIAG) {
     Launch L = launch_of(s);
-    s->ops.diag(L, diag_geo(s->cfg), s->U1[s->cur], s->U2[s->cur], s->diag_part, s->diag_blocks, s->diag_out);
+    s->ops.diag(L, uniform_diag_geo(s->cfg), s->U1[s->cur], s->U2[s->cur], s->diag_part, s->diag_blocks, s->diag_out);
     HIPCHK(hipGetLastError());
   }
   return readback_begin(s->stream, what, s->es, s->ncell, s->U1[s->cur], s->U2[s->cur], s->clk, s->diag_out, out);
@@ -548,7 +540,7 @@ int shx_run_to_end(shx_sim* s, int batch, int max_steps, int* steps) {
 int shx_diagnostics(shx_sim* s, double out[24]) {
   if (!s || !out) return fail(SHX_EINVAL, "bad argument");
   Launch L = launch_of(s);
-  s->ops.diag(L, diag_geo(s->cfg), s->U1[s->cur], s->U2[s->cur], s->diag_part, s->diag_blocks, s->diag_out);
+  s->ops.diag(L, uniform_diag_geo(s->cfg), s->U1[s->cur], s->U2[s->cur], s->diag_part, s->diag_blocks, s->diag_out);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, s->diag_out, DIAG_CH * 8, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));

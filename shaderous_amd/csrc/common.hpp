@@ -6,6 +6,7 @@
 #include <string>
 
 #include "../../include/shx.h"
+#include "diag.hpp"
 #include "hydro.hpp"
 
 namespace shx {
@@ -40,6 +41,15 @@ Params<T> make_params(const shx_config& c) {
     if (!(c.fluff[2] > 0)) P.fluff[2] = T(1e30);
   }
   return P;
+}
+
+// Where a uniform grid's cells are, for the diagnostics: dv = dx^ndim, the lower corner.
+inline DiagGeo uniform_diag_geo(const shx_config& c) {
+  DiagGeo G;
+  G.dv = c.ndim == 1 ? c.dx : c.ndim == 2 ? c.dx * c.dx : c.dx * c.dx * c.dx;
+  for (int a = 0; a < 3; a++) G.org[a] = c.org[a];
+  G.flat = c.n[2] == 1;
+  return G;
 }
 
 // Sim.runToEnd's loop (src/sim.ts:3689-3718) over a sim's step and clock read: batches double

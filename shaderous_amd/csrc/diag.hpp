@@ -95,6 +95,25 @@ __device__ __forceinline__ void diag_block(double r[DIAG_CH], double* sh, double
   }
 }
 
+// The uniform grid: one record of the channels per workgroup over a fixed set of cells (a
+// grid-stride loop), folded by k_diag_final in workgroup order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag(Params<T> P, DiagGeo G, const T4<T>* __restrict__ U1,
+                                              const T4<T>* __restrict__ U2, int ncell, double* part) {
+  __shared__ double sh[4 * DIAG_CH];
+  double r[DIAG_CH];
+  diag_init(r);
+  const int nx = P.n[0], ny = P.n[1];
+  const double dx = (double)P.dx;
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < ncell; o += gridDim.x * blockDim.x) {
+    const int i = o % nx, j = (o / nx) % ny, k = o / (nx * ny);
+    const double x = (i + 0.5) * dx + G.org[0], y = (j + 0.5) * dx + G.org[1];
+    const double z = G.flat ? 0.0 : (k + 0.5) * dx + G.org[2];
+    diag_cell(P, U1[o], U2[o], G.dv, x, y, z, r);
+  }
+  diag_block(r, sh, part + (size_t)blockIdx.x * DIAG_CH);
+}
+
 // The channels over `nparts` partial records: 32 groups of consecutive records per channel,
 // each folded in record order, then the groups in order (a fixed order: deterministic).
 // Launch: one workgroup of 32 x DIAG_CH threads. (T: the state's precision, which the records

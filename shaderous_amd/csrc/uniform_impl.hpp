@@ -575,26 +575,6 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 4 : 1)) void k_fused(
 #include "uniform_roles.hpp"
 namespace shx {
 
-// ---------------- diagnostics (FP64 sums) ----------------
-// One record of the 24 channels per workgroup over a fixed set of cells (a grid-stride loop),
-// folded by k_diag_final in workgroup order.
-template <typename T>
-__global__ __launch_bounds__(256) void k_diag(Params<T> P, DiagGeo G, const T4<T>* __restrict__ U1,
-                                              const T4<T>* __restrict__ U2, int ncell, double* part) {
-  __shared__ double sh[4 * DIAG_CH];
-  double r[DIAG_CH];
-  diag_init(r);
-  const int nx = P.n[0], ny = P.n[1];
-  const double dx = (double)P.dx;
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < ncell; o += gridDim.x * blockDim.x) {
-    const int i = o % nx, j = (o / nx) % ny, k = o / (nx * ny);
-    const double x = (i + 0.5) * dx + G.org[0], y = (j + 0.5) * dx + G.org[1];
-    const double z = G.flat ? 0.0 : (k + 0.5) * dx + G.org[2];
-    diag_cell(P, U1[o], U2[o], G.dv, x, y, z, r);
-  }
-  diag_block(r, sh, part + (size_t)blockIdx.x * DIAG_CH);
-}
-
 // ---------------- launchers ----------------
 template <typename T, int LIM, int RS, bool ISEN>
 struct UniformImpl {
