@@ -96,6 +96,8 @@ struct AmrOps {
 };
 
 AmrOps amr_ops_f32(int lim, int rs, int isen);
+AmrOps amr_ops_other32(int lim, int rs);
+AmrOps amr_ops_koren_hll32(int isen);
 AmrOps amr_ops_f64(int lim, int rs, int isen);
 
 inline AmrOps amr_ops(int precision, int lim, int rs, int isen) {

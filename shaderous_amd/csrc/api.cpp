@@ -40,8 +40,12 @@ struct shx_sim {
   void* U2[2] = {nullptr, nullptr};
   void* PA = nullptr;
   void* PB = nullptr;
-  void* clk = nullptr;       // 16 T: the Clock (11) and spare
-  unsigned long long* slot = nullptr;   // reduction slot: max rate of the current state
+  // the clock block (clock_block_bytes): the clocks of parity 0 and 1 (16 T each: the Clock's 11
+  // values and spare), three rate slots, two epochs. The plain path keeps its clock in the
+  // parity-0 clock and the rate in slot 0; the fused step rotates them (StepClock).
+  void* clk = nullptr;
+  unsigned long long* slot = nullptr;   // in the clock block
+  unsigned int* epoch = nullptr;        // in the clock block
   double* diag_part = nullptr;           // diag_blocks records of DIAG_CH channels
   double* diag_out = nullptr;            // DIAG_CH channels
   int diag_blocks = 256;
@@ -77,16 +81,25 @@ Launch launch_of(shx_sim* s) {
   return L;
 }
 
-// The fused path's one-pass update from parity p into 1 - p (after the clock and the boundary
-// planes' half steps): the 2D kernels on a flat FP32 grid, else the 3D march with tile `tile`.
+// The clocks of the fused step from parity p (its clock pass is in the step kernel).
+ClockRefs clock_refs(shx_sim* s, int p) { return ClockRefs{s->clk, s->slot, s->epoch, p}; }
+
+// The clock the next step reads: the fused path's of the current parity, the plain path's one.
+const void* current_clock(const shx_sim* s) {
+  return static_cast<const char*>(s->clk) + (s->fused ? (size_t)16 * s->es * s->cur : 0);
+}
+
+// The fused path's one-pass step from parity p into 1 - p (its clock pass included; after the
+// boundary planes' half steps): the 2D kernels on a flat FP32 grid, else the 3D march with tile
+// `tile`.
 void enqueue_fused(shx_sim* s, int p, int tile, int zc, bool debug) {
   if (s->flat2d && !debug) {
-    s->eops.fused(s->stream, tile, s->egrid, s->einst, s->etiles[tile], s->entiles[tile], s->U1[p], s->U2[p],
-                  s->U1[1 - p], s->U2[1 - p], s->clk, s->slot);
+    s->eops.fused(s->stream, tile, s->egrid, s->einst, 1, s->etiles[tile], s->entiles[tile], s->U1[p], s->U2[p],
+                  s->U1[1 - p], s->U2[1 - p], clock_refs(s, p));
     return;
   }
-  s->ops.fused(launch_of(s), tile, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], s->clk,
-               s->zbc ? s->PA : nullptr, s->zbc ? s->PB : nullptr, zc, s->slot, debug);
+  s->ops.fused(launch_of(s), tile, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], clock_refs(s, p),
+               s->zbc ? s->PA : nullptr, s->zbc ? s->PB : nullptr, zc, debug);
 }
 
 // One step from parity p into 1 - p.
@@ -94,8 +107,7 @@ void enqueue_step(shx_sim* s, int p, bool debug = false) {
   Launch L = launch_of(s);
   const UniformOps& o = s->ops;
   if (s->fused) {
-    o.clock(L, s->clk, s->slot);
-    if (s->zbc) o.zplanes(L, s->U1[p], s->U2[p], s->PA, s->PB, s->clk);
+    if (s->zbc) o.zplanes(L, s->U1[p], s->U2[p], s->PA, s->PB, clock_refs(s, p));
     enqueue_fused(s, p, s->flat2d && !debug ? s->etile : s->tile, s->zc, debug);
   } else {
     o.rates(L, s->U1[p], s->U2[p], s->slot);
@@ -126,11 +138,11 @@ int build_graphs(shx_sim* s) {
   return SHX_OK;
 }
 
-// The reduction slot holds the max rate of the current state: the plain path recomputes it each
-// step (k_rates), the fused path in the previous step's epilogue; after a state change it is
-// recomputed here.
+// The rate slots: the plain path recomputes the max rate each step (k_rates into slot 0); the
+// fused path's step leaves the rate of the state it writes in the next epoch's slot; after a
+// state change the epochs restart at 0 and the current state's rate is recomputed into slot 0.
 int reset_rates(shx_sim* s) {
-  HIPCHK(hipMemsetAsync(s->slot, 0, sizeof(unsigned long long), s->stream));
+  HIPCHK(hipMemsetAsync(s->slot, 0, 3 * sizeof(unsigned long long) + 2 * sizeof(unsigned int), s->stream));
   if (s->fused) {
     Launch L = launch_of(s);
     s->ops.rates(L, s->U1[s->cur], s->U2[s->cur], s->slot);
@@ -335,7 +347,8 @@ int shx_readback_begin(shx_sim* s, int what, shx_readback** out) {
     s->ops.diag(L, uniform_diag_geo(s->cfg), s->U1[s->cur], s->U2[s->cur], s->diag_part, s->diag_blocks, s->diag_out);
     HIPCHK(hipGetLastError());
   }
-  return readback_begin(s->stream, what, s->es, s->ncell, s->U1[s->cur], s->U2[s->cur], s->clk, s->diag_out, out);
+  return readback_begin(s->stream, what, s->es, s->ncell, s->U1[s->cur], s->U2[s->cur], current_clock(s), s->diag_out,
+                        out);
 }
 
 int shx_write_params(shx_sim* s, const shx_config* cfg) { return reconfigure(s, cfg, false); }
@@ -434,7 +447,7 @@ int shx_create(const shx_config* cfg, shx_sim** out) {
     const size_t zb = (size_t)2 * c.n[0] * c.n[1] * 4 * s->es;
     ok = alloc(&s->PA, zb) && alloc(&s->PB, zb);
   }
-  ok = ok && alloc(&s->clk, 16 * 8) && alloc((void**)&s->slot, 64) && alloc((void**)&s->diag_part, (size_t)s->diag_blocks * DIAG_CH * 8) &&
+  ok = ok && alloc(&s->clk, clock_block_bytes(1, s->es)) && alloc((void**)&s->diag_part, (size_t)s->diag_blocks * DIAG_CH * 8) &&
        alloc((void**)&s->diag_out, DIAG_CH * 8);
   std::vector<EnsTile> lists[ENS_NTILES];
   if (s->flat2d) {
@@ -455,8 +468,9 @@ int shx_create(const shx_config* cfg, shx_sim** out) {
     shx_destroy(s);
     return fail(SHX_ENOMEM, "device allocation failed for %zu cells (%.2f GB per state copy)", s->ncell, bytes * 2 / 1e9);
   }
-  (void)hipMemset(s->clk, 0, 16 * 8);
-  (void)hipMemset(s->slot, 0, 64);
+  (void)hipMemset(s->clk, 0, clock_block_bytes(1, s->es));
+  s->slot = reinterpret_cast<unsigned long long*>(static_cast<char*>(s->clk) + clock_slots_at(1, s->es));
+  s->epoch = reinterpret_cast<unsigned int*>(static_cast<char*>(s->clk) + clock_epochs_at(1, s->es));
   if (s->flat2d) {
     const EnsInst<float> one{s->pf, 0};
     const EnsInst<double> one64{s->pd, 0};
@@ -480,7 +494,7 @@ int shx_destroy(shx_sim* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   destroy_graphs(s);
   for (int p = 0; p < 2; p++) { (void)hipFree(s->U1[p]); (void)hipFree(s->U2[p]); }
-  (void)hipFree(s->PA); (void)hipFree(s->PB); (void)hipFree(s->clk); (void)hipFree(s->slot);
+  (void)hipFree(s->PA); (void)hipFree(s->PB); (void)hipFree(s->clk);
   (void)hipFree(s->diag_part); (void)hipFree(s->diag_out);
   (void)hipFree(s->einst);
   for (int k = 0; k < ENS_NTILES; k++) (void)hipFree(s->etiles[k]);
@@ -523,7 +537,7 @@ int shx_sync(shx_sim* s) {
 int shx_read_clock(shx_sim* s, double out[11]) {
   if (!s || !out) return fail(SHX_EINVAL, "bad argument");
   unsigned char buf[16 * 8];
-  HIPCHK(hipMemcpyAsync(buf, s->clk, sizeof buf, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(buf, current_clock(s), 16 * s->es, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   for (int i = 0; i < 11; i++) out[i] = s->es == 8 ? ((double*)buf)[i] : (double)((float*)buf)[i];
   return SHX_OK;
@@ -551,10 +565,8 @@ int shx_flux_mismatch(shx_sim* s, int64_t* mismatched, int64_t* faces) {
   if (!s) return fail(SHX_EINVAL, "null sim");
   // one step in DEBUG_FLUX mode: the face masses land in the other parity's state arrays; the
   // clock advances as for a step, so a copy of it is restored afterwards
-  unsigned char saved[16 * 8];
-  HIPCHK(hipMemcpyAsync(saved, s->clk, sizeof saved, hipMemcpyDeviceToHost, s->stream));
-  unsigned long long slot_saved;
-  HIPCHK(hipMemcpyAsync(&slot_saved, s->slot, 8, hipMemcpyDeviceToHost, s->stream));
+  std::vector<unsigned char> saved(clock_block_bytes(1, s->es));
+  HIPCHK(hipMemcpyAsync(saved.data(), s->clk, saved.size(), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   int p = s->cur;
   enqueue_step(s, p, true);
@@ -572,8 +584,7 @@ int shx_flux_mismatch(shx_sim* s, int64_t* mismatched, int64_t* faces) {
     HIPCHK(hipStreamSynchronize(s->stream));
     for (size_t i = 0; i < 4 * nc; i++) { a[i] = fa[i]; b[i] = fb[i]; }
   }
-  HIPCHK(hipMemcpyAsync(s->clk, saved, sizeof saved, hipMemcpyHostToDevice, s->stream));
-  HIPCHK(hipMemcpyAsync(s->slot, &slot_saved, 8, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->clk, saved.data(), saved.size(), hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   const int* n = s->cfg.n;
   int64_t mis = 0, tot = 0;
@@ -611,10 +622,10 @@ int shx_profile(shx_sim* s, int steps, double ms[5]) {
     int p = s->cur;
     HIPCHK(hipEventRecord(ev[0], s->stream));
     if (s->fused) {
-      s->ops.clock(L, s->clk, s->slot);
+      // the clock pass is inside the step kernel: ms[1] stays 0
       HIPCHK(hipEventRecord(ev[1], s->stream));
       HIPCHK(hipEventRecord(ev[2], s->stream));
-      if (s->zbc) s->ops.zplanes(L, s->U1[p], s->U2[p], s->PA, s->PB, s->clk);
+      if (s->zbc) s->ops.zplanes(L, s->U1[p], s->U2[p], s->PA, s->PB, clock_refs(s, p));
       HIPCHK(hipEventRecord(ev[3], s->stream));
       enqueue_fused(s, p, s->flat2d ? s->etile : s->tile, s->zc, false);
     } else {
@@ -656,16 +667,24 @@ int shx_autotune(shx_sim* s, int reps, int chosen[3]) {
     return SHX_OK;
   }
   if (reps < 1) reps = 3;
-  // keep the state: the candidates are timed on the current state and the state restored
+  // keep the state: each candidate takes `reps` steps from the current state (its clock and rate
+  // slots too), which is restored before the next and at the end
   size_t bytes = s->ncell * 16;
   void *b1 = nullptr, *b2 = nullptr;
   HIPCHK(hipMalloc(&b1, bytes));
   HIPCHK(hipMalloc(&b2, bytes));
   HIPCHK(hipMemcpyAsync(b1, s->U1[s->cur], bytes, hipMemcpyDeviceToDevice, s->stream));
   HIPCHK(hipMemcpyAsync(b2, s->U2[s->cur], bytes, hipMemcpyDeviceToDevice, s->stream));
-  unsigned char clk_saved[128];
-  HIPCHK(hipMemcpyAsync(clk_saved, s->clk, 128, hipMemcpyDeviceToHost, s->stream));
+  std::vector<unsigned char> clk_saved(clock_block_bytes(1, s->es));
+  HIPCHK(hipMemcpyAsync(clk_saved.data(), s->clk, clk_saved.size(), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
+  const int start = s->cur;
+  auto restore = [&]() -> int {
+    HIPCHK(hipMemcpyAsync(s->U1[start], b1, bytes, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->U2[start], b2, bytes, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->clk, clk_saved.data(), clk_saved.size(), hipMemcpyHostToDevice, s->stream));
+    return SHX_OK;
+  };
   const int tiles3[] = {FUSED_R32x16, FUSED_R32x8, FUSED_32x32, FUSED_32x16, FUSED_20x20};
   const int tiles2[] = {ENS_32x32, ENS_32x16, ENS_20x20, ENS_ROWS8, ENS_ROWS16, ENS_ROWS32};
   std::vector<int> tiles = s->flat2d ? std::vector<int>(tiles2, tiles2 + 6) : std::vector<int>(tiles3, tiles3 + 5);
@@ -673,8 +692,6 @@ int shx_autotune(shx_sim* s, int reps, int chosen[3]) {
   double best = 1e30;
   int bt = s->flat2d ? s->etile : s->tile, bz = s->zc;
   Launch L = launch_of(s);
-  // the boundary planes' half steps of the state every candidate starts from
-  if (s->zbc) s->ops.zplanes(L, s->U1[s->cur], s->U2[s->cur], s->PA, s->PB, s->clk);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -683,12 +700,17 @@ int shx_autotune(shx_sim* s, int reps, int chosen[3]) {
       // the 2D kernels have no plane chunk: one candidate per tile; a chunk beyond the grid's
       // planes is the whole grid, timed once
       if (s->flat2d ? z != zcs[0] : (z > s->cfg.n[2] && z != zcs[0])) continue;
-      int p = s->cur;
-      enqueue_fused(s, p, ti, z, false);  // warm
-      HIPCHK(hipEventRecord(e0, s->stream));
-      for (int r = 0; r < reps; r++) enqueue_fused(s, p, ti, z, false);
+      int rc = restore();
+      if (rc) return rc;
+      // a warm step, then `reps` timed steps (each with its boundary planes' half steps)
+      for (int r = 0, p = start; r <= reps; r++, p = 1 - p) {
+        if (r == 1) HIPCHK(hipEventRecord(e0, s->stream));
+        if (s->zbc) s->ops.zplanes(L, s->U1[p], s->U2[p], s->PA, s->PB, clock_refs(s, p));
+        enqueue_fused(s, p, ti, z, false);
+      }
       HIPCHK(hipEventRecord(e1, s->stream));
       HIPCHK(hipEventSynchronize(e1));
+      HIPCHK(hipGetLastError());
       float t;
       HIPCHK(hipEventElapsedTime(&t, e0, e1));
       if (t < best) { best = t; bt = ti; bz = z; }
@@ -701,12 +723,10 @@ int shx_autotune(shx_sim* s, int reps, int chosen[3]) {
     s->tile = bt;
     s->zc = std::min(bz, s->cfg.n[2]);
   }
-  HIPCHK(hipMemcpyAsync(s->U1[s->cur], b1, bytes, hipMemcpyDeviceToDevice, s->stream));
-  HIPCHK(hipMemcpyAsync(s->U2[s->cur], b2, bytes, hipMemcpyDeviceToDevice, s->stream));
-  HIPCHK(hipMemcpyAsync(s->clk, clk_saved, 128, hipMemcpyHostToDevice, s->stream));
-  int rc = reset_rates(s);
+  int rc = restore();
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
+  s->cur = start;
   (void)hipFree(b1);
   (void)hipFree(b2);
   rc = build_graphs(s);

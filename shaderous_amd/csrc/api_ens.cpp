@@ -25,8 +25,11 @@ struct shx_ens {
   int ntiles[ENS_NTILES] = {};
   void* U1[2] = {nullptr, nullptr};
   void* U2[2] = {nullptr, nullptr};
-  void* clk = nullptr;         // 16 T per instance
+  // the clock block (clock_block_bytes): per instance two clocks (parity 0 and 1, 16 T each),
+  // three rate slots and two epochs (StepClock)
+  void* clk = nullptr;
   unsigned long long* slot = nullptr;
+  unsigned int* epoch = nullptr;
   int cur = 0;
   int tile = ENS_ROWS16;
   int grid = 512;
@@ -37,11 +40,20 @@ struct shx_ens {
 
 namespace {
 
-void enqueue_step(shx_ens* e, int p) {
-  e->ops.clock(e->stream, e->inst, e->count, e->clk, e->slot);
-  e->ops.fused(e->stream, e->tile, e->grid, e->inst, e->tiles[e->tile], e->ntiles[e->tile], e->U1[p], e->U2[p],
-               e->U1[1 - p], e->U2[1 - p], e->clk, e->slot);
+size_t block_bytes(const shx_ens* e) { return clock_block_bytes(e->count, e->es); }
+
+// The clocks of the step from parity p (its clock pass is in the step kernel).
+ClockRefs clock_refs(shx_ens* e, int p) { return ClockRefs{e->clk, e->slot, e->epoch, p}; }
+
+// Instance i's clock that its next step reads.
+void* clock_of(shx_ens* e, int i) { return static_cast<char*>(e->clk) + (size_t)(2 * i + e->cur) * 16 * e->es; }
+
+void enqueue_fused(shx_ens* e, int p, int tile) {
+  e->ops.fused(e->stream, tile, e->grid, e->inst, e->count, e->tiles[tile], e->ntiles[tile], e->U1[p], e->U2[p],
+               e->U1[1 - p], e->U2[1 - p], clock_refs(e, p));
 }
+
+void enqueue_step(shx_ens* e, int p) { enqueue_fused(e, p, e->tile); }
 
 void destroy_graphs(shx_ens* e) {
   for (int p = 0; p < 2; p++)
@@ -65,8 +77,10 @@ int build_graphs(shx_ens* e) {
 // step after a state change (outside the graphs).
 int refresh_rates(shx_ens* e) {
   if (!e->rates_stale) return SHX_OK;
-  HIPCHK(hipMemsetAsync(e->slot, 0, sizeof(unsigned long long) * e->count, e->stream));
-  e->ops.rates(e->stream, e->inst, e->tiles[e->tile], e->ntiles[e->tile], e->tile, e->U1[e->cur], e->U2[e->cur], e->slot);
+  // every instance's epochs restart at 0: its rate into its slot 0
+  HIPCHK(hipMemsetAsync(e->slot, 0, block_bytes(e) - clock_slots_at(e->count, e->es), e->stream));
+  e->ops.rates(e->stream, e->inst, e->tiles[e->tile], e->ntiles[e->tile], e->tile, e->U1[e->cur], e->U2[e->cur],
+               e->slot, 3);
   HIPCHK(hipGetLastError());
   e->rates_stale = false;
   return SHX_OK;
@@ -86,7 +100,7 @@ int set_state_t(shx_ens* e, int i, const T* u1, const T* u2, size_t ncell) {
   }
   T clk[16] = {0};
   clk[4] = clk[5] = clk[6] = T(1);
-  HIPCHK(hipMemcpyAsync((char*)e->clk + (size_t)i * 16 * sizeof(T), clk, sizeof clk, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(clock_of(e, i), clk, sizeof clk, hipMemcpyHostToDevice, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
   e->rates_stale = true;
   return SHX_OK;
@@ -175,7 +189,7 @@ int shx_ens_create(const shx_config* cfg, int count, const int* sizes, shx_ens**
   bool ok = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess;
   const size_t bytes = e->ncell * 4 * e->es;
   for (int p = 0; ok && p < 2; p++) ok = alloc(&e->U1[p], bytes) && alloc(&e->U2[p], bytes);
-  ok = ok && alloc(&e->clk, (size_t)count * 16 * 8) && alloc((void**)&e->slot, (size_t)count * 8);
+  ok = ok && alloc(&e->clk, block_bytes(e));
   const size_t isz = e->es == 8 ? sizeof(EnsInst<double>) : sizeof(EnsInst<float>);
   ok = ok && alloc(&e->inst, (size_t)count * isz);
   // tile lists, largest instances first (their tiles spread over the device first)
@@ -206,7 +220,9 @@ int shx_ens_create(const shx_config* cfg, int count, const int* sizes, shx_ens**
   }
   for (int k = 0; k < ENS_NTILES; k++)
     copied = copied && hipMemcpy(e->tiles[k], lists[k].data(), lists[k].size() * sizeof(EnsTile), hipMemcpyHostToDevice) == hipSuccess;
-  copied = copied && hipMemset(e->clk, 0, (size_t)count * 16 * 8) == hipSuccess && hipMemset(e->slot, 0, (size_t)count * 8) == hipSuccess &&
+  e->slot = reinterpret_cast<unsigned long long*>(static_cast<char*>(e->clk) + clock_slots_at(count, e->es));
+  e->epoch = reinterpret_cast<unsigned int*>(static_cast<char*>(e->clk) + clock_epochs_at(count, e->es));
+  copied = copied && hipMemset(e->clk, 0, block_bytes(e)) == hipSuccess &&
            hipMemset(e->U1[0], 0, bytes) == hipSuccess && hipMemset(e->U2[0], 0, bytes) == hipSuccess &&
            hipMemset(e->U1[1], 0, bytes) == hipSuccess && hipMemset(e->U2[1], 0, bytes) == hipSuccess &&
            hipDeviceSynchronize() == hipSuccess;
@@ -226,7 +242,7 @@ int shx_ens_destroy(shx_ens* e) {
   destroy_graphs(e);
   for (int p = 0; p < 2; p++) { (void)hipFree(e->U1[p]); (void)hipFree(e->U2[p]); }
   for (int k = 0; k < ENS_NTILES; k++) (void)hipFree(e->tiles[k]);
-  (void)hipFree(e->inst); (void)hipFree(e->clk); (void)hipFree(e->slot);
+  (void)hipFree(e->inst); (void)hipFree(e->clk);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return SHX_OK;
@@ -261,7 +277,7 @@ int shx_ens_sync(shx_ens* e) {
 int shx_ens_read_clock(shx_ens* e, int i, double out[11]) {
   if (!e || !out || i < 0 || i >= e->count) return fail(SHX_EINVAL, "bad argument");
   unsigned char buf[16 * 8];
-  HIPCHK(hipMemcpyAsync(buf, (char*)e->clk + (size_t)i * 16 * e->es, 16 * e->es, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipMemcpyAsync(buf, clock_of(e, i), 16 * e->es, hipMemcpyDeviceToHost, e->stream));
   HIPCHK(hipStreamSynchronize(e->stream));
   for (int k = 0; k < 11; k++) out[k] = e->es == 8 ? ((double*)buf)[k] : (double)((float*)buf)[k];
   return SHX_OK;
@@ -276,11 +292,10 @@ int shx_ens_profile(shx_ens* e, int steps, double ms[3]) {
   double acc[3] = {0, 0, 0};
   for (int n = 0; n < steps; n++) {
     const int p = e->cur;
+    // the clock pass is inside the step kernel: ms[0] stays 0
     HIPCHK(hipEventRecord(ev[0], e->stream));
-    e->ops.clock(e->stream, e->inst, e->count, e->clk, e->slot);
     HIPCHK(hipEventRecord(ev[1], e->stream));
-    e->ops.fused(e->stream, e->tile, e->grid, e->inst, e->tiles[e->tile], e->ntiles[e->tile], e->U1[p], e->U2[p],
-                 e->U1[1 - p], e->U2[1 - p], e->clk, e->slot);
+    enqueue_fused(e, p, e->tile);
     HIPCHK(hipEventRecord(ev[2], e->stream));
     HIPCHK(hipEventSynchronize(ev[2]));
     HIPCHK(hipGetLastError());
@@ -300,16 +315,23 @@ int shx_ens_autotune(shx_ens* e, int reps, int chosen[2]) {
   if (reps < 1) reps = 3;
   int rc = refresh_rates(e);
   if (rc) return rc;
+  // each candidate takes `reps` steps from the current states (clocks and rate slots too), which
+  // are restored before the next and at the end
   const size_t bytes = e->ncell * 4 * e->es;
-  void *b1 = nullptr, *b2 = nullptr, *cs = nullptr, *ss = nullptr;
+  void *b1 = nullptr, *b2 = nullptr, *cs = nullptr;
   HIPCHK(hipMalloc(&b1, bytes));
   HIPCHK(hipMalloc(&b2, bytes));
-  HIPCHK(hipMalloc(&cs, (size_t)e->count * 16 * 8));
-  HIPCHK(hipMalloc(&ss, (size_t)e->count * 8));
-  HIPCHK(hipMemcpyAsync(b1, e->U1[e->cur], bytes, hipMemcpyDeviceToDevice, e->stream));
-  HIPCHK(hipMemcpyAsync(b2, e->U2[e->cur], bytes, hipMemcpyDeviceToDevice, e->stream));
-  HIPCHK(hipMemcpyAsync(cs, e->clk, (size_t)e->count * 16 * 8, hipMemcpyDeviceToDevice, e->stream));
-  HIPCHK(hipMemcpyAsync(ss, e->slot, (size_t)e->count * 8, hipMemcpyDeviceToDevice, e->stream));
+  HIPCHK(hipMalloc(&cs, block_bytes(e)));
+  const int start = e->cur;
+  HIPCHK(hipMemcpyAsync(b1, e->U1[start], bytes, hipMemcpyDeviceToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(b2, e->U2[start], bytes, hipMemcpyDeviceToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(cs, e->clk, block_bytes(e), hipMemcpyDeviceToDevice, e->stream));
+  auto restore = [&]() -> int {
+    HIPCHK(hipMemcpyAsync(e->U1[start], b1, bytes, hipMemcpyDeviceToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->U2[start], b2, bytes, hipMemcpyDeviceToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->clk, cs, block_bytes(e), hipMemcpyDeviceToDevice, e->stream));
+    return SHX_OK;
+  };
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -318,13 +340,13 @@ int shx_ens_autotune(shx_ens* e, int reps, int chosen[2]) {
   const int cands[] = {ENS_32x32, ENS_32x16, ENS_20x20, ENS_ROWS8, ENS_ROWS16, ENS_ROWS32};
   for (int k : cands) {
     if (e->es == 8 && k != ENS_20x20) continue;
-    const int p = e->cur;
-    e->ops.fused(e->stream, k, e->grid, e->inst, e->tiles[k], e->ntiles[k], e->U1[p], e->U2[p], e->U1[1 - p],
-                 e->U2[1 - p], e->clk, e->slot);   // warm
-    HIPCHK(hipEventRecord(e0, e->stream));
-    for (int r = 0; r < reps; r++)
-      e->ops.fused(e->stream, k, e->grid, e->inst, e->tiles[k], e->ntiles[k], e->U1[p], e->U2[p], e->U1[1 - p],
-                   e->U2[1 - p], e->clk, e->slot);
+    rc = restore();
+    if (rc) return rc;
+    // a warm step, then `reps` timed steps
+    for (int r = 0, p = start; r <= reps; r++, p = 1 - p) {
+      if (r == 1) HIPCHK(hipEventRecord(e0, e->stream));
+      enqueue_fused(e, p, k);
+    }
     HIPCHK(hipEventRecord(e1, e->stream));
     HIPCHK(hipEventSynchronize(e1));
     HIPCHK(hipGetLastError());
@@ -335,12 +357,11 @@ int shx_ens_autotune(shx_ens* e, int reps, int chosen[2]) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   e->tile = bt;
-  HIPCHK(hipMemcpyAsync(e->U1[e->cur], b1, bytes, hipMemcpyDeviceToDevice, e->stream));
-  HIPCHK(hipMemcpyAsync(e->U2[e->cur], b2, bytes, hipMemcpyDeviceToDevice, e->stream));
-  HIPCHK(hipMemcpyAsync(e->clk, cs, (size_t)e->count * 16 * 8, hipMemcpyDeviceToDevice, e->stream));
-  HIPCHK(hipMemcpyAsync(e->slot, ss, (size_t)e->count * 8, hipMemcpyDeviceToDevice, e->stream));
+  rc = restore();
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(e->stream));
-  (void)hipFree(b1); (void)hipFree(b2); (void)hipFree(cs); (void)hipFree(ss);
+  e->cur = start;
+  (void)hipFree(b1); (void)hipFree(b2); (void)hipFree(cs);
   rc = build_graphs(e);
   if (rc) return rc;
   if (chosen) ens_tile_cells(bt, chosen[0], chosen[1]);

@@ -1,8 +1,8 @@
 // Batched ensembles of independent 2D grids for gfx950: every instance of an ensemble (each its
-// own size, boundaries and clock) advances in one persistent launch per step that consumes a
-// device list of column tiles drawn from all instances. Per step:
-//   k_ens_clock   one thread per instance: the clock pass (hydro/clock.wgsl:132-242) with that
-//                 instance's largest rate, so each instance takes its own dt
+// own size, boundaries and clock) advances in one launch per step that consumes a device list of
+// column tiles drawn from all instances. The clock pass (hydro/clock.wgsl:132-242) of each
+// instance, with that instance's largest rate, is taken inside the step (StepClock): every tile
+// forms its instance's dt, the first workgroup writes every instance's next clock. Per step, either of
 //   k_ens_fused   per tile (BX x BY columns, (BX - 4) x (BY - 4) cells), one plane: primitives
 //                 into LDS, the Hancock half step of the cells and the one-column ring, each face
 //                 once, the conservative update, and the next step's rate of the instance
@@ -28,20 +28,12 @@ struct EnsInst {
   long long off;
 };
 
-template <typename T>
-__global__ void k_ens_clock(const EnsInst<T>* __restrict__ inst, int count, T* clk, unsigned long long* slot) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  const T rate = from_bits<T>(slot[i]);
-  clock_pass(inst[i].P, clk + 16 * i, rate);
-  slot[i] = 0ull;
-}
-
 // The largest rate of every instance's cells (the first step after a state change)
 template <typename T>
 __global__ __launch_bounds__(256) void k_ens_rates(const EnsInst<T>* __restrict__ inst, const EnsTile* __restrict__ tiles,
                                                    int ntiles, int tx, int ty, const T4<T>* __restrict__ U1,
-                                                   const T4<T>* __restrict__ U2, unsigned long long* slot) {
+                                                   const T4<T>* __restrict__ U2, unsigned long long* slot,
+                                                   int stride) {
   const EnsTile w = tiles[blockIdx.x];
   const Params<T>& P = inst[w.inst].P;
   const long long off = inst[w.inst].off;
@@ -53,7 +45,7 @@ __global__ __launch_bounds__(256) void k_ens_rates(const EnsInst<T>* __restrict_
     const T4<T> u2 = U2[o];
     if (u2.w <= T(0.5)) m = tmax(m, rate_of(P, prim_from_cons(P, U1[o], u2)));
   }
-  block_max_to(m, slot + w.inst);
+  block_max_to(m, slot + (size_t)stride * w.inst);
 }
 
 // LDS of the 2D tile kernel (structure of arrays over the NT columns): q (7), the upper x and y
@@ -67,8 +59,7 @@ struct Ens2dLayout {
 template <typename T, int LIM, int RS, bool ISEN, int BX, int BY>
 __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 2 : 1)) void k_ens_fused(
     const EnsInst<T>* __restrict__ inst, const EnsTile* __restrict__ tiles, int ntiles, const T4<T>* __restrict__ U1,
-    const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1, T4<T>* __restrict__ O2, const T* __restrict__ clk,
-    unsigned long long* slot) {
+    const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1, T4<T>* __restrict__ O2, StepClock<T> C, int ninst) {
   constexpr int NT = BX * BY;
   using LY = Ens2dLayout<T, NT>;
   __shared__ T L[LY::SIZE];
@@ -76,13 +67,15 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 2 : 1)) void k_ens_fused(
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int t = tx + BX * ty;
   const bool in_r1 = tx >= 1 && tx <= BX - 2 && ty >= 1 && ty <= BY - 2;
+  if (blockIdx.x == 0)
+    for (int i = t; i < ninst; i += NT) step_clock_advance(inst[i].P, C, i);
 
   for (int wi = blockIdx.x; wi < ntiles; wi += gridDim.x) {
     const EnsTile w = tiles[wi];
     const Params<T> P = inst[w.inst].P;
     const long long off = inst[w.inst].off;
     const int nx = P.n[0], ny = P.n[1];
-    const T dt = clk[16 * w.inst];
+    const T dt = step_dt(P, C, w.inst);
     const T dtdx = dt / P.dx;
     const int x = w.x0 + tx - 2, y = w.y0 + ty - 2;
     const AxisMap mx = map_axis(x, nx, P.bc_lo[0], P.bc_hi[0]);
@@ -162,7 +155,7 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 2 : 1)) void k_ens_fused(
     if (t == 0) {
       T m = T(0);
       for (int k = 0; k < (NT + 63) / 64; k++) m = tmax(m, red[k]);
-      atomicMax(slot + w.inst, (unsigned long long)order_bits(m));
+      atomicMax(C.next(w.inst), (unsigned long long)order_bits(m));
     }
   }
 }
@@ -223,15 +216,17 @@ __global__ __launch_bounds__(64) void k_ens_rows(const EnsInst<float>* __restric
                                                  const EnsTile* __restrict__ strips, int nstrips,
                                                  const float4* __restrict__ U1, const float4* __restrict__ U2,
                                                  float4* __restrict__ O1, float4* __restrict__ O2,
-                                                 const float* __restrict__ clk, unsigned long long* slot) {
+                                                 StepClock<float> C, int ninst) {
   using T = float;
   const EnsTile w = strips[blockIdx.x];
   const Params<T> P = inst[w.inst].P;
   const long long off = inst[w.inst].off;
   const int nx = P.n[0], ny = P.n[1];
-  const T dt = clk[16 * w.inst];
+  const T dt = step_dt(P, C, w.inst);
   const T dtdx = dt / P.dx;
   const int lane = threadIdx.x;
+  if (blockIdx.x == 0)
+    for (int i = lane; i < ninst; i += 64) step_clock_advance(inst[i].P, C, i);
   const int x = w.x0 - 2 + lane;
   const int y0 = w.y0, y1 = w.y1;
   const AxisMap mx = map_axis(x, nx, P.bc_lo[0], P.bc_hi[0]);
@@ -334,7 +329,7 @@ __global__ __launch_bounds__(64) void k_ens_rows(const EnsInst<float>* __restric
     pc = pp;
   }
   const T v = wave_max(rate_max);
-  if (lane == 0) atomicMax(slot + w.inst, (unsigned long long)order_bits(v));
+  if (lane == 0) atomicMax(C.next(w.inst), (unsigned long long)order_bits(v));
 }
 
 }  // namespace shx
@@ -356,58 +351,54 @@ inline void ens_tile_cells(int tile, int& tx, int& ty) {
 }
 
 struct EnsOps {
-  void (*clock)(hipStream_t, const void* inst, int count, void* clk, unsigned long long* slot);
+  // the largest rate of each instance's state into slot[stride * instance]
   void (*rates)(hipStream_t, const void* inst, const void* tiles, int ntiles, int tile, const void* u1, const void* u2,
-                unsigned long long* slot);
-  void (*fused)(hipStream_t, int tile, int grid, const void* inst, const void* tiles, int ntiles, const void* u1,
-                const void* u2, void* o1, void* o2, const void* clk, unsigned long long* slot);
+                unsigned long long* slot, int stride);
+  // one step of every instance (count of them), its clock pass included
+  void (*fused)(hipStream_t, int tile, int grid, const void* inst, int count, const void* tiles, int ntiles,
+                const void* u1, const void* u2, void* o1, void* o2, const ClockRefs& ck);
 };
 
 template <typename T, int LIM, int RS, bool ISEN>
 struct EnsImpl {
-  static void clock(hipStream_t st, const void* inst, int count, void* clk, unsigned long long* slot) {
-    hipLaunchKernelGGL(k_ens_clock<T>, dim3((count + 63) / 64), dim3(64), 0, st, (const EnsInst<T>*)inst, count,
-                       (T*)clk, slot);
-  }
   static void rates(hipStream_t st, const void* inst, const void* tiles, int ntiles, int tile, const void* u1,
-                    const void* u2, unsigned long long* slot) {
+                    const void* u2, unsigned long long* slot, int stride) {
     int tx, ty;
     ens_tile_cells(tile, tx, ty);
     hipLaunchKernelGGL(k_ens_rates<T>, dim3(ntiles), dim3(256), 0, st, (const EnsInst<T>*)inst,
-                       (const EnsTile*)tiles, ntiles, tx, ty, (const T4<T>*)u1, (const T4<T>*)u2, slot);
+                       (const EnsTile*)tiles, ntiles, tx, ty, (const T4<T>*)u1, (const T4<T>*)u2, slot, stride);
   }
   template <int BX, int BY>
-  static void fused_tile(hipStream_t st, int grid, const void* inst, const void* tiles, int ntiles, const void* u1,
-                         const void* u2, void* o1, void* o2, const void* clk, unsigned long long* slot) {
+  static void fused_tile(hipStream_t st, int grid, const void* inst, int count, const void* tiles, int ntiles,
+                         const void* u1, const void* u2, void* o1, void* o2, const ClockRefs& ck) {
     hipLaunchKernelGGL((k_ens_fused<T, LIM, RS, ISEN, BX, BY>), dim3(grid), dim3(BX, BY), 0, st,
                        (const EnsInst<T>*)inst, (const EnsTile*)tiles, ntiles, (const T4<T>*)u1, (const T4<T>*)u2,
-                       (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, slot);
+                       (T4<T>*)o1, (T4<T>*)o2, step_clock<T>(ck), count);
   }
-  static void fused(hipStream_t st, int tile, int grid, const void* inst, const void* tiles, int ntiles,
-                    const void* u1, const void* u2, void* o1, void* o2, const void* clk, unsigned long long* slot) {
+  static void fused(hipStream_t st, int tile, int grid, const void* inst, int count, const void* tiles, int ntiles,
+                    const void* u1, const void* u2, void* o1, void* o2, const ClockRefs& ck) {
     if (grid > ntiles) grid = ntiles;
     if (grid < 1) return;
     if constexpr (sizeof(T) == 8) {
-      fused_tile<20, 20>(st, grid, inst, tiles, ntiles, u1, u2, o1, o2, clk, slot);
+      fused_tile<20, 20>(st, grid, inst, count, tiles, ntiles, u1, u2, o1, o2, ck);
     } else {
       switch (tile) {
-        case ENS_32x32: fused_tile<32, 32>(st, grid, inst, tiles, ntiles, u1, u2, o1, o2, clk, slot); break;
-        case ENS_32x16: fused_tile<32, 16>(st, grid, inst, tiles, ntiles, u1, u2, o1, o2, clk, slot); break;
+        case ENS_32x32: fused_tile<32, 32>(st, grid, inst, count, tiles, ntiles, u1, u2, o1, o2, ck); break;
+        case ENS_32x16: fused_tile<32, 16>(st, grid, inst, count, tiles, ntiles, u1, u2, o1, o2, ck); break;
         case ENS_ROWS8:
         case ENS_ROWS16:
         case ENS_ROWS32:
           // one wave per strip, every strip its own workgroup
           hipLaunchKernelGGL((k_ens_rows<LIM, RS, ISEN>), dim3(ntiles), dim3(64), 0, st,
                              (const EnsInst<float>*)inst, (const EnsTile*)tiles, ntiles, (const float4*)u1,
-                             (const float4*)u2, (float4*)o1, (float4*)o2, (const float*)clk, slot);
+                             (const float4*)u2, (float4*)o1, (float4*)o2, step_clock<float>(ck), count);
           break;
-        default: fused_tile<20, 20>(st, grid, inst, tiles, ntiles, u1, u2, o1, o2, clk, slot); break;
+        default: fused_tile<20, 20>(st, grid, inst, count, tiles, ntiles, u1, u2, o1, o2, ck); break;
       }
     }
   }
   static EnsOps ops() {
     EnsOps o;
-    o.clock = clock;
     o.rates = rates;
     o.fused = fused;
     return o;
@@ -415,6 +406,7 @@ struct EnsImpl {
 };
 
 EnsOps ens_ops_f32(int lim, int rs, int isen);
+EnsOps ens_ops_other32(int lim, int rs, int isen);
 EnsOps ens_ops_f64(int lim, int rs, int isen);
 
 }  // namespace shx

@@ -28,6 +28,21 @@ inline void fused_tile_cells(int tile, int& tx, int& ty) {
   }
 }
 
+// The clocks of a fused step (the step kernel takes its own clock pass, StepClock in
+// uniform_impl.hpp), per instance: two clocks of 16 values (parity 0, 1), three rate slots, two
+// epochs; p: the parity of the state the step reads.
+struct ClockRefs {
+  void* clk;
+  unsigned long long* slot;
+  unsigned int* epoch;
+  int p;
+};
+// Bytes of the clock block of n instances: the clocks (2 x 16 values of es bytes each), the
+// slots, the epochs (offsets clock_slots_at, clock_epochs_at).
+inline size_t clock_slots_at(int n, int es) { return (size_t)n * 32 * es; }
+inline size_t clock_epochs_at(int n, int es) { return clock_slots_at(n, es) + (size_t)n * 3 * 8; }
+inline size_t clock_block_bytes(int n, int es) { return clock_epochs_at(n, es) + (size_t)n * 2 * 4; }
+
 struct UniformOps {
   void (*rates)(const Launch&, const void* u1, const void* u2, unsigned long long* slot);
   void (*clock)(const Launch&, void* clk, unsigned long long* slot);
@@ -36,9 +51,9 @@ struct UniformOps {
                  const void* clk, bool debug);
   // the fused step; za, zb: the half steps of planes 0 and nz - 1 (zplanes) when a z face is not
   // periodic, else null
-  void (*fused)(const Launch&, int tile, const void* u1, const void* u2, void* o1, void* o2, const void* clk,
-                const void* za, const void* zb, int zc, unsigned long long* next_slot, bool debug);
-  void (*zplanes)(const Launch&, const void* u1, const void* u2, void* za, void* zb, const void* clk);
+  void (*fused)(const Launch&, int tile, const void* u1, const void* u2, void* o1, void* o2, const ClockRefs& ck,
+                const void* za, const void* zb, int zc, bool debug);
+  void (*zplanes)(const Launch&, const void* u1, const void* u2, void* za, void* zb, const ClockRefs& ck);
   // the 24 diagnostics channels of the state into out (FP64), per-workgroup records in part
   void (*diag)(const Launch&, const DiagGeo&, const void* u1, const void* u2, double* part, int nblocks, double* out);
 };

@@ -72,6 +72,76 @@ __global__ void k_clock(Params<T> P, T* clk, unsigned long long* slot) {
   *slot = 0ull;
 }
 
+// ---------------- the clock inside a fused step ----------------
+// A fused step takes the clock pass itself instead of a clock launch before it (one launch per
+// step). Each instance keeps two clocks (16 values each), one per parity of the state, three rate
+// slots and two epochs (steps taken since the rates were last reset): the step from parity p
+// reads clock p and the slot of its epoch e (mod 3), the largest rate of the state it reads;
+// every workgroup forms the step's dt from them, as the clock pass (hydro/clock.wgsl:132-242)
+// does, on a copy; the rates of the state it writes go to slot e + 1. The first workgroup also
+// writes the clock pass into clock 1 - p (what the clock launch would have left in the clock:
+// the dt being taken, the time it starts from), epoch e + 1, and clears slot e + 2, last read by
+// the step before and next written by the step after. No word is both read and written within
+// one step.
+template <typename T>
+struct StepClock {
+  T* clk;                        // per instance: clocks of parity 0 and 1, 16 values each
+  unsigned long long* slot;      // per instance: 3 rate slots
+  unsigned int* epoch;           // per instance: epochs of parity 0 and 1
+  int p;                         // the parity of the state the step reads
+  __device__ __forceinline__ const T* clock_in(int i) const { return clk + 32 * i + 16 * p; }
+  __device__ __forceinline__ unsigned int epoch_in(int i) const { return epoch[2 * i + p]; }
+  __device__ __forceinline__ unsigned long long* slot_at(int i, unsigned int e) const { return slot + 3 * i + e % 3u; }
+  // where the rates of the state the step writes go
+  __device__ __forceinline__ unsigned long long* next(int i) const { return slot_at(i, epoch_in(i) + 1u); }
+};
+
+template <typename T>
+StepClock<T> step_clock(const ClockRefs& r) {
+  return StepClock<T>{static_cast<T*>(r.clk), r.slot, r.epoch, r.p};
+}
+
+// A value every lane of the wave holds alike, as a scalar register (so that it does not take a
+// vector register for the rest of the kernel).
+__device__ __forceinline__ float wave_uniform(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double wave_uniform(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// The clock pass of instance i on a copy of its clock: c[0] is the dt of the step.
+template <typename T>
+__device__ __forceinline__ void step_clock_pass(const Params<T>& P, const StepClock<T>& C, int i, T c[11]) {
+  const T* in = C.clock_in(i);
+  for (int k = 0; k < 11; k++) c[k] = in[k];
+  clock_pass(P, c, from_bits<T>(*C.slot_at(i, C.epoch_in(i))));
+}
+
+// The dt of the step instance i takes. i is the same for every lane of a wave.
+template <typename T>
+__device__ __forceinline__ T step_dt(const Params<T>& P, const StepClock<T>& C, int i) {
+  T c[11];
+  step_clock_pass(P, C, i, c);
+  return wave_uniform(c[0]);
+}
+
+// By one thread of the step, for instance i: the clock the next step reads, its epoch, and the
+// rate slot the next step writes cleared.
+template <typename T>
+__device__ __forceinline__ void step_clock_advance(const Params<T>& P, const StepClock<T>& C, int i) {
+  T c[11];
+  step_clock_pass(P, C, i, c);
+  T* out = C.clk + 32 * i + 16 * (1 - C.p);
+  for (int k = 0; k < 11; k++) out[k] = c[k];
+  const unsigned int e = C.epoch_in(i);
+  C.epoch[2 * i + 1 - C.p] = e + 1u;
+  *C.slot_at(i, e + 2u) = 0ull;
+}
+
 template <typename T>
 __device__ __forceinline__ size_t cell_index(const Params<T>& P, int i, int j, int k) {
   return ((size_t)k * P.n[1] + j) * P.n[0] + i;
@@ -154,13 +224,13 @@ __global__ __launch_bounds__(256) void k_predict(Params<T> P, const T4<T>* __res
 template <typename T, int LIM, bool ISEN>
 __global__ __launch_bounds__(256) void k_zplanes(Params<T> P, const T4<T>* __restrict__ U1,
                                                  const T4<T>* __restrict__ U2, T4<T>* __restrict__ ZA,
-                                                 T4<T>* __restrict__ ZB, const T* clk) {
+                                                 T4<T>* __restrict__ ZB, StepClock<T> C) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= P.n[0] || j >= P.n[1]) return;
   const int k = blockIdx.z == 0 ? 0 : P.n[2] - 1;
   T4<T> a, b;
-  predict_cell<T, LIM, ISEN>(P, U1, U2, i, j, k, clk[0], a, b);
+  predict_cell<T, LIM, ISEN>(P, U1, U2, i, j, k, step_dt(P, C, 0), a, b);
   const size_t o = (size_t)blockIdx.z * P.n[0] * P.n[1] + (size_t)j * P.n[0] + i;
   ZA[o] = a;
   ZB[o] = b;
@@ -364,8 +434,7 @@ __device__ __forceinline__ void zghost_half(const Params<T>& P, const T4<T>* __r
 template <typename T, int LIM, int RS, bool ISEN, int BX, int BY, bool DEBUG, bool MAPBC>
 __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 4 : 1)) void k_fused(
     Params<T> P, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1,
-    T4<T>* __restrict__ O2, const T* __restrict__ clk, const T4<T>* __restrict__ ZA, const T4<T>* __restrict__ ZB,
-    int zc, unsigned long long* next_slot) {
+    T4<T>* __restrict__ O2, StepClock<T> C, const T4<T>* __restrict__ ZA, const T4<T>* __restrict__ ZB, int zc) {
   constexpr int NT = BX * BY;
   constexpr int TX = BX - 4, TY = BY - 4;
   using LY = FusedLayout<T, NT>;
@@ -378,8 +447,9 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 4 : 1)) void k_fused(
   const int y = blockIdx.y * TY + ty - 2;
   const int z0 = blockIdx.z * zc;
   const int z1 = min(z0 + zc, nz);
-  const T dt = clk[0];
+  const T dt = step_dt(P, C, 0);
   const T dtdx = dt / P.dx;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && t == 0) step_clock_advance(P, C, 0);
 
   // this column's source through the x/y boundaries (clamped in range for the ragged rim,
   // whose results are never used)
@@ -566,7 +636,7 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 4 : 1)) void k_fused(
     if (t == 0) {
       T m = T(0);
       for (int w = 0; w < (NT + 63) / 64; w++) m = tmax(m, red[w]);
-      atomicMax(next_slot, (unsigned long long)order_bits(m));
+      atomicMax(C.next(0), (unsigned long long)order_bits(m));
     }
   }
 }
@@ -617,8 +687,8 @@ struct UniformImpl {
     }
   }
   template <int BX, int BY>
-  static void fused_tile(const Launch& L, const void* u1, const void* u2, void* o1, void* o2, const void* clk,
-                         const void* za, const void* zb, int zc, unsigned long long* next_slot, bool debug) {
+  static void fused_tile(const Launch& L, const void* u1, const void* u2, void* o1, void* o2, const ClockRefs& ck,
+                         const void* za, const void* zb, int zc, bool debug) {
     const Params<T>& P = *static_cast<const Params<T>*>(L.params);
     constexpr int TX = BX - 4, TY = BY - 4;
     dim3 g((P.n[0] + TX - 1) / TX, (P.n[1] + TY - 1) / TY, (P.n[2] + zc - 1) / zc);
@@ -626,48 +696,47 @@ struct UniformImpl {
     for (int a = 0; a < 2; a++) mapbc |= P.bc_lo[a] != BC_PERIODIC || P.bc_hi[a] != BC_PERIODIC;
 #define SHX_FUSED(D, M)                                                                                          \
   hipLaunchKernelGGL((k_fused<T, LIM, RS, ISEN, BX, BY, D, M>), g, dim3(BX, BY), 0, L.stream, P, (const T4<T>*)u1, \
-                     (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, (const T4<T>*)za, (const T4<T>*)zb, zc,  \
-                     next_slot)
+                     (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, step_clock<T>(ck), (const T4<T>*)za, (const T4<T>*)zb, zc)
     if (debug) { if (mapbc) SHX_FUSED(true, true); else SHX_FUSED(true, false); }
     else { if (mapbc) SHX_FUSED(false, true); else SHX_FUSED(false, false); }
 #undef SHX_FUSED
   }
   template <int TX, int TY>
-  static void fused_roles(const Launch& L, const void* u1, const void* u2, void* o1, void* o2, const void* clk,
-                          const void* za, const void* zb, int zc, unsigned long long* next_slot, bool debug) {
+  static void fused_roles(const Launch& L, const void* u1, const void* u2, void* o1, void* o2, const ClockRefs& ck,
+                          const void* za, const void* zb, int zc, bool debug) {
     const Params<T>& P = *static_cast<const Params<T>*>(L.params);
     dim3 g((P.n[0] + TX - 1) / TX, (P.n[1] + TY - 1) / TY, (P.n[2] + zc - 1) / zc);
     bool mapbc = za != nullptr;
     for (int a = 0; a < 2; a++) mapbc |= P.bc_lo[a] != BC_PERIODIC || P.bc_hi[a] != BC_PERIODIC;
 #define SHX_FUSED_R(D, M)                                                                                            \
   hipLaunchKernelGGL((k_fused_r<T, LIM, RS, ISEN, TX, TY, D, M>), g, dim3(RoleTile<TX, TY>::NTH), 0, L.stream, P,     \
-                     (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, (const T4<T>*)za, \
-                     (const T4<T>*)zb, zc, next_slot)
+                     (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)o1, (T4<T>*)o2, step_clock<T>(ck), (const T4<T>*)za, \
+                     (const T4<T>*)zb, zc)
     if (debug) { if (mapbc) SHX_FUSED_R(true, true); else SHX_FUSED_R(true, false); }
     else { if (mapbc) SHX_FUSED_R(false, true); else SHX_FUSED_R(false, false); }
 #undef SHX_FUSED_R
   }
   // za, zb: the half steps of planes 0 and nz - 1 for a non-periodic z face (null: z periodic)
-  static void fused(const Launch& L, int tile, const void* u1, const void* u2, void* o1, void* o2, const void* clk,
-                    const void* za, const void* zb, int zc, unsigned long long* next_slot, bool debug) {
+  static void fused(const Launch& L, int tile, const void* u1, const void* u2, void* o1, void* o2, const ClockRefs& ck,
+                    const void* za, const void* zb, int zc, bool debug) {
     if constexpr (sizeof(T) == 8) {
-      fused_tile<20, 20>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug);
+      fused_tile<20, 20>(L, u1, u2, o1, o2, ck, za, zb, zc, debug);
     } else {
       switch (tile) {
-        case FUSED_R32x16: fused_roles<32, 16>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
-        case FUSED_R32x8: fused_roles<32, 8>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
-        case FUSED_32x32: fused_tile<32, 32>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
-        case FUSED_32x16: fused_tile<32, 16>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
+        case FUSED_R32x16: fused_roles<32, 16>(L, u1, u2, o1, o2, ck, za, zb, zc, debug); break;
+        case FUSED_R32x8: fused_roles<32, 8>(L, u1, u2, o1, o2, ck, za, zb, zc, debug); break;
+        case FUSED_32x32: fused_tile<32, 32>(L, u1, u2, o1, o2, ck, za, zb, zc, debug); break;
+        case FUSED_32x16: fused_tile<32, 16>(L, u1, u2, o1, o2, ck, za, zb, zc, debug); break;
         // 16 x 16 cells: threads by role in FP32
-        default: fused_roles<16, 16>(L, u1, u2, o1, o2, clk, za, zb, zc, next_slot, debug); break;
+        default: fused_roles<16, 16>(L, u1, u2, o1, o2, ck, za, zb, zc, debug); break;
       }
     }
   }
-  static void zplanes(const Launch& L, const void* u1, const void* u2, void* za, void* zb, const void* clk) {
+  static void zplanes(const Launch& L, const void* u1, const void* u2, void* za, void* zb, const ClockRefs& ck) {
     const Params<T>& P = *static_cast<const Params<T>*>(L.params);
     dim3 b(16, 16, 1), g((P.n[0] + 15) / 16, (P.n[1] + 15) / 16, 2);
     hipLaunchKernelGGL((k_zplanes<T, LIM, ISEN>), g, b, 0, L.stream, P, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)za,
-                       (T4<T>*)zb, (const T*)clk);
+                       (T4<T>*)zb, step_clock<T>(ck));
   }
   static void diag(const Launch& L, const DiagGeo& G, const void* u1, const void* u2, double* part, int nblocks,
                    double* out) {

@@ -92,8 +92,7 @@ __device__ __forceinline__ void role_column(int t, int& cx, int& cy) {
 template <typename T, int LIM, int RS, bool ISEN, int TX, int TY, bool DEBUG, bool MAPBC>
 __global__ __launch_bounds__((RoleTile<TX, TY>::NTH), (RoleTile<TX, TY>::MINW)) void k_fused_r(
     Params<T> P, const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1,
-    T4<T>* __restrict__ O2, const T* __restrict__ clk, const T4<T>* __restrict__ ZA, const T4<T>* __restrict__ ZB, int zc,
-    unsigned long long* next_slot) {
+    T4<T>* __restrict__ O2, StepClock<T> C, const T4<T>* __restrict__ ZA, const T4<T>* __restrict__ ZB, int zc) {
   using RT = RoleTile<TX, TY>;
   constexpr int NT = RT::NC, BX = RT::CX, BY = RT::CY;
   using LY = RoleLayout<NT>;
@@ -109,8 +108,9 @@ __global__ __launch_bounds__((RoleTile<TX, TY>::NTH), (RoleTile<TX, TY>::MINW)) 
   const int y = blockIdx.y * TY + cy;
   const int z0 = blockIdx.z * zc;
   const int z1 = min(z0 + zc, nz);
-  const T dt = clk[0];
+  const T dt = step_dt(P, C, 0);
   const T dtdx = dt / P.dx;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && tid == 0) step_clock_advance(P, C, 0);
 
   AxisMap mx = map_axis(x, nx, P.bc_lo[0], P.bc_hi[0]);
   AxisMap my = map_axis(y, ny, P.bc_lo[1], P.bc_hi[1]);
@@ -298,7 +298,7 @@ __global__ __launch_bounds__((RoleTile<TX, TY>::NTH), (RoleTile<TX, TY>::MINW)) 
     if (tid == 0) {
       T m = T(0);
       for (int w = 0; w < (NT + 63) / 64; w++) m = tmax(m, red[w]);
-      atomicMax(next_slot, (unsigned long long)order_bits(m));
+      atomicMax(C.next(0), (unsigned long long)order_bits(m));
     }
   }
 }

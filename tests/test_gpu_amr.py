@@ -121,6 +121,19 @@ def test_amr_fp32_matches_oracle_refined(lim, isen, compensated, path):
 
 
 @pytest.mark.parametrize("path", ["fused", "passes"])
+@pytest.mark.parametrize("lim,rs,isen", [("vanleer", "hllc", False), ("minmod", "hll", False), ("koren", "hll", True),
+                                         ("mc", "hll", False)])
+def test_amr_other_schemes_match_oracle(lim, rs, isen, path):
+    """Every limiter and solver the reference offers (src/problems/types.ts:81-84) on blocks."""
+    from dataclasses import replace
+    blocks = refined_tree()
+    cfg = replace(cfg_for((2, 2, 2), limiter=lim, isentropic=isen), riemann=rs)
+    worst, _, _ = run_pair(blocks, (2, 2, 2), cfg, 3, path=path)
+    print("L1", lim, rs, isen, path, worst)
+    assert max(worst.values()) <= L1_TOL, worst
+
+
+@pytest.mark.parametrize("path", ["fused", "passes"])
 def test_amr_fp32_matches_oracle_blast_walls(path):
     blocks = small_blast()
     cfg = cfg_for((4, 4, 4), bc_lo=(2, 1, 0), bc_hi=(1, 2, 0))

@@ -26,10 +26,11 @@ def sim_cfg(prm, path, precision=32):
 
 
 def l1(a, b):
+    """The bar's fields: density, momentum and total energy. (rho e taken as E - KE carries E's
+    rounding times E / rho e, 223 at Mach 20: it is checked against the closed forms instead.)"""
     out = {"rho": np.abs(a[0][:, 0] - b[0][:, 0]).sum() / np.abs(b[0][:, 0]).sum(),
            "mom": np.abs(a[0][:, 1:4] - b[0][:, 1:4]).sum() / max(np.abs(b[0][:, 1:4]).sum(), 1e-300),
-           "E": np.abs(a[1][:, 0] - b[1][:, 0]).sum() / np.abs(b[1][:, 0]).sum(),
-           "eint": np.abs(a[1][:, 1] - b[1][:, 1]).sum() / np.abs(b[1][:, 1]).sum()}
+           "E": np.abs(a[1][:, 0] - b[1][:, 0]).sum() / np.abs(b[1][:, 0]).sum()}
     return out
 
 
@@ -107,11 +108,13 @@ def test_fluff_limits_on_gpu(path, dims):
 
 @pytest.mark.parametrize("path", ["fused", "passes"])
 @pytest.mark.parametrize("g", [(0.0, 2.0, 0.0), (0.8, -1.2, 1.5)])
-@pytest.mark.parametrize("v", [(1.0, 1.0, 0.0), (-0.5, 1.0, 0.7)])
+@pytest.mark.parametrize("v", [(1.0, 0.8, 0.0), (-0.5, 1.0, 0.7)])
 def test_amr_rim_mask_on_gpu(path, g, v):
     """The rim mask and the edge texels' shock flags: FP32 blocks against the FP32 oracle (the L1
     bar) and against the closed-form error field to FP32 round-off; FP64 blocks against the closed
-    form to 1e-9 of its largest value."""
+    form to 1e-9 of its largest value. (No two velocity components of equal size: a corner ghost's
+    convergence would then be 0 exactly, and FP32's v = m * rcp(rho) decides its sign by a
+    rounding.)"""
     blocks, lin = rim_case(v, g)
     t = AmrTable(blocks, RIM_ROOT)
     n = tuple(16 * r for r in RIM_ROOT)
@@ -138,6 +141,10 @@ def test_amr_rim_mask_on_gpu(path, g, v):
         scale = np.abs(exp).max()
         dif = np.abs(err - exp)[inner]
         print(precision, path, "rim model", dif.max(), scale)
+        bad = np.argwhere((np.abs(err - exp) > (1e-9 * scale if precision == 64 else 1.5e-6)) & inner[:, None])
+        for sb, cb in bad[:12]:
+            print("  block", blocks[sb].tolist(), "cell (x, y, z)", (cb % 16, (cb // 16) % 16, cb // 256),
+                  "gpu err", err[sb, cb], "model", exp[sb, cb])
         if precision == 64:
             assert dif.max() < 1e-9 * scale
         else:

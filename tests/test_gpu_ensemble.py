@@ -25,9 +25,9 @@ pytestmark = pytest.mark.gpu
 OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
 
 
-def shared_cfg(precision=32, limiter="mc", isentropic=False):
+def shared_cfg(precision=32, limiter="mc", isentropic=False, riemann="hllc"):
     return SimConfig(n=(64, 64, 1), ndim=2, dx=1.0 / 64, gamma=1.4, bc_lo=(1, 1, 0), bc_hi=(1, 1, 0), limiter=limiter,
-                     isentropic=isentropic, cfl=0.4, precision=precision)
+                     isentropic=isentropic, cfl=0.4, precision=precision, riemann=riemann)
 
 
 def l1(a, b):
@@ -45,8 +45,22 @@ ENS_TILES = [(28, 12), (28, 28), (16, 16), (60, 8), (60, 16), (60, 32)]
 
 @pytest.mark.parametrize("lim,isen,tile", [("mc", False, t) for t in ENS_TILES] + [("koren", True, t) for t in ENS_TILES])
 def test_ensemble_equals_standalone_runs(lim, isen, tile):
+    equals_standalone(lim, isen, tile, "hllc")
+
+
+# every other limiter and solver of the reference (src/problems/types.ts:81-84), on the row march
+# and on the tile kernel
+@pytest.mark.parametrize("tile", [(60, 16), (16, 16)])
+@pytest.mark.parametrize("lim,rs,isen", [("vanleer", "hllc", False), ("minmod", "hll", False), ("koren", "hll", True),
+                                         ("mc", "hll", False)])
+def test_ensemble_other_schemes_equal_standalone_runs(lim, rs, isen, tile):
+    equals_standalone(lim, isen, tile, rs)
+
+
+def equals_standalone(lim, isen, tile, rs):
+    """Every instance of an ensemble gives the bits of its own plain-path Sim."""
     sizes = [(64, 64), (100, 37), (128, 128), (30, 90), (200, 64)]
-    cfg = shared_cfg(limiter=lim, isentropic=isen)
+    cfg = shared_cfg(limiter=lim, isentropic=isen, riemann=rs)
     cfg.tile = (tile[0], tile[1], 0)
     ens = Ensemble(cfg, sizes)
     pbs = []
@@ -58,8 +72,9 @@ def test_ensemble_equals_standalone_runs(lim, isen, tile):
                 st = problems.quadrant_states(100 + i)
                 q = (x > 0).astype(int) + 2 * (y > 0).astype(int)
                 pb = problems.Problem(id="q", n=(nx, ny, 1), box=1.0, gamma=1.4, bc_lo=(1, 1, 0), bc_hi=(1, 1, 0),
-                                      cfl=0.4, limiter=lim, isentropic=isen,
+                                      cfl=0.4, limiter=lim, isentropic=isen, riemann=rs,
                                       prims={"rho": st[q, 0], "vx": st[q, 1], "vy": st[q, 2], "p": st[q, 3]})
+            pb.riemann = rs
             pbs.append(pb)
             ens.set_state(i, *pb.state())
         ens.step(7)
