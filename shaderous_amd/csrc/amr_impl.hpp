@@ -541,7 +541,7 @@ __global__ __launch_bounds__(256) void k_amr_diag(Params<T> P, DiagGeo G, const 
     const size_t o = (size_t)s * AMR_CELLS + c;
     const double x = (bx + i + 0.5) * h + G.org[0], y = (by + j + 0.5) * h + G.org[1];
     const double z = G.flat ? 0.0 : (bz + k + 0.5) * h + G.org[2];
-    diag_cell(P, U1[o], U2[o], dv, x, y, z, r);
+    diag_cell<T>(G, U1[o], U2[o], dv, x, y, z, r);
   }
   diag_block(r, sh, part + (size_t)blockIdx.x * DIAG_CH);
 }

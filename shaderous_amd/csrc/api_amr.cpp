@@ -358,6 +358,9 @@ DiagGeo diag_geo(const shx_config& c) {
   G.dv = 0;   // per block: h^ndim
   for (int a = 0; a < 3; a++) G.org[a] = c.org[a];
   G.flat = c.n[2] == 1 || c.ndim < 3;
+  G.gamma = c.gamma;
+  G.rho_floor = c.rho_floor;
+  G.p_floor = c.p_floor;
   return G;
 }
 

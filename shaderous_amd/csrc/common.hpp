@@ -49,6 +49,9 @@ inline DiagGeo uniform_diag_geo(const shx_config& c) {
   G.dv = c.ndim == 1 ? c.dx : c.ndim == 2 ? c.dx * c.dx : c.dx * c.dx * c.dx;
   for (int a = 0; a < 3; a++) G.org[a] = c.org[a];
   G.flat = c.n[2] == 1;
+  G.gamma = c.gamma;
+  G.rho_floor = c.rho_floor;
+  G.p_floor = c.p_floor;
   return G;
 }
 

@@ -34,16 +34,18 @@ struct DiagGeo {
   double dv;
   double org[3];
   int flat;            // a 2D run lies in the plane z = 0 (diagnostics.wgsl:55)
+  // the gas and floors as configured (not rounded to the run's precision)
+  double gamma, rho_floor, p_floor;
 };
 
 // red_cell (hydro/diagnostics.wgsl:13-98) for one cell at (x, y, z), ideal gas, no gravity,
 // burning or partial cells, accumulated into r. A solid cell (mark > 1/2) holds no gas.
 template <typename T>
-__device__ __forceinline__ void diag_cell(const Params<T>& P, T4<T> u1, T4<T> u2, double dv, double x, double y,
+__device__ __forceinline__ void diag_cell(const DiagGeo& G, T4<T> u1, T4<T> u2, double dv, double x, double y,
                                           double z, double r[DIAG_CH]) {
   if ((double)u2.w > 0.5) return;
-  const double gamma = (double)P.gamma, gm1 = gamma - 1.0;
-  const double rho_floor = (double)P.rho_floor, p_floor = (double)P.p_floor;
+  const double gamma = G.gamma, gm1 = gamma - 1.0;
+  const double rho_floor = G.rho_floor, p_floor = G.p_floor;
   const double rho = fmax((double)u1.x, rho_floor);
   const double vx = (double)u1.y / rho, vy = (double)u1.z / rho, vz = (double)u1.w / rho;
   const double eint = (double)u2.y;
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(256) void k_diag(Params<T> P, DiagGeo G, const T4<T
     const int i = o % nx, j = (o / nx) % ny, k = o / (nx * ny);
     const double x = (i + 0.5) * dx + G.org[0], y = (j + 0.5) * dx + G.org[1];
     const double z = G.flat ? 0.0 : (k + 0.5) * dx + G.org[2];
-    diag_cell(P, U1[o], U2[o], G.dv, x, y, z, r);
+    diag_cell<T>(G, U1[o], U2[o], G.dv, x, y, z, r);
   }
   diag_block(r, sh, part + (size_t)blockIdx.x * DIAG_CH);
 }
