@@ -150,7 +150,8 @@ int shx_build_hydro(shx_sim* sim, const shx_config* cfg);
  * src/sim.ts:3598-3640, 3790-3807): the copy is enqueued when the call is made, and the caller
  * waits on it later. A readback is that copy: begun on the sim's stream (after every step
  * enqueued so far), into host memory the readback owns, so the sim may step, re-mesh or be
- * destroyed meanwhile. `what`: bit 0 the state, bit 1 the diagnostics; the clock always. */
+ * destroyed meanwhile. `what`: bit 0 the state, bit 1 the diagnostics; the clock always (what = 0:
+ * the clock alone, the run loop's clockCopy, src/ui/api.ts:503-534). */
 typedef struct shx_readback shx_readback;
 enum { SHX_READ_STATE = 1, SHX_READ_DIAG = 2 };
 int shx_readback_begin(shx_sim* sim, int what, shx_readback** out);
@@ -211,6 +212,14 @@ int shx_amr_read_state_f64(shx_amr* sim, double* u1, double* u2);
 int shx_amr_step(shx_amr* sim, int count, int* taken);
 int shx_amr_sync(shx_amr* sim);
 int shx_amr_read_clock(shx_amr* sim, double out[11]);
+/* AmrSim.runToEnd (src/amr/amr_sim.ts:3749-3775): as shx_run_to_end on the block grid (the
+ * caller re-meshes between runs; batches are sized as shx_run_to_end's). */
+int shx_amr_run_to_end(shx_amr* sim, int batch, int max_steps, int* steps);
+/* AmrSim.writeParams / buildHydro (src/amr/amr_sim.ts:1677, :1107): as shx_write_params /
+ * shx_build_hydro; the tree, boundaries, precision and device of *cfg must equal the sim's. The
+ * state, tree and clock are kept. */
+int shx_amr_write_params(shx_amr* sim, const shx_config* cfg);
+int shx_amr_build_hydro(shx_amr* sim, const shx_config* cfg);
 /* AmrSim.diagnostics (amr/diagnostics.wgsl:13-117): the channels of shx_diagnostics over leaf
  * cells, each weighted by its own volume h^3 and placed at its centre (blk_pos,
  * amr/grid.wgsl:172-177). */

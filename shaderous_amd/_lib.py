@@ -93,6 +93,9 @@ def load():
         "shx_amr_step": (I, [P, I, C.POINTER(I)]),
         "shx_amr_sync": (I, [P]),
         "shx_amr_read_clock": (I, [P, D]),
+        "shx_amr_run_to_end": (I, [P, I, I, C.POINTER(I)]),
+        "shx_amr_write_params": (I, [P, C.POINTER(ShxConfig)]),
+        "shx_amr_build_hydro": (I, [P, C.POINTER(ShxConfig)]),
         "shx_amr_diagnostics": (I, [P, D]),
         "shx_amr_profile": (I, [P, I, D]),
         "shx_amr_stats": (I, [P, C.POINTER(I)]),

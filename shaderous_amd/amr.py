@@ -169,6 +169,26 @@ class AmrSim:
                 break
         return taken
 
+    def run_to_end(self, batch: int = 16, max_steps: int = 1 << 30) -> int:
+        """AmrSim.runToEnd: step until the clock holds at cfg.t_end; the clock's step count then."""
+        n = C.c_int(0)
+        check(_lib.load().shx_amr_run_to_end(self._h, batch, max_steps, C.byref(n)))
+        return n.value
+
+    def _reconfigure(self, fn, cfg):
+        cfg = replace(cfg or self.cfg, n=tuple(BS * r for r in self.root), ndim=3)
+        c = cfg.to_c()
+        check(fn(self._h, C.byref(c)))
+        self.cfg = cfg
+
+    def write_params(self, cfg: SimConfig = None):
+        """AmrSim.writeParams: the run-time fields of `cfg` (default self.cfg) from the next step on."""
+        self._reconfigure(_lib.load().shx_amr_write_params, cfg)
+
+    def build_hydro(self, cfg: SimConfig = None):
+        """AmrSim.buildHydro: the block kernels of `cfg`'s limiter, Riemann solver and reconstruction."""
+        self._reconfigure(_lib.load().shx_amr_build_hydro, cfg)
+
     def diagnostics(self) -> Diagnostics:
         out = np.zeros(24)
         check(_lib.load().shx_amr_diagnostics(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))

@@ -387,6 +387,47 @@ int read_state_t(shx_amr* s, T* u1, T* u2) {
   return check_queue(s);
 }
 
+// writeParams / buildHydro on blocks: the run-time fields (and with `scheme` the limiter, solver
+// and reconstruction) of *cfg; the tree, its storage, the boundaries, precision and device stay.
+// The parameters are kernel arguments of the captured step graphs, so those are rebuilt; the
+// fused march's first rate is taken again from the current state (it depends on gamma).
+int amr_reconfigure(shx_amr* s, const shx_config* cfg, bool scheme) {
+  if (!s || !cfg) return fail(SHX_EINVAL, "null argument");
+  const shx_config& a = s->cfg;
+  const shx_config& b = *cfg;
+  bool same = a.ndim == b.ndim && a.precision == b.precision && a.device == b.device && a.dx == b.dx;
+  for (int k = 0; k < 3; k++)
+    same = same && a.n[k] == b.n[k] && a.bc_lo[k] == b.bc_lo[k] && a.bc_hi[k] == b.bc_hi[k];
+  if (!scheme)
+    same = same && a.limiter == b.limiter && a.riemann == b.riemann && (a.isentropic != 0) == (b.isentropic != 0);
+  if (!same)
+    return fail(SHX_EINVAL, scheme ? "the grid, boundaries, precision and device of a block sim are fixed"
+                                   : "the grid, boundaries, precision, device and scheme of a block sim are fixed "
+                                     "(shx_amr_build_hydro changes the scheme)");
+  if (!(b.gamma > 1) || !(b.cfl > 0)) return fail(SHX_EINVAL, "gamma must exceed 1 and cfl be positive");
+  AmrOps ops = s->ops;
+  if (scheme) {
+    ops = amr_ops(b.precision, b.limiter, b.riemann, b.isentropic ? 1 : 0);
+    if (!ops.assemble)
+      return fail(SHX_EINVAL, "no block kernels built for precision %d limiter %d riemann %d isentropic %d",
+                  b.precision, b.limiter, b.riemann, b.isentropic);
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  int rc = check_queue(s);
+  if (rc) return rc;
+  s->ops = ops;
+  s->cfg = b;
+  s->pf = make_params<float>(b);
+  s->pd = make_params<double>(b);
+  if (s->lay.path == SHX_AMR_PATH_FUSED) {
+    HIPCHK(hipMemsetAsync(s->slot, 0, sizeof(unsigned long long), s->stream));
+    s->ops.rates(launch_of(s), lp(s, s->leaf), s->leaf.count, s->U1[s->cur], s->U2[s->cur], s->slot);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return build_graphs(s);
+}
+
 }  // namespace
 
 extern "C" {
@@ -604,6 +645,18 @@ int shx_amr_read_clock(shx_amr* s, double out[11]) {
   for (int i = 0; i < 11; i++) out[i] = s->es == 8 ? ((double*)buf)[i] : (double)((float*)buf)[i];
   return SHX_OK;
 }
+
+int shx_amr_run_to_end(shx_amr* s, int batch, int max_steps, int* steps) {
+  if (!s) return fail(SHX_EINVAL, "null sim");
+  if (!(s->cfg.t_end < 1e30)) return fail(SHX_EINVAL, "run_to_end needs a finite t_end");
+  int rc = run_to_end_loop([s](int n) { return shx_amr_step(s, n, nullptr); },
+                           [s](double* c) { return shx_amr_read_clock(s, c); }, s->cfg.t_end, batch, max_steps, steps);
+  if (rc) return rc;
+  return check_queue(s);
+}
+
+int shx_amr_write_params(shx_amr* s, const shx_config* cfg) { return amr_reconfigure(s, cfg, false); }
+int shx_amr_build_hydro(shx_amr* s, const shx_config* cfg) { return amr_reconfigure(s, cfg, true); }
 
 int shx_amr_diagnostics(shx_amr* s, double out[24]) {
   if (!s || !out) return fail(SHX_EINVAL, "bad argument");

@@ -67,12 +67,66 @@ function dims(cfg) {
   return [cfg.n, cfg.n, cfg.n];
 }
 
+/** The box side: this facade's `box`, or the reference's `problem.box` (src/problems/types.ts:258). */
+function boxOf(cfg) {
+  const box = cfg.box !== undefined ? cfg.box : cfg.problem && cfg.problem.box;
+  if (!(box > 0) && cfg.dx === undefined) throw new Error('the configuration needs a box side (box or problem.box) or dx');
+  return box;
+}
+
 /** domainLo (src/domain.ts:17-26): -box/2 on every axis of a Cartesian box. */
 function domainLo(cfg) {
-  return cfg.org ? cfg.org.slice(0, 3) : [-cfg.box / 2, -cfg.box / 2, -cfg.box / 2];
+  if (cfg.org) return cfg.org.slice(0, 3);
+  const box = cfg.box !== undefined || (cfg.problem && cfg.problem.box !== undefined) ? boxOf(cfg) : dims(cfg)[0] * cfg.dx;
+  return [-box / 2, -box / 2, -box / 2];
+}
+
+/**
+ * The tenuous-gas limits as (rho, u, v): this facade's triple, or the reference's
+ * { rho, u, v } (SimConfig.fluff, src/sim.ts:187-191).
+ */
+function fluffOf(f) {
+  if (!f) return [0, 0, 0];
+  if (Array.isArray(f)) return f.slice(0, 3);
+  return [f.rho || 0, f.u || 0, f.v || 0];
+}
+
+/**
+ * Why this backend cannot run a configuration, or null: in the manner of refUnsupported
+ * (src/sim.ts:912-940), the physics outside the hydro step it serves (the ideal gas, explicit
+ * pressure, Cartesian coordinates, boundary codes 0-3 of src/problems/boundaries.ts:20-28, no
+ * source terms). Fields this facade's own configFor() lacks are read only when present, so the
+ * reference's SimConfig and this facade's both pass through.
+ */
+function hipUnsupported(cfg) {
+  const p = cfg.problem || {};
+  if (cfg.eos !== undefined && cfg.eos !== 'ideal') return `the '${cfg.eos}' equation of state (the ideal gas only)`;
+  if (cfg.gravity !== undefined && cfg.gravity !== 'off') return `gravity '${cfg.gravity}'`;
+  if (cfg.gext && cfg.gext.some((g) => g !== 0)) return 'a uniform external field (gext)';
+  if (cfg.pressure === 'implicit') return 'the implicit pressure solver';
+  if (cfg.reference !== undefined && cfg.reference !== 'off') return 'a state stored as a deviation from a reference state';
+  if (cfg.radiation !== undefined && cfg.radiation !== 'off') return 'radiation';
+  if (cfg.cooling !== undefined && cfg.cooling !== 'off') return `cooling '${cfg.cooling}'`;
+  if (cfg.viscosity > 0 || cfg.conductivity > 0 || cfg.resistivity > 0) return 'viscosity, conduction or resistivity';
+  if (cfg.cutCells) return 'cut cells';
+  if (cfg.timeBins > 0) return 'time bins (per-block steps)';
+  if ((cfg.driving && cfg.driving.accel > 0) || cfg.dampRate > 0 || (cfg.startDamping && cfg.startDamping.rate > 0))
+    return 'driving or damping';
+  if (cfg.symmetry && cfg.symmetry.some(Boolean)) return 'mirror symmetry';
+  if (cfg.orbitOmega) return 'a rotating frame';
+  if (cfg.fluff && !Array.isArray(cfg.fluff) && cfg.fluff.va) return 'an Alfven-speed limit on tenuous gas';
+  if (p.geometry && p.geometry !== 'cartesian') return `${p.geometry} coordinates`;
+  if (p.mhd || (p.species && p.species.length) || p.cosmology || p.shear || p.frame) return 'a magnetic field, species, comoving coordinates, a shearing box or a moving frame';
+  if ((p.bodies && p.bodies.length) || p.obstacleDistances || (p.moving && p.moving.length) || p.damping) return 'obstacles or damping zones';
+  if (typeof p.masses === 'function' && p.masses(cfg.params || {}, cfg.G || 0, boxOf(cfg) / dims(cfg)[0], cfg.ndim).length > 0) return 'point masses';
+  for (const b of [...cfg.bcLo, ...cfg.bcHi])
+    if (typeof b === 'number' && (b < 0 || b > 3)) return `boundary code ${b} (periodic, outflow, reflecting and outflow-only only)`;
+  return null;
 }
 
 function toNative(cfg) {
+  const why = hipUnsupported(cfg);
+  if (why) throw new Error(`the MI355X backend does not run this configuration: ${why}`);
   const n = dims(cfg);
   return {
     n,
@@ -82,10 +136,10 @@ function toNative(cfg) {
     limiter: code(LIMITERS, cfg.limiter, 'limiter'),
     riemann: code(RIEMANN, cfg.riemann, 'riemann solver'),
     isentropic: !!cfg.isentropicRecon,
-    precision: cfg.precision,
-    path: code(PATHS, cfg.path, 'path'),
-    device: cfg.device,
-    dx: cfg.dx !== undefined ? cfg.dx : cfg.box / n[0],
+    precision: cfg.precision !== undefined ? cfg.precision : 32,
+    path: code(PATHS, cfg.path !== undefined ? cfg.path : 'fused', 'path'),
+    device: cfg.device !== undefined ? cfg.device : 0,
+    dx: cfg.dx !== undefined ? cfg.dx : boxOf(cfg) / n[0],
     gamma: cfg.eosParams.gamma,
     cfl: cfg.cfl,
     rhoFloor: cfg.rhoFloor,
@@ -93,8 +147,8 @@ function toNative(cfg) {
     etaDual: cfg.etaDual,
     shockThresh: cfg.shockThresh,
     tEnd: Number.isFinite(cfg.tEnd) ? cfg.tEnd : 1e30,
-    fluff: cfg.fluff,
-    tile: cfg.tile,
+    fluff: fluffOf(cfg.fluff),
+    tile: cfg.tile !== undefined ? cfg.tile : [0, 0, 0],
     org: domainLo(cfg),
   };
 }
@@ -133,15 +187,26 @@ function diagnosticsFrom(r, clock) {
   };
 }
 
+/** Thrown by the members of `Simulation` that serve physics or WebGPU resources this backend has not. */
+function outside(member, what) {
+  return new Error(`${member}: ${what} is outside the hydro path the MI355X backend runs`);
+}
+
 /**
- * What both grids share: the stepping members of `Simulation` and the readbacks. `n` is
- * the sim's native entry table (uniform or block).
+ * What both grids share: every member of the reference's `Simulation` (src/sim.ts:465-627).
+ * The stepping members and readbacks run on the GPU; the members of subsystems this path has
+ * none of behave as the reference's Sim does when the subsystem is absent (no point masses, no
+ * self-gravity, no sources: src/sim.ts:3087-3090, 3383-3386, 3400-3414, 3447-3451, 3482-3491), and
+ * those that would need WebGPU objects or physics outside the path throw. `nat` is the sim's
+ * native entry table (uniform or block).
  */
 class Base {
-  constructor(cfg, n) {
+  constructor(cfg, nat) {
     this.cfg = cfg;
-    this.n = n;
+    this.nat = nat;
     this.h = null;
+    this.cur = 0;              // the native sim keeps its own parity; readState() hands out the current state
+    this.speciesCur = 0;
     this.initial = null;       // the state reset() loads: cfg.initialState, else the first setState()
     this.diagBusy = false;
     this.preparing = false;
@@ -154,7 +219,7 @@ class Base {
    * which this path leaves to its caller: every step asked for is taken.
    */
   encodeSteps(enc, count, mayHold = false, speculate = 0) {
-    return this.n.step(live(this.h), count);
+    return this.nat.step(live(this.h), count);
   }
 
   /** Call after submitting an encoder encodeSteps() wrote to: the steps are already queued. */
@@ -171,11 +236,21 @@ class Base {
 
   /** Wait until every enqueued step is done. */
   sync() {
-    this.n.sync(live(this.h));
+    this.nat.sync(live(this.h));
   }
 
+  /** The clock (dt, t, tc, steps, ...) after every step enqueued so far; waits for them. */
   clock() {
-    return this.n.readClock(live(this.h));
+    return this.nat.readClock(live(this.h));
+  }
+
+  /**
+   * The clock as a readback: enqueued now, after the steps so far, and resolved when it lands,
+   * without blocking (the run loop's clockCopy, src/ui/api.ts:503-534).
+   */
+  async readClock() {
+    const r = await this.nat.clockAsync(live(this.h));
+    return r.clock;
   }
 
   /**
@@ -188,7 +263,7 @@ class Base {
 
   /** The state and the clock it starts from (a restart: clockState(cfg, time, steps)). */
   load(u1, u2, time = 0, steps = 0) {
-    this.n.setState(live(this.h), u1, u2, time, steps);
+    this.nat.setState(live(this.h), u1, u2, time, steps);
     if (!this.initial) this.initial = { u1: u1.slice(), u2: u2.slice() };
   }
 
@@ -200,22 +275,22 @@ class Base {
     const ic = this.cfg.initialState || this.initial;
     if (!ic) throw new Error('reset() needs cfg.initialState or a state loaded by setState()');
     const r = this.cfg.restart || {};
-    this.n.setState(live(this.h), ic.u1, ic.u2, r.time || 0, r.steps || 0);
+    this.nat.setState(live(this.h), ic.u1, ic.u2, r.time || 0, r.steps || 0);
   }
 
   /** Current conserved state (Sim.readState): the copies are enqueued now, after the steps so far. */
   async readState() {
-    const r = await this.n.readStateAsync(live(this.h));
+    const r = await this.nat.readStateAsync(live(this.h));
     return { u1: r.u1, u2: r.u2 };
   }
 
-  readStateAsync() {
+  async readStateAsync() {
     return this.readState();
   }
 
   /** Reduce and read back the diagnostics (Sim.diagnostics). */
   async diagnostics() {
-    const r = await this.n.diagnosticsAsync(live(this.h));
+    const r = await this.nat.diagnosticsAsync(live(this.h));
     return diagnosticsFrom(r.diag, r.clock);
   }
 
@@ -231,6 +306,90 @@ class Base {
     } finally {
       this.diagBusy = false;
     }
+  }
+
+  // ---- members of subsystems outside this path (src/sim.ts:465-627) ----
+
+  /** Point masses (src/sim.ts:3414-3416): a run here holds none. */
+  get massCount() {
+    return 0;
+  }
+  /** The species texture's stride (src/sim.ts:3512-3514): no species, 0. */
+  get speciesStride() {
+    return 0;
+  }
+  get device() {
+    throw outside('device', 'a WebGPU device');
+  }
+  get paramBuffer() {
+    throw outside('paramBuffer', 'a WebGPU parameter buffer');
+  }
+  /** The clock lives on the GPU as well; clock() reads it. */
+  get clockBuffer() {
+    throw outside('clockBuffer', 'a WebGPU clock buffer (read the clock with clock())');
+  }
+  get fieldShader() {
+    throw outside('fieldShader', 'the WebGPU display pass');
+  }
+  fieldResources(parity, speciesParity, radParity) {
+    throw outside('fieldResources', 'the WebGPU display pass');
+  }
+  /** No self-gravity: nothing to refresh (src/sim.ts:3087-3088 returns when no field is on). */
+  refreshPotential() {}
+  encodeBrush(enc, op) {
+    throw outside('encodeBrush', 'the editing brush');
+  }
+  async placeStar(op) {
+    throw outside('placeStar', 'placing stars');
+  }
+  async settleStar(star) {
+    throw outside('settleStar', 'settling stars');
+  }
+  /** The gas mass feeds self-gravity's mean density (src/sim.ts:3487-3491): none here. */
+  massChanged(mass) {}
+  /** No room for point masses: false, as the reference's Sim says when it is full (src/sim.ts:3404-3405). */
+  addMass(m) {
+    return false;
+  }
+  /** "null if there are none" (src/sim.ts:539-542). */
+  async readMasses() {
+    return null;
+  }
+  /** None were added (src/sim.ts:3447-3448). */
+  removeAddedMasses() {}
+  /** An index among no masses (src/sim.ts:3400-3402). */
+  setTracerFixed(index, fixed) {}
+  /** Only added masses go (src/sim.ts:3419-3421): none here. */
+  removeMasses(indices) {}
+  /** Gravitationally bound lumps (gravity/lumps.wgsl): no gravity, no read. */
+  async readGasLumps() {
+    return null;
+  }
+  retuneMasses() {}
+  /** Injection sources (src/sim.ts:3376-3380): an empty list is the run's own. */
+  setSources(list) {
+    if (list && list.length) throw outside('setSources', 'injection sources');
+  }
+  /** No debris is tracked (src/sim.ts:3007-3013 without a debris buffer). */
+  async debris() {
+    return null;
+  }
+  async debrisLedger() {
+    return null;
+  }
+  status() {
+    return '';
+  }
+
+  /**
+   * Sim.probe (src/sim.ts:3455-3480): the cell at fractional position `f` of the box, its centre,
+   * index, dx and state. The stencil (neighbours, fields) is not served.
+   */
+  async probe(f, opts) {
+    if (opts && opts.stencil) throw outside('probe', 'the stencil probe');
+    const dd = this.displayDims;
+    const cell = (x, a) => Math.min(dd[a] - 1, Math.max(0, Math.floor(x * dd[a])));
+    return this.probeCell([cell(f[0], 0), cell(f[1], 1), cell(f[2], 2)]);
   }
 
   destroy() {
@@ -249,12 +408,38 @@ class Sim extends Base {
       setState: native.setState,
       readStateAsync: native.readStateAsync,
       diagnosticsAsync: native.diagnosticsAsync,
+      clockAsync: native.clockAsync,
     });
     this.h = native.create(toNative(cfg));
     const n = dims(cfg);
+    this.dims = n;
+    this.n = n[0];
     this.nz = n[2];
     this.ncell = n[0] * n[1] * n[2];
     if (cfg.initialState) this.reset();
+  }
+
+  /** The grid the display samples: the grid itself (src/sim.ts:3494-3496). */
+  get displayDims() {
+    return this.dims.slice();
+  }
+
+  /** One cell of the current state (the probe of src/sim.ts:3455-3480 without its stencil). */
+  async probeCell(index) {
+    const { u1, u2 } = await this.readState();
+    const [nx, ny] = this.dims;
+    const c = index[0] + nx * (index[1] + ny * index[2]);
+    const nat = toNative(this.cfg);
+    const pos = index.map((i, a) => (i + 0.5) * nat.dx + nat.org[a]);
+    if (this.nz === 1) pos[2] = 0;
+    return {
+      pos,
+      index: this.nz === 1 ? index.slice(0, 2) : index.slice(),
+      dx: nat.dx,
+      u1: Array.from(u1.subarray(4 * c, 4 * c + 4)),
+      u2: Array.from(u2.subarray(4 * c, 4 * c + 4)),
+      stencil: undefined,
+    };
   }
 
   static create(cfg) {
@@ -306,15 +491,69 @@ class AmrSim extends Base {
       setState: native.amrSetState,
       readStateAsync: native.readStateAsync,
       diagnosticsAsync: native.diagnosticsAsync,
+      clockAsync: native.clockAsync,
     });
     const n = root.map((r) => 16 * r);
     const ncfg = toNative({ ...cfg, n, ndim: 3 });
-    ncfg.dx = cfg.dx !== undefined ? cfg.dx : cfg.box / n[0];
     this.h = native.amrCreate(ncfg, { root, path: code(AMR_PATHS, path, 'AMR path'), compensated }, blocks);
-    this.nz = n[2];
+    this.root = root.slice(0, 3);
+    this.dx0 = ncfg.dx;
+    this.tree(blocks);
+    if (cfg.initialState) this.reset();
+  }
+
+  /** The tree's rows, its finest cells per side (`n`, `nz`: AmrSim's, src/amr/amr_sim.ts:184). */
+  tree(blocks) {
+    this.blocks = Int32Array.from(blocks);
     this.nblocks = blocks.length / 4;
     this.ncell = this.nblocks * 4096;
-    if (cfg.initialState) this.reset();
+    let top = 0;
+    for (let i = 0; i < this.nblocks; i++) top = Math.max(top, blocks[4 * i]);
+    this.levels = top + 1;
+    this.n = (16 * this.root[0]) << top;
+    this.nz = (16 * this.root[2]) << top;
+  }
+
+  /** The uniform volume the display samples: the finest grid, at most 256 a side (src/amr/amr_sim.ts:3780-3785). */
+  get displayDims() {
+    return this.root.map((r) => Math.min((16 * r) << (this.levels - 1), 256));
+  }
+
+  /**
+   * AmrSim.probe (src/amr/amr_sim.ts:3457-3495): the leaf cell at fractional position `f` of the
+   * box (its index counted at the leaf's level), its centre, dx and state.
+   */
+  async probe(f, opts) {
+    if (opts && opts.stencil) throw outside('probe', 'the stencil probe');
+    const b = this.blocks;
+    const parent = new Set();
+    for (let i = 0; i < this.nblocks; i++)
+      if (b[4 * i] > 0) parent.add([b[4 * i] - 1, b[4 * i + 1] >> 1, b[4 * i + 2] >> 1, b[4 * i + 3] >> 1].join());
+    for (let i = 0; i < this.nblocks; i++) {
+      const L = b[4 * i];
+      const c = [b[4 * i + 1], b[4 * i + 2], b[4 * i + 3]];
+      if (parent.has([L, ...c].join())) continue;
+      const index = [0, 1, 2].map((a) => {
+        const across = (16 * this.root[a]) << L;
+        return Math.min(across - 1, Math.max(0, Math.floor(f[a] * across)));
+      });
+      if (!index.every((x, a) => Math.floor(x / 16) === c[a])) continue;
+      const { u1, u2 } = await this.readState();
+      const local = index.map((x, a) => x - 16 * c[a]);
+      const cell = 4096 * i + local[0] + 16 * (local[1] + 16 * local[2]);
+      const dx = this.dx0 / (1 << L);
+      const lo = domainLo({ ...this.cfg, dx: this.dx0, n: this.root.map((r) => 16 * r) });
+      return {
+        pos: index.map((x, a) => (x + 0.5) * dx + lo[a]),
+        index,
+        dx,
+        u1: Array.from(u1.subarray(4 * cell, 4 * cell + 4)),
+        u2: Array.from(u2.subarray(4 * cell, 4 * cell + 4)),
+        block: { level: L, coords: c, slot: i, indicator: null },
+        stencil: undefined,
+      };
+    }
+    return null;
   }
 
   static create(cfg, tree) {
@@ -341,9 +580,30 @@ class AmrSim extends Base {
    */
   regrid(blocks) {
     native.amrRegrid(live(this.h), blocks);
-    this.nblocks = blocks.length / 4;
-    this.ncell = this.nblocks * 4096;
+    this.tree(blocks);
     this.initial = null;
+  }
+
+  /** AmrSim.runToEnd (src/amr/amr_sim.ts:3749-3775): the clock's step count once it holds at tEnd. */
+  async runToEnd(batch = 16, maxSteps = 1e6) {
+    if (!Number.isFinite(this.cfg.tEnd)) throw new Error('runToEnd needs a finite tEnd');
+    return native.runToEnd(live(this.h), batch, Math.min(maxSteps, 0x7fffffff));
+  }
+
+  /** AmrSim.writeParams (src/amr/amr_sim.ts:1677): this.cfg's run-time fields, from the next step on. */
+  writeParams() {
+    native.writeParams(live(this.h), this.nativeConfig());
+  }
+
+  /** AmrSim.buildHydro (src/amr/amr_sim.ts:1107): the block kernels of this.cfg's scheme. */
+  async buildHydro() {
+    native.buildHydro(live(this.h), this.nativeConfig());
+  }
+
+  nativeConfig() {
+    const n = this.root.map((r) => 16 * r);
+    const ncfg = toNative({ ...this.cfg, n, ndim: 3, dx: this.dx0 });
+    return ncfg;
   }
 
   async profile(steps = 5) {
@@ -357,4 +617,4 @@ function createSim(cfg, tree) {
   return tree ? AmrSim.create(cfg, tree) : Sim.create(cfg);
 }
 
-module.exports = { configFor, createSim, Sim, AmrSim, diagnosticsFrom, version: native.version, native };
+module.exports = { configFor, createSim, Sim, AmrSim, diagnosticsFrom, hipUnsupported, toNative, version: native.version, native };

@@ -66,7 +66,7 @@ static Handle* get_handle(napi_env env, napi_value v, int amr) {
     return NULL;
   }
   Handle* h = (Handle*)data;
-  if (h->amr != amr) {
+  if (amr >= 0 && h->amr != amr) {
     napi_throw_type_error(env, NULL, amr ? "expected a block-grid handle" : "expected a uniform-grid handle");
     return NULL;
   }
@@ -311,16 +311,22 @@ static napi_value js_step(napi_env env, napi_callback_info info) {
   return out;
 }
 
-/* writeParams / buildHydro (src/sim.ts:2680-2700, 2434-2560): the facade's current config. */
+/* writeParams / buildHydro (src/sim.ts:2680-2700, 2434-2560; src/amr/amr_sim.ts:1677, :1107): the
+ * facade's current config, on either grid. */
 static napi_value reconfigure(napi_env env, napi_callback_info info, int scheme) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return NULL;
-  Handle* h = get_handle(env, argv[0], 0);
+  Handle* h = get_handle(env, argv[0], -1);
   if (!h) return NULL;
   shx_config c;
   config_from(env, argv[1], &c);
-  if (scheme) SHX_CALL(env, shx_build_hydro((shx_sim*)h->p, &c));
-  else SHX_CALL(env, shx_write_params((shx_sim*)h->p, &c));
+  if (h->amr) {
+    if (scheme) SHX_CALL(env, shx_amr_build_hydro((shx_amr*)h->p, &c));
+    else SHX_CALL(env, shx_amr_write_params((shx_amr*)h->p, &c));
+  } else {
+    if (scheme) SHX_CALL(env, shx_build_hydro((shx_sim*)h->p, &c));
+    else SHX_CALL(env, shx_write_params((shx_sim*)h->p, &c));
+  }
   return undefined(env);
 }
 static napi_value js_write_params(napi_env env, napi_callback_info info) { return reconfigure(env, info, 0); }
@@ -338,13 +344,14 @@ static napi_value js_sync(napi_env env, napi_callback_info info) {
 static napi_value js_run_to_end(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   if (!get_args(env, info, 3, argv)) return NULL;
-  Handle* h = get_handle(env, argv[0], 0);
+  Handle* h = get_handle(env, argv[0], -1);
   if (!h) return NULL;
   int32_t batch, max_steps;
   NAPI_CALL(env, napi_get_value_int32(env, argv[1], &batch));
   NAPI_CALL(env, napi_get_value_int32(env, argv[2], &max_steps));
   int steps = 0;
-  SHX_CALL(env, shx_run_to_end((shx_sim*)h->p, batch, max_steps, &steps));
+  if (h->amr) SHX_CALL(env, shx_amr_run_to_end((shx_amr*)h->p, batch, max_steps, &steps));
+  else SHX_CALL(env, shx_run_to_end((shx_sim*)h->p, batch, max_steps, &steps));
   napi_value out;
   NAPI_CALL(env, napi_create_int32(env, steps, &out));
   return out;
@@ -398,7 +405,7 @@ typedef struct {
   napi_async_work work;
   napi_deferred deferred;
   shx_readback* rb;    /* begun on the JS thread; the worker only waits for it */
-  int kind;            /* 0 state, 1 diagnostics */
+  int kind;            /* 0 state, 1 diagnostics, 2 the clock alone */
   int rc;
   char err[512];
 } Async;
@@ -431,7 +438,7 @@ static napi_value readback_value(napi_env env, Async* a) {
     memcpy(p2, u2, n * es);
     if (napi_set_named_property(env, o, "u1", x) != napi_ok || napi_set_named_property(env, o, "u2", y) != napi_ok)
       return NULL;
-  } else {
+  } else if (a->kind == 1) {
     napi_value d = f64_array(env, diag, 24);
     if (!d || napi_set_named_property(env, o, "diag", d) != napi_ok) return NULL;
   }
@@ -460,7 +467,7 @@ static napi_value start_async(napi_env env, Handle* h, int kind) {
   Async* a = (Async*)calloc(1, sizeof(Async));
   if (!a) { napi_throw_error(env, NULL, "out of memory"); return NULL; }
   a->kind = kind;
-  const int what = kind == 0 ? SHX_READ_STATE : SHX_READ_DIAG;
+  const int what = kind == 0 ? SHX_READ_STATE : kind == 1 ? SHX_READ_DIAG : 0;
   int rc = h->amr ? shx_amr_readback_begin((shx_amr*)h->p, what, &a->rb)
                   : shx_readback_begin((shx_sim*)h->p, what, &a->rb);
   if (rc != SHX_OK) { free(a); return throw_status(env, rc); }
@@ -495,6 +502,17 @@ static napi_value js_diagnostics_async(napi_env env, napi_callback_info info) {
   Handle* h = (Handle*)data;
   if (!h || !h->p) { napi_throw_error(env, NULL, "the simulation was destroyed"); return NULL; }
   return start_async(env, h, 1);
+}
+
+/* The clock alone, read back as the run loop's clockCopy does (src/ui/api.ts:503-534). */
+static napi_value js_clock_async(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  void* data = NULL;
+  NAPI_CALL(env, napi_get_value_external(env, argv[0], &data));
+  Handle* h = (Handle*)data;
+  if (!h || !h->p) { napi_throw_error(env, NULL, "the simulation was destroyed"); return NULL; }
+  return start_async(env, h, 2);
 }
 
 /* ---------------- block grid ---------------- */
@@ -700,6 +718,7 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT("readClock", js_read_clock),
       EXPORT("diagnostics", js_diagnostics),
       EXPORT("diagnosticsAsync", js_diagnostics_async),
+      EXPORT("clockAsync", js_clock_async),
       EXPORT("fluxMismatch", js_flux_mismatch),
       EXPORT("profile", js_profile),
       EXPORT("amrCreate", js_amr_create),

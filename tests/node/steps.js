@@ -6,8 +6,10 @@
 // <dir>/out.json (clock, diagnostics).
 //
 // loop: the reference's API run loop (src/ui/api.ts:669-699) up to the config's tEnd: batches
-// through encodeSteps(enc, n, mayHold) -> submitted() -> a clock read, stepsResume() when a
-// batch took nothing, until the clock holds (dt = 0).
+// through encodeSteps(enc, n, mayHold) -> submitted() -> the clock read back without blocking
+// (readClock, as the loop's clockCopy), stepsResume() when a batch took nothing, until the clock
+// holds (dt = 0).
+// probe: the probe() of that fractional position after the steps. toEnd: runToEnd() after them.
 // pending: readbacks left pending while the sim steps, re-meshes and is destroyed; each must
 // still resolve with the state as it was when it was asked for.
 const fs = require('fs');
@@ -18,7 +20,7 @@ async function runLoop(sim, out) {
   let grow = 1;
   const batches = [];
   for (let guard = 0; guard < 100000; guard++) {
-    const c = sim.clock();
+    const c = await sim.readClock();
     if (c[0] === 0 && c[1] > 0) break;
     const n = grow;
     const taken = sim.encodeSteps({}, n, false);
@@ -104,7 +106,13 @@ async function main() {
     fs.writeFileSync(path.join(dir, 'o2.bin'), Buffer.from(st.u2.buffer));
     out.clock = Array.from(sim.clock());
     out.d1 = d1;
+    if (job.probe) out.probe = await sim.probe(job.probe);
     if (job.kind !== 'amr') out.flux = await sim.fluxMismatch();
+    if (job.toEnd) {
+      out.toEnd = await sim.runToEnd();
+      const e = await sim.readState();
+      fs.writeFileSync(path.join(dir, 'e1.bin'), Buffer.from(e.u1.buffer));
+    }
     if (job.reset) {
       // reset() reloads the first state and its clock
       sim.reset();

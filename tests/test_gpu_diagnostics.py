@@ -131,7 +131,7 @@ def test_run_to_end_after_steps_counts_from_the_start():
         end1 = sim.read_state()
     assert np.array_equal(end0[0], end1[0]) and np.array_equal(end0[1], end1[1])
     from shaderous_amd._lib import ShxError
-    with Sim(config_for(pb)) as sim:
+    with Sim(config_for(pb, t_end=math.inf)) as sim:
         sim.set_state(u1, u2)
         with pytest.raises(ShxError):
             sim.run_to_end()
@@ -160,3 +160,45 @@ def test_write_params_and_build_hydro():
         from shaderous_amd._lib import ShxError
         with pytest.raises(ShxError):
             a.write_params(config_for(pb, n=(32, 32, 1)))
+
+
+def test_amr_write_params_build_hydro_and_run_to_end():
+    """On blocks: writeParams takes a new CFL from the next step on, buildHydro switches the
+    scheme, each run equal to a block sim created with those settings bit for bit; runToEnd then
+    stops with the clock held at t_end and returns its banked step count."""
+    from dataclasses import replace
+    blocks = amr_tree.build_tree((2, 2, 2), 1, lambda L, c: c == (0, 0, 0))
+
+    def ic(x, y, z):
+        return {"rho": 1.0 + 0.5 * np.exp(-(x * x + y * y + z * z) / 0.02), "vx": 0.2, "p": 1.0}
+    u1, u2 = amr_tree.blocks_state(blocks, 1.0 / 32, ic, 1.4)
+    base = SimConfig(n=(32, 32, 32), dx=1.0 / 32, limiter="mc", isentropic=False, cfl=0.4)
+    c2 = replace(base, cfl=0.3, t_end=0.02)
+    c3 = replace(c2, limiter="koren", isentropic=True)
+    leaf = amr_tree.leaf_mask(blocks).repeat(4096)
+    sims = [AmrSim(c, blocks, (2, 2, 2)) for c in (base, c2, c3)]
+    a, b, c = sims
+    try:
+        a.set_state(u1, u2)
+        b.set_state(u1, u2)
+        a.write_params(c2)
+        a.step(3)
+        b.step(3)
+        (a1, a2), (b1, b2) = a.read_state(), b.read_state()
+        assert np.array_equal(a1[leaf], b1[leaf]) and np.array_equal(a2[leaf], b2[leaf])
+        a.build_hydro(c3)
+        a.step(2)
+        ck = b.clock()
+        c.set_state(b1, b2, time=ck[1], steps=int(ck[3]))
+        c.step(2)
+        (a1, a2), (g1, g2) = a.read_state(), c.read_state()
+        assert np.array_equal(a1[leaf], g1[leaf]) and np.array_equal(a2[leaf], g2[leaf])
+        steps = a.run_to_end()
+        end = a.clock()
+        assert end[0] == 0 and abs(end[1] - 0.02) < 1e-6 and steps == int(end[3])
+        from shaderous_amd._lib import ShxError
+        with pytest.raises(ShxError):
+            a.write_params(replace(c2, precision=64))
+    finally:
+        for s in sims:
+            s.destroy()

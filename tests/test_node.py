@@ -20,7 +20,7 @@ pytestmark = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reaso
 
 EXPORTS = ["version", "create", "destroy", "setState", "readState", "readStateAsync", "step", "writeParams",
            "buildHydro", "sync", "runToEnd",
-           "readClock", "diagnostics", "diagnosticsAsync", "fluxMismatch", "profile", "amrCreate", "amrTable",
+           "readClock", "diagnostics", "diagnosticsAsync", "clockAsync", "fluxMismatch", "profile", "amrCreate", "amrTable",
            "amrSetState", "amrReadState", "amrStep", "amrSync", "amrReadClock", "amrDiagnostics", "amrProfile",
            "amrStats", "amrIndicator", "amrRegrid"]
 
@@ -42,23 +42,72 @@ def test_addon_exports_and_defaults():
     assert c["cfl"] == 0.8 and c["etaDual"] == 1e-3 and c["shockThresh"] == 0.5 and c["eosParams"]["gamma"] == 1.4
 
 
+# Every method of the reference's Simulation (src/sim.ts:465-627, optional members aside), and its
+# fields and getters
+SIM_METHODS = ["encodeSteps", "submitted", "stepsResume", "readDiagnostics", "diagnostics", "runToEnd", "reset",
+               "writeParams", "buildHydro", "refreshPotential", "encodeBrush", "placeStar", "settleStar",
+               "massChanged", "addMass", "readMasses", "removeAddedMasses", "setTracerFixed", "removeMasses",
+               "readGasLumps", "retuneMasses", "probe", "setSources", "destroy", "fieldResources", "status",
+               "debris", "debrisLedger"]
+SIM_GETTERS = ["device", "paramBuffer", "clockBuffer", "massCount", "displayDims", "fieldShader", "speciesStride"]
+SIM_ASYNC = ["readState", "readStateAsync", "diagnostics", "readDiagnostics", "runToEnd", "buildHydro", "placeStar",
+             "settleStar", "readMasses", "readGasLumps", "probe", "debris", "debrisLedger", "readClock"]
+
+
 def test_facade_serves_the_simulation_members():
-    """Every member of the reference's Simulation (src/sim.ts:465-627) that its run loops call
-    on this path (src/ui/api.ts:669-699, src/validation/tests.ts withSim) is on both facades,
-    and the ones the reference makes asynchronous return promises."""
+    """Both facades carry every member of the reference's Simulation (src/sim.ts:465-627), so a
+    caller assigns them to it with no cast; the ones the reference makes asynchronous return
+    promises; the members of subsystems this path has none of answer as the reference's Sim does
+    without them, and those needing WebGPU objects or other physics throw."""
     out = json.loads(node(
         "const s=require('./shaderous_amd/node');"
-        "const names=['encodeSteps','submitted','stepsResume','readDiagnostics','diagnostics','reset',"
-        "'readState','setState','step','destroy'];"
-        "const u=names.filter(n=>typeof s.Sim.prototype[n]!=='function');"
-        "const a=names.filter(n=>typeof s.AmrSim.prototype[n]!=='function');"
-        "const only=['runToEnd','writeParams','buildHydro','fluxMismatch','profile']"
-        ".filter(n=>typeof s.Sim.prototype[n]!=='function');"
-        "const isAsync=f=>f.constructor.name==='AsyncFunction';"
-        "const asy=['readState','diagnostics','readDiagnostics','runToEnd','buildHydro','fluxMismatch','profile']"
-        ".filter(n=>!isAsync(s.Sim.prototype[n]));"
-        "console.log(JSON.stringify({u,a,only,asy}))"))
-    assert out == {"u": [], "a": [], "only": [], "asy": []}, out
+        f"const M={json.dumps(SIM_METHODS)}, G={json.dumps(SIM_GETTERS)}, A={json.dumps(SIM_ASYNC)};"
+        "const has=(C,n)=>{for(let p=C.prototype;p;p=Object.getPrototypeOf(p)){"
+        "const d=Object.getOwnPropertyDescriptor(p,n);if(d)return d;}return null;};"
+        "const miss=[];for(const C of [s.Sim,s.AmrSim]){for(const n of M){const d=has(C,n);"
+        "if(!d||typeof d.value!=='function')miss.push(C.name+'.'+n);}"
+        "for(const n of G){const d=has(C,n);if(!d||!d.get)miss.push(C.name+'.'+n);}"
+        "for(const n of A){const d=has(C,n);if(!d||d.value.constructor.name!=='AsyncFunction')miss.push(C.name+'.'+n+' async');}}"
+        "const only=['fluxMismatch','profile'].filter(n=>typeof s.Sim.prototype[n]!=='function');"
+        "const b=Object.create(s.Sim.prototype);"
+        "const thrown=[];for(const n of ['device','paramBuffer','clockBuffer','fieldShader'])"
+        "{try{b[n];}catch(e){thrown.push(n);}}"
+        "for(const f of [()=>b.encodeBrush({},{}),()=>b.fieldResources(0,0),()=>b.setSources([{}])])"
+        "{try{f();}catch(e){thrown.push(/outside the hydro path/.test(e.message));}}"
+        "Promise.all([b.readMasses(),b.readGasLumps(),b.debris(),b.debrisLedger(),b.placeStar({}).catch(e=>'threw')])"
+        ".then(v=>console.log(JSON.stringify({miss,only,thrown,v,mass:b.massCount,add:b.addMass({}),"
+        "stride:b.speciesStride,empty:b.setSources([])===undefined})));"))
+    assert out["miss"] == [] and out["only"] == [], out
+    assert out["thrown"] == ["device", "paramBuffer", "clockBuffer", "fieldShader", True, True, True], out
+    assert out["v"] == [None, None, None, None, "threw"]
+    assert out["mass"] == 0 and out["add"] is False and out["stride"] == 0 and out["empty"] is True
+
+
+def test_facade_takes_the_reference_config_and_refuses_other_physics():
+    """A configuration shaped as the reference's SimConfig (the box as problem.box, fluff as
+    { rho, u, v }, numeric boundary codes, the fields of physics this path does not run left
+    off) maps to the native config this module's configFor() gives; each physics option outside
+    the path is refused before anything touches a GPU, naming it."""
+    out = json.loads(node(
+        "const s=require('./shaderous_amd/node');"
+        "const ours=s.configFor(32,{ndim:2,box:2,bcLo:['periodic','outflow',0],bcHi:['periodic','reflect',0],"
+        "fluff:[0.01,2,3],limiter:'mc',tEnd:0.5});"
+        "const ref={n:32,ndim:2,gext:[0,0,0],problem:{box:2},eos:'ideal',eosParams:{gamma:1.4,K:0,R:1},"
+        "riemann:'hllc',limiter:'mc',gravity:'off',G:0,bcLo:[0,1,0],bcHi:[0,2,0],cfl:0.8,tEnd:0.5,"
+        "rhoFloor:1e-10,pFloor:1e-14,etaDual:1e-3,shockThresh:0.5,fluff:{rho:0.01,u:2,v:3},"
+        "isentropicRecon:true,cooling:'off',viscosity:0,conductivity:0,resistivity:0,radiation:'off',"
+        "pressure:'explicit',reference:'off',cutCells:false,timeBins:0,driving:{accel:0},dampRate:0,"
+        "startDamping:{rate:0,time:0},symmetry:[false,false,false]};"
+        "const same=JSON.stringify(s.toNative(ours))===JSON.stringify(s.toNative(ref));"
+        "const bad=[{gravity:'constant'},{gext:[0,-1,0]},{eos:'barotropic'},{cooling:'tabulated'},{viscosity:1e-3},"
+        "{radiation:'m1'},{pressure:'implicit'},{timeBins:2},{bcLo:[4,1,0]},{riemann:'hllclm'},"
+        "{problem:{box:2,geometry:'spherical'}},{fluff:{rho:1,u:1,v:1,va:2}}];"
+        "const why=bad.map(o=>{try{new s.Sim({...ref,...o});return 'created';}catch(e){return e.message;}});"
+        "console.log(JSON.stringify({same,why,a:s.toNative(ref)}))"))
+    assert out["same"], out["a"]
+    assert out["a"]["dx"] == 2 / 32 and out["a"]["org"] == [-1, -1, -1] and out["a"]["fluff"] == [0.01, 2, 3]
+    for m in out["why"]:
+        assert m != "created" and ("does not run this configuration" in m or "unknown riemann" in m), m
 
 
 def test_addon_rejects_bad_arguments():
@@ -276,3 +325,67 @@ def test_node_readbacks_pending_across_steps_regrid_and_destroy(tmp_path, kind):
         assert abs(out["d0"]["mass"] - sim.diagnostics().mass) < 1e-6 * out["d0"]["mass"]
     finally:
         sim.destroy()
+
+
+@pytest.mark.gpu
+def test_node_probe_reads_the_cell(tmp_path):
+    """probe(f) (src/sim.ts:3455-3480) after three steps: the cell at fraction f of the box, its
+    centre relative to the box centre and its state, which equal the ctypes caller's cell."""
+    from shaderous_amd.sim import Sim, config_for
+    pb = problems.kelvin_helmholtz(64)
+    u1, u2 = pb.state(np.float32)
+    f = [0.3, 0.71, 0.5]
+    _, _, out = _run(tmp_path, _kh_job(pb, probe=f), u1, u2)
+    with Sim(config_for(pb)) as sim:
+        sim.set_state(u1, u2)
+        sim.step(3)
+        p1, p2 = sim.read_state()
+    i, j = int(f[0] * 64), int(f[1] * 64)
+    pr = out["probe"]
+    assert pr["index"] == [i, j]
+    assert np.allclose(pr["pos"], [(i + 0.5) * pb.dx - pb.box / 2, (j + 0.5) * pb.dx - pb.box / 2, 0.0])
+    c = i + 64 * j
+    assert pr["u1"] == p1[c].tolist() and pr["u2"] == p2[c].tolist()
+
+
+@pytest.mark.gpu
+def test_node_amr_run_to_end_and_probe_match_python(tmp_path):
+    """AmrSim.runToEnd (src/amr/amr_sim.ts:3749-3775) through the facade returns the ctypes
+    caller's step count and state; its probe finds the leaf cell at a position on the refined
+    block (its index at that block's level)."""
+    from shaderous_amd.amr import AmrSim
+    from shaderous_amd.sim import SimConfig
+    blocks = amr_tree.build_tree((2, 2, 2), 1, lambda L, c: c == (0, 0, 0))
+
+    def ic(x, y, z):
+        return {"rho": 1.0 + 0.5 * np.exp(-(x * x + y * y + z * z) / 0.02), "vx": 0.2, "p": 1.0}
+    u1, u2 = amr_tree.blocks_state(blocks, 1.0 / 32, ic, 1.4)
+    f = [0.1, 0.2, 0.3]
+    job = {"kind": "amr", "steps": 2, "probe": f, "toEnd": True,
+           "cfg": {"n": 32, "ndim": 3, "limiter": "mc", "isentropicRecon": False, "cfl": 0.4, "box": 1.0,
+                   "tEnd": 0.01},
+           "tree": {"root": [2, 2, 2], "blocks": blocks.reshape(-1).tolist(), "path": "fused"}}
+    o1, o2, out = _run(tmp_path, job, u1, u2)
+    cfg = SimConfig(n=(32, 32, 32), dx=1.0 / 32, limiter="mc", isentropic=False, cfl=0.4, t_end=0.01)
+    sim = AmrSim(cfg, blocks, (2, 2, 2))
+    try:
+        sim.set_state(u1, u2)
+        sim.step(2)
+        p1, p2 = sim.read_state()
+        steps = sim.run_to_end()
+        e1, _ = sim.read_state()
+    finally:
+        sim.destroy()
+    assert out["toEnd"] == steps
+    leaf = amr_tree.leaf_mask(blocks).repeat(4096)
+    n1 = np.fromfile(tmp_path / "job" / "e1.bin", np.float32).reshape(-1, 4)
+    assert np.array_equal(n1[leaf], e1[leaf])
+    pr = out["probe"]
+    idx = [int(x * 64) for x in f]           # the refined root block's children cover [0, 0.5) of the box
+    assert pr["index"] == idx and pr["block"]["level"] == 1 and pr["block"]["coords"] == [x // 16 for x in idx]
+    assert tuple(blocks[pr["block"]["slot"]]) == (1, *[x // 16 for x in idx])
+    slot = pr["block"]["slot"]
+    loc = [x % 16 for x in idx]
+    cell = 4096 * slot + loc[0] + 16 * (loc[1] + 16 * loc[2])
+    assert pr["u1"] == p1[cell].tolist() and pr["u2"] == p2[cell].tolist()
+    assert np.allclose(pr["pos"], [(x + 0.5) / 64 - 0.5 for x in idx])
