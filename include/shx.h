@@ -118,6 +118,17 @@ int shx_read_clock(shx_sim* sim, double out[11]);
  * scalar min/max, [16, 17] p min/max, [18, 19] speed min/max, [20..22] sum of rho dv x (x the
  * cell centre; z = 0 on a flat grid), [23] nuclear heating (0). */
 int shx_diagnostics(shx_sim* sim, double out[24]);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). The 24 channels of
+ * shx_diagnostics over a state the caller holds in device memory (u1, u2: n[0] n[1] n[2] cells
+ * of the layout above in cfg->precision; out: 24 doubles of device memory), enqueued on
+ * `stream` (a hipStream_t; null for the default stream), returning at once. It allocates
+ * nothing: the per-workgroup partial sums live in the library's file-scope device array of 256
+ * records x 24 doubles (48 KiB per GPU, held until the process ends), which every call rewrites
+ * in full before reading it. Calls that overlap on one GPU share that array and race: serialise
+ * them (one stream, or events). At most 2^31 - 1 cells (SHX_EINVAL beyond). Same kernels and
+ * summation order as shx_diagnostics: the results are bit-identical. */
+int shx_diagnostics_noalloc(const shx_config* cfg, const void* u1, const void* u2, double* out, void* stream);
 /* Sim.fluxMismatch (src/sim.ts:3724-3787) with the DEBUG_FLUX override (hydro/update.wgsl:187):
  * every face's mass flux as each of its two cells computed it; *mismatched = faces whose two
  * values differ in any bit, *faces = faces compared. */
