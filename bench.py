@@ -9,7 +9,10 @@ Prints ONE JSON line: the fused one-pass path's throughput as `value`, the plain
 structured path beside it, the dominant kernel's HBM roofline (64 algorithmic bytes per
 zone-update: one read and one write of the 8-float state) timed with HIP events on the sim's
 stream, and the reference CPU scheme (the C restatement of RefHydro) timed on a bounded sample
-on this host's cores.
+on this host's usable cores (its CPU affinity; the model from /proc/cpuinfo). Every workload
+also carries a per-step fraction (64 B per zone-update over the whole step's time) and a CPU
+number: RefHydro for the uniform grids (KH 256^2 in full, Sedov 128^3 for 3 steps), the C
+restatement of the block step for the AMR blast and RefHydro on each ensemble grid in turn.
 
 The block-AMR blast (root 8^3 blocks of 16^3 cells, up to 3 levels around a seeded-offset sphere
 of radius 0.1 holding P = 100 against rho = 1, P = 1, gamma 1.4; 5024 blocks, 4460 leaves, 18.3 M
@@ -49,6 +52,14 @@ TRAFFIC_MEASURED = {
 }
 
 
+TRAFFIC_SOURCE = "profiles/k_fused_turbulence_256.md"
+# HBM bytes per step of the AMR blast's fused path (the ghost shell pass and the march), from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE summed over a step's kernels, scaled as above
+AMR_TRAFFIC_MEASURED = {"blast": (int((0.45 + 0.05 + 1.85 + 0.05 + 2.00 + 0.64) * 1e9),
+                                   "profiles/k_amr_fused_blast.md: k_amr_shell texels + half steps + k_amr_fused, "
+                                   "HBM read + write per step")}
+
+
 # VALU instructions (wave64) per launch of the dominant kernel, measured with rocprofv3 --pmc
 # SQ_INSTS_VALU for the configuration they were measured on (profiles/k_fused_turbulence_256.md)
 VALU_MEASURED = {
@@ -76,6 +87,28 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """The host's usable cores (this process's CPU affinity) and CPU model (lscpu's "Model name",
+    from /proc/cpuinfo)."""
+    threads = len(os.sched_getaffinity(0))
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, model
+
+
+def step_roofline(cells, ms):
+    """64 algorithmic bytes per zone-update over a whole step's time, against the HBM peak."""
+    ach = BYTES_PER_UPDATE * cells / (ms * 1e-3)
+    return {"achieved": ach / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK, "ms": ms}
+
+
 def time_steps(sim, steps):
     sim.sync()
     t0 = time.perf_counter()
@@ -91,7 +124,7 @@ def cpu_baseline(pb, budget_s=10.0):
     work have been timed."""
     import numpy as np
     import oracle
-    cores = min(os.cpu_count() or 1, 16)
+    cores, model = host_cpu()
     oracle.RefHydro.set_threads(cores)
     prm = oracle.params(pb.n, ndim=3, dx=pb.dx, gamma=pb.gamma, cfl=pb.cfl, limiter=pb.limiter,
                         isentropic=pb.isentropic, riemann=pb.riemann)
@@ -119,23 +152,24 @@ def cpu_baseline(pb, budget_s=10.0):
         slabs += 1
     cells1 = slabs * 16 * pb.n[0] * pb.n[1]
     oracle.RefHydro.set_threads(cores)
-    return {"value": cells / t, "unit": "zone-updates/s", "cores": cores, "kind": "port",
+    return {"value": cells / t, "unit": "zone-updates/s", "cores": cores, "cpu_model": model, "kind": "port",
             "sample": f"{reps} whole-grid updates of RefHydro (C restatement, float64, {cores} OpenMP threads) on "
                       f"the same {pb.n[0]}^3 state ({cells} zone-updates, {t:.1f} s)",
             "single_thread": {"value": cells1 / t1, "cores": 1,
                               "sample": f"{slabs} slabs of 16 planes, one thread ({cells1} zone-updates, {t1:.1f} s)"}}
 
 
-def cpu_baseline_kh(pb, cores, steps=100):
-    """RefHydro on the 2D KH grid, run in full (`steps` steps with dt from the state's own
-    rates, src/reference/hydro.ts:172-458 restated in C, float64) at `cores` threads, and its
-    first fifth at one thread."""
+def cpu_baseline_steps(pb, steps=100):
+    """RefHydro on a uniform grid, run for `steps` steps with dt from the state's own rates
+    (src/reference/hydro.ts:172-458 restated in C, float64) at all usable cores, and its first
+    fifth at one thread."""
     import numpy as np
     import oracle
+    cores, model = host_cpu()
     out = {}
     for threads, nsteps in ((cores, steps), (1, max(steps // 5, 1))):
         oracle.RefHydro.set_threads(threads)
-        prm = oracle.params(pb.n, ndim=2, dx=pb.dx, gamma=pb.gamma, cfl=pb.cfl, limiter=pb.limiter,
+        prm = oracle.params(pb.n, ndim=pb.ndim, dx=pb.dx, gamma=pb.gamma, cfl=pb.cfl, limiter=pb.limiter,
                             isentropic=pb.isentropic, riemann=pb.riemann)
         h = oracle.RefHydro(prm)
         h.set_cons(*pb.state(np.float64))
@@ -143,13 +177,64 @@ def cpu_baseline_kh(pb, cores, steps=100):
         for _ in range(nsteps):
             h.step()
         t = time.perf_counter() - t0
+        grid = "x".join(str(v) for v in pb.n[:pb.ndim])
         out[threads] = {"value": pb.ncell * nsteps / t, "unit": "zone-updates/s", "cores": threads, "kind": "port",
-                        "sample": f"{nsteps} steps of RefHydro (C restatement, float64) on the {pb.n[0]}^2 grid, "
+                        "cpu_model": model,
+                        "sample": f"{nsteps} steps of RefHydro (C restatement, float64) on the {grid} grid, "
                                   f"{threads} thread(s), {t:.2f} s"}
     oracle.RefHydro.set_threads(cores)
     res = out[cores]
     res["single_thread"] = out[1]
     return res
+
+
+def cpu_baseline_amr(blocks, root, dx0, u1, u2, cfg):
+    """The block step on the CPU: one step of the whole blast through the C restatement of the
+    block update the GPU runs (predict, faces, coarse-fine means and update of
+    src/amr/predict.wgsl and update.wgsl; FP32, OpenMP over blocks). The reference's own CPU
+    path (RefHydro) has no blocks, so this restatement stands in for it here."""
+    import numpy as np
+    import oracle
+    cores, model = host_cpu()
+    oracle.RefHydro.set_threads(cores)
+    t = oracle.AmrTable(blocks, root)
+    prm = oracle.params(cfg.n, ndim=3, dx=dx0, gamma=cfg.gamma, cfl=cfg.cfl, limiter=cfg.limiter,
+                        isentropic=cfg.isentropic)
+    a1, a2 = u1.copy(), u2.copy()
+    t0 = time.perf_counter()
+    oracle.wgsl32.amr_step(prm, t, dx0, a1, a2, np.zeros(4, np.float32))
+    dt = time.perf_counter() - t0
+    cells = int(oracle_leaf_cells(blocks))
+    return {"value": cells / dt, "unit": "zone-updates/s", "cores": cores, "cpu_model": model, "kind": "port",
+            "sample": f"one step of the whole blast ({cells} leaf zone-updates) through the C restatement of the "
+                      f"block step (FP32, {cores} OpenMP threads over blocks), {dt:.2f} s"}
+
+
+def oracle_leaf_cells(blocks):
+    from shaderous_amd import amr_tree
+    return amr_tree.leaf_mask(blocks).sum() * 4096
+
+
+def cpu_baseline_ensemble(ns, states, cfg):
+    """RefHydro (C restatement, float64) one step on each of the ensemble's grids in turn at all
+    usable cores: the reference runs one simulation at a time."""
+    import numpy as np
+    import oracle
+    cores, model = host_cpu()
+    oracle.RefHydro.set_threads(cores)
+    t, cells = 0.0, 0
+    for n, (u1, u2) in zip(ns, states):
+        prm = oracle.params((n, n, 1), ndim=2, dx=1.0 / n, gamma=cfg.gamma, cfl=cfg.cfl, limiter=cfg.limiter,
+                            isentropic=cfg.isentropic, bc_lo=cfg.bc_lo, bc_hi=cfg.bc_hi)
+        h = oracle.RefHydro(prm)
+        h.set_cons(u1.astype(np.float64), u2.astype(np.float64))
+        t0 = time.perf_counter()
+        h.step()
+        t += time.perf_counter() - t0
+        cells += n * n
+    return {"value": cells / t, "unit": "zone-updates/s", "cores": cores, "cpu_model": model, "kind": "port",
+            "sample": f"one step of RefHydro on each of the {len(ns)} grids in turn ({cells} zone-updates, "
+                      f"{cores} OpenMP threads), {t:.2f} s"}
 
 
 def amr_bench(args):
@@ -190,13 +275,21 @@ def amr_bench(args):
            "blocks": len(blocks), "leaves": leaves, "leaf_cells": cells, "steps": args.amr_steps, "paths": res}
     if "fused" in res:
         f = res["fused"]
+        step_ms = f["passes_ms"]["step"]
         kernel_ms = f["passes_ms"]["update"]
-        ach = BYTES_PER_UPDATE * cells / (kernel_ms * 1e-3)
+        ach = BYTES_PER_UPDATE * cells / (step_ms * 1e-3)
         out["value"] = f["value"]
+        # per step: the ghost shell pass and the march (every launch of a step), by HIP events
         out["roofline"] = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                           "frac": ach / HBM_PEAK, "kernel": "k_amr_fused", "kernel_ms": kernel_ms}
+                           "frac": ach / HBM_PEAK, "per": "step (shell + march)", "step_ms": step_ms,
+                           "traffic": AMR_TRAFFIC_MEASURED["blast"][0], "traffic_source": AMR_TRAFFIC_MEASURED["blast"][1],
+                           "march": {"kernel": "k_amr_fused", "kernel_ms": kernel_ms,
+                                     "frac": BYTES_PER_UPDATE * cells / (kernel_ms * 1e-3) / HBM_PEAK}}
         if "levels" in res:
             out["fused_over_plain_levels"] = f["value"] / res["levels"]["value"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_amr(blocks, (8, 8, 8), dx0, u1, u2, cfg)
+        log(f"[bench] AMR CPU: {out['cpu_baseline']['value']:.3e} upd/s")
     return out
 
 
@@ -214,9 +307,10 @@ def ensemble_bench(args):
     if args.tile2d:
         cfg = replace(cfg, tile=(*args.tile2d, 1))
     ens = Ensemble(cfg, [(n, n) for n in ns])
+    states = [problems.riemann_quadrants(n, seed=1000 + i).state() for i, n in enumerate(ns)]
     try:
-        for i, n in enumerate(ns):
-            ens.set_state(i, *problems.riemann_quadrants(n, seed=1000 + i).state())
+        for i, st in enumerate(states):
+            ens.set_state(i, *st)
         tile = tuple(args.tile2d) if args.tile2d else ens.autotune(reps=3)
         ens.step(5)
         ens.sync()
@@ -233,12 +327,17 @@ def ensemble_bench(args):
     ach = BYTES_PER_UPDATE * cells / (prof["fused"] * 1e-3)
     # 60-cell-wide tiles are the row march's strips, the others the LDS tile kernel's
     kernel = "k_ens_rows" if tile[0] == 60 else "k_ens_fused"
-    return {"workload": "64 2D four-quadrant Riemann problems, sizes from {64,128,256,512}^2 (seed 42), gamma 1.4, "
-                        "outflow, PLM (MC) + HLLC, CFL 0.4, per-instance dt",
-            "cells": cells, "steps": args.ens_steps, "value": value, "ms_per_step": t / args.ens_steps * 1e3,
-            "tile": tile, "passes_ms": prof,
-            "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                         "frac": ach / HBM_PEAK, "kernel": kernel, "kernel_ms": prof["fused"]}}
+    out = {"workload": "64 2D four-quadrant Riemann problems, sizes from {64,128,256,512}^2 (seed 42), gamma 1.4, "
+                       "outflow, PLM (MC) + HLLC, CFL 0.4, per-instance dt",
+           "cells": cells, "steps": args.ens_steps, "value": value, "ms_per_step": t / args.ens_steps * 1e3,
+           "tile": tile, "passes_ms": prof,
+           "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK, "kernel": kernel, "kernel_ms": prof["fused"]},
+           "per_step": step_roofline(cells, t / args.ens_steps * 1e3)}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_ensemble(ns, states, cfg)
+        log(f"[bench] ensemble CPU: {out['cpu_baseline']['value']:.3e} upd/s")
+    return out
 
 
 # The reference's published uniform-grid throughputs (zone-updates/s; FP32, its default Koren
@@ -269,7 +368,8 @@ def uniform_run(args, pb, steps):
     value = pb.ncell * steps / t
     return {"value": value, "ms_per_step": t / steps * 1e3, "steps": steps, "tile": list(tile[:2] if flat else tile),
             "kernel": kname, "kernel_ms": prof["update"],
-            "hbm_frac": BYTES_PER_UPDATE * pb.ncell / (prof["update"] * 1e-3) / HBM_PEAK}
+            "hbm_frac": BYTES_PER_UPDATE * pb.ncell / (prof["update"] * 1e-3) / HBM_PEAK,
+            "per_step": step_roofline(pb.ncell, t / steps * 1e3)}
 
 
 def kh_bench(args):
@@ -283,7 +383,7 @@ def kh_bench(args):
         r = uniform_run(args, pb, args.kh_steps if n == 256 else max(args.kh_steps // 2, 1))
         log(f"[bench] KH {n}^2: tile {r['tile']} ({r['kernel']}), {r['ms_per_step']:.4f} ms/step, {r['value']:.3e} upd/s")
         if n == 256 and not args.no_cpu:
-            r["cpu_baseline"] = cpu_baseline_kh(pb, min(os.cpu_count() or 1, 16))
+            r["cpu_baseline"] = cpu_baseline_steps(pb, 100)
             log(f"[bench] KH 256^2 CPU: {r['cpu_baseline']['value']:.3e} upd/s")
         out[f"{n}^2"] = r
     return {"workload": "2D Kelvin-Helmholtz, unit square, periodic, gamma 5/3, rho 2 / 1, vx +-0.5, P 2.5, "
@@ -310,6 +410,9 @@ def published_bench(args):
         r = uniform_run(args, pb, steps)
         r["reference_published"] = PUBLISHED[(kind, n)]
         r["over_h200"] = r["value"] / PUBLISHED[(kind, n)]["H200 NVL"]
+        if kind == "sedov" and n == 128 and not args.no_cpu:
+            r["cpu_baseline"] = cpu_baseline_steps(pb, 3)
+            log(f"[bench] Sedov 128^3 CPU: {r['cpu_baseline']['value']:.3e} upd/s")
         log(f"[bench] {name} (Koren): tile {r['tile']} ({r['kernel']}), {r['ms_per_step']:.4f} ms/step, "
             f"{r['value']:.3e} upd/s, {r['over_h200']:.1f}x the reference on an H200")
         out[name] = r
@@ -444,8 +547,11 @@ def main():
                    "tile": tile},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": TRAFFIC_MEASURED.get((kname, n, tuple(tile or ()))),
+                     "traffic_source": (f"{TRAFFIC_SOURCE}, {kname} tile {tuple(tile)}"
+                                        if (kname, n, tuple(tile or ())) in TRAFFIC_MEASURED else None),
                      "kernel": kname, "kernel_ms": kernel_ms, "bytes_per_update": BYTES_PER_UPDATE,
                      "valu": valu_issue(kname, n, tile, kernel_ms)},
+        "per_step": step_roofline(ncell, t / args.steps * 1e3),
         "plain": {"value": plain_value, "ms_per_step": tp / args.steps * 1e3 if tp else None, "passes_ms": pprof},
         "fused_over_plain": value / plain_value if plain_value else None,
         "conserved": {"mass": diag.mass, "e_tot": diag.e_tot},

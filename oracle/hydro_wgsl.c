@@ -859,6 +859,8 @@ static Cons from_coarse(const orc_amr* t, const P_t* P, const real* u1, const re
    returns the largest rate / dx over leaf interiors */
 static real amr_assemble(const orc_amr* t, const P_t* P, double dx0, const real* u1, const real* u2, real* W) {
   real best = RL(0.0);
+  /* blocks are independent (each writes only its own padded block); max is exact in any order */
+#pragma omp parallel for schedule(dynamic, 4) reduction(max : best)
   for (int s = 0; s < t->nblocks; s++) {
     real dx = (real)(dx0 / (double)(1 << bt_(t, s, 0)));
     for (int z = -NG_; z < BS_ + NG_; z++)
@@ -911,6 +913,7 @@ static Q predict_cell(const P_t* P, const real* W, int s, const int l[3], const 
 
 /* predict (src/amr/predict.wgsl:183-221) over leaves' interiors and first ghost layer */
 static void amr_predict(const orc_amr* t, const P_t* P, double dx0, const real* W, real dt, real* PA, real* PB) {
+#pragma omp parallel for schedule(dynamic, 4)
   for (int s = 0; s < t->nblocks; s++) {
     if (!leaf_(t, s)) continue;
     for (int z = -1; z < BS_ + 1; z++)
@@ -965,6 +968,7 @@ static void two_sum(real a, real b, real* s, real* e) {
 /* update (src/amr/update.wgsl:326-560) with match_faces (:207-290) */
 static void amr_update(const orc_amr* t, const P_t* P, double dx0, int comp, const real* u1, const real* u2,
                        const real* W, const real* PA, const real* PB, real dt, real* o1, real* o2) {
+#pragma omp parallel for schedule(dynamic, 4)
   for (int s = 0; s < t->nblocks; s++) {
     if (!leaf_(t, s)) continue;
     real dx = (real)(dx0 / (double)(1 << bt_(t, s, 0)));
