@@ -85,6 +85,11 @@ def lib():
             getattr(_lib, pfx + "amr_regrid").argtypes = [C.POINTER(OrcParams), C.POINTER(OrcAmr), P, P, C.c_int,
                                                           C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                                           P, P]
+            getattr(_lib, pfx + "amr_bins").argtypes = [C.POINTER(OrcParams), C.POINTER(OrcAmr), C.c_int, P, ct,
+                                                        C.POINTER(C.c_int)]
+            getattr(_lib, pfx + "amr_cycle").argtypes = [C.POINTER(OrcParams), C.POINTER(OrcAmr), C.c_double, C.c_int,
+                                                         C.c_int, C.POINTER(C.c_int), P, P, P, C.POINTER(C.c_int),
+                                                         C.POINTER(C.c_longlong)]
         I3 = C.c_int * 3
         _lib.orc_amr_build.argtypes = [C.POINTER(OrcAmr), C.POINTER(C.c_int), C.c_int, I3, I3, I3]
         _lib.orc_amr_free.argtypes = [C.POINTER(OrcAmr)]
@@ -210,6 +215,30 @@ class Wgsl:
         assert u2.dtype == self.dtype and u2.flags.c_contiguous and u2.shape == u1.shape
         self.fn("amr_step")(C.byref(prm), C.byref(table.t), dx0, int(bool(compensated)), _ptr(u1, self.ct),
                             _ptr(u2, self.ct), _ptr(clk, self.ct))
+
+    def amr_bins(self, prm, table, kmax, brate, dt):
+        """The bins of a cycle (per block) from each block's largest rate / dx and the cycle's dt."""
+        br = self.arr(brate)
+        assert br.shape == (table.nblocks,)
+        out = np.zeros(table.nblocks, np.int32)
+        self.fn("amr_bins")(C.byref(prm), C.byref(table.t), int(kmax), _ptr(br, self.ct), dt, _ptr(out, C.c_int))
+        return out
+
+    def amr_cycle(self, prm, table, dx0, u1, u2, clk, kmax, bins=None, compensated=True):
+        """One cycle of 2^kmax substeps in place (per-block timesteps). bins: fixed bins per block
+        (None: the cycle's own). Returns (the bins used, block updates made, leaves << kmax)."""
+        assert u1.dtype == self.dtype and u1.flags.c_contiguous and u1.shape == (table.nblocks * BS ** 3, 4)
+        assert u2.dtype == self.dtype and u2.flags.c_contiguous and u2.shape == u1.shape
+        given = None
+        if bins is not None:
+            given = np.ascontiguousarray(bins, np.int32)
+            assert given.shape == (table.nblocks,)
+        out = np.zeros(table.nblocks, np.int32)
+        work = (C.c_longlong * 2)()
+        self.fn("amr_cycle")(C.byref(prm), C.byref(table.t), dx0, int(bool(compensated)), int(kmax),
+                             _ptr(given, C.c_int) if given is not None else None, _ptr(u1, self.ct), _ptr(u2, self.ct),
+                             _ptr(clk, self.ct), _ptr(out, C.c_int), work)
+        return out, int(work[0]), int(work[1])
 
     def amr_assemble(self, prm, table, dx0, u1, u2):
         """The assemble pass: (W of shape (nblocks, 20, 20, 20, 8) [z, y, x], largest leaf rate/dx)."""

@@ -31,6 +31,13 @@ typedef REAL real;
 #define CAT(a, b) CAT2(a, b)
 #define FN(name) CAT(PFX, name)
 #define RL(x) ((real)(x))
+#if REAL_IS_DOUBLE
+#define RFLOOR floor
+#define RLOG2 log2
+#else
+#define RFLOOR floorf
+#define RLOG2 log2f
+#endif
 
 /* Q: primitive record (src/hydro/q.wgsl:56): a = (rho, vx, vy, vz), b = (p, s, c_s, solid). */
 typedef struct { real a[4]; real b[4]; } Q;
@@ -718,6 +725,24 @@ static size_t iidx(int s, const int l[3]) {
   return (size_t)s * BS_ * BS_ * BS_ + ((size_t)l[2] * BS_ + l[1]) * BS_ + l[0];
 }
 
+/* Per-block timesteps (src/amr/bins_on.wgsl): the cycle's bins and the substep being taken; the
+   persistent half-step atlases PA/PB hold each block's prediction from the start of its step.
+   A NULL Bins (or k NULL) is a single timestep (bins_off.wgsl). */
+typedef struct {
+  const int* k;        /* per slot bin */
+  int sub;             /* substep of the cycle */
+  const real* PA;      /* mid-step predictions, 4 reals per padded texel */
+  const real* PB;
+} Bins;
+static int bins_on_(const Bins* B) { return B && B->k; }
+static int bin_(const Bins* B, int s) { return bins_on_(B) ? B->k[s] : 0; }
+/* blk_active / blk_half (bins_on.wgsl:14-23) */
+static int active_(const Bins* B, int s) { return !bins_on_(B) || (B->sub & ((1 << B->k[s]) - 1)) == 0; }
+static int half_(const Bins* B, int s) {
+  int k = bin_(B, s);
+  return k > 0 && (B->sub & ((1 << k) - 1)) == (1 << (k - 1));
+}
+
 /* ghost_src (src/amr/grid.wgsl:184-222) */
 typedef struct { int kind, slot, l[3]; real flip[3], side[3]; } Src;
 static Src ghost_src(const orc_amr* t, int slot, const int l[3]) {
@@ -787,6 +812,21 @@ static int u_near(const orc_amr* t, const real* u1, const real* u2, int slot, co
   *out = u;
   return 1;
 }
+/* mid_block / mid_cons (src/amr/mid_on.wgsl:10-30): a leaf part way through a longer step is
+   seen through its half-step prediction (its cells and first ghost layer) */
+static int mid_(const orc_amr* t, const Bins* B, int s) { return bins_on_(B) && leaf_(t, s) && !active_(B, s); }
+static Cons mid_cons(const P_t* P, const Bins* B, int slot, const int l[3]) {
+  size_t o = pidx(slot, l) * 4;
+  const real* a = B->PA + o;
+  const real* b = B->PB + o;
+  real rho = a[0];
+  real eint = eos_eint_from_p(P, b[0]);
+  Cons c;
+  c.a[0] = rho; c.a[1] = rho * a[1]; c.a[2] = rho * a[2]; c.a[3] = rho * a[3];
+  c.b[0] = eint + RL(0.5) * rho * ((a[1] * a[1] + a[2] * a[2]) + a[3] * a[3]);
+  c.b[1] = eint; c.b[2] = rho * b[1]; c.b[3] = RL(0.0);
+  return c;
+}
 /* limit4 of the limiter slot (MC for mc and koren: koren.wgsl:294-297) */
 static real limit1(int lim, real a, real b) {
   if (lim == ORC_LIM_MINMOD) return limit_minmod(a, b);
@@ -833,8 +873,15 @@ static Cons prolong(const P_t* P, const Cons v[7], const int ok[7], const int pa
   }
   return o;
 }
-/* from_coarse (src/amr/assemble.wgsl:47-75) */
+/* from_coarse (src/amr/assemble.wgsl:47-75); a coarser leaf half way through its step is
+   interpolated from its half-step prediction (:60-68) */
+static Cons from_coarse_b(const orc_amr* t, const P_t* P, const real* u1, const real* u2, int slot, const int l[3],
+                          int cs, const Bins* B);
 static Cons from_coarse(const orc_amr* t, const P_t* P, const real* u1, const real* u2, int slot, const int l[3], int cs) {
+  return from_coarse_b(t, P, u1, u2, slot, l, cs, NULL);
+}
+static Cons from_coarse_b(const orc_amr* t, const P_t* P, const real* u1, const real* u2, int slot, const int l[3],
+                          int cs, const Bins* B) {
   int gf[3], lc[3], par[3];
   int Lc = bt_(t, cs, 0);
   for (int a = 0; a < 3; a++) {
@@ -847,21 +894,37 @@ static Cons from_coarse(const orc_amr* t, const P_t* P, const real* u1, const re
   }
   Cons v[7];
   int ok[7];
+  int mid = mid_(t, B, cs);
   for (int k = 0; k < 7; k++) {
     int at[3] = {lc[0], lc[1], lc[2]};
     if (k > 0) at[(k - 1) / 2] += (k & 1) == 0 ? 1 : -1;
-    ok[k] = u_near(t, u1, u2, cs, at, &v[k]);
+    if (mid) {
+      v[k] = mid_cons(P, B, cs, at);
+      ok[k] = 1;
+    } else {
+      ok[k] = u_near(t, u1, u2, cs, at, &v[k]);
+    }
   }
   return prolong(P, v, ok, par);
 }
 
 /* assemble (src/amr/assemble.wgsl:77-154) into W (8 reals per texel: rho, vx, vy, vz, p, cs, s, e);
    returns the largest rate / dx over leaf interiors */
+static real amr_assemble_b(const orc_amr* t, const P_t* P, double dx0, const real* u1, const real* u2, real* W,
+                           const Bins* B, real* brate);
 static real amr_assemble(const orc_amr* t, const P_t* P, double dx0, const real* u1, const real* u2, real* W) {
+  return amr_assemble_b(t, P, dx0, u1, u2, W, NULL, NULL);
+}
+/* With bins, only the blocks that start a step on this substep (parents included: a parent's
+   bin is its fastest child's); brate (may be NULL) takes each leaf's largest rate / dx. */
+static real amr_assemble_b(const orc_amr* t, const P_t* P, double dx0, const real* u1, const real* u2, real* W,
+                           const Bins* B, real* brate) {
   real best = RL(0.0);
   /* blocks are independent (each writes only its own padded block); max is exact in any order */
 #pragma omp parallel for schedule(dynamic, 4) reduction(max : best)
   for (int s = 0; s < t->nblocks; s++) {
+    if (!active_(B, s)) continue;
+    real bbest = RL(0.0);
     real dx = (real)(dx0 / (double)(1 << bt_(t, s, 0)));
     for (int z = -NG_; z < BS_ + NG_; z++)
       for (int y = -NG_; y < BS_ + NG_; y++)
@@ -869,7 +932,8 @@ static real amr_assemble(const orc_amr* t, const P_t* P, double dx0, const real*
           int l[3] = {x, y, z};
           Src g = ghost_src(t, s, l);
           if (g.kind == 2) continue;
-          Cons u = g.kind == 0 ? u_at(t, u1, u2, g.slot, g.l) : from_coarse(t, P, u1, u2, s, l, g.slot);
+          Cons u = g.kind == 0 ? (mid_(t, B, g.slot) ? mid_cons(P, B, g.slot, g.l) : u_at(t, u1, u2, g.slot, g.l))
+                               : from_coarse_b(t, P, u1, u2, s, l, g.slot, B);
           Q w = prim_from_cons(P, u.a, u.b);
           Mapped m;
           for (int a = 0; a < 3; a++) { m.flip[a] = g.flip[a]; m.side[a] = g.side[a]; }
@@ -884,8 +948,10 @@ static real amr_assemble(const orc_amr* t, const P_t* P, double dx0, const real*
             real rate = rabs(v[0]) + rabs(v[1]) + rabs(v[2]) + (real)P->ndim * signal + RL(0.0);
             real r = rate / dx;
             if (r > best) best = r;
+            if (r > bbest) bbest = r;
           }
         }
+    if (brate) brate[s] = bbest;
   }
   return best;
 }
@@ -912,24 +978,35 @@ static Q predict_cell(const P_t* P, const real* W, int s, const int l[3], const 
 }
 
 /* predict (src/amr/predict.wgsl:183-221) over leaves' interiors and first ghost layer */
+static void amr_predict_b(const orc_amr* t, const P_t* P, double dx0, const real* W, real dt, real* PA, real* PB,
+                          const Bins* B);
 static void amr_predict(const orc_amr* t, const P_t* P, double dx0, const real* W, real dt, real* PA, real* PB) {
+  amr_predict_b(t, P, dx0, W, dt, PA, PB, NULL);
+}
+/* With bins: the leaves starting a step, each over its own step (blk_dt, amr/blk_dt.wgsl); a
+   ghost whose source leaf is part way through a longer step was filled from its prediction and
+   is predicted in place (predict.wgsl:193-197). */
+static void amr_predict_b(const orc_amr* t, const P_t* P, double dx0, const real* W, real dt, real* PA, real* PB,
+                          const Bins* B) {
 #pragma omp parallel for schedule(dynamic, 4)
   for (int s = 0; s < t->nblocks; s++) {
-    if (!leaf_(t, s)) continue;
+    if (!leaf_(t, s) || !active_(B, s)) continue;
+    real dtb = dt * (real)(1 << bin_(B, s));
     for (int z = -1; z < BS_ + 1; z++)
       for (int y = -1; y < BS_ + 1; y++)
         for (int x = -1; x < BS_ + 1; x++) {
           int l[3] = {x, y, z};
           Src g = ghost_src(t, s, l);
           if (g.kind == 2) continue;
-          int at_source = g.kind == 0;
+          int stale = bins_on_(B) && g.slot != s && leaf_(t, g.slot) && !active_(B, g.slot);
+          int at_source = g.kind == 0 && !stale;
           int slot = at_source ? g.slot : s;
           int cl[3] = {at_source ? g.l[0] : l[0], at_source ? g.l[1] : l[1], at_source ? g.l[2] : l[2]};
           real mask[3] = {1, 1, 1};
           if (!at_source)
             for (int a = 0; a < 3; a++) if (l[a] == 0 || l[a] == BS_ - 1) mask[a] = 0;
           real dx = (real)(dx0 / (double)(1 << bt_(t, slot, 0)));
-          Q pr = predict_cell(P, W, slot, cl, mask, dt, dx);
+          Q pr = predict_cell(P, W, slot, cl, mask, dtb, dx);
           if (at_source) {
             Mapped m;
             for (int a = 0; a < 3; a++) { m.flip[a] = g.flip[a]; m.side[a] = g.side[a]; }
@@ -965,19 +1042,41 @@ static void two_sum(real a, real b, real* s, real* e) {
   *s = x;
 }
 
-/* update (src/amr/update.wgsl:326-560) with match_faces (:207-290) */
+/* update (src/amr/update.wgsl:326-560) with match_faces (:207-290). With bins (share,
+   :179-186): the leaves starting a step, over their own step, and the leaves half way through
+   one that have a faster neighbour, which take only the second half of that neighbour's fluxes
+   through the shared faces (a deposit: their own fluxes are zero, cells on no such face are
+   copied). A face on a faster leaf of the same level takes half of that leaf's flux, as it
+   computes it; a face on a refined neighbour the mean of the fine fluxes times the child's
+   share (one half if faster, nothing in a deposit if in step). */
+static void amr_update_b(const orc_amr* t, const P_t* P, double dx0, int comp, const real* u1, const real* u2,
+                         const real* W, const real* PA, const real* PB, real dt, real* o1, real* o2, const Bins* B,
+                         const char* lu);
 static void amr_update(const orc_amr* t, const P_t* P, double dx0, int comp, const real* u1, const real* u2,
                        const real* W, const real* PA, const real* PB, real dt, real* o1, real* o2) {
+  amr_update_b(t, P, dx0, comp, u1, u2, W, PA, PB, dt, o1, o2, NULL, NULL);
+}
+static void amr_update_b(const orc_amr* t, const P_t* P, double dx0, int comp, const real* u1, const real* u2,
+                         const real* W, const real* PA, const real* PB, real dt, real* o1, real* o2, const Bins* B,
+                         const char* lu) {
 #pragma omp parallel for schedule(dynamic, 4)
   for (int s = 0; s < t->nblocks; s++) {
     if (!leaf_(t, s)) continue;
+    if (lu && !lu[s]) continue;
     real dx = (real)(dx0 / (double)(1 << bt_(t, s, 0)));
+    int gbin = bin_(B, s);
+    int deposit = bins_on_(B) && !active_(B, s);
+    real dtb = dt * (real)(1 << gbin);
     for (int z = 0; z < BS_; z++)
       for (int y = 0; y < BS_; y++)
         for (int x = 0; x < BS_; x++) {
           int l[3] = {x, y, z};
           real fl[3][8], fh[3][8];
           for (int axis = 0; axis < 3; axis++) {
+            if (deposit) {
+              for (int q = 0; q < 8; q++) { fl[axis][q] = RL(0.0); fh[axis][q] = RL(0.0); }
+              continue;
+            }
             int m[3] = {x, y, z};
             m[axis] -= 1;
             Flux a = amr_flux_between(P, W, PA, PB, s, m, axis);
@@ -985,43 +1084,66 @@ static void amr_update(const orc_amr* t, const P_t* P, double dx0, int comp, con
             flux_pack(&a, fl[axis]);
             flux_pack(&b, fh[axis]);
           }
-          /* faces next to a refined neighbour: the mean of the fine fluxes */
+          int touched = 0;
+          /* faces next to a refined neighbour: the mean of the fine fluxes; with bins, faces next
+             to a faster leaf of the same level: half of its flux */
           for (int k = 0; k < 6; k++) {
             int axis = k / 2, hi = k & 1;
             if (l[axis] != (hi ? BS_ - 1 : 0)) continue;
             int np = bt_(t, s, 6 + k);
-            if (np < 0 || leaf_(t, np)) continue;
+            if (np < 0) continue;
             int upper = !hi;
-            int h[3] = {l[0] >= BS_ / 2, l[1] >= BS_ / 2, l[2] >= BS_ / 2};
-            int bits[3] = {h[0], h[1], h[2]};
-            bits[axis] = upper;
-            int ch = bt_(t, np, 12 + (bits[0] | (bits[1] << 1) | (bits[2] << 2)));
-            int a0[3] = {2 * (l[0] - (BS_ / 2) * h[0]), 2 * (l[1] - (BS_ / 2) * h[1]), 2 * (l[2] - (BS_ / 2) * h[2])};
-            a0[axis] = upper ? BS_ - 1 : -1;
-            int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
-            real sum[8] = {0, 0, 0, 0, 0, 0, 0, 0}, err[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            for (int j = 0; j < 2; j++)
-              for (int i = 0; i < 2; i++) {
-                int a[3] = {a0[0], a0[1], a0[2]};
-                a[t1] += i;
-                a[t2] += j;
-                Flux f = amr_flux_between(P, W, PA, PB, ch, a, axis);
-                real fv[8];
-                flux_pack(&f, fv);
-                for (int q = 0; q < 8; q++) {
-                  if (comp) { real e; two_sum(sum[q], fv[q], &sum[q], &e); err[q] = err[q] + e; }
-                  else sum[q] = sum[q] + fv[q];
-                }
-              }
             real* dst = hi ? fh[axis] : fl[axis];
-            for (int q = 0; q < 8; q++) dst[q] = (sum[q] + err[q]) * RL(0.25);
+            if (!leaf_(t, np)) {
+              int h[3] = {l[0] >= BS_ / 2, l[1] >= BS_ / 2, l[2] >= BS_ / 2};
+              int bits[3] = {h[0], h[1], h[2]};
+              bits[axis] = upper;
+              int ch = bt_(t, np, 12 + (bits[0] | (bits[1] << 1) | (bits[2] << 2)));
+              real part = RL(1.0);
+              if (bins_on_(B)) part = bin_(B, ch) < gbin ? RL(0.5) : (deposit ? RL(0.0) : RL(1.0));
+              if (part == RL(0.0)) {
+                for (int q = 0; q < 8; q++) dst[q] = RL(0.0);
+                continue;
+              }
+              int a0[3] = {2 * (l[0] - (BS_ / 2) * h[0]), 2 * (l[1] - (BS_ / 2) * h[1]), 2 * (l[2] - (BS_ / 2) * h[2])};
+              a0[axis] = upper ? BS_ - 1 : -1;
+              int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
+              real sum[8] = {0, 0, 0, 0, 0, 0, 0, 0}, err[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+              for (int j = 0; j < 2; j++)
+                for (int i = 0; i < 2; i++) {
+                  int a[3] = {a0[0], a0[1], a0[2]};
+                  a[t1] += i;
+                  a[t2] += j;
+                  Flux f = amr_flux_between(P, W, PA, PB, ch, a, axis);
+                  real fv[8];
+                  flux_pack(&f, fv);
+                  for (int q = 0; q < 8; q++) {
+                    if (comp) { real e; two_sum(sum[q], fv[q], &sum[q], &e); err[q] = err[q] + e; }
+                    else sum[q] = sum[q] + fv[q];
+                  }
+                }
+              for (int q = 0; q < 8; q++) dst[q] = (sum[q] + err[q]) * RL(0.25) * part;
+              touched = 1;
+            } else if (bins_on_(B) && np != s && bin_(B, np) < gbin) {
+              int a[3] = {l[0], l[1], l[2]};
+              a[axis] = upper ? BS_ - 1 : -1;
+              Flux f = amr_flux_between(P, W, PA, PB, np, a, axis);
+              real fv[8];
+              flux_pack(&f, fv);
+              for (int q = 0; q < 8; q++) dst[q] = fv[q] * RL(0.5);
+              touched = 1;
+            }
+          }
+          size_t o = iidx(s, l) * 4;
+          if (deposit && !touched) {
+            for (int q = 0; q < 4; q++) { o1[o + q] = u1[o + q]; o2[o + q] = u2[o + q]; }
+            continue;
           }
           real net[8];
           for (int q = 0; q < 8; q++) net[q] = ((fl[0][q] - fh[0][q]) + (fl[1][q] - fh[1][q])) + (fl[2][q] - fh[2][q]);
-          size_t o = iidx(s, l) * 4;
           P_t Pl = *P;
           Pl.dx = dx;
-          advance(&Pl, u1 + o, u2 + o, net, PB[pidx(s, l) * 4], dt, o1 + o, o2 + o);
+          advance(&Pl, u1 + o, u2 + o, net, PB[pidx(s, l) * 4], dtb, o1 + o, o2 + o);
         }
   }
 }
@@ -1053,6 +1175,162 @@ void FN(amr_step)(const orc_params* prm, const orc_amr* t, double dx0, int compe
 real FN(amr_assemble)(const orc_params* prm, const orc_amr* t, double dx0, const real* u1, const real* u2, real* W) {
   P_t P = load_params(prm);
   return amr_assemble(t, &P, dx0, u1, u2, W);
+}
+
+/* ---------------- per-block timesteps (src/amr/bins_*.wgsl, amr_sim.ts encodeCycle :1509-1660) ---------------- */
+
+/* bins_tick.wgsl MODE 0: bank the substep just taken (Kahan-summed t) */
+static void clock_bank(real clk[4]) {
+  real prev = clk[0];
+  if (prev > RL(0.0)) {
+    real y = prev - clk[2];
+    real t = clk[1] + y;
+    clk[2] = (t - clk[1]) - y;
+    clk[1] = t;
+    clk[3] = clk[3] + RL(1.0);
+  }
+}
+/* hydro/clock.wgsl with PER_DX, BANK off and CYCLE substeps: the cycle's dt, the t_end remainder
+   shared among its substeps (:119-131) */
+static void clock_set_cycle(const P_t* P, real clk[4], real max_rate, real cycle) {
+  real dt = P->cfl * RL(1.0) / rmax(max_rate, RL(1.0e-30));
+  dt = rmin(dt, P->dt_max);
+  real remaining = P->t_end - clk[1];
+  if (remaining <= RL(1.0e-6) * rabs(P->t_end)) dt = RL(0.0);
+  else dt = rmin(dt, remaining / cycle);
+  if (RL(1e30) - clk[1] < RL(0.5) * cycle * dt) dt = RL(0.0);
+  clk[0] = dt;
+}
+
+/* The bins of a cycle: bins_init.wgsl (each leaf's largest power-of-two multiple of dt its own
+   rate allows, at most kmax; parents and a held clock kmax), one sweep of bins_relax.wgsl with
+   DILATE (every block down to its fastest face neighbour's bin), then (kmax + levels + 2) | 1
+   sweeps without: a leaf at most one bin slower than a face neighbour (a refined neighbour
+   stands for its fastest child) and never slower than a coarser neighbour; a parent its fastest
+   child's bin. Sweeps are Jacobi (each reads the previous sweep's bins). */
+void FN(amr_bins)(const orc_params* prm, const orc_amr* t, int kmax, const real* brate, real dt, int* out) {
+  P_t P = load_params(prm);
+  int n = t->nblocks, levels = 0;
+  for (int s = 0; s < n; s++) if (bt_(t, s, 0) + 1 > levels) levels = bt_(t, s, 0) + 1;
+  int* A = (int*)malloc(sizeof(int) * (size_t)n);
+  int* Bk = (int*)malloc(sizeof(int) * (size_t)n);
+  for (int s = 0; s < n; s++) {
+    int k = kmax;
+    if (leaf_(t, s) && dt > RL(0.0)) {
+      real dtb = P.cfl / rmax(brate[s], RL(1.0e-30));
+      real f = RFLOOR(RLOG2(dtb / dt));
+      if (f < RL(0.0)) f = RL(0.0);
+      if (f > (real)kmax) f = (real)kmax;
+      k = (int)f;
+    }
+    A[s] = k;
+  }
+  int sweeps = (kmax + levels + 2) | 1;
+  for (int it = 0; it <= sweeps; it++) {
+    const int* kin = (it % 2 == 0) ? A : Bk;
+    int* kout = (it % 2 == 0) ? Bk : A;
+    int dilate = it == 0;
+    for (int s = 0; s < n; s++) {
+      int k = kin[s];
+      if (!leaf_(t, s)) {
+        k = kmax;
+        for (int ch = 0; ch < 8; ch++) {
+          int c = bt_(t, s, 12 + ch);
+          if (c >= 0 && kin[c] < k) k = kin[c];
+        }
+      } else {
+        for (int f = 0; f < 6; f++) {
+          int nb = bt_(t, s, 6 + f);
+          if (nb >= 0 && nb != s) {
+            int v = kin[nb] + (dilate ? 0 : 1);
+            if (v < k) k = v;
+          } else if (nb == -1) {
+            int v = kin[bt_(t, s, 20 + f)];
+            if (v < k) k = v;
+          }
+        }
+      }
+      kout[s] = k;
+    }
+  }
+  /* 1 + sweeps passes in all, an even count: the last wrote A */
+  memcpy(out, A, sizeof(int) * (size_t)n);
+  free(A);
+  free(Bk);
+}
+
+/* faster_neighbor (bins_lists.wgsl:22-52): a face neighbour leaf, or a child of a refined one
+   touching the shared face, in a faster bin */
+static int faster_neighbor(const orc_amr* t, const int* k, int s) {
+  for (int f = 0; f < 6; f++) {
+    int nb = bt_(t, s, 6 + f);
+    if (nb < 0 || nb == s) continue;
+    if (leaf_(t, nb)) {
+      if (k[nb] < k[s]) return 1;
+      continue;
+    }
+    int axis = f >> 1, want = 1 - (f & 1);
+    for (int ch = 0; ch < 8; ch++) {
+      if (((ch >> axis) & 1) != want) continue;
+      int c = bt_(t, nb, 12 + ch);
+      if (c >= 0 && leaf_(t, c) && k[c] < k[s]) return 1;
+    }
+  }
+  return 0;
+}
+
+/* One cycle of 2^kmax substeps (encodeCycle, amr_sim.ts:1509-1660, hydro passes): substep 0 banks
+   the last step, assembles every block, sets the cycle's dt from the leaves' rates and the bins
+   (or takes `given` bins when non-NULL), then each substep s: (s > 0: bank, assemble the blocks
+   starting a step), predict the leaves starting a step, update those and the leaves owed a
+   deposit, and copy the updated blocks back (the commit). bins_out (may be NULL) takes the bins;
+   work[0] the block updates made (deposits included), work[1] what single steps would have made
+   (leaves << kmax). */
+void FN(amr_cycle)(const orc_params* prm, const orc_amr* t, double dx0, int compensated, int kmax, const int* given,
+                   real* u1, real* u2, real clk[4], int* bins_out, long long work[2]) {
+  P_t P = load_params(prm);
+  int n = t->nblocks;
+  size_t np = (size_t)n * PD_ * PD_ * PD_;
+  size_t ni = (size_t)n * BS_ * BS_ * BS_;
+  real* W = (real*)calloc(np * 8, sizeof(real));
+  real* PA = (real*)calloc(np * 4, sizeof(real));
+  real* PB = (real*)calloc(np * 4, sizeof(real));
+  real* o1 = (real*)malloc(ni * 4 * sizeof(real));
+  real* o2 = (real*)malloc(ni * 4 * sizeof(real));
+  real* brate = (real*)calloc((size_t)n, sizeof(real));
+  int* k = (int*)malloc(sizeof(int) * (size_t)n);
+  char* lu = (char*)malloc((size_t)n);
+  memcpy(o1, u1, ni * 4 * sizeof(real));
+  memcpy(o2, u2, ni * 4 * sizeof(real));
+  const int cyc = 1 << kmax;
+  long long updates = 0, leaves = 0;
+  for (int s = 0; s < n; s++) leaves += leaf_(t, s);
+  clock_bank(clk);
+  real rate = amr_assemble_b(t, &P, dx0, u1, u2, W, NULL, brate);
+  clock_set_cycle(&P, clk, rate, (real)cyc);
+  if (given) memcpy(k, given, sizeof(int) * (size_t)n);
+  else FN(amr_bins)(prm, t, kmax, brate, clk[0], k);
+  if (bins_out) memcpy(bins_out, k, sizeof(int) * (size_t)n);
+  for (int sub = 0; sub < cyc; sub++) {
+    Bins B = {k, sub, PA, PB};
+    if (sub > 0) {
+      clock_bank(clk);
+      (void)amr_assemble_b(t, &P, dx0, u1, u2, W, &B, NULL);
+    }
+    amr_predict_b(t, &P, dx0, W, clk[0], PA, PB, &B);
+    for (int s = 0; s < n; s++)
+      lu[s] = leaf_(t, s) && (active_(&B, s) || (half_(&B, s) && faster_neighbor(t, k, s)));
+    amr_update_b(t, &P, dx0, compensated, u1, u2, W, PA, PB, clk[0], o1, o2, &B, lu);
+    for (int s = 0; s < n; s++) {
+      if (!lu[s]) continue;
+      updates++;
+      size_t o = (size_t)s * BS_ * BS_ * BS_ * 4;
+      memcpy(u1 + o, o1 + o, (size_t)BS_ * BS_ * BS_ * 4 * sizeof(real));
+      memcpy(u2 + o, o2 + o, (size_t)BS_ * BS_ * BS_ * 4 * sizeof(real));
+    }
+  }
+  if (work) { work[0] = updates; work[1] = leaves << kmax; }
+  free(W); free(PA); free(PB); free(o1); free(o2); free(brate); free(k); free(lu);
 }
 
 /* Re-meshing (src/amr/regrid.wgsl:40-139): every leaf of a new tree from the old tree's state.

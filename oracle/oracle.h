@@ -124,7 +124,14 @@ const int* orc_amr_table(const orc_amr* t);
      slot src[b], 2 prolongation of old leaf src[b] into its child oct[b], 3 restriction of old \
      parent src[b]; o1/o2 hold nnew blocks */                                                   \
   void P##amr_regrid(const orc_params* prm, const orc_amr* told, const T* u1, const T* u2, int nnew, \
-                     const int* kind, const int* src, const int* oct, T* o1, T* o2);
+                     const int* kind, const int* src, const int* oct, T* o1, T* o2);                    \
+  /* per-block timesteps (src/amr/bins_*.wgsl): the bins of a cycle from each block's largest rate \
+     / dx (brate) and the cycle's dt, at most kmax */                                              \
+  void P##amr_bins(const orc_params* prm, const orc_amr* t, int kmax, const T* brate, T dt, int* out); \
+  /* one cycle of 2^kmax substeps (src/amr/amr_sim.ts:1509-1660); given (may be NULL) fixes the   \
+     bins; bins_out (may be NULL) takes them; work = (block updates, leaves << kmax) */          \
+  void P##amr_cycle(const orc_params* prm, const orc_amr* t, double dx0, int compensated, int kmax,  \
+                    const int* given, T* u1, T* u2, T clk[4], int* bins_out, long long work[2]);
 
 ORC_DECLARE(float, orc_f32_)
 ORC_DECLARE(double, orc_f64_)
