@@ -199,6 +199,10 @@ typedef struct {
   int path;            /* SHX_AMR_PATH_* */
   int compensated;     /* 1: the four fine fluxes of a coarse face are summed with TwoSum before
                           their mean; 0: plain FP32 sum, as match_faces (amr/update.wgsl:207-290) */
+  int time_bins;       /* per-block timesteps (SimConfig.timeBins, amr/bins_on.wgsl): 0 off; K in 1..5:
+                          shx_amr_step advances in cycles of 2^K substeps of the fastest block's dt,
+                          each leaf stepping 2^k of them at a time (k its bin). PASSES and LEVELS
+                          paths only */
 } shx_amr_layout;
 
 /* AmrSim.create (amr_sim.ts:1924-1992 with uploadTree :2566-2625): validate the tree (levels,
@@ -230,6 +234,13 @@ int shx_amr_run_to_end(shx_amr* sim, int batch, int max_steps, int* steps);
  * shx_build_hydro; the tree, boundaries, precision and device of *cfg must equal the sim's. The
  * state, tree and clock are kept. */
 int shx_amr_write_params(shx_amr* sim, const shx_config* cfg);
+/* Per-block timesteps (layout time_bins > 0): out (may be null; len >= nblocks) the bin of every
+ * slot in the last cycle (2^bin substeps a step; a parent its fastest child's); work (may be null)
+ * = (cycles, block updates made, deposits included, what single steps would have made: leaves
+ * x 2^K per cycle), as AmrSim.binWork (amr_sim.ts:1663-1675). Counted since creation or the
+ * last re-mesh. Each cycle reads its rates back once (its bins and lists are planned on the host),
+ * so shx_amr_step waits for the previous cycle's steps before enqueueing the next. */
+int shx_amr_bins(shx_amr* sim, int* out, size_t len, int64_t work[3]);
 int shx_amr_build_hydro(shx_amr* sim, const shx_config* cfg);
 /* AmrSim.diagnostics (amr/diagnostics.wgsl:13-117): the channels of shx_diagnostics over leaf
  * cells, each weighted by its own volume h^3 and placed at its centre (blk_pos,

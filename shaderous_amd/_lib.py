@@ -37,7 +37,7 @@ class ShxConfig(C.Structure):
 
 
 class ShxAmrLayout(C.Structure):
-    _fields_ = [("root", C.c_int * 3), ("path", C.c_int), ("compensated", C.c_int)]
+    _fields_ = [("root", C.c_int * 3), ("path", C.c_int), ("compensated", C.c_int), ("time_bins", C.c_int)]
 
 
 _lib = None
@@ -94,6 +94,7 @@ def load():
         "shx_amr_sync": (I, [P]),
         "shx_amr_read_clock": (I, [P, D]),
         "shx_amr_run_to_end": (I, [P, I, I, C.POINTER(I)]),
+        "shx_amr_bins": (I, [P, C.POINTER(I), C.c_size_t, C.POINTER(C.c_int64)]),
         "shx_amr_write_params": (I, [P, C.POINTER(ShxConfig)]),
         "shx_amr_build_hydro": (I, [P, C.POINTER(ShxConfig)]),
         "shx_amr_diagnostics": (I, [P, D]),

@@ -27,7 +27,8 @@ class AmrSim:
     """A block-AMR hydro run on one GPU. `cfg` describes the level-0 grid (n = 16 x root blocks,
     dx = level-0 cell size); `blocks` is (nblocks, 4) of (level, ix, iy, iz)."""
 
-    def __init__(self, cfg: SimConfig, blocks, root, path: str = "fused", compensated: bool = True):
+    def __init__(self, cfg: SimConfig, blocks, root, path: str = "fused", compensated: bool = True,
+                 time_bins: int = 0):
         lib = _lib.load()
         root = tuple(int(r) for r in root)
         self.cfg = replace(cfg, n=tuple(BS * r for r in root), ndim=3)
@@ -40,6 +41,8 @@ class AmrSim:
         lay.root[:] = list(root)
         lay.path = _lib.AMR_PATHS[path]
         lay.compensated = int(bool(compensated))
+        lay.time_bins = int(time_bins)
+        self.time_bins = int(time_bins)
         self._h = C.c_void_p()
         c = self.cfg.to_c()
         check(lib.shx_amr_create(C.byref(c), C.byref(lay), self.blocks.ctypes.data_as(C.POINTER(C.c_int)),
@@ -168,6 +171,14 @@ class AmrSim:
             if ck[0] == 0:
                 break
         return taken
+
+    def bins(self):
+        """Per-block timesteps: (the bin of every slot in the last cycle, {cycles, updates, single}):
+        block updates made (deposits included) and what single steps would have made."""
+        out = np.zeros(self.nblocks, np.int32)
+        w = (C.c_int64 * 3)()
+        check(_lib.load().shx_amr_bins(self._h, out.ctypes.data_as(C.POINTER(C.c_int)), self.nblocks, w))
+        return out, {"cycles": int(w[0]), "updates": int(w[1]), "single": int(w[2])}
 
     def run_to_end(self, batch: int = 16, max_steps: int = 1 << 30) -> int:
         """AmrSim.runToEnd: step until the clock holds at cfg.t_end; the clock's step count then."""

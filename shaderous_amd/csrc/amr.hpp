@@ -21,6 +21,7 @@ constexpr int AMR_NG = 2;        // ghost layers
 constexpr int AMR_PAD = 20;      // texels per block side
 constexpr int AMR_BT = 32;       // ints per block-table row
 constexpr int AMR_MAXLEV = 8;
+constexpr int AMR_MAX_BINS = 5;  // cycles of at most 32 substeps (MAX_TIME_BINS, amr_sim.ts:115)
 constexpr int AMR_CELLS = AMR_BS * AMR_BS * AMR_BS;
 constexpr int AMR_TEXELS = AMR_PAD * AMR_PAD * AMR_PAD;
 
@@ -33,12 +34,24 @@ struct Levels {
   T dx[AMR_MAXLEV];
 };
 
+// Per-block timesteps (src/amr/bins_on.wgsl): the cycle's bin of every slot and the substep
+// being taken; a leaf whose step has not started on this substep is part way through a longer
+// one, seen through its half-step prediction, which the persistent PA/PB atlases hold from the
+// start of its step (amr/mid_on.wgsl). k null: every block steps every step (bins_off.wgsl).
+struct AmrBins {
+  const int* k = nullptr;
+  int sub = 0;
+  const void* pa = nullptr;
+  const void* pb = nullptr;
+};
+
 struct AmrLaunch {
   hipStream_t stream;
   const void* params;   // Params<T>: n = level-0 cells, dx = level-0 cell size
   const void* levels;   // Levels<T>
   const int* tab;       // device block table
   double dx0;
+  AmrBins bins;         // read by assemble, predict and update
 };
 
 // The fused march's queue and what its workgroups read besides the state: the queue of leaves
@@ -93,6 +106,14 @@ struct AmrOps {
                  void* n1b, void* n2b);
   // the refinement indicator of the listed leaves into out[slot] (precision of the state)
   void (*indicator)(const AmrLaunch&, const int* list, int count, const void* u1, void* out);
+  // per-block timesteps: each listed leaf's largest rate / dx into out[slot] (precision of the
+  // state); the clock's bank of the substep just taken (bins_tick.wgsl MODE 0); the cycle's dt
+  // from the reduced rate in slot (hydro/clock.wgsl with BANK off and CYCLE substeps); the
+  // updated blocks copied back into the state (the commit)
+  void (*block_rates)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2, void* out);
+  void (*bank)(const AmrLaunch&, void* clk);
+  void (*clock_cycle)(const AmrLaunch&, void* clk, unsigned long long* slot, int cycle);
+  void (*commit)(const AmrLaunch&, const int* list, int count, const void* o1, const void* o2, void* u1, void* u2);
 };
 
 AmrOps amr_ops_f32(int lim, int rs, int isen);

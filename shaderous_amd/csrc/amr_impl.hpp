@@ -136,6 +136,31 @@ __device__ __forceinline__ bool u_near(const int* __restrict__ tab, const T4<T>*
   return true;
 }
 
+// Per-block timesteps: whether block s starts a step on the substep (blk_active,
+// amr/bins_on.wgsl:14-17), and its own step (blk_dt, amr/blk_dt.wgsl)
+__device__ __forceinline__ bool bins_active(const AmrBins& B, int s) {
+  return !B.k || (B.sub & ((1 << B.k[s]) - 1)) == 0;
+}
+__device__ __forceinline__ int bins_bin(const AmrBins& B, int s) { return B.k ? B.k[s] : 0; }
+// mid_block (amr/mid_on.wgsl:10-12): a leaf part way through a longer step
+__device__ __forceinline__ bool mid_block(const AmrBins& B, const int* __restrict__ tab, int s) {
+  return B.k && is_leaf(tab, s) && !bins_active(B, s);
+}
+// mid_cons (amr/mid_on.wgsl:14-30): the conserved state of texel (x, y, z) of block `slot` (a cell
+// or a face ghost of the first layer) at the middle of its step, from its half-step prediction
+template <typename T>
+__device__ __forceinline__ Cons<T> mid_cons(const Params<T>& P, const AmrBins& B, int slot, int x, int y, int z) {
+  const size_t o = (size_t)slot * AMR_TEXELS + tex(x, y, z);
+  const T4<T> a = static_cast<const T4<T>*>(B.pa)[o];
+  const T4<T> b = static_cast<const T4<T>*>(B.pb)[o];
+  const T rho = a.x;
+  const T eint = b.x * P.igm1;
+  Cons<T> c;
+  c.a = v4<T>(rho, rho * a.y, rho * a.z, rho * a.w);
+  c.b = v4<T>(eint + T(0.5) * rho * ((a.y * a.y + a.z * a.z) + a.w * a.w), eint, rho * b.y, T(0));
+  return c;
+}
+
 // The limiter of the prolongation slopes (amr/cons.wgsl:65-74: MC for mc and koren)
 template <int LIM>
 constexpr int prolong_lim() { return (LIM == LIM_MINMOD || LIM == LIM_VANLEER) ? LIM : LIM_MC; }
@@ -203,7 +228,7 @@ __device__ __forceinline__ Cons<T> prolong(const Params<T>& P, const Cons<T> v[7
 template <typename T, int LIM>
 __device__ __forceinline__ Cons<T> from_coarse(const Params<T>& P, const int* __restrict__ tab,
                                                const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, int slot,
-                                               int x, int y, int z, int cs) {
+                                               int x, int y, int z, int cs, const AmrBins& B = AmrBins{}) {
   const int lc = bt(tab, cs, BT_LEVEL);
   int c[3], par[3];
   const int l[3] = {x, y, z};
@@ -218,6 +243,20 @@ __device__ __forceinline__ Cons<T> from_coarse(const Params<T>& P, const int* __
   }
   Cons<T> v[7];
   bool ok[7];
+  if (mid_block(B, tab, cs)) {
+    // a coarser leaf half way through its step: its prediction, cells and first ghost layer
+    // (amr/assemble.wgsl:60-68)
+    v[0] = mid_cons<T>(P, B, cs, c[0], c[1], c[2]);
+    v[1] = mid_cons<T>(P, B, cs, c[0] - 1, c[1], c[2]);
+    v[2] = mid_cons<T>(P, B, cs, c[0] + 1, c[1], c[2]);
+    v[3] = mid_cons<T>(P, B, cs, c[0], c[1] - 1, c[2]);
+    v[4] = mid_cons<T>(P, B, cs, c[0], c[1] + 1, c[2]);
+    v[5] = mid_cons<T>(P, B, cs, c[0], c[1], c[2] - 1);
+    v[6] = mid_cons<T>(P, B, cs, c[0], c[1], c[2] + 1);
+#pragma unroll
+    for (int k = 0; k < 7; k++) ok[k] = true;
+    return prolong<T, LIM>(P, v, ok, par[0], par[1], par[2]);
+  }
   ok[0] = u_near<T>(tab, U1, U2, cs, c[0], c[1], c[2], v[0]);
   ok[1] = u_near<T>(tab, U1, U2, cs, c[0] - 1, c[1], c[2], v[1]);
   ok[2] = u_near<T>(tab, U1, U2, cs, c[0] + 1, c[1], c[2], v[2]);
@@ -232,11 +271,12 @@ __device__ __forceinline__ Cons<T> from_coarse(const Params<T>& P, const int* __
 template <typename T, int LIM>
 __device__ __forceinline__ bool amr_texel(const Params<T>& P, const int* __restrict__ tab,
                                           const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, int slot, int x,
-                                          int y, int z, Prim<T>& w) {
+                                          int y, int z, Prim<T>& w, const AmrBins& B = AmrBins{}) {
   const GSrc g = ghost_src(tab, slot, x, y, z);
   if (g.kind == 2) return false;
-  const Cons<T> u = g.kind == 0 ? u_at<T>(tab, U1, U2, g.slot, g.x, g.y, g.z)
-                                : from_coarse<T, LIM>(P, tab, U1, U2, slot, x, y, z, g.slot);
+  const Cons<T> u = g.kind == 0 ? (mid_block(B, tab, g.slot) ? mid_cons<T>(P, B, g.slot, g.x, g.y, g.z)
+                                                             : u_at<T>(tab, U1, U2, g.slot, g.x, g.y, g.z))
+                                : from_coarse<T, LIM>(P, tab, U1, U2, slot, x, y, z, g.slot, B);
   w = prim_from_cons(P, u.a, u.b);
   w.vx = bc_vel(w.vx, g.fx, g.sx);
   w.vy = bc_vel(w.vy, g.fy, g.sy);
@@ -251,7 +291,7 @@ template <typename T, int LIM>
 __global__ __launch_bounds__(256) void k_amr_assemble(Params<T> P, Levels<T> LV, const int* __restrict__ tab,
                                                       const int* __restrict__ list, const T4<T>* __restrict__ U1,
                                                       const T4<T>* __restrict__ U2, T4<T>* __restrict__ W1,
-                                                      T4<T>* __restrict__ W2, unsigned long long* slot) {
+                                                      T4<T>* __restrict__ W2, unsigned long long* slot, AmrBins B) {
   const int b = blockIdx.x / ASM_CHUNKS;
   const int t = (blockIdx.x % ASM_CHUNKS) * 256 + threadIdx.x;
   T r = T(0);
@@ -259,7 +299,7 @@ __global__ __launch_bounds__(256) void k_amr_assemble(Params<T> P, Levels<T> LV,
     const int x = t % AMR_PAD - AMR_NG, y = (t / AMR_PAD) % AMR_PAD - AMR_NG, z = t / (AMR_PAD * AMR_PAD) - AMR_NG;
     const int s = list[b];
     Prim<T> w;
-    if (amr_texel<T, LIM>(P, tab, U1, U2, s, x, y, z, w)) {
+    if (amr_texel<T, LIM>(P, tab, U1, U2, s, x, y, z, w, B)) {
       const size_t o = (size_t)s * AMR_TEXELS + t;
       W1[o] = v4<T>(w.rho, w.vx, w.vy, w.vz);
       W2[o] = v4<T>(w.p, w.cs, w.s, T(0));
@@ -283,6 +323,67 @@ __global__ __launch_bounds__(256) void k_amr_rates(Params<T> P, Levels<T> LV, co
     r = tmax(r, rate_of(P, prim_from_cons(P, U1[o], U2[o])) / dx);
   }
   block_max_to(r, slot);
+}
+
+// Per-block timesteps: each listed leaf's largest rate / dx into out[slot] (the records
+// amr/bins_init.wgsl reads), by the arithmetic of the assembly's rates
+template <typename T>
+__global__ __launch_bounds__(256) void k_amr_block_rates(Params<T> P, const int* __restrict__ tab,
+                                                         const int* __restrict__ list, const T4<T>* __restrict__ U1,
+                                                         const T4<T>* __restrict__ U2, T* __restrict__ out) {
+  __shared__ T red[4];
+  const int s = list[blockIdx.x];
+  const T dx = level_dx(P, bt(tab, s, BT_LEVEL));
+  T r = T(0);
+  for (int c = threadIdx.x; c < AMR_CELLS; c += 256) {
+    const size_t o = (size_t)s * AMR_CELLS + c;
+    r = tmax(r, rate_of(P, prim_from_cons(P, U1[o], U2[o])) / dx);
+  }
+  r = wave_max(r);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) out[s] = tmax(tmax(red[0], red[1]), tmax(red[2], red[3]));
+}
+
+// bins_tick.wgsl MODE 0: bank the substep just taken (Kahan-summed t)
+template <typename T>
+__global__ void k_amr_bank(T* clk) {
+  const T prev = clk[0];
+  if (prev > T(0)) {
+    const T y = prev - clk[2];
+    const T t = clk[1] + y;
+    const T d = t - clk[1];
+    clk[2] = d - y;
+    clk[1] = t;
+    clk[3] = clk[3] + T(1);
+  }
+}
+
+// hydro/clock.wgsl with PER_DX (dx 1), BANK off and CYCLE substeps (:119-131): the cycle's dt;
+// clears the reduction slot
+template <typename T>
+__global__ void k_amr_clock_cycle(Params<T> P, T* clk, unsigned long long* slot, T cycle) {
+  const T rate = from_bits<T>(*slot);
+  T dt = P.cfl * T(1) / tmax(rate, T(1e-30));
+  dt = tmin(dt, P.dt_max);
+  const T remaining = P.t_end - clk[1];
+  if (remaining <= T(1e-6) * tabs(P.t_end)) dt = T(0);
+  else dt = tmin(dt, remaining / cycle);
+  if (T(1e30) - clk[1] < T(0.5) * cycle * dt) dt = T(0);
+  if (clk[10] != T(0)) dt = T(0);
+  clk[0] = dt;
+  *slot = 0ull;
+}
+
+// the commit: the listed blocks' updated cells back into the state
+template <typename T>
+__global__ __launch_bounds__(256) void k_amr_commit(const int* __restrict__ list, const T4<T>* __restrict__ O1,
+                                                    const T4<T>* __restrict__ O2, T4<T>* __restrict__ U1,
+                                                    T4<T>* __restrict__ U2) {
+  const int s = list[blockIdx.x / (AMR_CELLS / 256)];
+  const size_t o = (size_t)s * AMR_CELLS + (blockIdx.x % (AMR_CELLS / 256)) * 256 + threadIdx.x;
+  U1[o] = O1[o];
+  U2[o] = O2[o];
 }
 
 // ---------------- predict ----------------
@@ -331,7 +432,7 @@ template <typename T, int LIM, bool ISEN>
 __global__ __launch_bounds__(256) void k_amr_predict(Params<T> P, Levels<T> LV, const int* __restrict__ tab,
                                                      const int* __restrict__ list, const T4<T>* __restrict__ W1,
                                                      const T4<T>* __restrict__ W2, T4<T>* __restrict__ PA,
-                                                     T4<T>* __restrict__ PB, const T* __restrict__ clk) {
+                                                     T4<T>* __restrict__ PB, const T* __restrict__ clk, AmrBins B) {
   const int b = blockIdx.x / PRED_CHUNKS;
   const int t = (blockIdx.x % PRED_CHUNKS) * 256 + threadIdx.x;
   if (t >= PRED_TEXELS) return;
@@ -339,15 +440,19 @@ __global__ __launch_bounds__(256) void k_amr_predict(Params<T> P, Levels<T> LV, 
   const int s = list[b];
   const GSrc g = ghost_src(tab, s, x, y, z);
   if (g.kind == 2) return;
-  const bool at = g.kind == 0;
+  // with bins, a ghost from a leaf part way through a longer step was filled from its prediction
+  // and is predicted in place (amr/predict.wgsl:193-197); the leaf's own step is 2^bin dt
+  const bool stale = B.k && g.slot != s && mid_block(B, tab, g.slot);
+  const bool at = g.kind == 0 && !stale;
+  const T dt = B.k ? clk[0] * T(1 << bins_bin(B, s)) : clk[0];
   const int src = at ? g.slot : s;
   Params<T> Pl = P;
   Pl.dx = level_dx(P, bt(tab, src, BT_LEVEL));
   const T mx = (!at && (x == 0 || x == AMR_BS - 1)) ? T(0) : T(1);
   const T my = (!at && (y == 0 || y == AMR_BS - 1)) ? T(0) : T(1);
   const T mz = (!at && (z == 0 || z == AMR_BS - 1)) ? T(0) : T(1);
-  Half<T> h = at ? predict_texel<T, LIM, ISEN>(Pl, W1, W2, src, g.x, g.y, g.z, T(1), T(1), T(1), clk[0])
-                 : predict_texel<T, LIM, ISEN>(Pl, W1, W2, src, x, y, z, mx, my, mz, clk[0]);
+  Half<T> h = at ? predict_texel<T, LIM, ISEN>(Pl, W1, W2, src, g.x, g.y, g.z, T(1), T(1), T(1), dt)
+                 : predict_texel<T, LIM, ISEN>(Pl, W1, W2, src, x, y, z, mx, my, mz, dt);
   if (at) {
     h.vx = bc_vel(h.vx, g.fx, g.sx);
     h.vy = bc_vel(h.vy, g.fy, g.sy);
@@ -452,7 +557,7 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
     Params<T> P, Levels<T> LV, const int* __restrict__ tab, const int* __restrict__ list,
     const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, const T4<T>* __restrict__ W1,
     const T4<T>* __restrict__ W2, const T4<T>* __restrict__ PA, const T4<T>* __restrict__ PB, T4<T>* __restrict__ O1,
-    T4<T>* __restrict__ O2, const T* __restrict__ clk) {
+    T4<T>* __restrict__ O2, const T* __restrict__ clk, AmrBins B) {
   constexpr int WX = UPD_WX, WY = UPD_WY, WZ = UPD_WZ;
   constexpr int NFX = (WX + 1) * WY * WZ, NFY = WX * (WY + 1) * WZ, NFZ = WX * WY * (WZ + 1);
   constexpr int NF = NFX + NFY + NFZ, NT = WX * WY * WZ;
@@ -461,6 +566,12 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
   const int tile = blockIdx.x % UPD_TILES;
   const int x0 = (tile & 1) * WX, y0 = ((tile >> 1) & 1) * WY, z0 = (tile >> 2) * WZ;
   const int li = threadIdx.x;
+  // per-block timesteps (amr/update.wgsl:173-186, 338-340): the block's bin and its own step; a
+  // block half way through its step only deposits the second half of its faster neighbours'
+  // fluxes (its own fluxes are zero)
+  const int gbin = bins_bin(B, s);
+  const bool deposit = B.k && !bins_active(B, s);
+  if (!deposit)
   for (int f = li; f < NF; f += NT) {
     int axis, fx, fy, fz;
     if (f < NFX) {
@@ -479,12 +590,17 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
   __syncthreads();
   const int tx = li % WX, ty = (li / WX) % WY, tz = li / (WX * WY);
   const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
-  Flux<T> xl = get_flux(tf, NF, tx + (WX + 1) * (ty + WY * tz));
-  Flux<T> xh = get_flux(tf, NF, tx + 1 + (WX + 1) * (ty + WY * tz));
-  Flux<T> yl = get_flux(tf, NF, NFX + tx + WX * (ty + (WY + 1) * tz));
-  Flux<T> yh = get_flux(tf, NF, NFX + tx + WX * (ty + 1 + (WY + 1) * tz));
-  Flux<T> zl = get_flux(tf, NF, NFX + NFY + tx + WX * (ty + WY * tz));
-  Flux<T> zh = get_flux(tf, NF, NFX + NFY + tx + WX * (ty + WY * (tz + 1)));
+  Flux<T> xl = zero_flux<T>(), xh = zero_flux<T>(), yl = zero_flux<T>(), yh = zero_flux<T>(), zl = zero_flux<T>(),
+          zh = zero_flux<T>();
+  if (!deposit) {
+    xl = get_flux(tf, NF, tx + (WX + 1) * (ty + WY * tz));
+    xh = get_flux(tf, NF, tx + 1 + (WX + 1) * (ty + WY * tz));
+    yl = get_flux(tf, NF, NFX + tx + WX * (ty + (WY + 1) * tz));
+    yh = get_flux(tf, NF, NFX + tx + WX * (ty + 1 + (WY + 1) * tz));
+    zl = get_flux(tf, NF, NFX + NFY + tx + WX * (ty + WY * tz));
+    zh = get_flux(tf, NF, NFX + NFY + tx + WX * (ty + WY * (tz + 1)));
+  }
+  bool touched = false;
   if constexpr (MATCH) {
 #pragma unroll
     for (int k = 0; k < 6; k++) {
@@ -492,8 +608,40 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
       const int l = axis == 0 ? x : (axis == 1 ? y : z);
       if (l != (hi ? AMR_BS - 1 : 0)) continue;
       const int np = bt(tab, s, BT_NB + k);
-      if (np < 0 || is_leaf(tab, np)) continue;
-      const Flux<T> F = fine_face<T, LIM, RS, ISEN, COMP>(P, tab, W1, W2, PA, PB, np, k, x, y, z);
+      if (np < 0) continue;
+      Flux<T> F;
+      if (!is_leaf(tab, np)) {
+        // the refined neighbour's child on this face; with bins its share of this block's step
+        // (share, amr/update.wgsl:182-186): one half if faster, none in a deposit if in step
+        T part = T(1);
+        if (B.k) {
+          const int hx = x >= AMR_BS / 2, hy = y >= AMR_BS / 2, hz = z >= AMR_BS / 2;
+          const int upper = !hi;
+          const int bx = axis == 0 ? upper : hx, by = axis == 1 ? upper : hy, bz = axis == 2 ? upper : hz;
+          const int ch = bt(tab, np, BT_CHILD + (bx | (by << 1) | (bz << 2)));
+          part = bins_bin(B, ch) < gbin ? T(0.5) : (deposit ? T(0) : T(1));
+        }
+        if (part == T(0)) {
+          F = zero_flux<T>();
+        } else {
+          F = fine_face<T, LIM, RS, ISEN, COMP>(P, tab, W1, W2, PA, PB, np, k, x, y, z);
+          if (B.k) {
+            F.m = F.m * part; F.mx = F.mx * part; F.my = F.my * part; F.mz = F.mz * part;
+            F.e = F.e * part; F.ei = F.ei * part; F.s = F.s * part; F.vn = F.vn * part;
+          }
+          touched = true;
+        }
+      } else if (B.k && np != s && bins_bin(B, np) < gbin) {
+        // a faster leaf of the same level: half of its flux through the face, as it computes it
+        const int an = hi ? -1 : AMR_BS - 1;
+        F = amr_flux<T, LIM, RS, ISEN>(P, W1, W2, PA, PB, np, axis == 0 ? an : x, axis == 1 ? an : y,
+                                       axis == 2 ? an : z, axis);
+        F.m = F.m * T(0.5); F.mx = F.mx * T(0.5); F.my = F.my * T(0.5); F.mz = F.mz * T(0.5);
+        F.e = F.e * T(0.5); F.ei = F.ei * T(0.5); F.s = F.s * T(0.5); F.vn = F.vn * T(0.5);
+        touched = true;
+      } else {
+        continue;
+      }
       switch (k) {
         case 0: xl = F; break;
         case 1: xh = F; break;
@@ -510,13 +658,20 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
     O2[o] = v4<T>(zl.m, zh.m, T(0), T(0));
     return;
   }
+  const size_t o = (size_t)s * AMR_CELLS + cel(x, y, z);
+  if (deposit && !touched) {
+    // a cell on no face shared with a faster neighbour is copied as it is (amr/update.wgsl)
+    O1[o] = U1[o];
+    O2[o] = U2[o];
+    return;
+  }
   const Flux<T> net = net_of(xl, xh, yl, yh, zl, zh);
   const T dx = level_dx(P, bt(tab, s, BT_LEVEL));
   Params<T> Pl = P;
   Pl.dx = dx;
-  const size_t o = (size_t)s * AMR_CELLS + cel(x, y, z);
+  const T dt = B.k ? clk[0] * T(1 << gbin) : clk[0];
   T4<T> o1, o2;
-  advance(Pl, U1[o], U2[o], net, PB[(size_t)s * AMR_TEXELS + tex(x, y, z)].x, clk[0] / dx, o1, o2);
+  advance(Pl, U1[o], U2[o], net, PB[(size_t)s * AMR_TEXELS + tex(x, y, z)].x, dt / dx, o1, o2);
   O1[o] = o1;
   O2[o] = o2;
 }
@@ -629,7 +784,7 @@ struct AmrImpl {
                        void* w2, unsigned long long* slot) {
     if (count <= 0) return;
     hipLaunchKernelGGL((k_amr_assemble<T, LIM>), dim3(count * ASM_CHUNKS), dim3(256), 0, L.stream, P(L), LV(L), L.tab,
-                       list, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)w1, (T4<T>*)w2, slot);
+                       list, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)w1, (T4<T>*)w2, slot, L.bins);
   }
   static void clock(const AmrLaunch& L, void* clk, unsigned long long* slot) {
     Params<T> P1 = P(L);
@@ -640,7 +795,7 @@ struct AmrImpl {
                       void* pb, const void* clk) {
     if (count <= 0) return;
     hipLaunchKernelGGL((k_amr_predict<T, LIM, ISEN>), dim3(count * PRED_CHUNKS), dim3(256), 0, L.stream, P(L), LV(L),
-                       L.tab, list, (const T4<T>*)w1, (const T4<T>*)w2, (T4<T>*)pa, (T4<T>*)pb, (const T*)clk);
+                       L.tab, list, (const T4<T>*)w1, (const T4<T>*)w2, (T4<T>*)pa, (T4<T>*)pb, (const T*)clk, L.bins);
   }
   static void update(const AmrLaunch& L, const int* list, int count, bool match, bool comp, const void* u1,
                      const void* u2, const void* w1, const void* w2, const void* pa, const void* pb, void* o1, void* o2,
@@ -650,7 +805,7 @@ struct AmrImpl {
   hipLaunchKernelGGL((k_amr_update<T, LIM, RS, ISEN, M, C, D>), dim3(count * UPD_TILES),                              \
                      dim3(UPD_WX * UPD_WY * UPD_WZ), 0, L.stream, P(L), LV(L), L.tab, list, (const T4<T>*)u1,           \
                      (const T4<T>*)u2, (const T4<T>*)w1, (const T4<T>*)w2, (const T4<T>*)pa, (const T4<T>*)pb,         \
-                     (T4<T>*)o1, (T4<T>*)o2, (const T*)clk)
+                     (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, L.bins)
     if (debug) {
       if (!match) SHX_UPD(false, false, true);
       else if (comp) SHX_UPD(true, true, true);
@@ -690,10 +845,31 @@ struct AmrImpl {
     if (count <= 0) return;
     hipLaunchKernelGGL((k_amr_indicator<T>), dim3(count), dim3(256), 0, L.stream, list, (const T4<T>*)u1, (T*)out);
   }
+  static void block_rates(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2, void* out) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL((k_amr_block_rates<T>), dim3(count), dim3(256), 0, L.stream, P(L), L.tab, list,
+                       (const T4<T>*)u1, (const T4<T>*)u2, (T*)out);
+  }
+  static void bank(const AmrLaunch& L, void* clk) {
+    hipLaunchKernelGGL((k_amr_bank<T>), dim3(1), dim3(1), 0, L.stream, (T*)clk);
+  }
+  static void clock_cycle(const AmrLaunch& L, void* clk, unsigned long long* slot, int cycle) {
+    hipLaunchKernelGGL((k_amr_clock_cycle<T>), dim3(1), dim3(1), 0, L.stream, P(L), (T*)clk, slot, T(cycle));
+  }
+  static void commit(const AmrLaunch& L, const int* list, int count, const void* o1, const void* o2, void* u1,
+                     void* u2) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL((k_amr_commit<T>), dim3(count * (AMR_CELLS / 256)), dim3(256), 0, L.stream, list,
+                       (const T4<T>*)o1, (const T4<T>*)o2, (T4<T>*)u1, (T4<T>*)u2);
+  }
   static AmrOps ops() {
     AmrOps o;
     o.regrid = regrid;
     o.indicator = indicator;
+    o.block_rates = block_rates;
+    o.bank = bank;
+    o.clock_cycle = clock_cycle;
+    o.commit = commit;
     o.shell = shell;
     o.fused = fused;
     o.assemble = assemble;

@@ -78,6 +78,16 @@ struct shx_amr {
   AmrOps ops{};
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   int launches = 0;
+  // per-block timesteps: bins (0 off), each slot's bin of the last cycle, each leaf's largest
+  // rate / dx, the cycle's slot lists; the work done
+  int kbins = 0;
+  int* d_bins = nullptr;
+  void* d_brate = nullptr;
+  int* d_blists = nullptr;
+  std::vector<int> bins;
+  std::vector<int> all_host;   // the list of every block, as d_lists holds it
+  std::vector<int> blist_host; // the last cycle's lists
+  long long bin_cycles = 0, bin_updates = 0, bin_single = 0;
 };
 
 namespace {
@@ -428,6 +438,150 @@ int amr_reconfigure(shx_amr* s, const shx_config* cfg, bool scheme) {
   return build_graphs(s);
 }
 
+// ---------------- per-block timesteps (src/amr/bins_*.wgsl, amr_sim.ts encodeCycle :1509-1660) ----------------
+// The bins of a cycle are planned on the host from each leaf's largest rate / dx and the
+// cycle's dt, read back once per cycle (HIP has no indirect dispatch: every substep's lists are
+// then exact launches): bins_init.wgsl, one DILATE sweep and (K + levels + 2) | 1 sweeps of
+// bins_relax.wgsl (Jacobi, each reading the previous one's bins), in the precision of the run.
+template <typename T>
+void plan_bins(const shx_amr* s, const T* brate, T dt, T cfl, std::vector<int>& out) {
+  const int n = s->nblocks, K = s->kbins;
+  auto row = [&](int b) { return &s->tab[(size_t)b * AMR_BT]; };
+  auto leaf = [&](int b) { return (row(b)[BT_FLAGS] & 3) == 3; };
+  std::vector<int> A(n), B(n);
+  for (int b = 0; b < n; b++) {
+    int k = K;
+    if (leaf(b) && dt > T(0)) {
+      const T dtb = cfl / std::max(brate[b], T(1e-30));
+      T f = std::floor(std::log2(dtb / dt));
+      f = std::min(std::max(f, T(0)), T(K));
+      k = (int)f;
+    }
+    A[b] = k;
+  }
+  const int sweeps = (K + s->nlevels + 2) | 1;
+  for (int it = 0; it <= sweeps; it++) {
+    const std::vector<int>& kin = (it % 2 == 0) ? A : B;
+    std::vector<int>& kout = (it % 2 == 0) ? B : A;
+    const int plus = it == 0 ? 0 : 1;
+    for (int b = 0; b < n; b++) {
+      const int* r = row(b);
+      int k = kin[b];
+      if (!leaf(b)) {
+        k = K;
+        for (int c = 0; c < 8; c++)
+          if (r[BT_CHILD + c] >= 0) k = std::min(k, kin[r[BT_CHILD + c]]);
+      } else {
+        for (int f = 0; f < 6; f++) {
+          const int nb = r[BT_NB + f];
+          if (nb >= 0 && nb != b) k = std::min(k, kin[nb] + plus);
+          else if (nb == -1) k = std::min(k, kin[r[BT_COARSE + f]]);
+        }
+      }
+      kout[b] = k;
+    }
+  }
+  out = A;
+}
+
+// faster_neighbor (bins_lists.wgsl:22-52)
+bool faster_neighbor(const shx_amr* s, const std::vector<int>& k, int b) {
+  const int* r = &s->tab[(size_t)b * AMR_BT];
+  for (int f = 0; f < 6; f++) {
+    const int nb = r[BT_NB + f];
+    if (nb < 0 || nb == b) continue;
+    const int* q = &s->tab[(size_t)nb * AMR_BT];
+    if ((q[BT_FLAGS] & 3) == 3) {
+      if (k[nb] < k[b]) return true;
+      continue;
+    }
+    const int axis = f >> 1, want = 1 - (f & 1);
+    for (int c = 0; c < 8; c++) {
+      if (((c >> axis) & 1) != want) continue;
+      const int ch = q[BT_CHILD + c];
+      if (ch >= 0 && (s->tab[(size_t)ch * AMR_BT + BT_FLAGS] & 3) == 3 && k[ch] < k[b]) return true;
+    }
+  }
+  return false;
+}
+
+// One cycle of 2^K substeps. Substep 0: bank the last step, assemble every block (its rates in
+// the reduction slot), each leaf's own rate, the cycle's dt; then (after the one readback of the
+// cycle) the bins and every substep's lists (bins_lists.wgsl): the blocks starting a step (LA,
+// parents included), the leaves among them (LL), those and the leaves half way through a step
+// with a faster face neighbour (LU). Each substep (s > 0: bank, assemble LA) predicts LL,
+// updates LU into the other buffer and copies those blocks back (the commit): the state stays
+// in its buffer through the cycle.
+int run_cycle(shx_amr* s) {
+  const int n = 1 << s->kbins;
+  const int p = s->cur, q = 1 - p;
+  const AmrLaunch L = launch_of(s);
+  const AmrOps& o = s->ops;
+  o.bank(L, s->clk);
+  o.assemble(L, lp(s, s->all), s->all.count, s->U1[p], s->U2[p], s->W1, s->W2, s->slot);
+  o.block_rates(L, lp(s, s->leaf), s->leaf.count, s->U1[p], s->U2[p], s->d_brate);
+  o.clock_cycle(L, s->clk, s->slot, n);
+  HIPCHK(hipGetLastError());
+  std::vector<unsigned char> br((size_t)s->nblocks * s->es);
+  unsigned char ck[16 * 8];
+  HIPCHK(hipMemcpyAsync(br.data(), s->d_brate, br.size(), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(ck, s->clk, 16 * s->es, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  // rates of parents were never written: leaves alone are read
+  if (s->es == 8) plan_bins<double>(s, (const double*)br.data(), ((const double*)ck)[0], s->pd.cfl, s->bins);
+  else plan_bins<float>(s, (const float*)br.data(), ((const float*)ck)[0], s->pf.cfl, s->bins);
+  const std::vector<int>& k = s->bins;
+  auto leaf = [&](int b) { return (s->tab[(size_t)b * AMR_BT + BT_FLAGS] & 3) == 3; };
+  // (kept in the sim: the copy to the device reads it after this call returns)
+  std::vector<int>& lists = s->blist_host;
+  lists.clear();
+  std::vector<Range> la(n), ll(n), lu(n);
+  long long updates = 0;
+  // in the slot order of the plain lists (finest level first, Morton order within)
+  const std::vector<int>& all_order = s->all_host;
+  for (int sub = 0; sub < n; sub++) {
+    auto active = [&](int b) { return (sub & ((1 << k[b]) - 1)) == 0; };
+    auto half = [&](int b) { return k[b] > 0 && (sub & ((1 << k[b]) - 1)) == (1 << (k[b] - 1)); };
+    la[sub].off = (int)lists.size();
+    for (int b : all_order)
+      if (active(b)) lists.push_back(b);
+    la[sub].count = (int)lists.size() - la[sub].off;
+    ll[sub].off = (int)lists.size();
+    for (int b : all_order)
+      if (leaf(b) && active(b)) lists.push_back(b);
+    ll[sub].count = (int)lists.size() - ll[sub].off;
+    lu[sub].off = (int)lists.size();
+    for (int b : all_order)
+      if (leaf(b) && (active(b) || (half(b) && faster_neighbor(s, k, b)))) lists.push_back(b);
+    lu[sub].count = (int)lists.size() - lu[sub].off;
+    updates += lu[sub].count;
+  }
+  HIPCHK(hipMemcpyAsync(s->d_bins, k.data(), k.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->d_blists, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+  const bool comp = s->lay.compensated != 0;
+  for (int sub = 0; sub < n; sub++) {
+    AmrLaunch Lb = L;
+    Lb.bins.k = s->d_bins;
+    Lb.bins.sub = sub;
+    Lb.bins.pa = s->PA;
+    Lb.bins.pb = s->PB;
+    if (sub > 0) {
+      o.bank(L, s->clk);
+      // (the rates of these assemblies are not used: a scratch word of the slot takes them)
+      o.assemble(Lb, s->d_blists + la[sub].off, la[sub].count, s->U1[p], s->U2[p], s->W1, s->W2, s->slot + 1);
+    }
+    o.predict(Lb, s->d_blists + ll[sub].off, ll[sub].count, s->W1, s->W2, s->PA, s->PB, s->clk);
+    o.update(Lb, s->d_blists + lu[sub].off, lu[sub].count, true, comp, s->U1[p], s->U2[p], s->W1, s->W2, s->PA,
+             s->PB, s->U1[q], s->U2[q], s->clk, false);
+    o.commit(L, s->d_blists + lu[sub].off, lu[sub].count, s->U1[q], s->U2[q], s->U1[p], s->U2[p]);
+  }
+  HIPCHK(hipGetLastError());
+  s->bin_cycles++;
+  s->bin_updates += updates;
+  s->bin_single += (long long)s->leaf.count << s->kbins;
+  return SHX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -447,6 +601,10 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   if (!(c.gamma > 1) || !(c.dx > 0) || !(c.cfl > 0)) return fail(SHX_EINVAL, "gamma, dx and cfl must be positive (gamma > 1)");
   if (layout->path != SHX_AMR_PATH_FUSED && layout->path != SHX_AMR_PATH_PASSES && layout->path != SHX_AMR_PATH_LEVELS)
     return fail(SHX_EINVAL, "AMR path %d", layout->path);
+  if (layout->time_bins < 0 || layout->time_bins > AMR_MAX_BINS)
+    return fail(SHX_EINVAL, "time_bins %d (0 to %d)", layout->time_bins, AMR_MAX_BINS);
+  if (layout->time_bins > 0 && layout->path == SHX_AMR_PATH_FUSED)
+    return fail(SHX_EINVAL, "per-block timesteps run on the passes and levels paths");
   AmrOps ops = amr_ops(c.precision, c.limiter, c.riemann, c.isentropic ? 1 : 0);
   if (!ops.assemble) return fail(SHX_EINVAL, "no block kernels built for precision %d limiter %d riemann %d isentropic %d", c.precision, c.limiter, c.riemann, c.isentropic);
   int ndev = 0;
@@ -555,6 +713,16 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   ok = ok && alloc((void**)&s->d_tab, s->tab.size() * sizeof(int)) && alloc((void**)&s->d_lists, lists.size() * sizeof(int));
   ok = ok && alloc(&s->clk, 16 * 8) && alloc((void**)&s->slot, 64) && alloc((void**)&s->diag_part, (size_t)nblocks * DIAG_CH * 8) &&
        alloc((void**)&s->diag_out, DIAG_CH * 8);
+  s->kbins = layout->time_bins;
+  s->all_host.assign(lists.begin() + s->all.off, lists.begin() + s->all.off + s->all.count);
+  if (s->kbins > 0) {
+    // every substep's three lists (blocks starting a step, the leaves among them, those leaves
+    // and the leaves owed a deposit), each at most every block
+    const size_t lists_len = (size_t)(1 << s->kbins) * 3 * nblocks;
+    s->bins.assign(nblocks, 0);
+    ok = ok && alloc((void**)&s->d_bins, (size_t)nblocks * sizeof(int)) && alloc(&s->d_brate, (size_t)nblocks * s->es) &&
+         alloc((void**)&s->d_blists, lists_len * sizeof(int));
+  }
   if (!ok) {
     shx_amr_destroy(s);
     return fail(SHX_ENOMEM, "device allocation failed for %d blocks (%.2f GB of state and atlases)", nblocks,
@@ -599,6 +767,7 @@ int shx_amr_destroy(shx_amr* s) {
   (void)hipFree(s->d_dep_off); (void)hipFree(s->d_dep); (void)hipFree(s->d_qctl); (void)hipFree(s->d_done);
   (void)hipFree(s->d_tab); (void)hipFree(s->d_lists); (void)hipFree(s->clk); (void)hipFree(s->slot);
   (void)hipFree(s->diag_part); (void)hipFree(s->diag_out);
+  (void)hipFree(s->d_bins); (void)hipFree(s->d_brate); (void)hipFree(s->d_blists);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return SHX_OK;
@@ -623,6 +792,16 @@ int shx_amr_read_state_f64(shx_amr* s, double* u1, double* u2) { return read_sta
 int shx_amr_step(shx_amr* s, int count, int* taken) {
   if (taken) *taken = 0;
   if (!s || count < 0) return fail(SHX_EINVAL, "bad argument");
+  if (s->kbins > 0) {
+    // whole cycles only, as AmrSim.encodeSteps (amr_sim.ts:2783-2790): may run past `count`
+    const int n = 1 << s->kbins;
+    for (int done = 0; done < count; done += n) {
+      int rc = run_cycle(s);
+      if (rc) return rc;
+      if (taken) *taken = done + n;
+    }
+    return SHX_OK;
+  }
   for (int i = 0; i < count; i++) {
     HIPCHK(hipGraphLaunch(s->graph[s->cur], s->stream));
     s->cur = 1 - s->cur;
@@ -653,6 +832,20 @@ int shx_amr_run_to_end(shx_amr* s, int batch, int max_steps, int* steps) {
                            [s](double* c) { return shx_amr_read_clock(s, c); }, s->cfg.t_end, batch, max_steps, steps);
   if (rc) return rc;
   return check_queue(s);
+}
+
+int shx_amr_bins(shx_amr* s, int* out, size_t len, int64_t work[3]) {
+  if (!s) return fail(SHX_EINVAL, "null sim");
+  if (out) {
+    if (len < (size_t)s->nblocks) return fail(SHX_EINVAL, "bins need %d ints", s->nblocks);
+    for (int b = 0; b < s->nblocks; b++) out[b] = s->kbins > 0 ? s->bins[b] : 0;
+  }
+  if (work) {
+    work[0] = s->bin_cycles;
+    work[1] = s->bin_updates;
+    work[2] = s->bin_single;
+  }
+  return SHX_OK;
 }
 
 int shx_amr_write_params(shx_amr* s, const shx_config* cfg) { return amr_reconfigure(s, cfg, false); }

@@ -14,7 +14,7 @@ export type State = Float32Array | Float64Array;
  * SimConfig this module's configFor() makes. Physics the backend does not run is refused at
  * creation when the reference's fields ask for it (gravity, other equations of state, cooling,
  * diffusion, radiation, implicit pressure, species, obstacles, non-Cartesian coordinates,
- * boundary codes above 3, time bins).
+ * boundary codes above 3).
  */
 export interface HydroConfig {
   n: number | [number, number, number];
@@ -37,6 +37,8 @@ export interface HydroConfig {
   etaDual: number;
   shockThresh: number;
   fluff?: [number, number, number] | { rho: number; u: number; v: number; va?: number };
+  /** per-block timesteps on blocks (SimConfig.timeBins) */
+  timeBins?: number;
   precision?: 32 | 64;
   path?: 'fused' | 'plain';
   device?: number;
@@ -99,6 +101,8 @@ export interface AmrTree {
   blocks: Int32Array;
   path?: 'fused' | 'passes' | 'levels';
   compensated?: boolean;
+  /** per-block timesteps: cycles of 2^timeBins substeps (default cfg.timeBins; passes path) */
+  timeBins?: number;
 }
 
 /** The reference's Simulation (src/sim.ts:465-627) as this backend serves it. */

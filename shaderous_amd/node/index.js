@@ -109,7 +109,6 @@ function hipUnsupported(cfg) {
   if (cfg.cooling !== undefined && cfg.cooling !== 'off') return `cooling '${cfg.cooling}'`;
   if (cfg.viscosity > 0 || cfg.conductivity > 0 || cfg.resistivity > 0) return 'viscosity, conduction or resistivity';
   if (cfg.cutCells) return 'cut cells';
-  if (cfg.timeBins > 0) return 'time bins (per-block steps)';
   if ((cfg.driving && cfg.driving.accel > 0) || cfg.dampRate > 0 || (cfg.startDamping && cfg.startDamping.rate > 0))
     return 'driving or damping';
   if (cfg.symmetry && cfg.symmetry.some(Boolean)) return 'mirror symmetry';
@@ -483,7 +482,7 @@ class Sim extends Base {
  * Int32Array of (level, ix, iy, iz) rows, parents and leaves; `root` the root blocks per axis.
  */
 class AmrSim extends Base {
-  constructor(cfg, { root, blocks, path = 'fused', compensated = true }) {
+  constructor(cfg, { root, blocks, path, compensated = true, timeBins }) {
     super(cfg, {
       step: native.amrStep,
       sync: native.amrSync,
@@ -495,7 +494,11 @@ class AmrSim extends Base {
     });
     const n = root.map((r) => 16 * r);
     const ncfg = toNative({ ...cfg, n, ndim: 3 });
-    this.h = native.amrCreate(ncfg, { root, path: code(AMR_PATHS, path, 'AMR path'), compensated }, blocks);
+    // per-block timesteps (SimConfig.timeBins, at most 5): cycles of 2^K substeps, on the passes
+    // path (the fused march takes one dt)
+    const bins = Math.max(0, Math.min(5, Math.floor(timeBins !== undefined ? timeBins : cfg.timeBins || 0)));
+    const kind = path !== undefined ? path : bins > 0 ? 'passes' : 'fused';
+    this.h = native.amrCreate(ncfg, { root, path: code(AMR_PATHS, kind, 'AMR path'), compensated, timeBins: bins }, blocks);
     this.root = root.slice(0, 3);
     this.dx0 = ncfg.dx;
     this.tree(blocks);

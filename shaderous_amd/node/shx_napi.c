@@ -527,6 +527,7 @@ static napi_value js_amr_create(napi_env env, napi_callback_info info) {
   int3_prop(env, argv[1], "root", lay.root);
   lay.path = (int)num_prop(env, argv[1], "path", SHX_AMR_PATH_FUSED);
   lay.compensated = (int)num_prop(env, argv[1], "compensated", 1);
+  lay.time_bins = (int)num_prop(env, argv[1], "timeBins", 0);
   bool is = false;
   napi_is_typedarray(env, argv[2], &is);
   if (!is) { napi_throw_type_error(env, NULL, "blocks is an Int32Array of (level, ix, iy, iz) rows"); return NULL; }

@@ -100,7 +100,7 @@ def test_facade_takes_the_reference_config_and_refuses_other_physics():
         "startDamping:{rate:0,time:0},symmetry:[false,false,false]};"
         "const same=JSON.stringify(s.toNative(ours))===JSON.stringify(s.toNative(ref));"
         "const bad=[{gravity:'constant'},{gext:[0,-1,0]},{eos:'barotropic'},{cooling:'tabulated'},{viscosity:1e-3},"
-        "{radiation:'m1'},{pressure:'implicit'},{timeBins:2},{bcLo:[4,1,0]},{riemann:'hllclm'},"
+        "{radiation:'m1'},{pressure:'implicit'},{bcLo:[4,1,0]},{riemann:'hllclm'},"
         "{problem:{box:2,geometry:'spherical'}},{fluff:{rho:1,u:1,v:1,va:2}}];"
         "const why=bad.map(o=>{try{new s.Sim({...ref,...o});return 'created';}catch(e){return e.message;}});"
         "console.log(JSON.stringify({same,why,a:s.toNative(ref)}))"))
