@@ -273,6 +273,33 @@ def amr_bench(args):
     out = {"workload": "3D blast on 16^3 blocks: root 8^3, 3 levels, sphere r 0.1 at a seeded offset, P 100 vs 1, "
                        "rho 1, gamma 1.4, PLM (MC) + HLLC, CFL 0.4, static mesh",
            "blocks": len(blocks), "leaves": leaves, "leaf_cells": cells, "steps": args.amr_steps, "paths": res}
+    if args.time_bins > 0:
+        # per-block timesteps on the passes path: simulated time per wall second against single
+        # steps of the same path, and the block updates saved
+        rates = {}
+        for k in (0, args.time_bins):
+            sim = AmrSim(cfg, blocks, (8, 8, 8), path="passes", time_bins=k)
+            try:
+                sim.set_state(u1, u2)
+                sim.step(1 << args.time_bins)
+                sim.sync()
+                c0 = sim.clock()
+                t0 = time.perf_counter()
+                sim.step(2 << args.time_bins)
+                sim.sync()
+                t = time.perf_counter() - t0
+                c1 = sim.clock()
+                _, work = sim.bins()
+            finally:
+                sim.destroy()
+            rates[k] = {"sim_time_per_s": ((c1[1] + c1[0]) - (c0[1] + c0[0])) / t, "substeps": 2 << args.time_bins,
+                        "ms_per_substep": t / (2 << args.time_bins) * 1e3, "work": work}
+        kb = rates[args.time_bins]
+        out["time_bins"] = {"K": args.time_bins, "single": rates[0], "bins": kb,
+                            "update_saving": kb["work"]["single"] / kb["work"]["updates"],
+                            "speedup": kb["sim_time_per_s"] / rates[0]["sim_time_per_s"]}
+        log(f"[bench] AMR time bins K={args.time_bins}: {out['time_bins']['update_saving']:.2f}x fewer block updates, "
+            f"{out['time_bins']['speedup']:.2f}x the simulated time per second of single steps (passes path)")
     if "fused" in res:
         f = res["fused"]
         step_ms = f["passes_ms"]["step"]
@@ -432,6 +459,7 @@ def main():
     ap.add_argument("--paths", default="fused,plain", help="which uniform-grid paths to time")
     ap.add_argument("--amr-steps", type=int, default=20, help="timed steps of the AMR blast (0: skip it)")
     ap.add_argument("--amr-paths", default="fused,levels", help="AMR paths to time (fused, passes, levels)")
+    ap.add_argument("--time-bins", type=int, default=3, help="per-block timesteps on the AMR blast (0: skip)")
     ap.add_argument("--ens-steps", type=int, default=200, help="steps of the 2D ensemble (0: skip it)")
     ap.add_argument("--kh-steps", type=int, default=100,
                     help="steps of 2D KH at 256^2 (half at 2048^2 and Sedov 128^3, a fifth at 256^3; 0: skip)")

@@ -10,6 +10,7 @@
 // (readClock, as the loop's clockCopy), stepsResume() when a batch took nothing, until the clock
 // holds (dt = 0).
 // probe: the probe() of that fractional position after the steps. toEnd: runToEnd() after them.
+// tree.timeBins: per-block timesteps on the block grid.
 // pending: readbacks left pending while the sim steps, re-meshes and is destroyed; each must
 // still resolve with the state as it was when it was asked for.
 const fs = require('fs');
@@ -72,7 +73,7 @@ async function main() {
   let sim;
   if (job.kind === 'amr') {
     const blocks = Int32Array.from(job.tree.blocks);
-    sim = shx.createSim(cfg, { root: job.tree.root, blocks, path: job.tree.path || 'passes' });
+    sim = shx.createSim(cfg, { root: job.tree.root, blocks, path: job.tree.path || 'passes', timeBins: job.tree.timeBins });
   } else {
     sim = shx.createSim(cfg);
   }

@@ -389,3 +389,31 @@ def test_node_amr_run_to_end_and_probe_match_python(tmp_path):
     cell = 4096 * slot + loc[0] + 16 * (loc[1] + 16 * loc[2])
     assert pr["u1"] == p1[cell].tolist() and pr["u2"] == p2[cell].tolist()
     assert np.allclose(pr["pos"], [(x + 0.5) / 64 - 0.5 for x in idx])
+
+
+@pytest.mark.gpu
+def test_node_amr_time_bins_match_python(tmp_path):
+    """Per-block timesteps through the facade (tree.timeBins, the reference's cfg.timeBins): the
+    Node caller's cycles give the ctypes caller's bits."""
+    from shaderous_amd.amr import AmrSim
+    from shaderous_amd.sim import SimConfig
+    from test_gpu_bins import flow
+    from test_oracle_bins import refined_tree
+    blocks = refined_tree()
+    u1, u2 = amr_tree.blocks_state(blocks, 1.0 / 32, flow, 1.4)
+    job = {"kind": "amr", "steps": 8, "cfg": {"n": 32, "ndim": 3, "limiter": "mc", "isentropicRecon": False,
+                                              "cfl": 0.4, "box": 1.0},
+           "tree": {"root": [2, 2, 2], "blocks": blocks.reshape(-1).tolist(), "timeBins": 2}}
+    o1, o2, out = _run(tmp_path, job, u1, u2)
+    cfg = SimConfig(n=(32, 32, 32), dx=1.0 / 32, limiter="mc", isentropic=False, cfl=0.4)
+    sim = AmrSim(cfg, blocks, (2, 2, 2), path="passes", time_bins=2)
+    try:
+        sim.set_state(u1, u2)
+        sim.step(8)
+        p1, p2 = sim.read_state()
+        ck = sim.clock()
+    finally:
+        sim.destroy()
+    leaf = amr_tree.leaf_mask(blocks).repeat(4096)
+    assert np.array_equal(o1[leaf], p1[leaf]) and np.array_equal(o2[leaf], p2[leaf])
+    assert out["clock"][3] == ck[3]
