@@ -44,11 +44,9 @@ BYTES_PER_UPDATE = 64      # read 8 FP32 conserved values, write 8
 # WRITE_SIZE (scaled by the counters' reading of a 512 MiB device copy: 2.0 and 1.0), for the
 # configuration they were measured on: profiles/k_fused_turbulence_256.md
 TRAFFIC_MEASURED = {
-    ("k_fused_r", 256, (32, 8, 64)): (2 * 653295 + 524320) * 1024,
-    ("k_fused_r", 256, (32, 8, 128)): (2 * 635980 + 524304) * 1024,
-    ("k_fused_r", 256, (16, 16, 64)): (2 * 801034 + 524323) * 1024,
-    ("k_fused_r", 256, (16, 16, 128)): (2 * 780350 + 524308) * 1024,
-    ("k_fused_r", 256, (32, 16, 128)): (2 * 664318 + 524296) * 1024,
+    ("k_fused_r", 256, (32, 8, 128)): (2 * 501524 + 524304) * 1024,
+    ("k_fused_r", 256, (16, 16, 128)): (2 * 505142 + 524306) * 1024,
+    ("k_fused_r", 256, (32, 16, 128)): (2 * 544123 + 524296) * 1024,
 }
 
 

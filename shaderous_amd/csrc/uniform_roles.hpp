@@ -104,9 +104,23 @@ __global__ __launch_bounds__((RoleTile<TX, TY>::NTH), (RoleTile<TX, TY>::MINW)) 
   const int t = (cy + 2) * BX + (cx + 2);          // raster index of the column
   const bool cell = tid < RT::NCELL;               // wave-uniform
   const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
-  const int x = blockIdx.x * TX + cx;
-  const int y = blockIdx.y * TY + cy;
-  const int z0 = blockIdx.z * zc;
+  // XCD-aware tile order: the dispatcher deals workgroups to the 8 XCDs in turn (workgroup i to
+  // XCD i mod 8), so XCD k takes the k-th of 8 contiguous runs of tiles (the first n mod 8 runs
+  // one tile longer) and marches it in order, rows of tiles side by side: the halo columns two
+  // tiles share are read through one L2. The same tiles are stepped either way.
+  int bx, by, bz;
+  {
+    const int ntile = gridDim.x * gridDim.y * gridDim.z;
+    const int lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int k = lin & 7, q = ntile >> 3, r = ntile & 7;
+    const int m = k * q + min(k, r) + (lin >> 3);
+    bx = m % gridDim.x;
+    by = (m / gridDim.x) % gridDim.y;
+    bz = m / (gridDim.x * gridDim.y);
+  }
+  const int x = bx * TX + cx;
+  const int y = by * TY + cy;
+  const int z0 = bz * zc;
   const int z1 = min(z0 + zc, nz);
   const T dt = step_dt(P, C, 0);
   const T dtdx = dt / P.dx;
