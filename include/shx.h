@@ -182,7 +182,10 @@ int shx_readback_free(shx_readback* rb);
  * where block coordinates count blocks of that level from the domain's lower corner.
  * The shx_config describes the level-0 grid: n = 16 * root blocks per axis, dx = level-0 cell
  * size, ndim = 3. State layout: per slot 4096 cells (x fastest) of u1 and u2 as above; only
- * leaves' cells are read and written. */
+ * leaves' cells are read and written. ndim = 2 (tree.ts:138-175 with ndim 2): a quadtree of
+ * 16^2-cell blocks, root[2] = 1, n[2] = 1, every iz 0, four children per parent (octant
+ * x | y << 1), each z face of the table the block itself; 256 cells per slot; the PASSES and
+ * LEVELS paths. */
 typedef struct shx_amr shx_amr;
 
 /* Kernel path of the AMR step. FUSED (the batched form): a clock pass and one persistent kernel
@@ -216,7 +219,7 @@ int shx_amr_destroy(shx_amr* sim);
  * same-level slot, -1 where the region is one level coarser, -1 - boundary code at a closed
  * face), [12..19] children, [20..25] the coarser leaf beyond each -1 face; -1 elsewhere. */
 int shx_amr_table(shx_amr* sim, int* out, size_t len);
-/* AmrSim.setState / readLeaves: ncell = nblocks * 4096. Resets the clock as shx_set_state. */
+/* AmrSim.setState / readLeaves: ncell = nblocks * 4096 (* 256 in 2D). Resets the clock as shx_set_state. */
 int shx_amr_set_state(shx_amr* sim, const float* u1, const float* u2, size_t ncell, double time, int steps);
 int shx_amr_set_state_f64(shx_amr* sim, const double* u1, const double* u2, size_t ncell, double time, int steps);
 int shx_amr_read_state(shx_amr* sim, float* u1, float* u2);

@@ -40,7 +40,7 @@ class OrcParams(C.Structure):
 
 class OrcAmr(C.Structure):
     _fields_ = [("nblocks", C.c_int), ("root", C.c_int * 3), ("bc_lo", C.c_int * 3), ("bc_hi", C.c_int * 3),
-                ("blk", C.POINTER(C.c_int)), ("tab", C.POINTER(C.c_int))]
+                ("blk", C.POINTER(C.c_int)), ("tab", C.POINTER(C.c_int)), ("ndim", C.c_int)]
 
 
 BT_STRIDE = 32
@@ -92,6 +92,7 @@ def lib():
                                                          C.POINTER(C.c_longlong)]
         I3 = C.c_int * 3
         _lib.orc_amr_build.argtypes = [C.POINTER(OrcAmr), C.POINTER(C.c_int), C.c_int, I3, I3, I3]
+        _lib.orc_amr_build_nd.argtypes = [C.POINTER(OrcAmr), C.POINTER(C.c_int), C.c_int, I3, I3, I3, C.c_int]
         _lib.orc_amr_free.argtypes = [C.POINTER(OrcAmr)]
         _lib.orc_ref_create.argtypes = [C.POINTER(OrcParams)]
         _lib.orc_ref_create.restype = C.c_void_p
@@ -209,9 +210,9 @@ class Wgsl:
         return out
 
     def amr_step(self, prm, table, dx0, u1, u2, clk, compensated=True):
-        """One block-AMR step in place. u1, u2: (nblocks * 4096, 4) leaf interiors, block by block,
-        x fastest; clk: 4-array (dt, t, tc, steps)."""
-        assert u1.dtype == self.dtype and u1.flags.c_contiguous and u1.shape == (table.nblocks * BS ** 3, 4)
+        """One block-AMR step in place. u1, u2: (nblocks * 16^ndim, 4) leaf interiors, block by
+        block, x fastest; clk: 4-array (dt, t, tc, steps)."""
+        assert u1.dtype == self.dtype and u1.flags.c_contiguous and u1.shape == (table.nblocks * BS ** table.ndim, 4)
         assert u2.dtype == self.dtype and u2.flags.c_contiguous and u2.shape == u1.shape
         self.fn("amr_step")(C.byref(prm), C.byref(table.t), dx0, int(bool(compensated)), _ptr(u1, self.ct),
                             _ptr(u2, self.ct), _ptr(clk, self.ct))
@@ -227,7 +228,7 @@ class Wgsl:
     def amr_cycle(self, prm, table, dx0, u1, u2, clk, kmax, bins=None, compensated=True):
         """One cycle of 2^kmax substeps in place (per-block timesteps). bins: fixed bins per block
         (None: the cycle's own). Returns (the bins used, block updates made, leaves << kmax)."""
-        assert u1.dtype == self.dtype and u1.flags.c_contiguous and u1.shape == (table.nblocks * BS ** 3, 4)
+        assert u1.dtype == self.dtype and u1.flags.c_contiguous and u1.shape == (table.nblocks * BS ** table.ndim, 4)
         assert u2.dtype == self.dtype and u2.flags.c_contiguous and u2.shape == u1.shape
         given = None
         if bins is not None:
@@ -241,8 +242,9 @@ class Wgsl:
         return out, int(work[0]), int(work[1])
 
     def amr_assemble(self, prm, table, dx0, u1, u2):
-        """The assemble pass: (W of shape (nblocks, 20, 20, 20, 8) [z, y, x], largest leaf rate/dx)."""
-        W = np.zeros((table.nblocks, PAD, PAD, PAD, 8), self.dtype)
+        """The assemble pass: (W of shape (nblocks, 20, 20, 20, 8) [z, y, x], in 2D (nblocks, 1, 20,
+        20, 8); largest leaf rate/dx)."""
+        W = np.zeros((table.nblocks, PAD if table.ndim == 3 else 1, PAD, PAD, 8), self.dtype)
         r = self.fn("amr_assemble")(C.byref(prm), C.byref(table.t), dx0, _ptr(self.arr(u1), self.ct),
                                     _ptr(self.arr(u2), self.ct), _ptr(W, self.ct))
         return W, r
@@ -260,11 +262,11 @@ class Wgsl:
     def amr_regrid(self, prm, table, u1, u2, kind, src, oct):
         """Re-mesh: the new tree's leaf interiors from the old tree's (table, u1, u2), per new slot
         kind (0 none, 1 copy, 2 prolong, 3 restrict), source old slot and child octant. Returns
-        (o1, o2) of len(kind) * 4096 rows (zero where kind is 0)."""
+        (o1, o2) of len(kind) * 16^ndim rows (zero where kind is 0)."""
         kind, src, oct = (np.ascontiguousarray(a, np.int32) for a in (kind, src, oct))
         n = len(kind)
-        o1 = np.zeros((n * BS ** 3, 4), self.dtype)
-        o2 = np.zeros((n * BS ** 3, 4), self.dtype)
+        o1 = np.zeros((n * BS ** table.ndim, 4), self.dtype)
+        o2 = np.zeros((n * BS ** table.ndim, 4), self.dtype)
         self.fn("amr_regrid")(C.byref(prm), C.byref(table.t), _ptr(self.arr(u1), self.ct), _ptr(self.arr(u2), self.ct),
                               n, _ptr(kind, C.c_int), _ptr(src, C.c_int), _ptr(oct, C.c_int), _ptr(o1, self.ct),
                               _ptr(o2, self.ct))
@@ -339,14 +341,16 @@ class RefHydro:
 
 class AmrTable:
     """The block table of a fixed tree (src/amr/tree.ts:642-675): blocks (nblocks, 4) of
-    (level, ix, iy, iz), parents and leaves; boundary codes per axis (0 periodic)."""
+    (level, ix, iy, iz), parents and leaves; boundary codes per axis (0 periodic); ndim 2 for a
+    quadtree of 16^2-cell blocks (root[2] = 1, every iz 0)."""
 
-    def __init__(self, blocks, root, bc_lo=(0, 0, 0), bc_hi=(0, 0, 0)):
+    def __init__(self, blocks, root, bc_lo=(0, 0, 0), bc_hi=(0, 0, 0), ndim=3):
         self.blocks = np.ascontiguousarray(blocks, dtype=np.int32)
         self.nblocks = len(self.blocks)
+        self.ndim = ndim
         self.t = OrcAmr()
-        rc = lib().orc_amr_build(C.byref(self.t), _ptr(self.blocks, C.c_int), self.nblocks, (C.c_int * 3)(*root),
-                                 (C.c_int * 3)(*bc_lo), (C.c_int * 3)(*bc_hi))
+        rc = lib().orc_amr_build_nd(C.byref(self.t), _ptr(self.blocks, C.c_int), self.nblocks, (C.c_int * 3)(*root),
+                                    (C.c_int * 3)(*bc_lo), (C.c_int * 3)(*bc_hi), ndim)
         if rc != 0:
             lib().orc_amr_free(C.byref(self.t))
             raise ValueError(f"block {-1 - rc} breaks 2:1 balance")

@@ -706,7 +706,8 @@ void FN(predictor)(const orc_params* prm, const real* st, real dt, real out[8]) 
 }
 
 /* ====================== block AMR: the src/amr WGSL passes ======================
- * Blocks of 16^3 cells with 2 ghost layers (BS = 16, NGH = 2, PADW = 20: src/amr/grid.wgsl:7-9).
+ * Blocks of 16^3 cells with 2 ghost layers (BS = 16, NGH = 2, PADW = 20: src/amr/grid.wgsl:7-9);
+ * in 2D (t->ndim 2, is2d() in the WGSL) blocks of 16^2 cells, 20^2 texels, the z index 0 alone.
  * Per block: U (interior, leaves only), W (primitives of every texel but edges and corners),
  * PA/PB (half steps of leaves' interiors and first ghost layer). Edge and corner texels of W are
  * never written and stay zero, as in a fresh atlas (src/amr/grid.wgsl:4-5). One global dt. */
@@ -718,11 +719,18 @@ typedef struct { real a[4]; real b[4]; } Cons;
 
 static int bt_(const orc_amr* t, int s, int k) { return t->tab[ORC_BT_STRIDE * s + k]; }
 static int leaf_(const orc_amr* t, int s) { return (bt_(t, s, 4) & 3) == 3; }
-static size_t pidx(int s, const int l[3]) {
-  return (size_t)s * PD_ * PD_ * PD_ + ((size_t)(l[2] + NG_) * PD_ + (l[1] + NG_)) * PD_ + (l[0] + NG_);
+static int flat_(const orc_amr* t) { return t->ndim == 2; }
+/* texels and cells per block, and the z range of a loop reaching r layers beyond the cells
+   (0 alone in 2D: atl() keeps the single layer, amr/grid.wgsl:164-170) */
+static size_t ntex_(const orc_amr* t) { return flat_(t) ? (size_t)PD_ * PD_ : (size_t)PD_ * PD_ * PD_; }
+static size_t ncel_(const orc_amr* t) { return flat_(t) ? (size_t)BS_ * BS_ : (size_t)BS_ * BS_ * BS_; }
+static int zlo_(const orc_amr* t, int r) { return flat_(t) ? 0 : -r; }
+static int zhi_(const orc_amr* t, int r) { return flat_(t) ? 1 : BS_ + r; }
+static size_t pidx(const orc_amr* t, int s, const int l[3]) {
+  return (size_t)s * ntex_(t) + ((size_t)(flat_(t) ? 0 : l[2] + NG_) * PD_ + (l[1] + NG_)) * PD_ + (l[0] + NG_);
 }
-static size_t iidx(int s, const int l[3]) {
-  return (size_t)s * BS_ * BS_ * BS_ + ((size_t)l[2] * BS_ + l[1]) * BS_ + l[0];
+static size_t iidx(const orc_amr* t, int s, const int l[3]) {
+  return (size_t)s * ncel_(t) + ((size_t)(flat_(t) ? 0 : l[2]) * BS_ + l[1]) * BS_ + l[0];
 }
 
 /* Per-block timesteps (src/amr/bins_on.wgsl): the cycle's bins and the substep being taken; the
@@ -774,9 +782,8 @@ static Src ghost_src(const orc_amr* t, int slot, const int l[3]) {
 }
 
 static Cons u_leaf(const orc_amr* t, const real* u1, const real* u2, int slot, const int l[3]) {
-  (void)t;
   Cons c;
-  size_t o = iidx(slot, l) * 4;
+  size_t o = iidx(t, slot, l) * 4;
   for (int i = 0; i < 4; i++) { c.a[i] = u1[o + i]; c.b[i] = u2[o + i]; }
   return c;
 }
@@ -793,14 +800,17 @@ static Cons u_at(const orc_amr* t, const real* u1, const real* u2, int slot, con
   }
   int f[3] = {2 * (l[0] - (BS_ / 2) * h[0]), 2 * (l[1] - (BS_ / 2) * h[1]), 2 * (l[2] - (BS_ / 2) * h[2])};
   for (int i = 0; i < 4; i++) { c.a[i] = 0; c.b[i] = 0; }
-  for (int k = 0; k < 2; k++)
+  /* the children along z: 2, or 1 in 2D; each child's weight kids_w() (amr/grid.wgsl:51-53) */
+  int nz = flat_(t) ? 1 : 2;
+  real w = flat_(t) ? RL(0.25) : RL(0.125);
+  for (int k = 0; k < nz; k++)
     for (int j = 0; j < 2; j++)
       for (int i = 0; i < 2; i++) {
         int g[3] = {f[0] + i, f[1] + j, f[2] + k};
         Cons u = u_leaf(t, u1, u2, ch, g);
         for (int q = 0; q < 4; q++) { c.a[q] = c.a[q] + RL(1.0) * u.a[q]; c.b[q] = c.b[q] + RL(1.0) * u.b[q]; }
       }
-  for (int q = 0; q < 4; q++) { c.a[q] = c.a[q] * RL(0.125); c.b[q] = c.b[q] * RL(0.125); }
+  for (int q = 0; q < 4; q++) { c.a[q] = c.a[q] * w; c.b[q] = c.b[q] * w; }
   return c;
 }
 /* u_near (src/amr/cons.wgsl:49-55) */
@@ -815,8 +825,8 @@ static int u_near(const orc_amr* t, const real* u1, const real* u2, int slot, co
 /* mid_block / mid_cons (src/amr/mid_on.wgsl:10-30): a leaf part way through a longer step is
    seen through its half-step prediction (its cells and first ghost layer) */
 static int mid_(const orc_amr* t, const Bins* B, int s) { return bins_on_(B) && leaf_(t, s) && !active_(B, s); }
-static Cons mid_cons(const P_t* P, const Bins* B, int slot, const int l[3]) {
-  size_t o = pidx(slot, l) * 4;
+static Cons mid_cons(const orc_amr* t, const P_t* P, const Bins* B, int slot, const int l[3]) {
+  size_t o = pidx(t, slot, l) * 4;
   const real* a = B->PA + o;
   const real* b = B->PB + o;
   real rho = a[0];
@@ -895,11 +905,18 @@ static Cons from_coarse_b(const orc_amr* t, const P_t* P, const real* u1, const 
   Cons v[7];
   int ok[7];
   int mid = mid_(t, B, cs);
+  /* the centre and its neighbours along the axes there are (x and y in 2D: the z neighbours
+     are left unread, not ok; assemble.wgsl:53,62-69) */
+  if (flat_(t)) lc[2] = 0;
+  int n = flat_(t) ? 5 : 7;
   for (int k = 0; k < 7; k++) {
     int at[3] = {lc[0], lc[1], lc[2]};
     if (k > 0) at[(k - 1) / 2] += (k & 1) == 0 ? 1 : -1;
-    if (mid) {
-      v[k] = mid_cons(P, B, cs, at);
+    if (k >= n) {
+      for (int i = 0; i < 4; i++) { v[k].a[i] = 0; v[k].b[i] = 0; }
+      ok[k] = 0;
+    } else if (mid) {
+      v[k] = mid_cons(t, P, B, cs, at);
       ok[k] = 1;
     } else {
       ok[k] = u_near(t, u1, u2, cs, at, &v[k]);
@@ -926,23 +943,23 @@ static real amr_assemble_b(const orc_amr* t, const P_t* P, double dx0, const rea
     if (!active_(B, s)) continue;
     real bbest = RL(0.0);
     real dx = (real)(dx0 / (double)(1 << bt_(t, s, 0)));
-    for (int z = -NG_; z < BS_ + NG_; z++)
+    for (int z = zlo_(t, NG_); z < zhi_(t, NG_); z++)
       for (int y = -NG_; y < BS_ + NG_; y++)
         for (int x = -NG_; x < BS_ + NG_; x++) {
           int l[3] = {x, y, z};
           Src g = ghost_src(t, s, l);
           if (g.kind == 2) continue;
-          Cons u = g.kind == 0 ? (mid_(t, B, g.slot) ? mid_cons(P, B, g.slot, g.l) : u_at(t, u1, u2, g.slot, g.l))
+          Cons u = g.kind == 0 ? (mid_(t, B, g.slot) ? mid_cons(t, P, B, g.slot, g.l) : u_at(t, u1, u2, g.slot, g.l))
                                : from_coarse_b(t, P, u1, u2, s, l, g.slot, B);
           Q w = prim_from_cons(P, u.a, u.b);
           Mapped m;
           for (int a = 0; a < 3; a++) { m.flip[a] = g.flip[a]; m.side[a] = g.side[a]; }
           real v[3];
           bc_velocity(&m, w.a + 1, v);
-          real* o = W + pidx(s, l) * 8;
+          real* o = W + pidx(t, s, l) * 8;
           o[0] = w.a[0]; o[1] = v[0]; o[2] = v[1]; o[3] = v[2];
           o[4] = w.b[0]; o[5] = w.b[2]; o[6] = w.b[1]; o[7] = RL(0.0);
-          int inside = x >= 0 && y >= 0 && z >= 0 && x < BS_ && y < BS_ && z < BS_;
+          int inside = x >= 0 && y >= 0 && z >= 0 && x < BS_ && y < BS_ && z < (flat_(t) ? 1 : BS_);
           if (inside && leaf_(t, s)) {
             real signal = rsqrt_(w.b[2] * w.b[2] * RL(1.0) * RL(1.0) + RL(0.0));
             real rate = rabs(v[0]) + rabs(v[1]) + rabs(v[2]) + (real)P->ndim * signal + RL(0.0);
@@ -956,21 +973,24 @@ static real amr_assemble_b(const orc_amr* t, const P_t* P, double dx0, const rea
   return best;
 }
 
-static Q load_w(const real* W, int s, const int l[3]) {
-  const real* o = W + pidx(s, l) * 8;
+static Q load_w(const orc_amr* t, const real* W, int s, const int l[3]) {
+  const real* o = W + pidx(t, s, l) * 8;
   Q q;
   q.a[0] = rabs(o[0]); q.a[1] = o[1]; q.a[2] = o[2]; q.a[3] = o[3];
   q.b[0] = o[4]; q.b[1] = o[6]; q.b[2] = o[5]; q.b[3] = o[0] < 0 ? RL(1.0) : RL(0.0);
   return q;
 }
-/* predict_cell (src/amr/predict.wgsl:90-181): mask multiplies each axis's slopes */
-static Q predict_cell(const P_t* P, const real* W, int s, const int l[3], const real mask[3], real dt, real dx) {
-  Q q = load_w(W, s, l), nb[6];
+/* predict_cell (src/amr/predict.wgsl:90-181): mask multiplies each axis's slopes; in 2D the
+   cell stands for its own z neighbours (:103-108) */
+static Q predict_cell(const orc_amr* t, const P_t* P, const real* W, int s, const int l[3], const real mask[3],
+                      real dt, real dx) {
+  Q q = load_w(t, W, s, l), nb[6];
   for (int a = 0; a < 3; a++) {
+    if (a == 2 && flat_(t)) { nb[4] = q; nb[5] = q; continue; }
     int m[3] = {l[0], l[1], l[2]}, p[3] = {l[0], l[1], l[2]};
     m[a] -= 1; p[a] += 1;
-    nb[2 * a] = load_w(W, s, m);
-    nb[2 * a + 1] = load_w(W, s, p);
+    nb[2 * a] = load_w(t, W, s, m);
+    nb[2 * a + 1] = load_w(t, W, s, p);
   }
   P_t Pl = *P;
   Pl.dx = dx;
@@ -992,7 +1012,7 @@ static void amr_predict_b(const orc_amr* t, const P_t* P, double dx0, const real
   for (int s = 0; s < t->nblocks; s++) {
     if (!leaf_(t, s) || !active_(B, s)) continue;
     real dtb = dt * (real)(1 << bin_(B, s));
-    for (int z = -1; z < BS_ + 1; z++)
+    for (int z = zlo_(t, 1); z < zhi_(t, 1); z++)
       for (int y = -1; y < BS_ + 1; y++)
         for (int x = -1; x < BS_ + 1; x++) {
           int l[3] = {x, y, z};
@@ -1006,7 +1026,7 @@ static void amr_predict_b(const orc_amr* t, const P_t* P, double dx0, const real
           if (!at_source)
             for (int a = 0; a < 3; a++) if (l[a] == 0 || l[a] == BS_ - 1) mask[a] = 0;
           real dx = (real)(dx0 / (double)(1 << bt_(t, slot, 0)));
-          Q pr = predict_cell(P, W, slot, cl, mask, dtb, dx);
+          Q pr = predict_cell(t, P, W, slot, cl, mask, dtb, dx);
           if (at_source) {
             Mapped m;
             for (int a = 0; a < 3; a++) { m.flip[a] = g.flip[a]; m.side[a] = g.side[a]; }
@@ -1014,19 +1034,20 @@ static void amr_predict_b(const orc_amr* t, const P_t* P, double dx0, const real
             bc_velocity(&m, pr.a + 1, v);
             pr.a[1] = v[0]; pr.a[2] = v[1]; pr.a[3] = v[2];
           }
-          size_t o = pidx(s, l) * 4;
+          size_t o = pidx(t, s, l) * 4;
           for (int i = 0; i < 4; i++) { PA[o + i] = pr.a[i]; PB[o + i] = pr.b[i]; }
         }
   }
 }
 
 /* flux_between on block s (src/hydro/flux.wgsl:254-347 with stored ghosts: amr/update.wgsl:50-58) */
-static Flux amr_flux_between(const P_t* P, const real* W, const real* PA, const real* PB, int s, const int a[3], int axis) {
+static Flux amr_flux_between(const orc_amr* t, const P_t* P, const real* W, const real* PA, const real* PB, int s,
+                             const int a[3], int axis) {
   int ap[3] = {a[0], a[1], a[2]}, am[3] = {a[0], a[1], a[2]}, app[3] = {a[0], a[1], a[2]};
   ap[axis] += 1; am[axis] -= 1; app[axis] += 2;
-  Q wl = load_w(W, s, a), wr = load_w(W, s, ap), wll = load_w(W, s, am), wrr = load_w(W, s, app);
+  Q wl = load_w(t, W, s, a), wr = load_w(t, W, s, ap), wll = load_w(t, W, s, am), wrr = load_w(t, W, s, app);
   Q pl, pr;
-  size_t ol = pidx(s, a) * 4, orr = pidx(s, ap) * 4;
+  size_t ol = pidx(t, s, a) * 4, orr = pidx(t, s, ap) * 4;
   for (int i = 0; i < 4; i++) { pl.a[i] = PA[ol + i]; pl.b[i] = PB[ol + i]; pr.a[i] = PA[orr + i]; pr.b[i] = PB[orr + i]; }
   Q sl = edge(P, &wll, &wl, &wr, 1);
   Q sr = edge(P, &wl, &wr, &wrr, 0);
@@ -1067,27 +1088,28 @@ static void amr_update_b(const orc_amr* t, const P_t* P, double dx0, int comp, c
     int gbin = bin_(B, s);
     int deposit = bins_on_(B) && !active_(B, s);
     real dtb = dt * (real)(1 << gbin);
-    for (int z = 0; z < BS_; z++)
+    for (int z = 0; z < zhi_(t, 0); z++)
       for (int y = 0; y < BS_; y++)
         for (int x = 0; x < BS_; x++) {
           int l[3] = {x, y, z};
           real fl[3][8], fh[3][8];
           for (int axis = 0; axis < 3; axis++) {
-            if (deposit) {
+            /* no faces along z in 2D (update.wgsl:374,380) */
+            if (deposit || (axis == 2 && flat_(t))) {
               for (int q = 0; q < 8; q++) { fl[axis][q] = RL(0.0); fh[axis][q] = RL(0.0); }
               continue;
             }
             int m[3] = {x, y, z};
             m[axis] -= 1;
-            Flux a = amr_flux_between(P, W, PA, PB, s, m, axis);
-            Flux b = amr_flux_between(P, W, PA, PB, s, l, axis);
+            Flux a = amr_flux_between(t, P, W, PA, PB, s, m, axis);
+            Flux b = amr_flux_between(t, P, W, PA, PB, s, l, axis);
             flux_pack(&a, fl[axis]);
             flux_pack(&b, fh[axis]);
           }
           int touched = 0;
           /* faces next to a refined neighbour: the mean of the fine fluxes; with bins, faces next
              to a faster leaf of the same level: half of its flux */
-          for (int k = 0; k < 6; k++) {
+          for (int k = 0; k < (flat_(t) ? 4 : 6); k++) {
             int axis = k / 2, hi = k & 1;
             if (l[axis] != (hi ? BS_ - 1 : 0)) continue;
             int np = bt_(t, s, 6 + k);
@@ -1108,13 +1130,17 @@ static void amr_update_b(const orc_amr* t, const P_t* P, double dx0, int comp, c
               int a0[3] = {2 * (l[0] - (BS_ / 2) * h[0]), 2 * (l[1] - (BS_ / 2) * h[1]), 2 * (l[2] - (BS_ / 2) * h[2])};
               a0[axis] = upper ? BS_ - 1 : -1;
               int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
+              /* fine faces along each transverse axis: 2, or 1 along z in 2D, where the faces
+                 are edges, two fine ones per coarse one (update.wgsl:246-251) */
+              int n1 = (flat_(t) && t1 == 2) ? 1 : 2, n2 = (flat_(t) && t2 == 2) ? 1 : 2;
+              real w = RL(1.0) / (real)(n1 * n2);
               real sum[8] = {0, 0, 0, 0, 0, 0, 0, 0}, err[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-              for (int j = 0; j < 2; j++)
-                for (int i = 0; i < 2; i++) {
+              for (int j = 0; j < n2; j++)
+                for (int i = 0; i < n1; i++) {
                   int a[3] = {a0[0], a0[1], a0[2]};
                   a[t1] += i;
                   a[t2] += j;
-                  Flux f = amr_flux_between(P, W, PA, PB, ch, a, axis);
+                  Flux f = amr_flux_between(t, P, W, PA, PB, ch, a, axis);
                   real fv[8];
                   flux_pack(&f, fv);
                   for (int q = 0; q < 8; q++) {
@@ -1122,19 +1148,19 @@ static void amr_update_b(const orc_amr* t, const P_t* P, double dx0, int comp, c
                     else sum[q] = sum[q] + fv[q];
                   }
                 }
-              for (int q = 0; q < 8; q++) dst[q] = (sum[q] + err[q]) * RL(0.25) * part;
+              for (int q = 0; q < 8; q++) dst[q] = (sum[q] + err[q]) * w * part;
               touched = 1;
             } else if (bins_on_(B) && np != s && bin_(B, np) < gbin) {
               int a[3] = {l[0], l[1], l[2]};
               a[axis] = upper ? BS_ - 1 : -1;
-              Flux f = amr_flux_between(P, W, PA, PB, np, a, axis);
+              Flux f = amr_flux_between(t, P, W, PA, PB, np, a, axis);
               real fv[8];
               flux_pack(&f, fv);
               for (int q = 0; q < 8; q++) dst[q] = fv[q] * RL(0.5);
               touched = 1;
             }
           }
-          size_t o = iidx(s, l) * 4;
+          size_t o = iidx(t, s, l) * 4;
           if (deposit && !touched) {
             for (int q = 0; q < 4; q++) { o1[o + q] = u1[o + q]; o2[o + q] = u2[o + q]; }
             continue;
@@ -1143,7 +1169,7 @@ static void amr_update_b(const orc_amr* t, const P_t* P, double dx0, int comp, c
           for (int q = 0; q < 8; q++) net[q] = ((fl[0][q] - fh[0][q]) + (fl[1][q] - fh[1][q])) + (fl[2][q] - fh[2][q]);
           P_t Pl = *P;
           Pl.dx = dx;
-          advance(&Pl, u1 + o, u2 + o, net, PB[pidx(s, l) * 4], dtb, o1 + o, o2 + o);
+          advance(&Pl, u1 + o, u2 + o, net, PB[pidx(t, s, l) * 4], dtb, o1 + o, o2 + o);
         }
   }
 }
@@ -1151,8 +1177,8 @@ static void amr_update_b(const orc_amr* t, const P_t* P, double dx0, int comp, c
 void FN(amr_step)(const orc_params* prm, const orc_amr* t, double dx0, int compensated, real* u1, real* u2,
                   real clk[4]) {
   P_t P = load_params(prm);
-  size_t np = (size_t)t->nblocks * PD_ * PD_ * PD_;
-  size_t ni = (size_t)t->nblocks * BS_ * BS_ * BS_;
+  size_t np = (size_t)t->nblocks * ntex_(t);
+  size_t ni = (size_t)t->nblocks * ncel_(t);
   real* W = (real*)calloc(np * 8, sizeof(real));
   real* PA = (real*)calloc(np * 4, sizeof(real));
   real* PB = (real*)calloc(np * 4, sizeof(real));
@@ -1290,8 +1316,8 @@ void FN(amr_cycle)(const orc_params* prm, const orc_amr* t, double dx0, int comp
                    real* u1, real* u2, real clk[4], int* bins_out, long long work[2]) {
   P_t P = load_params(prm);
   int n = t->nblocks;
-  size_t np = (size_t)n * PD_ * PD_ * PD_;
-  size_t ni = (size_t)n * BS_ * BS_ * BS_;
+  size_t np = (size_t)n * ntex_(t);
+  size_t ni = (size_t)n * ncel_(t);
   real* W = (real*)calloc(np * 8, sizeof(real));
   real* PA = (real*)calloc(np * 4, sizeof(real));
   real* PB = (real*)calloc(np * 4, sizeof(real));
@@ -1324,9 +1350,9 @@ void FN(amr_cycle)(const orc_params* prm, const orc_amr* t, double dx0, int comp
     for (int s = 0; s < n; s++) {
       if (!lu[s]) continue;
       updates++;
-      size_t o = (size_t)s * BS_ * BS_ * BS_ * 4;
-      memcpy(u1 + o, o1 + o, (size_t)BS_ * BS_ * BS_ * 4 * sizeof(real));
-      memcpy(u2 + o, o2 + o, (size_t)BS_ * BS_ * BS_ * 4 * sizeof(real));
+      size_t o = (size_t)s * ncel_(t) * 4;
+      memcpy(u1 + o, o1 + o, ncel_(t) * 4 * sizeof(real));
+      memcpy(u2 + o, o2 + o, ncel_(t) * 4 * sizeof(real));
     }
   }
   if (work) { work[0] = updates; work[1] = leaves << kmax; }
@@ -1348,7 +1374,7 @@ void FN(amr_regrid)(const orc_params* prm, const orc_amr* told, const real* u1, 
   P_t P = load_params(prm);
   for (int b = 0; b < nnew; b++) {
     if (kind[b] == 0) continue;
-    for (int z = 0; z < BS_; z++)
+    for (int z = 0; z < zhi_(told, 0); z++)
       for (int y = 0; y < BS_; y++)
         for (int x = 0; x < BS_; x++) {
           int l[3] = {x, y, z};
@@ -1365,24 +1391,31 @@ void FN(amr_regrid)(const orc_params* prm, const orc_amr* told, const real* u1, 
             }
             Cons v[7];
             int ok[7];
+            /* 2D: no z neighbours, children along x and y alone (regrid.wgsl:58-62, 86-97) */
+            int n = flat_(told) ? 5 : 7;
             for (int k = 0; k < 7; k++) {
               int at[3] = {lc[0], lc[1], lc[2]};
               if (k > 0) at[(k - 1) / 2] += (k & 1) == 0 ? 1 : -1;
-              ok[k] = u_near(told, u1, u2, src[b], at, &v[k]);
+              if (k >= n) {
+                for (int i = 0; i < 4; i++) { v[k].a[i] = 0; v[k].b[i] = 0; }
+                ok[k] = 0;
+              } else {
+                ok[k] = u_near(told, u1, u2, src[b], at, &v[k]);
+              }
             }
             u = prolong(&P, v, ok, par);
             real ke = RL(0.0);
-            for (int k = 0; k < 2; k++)
+            for (int k = 0; k < (flat_(told) ? 1 : 2); k++)
               for (int j = 0; j < 2; j++)
                 for (int i = 0; i < 2; i++) {
                   int pc[3] = {i, j, k};
                   Cons ch = prolong(&P, v, ok, pc);
                   ke = ke + ke_of(&P, &ch);
                 }
-            ke = ke * RL(0.125);
+            ke = ke * (flat_(told) ? RL(0.25) : RL(0.125));
             u.b[0] = ke_of(&P, &u) + u.b[1] + ((v[0].b[0] - v[0].b[1]) - ke);
           }
-          size_t o = ((size_t)b * BS_ * BS_ * BS_ + ((size_t)z * BS_ + y) * BS_ + x) * 4;
+          size_t o = iidx(told, b, l) * 4;
           for (int i = 0; i < 4; i++) { o1[o + i] = u.a[i]; o2[o + i] = u.b[i]; }
         }
   }

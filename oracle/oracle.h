@@ -99,9 +99,12 @@ typedef struct {
   int* blk;   /* per block (level, ix, iy, iz) */
   int* tab;   /* per block ORC_BT_STRIDE ints: [0] level [1..3] coords [4] flags (1 active, 2 leaf)
                  [5] parent [6..11] face neighbours [12..19] children [20..25] coarser leaves */
+  int ndim;   /* 3, or 2: blocks of 16^2 cells (20^2 texels), 4 children, z faces the block itself */
 } orc_amr;
 /* 0, or -1 - s where block s breaks 2:1 balance */
 int orc_amr_build(orc_amr* t, const int* blocks, int nblocks, const int root[3], const int bc_lo[3], const int bc_hi[3]);
+int orc_amr_build_nd(orc_amr* t, const int* blocks, int nblocks, const int root[3], const int bc_lo[3],
+                     const int bc_hi[3], int ndim);
 void orc_amr_free(orc_amr* t);
 const int* orc_amr_table(const orc_amr* t);
 

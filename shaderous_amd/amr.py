@@ -1,7 +1,7 @@
 """The block-AMR simulation, mirroring the reference's `AmrSim` for the hydro path.
 
 The reference's AmrSim (src/amr/amr_sim.ts) runs the same hydro step on 16^3-cell blocks of a
-BlockTree (src/amr/tree.ts): ``create`` (:1924-1992, with uploadTree :2566-2625), ``step`` /
+BlockTree (src/amr/tree.ts), or on 16^2-cell blocks of a quadtree in 2D (cfg.ndim 2): ``create`` (:1924-1992, with uploadTree :2566-2625), ``step`` /
 encodeSteps (:2709-2947), ``readLeaves``, ``diagnostics``, re-meshing (the flagging pass
 :3029-3060 and applyRegrid :2648-2697), ``destroy``. The tree is given as the list of its blocks,
 parents and leaves, as (level, ix, iy, iz); ``regrid`` replaces it by one within a level of it
@@ -23,19 +23,27 @@ BS = 16
 CELLS = BS ** 3
 
 
+def _level0(root, ndim):
+    """The level-0 grid's cells per axis: 16 per root block (one layer along z in 2D)."""
+    return (BS * root[0], BS * root[1], 1 if ndim == 2 else BS * root[2])
+
+
 class AmrSim:
     """A block-AMR hydro run on one GPU. `cfg` describes the level-0 grid (n = 16 x root blocks,
-    dx = level-0 cell size); `blocks` is (nblocks, 4) of (level, ix, iy, iz)."""
+    dx = level-0 cell size; cfg.ndim 2 for a quadtree, root[2] = 1); `blocks` is (nblocks, 4) of
+    (level, ix, iy, iz)."""
 
     def __init__(self, cfg: SimConfig, blocks, root, path: str = "fused", compensated: bool = True,
                  time_bins: int = 0):
         lib = _lib.load()
         root = tuple(int(r) for r in root)
-        self.cfg = replace(cfg, n=tuple(BS * r for r in root), ndim=3)
+        self.ndim = 2 if cfg.ndim == 2 else 3
+        self.cells = BS ** self.ndim
+        self.cfg = replace(cfg, n=_level0(root, self.ndim), ndim=self.ndim)
         self.root = root
         self.blocks = np.ascontiguousarray(blocks, dtype=np.int32).reshape(-1, 4)
         self.nblocks = len(self.blocks)
-        self.ncell = self.nblocks * CELLS
+        self.ncell = self.nblocks * self.cells
         self.dtype = np.float64 if cfg.precision == 64 else np.float32
         lay = ShxAmrLayout()
         lay.root[:] = list(root)
@@ -73,7 +81,7 @@ class AmrSim:
         check(_lib.load().shx_amr_regrid(self._h, blocks.ctypes.data_as(C.POINTER(C.c_int)), len(blocks)))
         self.blocks = blocks
         self.nblocks = len(blocks)
-        self.ncell = self.nblocks * CELLS
+        self.ncell = self.nblocks * self.cells
 
     def adapt(self, refine_above: float, coarsen_below: float, max_level: int, periodic=None) -> bool:
         """One re-mesh from the indicator (amr_tree.adapt_tree): split leaves above
@@ -84,7 +92,7 @@ class AmrSim:
         if periodic is None:
             periodic = tuple(self.cfg.bc_lo[a] == 0 and self.cfg.bc_hi[a] == 0 for a in range(3))
         new = amr_tree.adapt_tree(self.blocks, self.root, self.indicator(), refine_above, coarsen_below, max_level,
-                                  periodic)
+                                  periodic, ndim=self.ndim)
         if len(new) == len(self.blocks) and {tuple(b) for b in new.tolist()} == {tuple(b) for b in self.blocks.tolist()}:
             return False
         self.regrid(new)
@@ -101,7 +109,7 @@ class AmrSim:
         u1 = np.ascontiguousarray(u1, self.dtype).reshape(-1)
         u2 = np.ascontiguousarray(u2, self.dtype).reshape(-1)
         if u1.size != 4 * self.ncell or u2.size != 4 * self.ncell:
-            raise ValueError(f"state must hold {self.nblocks} blocks x 4096 cells of 4 values")
+            raise ValueError(f"state must hold {self.nblocks} blocks x {self.cells} cells of 4 values")
         lib = _lib.load()
         if self.dtype == np.float64:
             P = C.POINTER(C.c_double)
@@ -112,7 +120,7 @@ class AmrSim:
             check(lib.shx_amr_set_state(self._h, u1.ctypes.data_as(P), u2.ctypes.data_as(P), self.ncell, time, steps))
 
     def read_state(self):
-        """(u1, u2) as (nblocks * 4096, 4) arrays, block by block; parents' cells are unused."""
+        """(u1, u2) as (nblocks * 16^ndim, 4) arrays, block by block; parents' cells are unused."""
         u1 = np.empty((self.ncell, 4), self.dtype)
         u2 = np.empty((self.ncell, 4), self.dtype)
         lib = _lib.load()
@@ -131,14 +139,15 @@ class AmrSim:
         dx0 = self.cfg.dx
         xs, ys, zs, dxs, sel = [], [], [], [], []
         ax = np.arange(BS) + 0.5
+        nc = self.cells
         for i in np.flatnonzero(leaf):
             L, bx, by, bz = (int(v) for v in self.blocks[i])
             h = dx0 / (1 << L)
-            z, y, x = np.meshgrid((ax + BS * bz) * h + box_lo, (ax + BS * by) * h + box_lo,
-                                  (ax + BS * bx) * h + box_lo, indexing="ij")
+            zax = (ax + BS * bz) * h + box_lo if self.ndim == 3 else np.zeros(1)
+            z, y, x = np.meshgrid(zax, (ax + BS * by) * h + box_lo, (ax + BS * bx) * h + box_lo, indexing="ij")
             xs.append(x.ravel()); ys.append(y.ravel()); zs.append(z.ravel())
-            dxs.append(np.full(CELLS, h))
-            sel.append(np.arange(i * CELLS, (i + 1) * CELLS))
+            dxs.append(np.full(nc, h))
+            sel.append(np.arange(i * nc, (i + 1) * nc))
         sel = np.concatenate(sel)
         return {"x": np.concatenate(xs), "y": np.concatenate(ys), "z": np.concatenate(zs), "dx": np.concatenate(dxs),
                 "u1": u1[sel], "u2": u2[sel], "count": sel.size}
@@ -187,7 +196,7 @@ class AmrSim:
         return n.value
 
     def _reconfigure(self, fn, cfg):
-        cfg = replace(cfg or self.cfg, n=tuple(BS * r for r in self.root), ndim=3)
+        cfg = replace(cfg or self.cfg, n=_level0(self.root, self.ndim), ndim=self.ndim)
         c = cfg.to_c()
         check(fn(self._h, C.byref(c)))
         self.cfg = cfg

@@ -120,8 +120,11 @@ AmrOps amr_ops_f32(int lim, int rs, int isen);
 AmrOps amr_ops_other32(int lim, int rs);
 AmrOps amr_ops_koren_hll32(int isen);
 AmrOps amr_ops_f64(int lim, int rs, int isen);
+AmrOps amr2d_ops_f32(int lim, int rs, int isen);
+AmrOps amr2d_ops_f64(int lim, int rs, int isen);
 
-inline AmrOps amr_ops(int precision, int lim, int rs, int isen) {
+inline AmrOps amr_ops(int precision, int lim, int rs, int isen, int ndim = 3) {
+  if (ndim == 2) return precision == 64 ? amr2d_ops_f64(lim, rs, isen) : amr2d_ops_f32(lim, rs, isen);
   return precision == 64 ? amr_ops_f64(lim, rs, isen) : amr_ops_f32(lim, rs, isen);
 }
 

@@ -589,9 +589,11 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
   }
 }
 
-template <typename T, int LIM, int RS, bool ISEN>
-void AmrImpl<T, LIM, RS, ISEN>::shell(const AmrLaunch& L, const int* work0, int n0, const int* work1, int n1,
-                                      const void* u1, const void* u2, void* sh, const void* clk) {
+// (3D blocks only: the ops of a 2D grid leave the fused form out)
+template <typename T, int LIM, int RS, bool ISEN, int D>
+void AmrImpl<T, LIM, RS, ISEN, D>::shell(const AmrLaunch& L, const int* work0, int n0, const int* work1, int n1,
+                                         const void* u1, const void* u2, void* sh, const void* clk) {
+  static_assert(D == 3, "the fused form steps 3D blocks");
   if (n0 > 0)
     hipLaunchKernelGGL((k_amr_shell<T, LIM, ISEN, 0>), dim3((n0 + 3) / 4), dim3(256), 0, L.stream, P(L), LV(L), L.tab,
                        (const int4*)work0, n0, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)sh, (const T*)clk);
@@ -600,9 +602,11 @@ void AmrImpl<T, LIM, RS, ISEN>::shell(const AmrLaunch& L, const int* work0, int 
                        (const int4*)work1, n1, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)sh, (const T*)clk);
 }
 
-template <typename T, int LIM, int RS, bool ISEN>
-void AmrImpl<T, LIM, RS, ISEN>::fused(const AmrLaunch& L, const AmrQueue& Q, const void* sh, const void* u1,
-                                      const void* u2, void* o1, void* o2, const void* clk, unsigned long long* slot) {
+template <typename T, int LIM, int RS, bool ISEN, int D>
+void AmrImpl<T, LIM, RS, ISEN, D>::fused(const AmrLaunch& L, const AmrQueue& Q, const void* sh, const void* u1,
+                                         const void* u2, void* o1, void* o2, const void* clk,
+                                         unsigned long long* slot) {
+  static_assert(D == 3, "the fused form steps 3D blocks");
   if (Q.count <= 0) return;
   constexpr int np = af_pair<T>();
   const int pairs = (Q.count + np - 1) / np;

@@ -10,7 +10,10 @@
 //                   face on a refined neighbour takes the mean of the four fine fluxes,
 //                   optionally summed with TwoSum (error-free FP32 addition) before the mean
 // Each kernel takes a slot list: the host launches them once per level (the plain form) or once
-// over every level (batched); the results are the same bits either way.
+// over every level (batched); the results are the same bits either way. D is the grid's
+// dimension: 3, or 2 for quadtrees of 16^2-cell blocks (the is2d() branches of the passes: the
+// z index is 0 throughout, no z neighbours, four children per parent, two fine faces per coarse
+// face).
 #pragma once
 #include "amr.hpp"
 #include "uniform_impl.hpp"
@@ -24,6 +27,23 @@ __device__ __forceinline__ int tex(int x, int y, int z) {
   return ((z + AMR_NG) * AMR_PAD + (y + AMR_NG)) * AMR_PAD + (x + AMR_NG);
 }
 __device__ __forceinline__ int cel(int x, int y, int z) { return (z * AMR_BS + y) * AMR_BS + x; }
+// per block: cells, texels (20^D) and the predicted texels (cells and first ghost layer, 18^D)
+template <int D>
+struct AmrGeo {
+  static constexpr int CELLS = D == 3 ? AMR_CELLS : AMR_BS * AMR_BS;
+  static constexpr int TEXELS = D == 3 ? AMR_TEXELS : AMR_PAD * AMR_PAD;
+  static constexpr int ZC = D == 3 ? AMR_BS : 1;   // cells along z
+};
+template <int D>
+__device__ __forceinline__ int texd(int x, int y, int z) {
+  if constexpr (D == 3) return tex(x, y, z);
+  return (y + AMR_NG) * AMR_PAD + (x + AMR_NG);
+}
+template <int D>
+__device__ __forceinline__ int celd(int x, int y, int z) {
+  if constexpr (D == 3) return cel(x, y, z);
+  return y * AMR_BS + x;
+}
 // The cell size of level L: the level-0 size times 2^-L, exact (the same value as rounding
 // dx0 / 2^L once); formed arithmetically, as an indexed kernel-argument array would be copied
 // to scratch memory
@@ -86,41 +106,42 @@ __device__ __forceinline__ T4<T> v4(T x, T y, T z, T w) {
   return r;
 }
 
-template <typename T>
+template <typename T, int D = 3>
 __device__ __forceinline__ Cons<T> u_leaf(const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, int slot, int x,
                                           int y, int z) {
-  size_t o = (size_t)slot * AMR_CELLS + cel(x, y, z);
+  size_t o = (size_t)slot * AmrGeo<D>::CELLS + celd<D>(x, y, z);
   return Cons<T>{U1[o], U2[o]};
 }
 
 // u_at (amr/cons.wgsl:19-43): a leaf's cell, or a parent's cell as the mean of its 8 children's
-template <typename T>
+// (4 in 2D, each of weight kids_w(), amr/grid.wgsl:51-53)
+template <typename T, int D = 3>
 __device__ __forceinline__ Cons<T> u_at(const int* __restrict__ tab, const T4<T>* __restrict__ U1,
                                         const T4<T>* __restrict__ U2, int slot, int x, int y, int z) {
-  if (is_leaf(tab, slot)) return u_leaf<T>(U1, U2, slot, x, y, z);
+  if (is_leaf(tab, slot)) return u_leaf<T, D>(U1, U2, slot, x, y, z);
   const int hx = x >= AMR_BS / 2, hy = y >= AMR_BS / 2, hz = z >= AMR_BS / 2;
   const int ch = bt(tab, slot, BT_CHILD + (hx | (hy << 1) | (hz << 2)));
   if (ch < 0 || !is_leaf(tab, ch)) return Cons<T>{v4<T>(1, 0, 0, 0), v4<T>(1, 1, 0, 0)};
   const int fx = 2 * (x - (AMR_BS / 2) * hx), fy = 2 * (y - (AMR_BS / 2) * hy), fz = 2 * (z - (AMR_BS / 2) * hz);
   Cons<T> c{v4<T>(0, 0, 0, 0), v4<T>(0, 0, 0, 0)};
 #pragma unroll
-  for (int k = 0; k < 2; k++)
+  for (int k = 0; k < (D == 3 ? 2 : 1); k++)
 #pragma unroll
     for (int j = 0; j < 2; j++)
 #pragma unroll
       for (int i = 0; i < 2; i++) {
-        Cons<T> u = u_leaf<T>(U1, U2, ch, fx + i, fy + j, fz + k);
+        Cons<T> u = u_leaf<T, D>(U1, U2, ch, fx + i, fy + j, fz + k);
         c.a.x += u.a.x; c.a.y += u.a.y; c.a.z += u.a.z; c.a.w += u.a.w;
         c.b.x += u.b.x; c.b.y += u.b.y; c.b.z += u.b.z; c.b.w += u.b.w;
       }
-  const T e = T(0.125);
+  const T e = D == 3 ? T(0.125) : T(0.25);
   c.a.x *= e; c.a.y *= e; c.a.z *= e; c.a.w *= e;
   c.b.x *= e; c.b.y *= e; c.b.z *= e; c.b.w *= e;
   return c;
 }
 
 // u_near (amr/cons.wgsl:49-55): the state at a coarse block's texel, when it is a cell
-template <typename T>
+template <typename T, int D = 3>
 __device__ __forceinline__ bool u_near(const int* __restrict__ tab, const T4<T>* __restrict__ U1,
                                        const T4<T>* __restrict__ U2, int cs, int x, int y, int z, Cons<T>& out) {
   GSrc g = ghost_src(tab, cs, x, y, z);
@@ -128,7 +149,7 @@ __device__ __forceinline__ bool u_near(const int* __restrict__ tab, const T4<T>*
     out = Cons<T>{v4<T>(0, 0, 0, 0), v4<T>(0, 0, 0, 0)};
     return false;
   }
-  Cons<T> u = u_at<T>(tab, U1, U2, g.slot, g.x, g.y, g.z);
+  Cons<T> u = u_at<T, D>(tab, U1, U2, g.slot, g.x, g.y, g.z);
   u.a.y = u.a.y * T(g.fx) + u.a.x * T(0);
   u.a.z = u.a.z * T(g.fy) + u.a.x * T(0);
   u.a.w = u.a.w * T(g.fz) + u.a.x * T(0);
@@ -148,9 +169,9 @@ __device__ __forceinline__ bool mid_block(const AmrBins& B, const int* __restric
 }
 // mid_cons (amr/mid_on.wgsl:14-30): the conserved state of texel (x, y, z) of block `slot` (a cell
 // or a face ghost of the first layer) at the middle of its step, from its half-step prediction
-template <typename T>
+template <typename T, int D = 3>
 __device__ __forceinline__ Cons<T> mid_cons(const Params<T>& P, const AmrBins& B, int slot, int x, int y, int z) {
-  const size_t o = (size_t)slot * AMR_TEXELS + tex(x, y, z);
+  const size_t o = (size_t)slot * AmrGeo<D>::TEXELS + texd<D>(x, y, z);
   const T4<T> a = static_cast<const T4<T>*>(B.pa)[o];
   const T4<T> b = static_cast<const T4<T>*>(B.pb)[o];
   const T rho = a.x;
@@ -224,8 +245,8 @@ __device__ __forceinline__ Cons<T> prolong(const Params<T>& P, const Cons<T> v[7
 }
 
 // from_coarse (amr/assemble.wgsl:47-75): texel (x, y, z) of block `slot` interpolated from the
-// coarser leaf cs. n0 = level-0 cells per axis.
-template <typename T, int LIM>
+// coarser leaf cs. n0 = level-0 cells per axis. In 2D the z neighbours are left unread (not ok).
+template <typename T, int LIM, int D = 3>
 __device__ __forceinline__ Cons<T> from_coarse(const Params<T>& P, const int* __restrict__ tab,
                                                const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, int slot,
                                                int x, int y, int z, int cs, const AmrBins& B = AmrBins{}) {
@@ -241,42 +262,52 @@ __device__ __forceinline__ Cons<T> from_coarse(const Params<T>& P, const int* __
     c[a] = gc - bt(tab, cs, BT_X + a) * AMR_BS;
     par[a] = gf & 1;
   }
+  if constexpr (D == 2) c[2] = 0;
   Cons<T> v[7];
   bool ok[7];
+  if constexpr (D == 2) {
+    v[5] = v[6] = Cons<T>{v4<T>(0, 0, 0, 0), v4<T>(0, 0, 0, 0)};
+    ok[5] = ok[6] = false;
+  }
   if (mid_block(B, tab, cs)) {
     // a coarser leaf half way through its step: its prediction, cells and first ghost layer
     // (amr/assemble.wgsl:60-68)
-    v[0] = mid_cons<T>(P, B, cs, c[0], c[1], c[2]);
-    v[1] = mid_cons<T>(P, B, cs, c[0] - 1, c[1], c[2]);
-    v[2] = mid_cons<T>(P, B, cs, c[0] + 1, c[1], c[2]);
-    v[3] = mid_cons<T>(P, B, cs, c[0], c[1] - 1, c[2]);
-    v[4] = mid_cons<T>(P, B, cs, c[0], c[1] + 1, c[2]);
-    v[5] = mid_cons<T>(P, B, cs, c[0], c[1], c[2] - 1);
-    v[6] = mid_cons<T>(P, B, cs, c[0], c[1], c[2] + 1);
+    v[0] = mid_cons<T, D>(P, B, cs, c[0], c[1], c[2]);
+    v[1] = mid_cons<T, D>(P, B, cs, c[0] - 1, c[1], c[2]);
+    v[2] = mid_cons<T, D>(P, B, cs, c[0] + 1, c[1], c[2]);
+    v[3] = mid_cons<T, D>(P, B, cs, c[0], c[1] - 1, c[2]);
+    v[4] = mid_cons<T, D>(P, B, cs, c[0], c[1] + 1, c[2]);
 #pragma unroll
-    for (int k = 0; k < 7; k++) ok[k] = true;
+    for (int k = 0; k < 5; k++) ok[k] = true;
+    if constexpr (D == 3) {
+      v[5] = mid_cons<T, D>(P, B, cs, c[0], c[1], c[2] - 1);
+      v[6] = mid_cons<T, D>(P, B, cs, c[0], c[1], c[2] + 1);
+      ok[5] = ok[6] = true;
+    }
     return prolong<T, LIM>(P, v, ok, par[0], par[1], par[2]);
   }
-  ok[0] = u_near<T>(tab, U1, U2, cs, c[0], c[1], c[2], v[0]);
-  ok[1] = u_near<T>(tab, U1, U2, cs, c[0] - 1, c[1], c[2], v[1]);
-  ok[2] = u_near<T>(tab, U1, U2, cs, c[0] + 1, c[1], c[2], v[2]);
-  ok[3] = u_near<T>(tab, U1, U2, cs, c[0], c[1] - 1, c[2], v[3]);
-  ok[4] = u_near<T>(tab, U1, U2, cs, c[0], c[1] + 1, c[2], v[4]);
-  ok[5] = u_near<T>(tab, U1, U2, cs, c[0], c[1], c[2] - 1, v[5]);
-  ok[6] = u_near<T>(tab, U1, U2, cs, c[0], c[1], c[2] + 1, v[6]);
+  ok[0] = u_near<T, D>(tab, U1, U2, cs, c[0], c[1], c[2], v[0]);
+  ok[1] = u_near<T, D>(tab, U1, U2, cs, c[0] - 1, c[1], c[2], v[1]);
+  ok[2] = u_near<T, D>(tab, U1, U2, cs, c[0] + 1, c[1], c[2], v[2]);
+  ok[3] = u_near<T, D>(tab, U1, U2, cs, c[0], c[1] - 1, c[2], v[3]);
+  ok[4] = u_near<T, D>(tab, U1, U2, cs, c[0], c[1] + 1, c[2], v[4]);
+  if constexpr (D == 3) {
+    ok[5] = u_near<T, D>(tab, U1, U2, cs, c[0], c[1], c[2] - 1, v[5]);
+    ok[6] = u_near<T, D>(tab, U1, U2, cs, c[0], c[1], c[2] + 1, v[6]);
+  }
   return prolong<T, LIM>(P, v, ok, par[0], par[1], par[2]);
 }
 
 // The assembled primitives of texel (x, y, z) of block `slot`; false for edges and corners.
-template <typename T, int LIM>
+template <typename T, int LIM, int D = 3>
 __device__ __forceinline__ bool amr_texel(const Params<T>& P, const int* __restrict__ tab,
                                           const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, int slot, int x,
                                           int y, int z, Prim<T>& w, const AmrBins& B = AmrBins{}) {
   const GSrc g = ghost_src(tab, slot, x, y, z);
   if (g.kind == 2) return false;
-  const Cons<T> u = g.kind == 0 ? (mid_block(B, tab, g.slot) ? mid_cons<T>(P, B, g.slot, g.x, g.y, g.z)
-                                                             : u_at<T>(tab, U1, U2, g.slot, g.x, g.y, g.z))
-                                : from_coarse<T, LIM>(P, tab, U1, U2, slot, x, y, z, g.slot, B);
+  const Cons<T> u = g.kind == 0 ? (mid_block(B, tab, g.slot) ? mid_cons<T, D>(P, B, g.slot, g.x, g.y, g.z)
+                                                             : u_at<T, D>(tab, U1, U2, g.slot, g.x, g.y, g.z))
+                                : from_coarse<T, LIM, D>(P, tab, U1, U2, slot, x, y, z, g.slot, B);
   w = prim_from_cons(P, u.a, u.b);
   w.vx = bc_vel(w.vx, g.fx, g.sx);
   w.vy = bc_vel(w.vy, g.fy, g.sy);
@@ -285,25 +316,29 @@ __device__ __forceinline__ bool amr_texel(const Params<T>& P, const int* __restr
 }
 
 // ---------------- assemble ----------------
-constexpr int ASM_CHUNKS = (AMR_TEXELS + 255) / 256;   // 32 workgroups of 256 per block
+template <int D>
+constexpr int asm_chunks() { return (AmrGeo<D>::TEXELS + 255) / 256; }   // 32 workgroups of 256 per block (2 in 2D)
+constexpr int ASM_CHUNKS = asm_chunks<3>();
 
-template <typename T, int LIM>
+template <typename T, int LIM, int D = 3>
 __global__ __launch_bounds__(256) void k_amr_assemble(Params<T> P, Levels<T> LV, const int* __restrict__ tab,
                                                       const int* __restrict__ list, const T4<T>* __restrict__ U1,
                                                       const T4<T>* __restrict__ U2, T4<T>* __restrict__ W1,
                                                       T4<T>* __restrict__ W2, unsigned long long* slot, AmrBins B) {
-  const int b = blockIdx.x / ASM_CHUNKS;
-  const int t = (blockIdx.x % ASM_CHUNKS) * 256 + threadIdx.x;
+  constexpr int NCH = asm_chunks<D>();
+  const int b = blockIdx.x / NCH;
+  const int t = (blockIdx.x % NCH) * 256 + threadIdx.x;
   T r = T(0);
-  if (t < AMR_TEXELS) {
-    const int x = t % AMR_PAD - AMR_NG, y = (t / AMR_PAD) % AMR_PAD - AMR_NG, z = t / (AMR_PAD * AMR_PAD) - AMR_NG;
+  if (t < AmrGeo<D>::TEXELS) {
+    const int x = t % AMR_PAD - AMR_NG, y = (t / AMR_PAD) % AMR_PAD - AMR_NG;
+    const int z = D == 3 ? t / (AMR_PAD * AMR_PAD) - AMR_NG : 0;
     const int s = list[b];
     Prim<T> w;
-    if (amr_texel<T, LIM>(P, tab, U1, U2, s, x, y, z, w, B)) {
-      const size_t o = (size_t)s * AMR_TEXELS + t;
+    if (amr_texel<T, LIM, D>(P, tab, U1, U2, s, x, y, z, w, B)) {
+      const size_t o = (size_t)s * AmrGeo<D>::TEXELS + t;
       W1[o] = v4<T>(w.rho, w.vx, w.vy, w.vz);
       W2[o] = v4<T>(w.p, w.cs, w.s, T(0));
-      const bool inside = x >= 0 && y >= 0 && z >= 0 && x < AMR_BS && y < AMR_BS && z < AMR_BS;
+      const bool inside = x >= 0 && y >= 0 && z >= 0 && x < AMR_BS && y < AMR_BS && z < AmrGeo<D>::ZC;
       if (inside && is_leaf(tab, s)) r = rate_of(P, w) / level_dx(P, bt(tab, s, BT_LEVEL));
     }
   }
@@ -311,15 +346,15 @@ __global__ __launch_bounds__(256) void k_amr_assemble(Params<T> P, Levels<T> LV,
 }
 
 // The rates of the listed leaves' cells alone (the fused form's first step)
-template <typename T>
+template <typename T, int D = 3>
 __global__ __launch_bounds__(256) void k_amr_rates(Params<T> P, Levels<T> LV, const int* __restrict__ tab,
                                                    const int* __restrict__ list, const T4<T>* __restrict__ U1,
                                                    const T4<T>* __restrict__ U2, unsigned long long* slot) {
   const int s = list[blockIdx.x];
   const T dx = level_dx(P, bt(tab, s, BT_LEVEL));
   T r = T(0);
-  for (int c = threadIdx.x; c < AMR_CELLS; c += 256) {
-    const size_t o = (size_t)s * AMR_CELLS + c;
+  for (int c = threadIdx.x; c < AmrGeo<D>::CELLS; c += 256) {
+    const size_t o = (size_t)s * AmrGeo<D>::CELLS + c;
     r = tmax(r, rate_of(P, prim_from_cons(P, U1[o], U2[o])) / dx);
   }
   block_max_to(r, slot);
@@ -327,7 +362,7 @@ __global__ __launch_bounds__(256) void k_amr_rates(Params<T> P, Levels<T> LV, co
 
 // Per-block timesteps: each listed leaf's largest rate / dx into out[slot] (the records
 // amr/bins_init.wgsl reads), by the arithmetic of the assembly's rates
-template <typename T>
+template <typename T, int D = 3>
 __global__ __launch_bounds__(256) void k_amr_block_rates(Params<T> P, const int* __restrict__ tab,
                                                          const int* __restrict__ list, const T4<T>* __restrict__ U1,
                                                          const T4<T>* __restrict__ U2, T* __restrict__ out) {
@@ -335,8 +370,8 @@ __global__ __launch_bounds__(256) void k_amr_block_rates(Params<T> P, const int*
   const int s = list[blockIdx.x];
   const T dx = level_dx(P, bt(tab, s, BT_LEVEL));
   T r = T(0);
-  for (int c = threadIdx.x; c < AMR_CELLS; c += 256) {
-    const size_t o = (size_t)s * AMR_CELLS + c;
+  for (int c = threadIdx.x; c < AmrGeo<D>::CELLS; c += 256) {
+    const size_t o = (size_t)s * AmrGeo<D>::CELLS + c;
     r = tmax(r, rate_of(P, prim_from_cons(P, U1[o], U2[o])) / dx);
   }
   r = wave_max(r);
@@ -376,20 +411,25 @@ __global__ void k_amr_clock_cycle(Params<T> P, T* clk, unsigned long long* slot,
 }
 
 // the commit: the listed blocks' updated cells back into the state
-template <typename T>
+template <typename T, int D = 3>
 __global__ __launch_bounds__(256) void k_amr_commit(const int* __restrict__ list, const T4<T>* __restrict__ O1,
                                                     const T4<T>* __restrict__ O2, T4<T>* __restrict__ U1,
                                                     T4<T>* __restrict__ U2) {
-  const int s = list[blockIdx.x / (AMR_CELLS / 256)];
-  const size_t o = (size_t)s * AMR_CELLS + (blockIdx.x % (AMR_CELLS / 256)) * 256 + threadIdx.x;
+  constexpr int C = AmrGeo<D>::CELLS;
+  const int s = list[blockIdx.x / (C / 256)];
+  const size_t o = (size_t)s * C + (blockIdx.x % (C / 256)) * 256 + threadIdx.x;
   U1[o] = O1[o];
   U2[o] = O2[o];
 }
 
 // ---------------- predict ----------------
 constexpr int PRED_SIDE = AMR_BS + 2;
-constexpr int PRED_TEXELS = PRED_SIDE * PRED_SIDE * PRED_SIDE;
-constexpr int PRED_CHUNKS = (PRED_TEXELS + 255) / 256;   // 23
+template <int D>
+constexpr int pred_texels() { return D == 3 ? PRED_SIDE * PRED_SIDE * PRED_SIDE : PRED_SIDE * PRED_SIDE; }
+template <int D>
+constexpr int pred_chunks() { return (pred_texels<D>() + 255) / 256; }   // 23 (2 in 2D)
+constexpr int PRED_TEXELS = pred_texels<3>();
+constexpr int PRED_CHUNKS = pred_chunks<3>();
 
 template <typename T>
 __device__ __forceinline__ Prim<T> load_w(const T4<T>* __restrict__ W1, const T4<T>* __restrict__ W2, size_t o) {
@@ -402,17 +442,18 @@ __device__ __forceinline__ Prim<T> load_w(const T4<T>* __restrict__ W1, const T4
 
 // predict_cell (amr/predict.wgsl:90-181) on the atlas of block `slot` at texel (x, y, z); a zero
 // in mask drops that axis's slopes (ghosts next to a coarser block whose transverse neighbour
-// is an unassembled edge texel)
-template <typename T, int LIM, bool ISEN>
+// is an unassembled edge texel). In 2D the cell stands for its own z neighbours (:103-108).
+template <typename T, int LIM, bool ISEN, int D = 3>
 __device__ __forceinline__ Half<T> predict_texel(const Params<T>& P, const T4<T>* __restrict__ W1,
                                                  const T4<T>* __restrict__ W2, int slot, int x, int y, int z, T mx,
                                                  T my, T mz, T dt) {
-  const size_t base = (size_t)slot * AMR_TEXELS;
-  const int c = tex(x, y, z);
+  const size_t base = (size_t)slot * AmrGeo<D>::TEXELS;
+  const int c = texd<D>(x, y, z);
   const Prim<T> q = load_w<T>(W1, W2, base + c);
   const Prim<T> xm = load_w<T>(W1, W2, base + c - 1), xp = load_w<T>(W1, W2, base + c + 1);
   const Prim<T> ym = load_w<T>(W1, W2, base + c - AMR_PAD), yp = load_w<T>(W1, W2, base + c + AMR_PAD);
-  const Prim<T> zm = load_w<T>(W1, W2, base + c - AMR_PAD * AMR_PAD), zp = load_w<T>(W1, W2, base + c + AMR_PAD * AMR_PAD);
+  const Prim<T> zm = D == 3 ? load_w<T>(W1, W2, base + c - AMR_PAD * AMR_PAD) : q;
+  const Prim<T> zp = D == 3 ? load_w<T>(W1, W2, base + c + AMR_PAD * AMR_PAD) : q;
   AxisSlope<T> sl[3];
   sl[0] = axis_slope<T, LIM, ISEN>(P, xm, q, xp, nullptr, nullptr);
   sl[1] = axis_slope<T, LIM, ISEN>(P, ym, q, yp, nullptr, nullptr);
@@ -428,15 +469,17 @@ __device__ __forceinline__ Half<T> predict_texel(const Params<T>& P, const T4<T>
   return hancock(P, q, sl, conv, jump, dt);
 }
 
-template <typename T, int LIM, bool ISEN>
+template <typename T, int LIM, bool ISEN, int D = 3>
 __global__ __launch_bounds__(256) void k_amr_predict(Params<T> P, Levels<T> LV, const int* __restrict__ tab,
                                                      const int* __restrict__ list, const T4<T>* __restrict__ W1,
                                                      const T4<T>* __restrict__ W2, T4<T>* __restrict__ PA,
                                                      T4<T>* __restrict__ PB, const T* __restrict__ clk, AmrBins B) {
-  const int b = blockIdx.x / PRED_CHUNKS;
-  const int t = (blockIdx.x % PRED_CHUNKS) * 256 + threadIdx.x;
-  if (t >= PRED_TEXELS) return;
-  const int x = t % PRED_SIDE - 1, y = (t / PRED_SIDE) % PRED_SIDE - 1, z = t / (PRED_SIDE * PRED_SIDE) - 1;
+  constexpr int NCH = pred_chunks<D>();
+  const int b = blockIdx.x / NCH;
+  const int t = (blockIdx.x % NCH) * 256 + threadIdx.x;
+  if (t >= pred_texels<D>()) return;
+  const int x = t % PRED_SIDE - 1, y = (t / PRED_SIDE) % PRED_SIDE - 1;
+  const int z = D == 3 ? t / (PRED_SIDE * PRED_SIDE) - 1 : 0;
   const int s = list[b];
   const GSrc g = ghost_src(tab, s, x, y, z);
   if (g.kind == 2) return;
@@ -451,14 +494,14 @@ __global__ __launch_bounds__(256) void k_amr_predict(Params<T> P, Levels<T> LV, 
   const T mx = (!at && (x == 0 || x == AMR_BS - 1)) ? T(0) : T(1);
   const T my = (!at && (y == 0 || y == AMR_BS - 1)) ? T(0) : T(1);
   const T mz = (!at && (z == 0 || z == AMR_BS - 1)) ? T(0) : T(1);
-  Half<T> h = at ? predict_texel<T, LIM, ISEN>(Pl, W1, W2, src, g.x, g.y, g.z, T(1), T(1), T(1), dt)
-                 : predict_texel<T, LIM, ISEN>(Pl, W1, W2, src, x, y, z, mx, my, mz, dt);
+  Half<T> h = at ? predict_texel<T, LIM, ISEN, D>(Pl, W1, W2, src, g.x, g.y, g.z, T(1), T(1), T(1), dt)
+                 : predict_texel<T, LIM, ISEN, D>(Pl, W1, W2, src, x, y, z, mx, my, mz, dt);
   if (at) {
     h.vx = bc_vel(h.vx, g.fx, g.sx);
     h.vy = bc_vel(h.vy, g.fy, g.sy);
     h.vz = bc_vel(h.vz, g.fz, g.sz);
   }
-  const size_t o = (size_t)s * AMR_TEXELS + tex(x, y, z);
+  const size_t o = (size_t)s * AmrGeo<D>::TEXELS + texd<D>(x, y, z);
   PA[o] = v4<T>(h.rho, h.vx, h.vy, h.vz);
   PB[o] = v4<T>(h.p, h.s, h.shock, h.fo);
 }
@@ -475,11 +518,11 @@ __device__ __forceinline__ Half<T> load_half_at(const T4<T>* __restrict__ PA, co
 
 // flux_between on block s's atlas (hydro/flux.wgsl:254-347 with stored ghosts, amr/update.wgsl:
 // 50-58): the face between texel (x, y, z) and the next one along axis
-template <typename T, int LIM, int RS, bool ISEN>
+template <typename T, int LIM, int RS, bool ISEN, int D = 3>
 __device__ __forceinline__ Flux<T> amr_flux(const Params<T>& P, const T4<T>* __restrict__ W1,
                                             const T4<T>* __restrict__ W2, const T4<T>* __restrict__ PA,
                                             const T4<T>* __restrict__ PB, int s, int x, int y, int z, int axis) {
-  const size_t o = (size_t)s * AMR_TEXELS + tex(x, y, z);
+  const size_t o = (size_t)s * AmrGeo<D>::TEXELS + texd<D>(x, y, z);
   const int e = axis == 0 ? 1 : (axis == 1 ? AMR_PAD : AMR_PAD * AMR_PAD);
   const Prim<T> wll = load_w<T>(W1, W2, o - e), wl = load_w<T>(W1, W2, o);
   const Prim<T> wr = load_w<T>(W1, W2, o + e), wrr = load_w<T>(W1, W2, o + 2 * e);
@@ -501,8 +544,9 @@ __device__ __forceinline__ void two_sum(T a, T b, T& s, T& e) {
 
 // The face of cell (x, y, z) of block s on side k (axis k / 2, upper if k odd), whose neighbour
 // across it is refined: the mean of the four fine fluxes through it (match_faces,
-// amr/update.wgsl:207-290), summed in a fixed order, with TwoSum when COMP.
-template <typename T, int LIM, int RS, bool ISEN, bool COMP>
+// amr/update.wgsl:207-290; two in 2D, whose faces are edges: :246-251), summed in a fixed order,
+// with TwoSum when COMP.
+template <typename T, int LIM, int RS, bool ISEN, bool COMP, int D = 3>
 __device__ __forceinline__ Flux<T> fine_face(const Params<T>& P, const int* __restrict__ tab,
                                              const T4<T>* __restrict__ W1, const T4<T>* __restrict__ W2,
                                              const T4<T>* __restrict__ PA, const T4<T>* __restrict__ PB, int np, int k,
@@ -515,6 +559,8 @@ __device__ __forceinline__ Flux<T> fine_face(const Params<T>& P, const int* __re
   const int an = upper ? AMR_BS - 1 : -1;
   if (axis == 0) ax = an; else if (axis == 1) ay = an; else az = an;
   const int t1 = axis == 2 ? 0 : axis + 1, t2 = axis == 0 ? 2 : axis - 1;   // (axis + 1) % 3, (axis + 2) % 3
+  // fine faces along t1 and t2: 2 each, or 1 along z in 2D
+  const int n1 = (D == 2 && t1 == 2) ? 1 : 2, n2 = (D == 2 && t2 == 2) ? 1 : 2;
   T sum[8], err[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) { sum[q] = T(0); err[q] = T(0); }
@@ -522,10 +568,11 @@ __device__ __forceinline__ Flux<T> fine_face(const Params<T>& P, const int* __re
   for (int j = 0; j < 2; j++)
 #pragma unroll
     for (int i = 0; i < 2; i++) {
+      if (j >= n2 || i >= n1) continue;
       const int fx = ax + (t1 == 0 ? i : 0) + (t2 == 0 ? j : 0);
       const int fy = ay + (t1 == 1 ? i : 0) + (t2 == 1 ? j : 0);
       const int fz = az + (t1 == 2 ? i : 0) + (t2 == 2 ? j : 0);
-      const Flux<T> f = amr_flux<T, LIM, RS, ISEN>(P, W1, W2, PA, PB, ch, fx, fy, fz, axis);
+      const Flux<T> f = amr_flux<T, LIM, RS, ISEN, D>(P, W1, W2, PA, PB, ch, fx, fy, fz, axis);
       const T fv[8] = {f.m, f.mx, f.my, f.mz, f.e, f.ei, f.s, f.vn};
 #pragma unroll
       for (int q = 0; q < 8; q++) {
@@ -538,32 +585,42 @@ __device__ __forceinline__ Flux<T> fine_face(const Params<T>& P, const int* __re
         }
       }
     }
+  const T w = D == 3 ? T(0.25) : T(0.5);
   Flux<T> F;
-  F.m = (sum[0] + err[0]) * T(0.25); F.mx = (sum[1] + err[1]) * T(0.25);
-  F.my = (sum[2] + err[2]) * T(0.25); F.mz = (sum[3] + err[3]) * T(0.25);
-  F.e = (sum[4] + err[4]) * T(0.25); F.ei = (sum[5] + err[5]) * T(0.25);
-  F.s = (sum[6] + err[6]) * T(0.25); F.vn = (sum[7] + err[7]) * T(0.25);
+  F.m = (sum[0] + err[0]) * w; F.mx = (sum[1] + err[1]) * w;
+  F.my = (sum[2] + err[2]) * w; F.mz = (sum[3] + err[3]) * w;
+  F.e = (sum[4] + err[4]) * w; F.ei = (sum[5] + err[5]) * w;
+  F.s = (sum[6] + err[6]) * w; F.vn = (sum[7] + err[7]) * w;
   return F;
 }
 
 // The block update: workgroup 8 x 8 x 4 cells (a quarter-plane slab; 16 per block), every face
-// of its box solved once into LDS (896 faces, 28 KB), then each cell's update.
+// of its box solved once into LDS (896 faces, 28 KB), then each cell's update. In 2D one
+// workgroup of 16 x 16 cells per block (544 faces, 17 KB), no z faces.
 constexpr int UPD_WX = 8, UPD_WY = 8, UPD_WZ = 4, UPD_TILES = 16;
+template <int D>
+struct UpdTile {
+  static constexpr int WX = D == 3 ? UPD_WX : AMR_BS, WY = D == 3 ? UPD_WY : AMR_BS, WZ = D == 3 ? UPD_WZ : 1;
+  static constexpr int TILES = D == 3 ? UPD_TILES : 1;
+  static constexpr int NFX = (WX + 1) * WY * WZ, NFY = WX * (WY + 1) * WZ, NFZ = D == 3 ? WX * WY * (WZ + 1) : 0;
+  static constexpr int NF = NFX + NFY + NFZ, NT = WX * WY * WZ;
+};
 
 // DEBUG (DEBUG_FLUX, amr/update.wgsl:324): instead of the update, each cell writes the mass
 // flux of its six faces, (x lo, x hi, y lo, y hi) to O1 and (z lo, z hi, 0, 0) to O2.
-template <typename T, int LIM, int RS, bool ISEN, bool MATCH, bool COMP, bool DEBUG = false>
-__global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
+template <typename T, int LIM, int RS, bool ISEN, bool MATCH, bool COMP, bool DEBUG = false, int D = 3>
+__global__ __launch_bounds__(UpdTile<D>::NT) void k_amr_update(
     Params<T> P, Levels<T> LV, const int* __restrict__ tab, const int* __restrict__ list,
     const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2, const T4<T>* __restrict__ W1,
     const T4<T>* __restrict__ W2, const T4<T>* __restrict__ PA, const T4<T>* __restrict__ PB, T4<T>* __restrict__ O1,
     T4<T>* __restrict__ O2, const T* __restrict__ clk, AmrBins B) {
-  constexpr int WX = UPD_WX, WY = UPD_WY, WZ = UPD_WZ;
-  constexpr int NFX = (WX + 1) * WY * WZ, NFY = WX * (WY + 1) * WZ, NFZ = WX * WY * (WZ + 1);
-  constexpr int NF = NFX + NFY + NFZ, NT = WX * WY * WZ;
+  using UT = UpdTile<D>;
+  constexpr int WX = UT::WX, WY = UT::WY, WZ = UT::WZ;
+  constexpr int NFX = UT::NFX, NFY = UT::NFY;
+  constexpr int NF = UT::NF, NT = UT::NT;
   __shared__ T tf[8 * NF];
-  const int s = list[blockIdx.x / UPD_TILES];
-  const int tile = blockIdx.x % UPD_TILES;
+  const int s = list[blockIdx.x / UT::TILES];
+  const int tile = blockIdx.x % UT::TILES;
   const int x0 = (tile & 1) * WX, y0 = ((tile >> 1) & 1) * WY, z0 = (tile >> 2) * WZ;
   const int li = threadIdx.x;
   // per-block timesteps (amr/update.wgsl:173-186, 338-340): the block's bin and its own step; a
@@ -583,8 +640,8 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
       const int g = f - NFX - NFY;
       axis = 2; fx = g % WX; fy = (g / WX) % WY; fz = g / (WX * WY);
     }
-    const Flux<T> F = amr_flux<T, LIM, RS, ISEN>(P, W1, W2, PA, PB, s, x0 + fx - (axis == 0), y0 + fy - (axis == 1),
-                                                 z0 + fz - (axis == 2), axis);
+    const Flux<T> F = amr_flux<T, LIM, RS, ISEN, D>(P, W1, W2, PA, PB, s, x0 + fx - (axis == 0),
+                                                    y0 + fy - (axis == 1), z0 + fz - (axis == 2), axis);
     put_flux(tf, NF, f, F);
   }
   __syncthreads();
@@ -597,13 +654,15 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
     xh = get_flux(tf, NF, tx + 1 + (WX + 1) * (ty + WY * tz));
     yl = get_flux(tf, NF, NFX + tx + WX * (ty + (WY + 1) * tz));
     yh = get_flux(tf, NF, NFX + tx + WX * (ty + 1 + (WY + 1) * tz));
-    zl = get_flux(tf, NF, NFX + NFY + tx + WX * (ty + WY * tz));
-    zh = get_flux(tf, NF, NFX + NFY + tx + WX * (ty + WY * (tz + 1)));
+    if constexpr (D == 3) {
+      zl = get_flux(tf, NF, NFX + NFY + tx + WX * (ty + WY * tz));
+      zh = get_flux(tf, NF, NFX + NFY + tx + WX * (ty + WY * (tz + 1)));
+    }
   }
   bool touched = false;
   if constexpr (MATCH) {
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
+    for (int k = 0; k < 2 * D; k++) {
       const int axis = k >> 1, hi = k & 1;
       const int l = axis == 0 ? x : (axis == 1 ? y : z);
       if (l != (hi ? AMR_BS - 1 : 0)) continue;
@@ -624,7 +683,7 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
         if (part == T(0)) {
           F = zero_flux<T>();
         } else {
-          F = fine_face<T, LIM, RS, ISEN, COMP>(P, tab, W1, W2, PA, PB, np, k, x, y, z);
+          F = fine_face<T, LIM, RS, ISEN, COMP, D>(P, tab, W1, W2, PA, PB, np, k, x, y, z);
           if (B.k) {
             F.m = F.m * part; F.mx = F.mx * part; F.my = F.my * part; F.mz = F.mz * part;
             F.e = F.e * part; F.ei = F.ei * part; F.s = F.s * part; F.vn = F.vn * part;
@@ -634,8 +693,8 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
       } else if (B.k && np != s && bins_bin(B, np) < gbin) {
         // a faster leaf of the same level: half of its flux through the face, as it computes it
         const int an = hi ? -1 : AMR_BS - 1;
-        F = amr_flux<T, LIM, RS, ISEN>(P, W1, W2, PA, PB, np, axis == 0 ? an : x, axis == 1 ? an : y,
-                                       axis == 2 ? an : z, axis);
+        F = amr_flux<T, LIM, RS, ISEN, D>(P, W1, W2, PA, PB, np, axis == 0 ? an : x, axis == 1 ? an : y,
+                                          axis == 2 ? an : z, axis);
         F.m = F.m * T(0.5); F.mx = F.mx * T(0.5); F.my = F.my * T(0.5); F.mz = F.mz * T(0.5);
         F.e = F.e * T(0.5); F.ei = F.ei * T(0.5); F.s = F.s * T(0.5); F.vn = F.vn * T(0.5);
         touched = true;
@@ -653,12 +712,12 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
     }
   }
   if constexpr (DEBUG) {
-    const size_t o = (size_t)s * AMR_CELLS + cel(x, y, z);
+    const size_t o = (size_t)s * AmrGeo<D>::CELLS + celd<D>(x, y, z);
     O1[o] = v4<T>(xl.m, xh.m, yl.m, yh.m);
     O2[o] = v4<T>(zl.m, zh.m, T(0), T(0));
     return;
   }
-  const size_t o = (size_t)s * AMR_CELLS + cel(x, y, z);
+  const size_t o = (size_t)s * AmrGeo<D>::CELLS + celd<D>(x, y, z);
   if (deposit && !touched) {
     // a cell on no face shared with a faster neighbour is copied as it is (amr/update.wgsl)
     O1[o] = U1[o];
@@ -671,7 +730,7 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
   Pl.dx = dx;
   const T dt = B.k ? clk[0] * T(1 << gbin) : clk[0];
   T4<T> o1, o2;
-  advance(Pl, U1[o], U2[o], net, PB[(size_t)s * AMR_TEXELS + tex(x, y, z)].x, dt / dx, o1, o2);
+  advance(Pl, U1[o], U2[o], net, PB[(size_t)s * AmrGeo<D>::TEXELS + texd<D>(x, y, z)].x, dt / dx, o1, o2);
   O1[o] = o1;
   O2[o] = o2;
 }
@@ -679,7 +738,7 @@ __global__ __launch_bounds__(UPD_WX* UPD_WY* UPD_WZ) void k_amr_update(
 // ---------------- diagnostics (FP64, volume-weighted over leaves) ----------------
 // The 24 channels of amr/diagnostics.wgsl:13-117: one workgroup per listed leaf (its cells of
 // volume h^3 at blk_pos, amr/grid.wgsl:172-177), records folded by k_diag_final in list order.
-template <typename T>
+template <typename T, int D = 3>
 __global__ __launch_bounds__(256) void k_amr_diag(Params<T> P, DiagGeo G, const int* __restrict__ tab,
                                                   const int* __restrict__ list, double dx0,
                                                   const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2,
@@ -691,9 +750,9 @@ __global__ __launch_bounds__(256) void k_amr_diag(Params<T> P, DiagGeo G, const 
   const int bx = bt(tab, s, BT_X) * AMR_BS, by = bt(tab, s, BT_X + 1) * AMR_BS, bz = bt(tab, s, BT_X + 2) * AMR_BS;
   double r[DIAG_CH];
   diag_init(r);
-  for (int c = threadIdx.x; c < AMR_CELLS; c += 256) {
+  for (int c = threadIdx.x; c < AmrGeo<D>::CELLS; c += 256) {
     const int i = c % AMR_BS, j = (c / AMR_BS) % AMR_BS, k = c / (AMR_BS * AMR_BS);
-    const size_t o = (size_t)s * AMR_CELLS + c;
+    const size_t o = (size_t)s * AmrGeo<D>::CELLS + c;
     const double x = (bx + i + 0.5) * h + G.org[0], y = (by + j + 0.5) * h + G.org[1];
     const double z = G.flat ? 0.0 : (bz + k + 0.5) * h + G.org[2];
     diag_cell<T>(G, U1[o], U2[o], dv, x, y, z, r);
@@ -715,59 +774,66 @@ __device__ __forceinline__ T ke_of(const Params<T>& P, const Cons<T>& u) {
   return tdiv(T(0.5) * ((u.a.y * u.a.y + u.a.z * u.a.z) + u.a.w * u.a.w), tmax(u.a.x, P.rho_floor));
 }
 
-template <typename T, int LIM>
+template <typename T, int LIM, int D = 3>
 __global__ __launch_bounds__(256) void k_amr_regrid(Params<T> P, const int* __restrict__ told,
                                                     const int4* __restrict__ map, const T4<T>* __restrict__ U1,
                                                     const T4<T>* __restrict__ U2, T4<T>* __restrict__ N1a,
                                                     T4<T>* __restrict__ N2a, T4<T>* __restrict__ N1b,
                                                     T4<T>* __restrict__ N2b) {
-  const int4 m = map[blockIdx.x / (AMR_CELLS / 256)];
-  const int c = (blockIdx.x % (AMR_CELLS / 256)) * 256 + threadIdx.x;
-  const int x = c % AMR_BS, y = (c / AMR_BS) % AMR_BS, z = c / (AMR_BS * AMR_BS);
+  constexpr int C = AmrGeo<D>::CELLS;
+  const int4 m = map[blockIdx.x / (C / 256)];
+  const int c = (blockIdx.x % (C / 256)) * 256 + threadIdx.x;
+  const int x = c % AMR_BS, y = (c / AMR_BS) % AMR_BS, z = D == 3 ? c / (AMR_BS * AMR_BS) : 0;
   Cons<T> u;
   if (m.y == 1) {
-    u = u_leaf<T>(U1, U2, m.z, x, y, z);
+    u = u_leaf<T, D>(U1, U2, m.z, x, y, z);
   } else if (m.y == 3) {
-    u = u_at<T>(told, U1, U2, m.z, x, y, z);
+    u = u_at<T, D>(told, U1, U2, m.z, x, y, z);
   } else {
     const int lc[3] = {(x >> 1) + (AMR_BS / 2) * (m.w & 1), (y >> 1) + (AMR_BS / 2) * ((m.w >> 1) & 1),
-                       (z >> 1) + (AMR_BS / 2) * ((m.w >> 2) & 1)};
+                       D == 3 ? (z >> 1) + (AMR_BS / 2) * ((m.w >> 2) & 1) : 0};
     Cons<T> v[7];
     bool ok[7];
 #pragma unroll
     for (int k = 0; k < 7; k++) {
       int at[3] = {lc[0], lc[1], lc[2]};
       if (k > 0) at[(k - 1) / 2] += (k & 1) == 0 ? 1 : -1;
-      ok[k] = u_near<T>(told, U1, U2, m.z, at[0], at[1], at[2], v[k]);
+      if (D == 2 && k >= 5) {
+        // 2D: no z neighbours (regrid.wgsl:58-62)
+        v[k] = Cons<T>{v4<T>(0, 0, 0, 0), v4<T>(0, 0, 0, 0)};
+        ok[k] = false;
+      } else {
+        ok[k] = u_near<T, D>(told, U1, U2, m.z, at[0], at[1], at[2], v[k]);
+      }
     }
     u = prolong<T, LIM>(P, v, ok, x & 1, y & 1, z & 1);
     T ke = T(0);
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < (D == 3 ? 2 : 1); k++)
       for (int j = 0; j < 2; j++)
         for (int i = 0; i < 2; i++) ke = ke + ke_of(P, prolong<T, LIM>(P, v, ok, i, j, k));
-    ke = ke * T(0.125);
+    ke = ke * (D == 3 ? T(0.125) : T(0.25));
     u.b.x = ke_of(P, u) + u.b.y + ((v[0].b.x - v[0].b.y) - ke);
   }
-  const size_t o = (size_t)m.x * AMR_CELLS + c;
+  const size_t o = (size_t)m.x * C + c;
   N1a[o] = u.a; N2a[o] = u.b;
   N1b[o] = u.a; N2b[o] = u.b;
 }
 
 // The refinement indicator of the listed leaves: the largest relative density jump
 // |rho_a - rho_b| / min(rho_a, rho_b) between face-adjacent cells of the block's interior.
-template <typename T>
+template <typename T, int D = 3>
 __global__ __launch_bounds__(256) void k_amr_indicator(const int* __restrict__ list, const T4<T>* __restrict__ U1,
                                                        T* __restrict__ out) {
   __shared__ T red[4];
   const int s = list[blockIdx.x];
   T r = T(0);
-  for (int c = threadIdx.x; c < AMR_CELLS; c += 256) {
+  for (int c = threadIdx.x; c < AmrGeo<D>::CELLS; c += 256) {
     const int x = c % AMR_BS, y = (c / AMR_BS) % AMR_BS, z = c / (AMR_BS * AMR_BS);
-    const size_t o = (size_t)s * AMR_CELLS + c;
+    const size_t o = (size_t)s * AmrGeo<D>::CELLS + c;
     const T a = U1[o].x;
     if (x + 1 < AMR_BS) { const T b = U1[o + 1].x; r = tmax(r, tdiv(tabs(a - b), tmin(a, b))); }
     if (y + 1 < AMR_BS) { const T b = U1[o + AMR_BS].x; r = tmax(r, tdiv(tabs(a - b), tmin(a, b))); }
-    if (z + 1 < AMR_BS) { const T b = U1[o + AMR_BS * AMR_BS].x; r = tmax(r, tdiv(tabs(a - b), tmin(a, b))); }
+    if (D == 3 && z + 1 < AMR_BS) { const T b = U1[o + AMR_BS * AMR_BS].x; r = tmax(r, tdiv(tabs(a - b), tmin(a, b))); }
   }
   r = wave_max(r);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r;
@@ -775,16 +841,17 @@ __global__ __launch_bounds__(256) void k_amr_indicator(const int* __restrict__ l
   if (threadIdx.x == 0) out[s] = tmax(tmax(red[0], red[1]), tmax(red[2], red[3]));
 }
 
-template <typename T, int LIM, int RS, bool ISEN>
+template <typename T, int LIM, int RS, bool ISEN, int D = 3>
 struct AmrImpl {
+  using G = AmrGeo<D>;
   static const Params<T>& P(const AmrLaunch& L) { return *static_cast<const Params<T>*>(L.params); }
   static const Levels<T>& LV(const AmrLaunch& L) { return *static_cast<const Levels<T>*>(L.levels); }
 
   static void assemble(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2, void* w1,
                        void* w2, unsigned long long* slot) {
     if (count <= 0) return;
-    hipLaunchKernelGGL((k_amr_assemble<T, LIM>), dim3(count * ASM_CHUNKS), dim3(256), 0, L.stream, P(L), LV(L), L.tab,
-                       list, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)w1, (T4<T>*)w2, slot, L.bins);
+    hipLaunchKernelGGL((k_amr_assemble<T, LIM, D>), dim3(count * asm_chunks<D>()), dim3(256), 0, L.stream, P(L),
+                       LV(L), L.tab, list, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)w1, (T4<T>*)w2, slot, L.bins);
   }
   static void clock(const AmrLaunch& L, void* clk, unsigned long long* slot) {
     Params<T> P1 = P(L);
@@ -794,18 +861,19 @@ struct AmrImpl {
   static void predict(const AmrLaunch& L, const int* list, int count, const void* w1, const void* w2, void* pa,
                       void* pb, const void* clk) {
     if (count <= 0) return;
-    hipLaunchKernelGGL((k_amr_predict<T, LIM, ISEN>), dim3(count * PRED_CHUNKS), dim3(256), 0, L.stream, P(L), LV(L),
-                       L.tab, list, (const T4<T>*)w1, (const T4<T>*)w2, (T4<T>*)pa, (T4<T>*)pb, (const T*)clk, L.bins);
+    hipLaunchKernelGGL((k_amr_predict<T, LIM, ISEN, D>), dim3(count * pred_chunks<D>()), dim3(256), 0, L.stream,
+                       P(L), LV(L), L.tab, list, (const T4<T>*)w1, (const T4<T>*)w2, (T4<T>*)pa, (T4<T>*)pb,
+                       (const T*)clk, L.bins);
   }
   static void update(const AmrLaunch& L, const int* list, int count, bool match, bool comp, const void* u1,
                      const void* u2, const void* w1, const void* w2, const void* pa, const void* pb, void* o1, void* o2,
                      const void* clk, bool debug) {
     if (count <= 0) return;
-#define SHX_UPD(M, C, D)                                                                                               \
-  hipLaunchKernelGGL((k_amr_update<T, LIM, RS, ISEN, M, C, D>), dim3(count * UPD_TILES),                              \
-                     dim3(UPD_WX * UPD_WY * UPD_WZ), 0, L.stream, P(L), LV(L), L.tab, list, (const T4<T>*)u1,           \
-                     (const T4<T>*)u2, (const T4<T>*)w1, (const T4<T>*)w2, (const T4<T>*)pa, (const T4<T>*)pb,         \
-                     (T4<T>*)o1, (T4<T>*)o2, (const T*)clk, L.bins)
+#define SHX_UPD(M, C, DB)                                                                                              \
+  hipLaunchKernelGGL((k_amr_update<T, LIM, RS, ISEN, M, C, DB, D>), dim3(count * UpdTile<D>::TILES),                   \
+                     dim3(UpdTile<D>::NT), 0, L.stream, P(L), LV(L), L.tab, list, (const T4<T>*)u1, (const T4<T>*)u2,  \
+                     (const T4<T>*)w1, (const T4<T>*)w2, (const T4<T>*)pa, (const T4<T>*)pb, (T4<T>*)o1, (T4<T>*)o2,   \
+                     (const T*)clk, L.bins)
     if (debug) {
       if (!match) SHX_UPD(false, false, true);
       else if (comp) SHX_UPD(true, true, true);
@@ -820,13 +888,13 @@ struct AmrImpl {
   static void rates(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2,
                     unsigned long long* slot) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(k_amr_rates<T>, dim3(count), dim3(256), 0, L.stream, P(L), LV(L), L.tab, list,
+    hipLaunchKernelGGL((k_amr_rates<T, D>), dim3(count), dim3(256), 0, L.stream, P(L), LV(L), L.tab, list,
                        (const T4<T>*)u1, (const T4<T>*)u2, slot);
   }
-  static void diag(const AmrLaunch& L, const DiagGeo& G, const int* list, int count, const void* u1, const void* u2,
+  static void diag(const AmrLaunch& L, const DiagGeo& Gd, const int* list, int count, const void* u1, const void* u2,
                    double* part, double* out) {
     if (count > 0)
-      hipLaunchKernelGGL(k_amr_diag<T>, dim3(count), dim3(256), 0, L.stream, P(L), G, L.tab, list, L.dx0,
+      hipLaunchKernelGGL((k_amr_diag<T, D>), dim3(count), dim3(256), 0, L.stream, P(L), Gd, L.tab, list, L.dx0,
                          (const T4<T>*)u1, (const T4<T>*)u2, part);
     hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(32 * DIAG_CH), 0, L.stream, (const double*)part, count, out);
   }
@@ -837,17 +905,17 @@ struct AmrImpl {
   static void regrid(const AmrLaunch& Lold, const int* map, int count, const void* u1, const void* u2, void* n1a,
                      void* n2a, void* n1b, void* n2b) {
     if (count <= 0) return;
-    hipLaunchKernelGGL((k_amr_regrid<T, LIM>), dim3(count * (AMR_CELLS / 256)), dim3(256), 0, Lold.stream, P(Lold),
+    hipLaunchKernelGGL((k_amr_regrid<T, LIM, D>), dim3(count * (G::CELLS / 256)), dim3(256), 0, Lold.stream, P(Lold),
                        Lold.tab, (const int4*)map, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)n1a, (T4<T>*)n2a,
                        (T4<T>*)n1b, (T4<T>*)n2b);
   }
   static void indicator(const AmrLaunch& L, const int* list, int count, const void* u1, void* out) {
     if (count <= 0) return;
-    hipLaunchKernelGGL((k_amr_indicator<T>), dim3(count), dim3(256), 0, L.stream, list, (const T4<T>*)u1, (T*)out);
+    hipLaunchKernelGGL((k_amr_indicator<T, D>), dim3(count), dim3(256), 0, L.stream, list, (const T4<T>*)u1, (T*)out);
   }
   static void block_rates(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2, void* out) {
     if (count <= 0) return;
-    hipLaunchKernelGGL((k_amr_block_rates<T>), dim3(count), dim3(256), 0, L.stream, P(L), L.tab, list,
+    hipLaunchKernelGGL((k_amr_block_rates<T, D>), dim3(count), dim3(256), 0, L.stream, P(L), L.tab, list,
                        (const T4<T>*)u1, (const T4<T>*)u2, (T*)out);
   }
   static void bank(const AmrLaunch& L, void* clk) {
@@ -859,7 +927,7 @@ struct AmrImpl {
   static void commit(const AmrLaunch& L, const int* list, int count, const void* o1, const void* o2, void* u1,
                      void* u2) {
     if (count <= 0) return;
-    hipLaunchKernelGGL((k_amr_commit<T>), dim3(count * (AMR_CELLS / 256)), dim3(256), 0, L.stream, list,
+    hipLaunchKernelGGL((k_amr_commit<T, D>), dim3(count * (G::CELLS / 256)), dim3(256), 0, L.stream, list,
                        (const T4<T>*)o1, (const T4<T>*)o2, (T4<T>*)u1, (T4<T>*)u2);
   }
   static AmrOps ops() {
@@ -870,8 +938,14 @@ struct AmrImpl {
     o.bank = bank;
     o.clock_cycle = clock_cycle;
     o.commit = commit;
-    o.shell = shell;
-    o.fused = fused;
+    if constexpr (D == 3) {
+      // the fused form (the ghost shells and the persistent march) is built for 3D blocks
+      o.shell = shell;
+      o.fused = fused;
+    } else {
+      o.shell = nullptr;
+      o.fused = nullptr;
+    }
     o.assemble = assemble;
     o.clock = clock;
     o.predict = predict;

@@ -27,6 +27,9 @@ struct Range {
 struct shx_amr {
   shx_config cfg;
   shx_amr_layout lay;
+  int ndim = 3;
+  int cells = AMR_CELLS;     // per block: 16^ndim cells, 20^ndim atlas texels
+  int texels = AMR_TEXELS;
   int es = 4;
   int nblocks = 0;
   int nlevels = 0;
@@ -111,6 +114,9 @@ uint64_t morton_of(const int* b, int maxlev) {
 int build_table(shx_amr* s) {
   const int n = s->nblocks;
   const int* root = s->lay.root;
+  // 2D: a quadtree (four children, octant x | y << 1); each z face names the block itself
+  // (tree.ts writeRow :662-666: the single layer is its own neighbour in z)
+  const int nchild = 1 << s->ndim, nface = 2 * s->ndim;
   std::unordered_map<uint64_t, int> at;
   at.reserve(2 * n);
   for (int i = 0; i < n; i++) {
@@ -118,6 +124,7 @@ int build_table(shx_amr* s) {
     if (b[0] < 0 || b[0] >= AMR_MAXLEV) return fail(SHX_EINVAL, "block %d: level %d outside [0, %d)", i, b[0], AMR_MAXLEV);
     for (int a = 0; a < 3; a++)
       if (b[1 + a] < 0 || b[1 + a] >= (root[a] << b[0])) return fail(SHX_EINVAL, "block %d: coordinate %d out of range", i, a);
+    if (s->ndim == 2 && b[3] != 0) return fail(SHX_EINVAL, "block %d: a 2D block has iz 0", i);
     if (!at.emplace(key_of(b[0], b[1], b[2], b[3]), i).second) return fail(SHX_EINVAL, "block %d repeats an earlier block", i);
   }
   auto find = [&](int L, int x, int y, int z) {
@@ -135,12 +142,12 @@ int build_table(shx_amr* s) {
     r[BT_PARENT] = L > 0 ? find(L - 1, b[1] >> 1, b[2] >> 1, b[3] >> 1) : -1;
     if (L > 0 && r[BT_PARENT] < 0) return fail(SHX_EINVAL, "block %d (level %d) has no parent", i, L);
     int nch = 0;
-    for (int o = 0; o < 8; o++) {
+    for (int o = 0; o < nchild; o++) {
       int ch = find(L + 1, 2 * b[1] + (o & 1), 2 * b[2] + ((o >> 1) & 1), 2 * b[3] + ((o >> 2) & 1));
       r[BT_CHILD + o] = ch;
       nch += ch >= 0;
     }
-    if (nch != 0 && nch != 8) return fail(SHX_EINVAL, "block %d has %d of its 8 children", i, nch);
+    if (nch != 0 && nch != nchild) return fail(SHX_EINVAL, "block %d has %d of its %d children", i, nch, nchild);
     r[BT_FLAGS] = 1 | (nch == 0 ? 2 : 0);
   }
   int nroot = 0;
@@ -151,6 +158,7 @@ int build_table(shx_amr* s) {
     int* r = &s->tab[(size_t)i * AMR_BT];
     const int L = b[0];
     for (int f = 0; f < 6; f++) {
+      if (f >= nface) { r[BT_NB + f] = i; continue; }
       const int axis = f >> 1, up = f & 1;
       int c[3] = {b[1], b[2], b[3]};
       const int nn = root[axis] << L;
@@ -334,10 +342,10 @@ template <typename T>
 int set_state_t(shx_amr* s, const T* u1, const T* u2, size_t ncell, double time, int steps) {
   if (!s) return fail(SHX_EINVAL, "null sim");
   if ((int)sizeof(T) != s->es) return fail(SHX_EINVAL, "state precision %d does not match the sim's %d", (int)sizeof(T) * 8, s->es * 8);
-  if (ncell != s->ncell) return fail(SHX_EINVAL, "ncell %zu != %zu (blocks x 4096)", ncell, s->ncell);
+  if (ncell != s->ncell) return fail(SHX_EINVAL, "ncell %zu != %zu (blocks x %d)", ncell, s->ncell, s->cells);
   for (int i = 0; i < s->nblocks; i++) {
     if (!(s->tab[(size_t)i * AMR_BT + BT_FLAGS] & 2)) continue;
-    for (size_t o = (size_t)i * AMR_CELLS * 4; o < (size_t)(i + 1) * AMR_CELLS * 4; o++)
+    for (size_t o = (size_t)i * s->cells * 4; o < (size_t)(i + 1) * s->cells * 4; o++)
       if (!isfinite((double)u1[o]) || !isfinite((double)u2[o])) return fail(SHX_ESTATE, "non-finite state in leaf %d", i);
   }
   const size_t bytes = ncell * 4 * sizeof(T);
@@ -417,7 +425,7 @@ int amr_reconfigure(shx_amr* s, const shx_config* cfg, bool scheme) {
   if (!(b.gamma > 1) || !(b.cfl > 0)) return fail(SHX_EINVAL, "gamma must exceed 1 and cfl be positive");
   AmrOps ops = s->ops;
   if (scheme) {
-    ops = amr_ops(b.precision, b.limiter, b.riemann, b.isentropic ? 1 : 0);
+    ops = amr_ops(b.precision, b.limiter, b.riemann, b.isentropic ? 1 : 0, s->ndim);
     if (!ops.assemble)
       return fail(SHX_EINVAL, "no block kernels built for precision %d limiter %d riemann %d isentropic %d",
                   b.precision, b.limiter, b.riemann, b.isentropic);
@@ -591,12 +599,18 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   if (!cfg || !layout || !blocks || !out || nblocks < 1) return fail(SHX_EINVAL, "null argument or no blocks");
   *out = nullptr;
   const shx_config& c = *cfg;
+  if (c.ndim != 2 && c.ndim != 3) return fail(SHX_EINVAL, "the block grid is 2D or 3D (ndim %d)", c.ndim);
   for (int a = 0; a < 3; a++) {
     if (layout->root[a] < 1) return fail(SHX_EINVAL, "root[%d] = %d", a, layout->root[a]);
-    if (c.n[a] != AMR_BS * layout->root[a]) return fail(SHX_EINVAL, "n[%d] = %d must be 16 x root[%d] = %d", a, c.n[a], a, AMR_BS * layout->root[a]);
+    if (a == 2 && c.ndim == 2) {
+      if (layout->root[2] != 1 || c.n[2] != 1) return fail(SHX_EINVAL, "a 2D block grid has root[2] = 1 and n[2] = 1");
+    } else if (c.n[a] != AMR_BS * layout->root[a]) {
+      return fail(SHX_EINVAL, "n[%d] = %d must be 16 x root[%d] = %d", a, c.n[a], a, AMR_BS * layout->root[a]);
+    }
     if (c.bc_lo[a] < 0 || c.bc_lo[a] > 3 || c.bc_hi[a] < 0 || c.bc_hi[a] > 3) return fail(SHX_EINVAL, "boundary code on axis %d", a);
   }
-  if (c.ndim != 3) return fail(SHX_EINVAL, "the block grid is 3D (ndim %d)", c.ndim);
+  if (c.ndim == 2 && layout->path == SHX_AMR_PATH_FUSED)
+    return fail(SHX_EINVAL, "2D block grids run on the passes and levels paths (the fused march steps 3D blocks)");
   if (c.precision != 32 && c.precision != 64) return fail(SHX_EINVAL, "precision %d", c.precision);
   if (!(c.gamma > 1) || !(c.dx > 0) || !(c.cfl > 0)) return fail(SHX_EINVAL, "gamma, dx and cfl must be positive (gamma > 1)");
   if (layout->path != SHX_AMR_PATH_FUSED && layout->path != SHX_AMR_PATH_PASSES && layout->path != SHX_AMR_PATH_LEVELS)
@@ -605,7 +619,7 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
     return fail(SHX_EINVAL, "time_bins %d (0 to %d)", layout->time_bins, AMR_MAX_BINS);
   if (layout->time_bins > 0 && layout->path == SHX_AMR_PATH_FUSED)
     return fail(SHX_EINVAL, "per-block timesteps run on the passes and levels paths");
-  AmrOps ops = amr_ops(c.precision, c.limiter, c.riemann, c.isentropic ? 1 : 0);
+  AmrOps ops = amr_ops(c.precision, c.limiter, c.riemann, c.isentropic ? 1 : 0, c.ndim);
   if (!ops.assemble) return fail(SHX_EINVAL, "no block kernels built for precision %d limiter %d riemann %d isentropic %d", c.precision, c.limiter, c.riemann, c.isentropic);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c.device) return fail(SHX_EDEVICE, "no HIP device %d", c.device);
@@ -614,8 +628,11 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   s->cfg = c;
   s->lay = *layout;
   s->es = c.precision == 64 ? 8 : 4;
+  s->ndim = c.ndim;
+  s->cells = c.ndim == 2 ? AMR_BS * AMR_BS : AMR_CELLS;
+  s->texels = c.ndim == 2 ? AMR_PAD * AMR_PAD : AMR_TEXELS;
   s->nblocks = nblocks;
-  s->ncell = (size_t)nblocks * AMR_CELLS;
+  s->ncell = (size_t)nblocks * s->cells;
   s->blocks.assign(blocks, blocks + 4 * (size_t)nblocks);
   int rc = build_table(s);
   if (rc) { delete s; return rc; }
@@ -643,7 +660,7 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
     s->grid = cus;
   }
   const size_t ubytes = s->ncell * 4 * s->es;
-  const size_t abytes = fused ? 0 : (size_t)nblocks * AMR_TEXELS * 4 * s->es;
+  const size_t abytes = fused ? 0 : (size_t)nblocks * s->texels * 4 * s->es;
   auto alloc = [&](void** p, size_t b) { return hipMalloc(p, b) == hipSuccess; };
   bool ok = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess;
   for (int p = 0; ok && p < 2; p++) ok = alloc(&s->U1[p], ubytes) && alloc(&s->U2[p], ubytes);
@@ -917,14 +934,15 @@ int shx_amr_regrid(shx_amr* s, const int* blocks, int nblocks) {
   for (int j = 0; j < nblocks; j++) {
     const int L = blocks[4 * j], x = blocks[4 * j + 1], y = blocks[4 * j + 2], z = blocks[4 * j + 3];
     bool new_leaf = true;
-    for (int o = 0; o < 8 && new_leaf; o++)
+    const int nchild = 1 << s->ndim;
+    for (int o = 0; o < nchild && new_leaf; o++)
       if (nkeys.count(key_of(L + 1, 2 * x + (o & 1), 2 * y + ((o >> 1) & 1), 2 * z + ((o >> 2) & 1)))) new_leaf = false;
     if (!new_leaf) continue;
     const int i = find(L, x, y, z);
     if (i >= 0 && is_leaf(i)) { map.insert(map.end(), {j, 1, i, 0}); continue; }
     if (i >= 0) {
       bool kids = true;
-      for (int o = 0; o < 8 && kids; o++) {
+      for (int o = 0; o < nchild && kids; o++) {
         const int k = find(L + 1, 2 * x + (o & 1), 2 * y + ((o >> 1) & 1), 2 * z + ((o >> 2) & 1));
         kids = k >= 0 && is_leaf(k);
       }
@@ -1003,7 +1021,7 @@ int shx_amr_flux_mismatch(shx_amr* s, int64_t counts[3], double* worst_coarse_fi
   // one step of the reference's passes with the update in DEBUG_FLUX form (the fused path gives
   // the same bits): atlases for a fused sim are made for the call and freed after it
   const bool own = s->W1 == nullptr;
-  const size_t abytes = (size_t)s->nblocks * AMR_TEXELS * 4 * s->es;
+  const size_t abytes = (size_t)s->nblocks * s->texels * 4 * s->es;
   if (own) {
     void** at[4] = {&s->W1, &s->W2, &s->PA, &s->PB};
     for (void** a : at) {
@@ -1054,21 +1072,23 @@ int shx_amr_flux_mismatch(shx_amr* s, int64_t counts[3], double* worst_coarse_fi
   // AmrSim.fluxMismatch (amr_sim.ts:4078-4160): same-level faces between blocks compared bit for
   // bit (once, from the lower block); a coarse face against the mean of the fine fluxes beyond it
   auto flux = [&](int b, int x, int y, int z, int f) {
-    const size_t o = ((size_t)b * AMR_CELLS + ((size_t)z * AMR_BS + y) * AMR_BS + x) * 4;
+    const size_t o = ((size_t)b * s->cells + ((size_t)z * AMR_BS + y) * AMR_BS + x) * 4;
     return f < 4 ? f1[o + f] : f2[o + f - 4];
   };
+  // 2D: no z faces, one layer of cells, two fine faces per coarse face
+  const int zc = s->ndim == 2 ? 1 : AMR_BS;
   auto is_leaf = [&](int i) { return (s->tab[(size_t)i * AMR_BT + BT_FLAGS] & 2) != 0; };
   int64_t faces = 0, mis = 0, cf = 0;
   double worst = 0;
   for (int b = 0; b < s->nblocks; b++) {
     if (!is_leaf(b)) continue;
-    for (int axis = 0; axis < 3; axis++)
+    for (int axis = 0; axis < s->ndim; axis++)
       for (int side = 0; side < 2; side++) {
         const int nb = s->tab[(size_t)b * AMR_BT + BT_NB + 2 * axis + side];
         if (nb < 0) continue;
         const bool refined = !is_leaf(nb);
         if (!refined && side == 0) continue;
-        for (int k = 0; k < AMR_BS; k++)
+        for (int k = 0; k < zc; k++)
           for (int j = 0; j < AMR_BS; j++)
             for (int i = 0; i < AMR_BS; i++) {
               int l[3] = {i, j, k};
@@ -1083,21 +1103,23 @@ int shx_amr_flux_mismatch(shx_amr* s, int64_t counts[3], double* worst_coarse_fi
                 continue;
               }
               const int t1 = axis == 2 ? 0 : axis + 1, t2 = axis == 0 ? 2 : axis - 1;
+              const int nfine = s->ndim == 2 ? 2 : 4;
               double sum = 0;
-              for (int sb = 0; sb < 4; sb++) {
+              for (int sb = 0; sb < nfine; sb++) {
                 int bits[3] = {0, 0, 0}, m[3] = {0, 0, 0};
                 bits[axis] = side ? 0 : 1;
                 m[axis] = side ? 0 : AMR_BS - 1;
                 const int q[2] = {t1, t2};
-                for (int n = 0; n < 2; n++) {
+                for (int n = 0, bit = 0; n < 2; n++) {
+                  if (s->ndim == 2 && q[n] == 2) continue;   // the z index stays 0
                   bits[q[n]] = l[q[n]] >= AMR_BS / 2;
-                  m[q[n]] = 2 * (l[q[n]] - (AMR_BS / 2) * bits[q[n]]) + ((sb >> n) & 1);
+                  m[q[n]] = 2 * (l[q[n]] - (AMR_BS / 2) * bits[q[n]]) + ((sb >> bit++) & 1);
                 }
                 const int ch = s->tab[(size_t)nb * AMR_BT + BT_CHILD + (bits[0] | (bits[1] << 1) | (bits[2] << 2))];
                 sum += flux(ch, m[0], m[1], m[2], 2 * axis + (side ? 0 : 1));
               }
               cf++;
-              worst = fmax(worst, fabs(mine - sum / 4));
+              worst = fmax(worst, fabs(mine - sum / nfine));
             }
       }
   }
