@@ -276,7 +276,7 @@ def test_2d_refusals():
     blocks = quadtree()
     with pytest.raises(ShxError, match="fused"):
         AmrSim(cfg2(), blocks, ROOT, path="fused")
-    with pytest.raises(ShxError, match="iz 0"):
+    with pytest.raises(ShxError, match="coordinate 2 out of range"):
         AmrSim(cfg2(), np.asarray([(0, i, j, 1) for j in range(4) for i in range(4)], np.int32), ROOT, path="passes")
     with pytest.raises(ShxError, match="children"):
         bad = amr_tree.build_tree(ROOT, 1, lambda L, c: c == (0, 0, 0), ndim=2).tolist()
