@@ -374,6 +374,7 @@ PUBLISHED = {
     ("kh", 2048): {"H200 NVL": 2.74e9, "H100 SXM": 2.60e9, "A100 SXM4": 1.74e9, "RTX 3080": 1.85e9},
     ("sedov", 128): {"H200 NVL": 1.42e9, "H100 SXM": 1.34e9, "A100 SXM4": 1.04e9, "RTX 3080": 1.26e9},
     ("sedov", 256): {"H200 NVL": 1.67e9, "H100 SXM": 1.57e9, "A100 SXM4": 1.20e9, "RTX 3080": 1.39e9},
+    ("kh_amr", 4096): {"H200 NVL": 1.34e9, "H100 SXM": 1.02e9, "A100 SXM4": 8.14e8, "RTX 3080": 9.86e8},
 }
 
 
@@ -444,6 +445,51 @@ def published_bench(args):
     return {"scheme": "PLM (Koren, isentropic reconstruction) + HLLC, CFL 0.4, FP32", "grids": out}
 
 
+def kh_amr_bench(args):
+    """The reference's adaptive KH figure's shape on 2D blocks: the shear layer on a quadtree of
+    256^2 root cells refined 4 times (finest 4096^2; problems.kh_adaptive), static mesh, with the
+    reference's default scheme (Koren + isentropic reconstruction, HLLC, CFL 0.4, FP32), on the
+    batched passes (each pass one launch over every level) and the per-level passes; leaf
+    zone-updates/s beside the reference's published figure for its 4096-finest, 4-level run."""
+    from shaderous_amd import amr_tree, problems
+    from shaderous_amd.amr import AmrSim
+    from shaderous_amd.sim import SimConfig
+    blocks, dx0, prims = problems.kh_adaptive()
+    u1, u2 = amr_tree.blocks_state(blocks, dx0, prims, 5.0 / 3.0, ndim=2)
+    leaves = int(amr_tree.leaf_mask(blocks).sum())
+    cells = leaves * 256
+    log(f"[bench] KH adaptive: {len(blocks)} blocks, {leaves} leaves ({cells} cells)")
+    cfg = SimConfig(n=(256, 256, 1), ndim=2, dx=dx0, gamma=5.0 / 3.0, limiter="koren", isentropic=True, cfl=0.4)
+    res = {}
+    for path in ("passes", "levels"):
+        sim = AmrSim(cfg, blocks, (16, 16, 1), path=path)
+        try:
+            sim.set_state(u1, u2)
+            sim.step(5)
+            sim.sync()
+            t0 = time.perf_counter()
+            sim.step(args.kh_amr_steps)
+            sim.sync()
+            t = time.perf_counter() - t0
+            prof = sim.profile(5)
+            st = sim.stats()
+        finally:
+            sim.destroy()
+        ms = t / args.kh_amr_steps * 1e3
+        res[path] = {"value": cells * args.kh_amr_steps / t, "ms_per_step": ms, "passes_ms": prof,
+                     "launches_per_step": st["launches_per_step"],
+                     "per_step": step_roofline(cells, prof["step"])}
+        log(f"[bench] KH adaptive {path}: {ms:.4f} ms/step, {cells * args.kh_amr_steps / t:.3e} upd/s, {prof}")
+    pub = PUBLISHED[("kh_amr", 4096)]
+    return {"workload": "2D Kelvin-Helmholtz shear layer on 16^2 blocks: root 256^2, 4 levels (finest 4096^2), "
+                        "static mesh around the interfaces, gamma 5/3, PLM (Koren, isentropic reconstruction) + HLLC, "
+                        "CFL 0.4, FP32",
+            "blocks": len(blocks), "leaves": leaves, "leaf_cells": cells, "steps": args.kh_amr_steps,
+            "value": res["passes"]["value"], "paths": res, "reference_published": pub,
+            "reference_leaf_cells": 772864, "over_h200": res["passes"]["value"] / pub["H200 NVL"],
+            "batched_over_levels": res["passes"]["value"] / res["levels"]["value"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -459,6 +505,7 @@ def main():
     ap.add_argument("--amr-paths", default="fused,levels", help="AMR paths to time (fused, passes, levels)")
     ap.add_argument("--time-bins", type=int, default=3, help="per-block timesteps on the AMR blast (0: skip)")
     ap.add_argument("--ens-steps", type=int, default=200, help="steps of the 2D ensemble (0: skip it)")
+    ap.add_argument("--kh-amr-steps", type=int, default=100, help="steps of the adaptive 2D KH on blocks (0: skip it)")
     ap.add_argument("--kh-steps", type=int, default=100,
                     help="steps of 2D KH at 256^2 (half at 2048^2 and Sedov 128^3, a fifth at 256^3; 0: skip)")
     ap.add_argument("--tile2d", default=None, metavar="TX,TY", type=lambda v: tuple(int(x) for x in v.split(",")),
@@ -544,9 +591,12 @@ def main():
     if args.kh_steps > 0:
         kh = kh_bench(args)
         pub = published_bench(args)
+    kh_amr = None
+    if args.kh_amr_steps > 0:
+        kh_amr = kh_amr_bench(args)
     if value is None:
-        if amr or ens or kh:
-            print(json.dumps({"amr": amr, "ensemble": ens, "kh": kh, "published_grids": pub}))
+        if amr or ens or kh or kh_amr:
+            print(json.dumps({"amr": amr, "ensemble": ens, "kh": kh, "published_grids": pub, "kh_adaptive": kh_amr}))
         return
 
     cpu = None
@@ -586,6 +636,7 @@ def main():
         "ensemble": ens,
         "kh": kh,
         "published_grids": pub,
+        "kh_adaptive": kh_amr,
     }
     print(json.dumps(out))
 

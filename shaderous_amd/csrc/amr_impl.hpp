@@ -483,6 +483,10 @@ __global__ __launch_bounds__(256) void k_amr_predict(Params<T> P, Levels<T> LV, 
   const int s = list[b];
   const GSrc g = ghost_src(tab, s, x, y, z);
   if (g.kind == 2) return;
+  // with one timestep a ghost whose source is a refined block (a parent, whose atlas the step
+  // does not assemble) is not predicted: the faces on that block are its children's
+  // (match_faces), and nothing else reads the ghost's half step
+  if (!B.k && g.kind == 0 && g.slot != s && !is_leaf(tab, g.slot)) return;
   // with bins, a ghost from a leaf part way through a longer step was filled from its prediction
   // and is predicted in place (amr/predict.wgsl:193-197); the leaf's own step is 2^bin dt
   const bool stale = B.k && g.slot != s && mid_block(B, tab, g.slot);

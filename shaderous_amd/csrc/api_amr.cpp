@@ -278,11 +278,15 @@ int enqueue_step(shx_amr* s, int p, int stage = -1) {
     return n;
   }
   if (run(0)) {
+    // the leaves' atlases alone: with one timestep a parent's atlas feeds only the half steps of
+    // its leaf neighbours' ghosts beside it, which the predict pass then skips (the faces there
+    // are formed from the parent's children: match_faces), and ghosts on a parent are restricted
+    // from its children's state
     if (levels) {
       for (int l = 0; l < s->nlevels; l++)
-        if (s->all_lv[l].count) { o.assemble(L, lp(s, s->all_lv[l]), s->all_lv[l].count, s->U1[p], s->U2[p], s->W1, s->W2, s->slot); n++; }
+        if (s->leaf_lv[l].count) { o.assemble(L, lp(s, s->leaf_lv[l]), s->leaf_lv[l].count, s->U1[p], s->U2[p], s->W1, s->W2, s->slot); n++; }
     } else {
-      o.assemble(L, lp(s, s->all), s->all.count, s->U1[p], s->U2[p], s->W1, s->W2, s->slot);
+      o.assemble(L, lp(s, s->leaf), s->leaf.count, s->U1[p], s->U2[p], s->W1, s->W2, s->slot);
       n++;
     }
   }
@@ -516,8 +520,10 @@ bool faster_neighbor(const shx_amr* s, const std::vector<int>& k, int b) {
 // One cycle of 2^K substeps. Substep 0: bank the last step, assemble every block (its rates in
 // the reduction slot), each leaf's own rate, the cycle's dt; then (after the one readback of the
 // cycle) the bins and every substep's lists (bins_lists.wgsl): the blocks starting a step (LA,
-// parents included), the leaves among them (LL), those and the leaves half way through a step
-// with a faster face neighbour (LU). Each substep (s > 0: bank, assemble LA) predicts LL,
+// parents included: a ghost of a leaf beside a refined block is predicted from the parent's
+// atlas, and a finer neighbour half way through the leaf's step interpolates from that
+// prediction), the leaves among them (LL), those and the leaves half way through a step with a
+// faster face neighbour (LU). Each substep (s > 0: bank, assemble LA) predicts LL,
 // updates LU into the other buffer and copies those blocks back (the commit): the state stays
 // in its buffer through the cycle.
 int run_cycle(shx_amr* s) {

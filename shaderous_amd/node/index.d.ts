@@ -97,8 +97,10 @@ export interface CellProbe {
 }
 
 export interface AmrTree {
+  /** root blocks per axis; a 2D tree (cfg.ndim 2: 16^2-cell blocks, every iz 0) has root[2] = 1 */
   root: [number, number, number];
   blocks: Int32Array;
+  /** default 'fused' in 3D; 'passes' in 2D and with timeBins (the fused march steps 3D blocks, one dt) */
   path?: 'fused' | 'passes' | 'levels';
   compensated?: boolean;
   /** per-block timesteps: cycles of 2^timeBins substeps (default cfg.timeBins; passes path) */

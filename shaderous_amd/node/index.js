@@ -480,6 +480,7 @@ class Sim extends Base {
 /**
  * The block-AMR simulation (the reference's `AmrSim`) on a 2:1-balanced tree: `blocks` is an
  * Int32Array of (level, ix, iy, iz) rows, parents and leaves; `root` the root blocks per axis.
+ * cfg.ndim 2: a quadtree of 16^2-cell blocks (root[2] = 1, every iz 0) on the passes paths.
  */
 class AmrSim extends Base {
   constructor(cfg, { root, blocks, path, compensated = true, timeBins }) {
@@ -492,12 +493,13 @@ class AmrSim extends Base {
       diagnosticsAsync: native.diagnosticsAsync,
       clockAsync: native.clockAsync,
     });
-    const n = root.map((r) => 16 * r);
-    const ncfg = toNative({ ...cfg, n, ndim: 3 });
+    this.ndim = cfg.ndim === 2 ? 2 : 3;
+    const n = level0(root, this.ndim);
+    const ncfg = toNative({ ...cfg, n, ndim: this.ndim });
     // per-block timesteps (SimConfig.timeBins, at most 5): cycles of 2^K substeps, on the passes
-    // path (the fused march takes one dt)
+    // path (the fused march takes one dt and steps 3D blocks)
     const bins = Math.max(0, Math.min(5, Math.floor(timeBins !== undefined ? timeBins : cfg.timeBins || 0)));
-    const kind = path !== undefined ? path : bins > 0 ? 'passes' : 'fused';
+    const kind = path !== undefined ? path : bins > 0 || this.ndim === 2 ? 'passes' : 'fused';
     this.h = native.amrCreate(ncfg, { root, path: code(AMR_PATHS, kind, 'AMR path'), compensated, timeBins: bins }, blocks);
     this.root = root.slice(0, 3);
     this.dx0 = ncfg.dx;
@@ -509,17 +511,19 @@ class AmrSim extends Base {
   tree(blocks) {
     this.blocks = Int32Array.from(blocks);
     this.nblocks = blocks.length / 4;
-    this.ncell = this.nblocks * 4096;
+    this.ncell = this.nblocks * 16 ** this.ndim;
     let top = 0;
     for (let i = 0; i < this.nblocks; i++) top = Math.max(top, blocks[4 * i]);
     this.levels = top + 1;
     this.n = (16 * this.root[0]) << top;
-    this.nz = (16 * this.root[2]) << top;
+    this.nz = this.ndim === 2 ? 1 : (16 * this.root[2]) << top;
   }
 
   /** The uniform volume the display samples: the finest grid, at most 256 a side (src/amr/amr_sim.ts:3780-3785). */
   get displayDims() {
-    return this.root.map((r) => Math.min((16 * r) << (this.levels - 1), 256));
+    const d = this.root.map((r) => Math.min((16 * r) << (this.levels - 1), 256));
+    if (this.ndim === 2) d[2] = 1;
+    return d;
   }
 
   /**
@@ -537,17 +541,19 @@ class AmrSim extends Base {
       const c = [b[4 * i + 1], b[4 * i + 2], b[4 * i + 3]];
       if (parent.has([L, ...c].join())) continue;
       const index = [0, 1, 2].map((a) => {
+        if (a === 2 && this.ndim === 2) return 0;
         const across = (16 * this.root[a]) << L;
         return Math.min(across - 1, Math.max(0, Math.floor(f[a] * across)));
       });
       if (!index.every((x, a) => Math.floor(x / 16) === c[a])) continue;
       const { u1, u2 } = await this.readState();
       const local = index.map((x, a) => x - 16 * c[a]);
-      const cell = 4096 * i + local[0] + 16 * (local[1] + 16 * local[2]);
+      const cell = 16 ** this.ndim * i + local[0] + 16 * (local[1] + 16 * local[2]);
       const dx = this.dx0 / (1 << L);
-      const lo = domainLo({ ...this.cfg, dx: this.dx0, n: this.root.map((r) => 16 * r) });
+      const lo = domainLo({ ...this.cfg, dx: this.dx0, n: level0(this.root, this.ndim) });
       return {
-        pos: index.map((x, a) => (x + 0.5) * dx + lo[a]),
+        // (a 2D block's cells lie on z = 0, blk_pos, src/amr/grid.wgsl:172-177)
+        pos: index.map((x, a) => (a === 2 && this.ndim === 2 ? 0 : (x + 0.5) * dx + lo[a])),
         index,
         dx,
         u1: Array.from(u1.subarray(4 * cell, 4 * cell + 4)),
@@ -604,15 +610,18 @@ class AmrSim extends Base {
   }
 
   nativeConfig() {
-    const n = this.root.map((r) => 16 * r);
-    const ncfg = toNative({ ...this.cfg, n, ndim: 3, dx: this.dx0 });
-    return ncfg;
+    return toNative({ ...this.cfg, n: level0(this.root, this.ndim), ndim: this.ndim, dx: this.dx0 });
   }
 
   async profile(steps = 5) {
     const ms = native.amrProfile(live(this.h), steps);
     return { assemble: ms[0], clock: ms[1], predict: ms[2], update: ms[3], step: ms[4] };
   }
+}
+
+/** The level-0 grid's cells per axis: 16 per root block (one layer along z in 2D). */
+function level0(root, ndim) {
+  return [16 * root[0], 16 * root[1], ndim === 2 ? 1 : 16 * root[2]];
 }
 
 /** createSim (src/create.ts:58-61): a block grid when a tree is given, else the uniform grid. */

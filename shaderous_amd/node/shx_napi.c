@@ -47,6 +47,7 @@ typedef struct {
   int precision;
   size_t ncell;
   int nblocks;
+  int cells;      /* per block: 4096, or 256 on a 2D block grid */
 } Handle;
 
 static void finalize_handle(napi_env env, void* data, void* hint) {
@@ -545,7 +546,8 @@ static napi_value js_amr_create(napi_env env, napi_callback_info info) {
   h->p = sim;
   h->precision = c.precision;
   h->nblocks = (int)(len / 4);
-  h->ncell = (size_t)h->nblocks * 4096;
+  h->cells = c.ndim == 2 ? 256 : 4096;
+  h->ncell = (size_t)h->nblocks * h->cells;
   return wrap_handle(env, h);
 }
 
@@ -696,7 +698,7 @@ static napi_value js_amr_regrid(napi_env env, napi_callback_info info) {
   }
   SHX_CALL(env, shx_amr_regrid((shx_amr*)h->p, (const int*)data, (int)(len / 4)));
   h->nblocks = (int)(len / 4);
-  h->ncell = (size_t)h->nblocks * 4096;
+  h->ncell = (size_t)h->nblocks * h->cells;
   return undefined(env);
 }
 

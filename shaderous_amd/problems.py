@@ -223,6 +223,31 @@ def ensemble_sizes(count: int = 64, seed: int = 42, choices=(64, 128, 256, 512))
     return [int(c) for c in rng.choice(np.asarray(choices), size=count)]
 
 
+def kh_adaptive(root: int = 16, max_level: int = 4, buffer: float = 0.5):
+    """The 2D Kelvin-Helmholtz shear layer (kelvin_helmholtz's state, without the noise) on a
+    quadtree: root^2 blocks of 16^2 cells (256^2 at the defaults), refined `max_level` times
+    (finest 4096^2) wherever a block grown by `buffer` blocks on each side holds one of the two
+    interfaces y = 1/4, 3/4. The defaults give 4096 blocks, 3136 leaves, 802,816 leaf cells, the
+    size of the reference's adaptive KH figure at 4096 finest and 4 levels (772,864 leaf cells,
+    refined by its own flagging). Returns (blocks, dx0, prim_fn)."""
+    from .amr_tree import BS, build_tree
+    dx0 = 1.0 / (root * BS)
+
+    def refine(level, b):
+        size = BS * dx0 / (1 << level)
+        lo = b[1] * size - 0.5
+        glo, ghi = lo - buffer * size, lo + (1 + buffer) * size
+        return any(glo <= yi <= ghi for yi in (-0.25, 0.25))
+    blocks = build_tree((root, root, 1), max_level, refine, ndim=2)
+
+    def prims(x, y, z):
+        inside = np.abs(y) < 0.25
+        return {"rho": np.where(inside, 2.0, 1.0), "vx": np.where(inside, 0.5, -0.5),
+                "vy": 0.01 * np.sin(4 * np.pi * (x + 0.5)), "vz": 0.0 * x, "p": np.full(x.shape, 2.5),
+                "s": np.where(inside, 1.0, 0.0)}
+    return blocks, dx0, prims
+
+
 def amr_blast(seed: int = 7, root: int = 8, max_level: int = 3, radius: float = 0.1, p_in: float = 100.0,
               p_out: float = 1.0, gamma: float = 1.4):
     """The block-AMR blast workload: the unit box, root^3 blocks of 16^3 cells, up to `max_level`

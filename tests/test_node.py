@@ -417,3 +417,35 @@ def test_node_amr_time_bins_match_python(tmp_path):
     leaf = amr_tree.leaf_mask(blocks).repeat(4096)
     assert np.array_equal(o1[leaf], p1[leaf]) and np.array_equal(o2[leaf], p2[leaf])
     assert out["clock"][3] == ck[3]
+
+
+@pytest.mark.gpu
+def test_node_amr_2d_matches_python(tmp_path):
+    """A quadtree (cfg.ndim 2) through the facade: the Node caller's steps give the ctypes
+    caller's bits, and its probe finds the leaf cell at a position in the refined patch."""
+    from shaderous_amd.amr import AmrSim
+    from shaderous_amd.sim import SimConfig
+    from test_gpu_amr2d import ROOT, flow, quadtree
+    blocks = quadtree()
+    u1, u2 = amr_tree.blocks_state(blocks, 1.0 / 64, flow, 1.4, ndim=2)
+    job = {"kind": "amr", "steps": 4, "cfg": {"n": 64, "ndim": 2, "limiter": "mc", "isentropicRecon": False,
+                                              "cfl": 0.4, "box": 1.0},
+           "tree": {"root": list(ROOT), "blocks": blocks.reshape(-1).tolist()}, "probe": [0.44, 0.45, 0.5]}
+    o1, o2, out = _run(tmp_path, job, u1, u2)
+    cfg = SimConfig(n=(64, 64, 1), ndim=2, dx=1.0 / 64, limiter="mc", isentropic=False, cfl=0.4)
+    sim = AmrSim(cfg, blocks, ROOT, path="passes")
+    try:
+        sim.set_state(u1, u2)
+        sim.step(4)
+        p1, p2 = sim.read_state()
+    finally:
+        sim.destroy()
+    leaf = amr_tree.leaf_mask(blocks).repeat(256)
+    assert np.array_equal(o1[leaf], p1[leaf]) and np.array_equal(o2[leaf], p2[leaf])
+    pr = out["probe"]
+    assert pr["block"]["level"] == 2 and pr["index"][2] == 0 and pr["pos"][2] == 0
+    L, bx, by, _ = blocks[pr["block"]["slot"]]
+    idx = pr["index"]
+    assert idx == [int(0.44 * (64 << 2)), int(0.45 * (64 << 2)), 0] and (bx, by) == (idx[0] // 16, idx[1] // 16)
+    cell = pr["block"]["slot"] * 256 + (idx[0] - 16 * bx) + 16 * (idx[1] - 16 * by)
+    assert np.array_equal(np.float32(pr["u1"]), p1[cell]) and np.array_equal(np.float32(pr["u2"]), p2[cell])
