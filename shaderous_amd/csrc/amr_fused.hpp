@@ -585,7 +585,7 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
   if (threadIdx.x == 0) {
     T m = T(0);
     for (int w = 0; w < AF_PAIR * AF_THREADS / 64; w++) m = tmax(m, red[w]);
-    atomicMax(next_slot, (unsigned long long)order_bits(m));
+    max_slot(next_slot, (unsigned long long)order_bits(m));
   }
 }
 

@@ -329,13 +329,33 @@ __global__ __launch_bounds__(256) void k_amr_assemble(Params<T> P, Levels<T> LV,
   const int b = blockIdx.x / NCH;
   const int t = (blockIdx.x % NCH) * 256 + threadIdx.x;
   T r = T(0);
-  if (t < AmrGeo<D>::TEXELS) {
-    const int x = t % AMR_PAD - AMR_NG, y = (t / AMR_PAD) % AMR_PAD - AMR_NG;
-    const int z = D == 3 ? t / (AMR_PAD * AMR_PAD) - AMR_NG : 0;
+  int x, y, z;
+  bool any = t < AmrGeo<D>::TEXELS;
+  if constexpr (D == 3) {
+    x = t % AMR_PAD - AMR_NG;
+    y = (t / AMR_PAD) % AMR_PAD - AMR_NG;
+    z = t / (AMR_PAD * AMR_PAD) - AMR_NG;
+  } else {
+    // 2D: the first workgroup's 256 threads take the cells, the second's the 128 face ghosts
+    // face by face (-x, +x, -y, +y; two layers of 16), so no wave mixes the cells' copy with a
+    // ghost's prolongation or restriction; the 16 corner texels are never assembled
+    z = 0;
+    if (t < AMR_BS * AMR_BS) {
+      x = t % AMR_BS;
+      y = t / AMR_BS;
+    } else {
+      const int g = t - AMR_BS * AMR_BS, f = g >> 5, layer = (g >> 4) & 1, i = g & 15;
+      any = g < 128;
+      const int a = (f & 1) ? AMR_BS + layer : -1 - layer;
+      x = f < 2 ? a : i;
+      y = f < 2 ? i : a;
+    }
+  }
+  if (any) {
     const int s = list[b];
     Prim<T> w;
     if (amr_texel<T, LIM, D>(P, tab, U1, U2, s, x, y, z, w, B)) {
-      const size_t o = (size_t)s * AmrGeo<D>::TEXELS + t;
+      const size_t o = (size_t)s * AmrGeo<D>::TEXELS + texd<D>(x, y, z);
       W1[o] = v4<T>(w.rho, w.vx, w.vy, w.vz);
       W2[o] = v4<T>(w.p, w.cs, w.s, T(0));
       const bool inside = x >= 0 && y >= 0 && z >= 0 && x < AMR_BS && y < AMR_BS && z < AmrGeo<D>::ZC;
@@ -478,8 +498,26 @@ __global__ __launch_bounds__(256) void k_amr_predict(Params<T> P, Levels<T> LV, 
   const int b = blockIdx.x / NCH;
   const int t = (blockIdx.x % NCH) * 256 + threadIdx.x;
   if (t >= pred_texels<D>()) return;
-  const int x = t % PRED_SIDE - 1, y = (t / PRED_SIDE) % PRED_SIDE - 1;
-  const int z = D == 3 ? t / (PRED_SIDE * PRED_SIDE) - 1 : 0;
+  int x, y, z;
+  if constexpr (D == 3) {
+    x = t % PRED_SIDE - 1;
+    y = (t / PRED_SIDE) % PRED_SIDE - 1;
+    z = t / (PRED_SIDE * PRED_SIDE) - 1;
+  } else {
+    // 2D: the cells first (four whole waves), then the 64 first-layer face ghosts face by face;
+    // the 4 corner texels are edges, never predicted
+    z = 0;
+    if (t < AMR_BS * AMR_BS) {
+      x = t % AMR_BS;
+      y = t / AMR_BS;
+    } else {
+      const int g = t - AMR_BS * AMR_BS, f = g >> 4, i = g & 15;
+      if (g >= 64) return;
+      const int a = (f & 1) ? AMR_BS : -1;
+      x = f < 2 ? a : i;
+      y = f < 2 ? i : a;
+    }
+  }
   const int s = list[b];
   const GSrc g = ghost_src(tab, s, x, y, z);
   if (g.kind == 2) return;

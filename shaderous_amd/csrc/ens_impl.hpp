@@ -155,7 +155,7 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 2 : 1)) void k_ens_fused(
     if (t == 0) {
       T m = T(0);
       for (int k = 0; k < (NT + 63) / 64; k++) m = tmax(m, red[k]);
-      atomicMax(C.next(w.inst), (unsigned long long)order_bits(m));
+      max_slot(C.next(w.inst), (unsigned long long)order_bits(m));
     }
   }
 }
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(64) void k_ens_rows(const EnsInst<float>* __restric
     pc = pp;
   }
   const T v = wave_max(rate_max);
-  if (lane == 0) atomicMax(C.next(w.inst), (unsigned long long)order_bits(v));
+  if (lane == 0) max_slot(C.next(w.inst), (unsigned long long)order_bits(v));
 }
 
 }  // namespace shx

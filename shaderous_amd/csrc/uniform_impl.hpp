@@ -30,6 +30,14 @@ __device__ __forceinline__ T wave_max(T v) {
   return v;
 }
 
+// atomicMax on a rate slot, which only grows during a launch (it is cleared before the launch
+// that fills it, and a launch starts with its caches invalidated): a value not above the slot's
+// current one, read from L2, changes nothing and skips the atomic. With thousands of workgroups
+// or waves (the block passes, the row march) the atomics on one address queue behind each other.
+__device__ __forceinline__ void max_slot(unsigned long long* slot, unsigned long long bits) {
+  if (bits > __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(slot, bits);
+}
+
 // Max over the block, then one atomicMax on the slot (ordered bit patterns of non-negative T).
 template <typename T>
 __device__ __forceinline__ void block_max_to(T v, unsigned long long* slot) {
@@ -42,7 +50,7 @@ __device__ __forceinline__ void block_max_to(T v, unsigned long long* slot) {
   if (lane == 0) {
     T m = T(0);
     for (int w = 0; w < (nthreads + 63) / 64; w++) m = tmax(m, red[w]);
-    atomicMax(slot, (unsigned long long)order_bits(m));
+    max_slot(slot, (unsigned long long)order_bits(m));
   }
 }
 
@@ -636,7 +644,7 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 4 : 1)) void k_fused(
     if (t == 0) {
       T m = T(0);
       for (int w = 0; w < (NT + 63) / 64; w++) m = tmax(m, red[w]);
-      atomicMax(C.next(0), (unsigned long long)order_bits(m));
+      max_slot(C.next(0), (unsigned long long)order_bits(m));
     }
   }
 }

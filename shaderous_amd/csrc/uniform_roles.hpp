@@ -312,7 +312,7 @@ __global__ __launch_bounds__((RoleTile<TX, TY>::NTH), (RoleTile<TX, TY>::MINW)) 
     if (tid == 0) {
       T m = T(0);
       for (int w = 0; w < (NT + 63) / 64; w++) m = tmax(m, red[w]);
-      atomicMax(C.next(0), (unsigned long long)order_bits(m));
+      max_slot(C.next(0), (unsigned long long)order_bits(m));
     }
   }
 }
