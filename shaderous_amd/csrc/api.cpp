@@ -95,7 +95,7 @@ const void* current_clock(const shx_sim* s) {
 void enqueue_fused(shx_sim* s, int p, int tile, int zc, bool debug) {
   if (s->flat2d && !debug) {
     s->eops.fused(s->stream, tile, s->egrid, s->einst, 1, s->etiles[tile], s->entiles[tile], s->U1[p], s->U2[p],
-                  s->U1[1 - p], s->U2[1 - p], clock_refs(s, p));
+                  s->U1[1 - p], s->U2[1 - p], clock_refs(s, p), s->es == 8 ? (const void*)&s->pd : (const void*)&s->pf);
     return;
   }
   s->ops.fused(launch_of(s), tile, s->U1[p], s->U2[p], s->U1[1 - p], s->U2[1 - p], clock_refs(s, p),

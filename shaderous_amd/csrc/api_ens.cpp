@@ -50,7 +50,7 @@ void* clock_of(shx_ens* e, int i) { return static_cast<char*>(e->clk) + (size_t)
 
 void enqueue_fused(shx_ens* e, int p, int tile) {
   e->ops.fused(e->stream, tile, e->grid, e->inst, e->count, e->tiles[tile], e->ntiles[tile], e->U1[p], e->U2[p],
-               e->U1[1 - p], e->U2[1 - p], clock_refs(e, p));
+               e->U1[1 - p], e->U2[1 - p], clock_refs(e, p), nullptr);
 }
 
 void enqueue_step(shx_ens* e, int p) { enqueue_fused(e, p, e->tile); }

@@ -56,10 +56,15 @@ struct Ens2dLayout {
   static constexpr int Q = 0, SX = 7 * NT, SY = 16 * NT, FX = 7 * NT, FY = 15 * NT, SIZE = 25 * NT;
 };
 
-template <typename T, int LIM, int RS, bool ISEN, int BX, int BY>
+// SINGLE: a single grid (a flat uniform grid stepped as a batch of one) whose parameters come
+// as the kernel argument P1 and whose tiles are raster (tile wi at (wi mod ntx, wi / ntx)), so a
+// workgroup's first loads wait on neither the tile list nor the instance record. The clock pass
+// of the next step is the last workgroup's, which the launch adds when every tile has one already.
+template <typename T, int LIM, int RS, bool ISEN, int BX, int BY, bool SINGLE>
 __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 2 : 1)) void k_ens_fused(
     const EnsInst<T>* __restrict__ inst, const EnsTile* __restrict__ tiles, int ntiles, const T4<T>* __restrict__ U1,
-    const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1, T4<T>* __restrict__ O2, StepClock<T> C, int ninst) {
+    const T4<T>* __restrict__ U2, T4<T>* __restrict__ O1, T4<T>* __restrict__ O2, StepClock<T> C, int ninst,
+    Params<T> P1, int ntx) {
   constexpr int NT = BX * BY;
   using LY = Ens2dLayout<T, NT>;
   __shared__ T L[LY::SIZE];
@@ -67,13 +72,13 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 2 : 1)) void k_ens_fused(
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int t = tx + BX * ty;
   const bool in_r1 = tx >= 1 && tx <= BX - 2 && ty >= 1 && ty <= BY - 2;
-  if (blockIdx.x == 0)
-    for (int i = t; i < ninst; i += NT) step_clock_advance(inst[i].P, C, i);
+  if (blockIdx.x == gridDim.x - 1)
+    for (int i = t; i < ninst; i += NT) step_clock_advance(SINGLE ? P1 : inst[i].P, C, i);
 
   for (int wi = blockIdx.x; wi < ntiles; wi += gridDim.x) {
-    const EnsTile w = tiles[wi];
-    const Params<T> P = inst[w.inst].P;
-    const long long off = inst[w.inst].off;
+    const EnsTile w = SINGLE ? EnsTile{0, (wi % ntx) * (BX - 4), (wi / ntx) * (BY - 4), 0} : tiles[wi];
+    const Params<T> P = SINGLE ? P1 : inst[w.inst].P;
+    const long long off = SINGLE ? 0ll : inst[w.inst].off;
     const int nx = P.n[0], ny = P.n[1];
     const T dt = step_dt(P, C, w.inst);
     const T dtdx = dt / P.dx;
@@ -354,9 +359,10 @@ struct EnsOps {
   // the largest rate of each instance's state into slot[stride * instance]
   void (*rates)(hipStream_t, const void* inst, const void* tiles, int ntiles, int tile, const void* u1, const void* u2,
                 unsigned long long* slot, int stride);
-  // one step of every instance (count of them), its clock pass included
+  // one step of every instance (count of them), its clock pass included; p1 (may be null): the
+  // parameters of a single instance (count 1, its tiles in raster order), passed by value
   void (*fused)(hipStream_t, int tile, int grid, const void* inst, int count, const void* tiles, int ntiles,
-                const void* u1, const void* u2, void* o1, void* o2, const ClockRefs& ck);
+                const void* u1, const void* u2, void* o1, void* o2, const ClockRefs& ck, const void* p1);
 };
 
 template <typename T, int LIM, int RS, bool ISEN>
@@ -370,21 +376,34 @@ struct EnsImpl {
   }
   template <int BX, int BY>
   static void fused_tile(hipStream_t st, int grid, const void* inst, int count, const void* tiles, int ntiles,
-                         const void* u1, const void* u2, void* o1, void* o2, const ClockRefs& ck) {
-    hipLaunchKernelGGL((k_ens_fused<T, LIM, RS, ISEN, BX, BY>), dim3(grid), dim3(BX, BY), 0, st,
-                       (const EnsInst<T>*)inst, (const EnsTile*)tiles, ntiles, (const T4<T>*)u1, (const T4<T>*)u2,
-                       (T4<T>*)o1, (T4<T>*)o2, step_clock<T>(ck), count);
+                         const void* u1, const void* u2, void* o1, void* o2, const ClockRefs& ck, const void* p1) {
+    // (single grid: raster tiles of (BX - 4) x (BY - 4) cells; ntx of them per row)
+    Params<T> P1{};
+    int ntx = 0;
+    if (p1 && count == 1) {
+      P1 = *static_cast<const Params<T>*>(p1);
+      ntx = (P1.n[0] + BX - 5) / (BX - 4);
+    }
+    // every tile its own workgroup and one more for the next step's clock pass
+    if (grid > ntiles) grid = ntiles + 1;
+    if (ntx > 0)
+      hipLaunchKernelGGL((k_ens_fused<T, LIM, RS, ISEN, BX, BY, true>), dim3(grid), dim3(BX, BY), 0, st,
+                         (const EnsInst<T>*)inst, (const EnsTile*)tiles, ntiles, (const T4<T>*)u1, (const T4<T>*)u2,
+                         (T4<T>*)o1, (T4<T>*)o2, step_clock<T>(ck), count, P1, ntx);
+    else
+      hipLaunchKernelGGL((k_ens_fused<T, LIM, RS, ISEN, BX, BY, false>), dim3(grid), dim3(BX, BY), 0, st,
+                         (const EnsInst<T>*)inst, (const EnsTile*)tiles, ntiles, (const T4<T>*)u1, (const T4<T>*)u2,
+                         (T4<T>*)o1, (T4<T>*)o2, step_clock<T>(ck), count, P1, ntx);
   }
   static void fused(hipStream_t st, int tile, int grid, const void* inst, int count, const void* tiles, int ntiles,
-                    const void* u1, const void* u2, void* o1, void* o2, const ClockRefs& ck) {
-    if (grid > ntiles) grid = ntiles;
-    if (grid < 1) return;
+                    const void* u1, const void* u2, void* o1, void* o2, const ClockRefs& ck, const void* p1) {
+    if (grid < 1 || ntiles < 1) return;
     if constexpr (sizeof(T) == 8) {
-      fused_tile<20, 20>(st, grid, inst, count, tiles, ntiles, u1, u2, o1, o2, ck);
+      fused_tile<20, 20>(st, grid, inst, count, tiles, ntiles, u1, u2, o1, o2, ck, p1);
     } else {
       switch (tile) {
-        case ENS_32x32: fused_tile<32, 32>(st, grid, inst, count, tiles, ntiles, u1, u2, o1, o2, ck); break;
-        case ENS_32x16: fused_tile<32, 16>(st, grid, inst, count, tiles, ntiles, u1, u2, o1, o2, ck); break;
+        case ENS_32x32: fused_tile<32, 32>(st, grid, inst, count, tiles, ntiles, u1, u2, o1, o2, ck, p1); break;
+        case ENS_32x16: fused_tile<32, 16>(st, grid, inst, count, tiles, ntiles, u1, u2, o1, o2, ck, p1); break;
         case ENS_ROWS8:
         case ENS_ROWS16:
         case ENS_ROWS32:
@@ -393,7 +412,7 @@ struct EnsImpl {
                              (const EnsInst<float>*)inst, (const EnsTile*)tiles, ntiles, (const float4*)u1,
                              (const float4*)u2, (float4*)o1, (float4*)o2, step_clock<float>(ck), count);
           break;
-        default: fused_tile<20, 20>(st, grid, inst, count, tiles, ntiles, u1, u2, o1, o2, ck); break;
+        default: fused_tile<20, 20>(st, grid, inst, count, tiles, ntiles, u1, u2, o1, o2, ck, p1); break;
       }
     }
   }

@@ -61,9 +61,10 @@ def test_role_tiles_keep_their_occupancy_and_lds(resources):
 
 
 def test_ensemble_default_tile_keeps_three_workgroups_per_cu(resources):
-    # k_ens_fused 32 x 16 threads (28 x 12 cells, 8 waves, 51 KB of LDS): three workgroups per CU
-    # need six waves per SIMD (at most 80 VGPRs); at five the ensemble step measured 7% slower
-    prefix = "_ZN3shx11k_ens_fusedIfLi1ELi0ELb0ELi32ELi16E"
+    # k_ens_fused 32 x 16 threads (28 x 12 cells, 8 waves, 51 KB of LDS) in its ensemble form
+    # (instances from the device list): three workgroups per CU need six waves per SIMD (at most
+    # 80 VGPRs); at five the ensemble step measured 7% slower
+    prefix = "_ZN3shx11k_ens_fusedIfLi1ELi0ELb0ELi32ELi16ELb0E"
     found = [k for k in resources if k["kernel"].startswith(prefix)]
     assert found
     for k in found:
