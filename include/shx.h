@@ -206,6 +206,13 @@ typedef struct {
                           shx_amr_step advances in cycles of 2^K substeps of the fastest block's dt,
                           each leaf stepping 2^k of them at a time (k its bin). PASSES and LEVELS
                           paths only */
+  int regrid_every;    /* 0: the tree changes only through shx_amr_regrid; N > 0: every N steps
+                          (REGRID_EVERY = 8 in the reference, amr_sim.ts:73, :3020-3060)
+                          shx_amr_step measures shx_amr_indicator, forms the next tree with
+                          shx_amr_adapt and re-meshes onto it (that step waits for the readback) */
+  int max_level;       /* the deepest level such a re-mesh refines to (SimConfig.amrLevels) */
+  double refine_above; /* a leaf whose indicator exceeds this splits (SimConfig.refineHi) */
+  double coarsen_below;/* a parent whose children are leaves all below this merges (refineLo) */
 } shx_amr_layout;
 
 /* AmrSim.create (amr_sim.ts:1924-1992 with uploadTree :2566-2625): validate the tree (levels,
@@ -225,8 +232,10 @@ int shx_amr_set_state_f64(shx_amr* sim, const double* u1, const double* u2, size
 int shx_amr_read_state(shx_amr* sim, float* u1, float* u2);
 int shx_amr_read_state_f64(shx_amr* sim, double* u1, double* u2);
 /* AmrSim.step / encodeSteps (amr_sim.ts:2709-2947), single global dt (PER_DX clock): enqueue
- * `count` steps; *taken (may be null) = the steps enqueued (`count`: re-meshing is the caller's,
- * through shx_amr_indicator / shx_amr_regrid, so no step waits on a refinement check). */
+ * `count` steps; *taken (may be null) = the steps enqueued (`count`). With layout regrid_every
+ * = N > 0 every N-th step is followed by a re-mesh (indicator, shx_amr_adapt, shx_amr_regrid),
+ * after which the table, lists and slot count are the new tree's (shx_amr_stats, shx_amr_table);
+ * otherwise re-meshing is the caller's, through shx_amr_indicator / shx_amr_regrid. */
 int shx_amr_step(shx_amr* sim, int count, int* taken);
 int shx_amr_sync(shx_amr* sim);
 int shx_amr_read_clock(shx_amr* sim, double out[11]);
@@ -268,6 +277,17 @@ int shx_amr_stats(shx_amr* sim, int out[5]);
  * the largest |rho_a - rho_b| / min(rho_a, rho_b) between face-adjacent cells of the leaf;
  * NaN for parents. out holds one double per block. */
 int shx_amr_indicator(shx_amr* sim, double* out);
+/* BlockTree.adapt (src/amr/tree.ts:489-641) on the host, no GPU: the next tree from `blocks`
+ * (nblocks x 4 ints, 2:1 balanced, the tree of a sim with root blocks `root` and dimension
+ * ndim 2 or 3) and one indicator per block (NaN or any value for parents): leaves above
+ * refine_above split (below max_level), parents whose children are leaves all below
+ * coarsen_below merge, then 2:1 balance across faces is restored by splitting (periodic[a]
+ * != 0: axis a wraps). Writes the blocks, parents first by level then z, y, x, into out (cap
+ * rows of 4 ints) and their count into *nout; every block lies within one level of the given
+ * tree. SHX_EINVAL if out is too small (then *nout is the count needed). */
+int shx_amr_adapt(const int* blocks, int nblocks, const int root[3], int ndim, const double* indicator,
+                  double refine_above, double coarsen_below, int max_level, const int periodic[3], int* out,
+                  int cap, int* nout);
 /* Re-mesh (AmrSim.applyRegrid, src/amr/amr_sim.ts:2648-2697, amr/regrid.wgsl:40-139): replace
  * the tree by `blocks` (nblocks x 4 ints as for shx_amr_create, 2:1 balanced), which must lie
  * within one level of the current tree everywhere: each new leaf is a current leaf (copied), a

@@ -37,7 +37,9 @@ class ShxConfig(C.Structure):
 
 
 class ShxAmrLayout(C.Structure):
-    _fields_ = [("root", C.c_int * 3), ("path", C.c_int), ("compensated", C.c_int), ("time_bins", C.c_int)]
+    _fields_ = [("root", C.c_int * 3), ("path", C.c_int), ("compensated", C.c_int), ("time_bins", C.c_int),
+                ("regrid_every", C.c_int), ("max_level", C.c_int), ("refine_above", C.c_double),
+                ("coarsen_below", C.c_double)]
 
 
 _lib = None
@@ -103,6 +105,8 @@ def load():
         "shx_amr_flux_mismatch": (I, [P, C.POINTER(C.c_int64), D]),
         "shx_amr_indicator": (I, [P, C.POINTER(C.c_double)]),
         "shx_amr_regrid": (I, [P, C.POINTER(I), I]),
+        "shx_amr_adapt": (I, [C.POINTER(I), I, C.POINTER(I), I, C.POINTER(C.c_double), C.c_double, C.c_double, I,
+                              C.POINTER(I), C.POINTER(I), I, C.POINTER(I)]),
         "shx_ens_create": (I, [C.POINTER(ShxConfig), I, C.POINTER(I), C.POINTER(P)]),
         "shx_ens_destroy": (I, [P]),
         "shx_ens_set_state": (I, [P, I, F, F, C.c_size_t]),

@@ -28,13 +28,39 @@ def _level0(root, ndim):
     return (BS * root[0], BS * root[1], 1 if ndim == 2 else BS * root[2])
 
 
+def adapt_tree(blocks, root, indicator, refine_above, coarsen_below, max_level, periodic=(True, True, True),
+               ndim=3):
+    """BlockTree.adapt (src/amr/tree.ts:489-641) in libshx (shx_amr_adapt, host code, no GPU):
+    the next tree from `blocks` and one indicator per block, as amr_tree.adapt_tree."""
+    lib = _lib.load()
+    blocks = np.ascontiguousarray(blocks, dtype=np.int32).reshape(-1, 4)
+    ind = np.ascontiguousarray(np.nan_to_num(np.asarray(indicator, np.float64), nan=0.0))
+    r = (C.c_int * 3)(*[int(v) for v in root])
+    per = (C.c_int * 3)(*[int(bool(v)) for v in periodic])
+    n = C.c_int(0)
+    cap = max(4 * len(blocks), 64)
+    while True:
+        out = np.zeros((cap, 4), np.int32)
+        rc = lib.shx_amr_adapt(blocks.ctypes.data_as(C.POINTER(C.c_int)), len(blocks), r, int(ndim),
+                               ind.ctypes.data_as(C.POINTER(C.c_double)), float(refine_above), float(coarsen_below),
+                               int(max_level), per, out.ctypes.data_as(C.POINTER(C.c_int)), cap, C.byref(n))
+        if rc == 0:
+            return out[:n.value].copy()
+        if n.value <= cap:
+            check(rc)
+        cap = n.value
+
+
 class AmrSim:
     """A block-AMR hydro run on one GPU. `cfg` describes the level-0 grid (n = 16 x root blocks,
     dx = level-0 cell size; cfg.ndim 2 for a quadtree, root[2] = 1); `blocks` is (nblocks, 4) of
-    (level, ix, iy, iz)."""
+    (level, ix, iy, iz). regrid_every = N > 0 re-meshes every N steps inside step() (the
+    indicator, adapt_tree with refine_above / coarsen_below / max_level, regrid), as the
+    reference's AmrSim does every REGRID_EVERY = 8 steps; self.blocks follows the tree."""
 
     def __init__(self, cfg: SimConfig, blocks, root, path: str = "fused", compensated: bool = True,
-                 time_bins: int = 0):
+                 time_bins: int = 0, regrid_every: int = 0, max_level: int = 0, refine_above: float = 0.8,
+                 coarsen_below: float = 0.2):
         lib = _lib.load()
         root = tuple(int(r) for r in root)
         self.ndim = 2 if cfg.ndim == 2 else 3
@@ -50,7 +76,12 @@ class AmrSim:
         lay.path = _lib.AMR_PATHS[path]
         lay.compensated = int(bool(compensated))
         lay.time_bins = int(time_bins)
+        lay.regrid_every = int(regrid_every)
+        lay.max_level = int(max_level)
+        lay.refine_above = float(refine_above)
+        lay.coarsen_below = float(coarsen_below)
         self.time_bins = int(time_bins)
+        self.regrid_every = int(regrid_every)
         self._h = C.c_void_p()
         c = self.cfg.to_c()
         check(lib.shx_amr_create(C.byref(c), C.byref(lay), self.blocks.ctypes.data_as(C.POINTER(C.c_int)),
@@ -154,10 +185,20 @@ class AmrSim:
 
     # -- stepping --
     def step(self, count: int = 1) -> int:
-        """Enqueue `count` steps (encodeSteps); returns the steps taken."""
+        """Enqueue `count` steps (encodeSteps); returns the steps taken. With regrid_every the
+        tree may change on the way: self.blocks is read back afterwards."""
         n = C.c_int(0)
         check(_lib.load().shx_amr_step(self._h, int(count), C.byref(n)))
+        if self.regrid_every > 0:
+            self._refresh_tree()
         return n.value
+
+    def _refresh_tree(self):
+        nb = self.stats()["blocks"]
+        if nb != self.nblocks:
+            self.nblocks = nb
+            self.ncell = nb * self.cells
+        self.blocks = np.ascontiguousarray(self.table()[:, :4])
 
     def sync(self):
         check(_lib.load().shx_amr_sync(self._h))
@@ -193,6 +234,8 @@ class AmrSim:
         """AmrSim.runToEnd: step until the clock holds at cfg.t_end; the clock's step count then."""
         n = C.c_int(0)
         check(_lib.load().shx_amr_run_to_end(self._h, batch, max_steps, C.byref(n)))
+        if self.regrid_every > 0:
+            self._refresh_tree()
         return n.value
 
     def _reconfigure(self, fn, cfg):

@@ -7,8 +7,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "amr.hpp"
@@ -91,6 +93,7 @@ struct shx_amr {
   std::vector<int> all_host;   // the list of every block, as d_lists holds it
   std::vector<int> blist_host; // the last cycle's lists
   long long bin_cycles = 0, bin_updates = 0, bin_single = 0;
+  int since_regrid = 0;        // steps since the last automatic re-mesh (layout regrid_every)
 };
 
 namespace {
@@ -181,6 +184,131 @@ int build_table(shx_amr* s) {
   }
   s->nlevels = maxlev + 1;
   return SHX_OK;
+}
+
+// BlockTree.adapt (tree.ts:489-641), the policy of amr_tree.adapt_tree: split the leaves above
+// `hi` (below max_level), merge the parents whose leaf children are all below `lo`, rebuild the
+// tree level by level from the roots with those decisions, then split whatever 2:1 face
+// balance needs (the balanced closure is unique, so the order of the splits does not matter).
+// out: (level, x, y, z) rows sorted by level, z, y, x.
+void adapt_tree(const int* blocks, int n, const int root[3], int ndim, const double* ind, double hi, double lo,
+                int max_level, const int periodic[3], std::vector<int>& out) {
+  const int nchild = 1 << ndim;
+  std::unordered_map<uint64_t, int> at;
+  at.reserve(2 * n);
+  for (int i = 0; i < n; i++) at[key_of(blocks[4 * i], blocks[4 * i + 1], blocks[4 * i + 2], blocks[4 * i + 3])] = i;
+  auto child = [&](int L, int x, int y, int z, int o, int c[3]) {
+    c[0] = 2 * x + (o & 1);
+    c[1] = 2 * y + ((o >> 1) & 1);
+    c[2] = ndim == 3 ? 2 * z + ((o >> 2) & 1) : 0;
+    (void)L;
+  };
+  auto is_leaf = [&](int i) {
+    const int* b = &blocks[4 * i];
+    for (int o = 0; o < nchild; o++) {
+      int c[3];
+      child(b[0], b[1], b[2], b[3], o, c);
+      if (at.count(key_of(b[0] + 1, c[0], c[1], c[2]))) return false;
+    }
+    return true;
+  };
+  std::unordered_set<uint64_t> keep;   // blocks of the current tree that stay refined or split
+  for (int i = 0; i < n; i++) {
+    const int* b = &blocks[4 * i];
+    const uint64_t k = key_of(b[0], b[1], b[2], b[3]);
+    if (is_leaf(i)) {
+      if (b[0] < max_level && ind[i] > hi) keep.insert(k);
+      continue;
+    }
+    bool merge = true;
+    for (int o = 0; o < nchild && merge; o++) {
+      int c[3];
+      child(b[0], b[1], b[2], b[3], o, c);
+      auto it = at.find(key_of(b[0] + 1, c[0], c[1], c[2]));
+      merge = it != at.end() && is_leaf(it->second) && ind[it->second] < lo;
+    }
+    if (!merge) keep.insert(k);
+  }
+  // (level, x, y, z) of the new tree's leaves and refined blocks
+  std::unordered_set<uint64_t> leaves, refined;
+  auto split = [&](int L, int x, int y, int z) {
+    const uint64_t k = key_of(L, x, y, z);
+    if (!leaves.erase(k)) return;
+    refined.insert(k);
+    for (int o = 0; o < nchild; o++) {
+      int c[3];
+      child(L, x, y, z, o, c);
+      leaves.insert(key_of(L + 1, c[0], c[1], c[2]));
+    }
+  };
+  auto unkey = [](uint64_t k, int& L, int& x, int& y, int& z) {
+    L = (int)(k >> 60);
+    x = (int)((k >> 40) & 0xFFFFF);
+    y = (int)((k >> 20) & 0xFFFFF);
+    z = (int)(k & 0xFFFFF);
+  };
+  for (int z = 0; z < root[2]; z++)
+    for (int y = 0; y < root[1]; y++)
+      for (int x = 0; x < root[0]; x++) leaves.insert(key_of(0, x, y, z));
+  for (int L = 0; L < max_level; L++) {
+    std::vector<uint64_t> want;
+    for (uint64_t k : leaves)
+      if ((int)(k >> 60) == L && keep.count(k)) want.push_back(k);
+    for (uint64_t k : want) {
+      int l, x, y, z;
+      unkey(k, l, x, y, z);
+      split(l, x, y, z);
+    }
+  }
+  for (bool changed = true; changed;) {
+    changed = false;
+    std::vector<uint64_t> snap(leaves.begin(), leaves.end());
+    for (uint64_t k : snap) {
+      if (!leaves.count(k)) continue;
+      int L, c0, c1, c2;
+      unkey(k, L, c0, c1, c2);
+      if (L == 0) continue;
+      for (int a = 0; a < ndim; a++) {
+        const int nn = root[a] << L;
+        for (int step = -1; step <= 1; step += 2) {
+          int d[3] = {c0, c1, c2};
+          int v = d[a] + step;
+          if (v < 0 || v >= nn) {
+            if (!periodic[a]) continue;
+            v = (v + nn) % nn;
+          }
+          d[a] = v;
+          const uint64_t dk = key_of(L, d[0], d[1], d[2]);
+          if (leaves.count(dk) || refined.count(dk)) continue;
+          if (leaves.count(key_of(L - 1, d[0] >> 1, d[1] >> 1, d[2] >> 1))) continue;
+          for (int up = 2; up <= L; up++) {
+            const uint64_t q = key_of(L - up, d[0] >> up, d[1] >> up, d[2] >> up);
+            if (leaves.count(q)) {
+              split(L - up, d[0] >> up, d[1] >> up, d[2] >> up);
+              changed = true;
+              break;
+            }
+          }
+        }
+      }
+    }
+  }
+  std::vector<std::array<int, 4>> rows;
+  rows.reserve(leaves.size() + refined.size());
+  for (const auto* set : {&refined, &leaves})
+    for (uint64_t k : *set) {
+      std::array<int, 4> r;
+      unkey(k, r[0], r[1], r[2], r[3]);
+      rows.push_back(r);
+    }
+  std::sort(rows.begin(), rows.end(), [](const std::array<int, 4>& a, const std::array<int, 4>& b) {
+    if (a[0] != b[0]) return a[0] < b[0];
+    if (a[3] != b[3]) return a[3] < b[3];
+    if (a[2] != b[2]) return a[2] < b[2];
+    return a[1] < b[1];
+  });
+  out.clear();
+  for (const auto& r : rows) out.insert(out.end(), r.begin(), r.end());
 }
 
 int build_lists(shx_amr* s, std::vector<int>& lists) {
@@ -625,6 +753,8 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
     return fail(SHX_EINVAL, "time_bins %d (0 to %d)", layout->time_bins, AMR_MAX_BINS);
   if (layout->time_bins > 0 && layout->path == SHX_AMR_PATH_FUSED)
     return fail(SHX_EINVAL, "per-block timesteps run on the passes and levels paths");
+  if (layout->regrid_every < 0 || (layout->regrid_every > 0 && (layout->max_level < 0 || layout->max_level >= AMR_MAXLEV)))
+    return fail(SHX_EINVAL, "regrid_every %d, max_level %d (0 to %d)", layout->regrid_every, layout->max_level, AMR_MAXLEV - 1);
   AmrOps ops = amr_ops(c.precision, c.limiter, c.riemann, c.isentropic ? 1 : 0, c.ndim);
   if (!ops.assemble) return fail(SHX_EINVAL, "no block kernels built for precision %d limiter %d riemann %d isentropic %d", c.precision, c.limiter, c.riemann, c.isentropic);
   int ndev = 0;
@@ -812,9 +942,24 @@ int shx_amr_set_state_f64(shx_amr* s, const double* u1, const double* u2, size_t
 int shx_amr_read_state(shx_amr* s, float* u1, float* u2) { return read_state_t<float>(s, u1, u2); }
 int shx_amr_read_state_f64(shx_amr* s, double* u1, double* u2) { return read_state_t<double>(s, u1, u2); }
 
-int shx_amr_step(shx_amr* s, int count, int* taken) {
-  if (taken) *taken = 0;
-  if (!s || count < 0) return fail(SHX_EINVAL, "bad argument");
+// The automatic re-mesh of layout regrid_every (AmrSim's flag / adapt / applyRegrid cycle,
+// amr_sim.ts:3020-3060 and :2648-2697): the indicator of every leaf, the next tree, and the
+// re-mesh when the tree changes. Axes wrap for the balance where both faces are periodic.
+static int auto_regrid(shx_amr* s) {
+  std::vector<double> ind(s->nblocks);
+  int rc = shx_amr_indicator(s, ind.data());
+  if (rc) return rc;
+  int periodic[3];
+  for (int a = 0; a < 3; a++) periodic[a] = s->cfg.bc_lo[a] == SHX_BC_PERIODIC && s->cfg.bc_hi[a] == SHX_BC_PERIODIC;
+  std::vector<int> next;
+  adapt_tree(s->blocks.data(), s->nblocks, s->lay.root, s->ndim, ind.data(), s->lay.refine_above,
+             s->lay.coarsen_below, std::min(s->lay.max_level, AMR_MAXLEV - 1), periodic, next);
+  if (next == s->blocks) return SHX_OK;
+  return shx_amr_regrid(s, next.data(), (int)(next.size() / 4));
+}
+
+// `count` steps (whole cycles with per-block timesteps) with no re-mesh between them
+static int enqueue_steps(shx_amr* s, int count, int* taken) {
   if (s->kbins > 0) {
     // whole cycles only, as AmrSim.encodeSteps (amr_sim.ts:2783-2790): may run past `count`
     const int n = 1 << s->kbins;
@@ -830,6 +975,45 @@ int shx_amr_step(shx_amr* s, int count, int* taken) {
     s->cur = 1 - s->cur;
     if (taken) *taken = i + 1;
   }
+  return SHX_OK;
+}
+
+int shx_amr_step(shx_amr* s, int count, int* taken) {
+  if (taken) *taken = 0;
+  if (!s || count < 0) return fail(SHX_EINVAL, "bad argument");
+  if (s->lay.regrid_every <= 0) return enqueue_steps(s, count, taken);
+  // steps up to each re-mesh, then the re-mesh (the sim may be a new tree's afterwards)
+  int done = 0;
+  while (done < count) {
+    const int n = std::min(count - done, s->lay.regrid_every - s->since_regrid);
+    int part = 0;
+    int rc = enqueue_steps(s, n, &part);
+    done += part;
+    if (taken) *taken = done;
+    if (rc) return rc;
+    s->since_regrid += part;
+    if (s->since_regrid >= s->lay.regrid_every) {
+      s->since_regrid = 0;
+      rc = auto_regrid(s);
+      if (rc) return rc;
+    }
+  }
+  return SHX_OK;
+}
+
+int shx_amr_adapt(const int* blocks, int nblocks, const int root[3], int ndim, const double* indicator,
+                  double refine_above, double coarsen_below, int max_level, const int periodic[3], int* out, int cap,
+                  int* nout) {
+  if (!blocks || nblocks < 1 || !root || !indicator || !periodic || !nout || (ndim != 2 && ndim != 3))
+    return fail(SHX_EINVAL, "bad argument");
+  if (max_level < 0 || max_level >= AMR_MAXLEV) return fail(SHX_EINVAL, "max_level %d outside [0, %d)", max_level, AMR_MAXLEV);
+  for (int a = 0; a < 3; a++)
+    if (root[a] < 1 || (ndim == 2 && a == 2 && root[a] != 1)) return fail(SHX_EINVAL, "root[%d] = %d", a, root[a]);
+  std::vector<int> next;
+  adapt_tree(blocks, nblocks, root, ndim, indicator, refine_above, coarsen_below, max_level, periodic, next);
+  *nout = (int)(next.size() / 4);
+  if (!out || cap < *nout) return fail(SHX_EINVAL, "the next tree has %d blocks (room for %d)", *nout, cap);
+  memcpy(out, next.data(), next.size() * sizeof(int));
   return SHX_OK;
 }
 

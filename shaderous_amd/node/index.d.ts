@@ -39,6 +39,11 @@ export interface HydroConfig {
   fluff?: [number, number, number] | { rho: number; u: number; v: number; va?: number };
   /** per-block timesteps on blocks (SimConfig.timeBins) */
   timeBins?: number;
+  /** levels of refinement below the root grid; > 0 turns on the block grid's re-mesh cadence */
+  amrLevels?: number;
+  /** the re-mesh's indicator thresholds (SimConfig.refineHi, refineLo) */
+  refineHi?: number;
+  refineLo?: number;
   precision?: 32 | 64;
   path?: 'fused' | 'plain';
   device?: number;
@@ -105,6 +110,13 @@ export interface AmrTree {
   compensated?: boolean;
   /** per-block timesteps: cycles of 2^timeBins substeps (default cfg.timeBins; passes path) */
   timeBins?: number;
+  /** re-mesh every N steps inside the steps (default 8 where cfg.amrLevels > 0, else 0) */
+  regridEvery?: number;
+  /** the deepest level a re-mesh refines to (default cfg.amrLevels) */
+  maxLevel?: number;
+  /** indicator thresholds of a re-mesh (default cfg.refineHi, cfg.refineLo: 0.8, 0.2) */
+  refineAbove?: number;
+  refineBelow?: number;
 }
 
 /** The reference's Simulation (src/sim.ts:465-627) as this backend serves it. */

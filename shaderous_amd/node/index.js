@@ -483,7 +483,7 @@ class Sim extends Base {
  * cfg.ndim 2: a quadtree of 16^2-cell blocks (root[2] = 1, every iz 0) on the passes paths.
  */
 class AmrSim extends Base {
-  constructor(cfg, { root, blocks, path, compensated = true, timeBins }) {
+  constructor(cfg, { root, blocks, path, compensated = true, timeBins, regridEvery, maxLevel, refineAbove, refineBelow }) {
     super(cfg, {
       step: native.amrStep,
       sync: native.amrSync,
@@ -500,11 +500,46 @@ class AmrSim extends Base {
     // path (the fused march takes one dt and steps 3D blocks)
     const bins = Math.max(0, Math.min(5, Math.floor(timeBins !== undefined ? timeBins : cfg.timeBins || 0)));
     const kind = path !== undefined ? path : bins > 0 || this.ndim === 2 ? 'passes' : 'fused';
-    this.h = native.amrCreate(ncfg, { root, path: code(AMR_PATHS, kind, 'AMR path'), compensated, timeBins: bins }, blocks);
+    // the re-mesh cadence (REGRID_EVERY = 8, src/amr/amr_sim.ts:73): on by default where the
+    // reference's config asks for levels of refinement (amrLevels), with its thresholds
+    // (refineHi, refineLo) applied to this backend's indicator
+    const levels = cfg.amrLevels || 0;
+    const every = regridEvery !== undefined ? regridEvery : levels > 0 ? 8 : 0;
+    const lay = {
+      root,
+      path: code(AMR_PATHS, kind, 'AMR path'),
+      compensated,
+      timeBins: bins,
+      regridEvery: every,
+      maxLevel: maxLevel !== undefined ? maxLevel : levels,
+      refineAbove: refineAbove !== undefined ? refineAbove : cfg.refineHi !== undefined ? cfg.refineHi : 0.8,
+      coarsenBelow: refineBelow !== undefined ? refineBelow : cfg.refineLo !== undefined ? cfg.refineLo : 0.2,
+    };
+    this.regridEvery = every;
+    this.h = native.amrCreate(ncfg, lay, blocks);
     this.root = root.slice(0, 3);
     this.dx0 = ncfg.dx;
     this.tree(blocks);
     if (cfg.initialState) this.reset();
+  }
+
+  /** Steps as Simulation.encodeSteps; a sim that re-meshes by itself reads its new tree back. */
+  encodeSteps(enc, count, mayHold = false, speculate = 0) {
+    const taken = super.encodeSteps(enc, count, mayHold, speculate);
+    if (this.regridEvery > 0) this.syncTree();
+    return taken;
+  }
+
+  /** The current tree's rows from the table (after a re-mesh made by the steps). */
+  syncTree() {
+    const nb = native.amrStats(live(this.h))[0];
+    this.nblocks = nb;
+    const t = native.amrTable(live(this.h));
+    const rows = new Int32Array(4 * nb);
+    for (let i = 0; i < nb; i++) for (let k = 0; k < 4; k++) rows[4 * i + k] = t[32 * i + k];
+    // the state reset() loads was of the old tree: dropped once the tree changed
+    if (rows.length !== this.blocks.length || rows.some((v, i) => v !== this.blocks[i])) this.initial = null;
+    this.tree(rows);
   }
 
   /** The tree's rows, its finest cells per side (`n`, `nz`: AmrSim's, src/amr/amr_sim.ts:184). */
@@ -596,7 +631,9 @@ class AmrSim extends Base {
   /** AmrSim.runToEnd (src/amr/amr_sim.ts:3749-3775): the clock's step count once it holds at tEnd. */
   async runToEnd(batch = 16, maxSteps = 1e6) {
     if (!Number.isFinite(this.cfg.tEnd)) throw new Error('runToEnd needs a finite tEnd');
-    return native.runToEnd(live(this.h), batch, Math.min(maxSteps, 0x7fffffff));
+    const n = native.runToEnd(live(this.h), batch, Math.min(maxSteps, 0x7fffffff));
+    if (this.regridEvery > 0) this.syncTree();
+    return n;
   }
 
   /** AmrSim.writeParams (src/amr/amr_sim.ts:1677): this.cfg's run-time fields, from the next step on. */

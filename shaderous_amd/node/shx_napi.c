@@ -342,6 +342,17 @@ static napi_value js_sync(napi_env env, napi_callback_info info) {
   return undefined(env);
 }
 
+/* After steps on a block grid that re-meshes by itself (layout regrid_every): the slot count
+   of the current tree, which sizes the table and the state. */
+static int amr_resync(Handle* h) {
+  int st[5];
+  int rc = shx_amr_stats((shx_amr*)h->p, st);
+  if (rc != SHX_OK) return rc;
+  h->nblocks = st[0];
+  h->ncell = (size_t)h->nblocks * h->cells;
+  return SHX_OK;
+}
+
 static napi_value js_run_to_end(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   if (!get_args(env, info, 3, argv)) return NULL;
@@ -351,8 +362,12 @@ static napi_value js_run_to_end(napi_env env, napi_callback_info info) {
   NAPI_CALL(env, napi_get_value_int32(env, argv[1], &batch));
   NAPI_CALL(env, napi_get_value_int32(env, argv[2], &max_steps));
   int steps = 0;
-  if (h->amr) SHX_CALL(env, shx_amr_run_to_end((shx_amr*)h->p, batch, max_steps, &steps));
-  else SHX_CALL(env, shx_run_to_end((shx_sim*)h->p, batch, max_steps, &steps));
+  if (h->amr) {
+    SHX_CALL(env, shx_amr_run_to_end((shx_amr*)h->p, batch, max_steps, &steps));
+    SHX_CALL(env, amr_resync(h));
+  } else {
+    SHX_CALL(env, shx_run_to_end((shx_sim*)h->p, batch, max_steps, &steps));
+  }
   napi_value out;
   NAPI_CALL(env, napi_create_int32(env, steps, &out));
   return out;
@@ -529,6 +544,10 @@ static napi_value js_amr_create(napi_env env, napi_callback_info info) {
   lay.path = (int)num_prop(env, argv[1], "path", SHX_AMR_PATH_FUSED);
   lay.compensated = (int)num_prop(env, argv[1], "compensated", 1);
   lay.time_bins = (int)num_prop(env, argv[1], "timeBins", 0);
+  lay.regrid_every = (int)num_prop(env, argv[1], "regridEvery", 0);
+  lay.max_level = (int)num_prop(env, argv[1], "maxLevel", 0);
+  lay.refine_above = num_prop(env, argv[1], "refineAbove", 0.8);
+  lay.coarsen_below = num_prop(env, argv[1], "coarsenBelow", 0.2);
   bool is = false;
   napi_is_typedarray(env, argv[2], &is);
   if (!is) { napi_throw_type_error(env, NULL, "blocks is an Int32Array of (level, ix, iy, iz) rows"); return NULL; }
@@ -607,6 +626,7 @@ static napi_value js_amr_step(napi_env env, napi_callback_info info) {
   NAPI_CALL(env, napi_get_value_int32(env, argv[1], &count));
   int taken = 0;
   SHX_CALL(env, shx_amr_step((shx_amr*)h->p, count, &taken));
+  SHX_CALL(env, amr_resync(h));
   napi_value out;
   NAPI_CALL(env, napi_create_int32(env, taken, &out));
   return out;

@@ -73,7 +73,7 @@ async function main() {
   let sim;
   if (job.kind === 'amr') {
     const blocks = Int32Array.from(job.tree.blocks);
-    sim = shx.createSim(cfg, { root: job.tree.root, blocks, path: job.tree.path || 'passes', timeBins: job.tree.timeBins });
+    sim = shx.createSim(cfg, { ...job.tree, blocks, path: job.tree.path || 'passes' });
   } else {
     sim = shx.createSim(cfg);
   }
@@ -106,6 +106,7 @@ async function main() {
     fs.writeFileSync(path.join(dir, 'o1.bin'), Buffer.from(st.u1.buffer));
     fs.writeFileSync(path.join(dir, 'o2.bin'), Buffer.from(st.u2.buffer));
     out.clock = Array.from(sim.clock());
+    if (job.kind === 'amr') out.blocks = Array.from(sim.blocks);
     out.d1 = d1;
     if (job.probe) out.probe = await sim.probe(job.probe);
     if (job.kind !== 'amr') out.flux = await sim.fluxMismatch();

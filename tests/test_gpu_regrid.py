@@ -154,3 +154,44 @@ def test_adaptive_run_paths_agree_and_conserve():
     assert out["fused"][2] > 0
     assert np.array_equal(out["fused"][0], out["passes"][0])
     assert np.array_equal(out["fused"][1][0], out["passes"][1][0]) and np.array_equal(out["fused"][1][1], out["passes"][1][1])
+
+
+@pytest.mark.parametrize("path", ["fused", "passes"])
+def test_auto_regrid_equals_the_callers_cadence(path):
+    """The re-mesh cadence inside the steps (layout regrid_every, the reference's REGRID_EVERY):
+    a sim stepping 12 steps with regrid_every 4 ends on the same tree, with the same bits, as one
+    whose caller steps 4 at a time and adapts from the indicator after each (AmrSim.adapt: the
+    Python policy, which tests/test_adapt_native.py holds equal to the library's)."""
+    blocks, dx0 = amr_tree.blast_tree(root=4, max_level=2, radius=0.2, center=(0.03, -0.02, 0.05))
+    cfg = cfg_for((4, 4, 4))
+
+    def blast(x, y, z):
+        r = np.sqrt((x - 0.03) ** 2 + (y + 0.02) ** 2 + (z - 0.05) ** 2)
+        return {"rho": 1.0 + 0.0 * x, "vx": 0.0 * x, "vy": 0.0 * x, "vz": 0.0 * x, "p": np.where(r < 0.2, 100.0, 1.0)}
+    u1, u2 = amr_tree.blocks_state(blocks, dx0, blast, cfg.gamma)
+    print(f"\n[gpu] auto regrid {path}", flush=True)
+    auto = AmrSim(cfg, blocks, (4, 4, 4), path=path, regrid_every=4, max_level=2, refine_above=0.05,
+                  coarsen_below=0.01)
+    hand = AmrSim(cfg, blocks, (4, 4, 4), path=path)
+    try:
+        auto.set_state(u1, u2)
+        hand.set_state(u1, u2)
+        assert auto.step(10) == 10                       # re-meshes after steps 4 and 8
+        assert auto.step(2) == 2                         # and 12
+        changes = 0
+        for _ in range(3):
+            hand.step(4)
+            changes += hand.adapt(0.05, 0.01, 2)
+        a = auto.read_state()
+        h = hand.read_state()
+        assert changes > 0 and not np.array_equal(hand.blocks, blocks)
+        assert np.array_equal(auto.blocks, hand.blocks) and np.array_equal(auto.table(), hand.table())
+        assert np.array_equal(a[0], h[0]) and np.array_equal(a[1], h[1])
+        assert np.array_equal(auto.clock(), hand.clock())
+        auto.step(3)                                     # within the next period: no re-mesh
+        hand.step(3)
+        assert np.array_equal(auto.blocks, hand.blocks)
+        assert np.array_equal(auto.read_state()[0], hand.read_state()[0])
+    finally:
+        auto.destroy()
+        hand.destroy()

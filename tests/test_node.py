@@ -449,3 +449,37 @@ def test_node_amr_2d_matches_python(tmp_path):
     assert idx == [int(0.44 * (64 << 2)), int(0.45 * (64 << 2)), 0] and (bx, by) == (idx[0] // 16, idx[1] // 16)
     cell = pr["block"]["slot"] * 256 + (idx[0] - 16 * bx) + 16 * (idx[1] - 16 * by)
     assert np.array_equal(np.float32(pr["u1"]), p1[cell]) and np.array_equal(np.float32(pr["u2"]), p2[cell])
+
+
+@pytest.mark.gpu
+def test_node_amr_regrid_cadence_matches_python(tmp_path):
+    """The re-mesh cadence through the facade (tree.regridEvery, on by default where the
+    reference's config has amrLevels): the Node caller's 12 steps re-mesh as the ctypes caller's
+    sim with regrid_every does, to the same tree and bits, and the facade follows the tree."""
+    from shaderous_amd.amr import AmrSim
+    from shaderous_amd.sim import SimConfig
+    blocks, dx0 = amr_tree.blast_tree(root=4, max_level=2, radius=0.2, center=(0.03, -0.02, 0.05))
+
+    def blast(x, y, z):
+        r = np.sqrt((x - 0.03) ** 2 + (y + 0.02) ** 2 + (z - 0.05) ** 2)
+        return {"rho": 1.0 + 0.0 * x, "p": np.where(r < 0.2, 100.0, 1.0)}
+    u1, u2 = amr_tree.blocks_state(blocks, dx0, blast, 1.4)
+    job = {"kind": "amr", "steps": 12, "cfg": {"n": 64, "ndim": 3, "limiter": "mc", "isentropicRecon": False,
+                                               "cfl": 0.4, "box": 1.0, "amrLevels": 2, "eosParams": {"gamma": 1.4}},
+           "tree": {"root": [4, 4, 4], "blocks": blocks.reshape(-1).tolist(), "path": "fused", "regridEvery": 4,
+                    "refineAbove": 0.05, "refineBelow": 0.01}}
+    o1, o2, out = _run(tmp_path, job, u1, u2)
+    cfg = SimConfig(n=(64, 64, 64), dx=1.0 / 64, gamma=1.4, limiter="mc", isentropic=False, cfl=0.4)
+    sim = AmrSim(cfg, blocks, (4, 4, 4), path="fused", regrid_every=4, max_level=2, refine_above=0.05,
+                 coarsen_below=0.01)
+    try:
+        sim.set_state(u1, u2)
+        sim.step(12)
+        p1, p2 = sim.read_state()
+        tree = sim.blocks.copy()
+    finally:
+        sim.destroy()
+    assert len(tree) != len(blocks) or not np.array_equal(tree, blocks)
+    assert np.array_equal(np.asarray(out["blocks"], np.int32).reshape(-1, 4), tree)
+    leaf = amr_tree.leaf_mask(tree).repeat(4096)
+    assert np.array_equal(o1[leaf], p1[leaf]) and np.array_equal(o2[leaf], p2[leaf])
