@@ -292,7 +292,7 @@ int shx_amr_adapt(const int* blocks, int nblocks, const int root[3], int ndim, c
  * the tree by `blocks` (nblocks x 4 ints as for shx_amr_create, 2:1 balanced), which must lie
  * within one level of the current tree everywhere: each new leaf is a current leaf (copied), a
  * child of one (prolonged: conservative, limited, the children's mean energy kept) or a current
- * parent of eight leaves (their mean). The clock carries on; table, lists and graphs are rebuilt.
+ * parent of eight leaves, four in 2D (their mean). The clock carries on; table, lists and graphs are rebuilt.
  * On failure the sim is unchanged. */
 int shx_amr_regrid(shx_amr* sim, const int* blocks, int nblocks);
 
