@@ -186,31 +186,33 @@ def cpu_baseline_steps(pb, steps=100):
     return res
 
 
-def cpu_baseline_amr(blocks, root, dx0, u1, u2, cfg):
-    """The block step on the CPU: one step of the whole blast through the C restatement of the
-    block update the GPU runs (predict, faces, coarse-fine means and update of
+def cpu_baseline_amr(blocks, root, dx0, u1, u2, cfg, ndim=3, name="blast", steps=1):
+    """The block step on the CPU: `steps` steps of the whole tree through the C restatement of
+    the block update the GPU runs (predict, faces, coarse-fine means and update of
     src/amr/predict.wgsl and update.wgsl; FP32, OpenMP over blocks). The reference's own CPU
     path (RefHydro) has no blocks, so this restatement stands in for it here."""
     import numpy as np
     import oracle
     cores, model = host_cpu()
     oracle.RefHydro.set_threads(cores)
-    t = oracle.AmrTable(blocks, root)
-    prm = oracle.params(cfg.n, ndim=3, dx=dx0, gamma=cfg.gamma, cfl=cfg.cfl, limiter=cfg.limiter,
+    t = oracle.AmrTable(blocks, root, ndim=ndim)
+    prm = oracle.params(cfg.n, ndim=ndim, dx=dx0, gamma=cfg.gamma, cfl=cfg.cfl, limiter=cfg.limiter,
                         isentropic=cfg.isentropic)
     a1, a2 = u1.copy(), u2.copy()
+    clk = np.zeros(4, np.float32)
     t0 = time.perf_counter()
-    oracle.wgsl32.amr_step(prm, t, dx0, a1, a2, np.zeros(4, np.float32))
+    for _ in range(steps):
+        oracle.wgsl32.amr_step(prm, t, dx0, a1, a2, clk)
     dt = time.perf_counter() - t0
-    cells = int(oracle_leaf_cells(blocks))
+    cells = int(oracle_leaf_cells(blocks, ndim)) * steps
     return {"value": cells / dt, "unit": "zone-updates/s", "cores": cores, "cpu_model": model, "kind": "port",
-            "sample": f"one step of the whole blast ({cells} leaf zone-updates) through the C restatement of the "
-                      f"block step (FP32, {cores} OpenMP threads over blocks), {dt:.2f} s"}
+            "sample": f"{steps} step(s) of the whole {name} ({cells} leaf zone-updates) through the C restatement of "
+                      f"the block step (FP32, {cores} OpenMP threads over blocks), {dt:.2f} s"}
 
 
-def oracle_leaf_cells(blocks):
+def oracle_leaf_cells(blocks, ndim=3):
     from shaderous_amd import amr_tree
-    return amr_tree.leaf_mask(blocks).sum() * 4096
+    return amr_tree.leaf_mask(blocks).sum() * 16 ** ndim
 
 
 def cpu_baseline_ensemble(ns, states, cfg):
@@ -481,13 +483,17 @@ def kh_amr_bench(args):
                      "per_step": step_roofline(cells, prof["step"])}
         log(f"[bench] KH adaptive {path}: {ms:.4f} ms/step, {cells * args.kh_amr_steps / t:.3e} upd/s, {prof}")
     pub = PUBLISHED[("kh_amr", 4096)]
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_amr(blocks, (16, 16, 1), dx0, u1, u2, cfg, ndim=2, name="adaptive KH", steps=3)
+        log(f"[bench] KH adaptive CPU: {cpu['value']:.3e} upd/s")
     return {"workload": "2D Kelvin-Helmholtz shear layer on 16^2 blocks: root 256^2, 4 levels (finest 4096^2), "
                         "static mesh around the interfaces, gamma 5/3, PLM (Koren, isentropic reconstruction) + HLLC, "
                         "CFL 0.4, FP32",
             "blocks": len(blocks), "leaves": leaves, "leaf_cells": cells, "steps": args.kh_amr_steps,
             "value": res["passes"]["value"], "paths": res, "reference_published": pub,
             "reference_leaf_cells": 772864, "over_h200": res["passes"]["value"] / pub["H200 NVL"],
-            "batched_over_levels": res["passes"]["value"] / res["levels"]["value"]}
+            "batched_over_levels": res["passes"]["value"] / res["levels"]["value"], "cpu_baseline": cpu}
 
 
 def main():
