@@ -839,8 +839,8 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   ok = ok && alloc((void**)&s->d_shell_of, shell_of.size() * sizeof(int));
   // the shell's work: a face whose neighbour is a same-level leaf needs only the half steps of
   // its rim (the march reads that leaf's cells and predicts the rest in place); a face on a
-  // coarser block needs its ghost texels only; any other face (refined neighbour, boundary
-  // image) its texels and all its half steps
+  // coarser or a refined block needs its ghost texels only; a boundary image its texels and all
+  // its half steps. Items no work writes stay zero (the buffer is cleared at creation).
   std::vector<int> shw[2];
   for (int i = 0; fused && i < s->leaf.count; i++) {
     const int b = lists[s->leaf.off + i];
@@ -854,7 +854,10 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
         add(1, 4);
       } else {
         for (int c = 0; c < 8; c++) add(0, c);
-        if (nb != -1)
+        // half steps beside a boundary image only: beside a coarser block the march predicts
+        // them in place, and beside a refined block the faces are the children's means (the
+        // coarse-face buffer replaces what those half steps would give)
+        if (nb < -1)
           for (int c = 0; c < 4; c++) add(1, c);
       }
     }
@@ -884,6 +887,7 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   // edge and corner texels of W are never assembled: zero, as a fresh atlas
   bool zeroed = (fused || (hipMemset(s->W1, 0, abytes) == hipSuccess && hipMemset(s->W2, 0, abytes) == hipSuccess &&
                             hipMemset(s->PA, 0, abytes) == hipSuccess && hipMemset(s->PB, 0, abytes) == hipSuccess)) &&
+                (!fused || hipMemset(s->shell, 0, (size_t)s->leaf.count * 5184 * 8 * s->es) == hipSuccess) &&
                 hipMemset(s->U1[0], 0, ubytes) == hipSuccess && hipMemset(s->U2[0], 0, ubytes) == hipSuccess &&
                 hipMemset(s->U1[1], 0, ubytes) == hipSuccess && hipMemset(s->U2[1], 0, ubytes) == hipSuccess &&
                 hipMemset(s->clk, 0, 16 * 8) == hipSuccess && hipMemset(s->slot, 0, 64) == hipSuccess;
