@@ -153,8 +153,11 @@ __global__ __launch_bounds__((RoleTile<TX, TY>::NTH), (RoleTile<TX, TY>::MINW)) 
   };
   auto prim_of = [&](T4<T> u1, T4<T> u2, int z) {
     Prim<T> q = prim_from_cons(P, u1, u2);
-    q.vx = bc_vel(q.vx, mx.flip, mx.side);
-    q.vy = bc_vel(q.vy, my.flip, my.side);
+    if constexpr (MAPBC) {
+      // (x and y periodic: every column's map has flip 1 and side 0, which bc_vel leaves as is)
+      q.vx = bc_vel(q.vx, mx.flip, mx.side);
+      q.vy = bc_vel(q.vy, my.flip, my.side);
+    }
     if (zbc && (z < 0 || z >= nz)) {
       const AxisMap mz = map_axis(z, nz, P.bc_lo[2], P.bc_hi[2]);
       q.vz = bc_vel(q.vz, mz.flip, mz.side);
