@@ -57,7 +57,9 @@ struct shx_sim {
   Params<float> pf;
   Params<double> pd;
   UniformOps ops{};
-  hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [parity][1 or 2 steps]
+  // [parity][1, 2 or 8 steps]: a step of a small grid is shorter than a graph launch's host
+  // cost, so runs of steps go out eight to a launch
+  hipGraphExec_t graph[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   // A flat grid on the fused path steps through the 2D kernels as a batch of one grid: in FP32
   // the row march (k_ens_rows, one wave per 60-cell strip) or the LDS tile kernel (k_ens_fused),
   // by tile; in FP64 the tile kernel's 16 x 16 form. The 3D march (ops.fused) remains its
@@ -119,18 +121,18 @@ void enqueue_step(shx_sim* s, int p, bool debug = false) {
 
 void destroy_graphs(shx_sim* s) {
   for (int p = 0; p < 2; p++)
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < 3; k++)
       if (s->graph[p][k]) { (void)hipGraphExecDestroy(s->graph[p][k]); s->graph[p][k] = nullptr; }
 }
 
 int build_graphs(shx_sim* s) {
   destroy_graphs(s);
   for (int p = 0; p < 2; p++)
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < 3; k++) {
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_step(s, p);
-      if (k == 1) enqueue_step(s, 1 - p);
+      const int n = k == 0 ? 1 : (k == 1 ? 2 : 8);
+      for (int i = 0; i < n; i++) enqueue_step(s, (p + i) & 1);
       HIPCHK(hipStreamEndCapture(s->stream, &g));
       HIPCHK(hipGraphInstantiate(&s->graph[p][k], g, nullptr, nullptr, 0));
       HIPCHK(hipGraphDestroy(g));
@@ -516,6 +518,10 @@ int shx_step(shx_sim* s, int count, int* taken) {
   if (taken) *taken = 0;
   if (!s || count < 0) return fail(SHX_EINVAL, "bad argument");
   const int want = count;
+  while (count >= 8) {
+    HIPCHK(hipGraphLaunch(s->graph[s->cur][2], s->stream));
+    count -= 8;
+  }
   while (count >= 2) {
     HIPCHK(hipGraphLaunch(s->graph[s->cur][1], s->stream));
     count -= 2;

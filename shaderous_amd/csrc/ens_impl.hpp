@@ -97,7 +97,8 @@ __global__ __launch_bounds__(BX* BY, (sizeof(T) == 4 ? 2 : 1)) void k_ens_fused(
     const bool mapped = (sx != tx) || (sy != ty);
     const int ts = min(max(sx, 1), BX - 2) + BX * min(max(sy, 1), BY - 2);
 
-    __syncthreads();   // the previous tile's last LDS reads are done
+    // the previous tile's last LDS reads are done (a workgroup's first tile has none before it)
+    if (wi != (int)blockIdx.x) __syncthreads();
     const size_t o = off + (size_t)my.i * nx + mx.i;
     const T4<T> u1 = U1[o], u2 = U2[o];
     Prim<T> q = prim_from_cons(P, u1, u2);
