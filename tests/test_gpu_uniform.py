@@ -124,6 +124,25 @@ def test_plain_and_fused_agree_bitwise():
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("path", ["plain", "fused"])
+@pytest.mark.parametrize("flat", [False, True])
+def test_step_runs_equal_single_steps(path, flat):
+    """shx_step(count) launches runs of eight steps, then pairs, then one, each its own graph:
+    19 steps in one call (8 + 8 + 2 + 1) and 19 calls of one step give the same bits and clock,
+    and so do two such calls in a row (the parity each call leaves)."""
+    pb = problems.kelvin_helmholtz(64) if flat else blob3d(24)
+    runs = []
+    for calls in ([19, 19], [1] * 38):
+        with Sim(config_for(pb, path=path)) as sim:
+            sim.set_state(*pb.state())
+            taken = sum(sim.step(c) for c in calls)
+            assert taken == 38
+            u1, u2 = sim.read_state()
+            runs.append((u1, u2, sim.clock()))
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    assert np.array_equal(runs[0][2], runs[1][2])
+
+
 FUSED_TILES = [(16, 16), (28, 12), (28, 28), (32, 16), (32, 8)]
 
 
