@@ -273,9 +273,11 @@ int shx_amr_flux_mismatch(shx_amr* sim, int64_t counts[3], double* worst_coarse_
  * [4] kernel launches per step. */
 int shx_amr_stats(shx_amr* sim, int out[5]);
 /* The refinement indicator of every leaf (AmrSim's flagging pass, src/amr/amr_sim.ts:3029-3060,
- * with relative density jumps in place of amr/flag.wgsl's Loehner estimate): out[slot] =
- * the largest |rho_a - rho_b| / min(rho_a, rho_b) between face-adjacent cells of the leaf;
- * NaN for parents. out holds one double per block. */
+ * with amr/flag.wgsl:66-155 for gas cells): out[slot] = the largest over the leaf's cells of
+ * Loehner's normalized second derivative of density and of pressure, neighbours from the
+ * assembled ghosts, at least 1e-30 (near 1 at a discontinuity, falling as dx / L on a smooth
+ * profile: the scale SimConfig.refineHi / refineLo, 0.8 / 0.2 by default, are set for); NaN for
+ * parents. out holds one double per block. */
 int shx_amr_indicator(shx_amr* sim, double* out);
 /* BlockTree.adapt (src/amr/tree.ts:489-641) on the host, no GPU: the next tree from `blocks`
  * (nblocks x 4 ints, 2:1 balanced, the tree of a sim with root blocks `root` and dimension

@@ -80,6 +80,7 @@ def lib():
             getattr(_lib, pfx + "amr_assemble").argtypes = [C.POINTER(OrcParams), C.POINTER(OrcAmr), C.c_double,
                                                             P, P, P]
             getattr(_lib, pfx + "amr_assemble").restype = ct
+            getattr(_lib, pfx + "amr_flag").argtypes = [C.POINTER(OrcParams), C.POINTER(OrcAmr), C.c_double, P, P, P]
             getattr(_lib, pfx + "amr_prolong").argtypes = [C.POINTER(OrcParams), P, C.POINTER(C.c_int),
                                                            C.POINTER(C.c_int), P]
             getattr(_lib, pfx + "amr_regrid").argtypes = [C.POINTER(OrcParams), C.POINTER(OrcAmr), P, P, C.c_int,
@@ -248,6 +249,14 @@ class Wgsl:
         r = self.fn("amr_assemble")(C.byref(prm), C.byref(table.t), dx0, _ptr(self.arr(u1), self.ct),
                                     _ptr(self.arr(u2), self.ct), _ptr(W, self.ct))
         return W, r
+
+    def amr_flag(self, prm, table, dx0, u1, u2):
+        """The refinement flag (src/amr/flag.wgsl:66-155): per block the largest Loehner estimate
+        of density and pressure over the leaf's cells, 0 for parents."""
+        out = np.zeros(table.nblocks, self.dtype)
+        self.fn("amr_flag")(C.byref(prm), C.byref(table.t), dx0, _ptr(self.arr(u1), self.ct),
+                            _ptr(self.arr(u2), self.ct), _ptr(out, self.ct))
+        return out
 
     def amr_prolong(self, prm, stencil, ok, par):
         """The fine child of parity par (3 ints of 0/1) of a coarse cell, from its 7-cell stencil

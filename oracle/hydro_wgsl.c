@@ -1203,6 +1203,54 @@ real FN(amr_assemble)(const orc_params* prm, const orc_amr* t, double dx0, const
   return amr_assemble(t, &P, dx0, u1, u2, W);
 }
 
+/* The refinement flag of every leaf (src/amr/flag.wgsl:66-155 for gas cells: second :66-83,
+   lohner :85-106, main :131-155 without the obstacle, species, burning, point-mass, density-
+   threshold and Jeans terms), on the atlas the assembly leaves: per cell Loehner's
+   chi = sqrt(sum_d (q+ - 2q + q-)^2 / max(sum_d (|q+ - q| + |q - q-| + EPS(|q+| + 2|q| + |q-|))^2,
+   1e-30)) of density and of pressure over the block's axes; out[s] = the largest
+   max(chi_rho, chi_p, GAS_MIN) over the leaf's cells, 0 for parents. */
+static void amr_flag_w(const orc_amr* t, const real* W, real* out) {
+  const real EPS = RL(0.01), GAS_MIN = RL(1e-30);
+  const int axes = flat_(t) ? 2 : 3;
+  for (int s = 0; s < t->nblocks; s++) {
+    real best = RL(0.0);
+    if (leaf_(t, s))
+      for (int z = zlo_(t, 0); z < zhi_(t, 0); z++)
+        for (int y = 0; y < BS_; y++)
+          for (int x = 0; x < BS_; x++) {
+            const int l[3] = {x, y, z};
+            const real* c = W + pidx(t, s, l) * 8;
+            const real q0[2] = {c[0], c[4]};
+            real num[2] = {RL(0.0), RL(0.0)}, den[2] = {RL(0.0), RL(0.0)};
+            for (int a = 0; a < axes; a++) {
+              int lp[3] = {x, y, z}, lm[3] = {x, y, z};
+              lp[a] += 1;
+              lm[a] -= 1;
+              const real* wp = W + pidx(t, s, lp) * 8;
+              const real* wm = W + pidx(t, s, lm) * 8;
+              for (int k = 0; k < 2; k++) {
+                const real qp = wp[4 * k], qm = wm[4 * k], q = q0[k];
+                const real d = rabs(qp - q) + rabs(q - qm) + EPS * (rabs(qp) + RL(2.0) * rabs(q) + rabs(qm));
+                const real s2 = qp - RL(2.0) * q + qm;
+                num[k] += s2 * s2;
+                den[k] += d * d;
+              }
+            }
+            const real chi = rmax(rsqrt_(num[0] / rmax(den[0], RL(1e-30))), rsqrt_(num[1] / rmax(den[1], RL(1e-30))));
+            best = rmax(best, rmax(chi, GAS_MIN));
+          }
+    out[s] = best;
+  }
+}
+
+void FN(amr_flag)(const orc_params* prm, const orc_amr* t, double dx0, const real* u1, const real* u2, real* out) {
+  P_t P = load_params(prm);
+  real* W = (real*)calloc((size_t)t->nblocks * ntex_(t) * 8, sizeof(real));
+  amr_assemble(t, &P, dx0, u1, u2, W);
+  amr_flag_w(t, W, out);
+  free(W);
+}
+
 /* ---------------- per-block timesteps (src/amr/bins_*.wgsl, amr_sim.ts encodeCycle :1509-1660) ---------------- */
 
 /* bins_tick.wgsl MODE 0: bank the substep just taken (Kahan-summed t) */

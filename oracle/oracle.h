@@ -118,6 +118,11 @@ const int* orc_amr_table(const orc_amr* t);
      edges and corners left as given); returns the largest leaf rate / dx */                    \
   T P##amr_assemble(const orc_params* prm, const orc_amr* t, double dx0, const T* u1, const T* u2, \
                     T* W);                                                                      \
+  /* the refinement flag of every leaf (src/amr/flag.wgsl:66-155, gas cells): Loehner's second- \
+     derivative estimator of density and pressure on the assembled atlas, the largest over the   \
+     leaf's cells (at least 1e-30); 0 for parents. out holds nblocks reals */                     \
+  void P##amr_flag(const orc_params* prm, const orc_amr* t, double dx0, const T* u1, const T* u2,  \
+                   T* out);                                                                      \
   /* prolongation of one coarse cell (src/amr/cons.wgsl:76-124) from its 7-cell stencil of    \
      conserved states (centre, -x, +x, -y, +y, -z, +z; 8 reals each; ok[k] = 0 drops it),     \
      to the fine child of parity par[3] */                                                     \

@@ -98,8 +98,9 @@ class AmrSim:
 
     # -- re-meshing --
     def indicator(self) -> np.ndarray:
-        """Per block the largest relative density jump between face-adjacent cells of the leaf
-        (NaN for parents): the refinement indicator."""
+        """The refinement indicator (amr/flag.wgsl:66-155, gas cells): per leaf the largest
+        Loehner estimate of density and pressure over its cells, ghosts included (near 1 at a
+        discontinuity); NaN for parents."""
         out = np.empty(self.nblocks, np.float64)
         check(_lib.load().shx_amr_indicator(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out

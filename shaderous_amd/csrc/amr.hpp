@@ -105,7 +105,7 @@ struct AmrOps {
   void (*regrid)(const AmrLaunch& L, const int* map, int count, const void* u1, const void* u2, void* n1a, void* n2a,
                  void* n1b, void* n2b);
   // the refinement indicator of the listed leaves into out[slot] (precision of the state)
-  void (*indicator)(const AmrLaunch&, const int* list, int count, const void* u1, void* out);
+  void (*indicator)(const AmrLaunch&, const int* list, int count, const void* u1, const void* u2, void* out);
   // per-block timesteps: each listed leaf's largest rate / dx into out[slot] (precision of the
   // state); the clock's bank of the substep just taken (bins_tick.wgsl MODE 0); the cycle's dt
   // from the reduced rate in slot (hydro/clock.wgsl with BANK off and CYCLE substeps); the

@@ -861,21 +861,48 @@ __global__ __launch_bounds__(256) void k_amr_regrid(Params<T> P, const int* __re
   N1b[o] = u.a; N2b[o] = u.b;
 }
 
-// The refinement indicator of the listed leaves: the largest relative density jump
-// |rho_a - rho_b| / min(rho_a, rho_b) between face-adjacent cells of the block's interior.
-template <typename T, int D = 3>
-__global__ __launch_bounds__(256) void k_amr_indicator(const int* __restrict__ list, const T4<T>* __restrict__ U1,
-                                                       T* __restrict__ out) {
+// The refinement indicator of the listed leaves (amr/flag.wgsl:66-155 for gas cells: the
+// obstacle, species, burning, point-mass, density-threshold and Jeans terms belong to physics this
+// path does not carry): per interior cell Loehner's normalized second derivative of density and
+// of pressure,
+//   chi^2 = sum_d (q+ - 2 q + q-)^2 / sum_d (|q+ - q| + |q - q-| + EPS (|q+| + 2 |q| + |q-|))^2,
+// over the block's D axes, the neighbours taken from the assembled texels (ghosts included, as
+// the flag pass reads the atlas the assembly left); the block's value is the largest
+// max(chi_rho, chi_p) over its cells, at least 1e-30 (GAS_MIN).
+constexpr double FLAG_EPS = 0.01, FLAG_GAS_MIN = 1e-30;
+
+template <typename T>
+__device__ __forceinline__ void flag_axis(T qm, T q0, T qp, T& num, T& den) {
+  const T eps = T(FLAG_EPS);
+  const T d = tabs(qp - q0) + tabs(q0 - qm) + eps * (tabs(qp) + T(2) * tabs(q0) + tabs(qm));
+  const T s2 = qp - T(2) * q0 + qm;
+  num = num + s2 * s2;
+  den = den + d * d;
+}
+
+template <typename T, int LIM, int D = 3>
+__global__ __launch_bounds__(256) void k_amr_indicator(Params<T> P, const int* __restrict__ tab,
+                                                       const int* __restrict__ list, const T4<T>* __restrict__ U1,
+                                                       const T4<T>* __restrict__ U2, T* __restrict__ out) {
   __shared__ T red[4];
   const int s = list[blockIdx.x];
   T r = T(0);
   for (int c = threadIdx.x; c < AmrGeo<D>::CELLS; c += 256) {
-    const int x = c % AMR_BS, y = (c / AMR_BS) % AMR_BS, z = c / (AMR_BS * AMR_BS);
-    const size_t o = (size_t)s * AmrGeo<D>::CELLS + c;
-    const T a = U1[o].x;
-    if (x + 1 < AMR_BS) { const T b = U1[o + 1].x; r = tmax(r, tdiv(tabs(a - b), tmin(a, b))); }
-    if (y + 1 < AMR_BS) { const T b = U1[o + AMR_BS].x; r = tmax(r, tdiv(tabs(a - b), tmin(a, b))); }
-    if (D == 3 && z + 1 < AMR_BS) { const T b = U1[o + AMR_BS * AMR_BS].x; r = tmax(r, tdiv(tabs(a - b), tmin(a, b))); }
+    const int x = c % AMR_BS, y = (c / AMR_BS) % AMR_BS, z = D == 3 ? c / (AMR_BS * AMR_BS) : 0;
+    Prim<T> w0;
+    amr_texel<T, LIM, D>(P, tab, U1, U2, s, x, y, z, w0);
+    T nr = T(0), dr = T(0), np = T(0), dp = T(0);
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      // a face neighbour of a cell is a cell or a face ghost: never an edge
+      Prim<T> wm, wp;
+      amr_texel<T, LIM, D>(P, tab, U1, U2, s, x - (a == 0), y - (a == 1), z - (a == 2), wm);
+      amr_texel<T, LIM, D>(P, tab, U1, U2, s, x + (a == 0), y + (a == 1), z + (a == 2), wp);
+      flag_axis(wm.rho, w0.rho, wp.rho, nr, dr);
+      flag_axis(wm.p, w0.p, wp.p, np, dp);
+    }
+    const T chi = tmax(tsqrt(tdiv(nr, tmax(dr, T(1e-30)))), tsqrt(tdiv(np, tmax(dp, T(1e-30)))));
+    r = tmax(r, tmax(chi, T(FLAG_GAS_MIN)));
   }
   r = wave_max(r);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r;
@@ -951,9 +978,10 @@ struct AmrImpl {
                        Lold.tab, (const int4*)map, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)n1a, (T4<T>*)n2a,
                        (T4<T>*)n1b, (T4<T>*)n2b);
   }
-  static void indicator(const AmrLaunch& L, const int* list, int count, const void* u1, void* out) {
+  static void indicator(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2, void* out) {
     if (count <= 0) return;
-    hipLaunchKernelGGL((k_amr_indicator<T, D>), dim3(count), dim3(256), 0, L.stream, list, (const T4<T>*)u1, (T*)out);
+    hipLaunchKernelGGL((k_amr_indicator<T, LIM, D>), dim3(count), dim3(256), 0, L.stream, P(L), L.tab, list,
+                       (const T4<T>*)u1, (const T4<T>*)u2, (T*)out);
   }
   static void block_rates(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2, void* out) {
     if (count <= 0) return;

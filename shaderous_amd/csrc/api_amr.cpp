@@ -1089,7 +1089,7 @@ int shx_amr_indicator(shx_amr* s, double* out) {
   void* d = nullptr;
   HIPCHK(hipMalloc(&d, (size_t)s->nblocks * s->es));
   const AmrLaunch L = launch_of(s);
-  s->ops.indicator(L, lp(s, s->leaf), s->leaf.count, s->U1[s->cur], d);
+  s->ops.indicator(L, lp(s, s->leaf), s->leaf.count, s->U1[s->cur], s->U2[s->cur], d);
   int rc = hipGetLastError() == hipSuccess ? SHX_OK : SHX_EDEVICE;
   std::vector<unsigned char> h((size_t)s->nblocks * s->es);
   if (rc == SHX_OK && (hipMemcpyAsync(h.data(), d, h.size(), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||

@@ -502,7 +502,7 @@ class AmrSim extends Base {
     const kind = path !== undefined ? path : bins > 0 || this.ndim === 2 ? 'passes' : 'fused';
     // the re-mesh cadence (REGRID_EVERY = 8, src/amr/amr_sim.ts:73): on by default where the
     // reference's config asks for levels of refinement (amrLevels), with its thresholds
-    // (refineHi, refineLo) applied to this backend's indicator
+    // (refineHi, refineLo) on the same indicator (amr/flag.wgsl's Loehner estimate)
     const levels = cfg.amrLevels || 0;
     const every = regridEvery !== undefined ? regridEvery : levels > 0 ? 8 : 0;
     const lay = {
@@ -613,7 +613,7 @@ class AmrSim extends Base {
     return { blocks: v[0], leaves: v[1], matchLeaves: v[2], levels: v[3], launchesPerStep: v[4] };
   }
 
-  /** Per block the largest relative density jump between face-adjacent cells (NaN for parents). */
+  /** Per leaf the refinement indicator (amr/flag.wgsl's Loehner estimate, largest over its cells); NaN for parents. */
   indicator() {
     return native.amrIndicator(live(this.h));
   }

@@ -240,6 +240,30 @@ def test_2d_regrid_matches_oracle(precision, tol):
     assert np.abs(b1[back] - u1[back]).max() <= 4 * tol * np.abs(u1[back]).max()
 
 
+@pytest.mark.parametrize("precision,tol", [(32, 1e-4), (64, 1e-12)])
+def test_2d_indicator_matches_oracle(precision, tol):
+    """The flag pass on 2D blocks (Loehner's estimator over two axes) against the oracle's
+    amr_flag, coarse-fine faces on both levels: same value per leaf, NaN for parents."""
+    blocks = quadtree()
+    impl = wgsl64 if precision == 64 else wgsl32
+    cfg = cfg2(precision)
+    u1, u2 = amr_tree.blocks_state(blocks, DX0, flow, 1.4, impl.dtype, ndim=2)
+    want = impl.amr_flag(orc_prm(cfg), AmrTable(blocks, ROOT, ndim=2), DX0, u1, u2).astype(np.float64)
+    sim = AmrSim(cfg, blocks, ROOT, path="passes")
+    try:
+        sim.set_state(u1, u2)
+        ind = sim.indicator()
+    finally:
+        sim.destroy()
+    leaf = amr_tree.leaf_mask(blocks)
+    assert np.all(np.isnan(ind[~leaf]))
+    err = np.abs(ind[leaf] - want[leaf]) / want[leaf]
+    print(f"\n[gpu] 2D indicator fp{precision}: range {want[leaf].min():.3g}..{want[leaf].max():.3g}, "
+          f"worst rel {err.max():.2e}", flush=True)
+    assert want[leaf].max() > 0.05
+    assert err.max() <= tol, err
+
+
 @pytest.mark.parametrize("precision", [32, 64])
 def test_2d_bins_match_oracle_cycle(precision):
     """Per-block timesteps on a quadtree: the bins the GPU plans and each cycle's state against

@@ -2,7 +2,7 @@
 
 - the GPU's new state against the oracle's re-meshing (FP64 to 1e-13, FP32 to 2e-6 per field,
   max-norm relative); the same tree again gives the state bit for bit;
-- the indicator against the same quantity formed with numpy from the state read back;
+- the indicator (Loehner's estimator, src/amr/flag.wgsl) against the oracle's amr_flag;
 - an adaptive run (re-mesh every 4 steps from the indicator): the fused and the pass paths stay
   equal bit for bit, and mass and energy stay conserved to FP32 round-off.
 """
@@ -102,10 +102,17 @@ def test_regrid_to_the_same_tree_is_exact():
     assert np.array_equal(a[0][sel], b[0][sel]) and np.array_equal(a[1][sel], b[1][sel])
 
 
-def test_indicator_matches_numpy():
+@pytest.mark.parametrize("precision,tol", [(64, 1e-12), (32, 1e-4)])
+def test_indicator_matches_oracle(precision, tol):
+    """The flag pass (Loehner's estimator of density and pressure over each leaf's cells, ghosts
+    from the assembly) against the oracle's amr_flag on a tree with coarse-fine faces on every
+    level: same value per leaf to the precision's rounding, NaN for parents."""
     blocks = refined_tree()
-    cfg = cfg_for((2, 2, 2))
-    u1, u2 = amr_tree.blocks_state(blocks, cfg.dx, blob, cfg.gamma)
+    cfg = cfg_for((2, 2, 2), precision)
+    impl = wgsl64 if precision == 64 else wgsl32
+    u1, u2 = amr_tree.blocks_state(blocks, cfg.dx, blob, cfg.gamma, impl.dtype)
+    prm = params(cfg.n, ndim=3, dx=cfg.dx, gamma=cfg.gamma, cfl=cfg.cfl, limiter="mc")
+    want = impl.amr_flag(prm, AmrTable(blocks, (2, 2, 2)), cfg.dx, u1, u2).astype(np.float64)
     sim = AmrSim(cfg, blocks, (2, 2, 2))
     try:
         sim.set_state(u1, u2)
@@ -114,14 +121,11 @@ def test_indicator_matches_numpy():
         sim.destroy()
     leaf = amr_tree.leaf_mask(blocks)
     assert np.all(np.isnan(ind[~leaf]))
-    rho = u1[:, 0].reshape(len(blocks), BS, BS, BS).astype(np.float64)
-    for i in np.flatnonzero(leaf):
-        r = rho[i]
-        m = 0.0
-        for ax in range(3):
-            a, b = np.moveaxis(r, ax, 0)[:-1], np.moveaxis(r, ax, 0)[1:]
-            m = max(m, float((np.abs(a - b) / np.minimum(a, b)).max()))
-        assert abs(ind[i] - m) <= 1e-6 * m + 1e-12, (i, ind[i], m)
+    err = np.abs(ind[leaf] - want[leaf]) / want[leaf]
+    print(f"\n[gpu] indicator fp{precision}: leaves {leaf.sum()}, range {want[leaf].min():.3g}..{want[leaf].max():.3g}, "
+          f"worst rel {err.max():.2e}", flush=True)
+    assert want[leaf].max() > 0.05   # the blob's blocks are flagged well above round-off
+    assert err.max() <= tol, err
 
 
 def test_adaptive_run_paths_agree_and_conserve():
