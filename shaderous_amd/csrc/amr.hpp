@@ -93,10 +93,11 @@ struct AmrOps {
   void (*diag)(const AmrLaunch&, const DiagGeo&, const int* list, int count, const void* u1, const void* u2,
                double* part, double* out);
   // the fused form: the ghost shells (per leaf shell index, the waves of work listed as (shell,
-  // slot, face, chunk): work0 the ghost texels, work1 the half steps); the march of every leaf
-  // in one launch, from the queue Q
-  void (*shell)(const AmrLaunch&, const int* work0, int n0, const int* work1, int n1, const void* u1, const void* u2,
-                void* sh, const void* clk);
+  // slot, face, chunk): work0 the ghost texels, work1 the half steps beside boundary images;
+  // work2 the half steps of face-rim cells, one record of 8 ints per source cell); the march of
+  // every leaf in one launch, from the queue Q
+  void (*shell)(const AmrLaunch&, const int* work0, int n0, const int* work1, int n1, const int* work2, int n2,
+                const void* u1, const void* u2, void* sh, const void* clk);
   void (*fused)(const AmrLaunch&, const AmrQueue& Q, const void* sh, const void* u1, const void* u2, void* o1, void* o2,
                 const void* clk, unsigned long long* slot);
   // re-meshing: `count` new leaves listed as int4 (new slot, kind 1 copy / 2 prolong / 3

@@ -210,22 +210,12 @@ __device__ __forceinline__ bool off_rim(int i, int j) {
 
 // Work of the shell, one wave per item (shell index, slot, face k, chunk), listed on the host from
 // the static mesh: PART 0, the ghost texels of face k (two layers, 512 texels: chunks 0-7) where
-// its neighbour is not a same-level leaf; PART 1, the first layer's half steps there (chunks 0-3)
-// or, beside a same-level leaf, the 60 on the face's rim (chunk 4). Faces on a coarser block
-// have no PART 1 work (the march predicts those ghosts in place).
-constexpr int SH_RIM = 4;
-
+// its neighbour is not a same-level leaf; PART 1, the first layer's half steps (chunks 0-3)
+// beside a boundary image. Faces on a coarser block have no PART 1 work (the march predicts
+// those ghosts in place); beside a same-level leaf only the face's rim needs its half steps,
+// which k_amr_rim forms from the cells of the leaf across.
 template <int PART>
 constexpr int sh_part_waves() { return PART == 0 ? 6 : 3; }
-
-// the face position (i, j) of rim lane r (r < 60): the rows j = 0 and 15, then the columns
-// i = 0 and 15 between them
-__device__ __forceinline__ void rim_position(int r, int& i, int& j) {
-  if (r < 16) { i = r; j = 0; }
-  else if (r < 32) { i = r - 16; j = AMR_BS - 1; }
-  else if (r < 46) { i = 0; j = r - 31; }
-  else { i = AMR_BS - 1; j = r - 45; }
-}
 
 template <typename T, int LIM, bool ISEN, int PART>
 __global__ __launch_bounds__(256, sh_part_waves<PART>()) void k_amr_shell(
@@ -258,15 +248,7 @@ __global__ __launch_bounds__(256, sh_part_waves<PART>()) void k_amr_shell(
     out[0] = v4<T>(w.rho, w.vx, w.vy, w.vz);
     out[1] = v4<T>(w.p, w.s, w.cs, T(0));
   } else {
-    int i, j;
-    if (code == SH_RIM) {
-      if (lane >= 60) return;
-      rim_position(lane, i, j);
-    } else {
-      const int e = code * 64 + lane;
-      i = e & 15;
-      j = e >> 4;
-    }
+    const int e = code * 64 + lane, i = e & 15, j = e >> 4;
     int h;
     if (k < 4) {
       const int col = k * 16 + i;
@@ -589,11 +571,46 @@ __global__ __launch_bounds__((sizeof(T) == 4 ? 2 : 1) * AF_THREADS, (sizeof(T) =
   }
 }
 
+// The half steps of the face-rim ghosts beside same-level leaves: a rim ghost's source is a cell
+// on an edge of the leaf across the face, whose transverse stencil reaches past the face (the
+// march holds only the face itself). One lane per source cell, from records listed on the host
+// (source slot, cell x | y << 4 | z << 8, then up to three (shell index, half-step item)
+// destinations, -1 for none): an edge cell feeds the shells of the leaves across both its faces,
+// a corner cell three, so each is predicted once, and the lanes of a wave take the edge cells of
+// one block. The value is the shell's own for that ghost (half_at_source at the source; a
+// same-level face maps no velocity).
+template <typename T, int LIM, bool ISEN>
+__global__ __launch_bounds__(256) void k_amr_rim(Params<T> P, const int* __restrict__ tab,
+                                                 const int4* __restrict__ work, int nwork,
+                                                 const T4<T>* __restrict__ U1, const T4<T>* __restrict__ U2,
+                                                 T4<T>* __restrict__ SH, const T* __restrict__ clk) {
+  const int w = blockIdx.x * 256 + threadIdx.x;
+  if (w >= nwork) return;
+  const int4 a = work[2 * w], b = work[2 * w + 1];
+  const int x = a.y & 15, y = (a.y >> 4) & 15, z = (a.y >> 8) & 15;
+  Params<T> Pl = P;
+  Pl.dx = level_dx(P, bt(tab, a.x, BT_LEVEL));
+  const Half<T> hc = half_at_source<T, LIM, ISEN>(Pl, tab, U1, U2, a.x, x, y, z, clk[0]);
+  const T4<T> v0 = v4<T>(hc.rho, hc.vx, hc.vy, hc.vz), v1 = v4<T>(hc.p, hc.s, hc.shock, hc.fo);
+  const int li[3] = {a.z, b.x, b.z}, h[3] = {a.w, b.y, b.w};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (li[d] < 0) continue;
+    T4<T>* out = SH + ((size_t)li[d] * SH_ITEMS + SH_TEXELS + h[d]) * 2;
+    out[0] = v0;
+    out[1] = v1;
+  }
+}
+
 // (3D blocks only: the ops of a 2D grid leave the fused form out)
 template <typename T, int LIM, int RS, bool ISEN, int D>
 void AmrImpl<T, LIM, RS, ISEN, D>::shell(const AmrLaunch& L, const int* work0, int n0, const int* work1, int n1,
-                                         const void* u1, const void* u2, void* sh, const void* clk) {
+                                         const int* work2, int n2, const void* u1, const void* u2, void* sh,
+                                         const void* clk) {
   static_assert(D == 3, "the fused form steps 3D blocks");
+  if (n2 > 0)
+    hipLaunchKernelGGL((k_amr_rim<T, LIM, ISEN>), dim3((n2 + 255) / 256), dim3(256), 0, L.stream, P(L), L.tab,
+                       (const int4*)work2, n2, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)sh, (const T*)clk);
   if (n0 > 0)
     hipLaunchKernelGGL((k_amr_shell<T, LIM, ISEN, 0>), dim3((n0 + 3) / 4), dim3(256), 0, L.stream, P(L), LV(L), L.tab,
                        (const int4*)work0, n0, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)sh, (const T*)clk);

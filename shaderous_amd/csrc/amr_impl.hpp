@@ -967,8 +967,8 @@ struct AmrImpl {
                          (const T4<T>*)u1, (const T4<T>*)u2, part);
     hipLaunchKernelGGL(k_diag_final<T>, dim3(1), dim3(32 * DIAG_CH), 0, L.stream, (const double*)part, count, out);
   }
-  static void shell(const AmrLaunch& L, const int* work0, int n0, const int* work1, int n1, const void* u1,
-                    const void* u2, void* sh, const void* clk);
+  static void shell(const AmrLaunch& L, const int* work0, int n0, const int* work1, int n1, const int* work2, int n2,
+                    const void* u1, const void* u2, void* sh, const void* clk);
   static void fused(const AmrLaunch& L, const AmrQueue& Q, const void* sh, const void* u1, const void* u2, void* o1,
                     void* o2, const void* clk, unsigned long long* slot);
   static void regrid(const AmrLaunch& Lold, const int* map, int count, const void* u1, const void* u2, void* n1a,

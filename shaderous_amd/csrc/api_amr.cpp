@@ -54,8 +54,8 @@ struct shx_amr {
   void* shell = nullptr;
   int* d_shell_of = nullptr;
   // the shell's waves of work (shell index, slot, face, chunk): ghost texels, half steps
-  int* d_shw[2] = {nullptr, nullptr};
-  int nshw[2] = {0, 0};
+  int* d_shw[3] = {nullptr, nullptr, nullptr};   // the shell's work: texels, half steps, rim records
+  int nshw[3] = {0, 0, 0};
   int* d_reg_of = nullptr;
   void* reg = nullptr;
   // coarse-fine faces seen from the coarse side: (slot, face) pairs by level, their buffers, and
@@ -311,6 +311,36 @@ void adapt_tree(const int* blocks, int n, const int root[3], int ndim, const dou
   for (const auto& r : rows) out.insert(out.end(), r.begin(), r.end());
 }
 
+// Rim lane r (< 60) of face f of a leaf's ghost shell, beside a same-level leaf: the half-step
+// item h it fills (faces 0-3: column f * 16 + i of the first ghost ring, plane j; faces 4 and 5:
+// the plane z = -1 or 16, texel (i, j)) and the cell of the leaf across that is its source, as
+// x | y << 4 | z << 8. The face positions (i, j) of the rim: the rows j = 0 and 15, then the
+// columns i = 0 and 15 between them.
+static void rim_ghost(int f, int r, int& h, int& cell) {
+  int i, j;
+  if (r < 16) { i = r; j = 0; }
+  else if (r < 32) { i = r - 16; j = AMR_BS - 1; }
+  else if (r < 46) { i = 0; j = r - 31; }
+  else { i = AMR_BS - 1; j = r - 45; }
+  int x, y, z;
+  if (f < 4) {
+    x = f == 0 ? -1 : (f == 1 ? AMR_BS : i);
+    y = f == 2 ? -1 : (f == 3 ? AMR_BS : i);
+    z = j;
+    h = (f * AMR_BS + i) * AMR_BS + z;
+  } else {
+    x = i;
+    y = j;
+    z = f == 4 ? -1 : AMR_BS;
+    h = 64 * AMR_BS + (f == 4 ? 0 : AMR_BS * AMR_BS) + y * AMR_BS + x;
+  }
+  // the ghost's source across a same-level face: the coordinate wraps by a block
+  x = (x + AMR_BS) % AMR_BS;
+  y = (y + AMR_BS) % AMR_BS;
+  z = (z + AMR_BS) % AMR_BS;
+  cell = x | (y << 4) | (z << 8);
+}
+
 int build_lists(shx_amr* s, std::vector<int>& lists) {
   const int n = s->nblocks;
   std::vector<int> order(n);
@@ -394,8 +424,9 @@ int enqueue_step(shx_amr* s, int p, int stage = -1) {
   if (s->lay.path == SHX_AMR_PATH_FUSED) {
     if (run(1)) { o.clock(L, s->clk, s->slot); n++; }
     if (run(2)) {
-      o.shell(L, s->d_shw[0], s->nshw[0], s->d_shw[1], s->nshw[1], s->U1[p], s->U2[p], s->shell, s->clk);
-      n += (s->nshw[0] > 0) + (s->nshw[1] > 0);
+      o.shell(L, s->d_shw[0], s->nshw[0], s->d_shw[1], s->nshw[1], s->d_shw[2], s->nshw[2], s->U1[p], s->U2[p],
+              s->shell, s->clk);
+      n += (s->nshw[0] > 0) + (s->nshw[1] > 0) + (s->nshw[2] > 0);
     }
     if (run(3) && s->queue.count > 0) {
       AmrQueue Q{lp(s, s->queue), s->queue.count, s->grid, s->d_shell_of, s->d_act, s->reg, s->cf, s->d_reg_of,
@@ -841,7 +872,9 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
   // its rim (the march reads that leaf's cells and predicts the rest in place); a face on a
   // coarser or a refined block needs its ghost texels only; a boundary image its texels and all
   // its half steps. Items no work writes stay zero (the buffer is cleared at creation).
-  std::vector<int> shw[2];
+  std::vector<int> shw[3];
+  // rim ghosts as (source slot, source cell x | y << 4 | z << 8, shell index, half-step item)
+  std::vector<std::array<int, 4>> rim;
   for (int i = 0; fused && i < s->leaf.count; i++) {
     const int b = lists[s->leaf.off + i];
     for (int f = 0; f < 6; f++) {
@@ -851,7 +884,11 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
         shw[part].insert(shw[part].end(), {i, b, f, chunk});
       };
       if (same_leaf) {
-        add(1, 4);
+        for (int r = 0; r < 60; r++) {
+          int h, cell;
+          rim_ghost(f, r, h, cell);
+          rim.push_back({nb, cell, i, h});
+        }
       } else {
         for (int c = 0; c < 8; c++) add(0, c);
         // half steps beside a boundary image only: beside a coarser block the march predicts
@@ -862,9 +899,26 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
       }
     }
   }
-  for (int part = 0; part < 2; part++) {
-    s->nshw[part] = (int)(shw[part].size() / 4);
-    ok = ok && alloc((void**)&s->d_shw[part], (shw[part].empty() ? 4 : shw[part].size()) * sizeof(int));
+  // one record per source cell, block by block (8 ints: slot, cell, then three (shell index,
+  // item) destinations, -1 for none): an edge cell of a leaf is a rim ghost of the leaves across
+  // both its faces, a corner cell of three
+  std::sort(rim.begin(), rim.end());
+  for (size_t a = 0; a < rim.size();) {
+    size_t e = a;
+    while (e < rim.size() && rim[e][0] == rim[a][0] && rim[e][1] == rim[a][1]) e++;
+    for (size_t c = a; c < e; c += 3) {
+      int rec[8] = {rim[a][0], rim[a][1], -1, -1, -1, -1, -1, -1};
+      for (size_t d = c; d < e && d < c + 3; d++) {
+        rec[2 + 2 * (d - c)] = rim[d][2];
+        rec[3 + 2 * (d - c)] = rim[d][3];
+      }
+      shw[2].insert(shw[2].end(), rec, rec + 8);
+    }
+    a = e;
+  }
+  for (int part = 0; part < 3; part++) {
+    s->nshw[part] = (int)(shw[part].size() / (part == 2 ? 8 : 4));
+    ok = ok && alloc((void**)&s->d_shw[part], (shw[part].empty() ? 8 : shw[part].size()) * sizeof(int));
   }
   ok = ok && alloc((void**)&s->d_tab, s->tab.size() * sizeof(int)) && alloc((void**)&s->d_lists, lists.size() * sizeof(int));
   ok = ok && alloc(&s->clk, 16 * 8) && alloc((void**)&s->slot, 64) && alloc((void**)&s->diag_part, (size_t)nblocks * DIAG_CH * 8) &&
@@ -901,6 +955,7 @@ int shx_amr_create(const shx_config* cfg, const shx_amr_layout* layout, const in
                 hipMemcpy(s->d_shell_of, shell_of.data(), shell_of.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
                 (shw[0].empty() || hipMemcpy(s->d_shw[0], shw[0].data(), shw[0].size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess) &&
                 (shw[1].empty() || hipMemcpy(s->d_shw[1], shw[1].data(), shw[1].size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess) &&
+                (shw[2].empty() || hipMemcpy(s->d_shw[2], shw[2].data(), shw[2].size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess) &&
                 hipMemcpy(s->d_lists, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
   if (!zeroed || !copied || hipDeviceSynchronize() != hipSuccess) {
     shx_amr_destroy(s);
@@ -919,7 +974,7 @@ int shx_amr_destroy(shx_amr* s) {
   for (int p = 0; p < 2; p++) { (void)hipFree(s->U1[p]); (void)hipFree(s->U2[p]); }
   (void)hipFree(s->W1); (void)hipFree(s->W2); (void)hipFree(s->PA); (void)hipFree(s->PB);
   (void)hipFree(s->d_reg_of); (void)hipFree(s->reg); (void)hipFree(s->shell); (void)hipFree(s->d_shell_of);
-  (void)hipFree(s->d_shw[0]); (void)hipFree(s->d_shw[1]);
+  (void)hipFree(s->d_shw[0]); (void)hipFree(s->d_shw[1]); (void)hipFree(s->d_shw[2]);
   (void)hipFree(s->d_act); (void)hipFree(s->cf);
   (void)hipFree(s->d_dep_off); (void)hipFree(s->d_dep); (void)hipFree(s->d_qctl); (void)hipFree(s->d_done);
   (void)hipFree(s->d_tab); (void)hipFree(s->d_lists); (void)hipFree(s->clk); (void)hipFree(s->slot);
