@@ -53,8 +53,8 @@ TRAFFIC_MEASURED = {
 TRAFFIC_SOURCE = "profiles/k_fused_turbulence_256.md"
 # HBM bytes per step of the AMR blast's fused path (the ghost shell pass and the march), from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE summed over a step's kernels, scaled as above
-AMR_TRAFFIC_MEASURED = {"blast": (int((0.45 + 0.05 + 1.34 + 0.05 + 2.00 + 0.64) * 1e9),
-                                   "profiles/k_amr_fused_blast.md: k_amr_shell texels + half steps + k_amr_fused, "
+AMR_TRAFFIC_MEASURED = {"blast": (int((0.45 + 0.05 + 0.79 + 0.05 + 2.00 + 0.64) * 1e9),
+                                   "profiles/k_amr_fused_blast.md: k_amr_shell texels + k_amr_rim + k_amr_fused, "
                                    "HBM read + write per step")}
 
 
