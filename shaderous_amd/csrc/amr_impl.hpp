@@ -324,28 +324,39 @@ template <typename T, int LIM, int D = 3>
 __global__ __launch_bounds__(256) void k_amr_assemble(Params<T> P, Levels<T> LV, const int* __restrict__ tab,
                                                       const int* __restrict__ list, const T4<T>* __restrict__ U1,
                                                       const T4<T>* __restrict__ U2, T4<T>* __restrict__ W1,
-                                                      T4<T>* __restrict__ W2, unsigned long long* slot, AmrBins B) {
+                                                      T4<T>* __restrict__ W2, unsigned long long* slot, AmrBins B,
+                                                      int count) {
   constexpr int NCH = asm_chunks<D>();
-  const int b = blockIdx.x / NCH;
-  const int t = (blockIdx.x % NCH) * 256 + threadIdx.x;
+  int b, t;
+  if constexpr (D == 3) {
+    b = blockIdx.x / NCH;
+    t = (blockIdx.x % NCH) * 256 + threadIdx.x;
+  } else if ((int)blockIdx.x < count) {
+    // 2D: one workgroup per block for its 256 cells, then one per two blocks for their 128 face
+    // ghosts each
+    b = blockIdx.x;
+    t = threadIdx.x;
+  } else {
+    b = 2 * ((int)blockIdx.x - count) + (int)(threadIdx.x >> 7);
+    t = AMR_BS * AMR_BS + (int)(threadIdx.x & 127);
+  }
   T r = T(0);
   int x, y, z;
-  bool any = t < AmrGeo<D>::TEXELS;
+  bool any = t < AmrGeo<D>::TEXELS && b < count;
   if constexpr (D == 3) {
     x = t % AMR_PAD - AMR_NG;
     y = (t / AMR_PAD) % AMR_PAD - AMR_NG;
     z = t / (AMR_PAD * AMR_PAD) - AMR_NG;
   } else {
-    // 2D: the first workgroup's 256 threads take the cells, the second's the 128 face ghosts
-    // face by face (-x, +x, -y, +y; two layers of 16), so no wave mixes the cells' copy with a
-    // ghost's prolongation or restriction; the 16 corner texels are never assembled
+    // 2D: the cells' workgroups take only cells, the others the face ghosts face by face (-x,
+    // +x, -y, +y; two layers of 16), so no wave mixes the cells' copy with a ghost's
+    // prolongation or restriction; the 16 corner texels are never assembled
     z = 0;
     if (t < AMR_BS * AMR_BS) {
       x = t % AMR_BS;
       y = t / AMR_BS;
     } else {
       const int g = t - AMR_BS * AMR_BS, f = g >> 5, layer = (g >> 4) & 1, i = g & 15;
-      any = g < 128;
       const int a = (f & 1) ? AMR_BS + layer : -1 - layer;
       x = f < 2 ? a : i;
       y = f < 2 ? i : a;
@@ -493,10 +504,23 @@ template <typename T, int LIM, bool ISEN, int D = 3>
 __global__ __launch_bounds__(256) void k_amr_predict(Params<T> P, Levels<T> LV, const int* __restrict__ tab,
                                                      const int* __restrict__ list, const T4<T>* __restrict__ W1,
                                                      const T4<T>* __restrict__ W2, T4<T>* __restrict__ PA,
-                                                     T4<T>* __restrict__ PB, const T* __restrict__ clk, AmrBins B) {
+                                                     T4<T>* __restrict__ PB, const T* __restrict__ clk, AmrBins B,
+                                                     int count) {
   constexpr int NCH = pred_chunks<D>();
-  const int b = blockIdx.x / NCH;
-  const int t = (blockIdx.x % NCH) * 256 + threadIdx.x;
+  int b, t;
+  if constexpr (D == 3) {
+    b = blockIdx.x / NCH;
+    t = (blockIdx.x % NCH) * 256 + threadIdx.x;
+  } else if ((int)blockIdx.x < count) {
+    // 2D: one workgroup per leaf for its 256 cells, then one per four leaves for their 64
+    // first-layer face ghosts each
+    b = blockIdx.x;
+    t = threadIdx.x;
+  } else {
+    b = 4 * ((int)blockIdx.x - count) + (int)(threadIdx.x >> 6);
+    t = AMR_BS * AMR_BS + (int)(threadIdx.x & 63);
+    if (b >= count) return;
+  }
   if (t >= pred_texels<D>()) return;
   int x, y, z;
   if constexpr (D == 3) {
@@ -919,8 +943,9 @@ struct AmrImpl {
   static void assemble(const AmrLaunch& L, const int* list, int count, const void* u1, const void* u2, void* w1,
                        void* w2, unsigned long long* slot) {
     if (count <= 0) return;
-    hipLaunchKernelGGL((k_amr_assemble<T, LIM, D>), dim3(count * asm_chunks<D>()), dim3(256), 0, L.stream, P(L),
-                       LV(L), L.tab, list, (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)w1, (T4<T>*)w2, slot, L.bins);
+    const int grid = D == 3 ? count * asm_chunks<D>() : count + (count + 1) / 2;
+    hipLaunchKernelGGL((k_amr_assemble<T, LIM, D>), dim3(grid), dim3(256), 0, L.stream, P(L), LV(L), L.tab, list,
+                       (const T4<T>*)u1, (const T4<T>*)u2, (T4<T>*)w1, (T4<T>*)w2, slot, L.bins, count);
   }
   static void clock(const AmrLaunch& L, void* clk, unsigned long long* slot) {
     Params<T> P1 = P(L);
@@ -930,9 +955,9 @@ struct AmrImpl {
   static void predict(const AmrLaunch& L, const int* list, int count, const void* w1, const void* w2, void* pa,
                       void* pb, const void* clk) {
     if (count <= 0) return;
-    hipLaunchKernelGGL((k_amr_predict<T, LIM, ISEN, D>), dim3(count * pred_chunks<D>()), dim3(256), 0, L.stream,
-                       P(L), LV(L), L.tab, list, (const T4<T>*)w1, (const T4<T>*)w2, (T4<T>*)pa, (T4<T>*)pb,
-                       (const T*)clk, L.bins);
+    const int grid = D == 3 ? count * pred_chunks<D>() : count + (count + 3) / 4;
+    hipLaunchKernelGGL((k_amr_predict<T, LIM, ISEN, D>), dim3(grid), dim3(256), 0, L.stream, P(L), LV(L), L.tab, list,
+                       (const T4<T>*)w1, (const T4<T>*)w2, (T4<T>*)pa, (T4<T>*)pb, (const T*)clk, L.bins, count);
   }
   static void update(const AmrLaunch& L, const int* list, int count, bool match, bool comp, const void* u1,
                      const void* u2, const void* w1, const void* w2, const void* pa, const void* pb, void* o1, void* o2,
