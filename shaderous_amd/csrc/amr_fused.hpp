@@ -5,7 +5,8 @@
 //  - k_amr_shell: per leaf, the ghost texels of its 20^3 region, formed from U (same-level
 //    neighbours, boundary images, prolongation from a coarser leaf, restriction of refined
 //    children: amr_texel), and the half steps of its first ghost layer where those are another
-//    block's cells ("at source", amr/predict.wgsl:183-221), all lanes busy;
+//    block's cells ("at source", amr/predict.wgsl:183-221), all lanes busy; k_amr_rim, the half
+//    steps on the rims of faces beside same-level leaves, once per source cell;
 //  - k_amr_fused, one persistent kernel per launch: a workgroup owns one 16^3 block at a time
 //    and marches along z: its cells from U, its ghosts from the shell; the half step of every
 //    cell and of the ghosts beside a coarser block (masked transverse slopes), in LDS;
